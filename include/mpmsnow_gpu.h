@@ -1,0 +1,168 @@
+/* mpmsnow_gpu.h -- C-ABI of the MI355X (gfx950) implicit MPM snow step.
+ *
+ * This is the drop-in boundary under the MpmSim library interfaces of
+ * davidsminor/mpmsnow (include/MpmSim/*.h in the reference checkout): every
+ * entry point below carries out one member function of those classes on the
+ * GPU, and is cited to the reference declaration it replaces. Plain C types
+ * only: float/int arrays in the reference's host layouts (Vector3f = 3
+ * floats, Matrix3f = 9 floats column-major, dense grid vectors in node order
+ * i + n0 (j + n1 k) with xyz interleaved), copied in and out (no aliasing).
+ *
+ * Every function returns MPM_OK (0) or an error code; mpm_last_error()
+ * returns the message, which for the reference's own failures is the
+ * reference's exact std::runtime_error text (e.g. "grid is too big",
+ * "nan in explicit momenta!"). One context = one HIP stream = one caller
+ * thread; calls return when their results are complete.
+ */
+#ifndef MPMSNOW_GPU_H
+#define MPMSNOW_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MPM_OK = 0,
+  MPM_ERR_INVALID = 1,   /* bad argument / unknown name / wrong state */
+  MPM_ERR_RUNTIME = 2,   /* a reference runtime_error (message is the reference's text) */
+  MPM_ERR_HIP = 3,       /* HIP runtime failure: "hip: <name>" */
+  MPM_ERR_NO_DEVICE = 4
+};
+
+typedef struct mpm_context mpm_context;
+typedef struct mpm_grid mpm_grid;
+
+/* Collision object (reference include/MpmSim/CollisionObject.h:10-45):
+ * type 0 = CollisionPlane (phi = c0 x + c1 y + c2 z + c3, CollisionPlane.h:14-40),
+ * type 1 = analytic sphere (phi = |x - center| - radius). */
+typedef struct {
+  int type;
+  float coeffs[4];
+  float center[3];
+  float radius;
+  float velocity[3];
+  float friction; /* coulombFriction() */
+  int sticky;     /* sticky() */
+} mpm_collider;
+
+/* Solver settings: SquareMagnitudeTermination(maxIters, tolError)
+ * (reference include/MpmSim/SquareMagnitudeTermination.h:10-26) plus optional
+ * host hooks standing in for a custom TerminationCriterion / LinearSolver::Debug
+ * (reference TerminationCriterion.h:10-16, LinearSolver.h:14-21). When a hook
+ * is set the solve synchronises every iteration and passes dense vectors. */
+typedef struct {
+  int max_iters;
+  float tolerance;
+  /* termination(r, n, i): nonzero stops the solve; replaces the built-in test */
+  int (*termination)(void* user, const float* r, int64_t n, int iteration);
+  /* debug(x, n): called after every x update with the velocities relative to the colliders */
+  void (*debug)(void* user, const float* x, int64_t n);
+  /* cancelled(): polled after each body's solve (TerminationCriterion::cancelled) */
+  int (*cancelled)(void* user);
+  void* user;
+} mpm_solver_params;
+
+typedef struct {
+  int iterations;          /* x updates (ConjugateResiduals.cpp:84-128) */
+  int early_exit;          /* stopped on a zero denominator */
+  double rhs_norm2;        /* |b|^2 */
+  double residual_norm2;   /* last |r|^2 (r = M z) */
+  int history_length;      /* entries written to history */
+  int history_capacity;    /* caller-provided capacity of history */
+  double* history;         /* caller-provided: |r|^2 per iteration, may be NULL */
+} mpm_solve_stats;
+
+typedef struct {
+  int num_bodies;
+  int num_ballistic;
+  int total_iterations;
+  int max_iterations;
+  int cancelled;
+} mpm_step_stats;
+
+/* ---- context ------------------------------------------------------------ */
+int mpm_create(mpm_context** ctx, int device);
+int mpm_destroy(mpm_context* ctx);
+const char* mpm_last_error(const mpm_context* ctx);
+int mpm_version(void);
+
+/* ---- particles: MaterialPointData (reference MaterialPointData.h:14-44) ---
+ * mpm_particles_create gives the default variables p, v (0), F (identity),
+ * m (0), volume (0) (MaterialPointData.cpp:8-16, Sim.cpp:34-38).
+ * Names: p v F m volume, and after mpm_model_snow: Fp FinvTrans R Ginv J mu
+ * lambda, plus the per-particle material mu0 lambda0 xi. count = number of
+ * floats (3n, 9n or n). */
+int mpm_particles_create(mpm_context* ctx, int64_t n);
+int64_t mpm_particles_count(const mpm_context* ctx);
+int mpm_particles_num_variables(const mpm_context* ctx);
+int mpm_particles_set(mpm_context* ctx, const char* name, const float* data, int64_t count);
+int mpm_particles_get(mpm_context* ctx, const char* name, float* data, int64_t count);
+
+/* ---- constitutive model: SnowConstitutiveModel (SnowConstitutiveModel.h:13-32) */
+/* constructor + setParticles (SnowConstitutiveModel.cpp:9-25, 123-169) */
+int mpm_model_snow(mpm_context* ctx, float youngs_modulus, float poisson_ratio, float hardening,
+                   float compressive_strength, float tensile_strength);
+/* updateParticleData over all particles (SnowConstitutiveModel.cpp:27-121) */
+int mpm_model_update_particle_data(mpm_context* ctx);
+/* energyDensity / dEnergyDensitydF / dEdFDifferential for a list of particles
+ * (SnowConstitutiveModel.cpp:171-256); out: n, 9n, 9n floats */
+int mpm_model_energy_density(mpm_context* ctx, const int32_t* particles, int64_t n, float* out);
+int mpm_model_stress(mpm_context* ctx, const int32_t* particles, int64_t n, float* out);
+int mpm_model_stress_differential(mpm_context* ctx, const int32_t* particles, const float* dF, int64_t n,
+                                  float* out);
+
+/* ---- colliders and force fields (CollisionObjectSet, ForceFieldSet) -------- */
+int mpm_set_colliders(mpm_context* ctx, const mpm_collider* colliders, int n);
+/* GravityField list: n vectors of 3 floats (GravityField.cpp:9-17) */
+int mpm_set_gravity(mpm_context* ctx, const float* gravity, int n);
+
+/* ---- grid of one body: Grid (reference include/MpmSim/Grid.h:23-217) ----- */
+/* Grid::Grid (Grid.cpp:156-263): bbox, binning, mass and velocity splat */
+int mpm_grid_create(mpm_context* ctx, const int32_t* particles, int64_t count, float grid_size,
+                    const float frame_velocity[3], int dimension, mpm_grid** out);
+int mpm_grid_destroy(mpm_grid* grid);
+/* m_n, m_min */
+int mpm_grid_dims(const mpm_grid* grid, int n[3], float min[3]);
+/* names: "masses" (n floats), "velocities", "prev_velocities" (3n floats),
+ * "node_collided" (n int8: -1 none, -2 several, >= 0 object) */
+int mpm_grid_get(mpm_grid* grid, const char* name, void* out);
+int mpm_grid_set(mpm_grid* grid, const char* name, const void* in);
+/* computeParticleVolumes (Grid.cpp:265-292) */
+int mpm_grid_compute_particle_volumes(mpm_grid* grid);
+/* calculateForces (Grid.cpp:438-463) */
+int mpm_grid_calculate_forces(mpm_grid* grid, float* forces);
+/* calculateForceDifferentials (Grid.cpp:620-632) */
+int mpm_grid_force_differentials(mpm_grid* grid, const float* dx, float* df);
+/* dForceidXi (Grid.cpp:539-546) */
+int mpm_grid_dforce_dxi(mpm_grid* grid, float* out);
+/* calculateExplicitMomenta (Grid.cpp:634-690); collided: n int8 */
+int mpm_grid_explicit_momenta(mpm_grid* grid, float dt, float* momenta, int8_t* collided);
+/* collisionVelocities (Grid.cpp:692-724) */
+int mpm_grid_collision_velocities(mpm_grid* grid, const int8_t* collided, float* vc);
+/* updateGridVelocities (Grid.cpp:889-1016): implicit solve by diagonal-preconditioned CR */
+int mpm_grid_update_grid_velocities(mpm_grid* grid, float dt, const mpm_solver_params* params,
+                                    mpm_solve_stats* stats);
+/* updateParticleVelocities (Grid.cpp:1049-1081) */
+int mpm_grid_update_particle_velocities(mpm_grid* grid);
+/* updateDeformationGradients (Grid.cpp:1018-1047) */
+int mpm_grid_update_deformation_gradients(mpm_grid* grid, float dt);
+
+/* ---- simulation driver: Sim (reference include/MpmSim/Sim.h:32-55) -------- */
+/* Sim::Sim after the particles and model are set (Sim.cpp:17-50): bodies + volumes */
+int mpm_sim_init(mpm_context* ctx, float grid_size, int dimension);
+/* Sim::advance (Sim.cpp:57-134) */
+int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params, mpm_step_stats* stats);
+int mpm_sim_num_bodies(const mpm_context* ctx);
+/* particle indices of body b (ascending); returns the count (out may be NULL) */
+int64_t mpm_sim_body(const mpm_context* ctx, int body, int32_t* out);
+int64_t mpm_sim_ballistic(const mpm_context* ctx, int32_t* out);
+/* solve statistics of body b in the last advance */
+int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPMSNOW_GPU_H */

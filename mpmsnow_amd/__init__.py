@@ -1,0 +1,410 @@
+"""mpmsnow_amd: MI355X (gfx950) implicit MPM snow step behind a C-ABI.
+
+This module binds include/mpmsnow_gpu.h through ctypes and mirrors the
+reference MpmSim classes (Sim, Grid, SnowConstitutiveModel, ...) closely
+enough that parity tests read like the reference's own tests. All compute
+runs in mpmsnow_amd/_build/libmpmsnow_hip.so; there is no CPU fallback: if
+the library or a GPU is missing, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmpmsnow_hip.so")
+
+MPM_OK = 0
+
+VEC3 = {"p", "v"}
+MAT3 = {"F", "Fp", "FinvTrans", "R", "Ginv"}
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_bp = C.POINTER(C.c_int8)
+_dp = C.POINTER(C.c_double)
+
+TERM_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, _fp, C.c_int64, C.c_int)
+DEBUG_CB = C.CFUNCTYPE(None, C.c_void_p, _fp, C.c_int64)
+CANCEL_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class Collider(C.Structure):
+    _fields_ = [("type", C.c_int), ("coeffs", C.c_float * 4), ("center", C.c_float * 3), ("radius", C.c_float),
+                ("velocity", C.c_float * 3), ("friction", C.c_float), ("sticky", C.c_int)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("max_iters", C.c_int), ("tolerance", C.c_float), ("termination", TERM_CB), ("debug", DEBUG_CB),
+                ("cancelled", CANCEL_CB), ("user", C.c_void_p)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("early_exit", C.c_int), ("rhs_norm2", C.c_double),
+                ("residual_norm2", C.c_double), ("history_length", C.c_int), ("history_capacity", C.c_int),
+                ("history", _dp)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("num_bodies", C.c_int), ("num_ballistic", C.c_int), ("total_iterations", C.c_int),
+                ("max_iterations", C.c_int), ("cancelled", C.c_int)]
+
+
+class MpmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# every symbol include/mpmsnow_gpu.h declares, with its ctypes signature
+SIGNATURES = {
+    "mpm_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "mpm_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_last_error": (C.c_char_p, [C.c_void_p]),
+    "mpm_version": (C.c_int, []),
+    "mpm_particles_create": (C.c_int, [C.c_void_p, C.c_int64]),
+    "mpm_particles_count": (C.c_int64, [C.c_void_p]),
+    "mpm_particles_num_variables": (C.c_int, [C.c_void_p]),
+    "mpm_particles_set": (C.c_int, [C.c_void_p, C.c_char_p, _fp, C.c_int64]),
+    "mpm_particles_get": (C.c_int, [C.c_void_p, C.c_char_p, _fp, C.c_int64]),
+    "mpm_model_snow": (C.c_int, [C.c_void_p] + [C.c_float] * 5),
+    "mpm_model_update_particle_data": (C.c_int, [C.c_void_p]),
+    "mpm_model_energy_density": (C.c_int, [C.c_void_p, _ip, C.c_int64, _fp]),
+    "mpm_model_stress": (C.c_int, [C.c_void_p, _ip, C.c_int64, _fp]),
+    "mpm_model_stress_differential": (C.c_int, [C.c_void_p, _ip, _fp, C.c_int64, _fp]),
+    "mpm_set_colliders": (C.c_int, [C.c_void_p, C.POINTER(Collider), C.c_int]),
+    "mpm_set_gravity": (C.c_int, [C.c_void_p, _fp, C.c_int]),
+    "mpm_grid_create": (C.c_int, [C.c_void_p, _ip, C.c_int64, C.c_float, _fp, C.c_int, C.POINTER(C.c_void_p)]),
+    "mpm_grid_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_grid_dims": (C.c_int, [C.c_void_p, _ip, _fp]),
+    "mpm_grid_get": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    "mpm_grid_set": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    "mpm_grid_compute_particle_volumes": (C.c_int, [C.c_void_p]),
+    "mpm_grid_calculate_forces": (C.c_int, [C.c_void_p, _fp]),
+    "mpm_grid_force_differentials": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "mpm_grid_dforce_dxi": (C.c_int, [C.c_void_p, _fp]),
+    "mpm_grid_explicit_momenta": (C.c_int, [C.c_void_p, C.c_float, _fp, _bp]),
+    "mpm_grid_collision_velocities": (C.c_int, [C.c_void_p, _bp, _fp]),
+    "mpm_grid_update_grid_velocities": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(SolverParams),
+                                                  C.POINTER(SolveStats)]),
+    "mpm_grid_update_particle_velocities": (C.c_int, [C.c_void_p]),
+    "mpm_grid_update_deformation_gradients": (C.c_int, [C.c_void_p, C.c_float]),
+    "mpm_sim_init": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
+    "mpm_sim_advance": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(SolverParams), C.POINTER(StepStats)]),
+    "mpm_sim_num_bodies": (C.c_int, [C.c_void_p]),
+    "mpm_sim_body": (C.c_int64, [C.c_void_p, C.c_int, _ip]),
+    "mpm_sim_ballistic": (C.c_int64, [C.c_void_p, _ip]),
+    "mpm_sim_body_solve_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolveStats)]),
+}
+
+
+def lib():
+    """Load the gfx950 library; raises if it was not built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MpmError("libmpmsnow_hip.so is not built: run __graft_entry__.build() or python mpmsnow_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def to_colmajor(M):
+    M = np.asarray(M, dtype=np.float32).reshape(-1, 3, 3)
+    return np.ascontiguousarray(M.transpose(0, 2, 1)).reshape(-1)
+
+
+def from_colmajor(a):
+    return np.asarray(a).reshape(-1, 3, 3).transpose(0, 2, 1).copy()
+
+
+def make_params(max_iters, tol, termination=None, debug=None, cancelled=None):
+    """SolverParams for SquareMagnitudeTermination(max_iters, tol) plus optional Python hooks."""
+    p = SolverParams()
+    p.max_iters = int(max_iters)
+    p.tolerance = float(tol)
+    keep = []
+    if termination is not None:
+        def _t(user, r, n, it):
+            return int(bool(termination(np.ctypeslib.as_array(r, shape=(n,)).copy(), it)))
+        cb = TERM_CB(_t)
+        keep.append(cb)
+        p.termination = cb
+    if debug is not None:
+        def _d(user, x, n):
+            debug(np.ctypeslib.as_array(x, shape=(n,)).copy())
+        cb = DEBUG_CB(_d)
+        keep.append(cb)
+        p.debug = cb
+    if cancelled is not None:
+        cb = CANCEL_CB(lambda user: int(bool(cancelled())))
+        keep.append(cb)
+        p.cancelled = cb
+    p._keep = keep
+    return p
+
+
+def _stats_dict(st, hist):
+    return {"iterations": st.iterations, "early_exit": bool(st.early_exit), "bnorm2": st.rhs_norm2,
+            "residual_norm2_last": st.residual_norm2, "residual_norm2": hist[:st.history_length].copy()}
+
+
+class Context:
+    """One device context: the particle store (MaterialPointData), the snow model, colliders and fields."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().mpm_create(C.byref(h), device)
+        if rc != MPM_OK:
+            raise MpmError("mpm_create failed (code %d): no usable GPU" % rc)
+        self.ptr = h.value
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.mpm_destroy(self.ptr)
+            self.ptr = None
+
+    __del__ = close
+
+    def check(self, rc):
+        if rc != MPM_OK:
+            raise MpmError(lib().mpm_last_error(self.ptr).decode())
+
+    # MaterialPointData
+    def particles_create(self, n):
+        self.check(lib().mpm_particles_create(self.ptr, int(n)))
+        self.n = int(n)
+
+    def num_variables(self):
+        return lib().mpm_particles_num_variables(self.ptr)
+
+    def set(self, name, arr):
+        flat = to_colmajor(arr) if name in MAT3 else np.asarray(arr, dtype=np.float32).reshape(-1)
+        a, p = _f(flat)
+        self.check(lib().mpm_particles_set(self.ptr, name.encode(), p, a.size))
+
+    def get(self, name):
+        w = 9 if name in MAT3 else 3 if name in VEC3 else 1
+        out = np.zeros(self.n * w, dtype=np.float32)
+        self.check(lib().mpm_particles_get(self.ptr, name.encode(), out.ctypes.data_as(_fp), out.size))
+        if name in MAT3:
+            return from_colmajor(out)
+        if name in VEC3:
+            return out.reshape(self.n, 3)
+        return out
+
+    # SnowConstitutiveModel
+    def model_snow(self, E, nu, xi, theta_c, theta_s):
+        self.check(lib().mpm_model_snow(self.ptr, E, nu, xi, theta_c, theta_s))
+
+    def update_particle_data(self):
+        self.check(lib().mpm_model_update_particle_data(self.ptr))
+
+    def energy_density(self, particles):
+        ii, pi = _i(particles)
+        out = np.zeros(ii.size, dtype=np.float32)
+        self.check(lib().mpm_model_energy_density(self.ptr, pi, ii.size, out.ctypes.data_as(_fp)))
+        return out
+
+    def stress(self, particles):
+        ii, pi = _i(particles)
+        out = np.zeros(9 * ii.size, dtype=np.float32)
+        self.check(lib().mpm_model_stress(self.ptr, pi, ii.size, out.ctypes.data_as(_fp)))
+        return from_colmajor(out)
+
+    def stress_differential(self, particles, dF):
+        ii, pi = _i(particles)
+        d, pd = _f(to_colmajor(dF))
+        out = np.zeros(9 * ii.size, dtype=np.float32)
+        self.check(lib().mpm_model_stress_differential(self.ptr, pi, pd, ii.size, out.ctypes.data_as(_fp)))
+        return from_colmajor(out)
+
+    # CollisionObjectSet / ForceFieldSet
+    def set_colliders(self, planes=(), spheres=()):
+        """planes: (coeffs4, velocity3, friction, sticky); spheres: (center3, radius, velocity3, friction, sticky)."""
+        items = []
+        for coeffs, v, mu, sticky in planes:
+            c = Collider()
+            c.type = 0
+            c.coeffs[:] = list(map(float, coeffs))
+            c.velocity[:] = list(map(float, v))
+            c.friction = mu
+            c.sticky = int(sticky)
+            items.append(c)
+        for center, r, v, mu, sticky in spheres:
+            c = Collider()
+            c.type = 1
+            c.center[:] = list(map(float, center))
+            c.radius = r
+            c.velocity[:] = list(map(float, v))
+            c.friction = mu
+            c.sticky = int(sticky)
+            items.append(c)
+        arr = (Collider * max(1, len(items)))(*items)
+        self.check(lib().mpm_set_colliders(self.ptr, arr, len(items)))
+
+    def set_gravity(self, gs):
+        g = np.asarray(gs, dtype=np.float32).reshape(-1, 3)
+        a, p = _f(g.reshape(-1))
+        self.check(lib().mpm_set_gravity(self.ptr, p, len(g)))
+
+
+class Grid:
+    """Grid of one body (reference include/MpmSim/Grid.h); construction splats mass and velocity."""
+
+    def __init__(self, ctx, inds, h, frame_v=(0.0, 0.0, 0.0), dim=3):
+        self.ctx = ctx
+        ii, pi = _i(inds)
+        fv, pf = _f(frame_v)
+        g = C.c_void_p()
+        ctx.check(lib().mpm_grid_create(ctx.ptr, pi, ii.size, h, pf, dim, C.byref(g)))
+        self.ptr = g.value
+        n = np.zeros(3, dtype=np.int32)
+        mn = np.zeros(3, dtype=np.float32)
+        lib().mpm_grid_dims(self.ptr, n.ctypes.data_as(_ip), mn.ctypes.data_as(_fp))
+        self.n = n
+        self.min = mn
+        self.h = np.float32(h)
+        self.num_nodes = int(np.prod(n.astype(np.int64)))
+
+    def close(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.mpm_grid_destroy(self.ptr)
+            self.ptr = None
+
+    __del__ = close
+
+    def get(self, name):
+        if name == "node_collided":
+            out = np.zeros(self.num_nodes, dtype=np.int8)
+        else:
+            out = np.zeros(self.num_nodes * (3 if "velocities" in name else 1), dtype=np.float32)
+        self.ctx.check(lib().mpm_grid_get(self.ptr, name.encode(), out.ctypes.data))
+        return out
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.int8 if name == "node_collided" else np.float32)
+        self.ctx.check(lib().mpm_grid_set(self.ptr, name.encode(), a.ctypes.data))
+
+    def compute_particle_volumes(self):
+        self.ctx.check(lib().mpm_grid_compute_particle_volumes(self.ptr))
+
+    def calculate_forces(self):
+        out = np.zeros(3 * self.num_nodes, dtype=np.float32)
+        self.ctx.check(lib().mpm_grid_calculate_forces(self.ptr, out.ctypes.data_as(_fp)))
+        return out
+
+    def force_differentials(self, dx):
+        d, pd = _f(dx)
+        out = np.zeros(3 * self.num_nodes, dtype=np.float32)
+        self.ctx.check(lib().mpm_grid_force_differentials(self.ptr, pd, out.ctypes.data_as(_fp)))
+        return out
+
+    def dforce_dxi(self):
+        out = np.zeros(3 * self.num_nodes, dtype=np.float32)
+        self.ctx.check(lib().mpm_grid_dforce_dxi(self.ptr, out.ctypes.data_as(_fp)))
+        return out
+
+    def explicit_momenta(self, dt):
+        em = np.zeros(3 * self.num_nodes, dtype=np.float32)
+        col = np.zeros(self.num_nodes, dtype=np.int8)
+        self.ctx.check(lib().mpm_grid_explicit_momenta(self.ptr, dt, em.ctypes.data_as(_fp), col.ctypes.data_as(_bp)))
+        return em, col
+
+    def collision_velocities(self, collided):
+        c = np.ascontiguousarray(collided, dtype=np.int8)
+        out = np.zeros(3 * self.num_nodes, dtype=np.float32)
+        self.ctx.check(lib().mpm_grid_collision_velocities(self.ptr, c.ctypes.data_as(_bp), out.ctypes.data_as(_fp)))
+        return out
+
+    def update_grid_velocities(self, dt, max_iters, tol, termination=None, debug=None, hist_cap=4096):
+        prm = make_params(max_iters, tol, termination, debug)
+        hist = np.zeros(hist_cap)
+        st = SolveStats()
+        st.history_capacity = hist_cap
+        st.history = hist.ctypes.data_as(_dp)
+        self.ctx.check(lib().mpm_grid_update_grid_velocities(self.ptr, dt, C.byref(prm), C.byref(st)))
+        return _stats_dict(st, hist)
+
+    def update_particle_velocities(self):
+        self.ctx.check(lib().mpm_grid_update_particle_velocities(self.ptr))
+
+    def update_deformation_gradients(self, dt):
+        self.ctx.check(lib().mpm_grid_update_deformation_gradients(self.ptr, dt))
+
+
+class Sim:
+    """Reference Sim (include/MpmSim/Sim.h) on the GPU: particles, snow model, colliders, gravity."""
+
+    def __init__(self, x, m, h, material=None, planes=(), spheres=(), gravity=(), device=0, mu0=None, lambda0=None,
+                 xi=None, v=None):
+        self.ctx = Context(device)
+        x = np.asarray(x, dtype=np.float32).reshape(-1, 3)
+        self.ctx.particles_create(len(x))
+        self.ctx.set("p", x)
+        self.ctx.set("m", np.asarray(m, dtype=np.float32))
+        if v is not None:
+            self.ctx.set("v", v)
+        if material is not None:
+            mat = dict(material)
+            self.ctx.model_snow(mat["E"], mat["nu"], mat["xi"], mat["theta_c"], mat["theta_s"])
+            if mu0 is not None:
+                self.ctx.set("mu0", mu0)
+                self.ctx.set("lambda0", lambda0)
+                self.ctx.set("xi", xi)
+        self.ctx.set_colliders(planes, spheres)
+        if len(gravity):
+            self.ctx.set_gravity(gravity)
+        self.ctx.check(lib().mpm_sim_init(self.ctx.ptr, h, 3))
+        self.h = h
+
+    def advance(self, dt, max_iters, tol, cancelled=None, termination=None, debug=None):
+        prm = make_params(max_iters, tol, termination, debug, cancelled)
+        st = StepStats()
+        self.ctx.check(lib().mpm_sim_advance(self.ctx.ptr, dt, C.byref(prm), C.byref(st)))
+        return {"num_bodies": st.num_bodies, "num_ballistic": st.num_ballistic,
+                "total_iterations": st.total_iterations, "max_iterations": st.max_iterations,
+                "cancelled": bool(st.cancelled)}
+
+    def bodies(self):
+        out = []
+        for b in range(lib().mpm_sim_num_bodies(self.ctx.ptr)):
+            n = lib().mpm_sim_body(self.ctx.ptr, b, None)
+            a = np.zeros(n, dtype=np.int32)
+            lib().mpm_sim_body(self.ctx.ptr, b, a.ctypes.data_as(_ip))
+            out.append(a)
+        return out
+
+    def ballistic(self):
+        n = lib().mpm_sim_ballistic(self.ctx.ptr, None)
+        a = np.zeros(n, dtype=np.int32)
+        lib().mpm_sim_ballistic(self.ctx.ptr, a.ctypes.data_as(_ip))
+        return a
+
+    def body_stats(self, b, hist_cap=4096):
+        hist = np.zeros(hist_cap)
+        st = SolveStats()
+        st.history_capacity = hist_cap
+        st.history = hist.ctypes.data_as(_dp)
+        rc = lib().mpm_sim_body_solve_stats(self.ctx.ptr, b, C.byref(st))
+        if rc != MPM_OK:
+            raise MpmError("no solve statistics for body %d" % b)
+        return _stats_dict(st, hist)
+
+    def get(self, name):
+        return self.ctx.get(name)

@@ -1,0 +1,417 @@
+// mpm_math.hpp -- per-particle / per-node math of the implicit MPM snow step,
+// written for gfx950 device code (also compiles for the host).
+//
+// Semantics follow davidsminor/mpmsnow (paths relative to the reference
+// checkout): B-spline src/MpmSim/CubicBsplineShapeFunction.cpp:11-48, shape
+// iterator src/MpmSim/Grid.cpp:1105-1193, snow model
+// src/MpmSim/SnowConstitutiveModel.cpp:27-287, collisions
+// src/CollisionObject.cpp:19-96 and src/MpmSim/CollisionPlane.cpp:23-46.
+// Matrices are 3x3 column-major float[9] (element (r,c) at r + 3c), the
+// memory order of the reference's Eigen::Matrix3f.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define MPM_HD __host__ __device__ __forceinline__
+
+namespace mpm {
+
+constexpr int kMaxColliders = 16;
+
+MPM_HD float msqrt(float x) { return sqrtf(x); }
+MPM_HD double msqrt(double x) { return sqrt(x); }
+MPM_HD float mabs(float x) { return fabsf(x); }
+MPM_HD double mabs(double x) { return fabs(x); }
+
+// ------------------------------------------------------------ B-spline
+MPM_HD float bspline_w(float x) {
+  float ax = fabsf(x);
+  if (ax < 1.0f) return (0.5f * ax - 1.0f) * ax * ax + 2.0f / 3.0f;
+  if (ax < 2.0f) return ((-ax / 6.0f + 1.0f) * ax - 2.0f) * ax + 4.0f / 3.0f;
+  return 0.0f;
+}
+MPM_HD float bspline_dw(float x) {
+  float s = 1.0f;
+  if (x < 0.0f) { x = -x; s = -1.0f; }
+  if (x < 1.0f) return s * (x * (1.5f * x - 2.0f));
+  if (x < 2.0f) { x -= 2.0f; return s * (-0.5f * x * x); }
+  return 0.0f;
+}
+
+// Per-particle stencil: base node and the 4 weights (and derivatives) per axis.
+struct Stencil {
+  int base[3];
+  float w[3][4];
+  float dw[3][4];  // already divided by h
+};
+
+// Grid.cpp:1105-1136 for dim = 3: base = floor((p - min)/h) - 1; weights at j - frac, j = -1..2.
+MPM_HD void make_stencil(const float p[3], const float gmin[3], float h, bool derivs, Stencil& s) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float frac = (p[d] - gmin[d]) / h;
+    int dimPos = (int)floorf(frac);
+    frac -= (float)dimPos;
+    s.base[d] = dimPos - 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float t = (float)(i - 1) - frac;
+      s.w[d][i] = bspline_w(t);
+      if (derivs) s.dw[d][i] = bspline_dw(t) / h;
+    }
+  }
+}
+// weight of stencil node (a,b,c): w_x * w_y * w_z in the reference's product order
+MPM_HD float st_w(const Stencil& s, int a, int b, int c) { return s.w[0][a] * s.w[1][b] * s.w[2][c]; }
+MPM_HD void st_grad(const Stencil& s, int a, int b, int c, float g[3]) {
+  g[0] = -s.dw[0][a] * s.w[1][b] * s.w[2][c];
+  g[1] = -s.w[0][a] * s.dw[1][b] * s.w[2][c];
+  g[2] = -s.w[0][a] * s.w[1][b] * s.dw[2][c];
+}
+
+// ------------------------------------------------------------ 3x3 algebra
+template <class T> MPM_HD T& at(T* m, int r, int c) { return m[r + 3 * c]; }
+template <class T> MPM_HD T at(const T* m, int r, int c) { return m[r + 3 * c]; }
+
+template <class T> MPM_HD void mat_mul(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) C[r + 3 * c] = A[r] * B[3 * c] + A[r + 3] * B[3 * c + 1] + A[r + 6] * B[3 * c + 2];
+}
+// C = A * B^T
+template <class T> MPM_HD void mat_mul_bt(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) C[r + 3 * c] = A[r] * B[c] + A[r + 3] * B[c + 3] + A[r + 6] * B[c + 6];
+}
+// C = A^T * B
+template <class T> MPM_HD void mat_mul_at(const T* A, const T* B, T* C) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      C[r + 3 * c] = A[3 * r] * B[3 * c] + A[3 * r + 1] * B[3 * c + 1] + A[3 * r + 2] * B[3 * c + 2];
+}
+template <class T> MPM_HD void mat_vec(const T* A, const T* v, T* o) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) o[r] = A[r] * v[0] + A[r + 3] * v[1] + A[r + 6] * v[2];
+}
+// o = A^T v
+template <class T> MPM_HD void mat_tvec(const T* A, const T* v, T* o) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) o[r] = A[3 * r] * v[0] + A[3 * r + 1] * v[1] + A[3 * r + 2] * v[2];
+}
+template <class T> MPM_HD T det3(const T* m) {
+  return at(m, 0, 0) * (at(m, 1, 1) * at(m, 2, 2) - at(m, 2, 1) * at(m, 1, 2)) -
+         at(m, 1, 0) * (at(m, 0, 1) * at(m, 2, 2) - at(m, 2, 1) * at(m, 0, 2)) +
+         at(m, 2, 0) * (at(m, 0, 1) * at(m, 1, 2) - at(m, 1, 1) * at(m, 0, 2));
+}
+template <class T> MPM_HD void inv3(const T* m, T* inv) {
+  T c00 = at(m, 1, 1) * at(m, 2, 2) - at(m, 1, 2) * at(m, 2, 1);
+  T c10 = at(m, 1, 2) * at(m, 2, 0) - at(m, 1, 0) * at(m, 2, 2);
+  T c20 = at(m, 1, 0) * at(m, 2, 1) - at(m, 1, 1) * at(m, 2, 0);
+  T det = at(m, 0, 0) * c00 + at(m, 0, 1) * c10 + at(m, 0, 2) * c20;
+  T id = T(1) / det;
+  at(inv, 0, 0) = c00 * id;
+  at(inv, 1, 0) = c10 * id;
+  at(inv, 2, 0) = c20 * id;
+  at(inv, 0, 1) = (at(m, 0, 2) * at(m, 2, 1) - at(m, 0, 1) * at(m, 2, 2)) * id;
+  at(inv, 1, 1) = (at(m, 0, 0) * at(m, 2, 2) - at(m, 0, 2) * at(m, 2, 0)) * id;
+  at(inv, 2, 1) = (at(m, 0, 1) * at(m, 2, 0) - at(m, 0, 0) * at(m, 2, 1)) * id;
+  at(inv, 0, 2) = (at(m, 0, 1) * at(m, 1, 2) - at(m, 0, 2) * at(m, 1, 1)) * id;
+  at(inv, 1, 2) = (at(m, 0, 2) * at(m, 1, 0) - at(m, 0, 0) * at(m, 1, 2)) * id;
+  at(inv, 2, 2) = (at(m, 0, 0) * at(m, 1, 1) - at(m, 0, 1) * at(m, 1, 0)) * id;
+}
+// reference SnowConstitutiveModel.cpp:258-264
+template <class T> MPM_HD T ddot(const T* a, const T* b) {
+  return at(a, 0, 0) * at(b, 0, 0) + at(a, 0, 1) * at(b, 0, 1) + at(a, 0, 2) * at(b, 0, 2) + at(a, 1, 0) * at(b, 1, 0) +
+         at(a, 1, 1) * at(b, 1, 1) + at(a, 1, 2) * at(b, 1, 2) + at(a, 2, 0) * at(b, 2, 0) + at(a, 2, 1) * at(b, 2, 1) +
+         at(a, 2, 2) * at(b, 2, 2);
+}
+
+// ------------------------------------------------------------ 3x3 SVD
+// One-sided (Hestenes) Jacobi: right rotations orthogonalise the columns of
+// B = A V; sigma_i = |B_i|, U_i = B_i / sigma_i. sigma >= 0, sorted
+// descending; U, V orthogonal (the conventions of Eigen's JacobiSVD). All the
+// model derives from it (R, F^-T, S, J, the clamp) is independent of the
+// sign/order freedom of the decomposition for nonsingular F.
+template <class T, int SWEEPS> MPM_HD void svd3_jacobi(const T* A, T* U, T* s, T* V) {
+  T B[9], W[9] = {T(1), T(0), T(0), T(0), T(1), T(0), T(0), T(0), T(1)};
+#pragma unroll
+  for (int i = 0; i < 9; ++i) B[i] = A[i];
+#pragma unroll
+  for (int sweep = 0; sweep < SWEEPS; ++sweep) {
+#pragma unroll
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0;
+      const int q = pq == 0 ? 1 : 2;
+      T a = B[3 * p] * B[3 * p] + B[3 * p + 1] * B[3 * p + 1] + B[3 * p + 2] * B[3 * p + 2];
+      T b = B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2];
+      T g = B[3 * p] * B[3 * q] + B[3 * p + 1] * B[3 * q + 1] + B[3 * p + 2] * B[3 * q + 2];
+      if (g != T(0)) {
+        T zeta = (b - a) / (T(2) * g);
+        T t = (zeta >= T(0) ? T(1) : T(-1)) / (mabs(zeta) + msqrt(T(1) + zeta * zeta));
+        T cs = T(1) / msqrt(T(1) + t * t);
+        T sn = cs * t;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T bp = B[r + 3 * p], bq = B[r + 3 * q];
+          B[r + 3 * p] = cs * bp - sn * bq;
+          B[r + 3 * q] = sn * bp + cs * bq;
+          T wp = W[r + 3 * p], wq = W[r + 3 * q];
+          W[r + 3 * p] = cs * wp - sn * wq;
+          W[r + 3 * q] = sn * wp + cs * wq;
+        }
+      }
+    }
+  }
+  T n[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) n[c] = msqrt(B[3 * c] * B[3 * c] + B[3 * c + 1] * B[3 * c + 1] + B[3 * c + 2] * B[3 * c + 2]);
+  // sort descending (3-element network), permuting columns of B and W
+  int o0 = 0, o1 = 1, o2 = 2;
+  if (n[o0] < n[o1]) { int t = o0; o0 = o1; o1 = t; }
+  if (n[o1] < n[o2]) { int t = o1; o1 = o2; o2 = t; }
+  if (n[o0] < n[o1]) { int t = o0; o0 = o1; o1 = t; }
+  const int ord[3] = {o0, o1, o2};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s[k] = n[ord[k]];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      V[r + 3 * k] = W[r + 3 * ord[k]];
+      U[r + 3 * k] = B[r + 3 * ord[k]];
+    }
+  }
+  // U columns: normalise; complete a degenerate column orthonormally
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (s[k] > T(1e-30) && s[k] > T(1e-12) * s[0]) {
+      T inv = T(1) / s[k];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) U[r + 3 * k] *= inv;
+    } else if (k == 2) {
+      U[6] = U[1] * U[5] - U[2] * U[4];
+      U[7] = U[2] * U[3] - U[0] * U[5];
+      U[8] = U[0] * U[4] - U[1] * U[3];
+    } else {
+      // rank <= 1: pick e_m (smallest |u0| component) orthogonalised against u0 for k = 1,
+      // or e_x for k = 0
+      T c0 = T(1), c1 = T(0), c2 = T(0);
+      if (k == 1) {
+        int m = 0;
+        if (mabs(U[1]) < mabs(U[m])) m = 1;
+        if (mabs(U[2]) < mabs(U[m])) m = 2;
+        T e[3] = {T(0), T(0), T(0)};
+        e[m] = T(1);
+        T d = e[0] * U[0] + e[1] * U[1] + e[2] * U[2];
+        c0 = e[0] - d * U[0];
+        c1 = e[1] - d * U[1];
+        c2 = e[2] - d * U[2];
+        T nn = T(1) / msqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        c0 *= nn; c1 *= nn; c2 *= nn;
+      }
+      U[3 * k] = c0; U[3 * k + 1] = c1; U[3 * k + 2] = c2;
+    }
+  }
+}
+
+// ------------------------------------------------------- snow model
+struct ParticleCache {  // the model's per-particle state (SnowConstitutiveModel.h:48-57)
+  float F[9], Fp[9], Finv[9], R[9], Ginv[9];
+  float J, mu, lambda;
+};
+
+// updateParticleData for one particle (SnowConstitutiveModel.cpp:27-121).
+// FP32 Jacobi SVD; when sigma_min < 1e-3 the SVD is redone in FP64.
+// Returns false when the hardening factor is not finite ("infinite hardness!!!").
+MPM_HD bool snow_update(float F[9], float Fp[9], float Finv[9], float R[9], float Ginv[9], float& J, float& mu,
+                        float& lambda, float mu0, float lambda0, float xi, float thc, float ths) {
+  float U[9], V[9], sv[3];
+  svd3_jacobi<float, 5>(F, U, sv, V);
+  if (!(sv[2] >= 1e-3f)) {
+    double Fd[9], Ud[9], Vd[9], sd[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fd[i] = (double)F[i];
+    svd3_jacobi<double, 8>(Fd, Ud, sd, Vd);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { U[i] = (float)Ud[i]; V[i] = (float)Vd[i]; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sv[i] = (float)sd[i];
+  }
+  float d[3], di[3];
+  bool modified = false;
+  const float hi = 1.0f + ths, lo = 1.0f - thc;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float x = sv[i];
+    if (x > hi) { modified = true; x = hi; }
+    if (x < lo) { modified = true; x = lo; }
+    d[i] = x;
+    di[i] = 1.0f / x;
+  }
+  // U * diag(d): scale columns
+  float UD[9], UDi[9], VD[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      UD[r + 3 * c] = U[r + 3 * c] * d[c];
+      UDi[r + 3 * c] = U[r + 3 * c] * di[c];
+      VD[r + 3 * c] = V[r + 3 * c] * d[c];
+    }
+  if (modified) {
+    float FN[9], VDi[9], T1[9];
+    mat_mul(F, Fp, FN);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) VDi[r + 3 * c] = V[r + 3 * c] * di[c];
+    mat_mul_bt(VDi, U, T1);  // V D^-1 U^T
+    mat_mul(T1, FN, Fp);
+    mat_mul_bt(UD, V, F);    // U D V^T
+  }
+  mat_mul_bt(UDi, V, Finv);  // U D^-1 V^T
+  mat_mul_bt(U, V, R);       // U V^T
+  float S[9];
+  mat_mul_bt(VD, V, S);      // V D V^T
+  float G[9];
+  at(G, 0, 0) = at(S, 0, 0) + at(S, 1, 1);
+  at(G, 1, 1) = at(S, 0, 0) + at(S, 2, 2);
+  at(G, 2, 2) = at(S, 1, 1) + at(S, 2, 2);
+  at(G, 0, 1) = at(G, 1, 0) = at(S, 1, 2);
+  at(G, 0, 2) = at(G, 2, 0) = -at(S, 0, 2);
+  at(G, 1, 2) = at(G, 2, 1) = at(S, 0, 1);
+  inv3(G, Ginv);
+  J = d[0] * d[1] * d[2];
+  float hf = xi * (1.0f - det3(Fp));
+  if (hf > 2.0f) hf = 2.0f;
+  hf = expf(hf);
+  mu = mu0 * hf;
+  lambda = lambda0 * hf;
+  return isfinite(hf);
+}
+
+// dEnergyDensitydF (SnowConstitutiveModel.cpp:187-216): 2 mu (F - R) + lambda (J-1) J F^-T
+MPM_HD void snow_stress(const float* F, const float* R, const float* Finv, float J, float mu, float lambda, float* P) {
+  float a = 2.0f * mu;
+  float b = lambda * (J - 1.0f) * J;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) P[i] = a * (F[i] - R[i]) + b * Finv[i];
+}
+// energyDensity (SnowConstitutiveModel.cpp:171-185)
+MPM_HD float snow_energy(const float* F, const float* R, float J, float mu, float lambda) {
+  float rd[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) rd[i] = F[i] - R[i];
+  float jm1 = J - 1.0f;
+  return mu * ddot(rd, rd) + 0.5f * lambda * jm1 * jm1;
+}
+// dEdFDifferential (SnowConstitutiveModel.cpp:218-256) with computeRdifferential (:266-287)
+MPM_HD void snow_stress_differential(const float* dF, const float* R, const float* Finv, const float* Ginv, float J,
+                                     float mu, float lambda, float* dP) {
+  float dJ = J * ddot(Finv, dF);
+  // dFinvT = -F^-T dF^T F^-T
+  float T1[9], dFi[9];
+  mat_mul_bt(Finv, dF, T1);
+  mat_mul(T1, Finv, dFi);
+  // dR = R * skew(Ginv * w(R^T dF))
+  float M[9];
+  mat_mul_at(R, dF, M);
+  float w0 = at(M, 0, 1) - at(M, 1, 0);
+  float w1 = at(M, 0, 2) - at(M, 2, 0);
+  float w2 = at(M, 1, 2) - at(M, 2, 1);
+  float g0 = at(Ginv, 0, 0) * w0 + at(Ginv, 0, 1) * w1 + at(Ginv, 0, 2) * w2;
+  float g1 = at(Ginv, 1, 0) * w0 + at(Ginv, 1, 1) * w1 + at(Ginv, 1, 2) * w2;
+  float g2 = at(Ginv, 2, 0) * w0 + at(Ginv, 2, 1) * w1 + at(Ginv, 2, 2) * w2;
+  // R * [[0, g0, g1], [-g0, 0, g2], [-g1, -g2, 0]]
+  float dR[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float r0 = at(R, r, 0), r1 = at(R, r, 1), r2 = at(R, r, 2);
+    at(dR, r, 0) = -r1 * g0 - r2 * g1;
+    at(dR, r, 1) = r0 * g0 - r2 * g2;
+    at(dR, r, 2) = r0 * g1 + r1 * g2;
+  }
+  float twoMu = 2.0f * mu;
+  float dJJ = dJ * J;
+  float jm1 = J - 1.0f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    dP[i] = twoMu * (dF[i] - dR[i]) + lambda * (dJJ * Finv[i] + jm1 * (dJ * Finv[i] - J * dFi[i]));
+}
+
+// ------------------------------------------------------------ colliders
+struct Collider {
+  int type;  // 0 plane, 1 sphere
+  float c[4];
+  float center[3];
+  float radius;
+  float v[3];
+  float friction;
+  int sticky;
+};
+struct ColliderSet {
+  int n;
+  Collider o[kMaxColliders];
+};
+
+MPM_HD float col_phi(const Collider& o, const float x[3]) {
+  if (o.type == 0) return o.c[0] * x[0] + o.c[1] * x[1] + o.c[2] * x[2] + o.c[3];
+  float d0 = x[0] - o.center[0], d1 = x[1] - o.center[1], d2 = x[2] - o.center[2];
+  return sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o.radius;
+}
+// normalised gradient (CollisionObject::grad + Vector3f::normalize)
+MPM_HD void col_normal(const Collider& o, const float x[3], float n[3]) {
+  if (o.type == 0) {
+    n[0] = o.c[0]; n[1] = o.c[1]; n[2] = o.c[2];
+  } else {
+    n[0] = x[0] - o.center[0]; n[1] = x[1] - o.center[1]; n[2] = x[2] - o.center[2];
+  }
+  float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  n[0] /= nn; n[1] /= nn; n[2] /= nn;
+}
+
+// CollisionObjectSet::collide (src/CollisionObject.cpp:19-96)
+MPM_HD int collide(const ColliderSet& cs, float v[3], const float x[3], const float fv[3], bool addVel) {
+  int obj = -1;
+  bool already = false;
+  for (int i = 0; i < cs.n; ++i) {
+    const Collider& o = cs.o[i];
+    if (col_phi(o, x) <= 0.0f) {
+      if (already) {
+        obj = -2;
+        v[0] = v[1] = v[2] = 0.0f;
+        break;
+      }
+      already = true;
+      float vo[3] = {o.v[0] - fv[0], o.v[1] - fv[1], o.v[2] - fv[2]};
+      v[0] -= vo[0]; v[1] -= vo[1]; v[2] -= vo[2];
+      float n[3];
+      col_normal(o, x, n);
+      float nd = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+      if (nd < 0.0f) {
+        obj = i;
+        if (o.sticky) {
+          v[0] = v[1] = v[2] = 0.0f;
+        } else {
+          float t0 = v[0] - nd * n[0], t1 = v[1] - nd * n[1], t2 = v[2] - nd * n[2];
+          float tn = sqrtf(t0 * t0 + t1 * t1 + t2 * t2);
+          if (tn >= -nd * o.friction) {
+            float s = 1.0f + o.friction * nd / tn;
+            v[0] = t0 * s; v[1] = t1 * s; v[2] = t2 * s;
+          } else {
+            v[0] = v[1] = v[2] = 0.0f;
+          }
+        }
+      }
+      if (addVel) { v[0] += vo[0]; v[1] += vo[1]; v[2] += vo[2]; }
+    }
+  }
+  return obj;
+}
+
+}  // namespace mpm

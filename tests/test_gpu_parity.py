@@ -1,0 +1,368 @@
+"""GPU parity: the gfx950 library (through its C-ABI) against the CPU oracle on identical seeded inputs.
+
+Bars (north-star): cell binning and node indexing bit-exact; per-phase grid
+quantities, CR residual histories and particle F within 1e-5 relative (FP32
+bulk against the FP64 oracle, scaled by the array's magnitude); positions
+within 1e-4 h after multi-step runs. Printed case names precede every GPU step.
+"""
+import numpy as np
+import pytest
+
+import mpmsnow_amd as G
+
+pytestmark = pytest.mark.gpu
+
+F32, F64 = "float", "double"
+REL = 1e-5
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def rand_F(rng, n, scale):
+    return np.eye(3)[None] + scale * rng.uniform(-1, 1, (n, 3, 3))
+
+
+def ball(rng, n, c, r):
+    p = rng.uniform(-1, 1, (4 * n, 3))
+    p = p[(p * p).sum(1) < 1][:n]
+    return (np.asarray(c) + r * p).astype(np.float32)
+
+
+class Pair:
+    """The same particle state in the GPU context and in an oracle ParticleData (float and double)."""
+
+    def __init__(self, oracle, x, v, m, F=None, E=1.4e5, nu=0.2, xi=10.0, thc=2.5e-2, ths=7.5e-3, vol=None,
+                 update=True):
+        self.n = len(x)
+        self.ctx = G.Context()
+        self.ctx.particles_create(self.n)
+        self.ctx.set("p", x)
+        self.ctx.set("v", v)
+        self.ctx.set("m", m)
+        if F is not None:
+            self.ctx.set("F", F.astype(np.float32))
+        if vol is not None:
+            self.ctx.set("volume", vol)
+        self.ctx.model_snow(E, nu, xi, thc, ths)
+        self.o = {}
+        self.om = {}
+        for prec in (F32, F64):
+            pd = oracle.ParticleData.create(self.n, prec)
+            pd.set("p", x)
+            pd.set("v", v)
+            pd.set("m", m)
+            if F is not None:
+                pd.set("F", F.astype(np.float32))
+            if vol is not None:
+                pd.set("volume", vol)
+            model = oracle.Model.snow(E, nu, xi, thc, ths, prec)
+            model.set_particles(pd)
+            self.o[prec] = pd
+            self.om[prec] = model
+        if update:
+            self.ctx.update_particle_data()
+            for prec in (F32, F64):
+                self.om[prec].update_particle_data()
+
+
+def deformed_body(oracle, seed=0, n=3000, h=1.0 / 32, vol=True):
+    rng = np.random.default_rng(seed)
+    x = ball(rng, n, (0.5, 0.5, 0.5), 0.12)
+    v = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    m = np.full(n, 1e-3, dtype=np.float32)
+    F = rand_F(rng, n, 0.02).astype(np.float32)
+    volume = np.full(n, 2e-6, dtype=np.float32) if vol else None
+    return Pair(oracle, x, v, m, F=F, vol=volume), h
+
+
+def test_binning_and_p2g(oracle):
+    say("binning_and_p2g seed=0 n=3000")
+    P, h = deformed_body(oracle, vol=False)
+    inds = np.arange(P.n)
+    fv = np.array([0.1, -0.2, 0.05], dtype=np.float32)
+    g = G.Grid(P.ctx, inds, h, frame_v=fv)
+    og = oracle.Grid(P.o[F32], inds, h, frame_v=fv)
+    og64 = oracle.Grid(P.o[F64], inds, float(np.float32(h)), frame_v=fv)
+    # bit-exact geometry: n and m_min (float) equal the reference formula
+    assert (g.n == og.n).all()
+    assert (g.min == og.min.astype(np.float32)).all()
+    # bit-exact base cells: compare each particle's stencil base via the oracle iterator
+    x = P.ctx.get("p")
+    base_gpu = np.floor((x - g.min) / np.float32(h)).astype(np.int64) - 1
+    for k in range(0, P.n, 97):
+        nodes, w, dw = og.stencil(x[k].astype(np.float64))
+        assert (nodes[0] == base_gpu[k]).all()
+    M = g.get("masses")
+    V = g.get("velocities")
+    assert relerr(M, og64.get("masses")) < REL
+    assert relerr(M, og.get("masses")) < REL
+    Mo = og64.get("masses")
+    heavy = np.repeat(Mo > 1e-3 * Mo.max(), 3)
+    assert relerr(V[heavy], og64.get("velocities")[heavy]) < 1e-4
+    assert np.abs(g.get("prev_velocities") - V).max() == 0
+
+
+def test_model_probes_and_plasticity(oracle):
+    say("model_probes_and_plasticity seed=1 n=512")
+    rng = np.random.default_rng(1)
+    n = 512
+    F = np.eye(3)[None] + 0.06 * rng.uniform(-1, 1, (n, 3, 3))
+    F[::7] = np.diag([1.05, 0.9, 1.0])  # clamped on both sides
+    Fp = np.eye(3)[None] + 0.01 * rng.uniform(-1, 1, (n, 3, 3))
+    x = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    P = Pair(oracle, x, np.zeros((n, 3), np.float32), np.ones(n, np.float32), F=F, update=False)
+    P.ctx.set("Fp", Fp.astype(np.float32))
+    for prec in (F32, F64):
+        P.o[prec].set("Fp", Fp.astype(np.float32))
+    P.ctx.update_particle_data()
+    for prec in (F32, F64):
+        P.om[prec].update_particle_data()
+    for name in ("F", "Fp", "R", "FinvTrans", "Ginv", "J", "mu", "lambda"):
+        a = P.ctx.get(name)
+        b = P.o[F64].get(name)
+        assert relerr(a, b) < REL, name
+    idx = np.arange(n)
+    e = P.ctx.energy_density(idx)
+    eo = np.array([P.om[F64].energy_density(i) for i in idx])
+    assert np.abs(e - eo).max() < REL * np.abs(eo).max()
+    S = P.ctx.stress(idx)
+    So = np.array([P.om[F64].dEdF(i) for i in idx])
+    assert relerr(S, So) < REL
+    dF = 1e-3 * rng.uniform(-1, 1, (n, 3, 3))
+    dS = P.ctx.stress_differential(idx, dF)
+    dSo = np.array([P.om[F64].dEdF_differential(dF[i].astype(np.float32), i) for i in idx])
+    assert relerr(dS, dSo) < REL
+
+
+def test_forces_hvp_diagonal(oracle):
+    say("forces_hvp_diagonal seed=0 n=3000")
+    P, h = deformed_body(oracle)
+    P.ctx.set_gravity([[0, -9.81, 0]])
+    inds = np.arange(P.n)
+    g = G.Grid(P.ctx, inds, h)
+    og = oracle.Grid(P.o[F64], inds, float(np.float32(h)))
+    fields = oracle.Fields.create(F64)
+    fields.add_gravity([0, -9.81, 0])
+    f = g.calculate_forces()
+    fo = og.calculate_forces(P.om[F64], fields)
+    assert relerr(f, fo) < REL
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=3 * g.num_nodes).astype(np.float32)
+    df = g.force_differentials(d)
+    dfo = og.force_differentials(P.om[F64], d.astype(np.float64))
+    assert relerr(df, dfo) < REL
+    dg = g.dforce_dxi()
+    dgo = og.dforce_dxi(P.om[F64])
+    assert relerr(dg, dgo) < REL
+
+
+def test_dfidxi_equals_hvp_columns(oracle):
+    """TestGrid.cpp:806-852 on the GPU: diagonal entry = HVP(e_i)_i (FP32 rounding, relative 1e-5)."""
+    say("dfidxi_vs_hvp single particle")
+    rng = np.random.default_rng(29)
+    F = (np.eye(3) + 0.1 * rng.uniform(-1, 1, (3, 3)))[None]
+    P = Pair(oracle, np.zeros((1, 3), np.float32), np.zeros((1, 3), np.float32), np.ones(1, np.float32), F=F,
+             E=1.4e5, nu=0.2, xi=0, thc=1e5, ths=1e5, vol=np.array([0.5], np.float32))
+    g = G.Grid(P.ctx, [0], 1.0)
+    diag = g.dforce_dxi()
+    e = np.zeros(3 * g.num_nodes, np.float32)
+    worst = 0.0
+    for i in np.nonzero(diag)[0]:
+        e[i] = 1
+        worst = max(worst, abs(g.force_differentials(e)[i] - diag[i]))
+        e[i] = 0
+    assert worst <= REL * np.abs(diag).max()
+
+
+def implicit_pair(oracle, fv=(0, 0, 0)):
+    rng = np.random.default_rng(31)
+    h = 0.5
+    distortion = 0.001 * rng.uniform(-1, 1, (3, 3))
+    pos, vel, Fs = [], [], []
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                p = np.array([(i - 0.5) * h, (j - 0.5) * h, (k - 0.5) * h], dtype=np.float32)
+                pos.append(p)
+                vel.append(0.1 + 0.01 * np.sin(2 * p))
+                Fs.append(np.eye(3) + distortion * np.cos(2 * p[0]))
+    n = len(pos)
+    P = Pair(oracle, np.array(pos, np.float32), np.array(vel, np.float32), np.ones(n, np.float32),
+             F=np.array(Fs), E=1.4e5, nu=0.2, xi=0, thc=1e5, ths=1e5, vol=np.ones(n, np.float32), update=False)
+    planes = [((-1, -1, -1, 1), (-0.1, -0.2, -0.3), 0.5, False), ((-1, 0, 0, 1.5), (-0.2, -0.1, -0.4), 0.5, False)]
+    P.ctx.set_colliders(planes)
+    cols = oracle.Colliders.create(F64)
+    for c, v, mu, st in planes:
+        cols.add_plane(c, v=v, friction=mu, sticky=st)
+    return P, h, cols
+
+
+def test_explicit_momenta_and_collisions(oracle):
+    say("explicit_momenta two moving planes")
+    P, h, cols = implicit_pair(oracle)
+    inds = np.arange(P.n)
+    g = G.Grid(P.ctx, inds, h)
+    og = oracle.Grid(P.o[F64], inds, h)
+    g.compute_particle_volumes()
+    og.compute_particle_volumes()
+    assert relerr(P.ctx.get("volume"), P.o[F64].get("volume")) < REL
+    fields = oracle.Fields.create(F64)
+    em, col = g.explicit_momenta(0.002)
+    emo, colo = og.explicit_momenta(0.002, P.om[F64], cols, fields)
+    assert (col == colo).all()
+    assert relerr(em, emo) < REL
+    vc = g.collision_velocities(col)
+    vco = og.collision_velocities(cols, colo)
+    assert np.abs(vc - vco).max() < 1e-7
+
+
+def test_implicit_solve_against_oracle(oracle):
+    """updateGridVelocities: final velocities, iteration counts and residual history vs the FP64 oracle."""
+    say("implicit_solve two moving planes maxIters=40 tol=0")
+    P, h, cols = implicit_pair(oracle)
+    inds = np.arange(P.n)
+    g = G.Grid(P.ctx, inds, h)
+    og = oracle.Grid(P.o[F64], inds, h)
+    g.compute_particle_volumes()
+    og.compute_particle_volumes()
+    fields = oracle.Fields.create(F64)
+    st = g.update_grid_velocities(0.002, 40, 0.0)
+    sto = og.update_grid_velocities(0.002, P.om[F64], cols, fields, 40, 0.0)
+    assert st["iterations"] == sto["iterations"] == 41
+    v = g.get("velocities")
+    vo = og.get("velocities")
+    assert relerr(v, vo) < 1e-4
+    assert abs(st["bnorm2"] - sto["bnorm2"]) < REL * sto["bnorm2"]
+    # residual histories agree while they are above the FP32 floor
+    rh, rho = st["residual_norm2"], sto["residual_norm2"]
+    k = np.nonzero(rho > 1e-8 * rho[0])[0]
+    k = k[k < 10]
+    assert np.all(np.abs(rh[k] - rho[k]) <= 1e-3 * rho[k])
+
+
+def test_implicit_solve_tolerance_terminates_like_oracle(oracle):
+    say("implicit_solve tol=1e-5")
+    P, h, cols = implicit_pair(oracle)
+    inds = np.arange(P.n)
+    g = G.Grid(P.ctx, inds, h)
+    og = oracle.Grid(P.o[F64], inds, h)
+    g.compute_particle_volumes()
+    og.compute_particle_volumes()
+    fields = oracle.Fields.create(F64)
+    st = g.update_grid_velocities(0.002, 200, 1e-5)
+    sto = og.update_grid_velocities(0.002, P.om[F64], cols, fields, 200, 1e-5)
+    assert abs(st["iterations"] - sto["iterations"]) <= 1
+    assert relerr(g.get("velocities"), og.get("velocities")) < 1e-4
+
+
+def test_g2p_and_deformation_gradient(oracle):
+    say("g2p_and_F seed=0 n=3000")
+    P, h = deformed_body(oracle)
+    inds = np.arange(P.n)
+    g = G.Grid(P.ctx, inds, h, frame_v=(0.1, 0, 0))
+    og = oracle.Grid(P.o[F64], inds, float(np.float32(h)), frame_v=(0.1, 0, 0))
+    rng = np.random.default_rng(3)
+    newv = (og.get("velocities") + rng.normal(scale=0.1, size=3 * og.num_nodes)).astype(np.float32)
+    g.set("velocities", newv)
+    og.set("velocities", newv.astype(np.float64))
+    g.update_particle_velocities()
+    og.update_particle_velocities()
+    assert relerr(P.ctx.get("v"), P.o[F64].get("v")) < REL
+    g.update_deformation_gradients(1e-3)
+    og.update_deformation_gradients(1e-3)
+    assert relerr(P.ctx.get("F"), P.o[F64].get("F")) < REL
+
+
+def reference_sim_scene(ballistic=True):
+    h = 0.1
+    pos = []
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                pos.append([0.5 * h * (i + 0.5), 0.5 * h * (j + 0.5), 0.5 * h * (k + 0.5)])
+                pos.append([0.5 * h * (i + 0.5) + 6, 0.5 * h * (j + 0.5), 0.5 * h * (k + 0.5)])
+    if ballistic:
+        for i in range(4):
+            for j in range(4):
+                for k in range(4):
+                    pos.append([1.1 * h * i + 20, 1.1 * h * j, 1.1 * h * k])
+    return np.array(pos, dtype=np.float32), h
+
+
+def test_sim_timestep_advance_reference_case():
+    """TestSimClass.cpp:120-190 on the GPU: mean v_y = 0.03 +- 1e-5, v_x, v_z within 1e-6."""
+    say("sim reference timestep advance")
+    pos, h = reference_sim_scene()
+    sim = G.Sim(pos, np.ones(len(pos)), h, material=dict(E=1.4e5, nu=0.2, xi=0, theta_c=1e5, theta_s=1e5),
+                gravity=[[0, 1.0, 0]])
+    assert len(sim.bodies()) == 2 and len(sim.ballistic()) == 64
+    for _ in range(3):
+        sim.advance(0.01, 10, 0.0)
+    v = sim.get("v").mean(0)
+    assert abs(v[0]) < 1e-6 and abs(v[2]) < 1e-6 and abs(v[1] - 0.03) < 1e-5
+
+
+def test_sim_initialization_reference_case():
+    """TestSimClass.cpp:45-118: 2 bodies x 4096 particles and 4096 ballistic particles."""
+    say("sim reference initialization 16^3")
+    h = 0.1
+    pos = []
+    for i in range(16):
+        for j in range(16):
+            for k in range(16):
+                pos.append([0.5 * h * (i + 0.5), 0.5 * h * (j + 0.5), 0.5 * h * (k + 0.5)])
+                pos.append([0.5 * h * (i + 0.5) + 6, 0.5 * h * (j + 0.5), 0.5 * h * (k + 0.5)])
+    for i in range(16):
+        for j in range(16):
+            for k in range(16):
+                pos.append([1.1 * h * i + 20, 1.1 * h * j, 1.1 * h * k])
+    pos = np.array(pos, dtype=np.float32)
+    ctx = G.Context()
+    ctx.particles_create(len(pos))
+    ctx.set("p", pos)
+    ctx.set("m", np.ones(len(pos)))
+    assert ctx.num_variables() == 5
+    with pytest.raises(G.MpmError):
+        ctx.get("Fiddlesticks")
+    ctx.check(G.lib().mpm_sim_init(ctx.ptr, h, 3))
+    nb = G.lib().mpm_sim_num_bodies(ctx.ptr)
+    assert nb == 2
+    assert G.lib().mpm_sim_body(ctx.ptr, 0, None) == 4096 and G.lib().mpm_sim_body(ctx.ptr, 1, None) == 4096
+    assert G.lib().mpm_sim_ballistic(ctx.ptr, None) == 4096
+
+
+def test_snowball_steps_against_oracle(oracle):
+    """Snowball drop (2000 particles, h=1/64, plane + gravity): 10 steps, GPU vs FP64 oracle."""
+    from mpmsnow_amd import scenes
+
+    say("snowball 2000 particles 10 steps")
+    sc = scenes.snowball_drop(n=2000, seed=3, contact=True)
+    mat = sc["material"]
+    sim = G.Sim(sc["x"], sc["m"], sc["h"], material=mat, planes=sc["planes"], gravity=[sc["gravity"]], v=sc["v"])
+    cols = oracle.Colliders.create(F64)
+    for c, v, mu, st in sc["planes"]:
+        cols.add_plane(c, v=v, friction=mu, sticky=st)
+    fields = oracle.Fields.create(F64)
+    fields.add_gravity(sc["gravity"])
+    model = oracle.Model.snow(mat["E"], mat["nu"], mat["xi"], mat["theta_c"], mat["theta_s"], F64)
+    osim = oracle.Sim(sc["x"], sc["m"], float(np.float32(sc["h"])), model, cols, fields, prec=F64)
+    osim.pd.set("v", sc["v"])
+    for step in range(10):
+        st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        osim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        its = [sim.body_stats(b)["iterations"] for b in range(st["num_bodies"])]
+        ots = [s["iterations"] for s in osim.last_stats()]
+        assert len(its) == len(ots)
+        assert all(abs(a - b) <= 1 for a, b in zip(its, ots)), (step, its, ots)
+    x, xo = sim.get("p"), osim.pd.get("p")
+    assert np.abs(x - xo).max() < 1e-4 * sc["h"]
+    assert relerr(sim.get("v"), osim.pd.get("v")) < 1e-4
+    assert relerr(sim.get("F"), osim.pd.get("F")) < REL
