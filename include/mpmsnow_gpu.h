@@ -161,6 +161,15 @@ int64_t mpm_sim_ballistic(const mpm_context* ctx, int32_t* out);
 /* solve statistics of body b in the last advance */
 int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* stats);
 
+/* ---- measurement (no counterpart in the reference) ----------------------------
+ * Kernel timing with HIP events on the context's stream. Kernel classes:
+ * p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p.
+ * algorithmic_bytes sums, over the timed launches, bytes/particle * particles +
+ * bytes/node * nodes with m > 0 of that kernel's minimal HBM traffic. */
+int mpm_profile_enable(mpm_context* ctx, int on); /* also resets the counters */
+int mpm_profile_query(mpm_context* ctx, const char* kernel, double* total_ms, int64_t* launches,
+                      double* algorithmic_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,7 +95,12 @@ SIGNATURES = {
     "mpm_sim_body": (C.c_int64, [C.c_void_p, C.c_int, _ip]),
     "mpm_sim_ballistic": (C.c_int64, [C.c_void_p, _ip]),
     "mpm_sim_body_solve_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolveStats)]),
+    "mpm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_double)]),
 }
+
+PROFILE_KERNELS = ("p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p", "cr_update_x", "cr_ar", "cr_update_p")
 
 
 def lib():
@@ -408,3 +413,16 @@ class Sim:
 
     def get(self, name):
         return self.ctx.get(name)
+
+    def profile(self, on=True):
+        self.ctx.check(lib().mpm_profile_enable(self.ctx.ptr, int(on)))
+
+    def profile_query(self):
+        out = {}
+        for k in PROFILE_KERNELS:
+            ms = C.c_double()
+            n = C.c_int64()
+            by = C.c_double()
+            self.ctx.check(lib().mpm_profile_query(self.ctx.ptr, k.encode(), C.byref(ms), C.byref(n), C.byref(by)))
+            out[k] = {"ms": ms.value, "launches": n.value, "bytes": by.value}
+        return out

@@ -91,6 +91,7 @@ struct CRScalars {
   int early_exit;          // zero denominator
   int max_iters;
   int errors;              // ErrorBits
+  int active_nodes;        // nodes with m > 0 (measurement only)
   unsigned int ticket[4];  // last-block-done counters
 };
 

@@ -615,6 +615,7 @@ __global__ void k_node_explicit(GridView g, const float* mass, const float* vel,
   int64_t nn = (int64_t)g.NT * 64;
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int bad = 0;
+  int active = 0;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nn; i += (int64_t)gridDim.x * kBlock) {
     int t = (int)(i >> 6), l = (int)(i & 63);
     int64_t b = (int64_t)t * 192 + l;
@@ -627,6 +628,7 @@ __global__ void k_node_explicit(GridView g, const float* mass, const float* vel,
       continue;
     }
     float m = mass[i];
+    active += m > 0.0f ? 1 : 0;
     float f[3] = {0.0f, 0.0f, 0.0f};
     field_force(grav, ngrav, m, f);
     f[0] += acc[b];
@@ -667,6 +669,8 @@ __global__ void k_node_explicit(GridView g, const float* mass, const float* vel,
     s[3] += (double)e2;
   }
   if (bad) atomicOr(&sc->errors, ERR_NAN_MOMENTA);
+  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
+  if ((threadIdx.x & 63) == 0 && active) atomicAdd(&sc->active_nodes, active);
   double tot[8];
   if (grid_sum<8>(s, partials, &sc->ticket[0], tot) && threadIdx.x == 0) {
     for (int k = 0; k < 8; ++k) sc->sums[k] = tot[k];

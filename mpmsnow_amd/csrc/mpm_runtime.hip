@@ -101,6 +101,15 @@ float decode_ord(unsigned int u) {
 
 }  // namespace
 
+// Kernel timing for measurement: HIP events recorded on the library's stream around
+// each instrumented launch, resolved at the next synchronisation point. Algorithmic
+// bytes per launch = bytes per particle * particles + bytes per node * nodes with m > 0.
+enum ProfKind { PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_COUNT };
+const char* const kProfNames[PK_COUNT] = {"p2g_mass", "p2g_vel", "forces", "hvp", "diag",
+                                          "g2p", "cr_update_x", "cr_ar", "cr_update_p"};
+const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0};
+const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84};
+
 struct SolveRecord {
   int iterations = 0;
   int early_exit = 0;
@@ -133,6 +142,47 @@ struct mpm_context {
   std::vector<std::unique_ptr<mpm_grid>> gridPool;
   DevBuf new_to_old, ball_slots, other_slots;
   int iterHint = 8;
+  // measurement
+  struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+    const mpm_grid* grid;
+  };
+  bool profOn = false;
+  std::vector<ProfRec> profPending;
+  std::vector<hipEvent_t> eventPool;
+  double profMs[PK_COUNT] = {};
+  int64_t profCount[PK_COUNT] = {};
+  double profBytes[PK_COUNT] = {};
+  hipEvent_t new_event() {
+    if (!eventPool.empty()) {
+      hipEvent_t e = eventPool.back();
+      eventPool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  int prof_begin(int kind, const mpm_grid* g) {
+    if (!profOn) return -1;
+    ProfRec r{kind, new_event(), new_event(), g};
+    HIPCHECK(hipEventRecord(r.a, stream));
+    profPending.push_back(r);
+    return (int)profPending.size() - 1;
+  }
+  void prof_end(int idx) {
+    if (idx < 0) return;
+    HIPCHECK(hipEventRecord(profPending[idx].b, stream));
+  }
+  void prof_resolve();
+  ~mpm_context() {
+    for (auto& r : profPending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : eventPool) (void)hipEventDestroy(e);
+  }
 
   PView view(int which = -1) const {
     int w = which < 0 ? cur : which;
@@ -178,6 +228,7 @@ struct mpm_grid {
   mpm_context* ctx = nullptr;
   GridView g{};
   int np = 0;
+  int activeNodes = 0;
   float fv[3] = {0, 0, 0};
   DevBuf slots, perm, key, rank, count, offset, nch, chunk_off, chunks, scantmp, totals;
   DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, acc, acc2, diag, draw, sc, partials, hist, dense, dense2;
@@ -275,9 +326,13 @@ struct mpm_grid {
     HIPCHECK(hipMemsetAsync(flags.p, 0xFF, NN, s));  // -1: no collision until updateGridVelocities
     HIPCHECK(hipMemsetAsync(sc.p, 0, sizeof(CRScalars), s));
     // MassSplatter, clamp/NaN, VelocitySplatter, NaN, prevVelocities
+    int pr = c.prof_begin(PK_P2G_MASS, this);
     launch_p2g_mass(c.view(), g, f(mass), c.errp(), s);
+    c.prof_end(pr);
     launch_mass_fix(f(mass), g, c.errp(), s);
+    pr = c.prof_begin(PK_P2G_VEL, this);
     launch_p2g_vel(c.view(), g, f(mass), f(vel), c.errp(), s);
+    c.prof_end(pr);
     launch_check_vel(f(vel), f(prev), g, c.errp(), s);
   }
 
@@ -302,7 +357,14 @@ struct mpm_grid {
   // internal forces into acc (zeroed first)
   void forces_into_acc() {
     HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), ctx->stream));
+    int pr = ctx->prof_begin(PK_FORCES, this);
     launch_forces(ctx->view(), g, f(acc), ctx->errp(), ctx->stream);
+    ctx->prof_end(pr);
+  }
+  void hvp(const float* d, bool project, float* out, const CRScalars* scp) {
+    int pr = ctx->prof_begin(PK_HVP, this);
+    launch_hvp(ctx->view(), g, d, project, flags.as<int8_t>(), ctx->cols(), out, scp, ctx->errp(), ctx->stream);
+    ctx->prof_end(pr);
   }
 
   // Grid::updateGridVelocities (Grid.cpp:889-1016)
@@ -331,17 +393,19 @@ struct mpm_grid {
                          scal(), partials.as<double>(), s);
     // DF(vc) (no input projection) and the diagonal
     HIPCHECK(hipMemsetAsync(acc2.p, 0, vecBytes(), s));
-    launch_hvp(c.view(), g, f(vc), false, flags.as<int8_t>(), c.cols(), f(acc2), nullptr, c.errp(), s);
+    hvp(f(vc), false, f(acc2), nullptr);
     HIPCHECK(hipMemsetAsync(draw.p, 0, vecBytes(), s));
+    int pr = c.prof_begin(PK_DIAG, this);
     launch_diag(c.view(), g, f(draw), c.errp(), s);
+    c.prof_end(pr);
     launch_rhs(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc2), f(draw), dt, tol, f(b), f(x), f(diag), scal(),
                partials.as<double>(), s);
     // CR setup: A x, A z (ConjugateResiduals.cpp:50-76)
     HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), s));
-    launch_hvp(c.view(), g, f(x), true, flags.as<int8_t>(), c.cols(), f(acc), scal(), c.errp(), s);
+    hvp(f(x), true, f(acc), scal());
     launch_cr_setup1(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc), f(x), f(b), f(diag), dt, f(z), f(p), scal(), s);
     HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), s));
-    launch_hvp(c.view(), g, f(z), true, flags.as<int8_t>(), c.cols(), f(acc), scal(), c.errp(), s);
+    hvp(f(z), true, f(acc), scal());
     launch_cr_setup2(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc), f(z), f(diag), dt, f(Ap), scal(),
                      partials.as<double>(), s);
     CRScalars* hsc = c.pinned.as<CRScalars>();
@@ -350,6 +414,7 @@ struct mpm_grid {
       c.sync();
     };
     fetch();
+    activeNodes = hsc->active_nodes;
     mpm_context::raise(hsc->errors);
     int e0 = 0;
     HIPCHECK(hipMemcpyAsync(&e0, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -363,8 +428,10 @@ struct mpm_grid {
       if (enqueued > iterCap) throw std::runtime_error("conjugate residual did not terminate");
       int batch = hooks ? 1 : (enqueued == 0 ? std::max(1, c.iterHint) : 8);
       for (int k = 0; k < batch; ++k) {
+        int px = c.prof_begin(PK_CR_X, this);
         launch_cr_update_x(g, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), scal(),
                            partials.as<double>(), hist.as<double>(), histCap, s);
+        c.prof_end(px);
         if (hooks) {
           fetch();
           int it = hsc->iterations - 1;
@@ -385,10 +452,14 @@ struct mpm_grid {
           }
         }
         HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), s));
-        launch_hvp(c.view(), g, f(z), true, flags.as<int8_t>(), c.cols(), f(acc), scal(), c.errp(), s);
+        hvp(f(z), true, f(acc), scal());
+        int pa = c.prof_begin(PK_CR_AR, this);
         launch_cr_ar(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc), f(z), dt, f(Ar), scal(), partials.as<double>(),
                      s);
+        c.prof_end(pa);
+        int pp = c.prof_begin(PK_CR_P, this);
         launch_cr_update_p(g, f(z), f(Ar), f(diag), f(p), f(Ap), scal(), partials.as<double>(), s);
+        c.prof_end(pp);
       }
       enqueued += batch;
       fetch();
@@ -408,6 +479,25 @@ struct mpm_grid {
     if (rec.iterations > 0) c.iterHint = rec.iterations;
   }
 };
+
+void mpm_context::prof_resolve() {
+  if (profPending.empty()) return;
+  sync();
+  for (auto& r : profPending) {
+    float ms = 0.0f;
+    hipError_t e = hipEventElapsedTime(&ms, r.a, r.b);
+    // launches skipped after the solve finished still count: they are part of the solve's cost
+    if (e == hipSuccess) {
+      profMs[r.kind] += ms;
+      profCount[r.kind] += 1;
+      profBytes[r.kind] += kProfBytesPerParticle[r.kind] * (r.grid ? r.grid->np : 0) +
+                           kProfBytesPerNode[r.kind] * (r.grid ? r.grid->activeNodes : 0);
+    }
+    eventPool.push_back(r.a);
+    eventPool.push_back(r.b);
+  }
+  profPending.clear();
+}
 
 namespace {
 
@@ -1015,8 +1105,10 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
       }
       // G2P + F + plasticity (+ advection when no cancellation can intervene)
       c.clear_err();
+      int pg = c.prof_begin(PK_G2P, g);
       launch_g2p(c.view(), g->g, g->f(g->vel), g->f(g->prev), cancelHook ? 3 : 2, dt, c.thc, c.ths, c.cols(), c.errp(),
                  s);
+      c.prof_end(pg);
       if (b == 0 && (int64_t)B.size() < c.n) {
         // updateParticleData runs over all particles after each body (Sim.cpp:118); the other
         // particles are brought up to date here once, from their unchanged F
@@ -1070,12 +1162,41 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
       c.cur = nxt;
     }
     c.sync();
+    c.prof_resolve();
     calculate_bodies(c);
     if (stats) {
       stats->num_bodies = (int)c.bodies.size();
       stats->num_ballistic = (int)c.ballistic.size();
       stats->cancelled = cancelled ? 1 : 0;
     }
+  });
+}
+
+int mpm_profile_enable(mpm_context* ctx, int on) {
+  return guarded(ctx, [&] {
+    ctx->prof_resolve();
+    ctx->profOn = on != 0;
+    for (int k = 0; k < PK_COUNT; ++k) {
+      ctx->profMs[k] = 0;
+      ctx->profCount[k] = 0;
+      ctx->profBytes[k] = 0;
+    }
+  });
+}
+
+int mpm_profile_query(mpm_context* ctx, const char* kernel, double* total_ms, int64_t* launches,
+                      double* algorithmic_bytes) {
+  return guarded(ctx, [&] {
+    ctx->prof_resolve();
+    std::string k(kernel ? kernel : "");
+    for (int i = 0; i < PK_COUNT; ++i)
+      if (k == kProfNames[i]) {
+        if (total_ms) *total_ms = ctx->profMs[i];
+        if (launches) *launches = ctx->profCount[i];
+        if (algorithmic_bytes) *algorithmic_bytes = ctx->profBytes[i];
+        return;
+      }
+    throw InvalidArg("unknown kernel class '" + k + "'");
   });
 }
 
