@@ -5,14 +5,24 @@
 // column-major element order. Slots are a permutation of the caller's
 // particle indices that the runtime keeps body- and tile-sorted.
 //
-// Grid: the reference's dense per-body grid (n0 x n1 x n2 nodes, reference
-// node index i + n0 (j + n1 k)) stored tile-major: the grid is cut into
-// 4x4x4-node tiles, tile t = tx + T0 (ty + T1 tz) owns 64 consecutive nodes
-// (local l = (i&3) + 4 (j&3) + 16 (k&3)); 3-vectors store each tile as three
-// 64-float component runs ((t*3 + c) * 64 + l). A particle whose stencil base
-// node lies in tile t only touches nodes in tiles t + {0,1}^3, so a workgroup
-// that owns the particles of one tile stages an 8-tile (512-node) footprint
-// in LDS.
+// Grids: every body of a step gets the reference's dense grid over its own
+// bounding box (n0 x n1 x n2 nodes, reference node index i + n0 (j + n1 k)),
+// and all bodies' grids live in one batched node space so that each kernel
+// launch serves every body. A body's grid is cut into 4x4x4-node tiles
+// (tile t = tx + T0 (ty + T1 tz) owns 64 consecutive nodes, local
+// l = (i&3) + 4 (j&3) + 16 (k&3)); its tiles occupy the global tile range
+// [tileBase, tileBase + NT) with NT padded to a multiple of 4, so one
+// 256-thread node block (4 tiles) never spans two bodies. 3-vectors store each
+// tile as three 64-float component runs ((t*3 + c) * 64 + l).
+//
+// Particle-side kernels run one workgroup per chunk (up to 256 particles whose
+// stencil base node lies in one tile). A particle in tile t only touches nodes
+// in tiles t + {0,1}^3, so the workgroup stages that 8-tile (512-node)
+// footprint in LDS, accumulates its scatter there with ds_add_f32, and writes
+// the footprint to a per-chunk staging slot with plain coalesced stores. Node
+// kernels gather, for each node, the staged footprints of the chunks of the
+// (up to 8) tiles whose footprint covers it. No device-scope float atomics are
+// used: on MI355X those resolve beyond the per-XCD L2s and were the bottleneck.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,26 +54,41 @@ enum Column : int {
 struct PView {
   float* base;
   int64_t stride;
-  __device__ __forceinline__ float* col(int c) const { return base + (int64_t)c * stride; }
   __device__ __forceinline__ float& at(int c, int s) const { return base[(int64_t)c * stride + s]; }
 };
 
-constexpr int kTile = 4;          // cells per tile edge
-constexpr int kTileNodes = 64;    // nodes per tile
-constexpr int kFoot = 512;        // nodes in a particle tile's 2x2x2-tile footprint
+constexpr int kFoot = 512;        // nodes in a tile's 2x2x2-tile footprint
+constexpr int kStage = 3 * kFoot; // floats per staged chunk footprint
 constexpr int kChunk = 256;       // max particles per workgroup chunk
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;       // threads per workgroup; node blocks cover 4 tiles
+constexpr int kHistCap = 4096;    // residual history entries kept per body
 
-struct GridView {
-  int n[3];         // reference node counts
-  int T[3];         // tiles per axis
-  int NT;           // number of tiles
-  float gmin[3];    // m_min
+// Grid geometry of one body (Grid.cpp:193-213) and its place in the batch.
+struct BodyGrid {
+  int n[3];        // reference node counts
+  int T[3];        // tiles per axis
+  int NT;          // tiles, padded to a multiple of 4
+  int tileBase;    // first global tile
+  int blockBase;   // first global node block (= tileBase / 4)
+  int nblocks;     // node blocks (= NT / 4)
+  int np;          // particles
+  float gmin[3];   // m_min
   float h;
-  float fv[3];      // frame velocity
-  const int* perm;  // tile-sorted particle slots
-  const int4* chunks;  // (tile, start in perm, count, 0)
+  float fv[3];     // comoving frame velocity
+};
+
+struct Batch {
+  const BodyGrid* bodies;
+  int nbodies;
+  const int* blockBody;     // body of each node block
+  const int* tileBody;      // body of each global tile
+  const int* perm;          // tile-sorted particle slots of all bodies
+  const int4* chunks;       // (global tile, start in perm, count, body)
   int nchunks;
+  const int* tileChunk0;    // first chunk of each global tile
+  const int* tileChunkN;    // number of chunks of each global tile
+  int NT;                   // global tiles
+  int nblocks;              // global node blocks
 };
 
 // error bits raised by kernels (reference exception messages are mapped by the host)
@@ -77,8 +102,9 @@ enum ErrorBits : int {
   ERR_NAN_PVEL = 64        // "nans in particle velocity data!"
 };
 
-// Conjugate-residual scalars, resident in device memory. Reductions are FP64,
-// deterministic (fixed-order combination of per-block partials).
+// Conjugate-residual scalars of one body, resident in device memory. Reductions are
+// FP64 and deterministic (per-block partials combined in a fixed order by the last
+// block of the body to finish).
 struct CRScalars {
   double bnorm2, threshold;
   double rz, ApdPAp, alpha, beta, rnorm2;
@@ -91,8 +117,14 @@ struct CRScalars {
   int early_exit;          // zero denominator
   int max_iters;
   int errors;              // ErrorBits
-  int active_nodes;        // nodes with m > 0 (measurement only)
+  int active_nodes;        // nodes with m > 0 (measurement)
   unsigned int ticket[4];  // last-block-done counters
+};
+
+// Counters shared by all bodies of a batch.
+struct BatchStatus {
+  int bodies_done;
+  int errors;
 };
 
 }  // namespace mpm

@@ -1,17 +1,8 @@
 // mpm_kernels.hip -- gfx950 kernels of the implicit MPM snow step.
 //
-// Particle-side kernels (P2G, volumes, forces, HVP, diagonal, G2P) run one
-// workgroup per "chunk" (up to 256 particles whose stencil base node lies in
-// one 4x4x4-node tile). The workgroup stages the 8-tile footprint of the grid
-// vectors it reads in LDS, accumulates its scatter into an LDS footprint with
-// ds_add_f32, and flushes the footprint to HBM with one global f32 atomic per
-// touched node component (instead of 64 x 3 atomics per particle).
-// Node-side kernels are grid-stride loops over the tiled node arrays; their
-// reductions are FP64 and deterministic (per-block partials combined in a
-// fixed order by the last block to finish).
-//
-// The arithmetic of each step follows the reference (davidsminor/mpmsnow,
-// paths relative to its checkout) as cited per kernel.
+// Layout and scatter scheme: see mpm_device.hpp. The arithmetic of each step
+// follows the reference (davidsminor/mpmsnow, paths relative to its checkout)
+// as cited per kernel.
 #include "mpm_kernels.hpp"
 
 namespace mpm {
@@ -19,46 +10,32 @@ namespace mpm {
 namespace {
 
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
-inline int grid_for(int64_t n, int cap = 4096) {
+inline int grid_for(int64_t n, int cap = 8192) {
   int b = cdiv(n, kBlock);
   return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
-// ------------------------------------------------------------ footprints
-__device__ __forceinline__ void tile_coords(const GridView& g, int t, int& tx, int& ty, int& tz) {
-  tx = t % g.T[0];
-  int r = t / g.T[0];
-  ty = r % g.T[1];
-  tz = r / g.T[1];
+#define GRID_STRIDE(i, n) for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (n); i += (int64_t)gridDim.x * kBlock)
+
+// ------------------------------------------------------------ geometry helpers
+__device__ __forceinline__ void tile_xyz(const BodyGrid& G, int lt, int& tx, int& ty, int& tz) {
+  tx = lt % G.T[0];
+  int r = lt / G.T[0];
+  ty = r % G.T[1];
+  tz = r / G.T[1];
 }
-__device__ __forceinline__ int foot_tile(const GridView& g, int t, int f) {
-  int tx, ty, tz;
-  tile_coords(g, t, tx, ty, tz);
-  tx += f & 1;
-  ty += (f >> 1) & 1;
-  tz += f >> 2;
-  if (tx >= g.T[0] || ty >= g.T[1] || tz >= g.T[2]) return -1;
-  return tx + g.T[0] * (ty + g.T[1] * tz);
+__device__ __forceinline__ int tile_id(const BodyGrid& G, int tx, int ty, int tz) {
+  if (tx < 0 || ty < 0 || tz < 0 || tx >= G.T[0] || ty >= G.T[1] || tz >= G.T[2]) return -1;
+  return G.tileBase + tx + G.T[0] * (ty + G.T[1] * tz);
 }
-// footprint-relative node (rx, ry, rz) in [0, 8) -> LDS index in [0, 512)
+// footprint-relative node (rx, ry, rz) in [0, 8) -> LDS index in [0, 512): 64 * footprint tile + local
 __device__ __forceinline__ int foot_index(int rx, int ry, int rz) {
   return ((((rz >> 2) << 2) | ((ry >> 2) << 1) | (rx >> 2)) << 6) | ((rz & 3) << 4) | ((ry & 3) << 2) | (rx & 3);
 }
-// node coordinates of tiled node index (tile, local)
-__device__ __forceinline__ void node_coords(const GridView& g, int t, int l, int& i, int& j, int& k) {
-  int tx, ty, tz;
-  tile_coords(g, t, tx, ty, tz);
-  i = tx * 4 + (l & 3);
-  j = ty * 4 + ((l >> 2) & 3);
-  k = tz * 4 + (l >> 4);
-}
-__device__ __forceinline__ bool node_valid(const GridView& g, int i, int j, int k) {
-  return i < g.n[0] && j < g.n[1] && k < g.n[2];
-}
-__device__ __forceinline__ void node_pos(const GridView& g, int i, int j, int k, float x[3]) {
-  x[0] = g.h * (float)i + g.gmin[0];
-  x[1] = g.h * (float)j + g.gmin[1];
-  x[2] = g.h * (float)k + g.gmin[2];
+__device__ __forceinline__ void node_pos(const BodyGrid& G, int i, int j, int k, float x[3]) {
+  x[0] = G.h * (float)i + G.gmin[0];
+  x[1] = G.h * (float)j + G.gmin[1];
+  x[2] = G.h * (float)k + G.gmin[2];
 }
 // ImplicitUpdateMatrix::subspaceProject for one node (Grid.cpp:778-818)
 __device__ __forceinline__ void project(const ColliderSet* cols, int flag, const float x[3], float v[3]) {
@@ -74,22 +51,6 @@ __device__ __forceinline__ void project(const ColliderSet* cols, int flag, const
   v[1] -= nd * n[1];
   v[2] -= nd * n[2];
 }
-
-struct ChunkCtx {
-  int4 ch;
-  int ox, oy, oz;  // node origin of the chunk's tile
-};
-__device__ __forceinline__ ChunkCtx chunk_begin(const GridView& g, int* ft) {
-  ChunkCtx c;
-  c.ch = g.chunks[blockIdx.x];
-  int tx, ty, tz;
-  tile_coords(g, c.ch.x, tx, ty, tz);
-  c.ox = tx * 4;
-  c.oy = ty * 4;
-  c.oz = tz * 4;
-  if (threadIdx.x < 8) ft[threadIdx.x] = foot_tile(g, c.ch.x, threadIdx.x);
-  return c;
-}
 __device__ __forceinline__ void load_x(const PView& P, int s, float x[3]) {
   x[0] = P.at(C_X, s);
   x[1] = P.at(C_X + 1, s);
@@ -103,19 +64,40 @@ __device__ __forceinline__ void store_m9(const PView& P, int col, int s, const f
 #pragma unroll
   for (int i = 0; i < 9; ++i) P.at(col + i, s) = m[i];
 }
-// the stencil of particle s relative to the chunk footprint; false if it lies outside
-__device__ __forceinline__ bool chunk_stencil(const GridView& g, const ChunkCtx& c, const float x[3], bool derivs,
-                                              Stencil& st, int& rx, int& ry, int& rz) {
-  make_stencil(x, g.gmin, g.h, derivs, st);
+
+// ------------------------------------------------------------ chunk context
+struct ChunkCtx {
+  int4 ch;       // (global tile, start in perm, count, body)
+  BodyGrid G;
+  int ox, oy, oz;  // node origin of the chunk's tile
+};
+// all threads: the chunk, its body, and (threads 0..7) the footprint's global tiles
+__device__ __forceinline__ ChunkCtx chunk_begin(const Batch& B, int* ft) {
+  ChunkCtx c;
+  c.ch = B.chunks[blockIdx.x];
+  c.G = B.bodies[c.ch.w];
+  int tx, ty, tz;
+  tile_xyz(c.G, c.ch.x - c.G.tileBase, tx, ty, tz);
+  c.ox = tx * 4;
+  c.oy = ty * 4;
+  c.oz = tz * 4;
+  if (threadIdx.x < 8) {
+    int f = threadIdx.x;
+    ft[f] = tile_id(c.G, tx + (f & 1), ty + ((f >> 1) & 1), tz + (f >> 2));
+  }
+  return c;
+}
+__device__ __forceinline__ bool chunk_stencil(const ChunkCtx& c, const float x[3], bool derivs, Stencil& st, int& rx,
+                                              int& ry, int& rz) {
+  make_stencil(x, c.G.gmin, c.G.h, derivs, st);
   rx = st.base[0] - c.ox;
   ry = st.base[1] - c.oy;
   rz = st.base[2] - c.oz;
   return rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4;
 }
-// load a tiled 3-vector footprint into LDS (optionally projected)
-__device__ __forceinline__ void load_foot3(const GridView& g, const ChunkCtx& c, const int* ft, const float* d,
-                                           float (*sd)[kFoot], const int8_t* flags, const ColliderSet* cols,
-                                           bool proj) {
+// load a batch 3-vector footprint into LDS (optionally projected)
+__device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, const float* d, float (*sd)[kFoot],
+                                           const int8_t* flags, const ColliderSet* cols, bool proj) {
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int f = i >> 6, l = i & 63;
     int gt = ft[f];
@@ -128,10 +110,9 @@ __device__ __forceinline__ void load_foot3(const GridView& g, const ChunkCtx& c,
       if (proj) {
         int fl = flags[(int64_t)gt * 64 + l];
         if (fl != -1) {
-          int ni = c.ox + (f & 1) * 4 + (l & 3), nj = c.oy + ((f >> 1) & 1) * 4 + ((l >> 2) & 3),
-              nk = c.oz + (f >> 2) * 4 + (l >> 4);
           float xn[3];
-          node_pos(g, ni, nj, nk, xn);
+          node_pos(c.G, c.ox + (f & 1) * 4 + (l & 3), c.oy + ((f >> 1) & 1) * 4 + ((l >> 2) & 3),
+                   c.oz + (f >> 2) * 4 + (l >> 4), xn);
           project(cols, fl, xn, v);
         }
       }
@@ -141,20 +122,107 @@ __device__ __forceinline__ void load_foot3(const GridView& g, const ChunkCtx& c,
     sd[2][i] = v[2];
   }
 }
-__device__ __forceinline__ void flush_foot3(const int* ft, float (*so)[kFoot], float* out) {
+// staged layout per chunk: [footprint tile f][component c][64 local nodes]
+__device__ __forceinline__ void store_stage3(float (*so)[kFoot], float* stage) {
+  float* dst = stage + (int64_t)blockIdx.x * kStage;
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    int gt = ft[i >> 6];
-    if (gt < 0) continue;
-    int64_t b = (int64_t)gt * 192 + (i & 63);
+    int f = i >> 6, l = i & 63;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float v = so[c][i];
-      if (v != 0.0f) atomicAdd(out + b + 64 * c, v);
-    }
+    for (int c = 0; c < 3; ++c) dst[f * 192 + c * 64 + l] = so[c][i];
   }
 }
 
-// ------------------------------------------------------------ reductions
+// gather of staged footprints for node (tile T, local l) of body G, component c
+__device__ __forceinline__ float gather_node(const Batch& B, const BodyGrid& G, const float* stage, int T, int l,
+                                             int c) {
+  int tx, ty, tz;
+  tile_xyz(G, T - G.tileBase, tx, ty, tz);
+  float v = 0.0f;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
+    if (ts < 0) continue;
+    int c0 = B.tileChunk0[ts], cn = B.tileChunkN[ts];
+    for (int q = 0; q < cn; ++q) v += stage[(int64_t)(c0 + q) * kStage + f * 192 + c * 64 + l];
+  }
+  return v;
+}
+__device__ __forceinline__ void gather_node3(const Batch& B, const BodyGrid& G, const float* stage, int T, int l,
+                                             float o[3]) {
+  int tx, ty, tz;
+  tile_xyz(G, T - G.tileBase, tx, ty, tz);
+  float a = 0.0f, b = 0.0f, cc = 0.0f;
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
+    if (ts < 0) continue;
+    int c0 = B.tileChunk0[ts], cn = B.tileChunkN[ts];
+    for (int q = 0; q < cn; ++q) {
+      const float* s = stage + (int64_t)(c0 + q) * kStage + f * 192 + l;
+      a += s[0];
+      b += s[64];
+      cc += s[128];
+    }
+  }
+  o[0] = a;
+  o[1] = b;
+  o[2] = cc;
+}
+
+// ------------------------------------------------------------ node block context
+struct NodeCtx {
+  int b;        // body
+  BodyGrid G;
+  int64_t i;    // global node
+  int T, l;     // global tile, local node
+  int ni, nj, nk;
+  bool valid;   // inside the body's reference grid
+};
+__device__ __forceinline__ NodeCtx node_begin(const Batch& B) {
+  NodeCtx c;
+  c.b = B.blockBody[blockIdx.x];
+  c.G = B.bodies[c.b];
+  c.i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  c.T = (int)(c.i >> 6);
+  c.l = (int)(c.i & 63);
+  int tx, ty, tz;
+  tile_xyz(c.G, c.T - c.G.tileBase, tx, ty, tz);
+  c.ni = tx * 4 + (c.l & 3);
+  c.nj = ty * 4 + ((c.l >> 2) & 3);
+  c.nk = tz * 4 + (c.l >> 4);
+  c.valid = c.ni < c.G.n[0] && c.nj < c.G.n[1] && c.nk < c.G.n[2] && (c.T - c.G.tileBase) < c.G.T[0] * c.G.T[1] * c.G.T[2];
+  return c;
+}
+__device__ __forceinline__ void ld3(const float* v, int64_t i, float o[3]) {
+  int64_t b = (i >> 6) * 192 + (i & 63);
+  o[0] = v[b];
+  o[1] = v[b + 64];
+  o[2] = v[b + 128];
+}
+__device__ __forceinline__ void st3(float* v, int64_t i, const float o[3]) {
+  int64_t b = (i >> 6) * 192 + (i & 63);
+  v[b] = o[0];
+  v[b + 64] = o[1];
+  v[b + 128] = o[2];
+}
+__device__ __forceinline__ void node_project(const NodeCtx& c, const ColliderSet* cols, int fl, float v[3]) {
+  if (fl == -1) return;
+  float x[3];
+  node_pos(c.G, c.ni, c.nj, c.nk, x);
+  project(cols, fl, x, v);
+}
+// ImplicitUpdateMatrix::multVector epilogue (Grid.cpp:764-770): P(m P d - dt^2 df)
+__device__ __forceinline__ void apply_A(const NodeCtx& c, const ColliderSet* cols, int fl, float m, const float d[3],
+                                        const float df[3], float dt2, float out[3]) {
+  float pd[3] = {d[0], d[1], d[2]};
+  node_project(c, cols, fl, pd);
+  out[0] = m * pd[0] - dt2 * df[0];
+  out[1] = m * pd[1] - dt2 * df[1];
+  out[2] = m * pd[2] - dt2 * df[2];
+  node_project(c, cols, fl, out);
+}
+
+// ------------------------------------------------------------ per-body reductions
 template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sm)[NV]) {
 #pragma unroll
   for (int k = 0; k < NV; ++k)
@@ -169,10 +237,11 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], dou
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = ((sm[0][k] + sm[1][k]) + sm[2][k]) + sm[3][k];
 }
-// Writes this block's partials; returns true in every thread of the last block to finish,
-// with out[] = the fixed-order total over all blocks.
+// Writes this block's partials; returns true in every thread of the last block of the body
+// to finish, with out[] = the fixed-order total over the body's blocks.
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned int* ticket, double (&out)[NV]) {
+__device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, unsigned int* ticket,
+                         double (&out)[NV]) {
   __shared__ double sm[kBlock / 64][NV];
   __shared__ bool amLast;
   block_sum<NV>(v, sm);
@@ -181,7 +250,7 @@ __device__ bool grid_sum(double (&v)[NV], double* partials, unsigned int* ticket
     for (int k = 0; k < NV; ++k) partials[(int64_t)blockIdx.x * NV + k] = v[k];
     __threadfence();
     unsigned int t = atomicAdd(ticket, 1u);
-    amLast = (t == gridDim.x - 1);
+    amLast = (t == (unsigned int)G.nblocks - 1u);
   }
   __syncthreads();
   if (!amLast) return false;
@@ -189,10 +258,11 @@ __device__ bool grid_sum(double (&v)[NV], double* partials, unsigned int* ticket
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+  for (int b = threadIdx.x; b < G.nblocks; b += kBlock)
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      acc[k] += __hip_atomic_load(partials + (int64_t)b * NV + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      acc[k] += __hip_atomic_load(partials + (int64_t)(G.blockBase + b) * NV + k, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
   block_sum<NV>(acc, sm);
   __shared__ double res[NV];
   if (threadIdx.x == 0) {
@@ -205,33 +275,42 @@ __device__ bool grid_sum(double (&v)[NV], double* partials, unsigned int* ticket
   for (int k = 0; k < NV; ++k) out[k] = res[k];
   return true;
 }
+__device__ __forceinline__ void set_done(CRScalars& s, BatchStatus* st) {
+  if (!s.done) {
+    s.done = 1;
+    atomicAdd(&st->bodies_done, 1);
+  }
+}
 
-// ------------------------------------------------------------ particles
+// ------------------------------------------------------------ particle store kernels
 __global__ void k_set_column(PView P, int col0, int ncomp, const float* aos, const int* slot_of, int n) {
-  int64_t total = (int64_t)n * ncomp;
-  for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+  GRID_STRIDE(t, (int64_t)n * ncomp) {
     int i = (int)(t / ncomp), c = (int)(t % ncomp);
     P.at(col0 + c, slot_of[i]) = aos[t];
   }
 }
 __global__ void k_get_column(PView P, int col0, int ncomp, float* aos, const int* slot_of, int n) {
-  int64_t total = (int64_t)n * ncomp;
-  for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+  GRID_STRIDE(t, (int64_t)n * ncomp) {
     int i = (int)(t / ncomp), c = (int)(t % ncomp);
     aos[t] = P.at(col0 + c, slot_of[i]);
   }
 }
 __global__ void k_fill_column(PView P, int col, float value, int n) {
-  for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) P.at(col, s) = value;
+  GRID_STRIDE(s, n) P.at(col, (int)s) = value;
+}
+__global__ void k_fill_int(int* p, int value, int64_t n) {
+  GRID_STRIDE(i, n) p[i] = value;
 }
 __global__ void k_identity_columns(PView P, int col0, int n) {
-  for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
+  GRID_STRIDE(s, n) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) P.at(col0 + i, s) = (i == 0 || i == 4 || i == 8) ? 1.0f : 0.0f;
+    for (int i = 0; i < 9; ++i) P.at(col0 + i, (int)s) = (i == 0 || i == 4 || i == 8) ? 1.0f : 0.0f;
+  }
 }
 __global__ void k_permute(PView src, PView dst, const int* new_to_old, int n, const int* orig_old, int* orig_new,
                           int* slot_of) {
-  for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
     int o = new_to_old[s];
 #pragma unroll 4
     for (int c = 0; c < NCOL; ++c) dst.at(c, s) = src.at(c, o);
@@ -241,64 +320,225 @@ __global__ void k_permute(PView src, PView dst, const int* new_to_old, int n, co
   }
 }
 __global__ void k_gather_slots(const int* slot_of, const int* orig, int n, int* slots) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) slots[k] = slot_of[orig[k]];
+  GRID_STRIDE(k, n) slots[k] = slot_of[orig[k]];
 }
 
+// ------------------------------------------------------------ body partition (Sim.cpp:136-203)
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
+  return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) & mask;
+}
+__device__ __forceinline__ void cell_of(const float x[3], float cell, int c[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) c[a] = (int)floorf(x[a] / cell);
+}
+__global__ void k_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count) {
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    float x[3];
+    load_x(P, s, x);
+    int c[3];
+    cell_of(x, cell, c);
+    uint32_t h = cell_hash(c[0], c[1], c[2], mask);
+    bucket[s] = (int)h;
+    rank[s] = atomicAdd(count + h, 1);
+  }
+}
+__global__ void k_cc_scatter(const int* bucket, const int* rank, const int* offset, int n, int* sorted) {
+  GRID_STRIDE(s, n) sorted[offset[bucket[s]] + rank[s]] = (int)s;
+}
+__device__ __forceinline__ int cc_find(int* parent, int x) {
+  int p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (p != x) {
+    int gp = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // path halving
+    x = p;
+    p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return x;
+}
+// hook the larger root under the smaller one; parent pointers only ever decrease
+__device__ __forceinline__ void cc_union(int* parent, int a, int b) {
+  while (true) {
+    a = cc_find(parent, a);
+    b = cc_find(parent, b);
+    if (a == b) return;
+    if (a < b) {
+      int t = a;
+      a = b;
+      b = t;
+    }
+    int old = atomicCAS(parent + a, a, b);
+    if (old == a) return;
+  }
+}
+// neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190)
+__global__ void k_cc_link(PView P, int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                          const int* sorted, int* parent, int* hasNb) {
+  const float r2 = h * h;
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    float x[3];
+    load_x(P, s, x);
+    int c[3];
+    cell_of(x, cell, c);
+    bool nb = false;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          uint32_t hb = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, mask);
+          int o = offset[hb], e = o + count[hb];
+          for (int q = o; q < e; ++q) {
+            int j = sorted[q];
+            if (j == s) continue;
+            float y[3];
+            load_x(P, j, y);
+            // (d0 d0 + d1 d1) + d2 d2 with separate roundings, as Eigen's squaredNorm on the host
+            float d0 = x[0] - y[0], d1 = x[1] - y[1], d2 = x[2] - y[2];
+            float d = __fadd_rn(__fadd_rn(__fmul_rn(d0, d0), __fmul_rn(d1, d1)), __fmul_rn(d2, d2));
+            if (d < r2) {
+              nb = true;
+              if (j > s) cc_union(parent, s, j);
+            }
+          }
+        }
+    hasNb[s] = nb ? 1 : 0;
+  }
+}
+__global__ void k_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig) {
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    int r = cc_find(parent, s);
+    parent[s] = r;
+    if (hasNb[s]) {
+      atomicMin(minOrig + r, orig_of[s]);
+      if (r == s) isRoot[s] = 1;
+    }
+  }
+}
+__global__ void k_cc_compact(int n, const int* isRoot, const int* rootIdx, const int* minOrig, int* rootMin) {
+  GRID_STRIDE(s, n) if (isRoot[s]) rootMin[rootIdx[s]] = minOrig[s];
+}
+__global__ void k_cc_assign(int n, const int* parent, const int* hasNb, const int* rootIdx, const int* rootRank,
+                            int* labels) {
+  GRID_STRIDE(s, n) labels[s] = hasNb[s] ? rootRank[rootIdx[parent[s]]] : -1;
+}
+// velocity finiteness (Sim.cpp:151-159)
+__global__ void k_check_pvel(PView P, int n, int* err) {
+  int bad = 0;
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    if (!isfinite(P.at(C_V, s) * P.at(C_V + 1, s) * P.at(C_V + 2, s))) bad = 1;
+  }
+  if (bad) atomicOr(err, ERR_NAN_PVEL);
+}
+
+// ------------------------------------------------------------ body reductions
 __device__ __forceinline__ unsigned int ord_enc(float f) {
   unsigned int u = __float_as_uint(f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-// bbox (Grid.cpp:177-191) and comoving-frame sums (Sim.cpp:84-93) of one body
-__global__ void k_body_reduce(PView P, const int* slots, int base, int n, unsigned int* bbox, double* mom) {
-  unsigned int mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    int s = slots ? slots[k] : base + k;
-    float m = P.at(C_M, s);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      unsigned int e = ord_enc(P.at(C_X + a, s));
-      mn[a] = min(mn[a], e);
-      mx[a] = max(mx[a], e);
-      acc[a] += (double)(P.at(C_V + a, s) * m);
+// bbox (Grid.cpp:177-191) and comoving-frame sums (Sim.cpp:84-93) per body; waves whose
+// particles all belong to one body reduce in registers first
+__global__ void k_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
+                              int* count) {
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    int64_t k = base + threadIdx.x;
+    int b = -1;
+    int s = 0;
+    if (k < n) {
+      s = slots ? slots[k] : (int)k;
+      b = labels ? labels[s] : 0;
     }
-    acc[3] += (double)m;
-  }
+    unsigned int mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (b >= 0) {
+      float m = P.at(C_M, s);
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-    for (int off = 32; off > 0; off >>= 1) {
-      mn[a] = min(mn[a], (unsigned int)__shfl_down((int)mn[a], off, 64));
-      mx[a] = max(mx[a], (unsigned int)__shfl_down((int)mx[a], off, 64));
+      for (int a = 0; a < 3; ++a) {
+        unsigned int e = ord_enc(P.at(C_X + a, s));
+        mn[a] = e;
+        mx[a] = e;
+        acc[a] = (double)(P.at(C_V + a, s) * m);
+      }
+      acc[3] = (double)m;
     }
+    // waves whose particles all belong to one body (the common case: the store is body-sorted)
+    // reduce in registers and issue one set of atomics
+    unsigned long long have = __ballot(b >= 0);
+    if (have == 0ull) continue;
+    int first = __ffsll((long long)have) - 1;
+    int bb = __shfl(b, first, 64);
+    if (__all((b == bb) || (b < 0))) {
+      int cnt = b >= 0 ? 1 : 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
-  if ((threadIdx.x & 63) == 0) {
+      for (int a = 0; a < 3; ++a)
+        for (int off = 32; off > 0; off >>= 1) {
+          mn[a] = min(mn[a], (unsigned int)__shfl_down((int)mn[a], off, 64));
+          mx[a] = max(mx[a], (unsigned int)__shfl_down((int)mx[a], off, 64));
+        }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(bbox + a, mn[a]);
-      atomicMax(bbox + 3 + a, mx[a]);
+      for (int q = 0; q < 4; ++q)
+        for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_down(acc[q], off, 64);
+      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+      if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          atomicMin(bbox + 6 * bb + a, mn[a]);
+          atomicMax(bbox + 6 * bb + 3 + a, mx[a]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) atomicAdd(mom + 4 * bb + q, acc[q]);
+        atomicAdd(count + bb, cnt);
+      }
+      continue;
     }
+    if (b >= 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) atomicAdd(mom + k, acc[k]);
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(bbox + 6 * b + a, mn[a]);
+        atomicMax(bbox + 6 * b + 3 + a, mx[a]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) atomicAdd(mom + 4 * b + q, acc[q]);
+      atomicAdd(count + b, 1);
+    }
   }
 }
 
 // ------------------------------------------------------------ binning
-__global__ void k_bin(PView P, const int* slots, int base, int n, GridView g, int* key, int* rank, int* count,
-                      int* err) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    int s = slots ? slots[k] : base + k;
+__global__ void k_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody) {
+  GRID_STRIDE(ti, NT) {
+    int t = (int)ti;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {  // last body with tileBase <= t
+      int mid = (lo + hi + 1) >> 1;
+      if (bodies[mid].tileBase <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    tileBody[t] = lo;
+    if ((t & 3) == 0) blockBody[t >> 2] = lo;
+  }
+}
+__global__ void k_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
+                      int* count, int* err) {
+  GRID_STRIDE(k, n) {
+    int s = slots ? slots[k] : (int)k;
+    int b = labels ? labels[s] : 0;
+    if (b < 0) {
+      key[k] = -1;
+      continue;
+    }
+    const BodyGrid& G = bodies[b];
     float x[3];
     load_x(P, s, x);
     int t3[3];
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      float frac = (x[d] - g.gmin[d]) / g.h;
-      int b = (int)floorf(frac) - 1;
-      int t = b >> 2;
-      ok = ok && b >= 0 && t < g.T[d];
+      float frac = (x[d] - G.gmin[d]) / G.h;
+      int bc = (int)floorf(frac) - 1;
+      int t = bc >> 2;
+      ok = ok && bc >= 0 && t < G.T[d];
       t3[d] = t;
     }
     if (!ok) {
@@ -306,42 +546,42 @@ __global__ void k_bin(PView P, const int* slots, int base, int n, GridView g, in
       key[k] = -1;
       continue;
     }
-    int t = t3[0] + g.T[0] * (t3[1] + g.T[1] * t3[2]);
+    int t = G.tileBase + t3[0] + G.T[0] * (t3[1] + G.T[1] * t3[2]);
     key[k] = t;
     rank[k] = atomicAdd(count + t, 1);
   }
 }
-__global__ void k_scatter(const int* slots, int base, const int* key, const int* rank, const int* offset, int n,
-                          int* perm) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+__global__ void k_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm) {
+  GRID_STRIDE(k, n) {
     int t = key[k];
     if (t < 0) continue;
-    perm[offset[t] + rank[k]] = slots ? slots[k] : base + k;
+    perm[offset[t] + rank[k]] = slots ? slots[k] : (int)k;
   }
 }
 __global__ void k_chunk_counts(const int* count, int NT, int* nch) {
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < NT; t += gridDim.x * kBlock) nch[t] = (count[t] + kChunk - 1) / kChunk;
+  GRID_STRIDE(t, NT) nch[t] = (count[t] + kChunk - 1) / kChunk;
 }
-__global__ void k_make_chunks(const int* count, const int* offset, const int* chunk_off, int NT, int4* chunks) {
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < NT; t += gridDim.x * kBlock) {
+__global__ void k_make_chunks(const int* count, const int* offset, const int* chunk_off, const int* tileBody, int NT,
+                              int4* chunks) {
+  GRID_STRIDE(ti, NT) {
+    int t = (int)ti;
     int c = count[t];
     int o = chunk_off[t];
     for (int q = 0; q * kChunk < c; ++q) {
       int cnt = c - q * kChunk;
-      chunks[o + q] = make_int4(t, offset[t] + q * kChunk, cnt < kChunk ? cnt : kChunk, 0);
+      chunks[o + q] = make_int4(t, offset[t] + q * kChunk, cnt < kChunk ? cnt : kChunk, tileBody[t]);
     }
   }
 }
-
 // exclusive scan: 1024 elements per block
 __global__ void k_scan_block(const int* in, int* out, int n, int* bsum) {
   __shared__ int sm[kBlock];
-  int b0 = blockIdx.x * 1024;
+  int64_t b0 = (int64_t)blockIdx.x * 1024;
   int v[4];
   int tot = 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    int i = b0 + threadIdx.x * 4 + q;
+    int64_t i = b0 + threadIdx.x * 4 + q;
     v[q] = i < n ? in[i] : 0;
     tot += v[q];
   }
@@ -356,14 +596,14 @@ __global__ void k_scan_block(const int* in, int* out, int n, int* bsum) {
   int run = sm[threadIdx.x] - tot;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    int i = b0 + threadIdx.x * 4 + q;
+    int64_t i = b0 + threadIdx.x * 4 + q;
     if (i < n) out[i] = run;
     run += v[q];
   }
   if (threadIdx.x == kBlock - 1) bsum[blockIdx.x] = sm[kBlock - 1];
 }
 __global__ void k_scan_add(int* out, int n, const int* boff) {
-  int i = blockIdx.x * 1024 + threadIdx.x;
+  int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
   int add = boff[blockIdx.x];
 #pragma unroll
   for (int q = 0; q < 4; ++q, i += kBlock)
@@ -371,15 +611,15 @@ __global__ void k_scan_add(int* out, int n, const int* boff) {
 }
 __global__ void k_copy_int(const int* src, int* dst) { *dst = *src; }
 
-// ------------------------------------------------------------ dense <-> tiled
-template <class T> __global__ void k_to_dense(const T* src, int ncomp, T* dst, GridView g) {
-  int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * kBlock) {
-    int i = (int)(idx % g.n[0]);
-    int64_t r = idx / g.n[0];
-    int j = (int)(r % g.n[1]);
-    int k = (int)(r / g.n[1]);
-    int t = (i >> 2) + g.T[0] * ((j >> 2) + g.T[1] * (k >> 2));
+// ------------------------------------------------------------ dense <-> batch
+template <class T> __global__ void k_to_dense(const T* src, int ncomp, T* dst, BodyGrid G) {
+  int64_t nn = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  GRID_STRIDE(idx, nn) {
+    int i = (int)(idx % G.n[0]);
+    int64_t r = idx / G.n[0];
+    int j = (int)(r % G.n[1]);
+    int k = (int)(r / G.n[1]);
+    int t = G.tileBase + (i >> 2) + G.T[0] * ((j >> 2) + G.T[1] * (k >> 2));
     int l = (i & 3) + 4 * (j & 3) + 16 * (k & 3);
     for (int c = 0; c < ncomp; ++c) {
       int64_t si = ncomp == 1 ? (int64_t)t * 64 + l : ((int64_t)t * 3 + c) * 64 + l;
@@ -387,14 +627,14 @@ template <class T> __global__ void k_to_dense(const T* src, int ncomp, T* dst, G
     }
   }
 }
-template <class T> __global__ void k_from_dense(const T* src, int ncomp, T* dst, GridView g) {
-  int64_t nn = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * kBlock) {
-    int i = (int)(idx % g.n[0]);
-    int64_t r = idx / g.n[0];
-    int j = (int)(r % g.n[1]);
-    int k = (int)(r / g.n[1]);
-    int t = (i >> 2) + g.T[0] * ((j >> 2) + g.T[1] * (k >> 2));
+template <class T> __global__ void k_from_dense(const T* src, int ncomp, T* dst, BodyGrid G) {
+  int64_t nn = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  GRID_STRIDE(idx, nn) {
+    int i = (int)(idx % G.n[0]);
+    int64_t r = idx / G.n[0];
+    int j = (int)(r % G.n[1]);
+    int k = (int)(r / G.n[1]);
+    int t = G.tileBase + (i >> 2) + G.T[0] * ((j >> 2) + G.T[1] * (k >> 2));
     int l = (i & 3) + 4 * (j & 3) + 16 * (k & 3);
     for (int c = 0; c < ncomp; ++c) {
       int64_t si = ncomp == 1 ? (int64_t)t * 64 + l : ((int64_t)t * 3 + c) * 64 + l;
@@ -405,20 +645,20 @@ template <class T> __global__ void k_from_dense(const T* src, int ncomp, T* dst,
 
 // ------------------------------------------------------------ P2G
 // MassSplatter (Grid.cpp:62-99): m_i += m_p * w_ip
-__global__ __launch_bounds__(kBlock) void k_p2g_mass(PView P, GridView g, float* mass, int* err) {
+__global__ __launch_bounds__(kBlock) void k_p2g_mass(PView P, Batch B, float* stage, int* err) {
   __shared__ float acc[kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   for (int i = threadIdx.x; i < kFoot; i += kBlock) acc[i] = 0.0f;
   __syncthreads();
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
     float m = P.at(C_M, s);
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, false, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -430,28 +670,22 @@ __global__ __launch_bounds__(kBlock) void k_p2g_mass(PView P, GridView g, float*
         for (int cx = 0; cx < 4; ++cx) atomicAdd(&acc[foot_index(rx + cx, ry + cy, rz + cz)], m * st_w(st, cx, cy, cz));
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    float v = acc[i];
-    int gt = ft[i >> 6];
-    if (v != 0.0f && gt >= 0) atomicAdd(mass + (int64_t)gt * 64 + (i & 63), v);
-  }
+  float* dst = stage + (int64_t)blockIdx.x * kStage;
+  for (int i = threadIdx.x; i < kFoot; i += kBlock) dst[(i >> 6) * 192 + (i & 63)] = acc[i];
 }
-// clamp negative masses, NaN check (Grid.cpp:225-241)
-__global__ void k_mass_fix(float* mass, int64_t nn, int* err) {
-  int bad = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nn; i += (int64_t)gridDim.x * kBlock) {
-    float m = mass[i];
-    if (m < 0.0f) mass[i] = 0.0f;
-    if (!isfinite(m)) bad = 1;
-  }
-  if (bad) atomicOr(err, ERR_NAN_MASS);
+// mass gather, clamp negative masses, NaN check (Grid.cpp:225-241)
+__global__ void k_gather_mass(Batch B, const float* stage, float* mass, int* err) {
+  NodeCtx c = node_begin(B);
+  float m = c.valid ? gather_node(B, c.G, stage, c.T, c.l, 0) : 0.0f;
+  if (!isfinite(m)) atomicOr(err, ERR_NAN_MASS);
+  mass[c.i] = m < 0.0f ? 0.0f : m;
 }
 // VelocitySplatter (Grid.cpp:102-150): v_i += w (m_p / m_i) (v_p - v_f) where m_i > 0
-__global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, GridView g, const float* mass, float* vel, int* err) {
+__global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const float* mass, float* stage, int* err) {
   __shared__ float sm[kFoot];
   __shared__ float acc[3][kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int gt = ft[i >> 6];
@@ -460,14 +694,14 @@ __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, GridView g, const f
   }
   __syncthreads();
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
     float m = P.at(C_M, s);
-    float dv[3] = {P.at(C_V, s) - g.fv[0], P.at(C_V + 1, s) - g.fv[1], P.at(C_V + 2, s) - g.fv[2]};
+    float dv[3] = {P.at(C_V, s) - c.G.fv[0], P.at(C_V + 1, s) - c.G.fv[1], P.at(C_V + 2, s) - c.G.fv[2]};
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, false, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -488,37 +722,42 @@ __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, GridView g, const f
         }
   }
   __syncthreads();
-  flush_foot3(ft, acc, vel);
+  store_stage3(acc, stage);
 }
-// NaN check (Grid.cpp:250-260) and m_prevVelocities = m_velocities (Grid.cpp:262)
-__global__ void k_check_vel(const float* vel, float* prev, int64_t n3, int* err) {
-  int bad = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
-    float v = vel[i];
-    prev[i] = v;
-    if (!isfinite(v)) bad = 1;
-  }
-  if (bad) atomicOr(err, ERR_NAN_VEL);
+// velocity gather, NaN check (Grid.cpp:250-260) and m_prevVelocities = m_velocities (Grid.cpp:262)
+__global__ void k_gather_vel(Batch B, const float* stage, float* vel, float* prev, int* err) {
+  NodeCtx c = node_begin(B);
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, v);
+  if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2])) atomicOr(err, ERR_NAN_VEL);
+  st3(vel, c.i, v);
+  st3(prev, c.i, v);
+}
+__global__ void k_gather3(Batch B, const float* stage, float* out) {
+  NodeCtx c = node_begin(B);
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, v);
+  st3(out, c.i, v);
 }
 // computeParticleVolumes (Grid.cpp:265-292): V_p = m_p / sum_i w_ip m_i / h^3
-__global__ __launch_bounds__(kBlock) void k_volumes(PView P, GridView g, const float* mass, int* err) {
+__global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const float* mass, int* err) {
   __shared__ float sm[kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int gt = ft[i >> 6];
     sm[i] = gt >= 0 ? mass[(int64_t)gt * 64 + (i & 63)] : 0.0f;
   }
   __syncthreads();
-  const float cellVolume = g.h * g.h * g.h;
+  const float cellVolume = c.G.h * c.G.h * c.G.h;
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, false, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -536,14 +775,14 @@ __global__ __launch_bounds__(kBlock) void k_volumes(PView P, GridView g, const f
 
 // ------------------------------------------------------------ forces
 // ForceSplatter (Grid.cpp:387-435): f_i -= V_p P_p F_p^T grad w_ip
-__global__ __launch_bounds__(kBlock) void k_forces(PView P, GridView g, float* out, int* err) {
+__global__ __launch_bounds__(kBlock) void k_forces(PView P, Batch B, float* stage, int* err) {
   __shared__ float acc[3][kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   for (int i = threadIdx.x; i < kFoot; i += kBlock) acc[0][i] = acc[1][i] = acc[2][i] = 0.0f;
   __syncthreads();
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3], F[9], R[9], Fi[9], Pm[9];
     load_x(P, s, x);
     load_m9(P, C_F, s, F);
@@ -561,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(PView P, GridView g, float* o
     mat_mul_bt(Pm, F, fm);
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, true, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -581,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(PView P, GridView g, float* o
         }
   }
   __syncthreads();
-  flush_foot3(ft, acc, out);
+  store_stage3(acc, stage);
 }
 
 __device__ __forceinline__ void field_force(const float* grav, int ngrav, float m, float f[3]) {
@@ -593,87 +832,78 @@ __device__ __forceinline__ void field_force(const float* grav, int ngrav, float 
 }
 
 // calculateForces output (Grid.cpp:438-463): f = sum_fields g m + internal
-__global__ void k_node_forces(GridView g, const float* mass, const float* acc, const float* grav, int ngrav,
+__global__ void k_node_forces(Batch B, const float* mass, const float* stage, const float* grav, int ngrav,
                               float* out) {
-  int64_t nn = (int64_t)g.NT * 64;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nn; i += (int64_t)gridDim.x * kBlock) {
-    int t = (int)(i >> 6), l = (int)(i & 63);
-    int64_t b = (int64_t)t * 192 + l;
-    float f[3] = {0.0f, 0.0f, 0.0f};
-    field_force(grav, ngrav, mass[i], f);
-    out[b] = f[0] + acc[b];
-    out[b + 64] = f[1] + acc[b + 64];
-    out[b + 128] = f[2] + acc[b + 128];
+  NodeCtx c = node_begin(B);
+  float f[3] = {0.0f, 0.0f, 0.0f}, fi[3] = {0.0f, 0.0f, 0.0f};
+  if (c.valid) {
+    field_force(grav, ngrav, mass[c.i], f);
+    gather_node3(B, c.G, stage, c.T, c.l, fi);
   }
+  f[0] += fi[0];
+  f[1] += fi[1];
+  f[2] += fi[2];
+  st3(out, c.i, f);
 }
 
 // calculateExplicitMomenta (Grid.cpp:634-690) + collisionVelocities (Grid.cpp:692-724)
 // + the sums of the initial guess (Grid.cpp:911-947)
-__global__ void k_node_explicit(GridView g, const float* mass, const float* vel, const float* acc, const float* grav,
+__global__ void k_node_explicit(Batch B, const float* mass, const float* vel, const float* stage, const float* grav,
                                 int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                                CRScalars* sc, double* partials) {
-  int64_t nn = (int64_t)g.NT * 64;
+                                CRScalars* sc, BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int bad = 0;
-  int active = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nn; i += (int64_t)gridDim.x * kBlock) {
-    int t = (int)(i >> 6), l = (int)(i & 63);
-    int64_t b = (int64_t)t * 192 + l;
-    int ni, nj, nk;
-    node_coords(g, t, l, ni, nj, nk);
-    if (!node_valid(g, ni, nj, nk)) {
-      em[b] = em[b + 64] = em[b + 128] = 0.0f;
-      vc[b] = vc[b + 64] = vc[b + 128] = 0.0f;
-      flags[i] = -1;
-      continue;
-    }
-    float m = mass[i];
-    active += m > 0.0f ? 1 : 0;
-    float f[3] = {0.0f, 0.0f, 0.0f};
+  int bad = 0, active = 0;
+  float e[3] = {0.0f, 0.0f, 0.0f}, cv[3] = {0.0f, 0.0f, 0.0f};
+  int fl = -1;
+  if (c.valid) {
+    float m = mass[c.i];
+    active = m > 0.0f ? 1 : 0;
+    float f[3] = {0.0f, 0.0f, 0.0f}, fi[3];
     field_force(grav, ngrav, m, f);
-    f[0] += acc[b];
-    f[1] += acc[b + 64];
-    f[2] += acc[b + 128];
-    float ev[3] = {vel[b], vel[b + 64], vel[b + 128]};
+    gather_node3(B, c.G, stage, c.T, c.l, fi);
+    f[0] += fi[0];
+    f[1] += fi[1];
+    f[2] += fi[2];
+    float ev[3];
+    ld3(vel, c.i, ev);
     if (m > 0.0f) {
       ev[0] += dt * f[0] / m;
       ev[1] += dt * f[1] / m;
       ev[2] += dt * f[2] / m;
     }
     float x[3];
-    node_pos(g, ni, nj, nk, x);
-    int fl = collide(*cols, ev, x, g.fv, false);
-    flags[i] = (int8_t)fl;
-    float e0 = ev[0] * m, e1 = ev[1] * m, e2 = ev[2] * m;
-    em[b] = e0;
-    em[b + 64] = e1;
-    em[b + 128] = e2;
-    if (!isfinite(e0 * e1 * e2)) bad = 1;
-    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    node_pos(c.G, c.ni, c.nj, c.nk, x);
+    fl = collide(*cols, ev, x, c.G.fv, false);
+    e[0] = ev[0] * m;
+    e[1] = ev[1] * m;
+    e[2] = ev[2] * m;
+    if (!isfinite(e[0] * e[1] * e[2])) bad = 1;
     if (fl >= 0) {
       const Collider& o = cols->o[fl];
-      c0 = o.v[0] - g.fv[0];
-      c1 = o.v[1] - g.fv[1];
-      c2 = o.v[2] - g.fv[2];
-      s[4] += (double)m;
-      s[5] += (double)(m * c0);
-      s[6] += (double)(m * c1);
-      s[7] += (double)(m * c2);
+      cv[0] = o.v[0] - c.G.fv[0];
+      cv[1] = o.v[1] - c.G.fv[1];
+      cv[2] = o.v[2] - c.G.fv[2];
+      s[4] = (double)m;
+      s[5] = (double)(m * cv[0]);
+      s[6] = (double)(m * cv[1]);
+      s[7] = (double)(m * cv[2]);
     }
-    vc[b] = c0;
-    vc[b + 64] = c1;
-    vc[b + 128] = c2;
-    s[0] += (double)m;
-    s[1] += (double)e0;
-    s[2] += (double)e1;
-    s[3] += (double)e2;
+    s[0] = (double)m;
+    s[1] = (double)e[0];
+    s[2] = (double)e[1];
+    s[3] = (double)e[2];
   }
-  if (bad) atomicOr(&sc->errors, ERR_NAN_MOMENTA);
+  st3(em, c.i, e);
+  st3(vc, c.i, cv);
+  flags[c.i] = (int8_t)fl;
+  CRScalars& S = sc[c.b];
+  if (bad) atomicOr(&S.errors, ERR_NAN_MOMENTA);
   for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
-  if ((threadIdx.x & 63) == 0 && active) atomicAdd(&sc->active_nodes, active);
+  if ((threadIdx.x & 63) == 0 && active) atomicAdd(&S.active_nodes, active);
   double tot[8];
-  if (grid_sum<8>(s, partials, &sc->ticket[0], tot) && threadIdx.x == 0) {
-    for (int k = 0; k < 8; ++k) sc->sums[k] = tot[k];
+  if (body_sum<8>(s, c.G, partials, &S.ticket[0], tot) && threadIdx.x == 0) {
+    for (int k = 0; k < 8; ++k) S.sums[k] = tot[k];
     float gv[3] = {0.0f, 0.0f, 0.0f};
     if (tot[0] == 0.0) {
     } else if (tot[4] == 0.0) {
@@ -681,51 +911,45 @@ __global__ void k_node_explicit(GridView g, const float* mass, const float* vel,
     } else {
       for (int a = 0; a < 3; ++a) gv[a] = (float)(tot[5 + a] / tot[4]);
     }
-    sc->guess[0] = gv[0];
-    sc->guess[1] = gv[1];
-    sc->guess[2] = gv[2];
+    S.guess[0] = gv[0];
+    S.guess[1] = gv[1];
+    S.guess[2] = gv[2];
   }
 }
 
-__global__ void k_collision_velocities(GridView g, const int8_t* flags, const ColliderSet* cols, float* vc) {
-  int64_t nn = (int64_t)g.NT * 64;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nn; i += (int64_t)gridDim.x * kBlock) {
-    int t = (int)(i >> 6), l = (int)(i & 63);
-    int64_t b = (int64_t)t * 192 + l;
-    int fl = flags[i];
-    float c[3] = {0.0f, 0.0f, 0.0f};
-    if (fl >= 0) {
-      const Collider& o = cols->o[fl];
-      for (int a = 0; a < 3; ++a) c[a] = o.v[a] - g.fv[a];
-    }
-    vc[b] = c[0];
-    vc[b + 64] = c[1];
-    vc[b + 128] = c[2];
+__global__ void k_collision_velocities(Batch B, const int8_t* flags, const ColliderSet* cols, float* vc) {
+  NodeCtx c = node_begin(B);
+  int fl = flags[c.i];
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  if (fl >= 0 && c.valid) {
+    const Collider& o = cols->o[fl];
+    for (int a = 0; a < 3; ++a) v[a] = o.v[a] - c.G.fv[a];
   }
+  st3(vc, c.i, v);
 }
 
 // ------------------------------------------------------------ HVP
 // ForceDifferentialSplatter (Grid.cpp:549-618), factorised:
 //   dF_p = (sum_j d_j grad w_jp^T) F_p ; df_i -= V_p dP(dF_p) F_p^T grad w_ip
 template <bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_hvp(PView P, GridView g, const float* d, const int8_t* flags,
-                                                const ColliderSet* cols, float* out, const CRScalars* sc, int* err) {
-  if (sc && sc->done) return;
+__global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d, const int8_t* flags,
+                                                const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
+  if (sc && sc[B.chunks[blockIdx.x].w].done) return;
   __shared__ float sd[3][kFoot];
   __shared__ float so[3][kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   __syncthreads();
-  load_foot3(g, c, ft, d, sd, flags, cols, PROJ);
+  load_foot3(c, ft, d, sd, flags, cols, PROJ);
   for (int i = threadIdx.x; i < kFoot; i += kBlock) so[0][i] = so[1][i] = so[2][i] = 0.0f;
   __syncthreads();
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, true, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -776,20 +1000,20 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, GridView g, const float
         }
   }
   __syncthreads();
-  flush_foot3(ft, so, out);
+  store_stage3(so, stage);
 }
 
-// dFidXiSplatter (Grid.cpp:466-537) through the per-particle quadratic forms:
-// with g = F^T grad w, the reference's entry [-V dP(e_a g^T) F^T grad w]_a equals
-// -V g^T H_a g where H_a(c, d) = dP(e_a e_d^T)(a, c) (dP is linear in dF).
-__global__ __launch_bounds__(kBlock) void k_diag(PView P, GridView g, float* out, int* err) {
+// dFidXiSplatter (Grid.cpp:466-537) through per-particle quadratic forms: with g = F^T grad w,
+// the reference's entry [-V dP(e_a g^T) F^T grad w]_a equals -V g^T H_a g where
+// H_a(c, d) = dP(e_a e_d^T)(a, c) (dP is linear in dF).
+__global__ __launch_bounds__(kBlock) void k_diag(PView P, Batch B, float* stage, int* err) {
   __shared__ float so[3][kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   for (int i = threadIdx.x; i < kFoot; i += kBlock) so[0][i] = so[1][i] = so[2][i] = 0.0f;
   __syncthreads();
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
     float F[9], R[9], Fi[9], Gi[9];
@@ -813,7 +1037,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(PView P, GridView g, float* out
       }
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, true, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -838,269 +1062,231 @@ __global__ __launch_bounds__(kBlock) void k_diag(PView P, GridView g, float* out
         }
   }
   __syncthreads();
-  flush_foot3(ft, so, out);
+  store_stage3(so, stage);
 }
 
 // ------------------------------------------------------------ solver node kernels
-#define NODE_LOOP                                                                                   \
-  int64_t nn = (int64_t)g.NT * 64;                                                                  \
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nn; i += (int64_t)gridDim.x * kBlock)
-
-__device__ __forceinline__ void node_project(const GridView& g, const ColliderSet* cols, const int8_t* flags,
-                                             int64_t i, float v[3]) {
-  int fl = flags[i];
-  if (fl == -1) return;
-  int ni, nj, nk;
-  node_coords(g, (int)(i >> 6), (int)(i & 63), ni, nj, nk);
-  float x[3];
-  node_pos(g, ni, nj, nk, x);
-  project(cols, fl, x, v);
-}
-// ImplicitUpdateMatrix::multVector epilogue (Grid.cpp:764-770): P(m P d - dt^2 df)
-__device__ __forceinline__ void apply_A(const GridView& g, const ColliderSet* cols, const int8_t* flags, int64_t i,
-                                        float m, const float d[3], const float df[3], float dt2, float out[3]) {
-  float pd[3] = {d[0], d[1], d[2]};
-  node_project(g, cols, flags, i, pd);
-  out[0] = m * pd[0] - dt2 * df[0];
-  out[1] = m * pd[1] - dt2 * df[1];
-  out[2] = m * pd[2] - dt2 * df[2];
-  node_project(g, cols, flags, i, out);
-}
-__device__ __forceinline__ void ld3(const float* v, int64_t i, float o[3]) {
-  int64_t b = (i >> 6) * 192 + (i & 63);
-  o[0] = v[b];
-  o[1] = v[b + 64];
-  o[2] = v[b + 128];
-}
-__device__ __forceinline__ void st3(float* v, int64_t i, const float o[3]) {
-  int64_t b = (i >> 6) * 192 + (i & 63);
-  v[b] = o[0];
-  v[b + 64] = o[1];
-  v[b + 128] = o[2];
-}
-
 // rhs (Grid.cpp:982-1002), initial guess fill (Grid.cpp:944-947), diagonal (Grid.cpp:847-866), |b|^2
-__global__ void k_rhs(GridView g, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* accvc,
-                      const float* draw, float dt, float tol, float* b, float* x, float* diag, CRScalars* sc,
-                      double* partials) {
+__global__ void k_rhs(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stageVc,
+                      const float* stageDiag, float dt, float tol, float* b, float* x, float* diag, CRScalars* sc,
+                      BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
   double s[1] = {0.0};
   int bad = 0;
   const float dt2 = dt * dt;
   const float ndt2 = -dt * dt;
-  const float g0 = sc->guess[0], g1 = sc->guess[1], g2 = sc->guess[2];
-  NODE_LOOP {
-    float df[3], bb[3], dr[3];
-    ld3(accvc, i, df);
-    node_project(g, cols, flags, i, df);
-    ld3(b, i, bb);
+  float bb[3], gx[3] = {S.guess[0], S.guess[1], S.guess[2]}, dr[3] = {0.0f, 0.0f, 0.0f};
+  ld3(b, c.i, bb);
+  float m = mass[c.i];
+  if (c.valid) {
+    float df[3];
+    gather_node3(B, c.G, stageVc, c.T, c.l, df);
+    node_project(c, cols, flags[c.i], df);
     bb[0] += dt2 * df[0];
     bb[1] += dt2 * df[1];
     bb[2] += dt2 * df[2];
     if (!isfinite(bb[0] * bb[1] * bb[2])) bad = 1;
-    st3(b, i, bb);
-    float gx[3] = {g0, g1, g2};
-    st3(x, i, gx);
-    ld3(draw, i, dr);
-    float m = mass[i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      dr[a] = dr[a] * ndt2 + m;
-      if (dr[a] == 0.0f) dr[a] = 1.0f;
-    }
-    st3(diag, i, dr);
-    s[0] += (double)bb[0] * bb[0] + (double)bb[1] * bb[1] + (double)bb[2] * bb[2];
+    gather_node3(B, c.G, stageDiag, c.T, c.l, dr);
   }
-  if (bad) atomicOr(&sc->errors, ERR_NAN_MOMENTA);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dr[a] = dr[a] * ndt2 + m;
+    if (dr[a] == 0.0f) dr[a] = 1.0f;
+  }
+  st3(b, c.i, bb);
+  st3(x, c.i, gx);
+  st3(diag, c.i, dr);
+  s[0] = (double)bb[0] * bb[0] + (double)bb[1] * bb[1] + (double)bb[2] * bb[2];
+  if (bad) atomicOr(&S.errors, ERR_NAN_MOMENTA);
   double tot[1];
-  if (grid_sum<1>(s, partials, &sc->ticket[1], tot) && threadIdx.x == 0) {
-    sc->bnorm2 = tot[0];
-    sc->threshold = (double)tol * (double)tol * tot[0];
-    sc->iter = 0;
-    sc->iterations = 0;
-    sc->early_exit = 0;
-    sc->zero_x = 0;
-    sc->done = 0;
-    if (sc->errors & ERR_NAN_MOMENTA) sc->done = 1;
-    if (tot[0] == 0.0) {
-      sc->done = 1;
-      sc->zero_x = 1;
+  if (body_sum<1>(s, c.G, partials, &S.ticket[1], tot) && threadIdx.x == 0) {
+    S.bnorm2 = tot[0];
+    S.threshold = (double)tol * (double)tol * tot[0];
+    S.iter = 0;
+    S.iterations = 0;
+    S.early_exit = 0;
+    S.zero_x = 0;
+    if (S.errors & ERR_NAN_MOMENTA) {
+      atomicOr(&bst->errors, ERR_NAN_MOMENTA);
+      set_done(S, bst);
+    } else if (tot[0] == 0.0) {
+      S.zero_x = 1;
+      set_done(S, bst);
     }
   }
 }
 
 // CR setup part 1 (ConjugateResiduals.cpp:50-67): r = b - A x ; z = M^-1 r ; p = z
-__global__ void k_cr_setup1(GridView g, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                            const float* acc, const float* x, const float* b, const float* diag, float dt, float* z,
+__global__ void k_cr_setup1(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                            const float* stage, const float* x, const float* b, const float* diag, float dt, float* z,
                             float* p, const CRScalars* sc) {
-  if (sc->done) return;
+  NodeCtx c = node_begin(B);
+  if (sc[c.b].done) return;
   const float dt2 = dt * dt;
-  NODE_LOOP {
-    float xv[3], df[3], Ax[3], bb[3], dg[3], zz[3];
-    ld3(x, i, xv);
-    ld3(acc, i, df);
-    apply_A(g, cols, flags, i, mass[i], xv, df, dt2, Ax);
-    ld3(b, i, bb);
-    ld3(diag, i, dg);
+  float xv[3], df[3] = {0.0f, 0.0f, 0.0f}, Ax[3], bb[3], dg[3], zz[3];
+  ld3(x, c.i, xv);
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
+  apply_A(c, cols, flags[c.i], mass[c.i], xv, df, dt2, Ax);
+  ld3(b, c.i, bb);
+  ld3(diag, c.i, dg);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) zz[a] = (bb[a] - Ax[a]) / dg[a];
-    st3(z, i, zz);
-    st3(p, i, zz);
-  }
+  for (int a = 0; a < 3; ++a) zz[a] = (bb[a] - Ax[a]) / dg[a];
+  st3(z, c.i, zz);
+  st3(p, c.i, zz);
 }
 // CR setup part 2 (ConjugateResiduals.cpp:69-76 and the first alpha, :86-102):
 // Ap = Ar = A z ; rz = <z, Ar> ; alpha = rz / <Ap, M^-1 Ap>
-__global__ void k_cr_setup2(GridView g, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                            const float* acc, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
-                            double* partials) {
-  if (sc->done) return;
+__global__ void k_cr_setup2(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                            const float* stage, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
+                            BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
   const float dt2 = dt * dt;
   double s[2] = {0.0, 0.0};
-  NODE_LOOP {
-    float zz[3], df[3], Ar[3], dg[3];
-    ld3(z, i, zz);
-    ld3(acc, i, df);
-    apply_A(g, cols, flags, i, mass[i], zz, df, dt2, Ar);
-    st3(Ap, i, Ar);
-    ld3(diag, i, dg);
+  float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, Ar[3], dg[3];
+  ld3(z, c.i, zz);
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
+  apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, Ar);
+  st3(Ap, c.i, Ar);
+  ld3(diag, c.i, dg);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      s[0] += (double)zz[a] * Ar[a];
-      s[1] += (double)Ar[a] * (Ar[a] / dg[a]);
-    }
+  for (int a = 0; a < 3; ++a) {
+    s[0] += (double)zz[a] * Ar[a];
+    s[1] += (double)Ar[a] * (Ar[a] / dg[a]);
   }
   double tot[2];
-  if (grid_sum<2>(s, partials, &sc->ticket[2], tot) && threadIdx.x == 0) {
-    sc->rz = tot[0];
-    sc->ApdPAp = tot[1];
+  if (body_sum<2>(s, c.G, partials, &S.ticket[2], tot) && threadIdx.x == 0) {
+    S.rz = tot[0];
+    S.ApdPAp = tot[1];
+    S.iter = 0;
     if (tot[1] == 0.0) {
-      sc->done = 1;
-      sc->early_exit = 1;
+      S.early_exit = 1;
+      set_done(S, bst);
     } else {
-      sc->alpha = tot[0] / tot[1];
+      S.alpha = tot[0] / tot[1];
     }
-    sc->iter = 0;
   }
 }
 // iteration part 1 (ConjugateResiduals.cpp:104-128): x = P(x + alpha p) ; z -= alpha M^-1 Ap ;
 // r = M z ; termination on |r|^2 (SquareMagnitudeTermination.cpp:19-28)
-__global__ void k_cr_update_x(GridView g, const int8_t* flags, const ColliderSet* cols, const float* p,
-                              const float* Ap, const float* diag, float* x, float* z, CRScalars* sc, double* partials,
-                              double* hist, int hist_cap) {
-  if (sc->done) return;
-  const float alpha = (float)sc->alpha;
+__global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* cols, const float* p, const float* Ap,
+                              const float* diag, float* x, float* z, CRScalars* sc, BatchStatus* bst, double* partials,
+                              double* hist) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
+  const float alpha = (float)S.alpha;
   double s[1] = {0.0};
-  NODE_LOOP {
-    float xv[3], pv[3], ap[3], dg[3], zz[3];
-    ld3(x, i, xv);
-    ld3(p, i, pv);
+  float xv[3], pv[3], ap[3], dg[3], zz[3];
+  ld3(x, c.i, xv);
+  ld3(p, c.i, pv);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) xv[a] = xv[a] + alpha * pv[a];
-    node_project(g, cols, flags, i, xv);
-    st3(x, i, xv);
-    ld3(Ap, i, ap);
-    ld3(diag, i, dg);
-    ld3(z, i, zz);
+  for (int a = 0; a < 3; ++a) xv[a] = xv[a] + alpha * pv[a];
+  node_project(c, cols, flags[c.i], xv);
+  st3(x, c.i, xv);
+  ld3(Ap, c.i, ap);
+  ld3(diag, c.i, dg);
+  ld3(z, c.i, zz);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      zz[a] = zz[a] - alpha * (ap[a] / dg[a]);
-      float r = zz[a] * dg[a];
-      s[0] += (double)r * r;
-    }
-    st3(z, i, zz);
+  for (int a = 0; a < 3; ++a) {
+    zz[a] = zz[a] - alpha * (ap[a] / dg[a]);
+    float r = zz[a] * dg[a];
+    s[0] += (double)r * r;
   }
+  st3(z, c.i, zz);
   double tot[1];
-  if (grid_sum<1>(s, partials, &sc->ticket[0], tot) && threadIdx.x == 0) {
-    int it = sc->iter;
-    sc->rnorm2 = tot[0];
-    sc->iterations = it + 1;
-    if (hist && it < hist_cap) hist[it] = tot[0];
-    if (it >= sc->max_iters || tot[0] < sc->threshold) sc->done = 1;
+  if (body_sum<1>(s, c.G, partials, &S.ticket[0], tot) && threadIdx.x == 0) {
+    int it = S.iter;
+    S.rnorm2 = tot[0];
+    S.iterations = it + 1;
+    if (hist && it < kHistCap) hist[(int64_t)c.b * kHistCap + it] = tot[0];
+    if (it >= S.max_iters || tot[0] < S.threshold) set_done(S, bst);
   }
 }
 // iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta
-__global__ void k_cr_ar(GridView g, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* acc,
-                        const float* z, float dt, float* Ar, CRScalars* sc, double* partials) {
-  if (sc->done) return;
+__global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
+                        const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
   const float dt2 = dt * dt;
   double s[1] = {0.0};
-  NODE_LOOP {
-    float zz[3], df[3], ar[3];
-    ld3(z, i, zz);
-    ld3(acc, i, df);
-    apply_A(g, cols, flags, i, mass[i], zz, df, dt2, ar);
-    st3(Ar, i, ar);
-    s[0] += (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
-  }
+  float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, ar[3];
+  ld3(z, c.i, zz);
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
+  apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, ar);
+  st3(Ar, c.i, ar);
+  s[0] = (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
   double tot[1];
-  if (grid_sum<1>(s, partials, &sc->ticket[1], tot) && threadIdx.x == 0) {
-    double rz_old = sc->rz;
-    sc->rz = tot[0];
+  if (body_sum<1>(s, c.G, partials, &S.ticket[1], tot) && threadIdx.x == 0) {
+    double rz_old = S.rz;
+    S.rz = tot[0];
     if (rz_old == 0.0) {
-      sc->done = 1;
-      sc->early_exit = 1;
+      S.early_exit = 1;
+      set_done(S, bst);
     } else {
-      sc->beta = tot[0] / rz_old;
+      S.beta = tot[0] / rz_old;
     }
   }
 }
 // iteration part 3 (ConjugateResiduals.cpp:147-151 and the next alpha, :86-102)
-__global__ void k_cr_update_p(GridView g, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                              CRScalars* sc, double* partials) {
-  if (sc->done) return;
-  const float beta = (float)sc->beta;
+__global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
+                              CRScalars* sc, BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
+  const float beta = (float)S.beta;
   double s[1] = {0.0};
-  NODE_LOOP {
-    float zz[3], pv[3], ar[3], ap[3], dg[3];
-    ld3(z, i, zz);
-    ld3(p, i, pv);
-    ld3(Ar, i, ar);
-    ld3(Ap, i, ap);
-    ld3(diag, i, dg);
+  float zz[3], pv[3], ar[3], ap[3], dg[3];
+  ld3(z, c.i, zz);
+  ld3(p, c.i, pv);
+  ld3(Ar, c.i, ar);
+  ld3(Ap, c.i, ap);
+  ld3(diag, c.i, dg);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      pv[a] = zz[a] + beta * pv[a];
-      ap[a] = ar[a] + beta * ap[a];
-      s[0] += (double)ap[a] * (ap[a] / dg[a]);
-    }
-    st3(p, i, pv);
-    st3(Ap, i, ap);
+  for (int a = 0; a < 3; ++a) {
+    pv[a] = zz[a] + beta * pv[a];
+    ap[a] = ar[a] + beta * ap[a];
+    s[0] += (double)ap[a] * (ap[a] / dg[a]);
   }
+  st3(p, c.i, pv);
+  st3(Ap, c.i, ap);
   double tot[1];
-  if (grid_sum<1>(s, partials, &sc->ticket[2], tot) && threadIdx.x == 0) {
-    sc->ApdPAp = tot[0];
+  if (body_sum<1>(s, c.G, partials, &S.ticket[2], tot) && threadIdx.x == 0) {
+    S.ApdPAp = tot[0];
     if (tot[0] == 0.0) {
-      sc->done = 1;
-      sc->early_exit = 1;
+      S.early_exit = 1;
+      set_done(S, bst);
     } else {
-      sc->alpha = sc->rz / tot[0];
+      S.alpha = S.rz / tot[0];
     }
-    sc->iter = sc->iter + 1;
+    S.iter = S.iter + 1;
   }
 }
-__global__ void k_residual_vector(GridView g, const float* z, const float* diag, float* r) {
-  NODE_LOOP {
-    float zz[3], dg[3];
-    ld3(z, i, zz);
-    ld3(diag, i, dg);
-    zz[0] *= dg[0];
-    zz[1] *= dg[1];
-    zz[2] *= dg[2];
-    st3(r, i, zz);
-  }
+__global__ void k_residual_vector(Batch B, const float* z, const float* diag, float* r) {
+  NodeCtx c = node_begin(B);
+  float zz[3], dg[3];
+  ld3(z, c.i, zz);
+  ld3(diag, c.i, dg);
+  zz[0] *= dg[0];
+  zz[1] *= dg[1];
+  zz[2] *= dg[2];
+  st3(r, c.i, zz);
 }
 // m_velocities += vc (Grid.cpp:1014); x = 0 when the rhs was zero
-__global__ void k_finalize(GridView g, const float* x, const float* vc, float* vel, const CRScalars* sc) {
-  const bool zx = sc->zero_x != 0;
-  NODE_LOOP {
-    float xv[3], cv[3];
-    ld3(x, i, xv);
-    ld3(vc, i, cv);
-    if (zx) xv[0] = xv[1] = xv[2] = 0.0f;
-    xv[0] += cv[0];
-    xv[1] += cv[1];
-    xv[2] += cv[2];
-    st3(vel, i, xv);
-  }
+__global__ void k_finalize(Batch B, const float* x, const float* vc, float* vel, const CRScalars* sc) {
+  NodeCtx c = node_begin(B);
+  const bool zx = sc[c.b].zero_x != 0;
+  float xv[3], cv[3];
+  ld3(x, c.i, xv);
+  ld3(vc, c.i, cv);
+  if (zx) xv[0] = xv[1] = xv[2] = 0.0f;
+  xv[0] += cv[0];
+  xv[1] += cv[1];
+  xv[2] += cv[2];
+  st3(vel, c.i, xv);
 }
 
 // ------------------------------------------------------------ G2P
@@ -1108,14 +1294,14 @@ __global__ void k_finalize(GridView g, const float* x, const float* vc, float* v
 // and for the full step SnowConstitutiveModel::updateParticleData (SnowConstitutiveModel.cpp:27-121)
 // plus the advection of Sim.cpp:123-131.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_g2p(PView P, GridView g, const float* vel, const float* prev, float dt,
+__global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* vel, const float* prev, float dt,
                                                 float thc, float ths, const ColliderSet* cols, int* err) {
   constexpr bool DO_V = MODE != 1;
   constexpr bool DO_F = MODE != 0;
   __shared__ float sv[3][kFoot];
   __shared__ float sp[DO_V ? 3 : 1][kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(g, ft);
+  ChunkCtx c = chunk_begin(B, ft);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int gt = ft[i >> 6];
@@ -1130,12 +1316,12 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, GridView g, const float
   const float alpha = 0.95f;
   const float beta = 1.0f - alpha;
   for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
-    int s = g.perm[c.ch.y + k];
+    int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
     Stencil st;
     int rx, ry, rz;
-    if (!chunk_stencil(g, c, x, DO_F, st, rx, ry, rz)) {
+    if (!chunk_stencil(c, x, DO_F, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       continue;
     }
@@ -1161,7 +1347,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, GridView g, const float
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               vf[a] += w * (vi[a] - sp[a][idx]);
-              vp[a] += w * (vi[a] + g.fv[a]);
+              vp[a] += w * (vi[a] + c.G.fv[a]);
             }
           }
           if (DO_F) {
@@ -1220,9 +1406,10 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, GridView g, const float
   }
 }
 
-__global__ void k_plasticity(PView P, const int* slots, int base, int n, float thc, float ths, int* err) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    int s = slots ? slots[k] : base + k;
+__global__ void k_plasticity(PView P, const int* labels, int skipLabel, int n, float thc, float ths, int* err) {
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    if (labels && labels[s] == skipLabel) continue;
     float F[9], Fp[9], Fi[9], R[9], Gi[9], J, mu, lam;
     load_m9(P, C_F, s, F);
     load_m9(P, C_FP, s, Fp);
@@ -1240,10 +1427,11 @@ __global__ void k_plasticity(PView P, const int* slots, int base, int n, float t
 }
 
 // ballistic particles (Sim.cpp:64-73) and/or their advection (Sim.cpp:123-131)
-__global__ void k_ballistic(PView P, const int* slots, int n, const float* grav, int ngrav, float dt,
+__global__ void k_ballistic(PView P, const int* labels, int n, const float* grav, int ngrav, float dt,
                             const ColliderSet* cols, int mode) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    int s = slots[k];
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    if (labels[s] != -1) continue;
     float v[3], x[3];
     load_x(P, s, x);
 #pragma unroll
@@ -1268,9 +1456,10 @@ __global__ void k_ballistic(PView P, const int* slots, int n, const float* grav,
     for (int q = 0; q < 3; ++q) P.at(C_V + q, s) = v[q];
   }
 }
-__global__ void k_advect(PView P, const int* slots, int n, float dt, const ColliderSet* cols) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    int s = slots[k];
+__global__ void k_advect(PView P, const int* labels, int n, float dt, const ColliderSet* cols) {
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    if (labels[s] < 0) continue;
     float v[3], x[3];
     load_x(P, s, x);
 #pragma unroll
@@ -1284,9 +1473,8 @@ __global__ void k_advect(PView P, const int* slots, int n, float dt, const Colli
     }
   }
 }
-
 __global__ void k_probe(PView P, const int* slots, int n, int mode, const float* dF, float* out) {
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+  GRID_STRIDE(k, n) {
     int s = slots[k];
     float F[9], R[9], Fi[9], Gi[9];
     load_m9(P, C_F, s, F);
@@ -1303,12 +1491,11 @@ __global__ void k_probe(PView P, const int* slots, int n, int mode, const float*
     }
   }
 }
-// velocity finiteness (Sim.cpp:151-159)
-__global__ void k_check_pvel(PView P, int n, int* err) {
-  int bad = 0;
-  for (int s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
-    if (!isfinite(P.at(C_V, s) * P.at(C_V + 1, s) * P.at(C_V + 2, s))) bad = 1;
-  if (bad) atomicOr(err, ERR_NAN_PVEL);
+__global__ void k_is_ballistic(const int* labels, int n, int* flag) {
+  GRID_STRIDE(s, n) flag[s] = labels[s] == -1 ? 1 : 0;
+}
+__global__ void k_ballistic_compact(const int* labels, const int* pos, int n, int* out, int offset) {
+  GRID_STRIDE(s, n) if (labels[s] == -1) out[offset + pos[s]] = (int)s;
 }
 
 }  // namespace
@@ -1325,39 +1512,71 @@ void launch_get_column(PView P, int col0, int ncomp, float* aos, const int* slot
 void launch_fill_column(PView P, int col, float value, int n, hipStream_t s) {
   if (n > 0) LAUNCH(k_fill_column, grid_for(n), P, col, value, n);
 }
+void launch_fill_int(int* p, int value, int64_t n, hipStream_t s) {
+  if (n > 0) LAUNCH(k_fill_int, grid_for(n), p, value, n);
+}
 void launch_identity_columns(PView P, int col0, int n, hipStream_t s) {
   if (n > 0) LAUNCH(k_identity_columns, grid_for(n), P, col0, n);
 }
 void launch_permute(PView src, PView dst, const int* new_to_old, int n, const int* orig_of_old, int* orig_of_new,
                     int* slot_of, hipStream_t s) {
-  if (n > 0) LAUNCH(k_permute, grid_for(n, 8192), src, dst, new_to_old, n, orig_of_old, orig_of_new, slot_of);
+  if (n > 0) LAUNCH(k_permute, grid_for(n, 16384), src, dst, new_to_old, n, orig_of_old, orig_of_new, slot_of);
 }
 void launch_gather_slots(const int* slot_of, const int* orig, int n, int* slots, hipStream_t s) {
   if (n > 0) LAUNCH(k_gather_slots, grid_for(n), slot_of, orig, n, slots);
 }
-void launch_body_reduce(PView P, const int* slots, int base, int n, unsigned int* bbox6, double* mom4, hipStream_t s) {
-  if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 2048), P, slots, base, n, bbox6, mom4);
+void launch_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count, hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_hash, grid_for(n, 16384), P, n, cell, mask, bucket, rank, count);
 }
-void launch_bin(PView P, const int* slots, int base, int n, const GridView& g, int* key, int* rank, int* count, int* err,
-                hipStream_t s) {
-  if (n > 0) LAUNCH(k_bin, grid_for(n, 8192), P, slots, base, n, g, key, rank, count, err);
+void launch_cc_scatter(const int* bucket, const int* rank, const int* offset, int n, int* sorted, hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_scatter, grid_for(n, 16384), bucket, rank, offset, n, sorted);
 }
-void launch_scatter(const int* slots, int base, const int* key, const int* rank, const int* offset, int n, int* perm,
+void launch_cc_link(PView P, int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                    const int* sorted, int* parent, int* hasNb, hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_link, grid_for(n, 16384), P, n, h, cell, mask, offset, count, sorted, parent, hasNb);
+}
+void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
+                     hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_roots, grid_for(n, 16384), n, parent, hasNb, orig_of, isRoot, minOrig);
+}
+void launch_cc_compact(int n, const int* isRoot, const int* rootIdx, const int* minOrig, int* rootMin, hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_compact, grid_for(n, 16384), n, isRoot, rootIdx, minOrig, rootMin);
+}
+void launch_cc_assign(int n, const int* parent, const int* hasNb, const int* rootIdx, const int* rootRank,
+                      int* labels, hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_assign, grid_for(n, 16384), n, parent, hasNb, rootIdx, rootRank, labels);
+}
+void launch_check_pvel(PView P, int n, int* err, hipStream_t s) {
+  if (n > 0) LAUNCH(k_check_pvel, grid_for(n), P, n, err);
+}
+void launch_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
+                        int* count, hipStream_t s) {
+  if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 4096), P, slots, labels, n, bbox, mom, count);
+}
+void launch_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody, hipStream_t s) {
+  if (NT > 0) LAUNCH(k_tile_body, grid_for(NT), bodies, nb, NT, tileBody, blockBody);
+}
+void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
+                int* count, int* err, hipStream_t s) {
+  if (n > 0) LAUNCH(k_bin, grid_for(n, 16384), P, slots, labels, n, bodies, key, rank, count, err);
+}
+void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
                     hipStream_t s) {
-  if (n > 0) LAUNCH(k_scatter, grid_for(n, 8192), slots, base, key, rank, offset, n, perm);
+  if (n > 0) LAUNCH(k_scatter, grid_for(n, 16384), slots, key, rank, offset, n, perm);
 }
 void launch_chunk_counts(const int* count, int NT, int* nch, hipStream_t s) {
-  LAUNCH(k_chunk_counts, grid_for(NT), count, NT, nch);
+  if (NT > 0) LAUNCH(k_chunk_counts, grid_for(NT), count, NT, nch);
 }
-void launch_make_chunks(const int* count, const int* offset, const int* chunk_off, int NT, int4* chunks, hipStream_t s) {
-  LAUNCH(k_make_chunks, grid_for(NT), count, offset, chunk_off, NT, chunks);
+void launch_make_chunks(const int* count, const int* offset, const int* chunk_off, const int* tileBody, int NT,
+                        int4* chunks, hipStream_t s) {
+  if (NT > 0) LAUNCH(k_make_chunks, grid_for(NT), count, offset, chunk_off, tileBody, NT, chunks);
 }
 // per level: block sums (nb) + 8 pad + block offsets (nb) + 8 pad, then the next level
-int64_t scan_tmp_ints(int n) {
+int64_t scan_tmp_ints(int64_t n) {
   int64_t tot = 0;
   while (n > 1) {
     n = cdiv(n, 1024);
-    tot += 2 * (int64_t)n + 16;
+    tot += 2 * n + 16;
   }
   return tot + 16;
 }
@@ -1374,123 +1593,132 @@ void scan_exclusive(const int* in, int* out, int n, int* total, int* tmp, hipStr
   scan_exclusive(bsum, boff, nb, total, boff + nb + 8, s);
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kBlock), 0, s, out, n, (const int*)boff);
 }
-void launch_to_dense_f(const float* src, int ncomp, float* dst, const GridView& g, hipStream_t s) {
-  LAUNCH(k_to_dense<float>, grid_for((int64_t)g.n[0] * g.n[1] * g.n[2]), src, ncomp, dst, g);
+void launch_to_dense_f(const float* src, int ncomp, float* dst, const BodyGrid& G, hipStream_t s) {
+  LAUNCH(k_to_dense<float>, grid_for((int64_t)G.n[0] * G.n[1] * G.n[2]), src, ncomp, dst, G);
 }
-void launch_from_dense_f(const float* src, int ncomp, float* dst, const GridView& g, hipStream_t s) {
-  LAUNCH(k_from_dense<float>, grid_for((int64_t)g.n[0] * g.n[1] * g.n[2]), src, ncomp, dst, g);
+void launch_from_dense_f(const float* src, int ncomp, float* dst, const BodyGrid& G, hipStream_t s) {
+  LAUNCH(k_from_dense<float>, grid_for((int64_t)G.n[0] * G.n[1] * G.n[2]), src, ncomp, dst, G);
 }
-void launch_to_dense_i8(const int8_t* src, int8_t* dst, const GridView& g, hipStream_t s) {
-  LAUNCH(k_to_dense<int8_t>, grid_for((int64_t)g.n[0] * g.n[1] * g.n[2]), src, 1, dst, g);
+void launch_to_dense_i8(const int8_t* src, int8_t* dst, const BodyGrid& G, hipStream_t s) {
+  LAUNCH(k_to_dense<int8_t>, grid_for((int64_t)G.n[0] * G.n[1] * G.n[2]), src, 1, dst, G);
 }
-void launch_from_dense_i8(const int8_t* src, int8_t* dst, const GridView& g, hipStream_t s) {
-  LAUNCH(k_from_dense<int8_t>, grid_for((int64_t)g.n[0] * g.n[1] * g.n[2]), src, 1, dst, g);
+void launch_from_dense_i8(const int8_t* src, int8_t* dst, const BodyGrid& G, hipStream_t s) {
+  LAUNCH(k_from_dense<int8_t>, grid_for((int64_t)G.n[0] * G.n[1] * G.n[2]), src, 1, dst, G);
 }
-void launch_p2g_mass(PView P, const GridView& g, float* mass, int* err, hipStream_t s) {
-  if (g.nchunks > 0) LAUNCH(k_p2g_mass, g.nchunks, P, g, mass, err);
+void launch_p2g_mass(PView P, const Batch& B, float* stage, int* err, hipStream_t s) {
+  if (B.nchunks > 0) LAUNCH(k_p2g_mass, B.nchunks, P, B, stage, err);
 }
-void launch_mass_fix(float* mass, const GridView& g, int* err, hipStream_t s) {
-  LAUNCH(k_mass_fix, grid_for((int64_t)g.NT * 64), mass, (int64_t)g.NT * 64, err);
+void launch_p2g_vel(PView P, const Batch& B, const float* mass, float* stage, int* err, hipStream_t s) {
+  if (B.nchunks > 0) LAUNCH(k_p2g_vel, B.nchunks, P, B, mass, stage, err);
 }
-void launch_p2g_vel(PView P, const GridView& g, const float* mass, float* vel, int* err, hipStream_t s) {
-  if (g.nchunks > 0) LAUNCH(k_p2g_vel, g.nchunks, P, g, mass, vel, err);
+void launch_volumes(PView P, const Batch& B, const float* mass, int* err, hipStream_t s) {
+  if (B.nchunks > 0) LAUNCH(k_volumes, B.nchunks, P, B, mass, err);
 }
-void launch_check_vel(const float* vel, float* prev, const GridView& g, int* err, hipStream_t s) {
-  LAUNCH(k_check_vel, grid_for((int64_t)g.NT * 192), vel, prev, (int64_t)g.NT * 192, err);
+void launch_forces(PView P, const Batch& B, float* stage, int* err, hipStream_t s) {
+  if (B.nchunks > 0) LAUNCH(k_forces, B.nchunks, P, B, stage, err);
 }
-void launch_volumes(PView P, const GridView& g, const float* mass, int* err, hipStream_t s) {
-  if (g.nchunks > 0) LAUNCH(k_volumes, g.nchunks, P, g, mass, err);
-}
-void launch_forces(PView P, const GridView& g, float* acc, int* err, hipStream_t s) {
-  if (g.nchunks > 0) LAUNCH(k_forces, g.nchunks, P, g, acc, err);
-}
-void launch_node_forces(const GridView& g, const float* mass, const float* acc, const float* grav, int ngrav,
-                        float* out, hipStream_t s) {
-  LAUNCH(k_node_forces, grid_for((int64_t)g.NT * 64), g, mass, acc, grav, ngrav, out);
-}
-void launch_node_explicit(const GridView& g, const float* mass, const float* vel, const float* acc, const float* grav,
-                          int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                          CRScalars* sc, double* partials, hipStream_t s) {
-  LAUNCH(k_node_explicit, grid_for((int64_t)g.NT * 64), g, mass, vel, acc, grav, ngrav, dt, cols, em, flags, vc, sc,
-         partials);
-}
-void launch_collision_velocities(const GridView& g, const int8_t* flags, const ColliderSet* cols, float* vc,
-                                 hipStream_t s) {
-  LAUNCH(k_collision_velocities, grid_for((int64_t)g.NT * 64), g, flags, cols, vc);
-}
-void launch_hvp(PView P, const GridView& g, const float* d, bool project, const int8_t* flags, const ColliderSet* cols,
-                float* out, const CRScalars* sc, int* err, hipStream_t s) {
-  if (g.nchunks <= 0) return;
+void launch_hvp(PView P, const Batch& B, const float* d, bool project, const int8_t* flags, const ColliderSet* cols,
+                float* stage, const CRScalars* sc, int* err, hipStream_t s) {
+  if (B.nchunks <= 0) return;
   if (project)
-    LAUNCH(k_hvp<true>, g.nchunks, P, g, d, flags, cols, out, sc, err);
+    LAUNCH(k_hvp<true>, B.nchunks, P, B, d, flags, cols, stage, sc, err);
   else
-    LAUNCH(k_hvp<false>, g.nchunks, P, g, d, flags, cols, out, sc, err);
+    LAUNCH(k_hvp<false>, B.nchunks, P, B, d, flags, cols, stage, sc, err);
 }
-void launch_diag(PView P, const GridView& g, float* out, int* err, hipStream_t s) {
-  if (g.nchunks > 0) LAUNCH(k_diag, g.nchunks, P, g, out, err);
+void launch_diag(PView P, const Batch& B, float* stage, int* err, hipStream_t s) {
+  if (B.nchunks > 0) LAUNCH(k_diag, B.nchunks, P, B, stage, err);
 }
-void launch_rhs(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* accvc,
-                const float* draw, float dt, float tol, float* b, float* x, float* diag, CRScalars* sc,
-                double* partials, hipStream_t s) {
-  LAUNCH(k_rhs, grid_for((int64_t)g.NT * 64), g, mass, flags, cols, accvc, draw, dt, tol, b, x, diag, sc, partials);
-}
-void launch_cr_setup1(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                      const float* acc, const float* x, const float* b, const float* diag, float dt, float* z, float* p,
-                      const CRScalars* sc, hipStream_t s) {
-  LAUNCH(k_cr_setup1, grid_for((int64_t)g.NT * 64), g, mass, flags, cols, acc, x, b, diag, dt, z, p, sc);
-}
-void launch_cr_setup2(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                      const float* acc, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
-                      double* partials, hipStream_t s) {
-  LAUNCH(k_cr_setup2, grid_for((int64_t)g.NT * 64), g, mass, flags, cols, acc, z, diag, dt, Ap, sc, partials);
-}
-void launch_cr_update_x(const GridView& g, const int8_t* flags, const ColliderSet* cols, const float* p,
-                        const float* Ap, const float* diag, float* x, float* z, CRScalars* sc, double* partials,
-                        double* hist, int hist_cap, hipStream_t s) {
-  LAUNCH(k_cr_update_x, grid_for((int64_t)g.NT * 64), g, flags, cols, p, Ap, diag, x, z, sc, partials, hist, hist_cap);
-}
-void launch_cr_ar(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* acc,
-                  const float* z, float dt, float* Ar, CRScalars* sc, double* partials, hipStream_t s) {
-  LAUNCH(k_cr_ar, grid_for((int64_t)g.NT * 64), g, mass, flags, cols, acc, z, dt, Ar, sc, partials);
-}
-void launch_cr_update_p(const GridView& g, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                        CRScalars* sc, double* partials, hipStream_t s) {
-  LAUNCH(k_cr_update_p, grid_for((int64_t)g.NT * 64), g, z, Ar, diag, p, Ap, sc, partials);
-}
-void launch_residual_vector(const GridView& g, const float* z, const float* diag, float* r, hipStream_t s) {
-  LAUNCH(k_residual_vector, grid_for((int64_t)g.NT * 64), g, z, diag, r);
-}
-void launch_finalize(const GridView& g, const float* x, const float* vc, float* vel, const CRScalars* sc,
-                     hipStream_t s) {
-  LAUNCH(k_finalize, grid_for((int64_t)g.NT * 64), g, x, vc, vel, sc);
-}
-void launch_g2p(PView P, const GridView& g, const float* vel, const float* prev, int mode, float dt, float thc,
-                float ths, const ColliderSet* cols, int* err, hipStream_t s) {
-  if (g.nchunks <= 0) return;
+void launch_g2p(PView P, const Batch& B, const float* vel, const float* prev, int mode, float dt, float thc, float ths,
+                const ColliderSet* cols, int* err, hipStream_t s) {
+  if (B.nchunks <= 0) return;
   if (mode == 0)
-    LAUNCH(k_g2p<0>, g.nchunks, P, g, vel, prev, dt, thc, ths, cols, err);
+    LAUNCH(k_g2p<0>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
   else if (mode == 1)
-    LAUNCH(k_g2p<1>, g.nchunks, P, g, vel, prev, dt, thc, ths, cols, err);
+    LAUNCH(k_g2p<1>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
   else if (mode == 2)
-    LAUNCH(k_g2p<2>, g.nchunks, P, g, vel, prev, dt, thc, ths, cols, err);
+    LAUNCH(k_g2p<2>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
   else
-    LAUNCH(k_g2p<3>, g.nchunks, P, g, vel, prev, dt, thc, ths, cols, err);
+    LAUNCH(k_g2p<3>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
 }
-void launch_plasticity(PView P, const int* slots, int base, int n, float thc, float ths, int* err, hipStream_t s) {
-  if (n > 0) LAUNCH(k_plasticity, grid_for(n, 8192), P, slots, base, n, thc, ths, err);
+#define NODE_LAUNCH(kernel, ...) \
+  if (B.nblocks > 0) hipLaunchKernelGGL(kernel, dim3(B.nblocks), dim3(kBlock), 0, s, __VA_ARGS__)
+void launch_gather_mass(const Batch& B, const float* stage, float* mass, int* err, hipStream_t s) {
+  NODE_LAUNCH(k_gather_mass, B, stage, mass, err);
 }
-void launch_ballistic(PView P, const int* slots, int n, const float* grav, int ngrav, float dt, const ColliderSet* cols,
-                      int mode, hipStream_t s) {
-  if (n > 0) LAUNCH(k_ballistic, grid_for(n), P, slots, n, grav, ngrav, dt, cols, mode);
+void launch_gather_vel(const Batch& B, const float* stage, float* vel, float* prev, int* err, hipStream_t s) {
+  NODE_LAUNCH(k_gather_vel, B, stage, vel, prev, err);
 }
-void launch_advect(PView P, const int* slots, int n, float dt, const ColliderSet* cols, hipStream_t s) {
-  if (n > 0) LAUNCH(k_advect, grid_for(n), P, slots, n, dt, cols);
+void launch_gather3(const Batch& B, const float* stage, float* out, hipStream_t s) {
+  NODE_LAUNCH(k_gather3, B, stage, out);
+}
+void launch_node_forces(const Batch& B, const float* mass, const float* stage, const float* grav, int ngrav,
+                        float* out, hipStream_t s) {
+  NODE_LAUNCH(k_node_forces, B, mass, stage, grav, ngrav, out);
+}
+void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
+                          int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
+                          CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_node_explicit, B, mass, vel, stage, grav, ngrav, dt, cols, em, flags, vc, sc, st, partials);
+}
+void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
+                                 hipStream_t s) {
+  NODE_LAUNCH(k_collision_velocities, B, flags, cols, vc);
+}
+void launch_rhs(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stageVc,
+                const float* stageDiag, float dt, float tol, float* b, float* x, float* diag, CRScalars* sc,
+                BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_rhs, B, mass, flags, cols, stageVc, stageDiag, dt, tol, b, x, diag, sc, st, partials);
+}
+void launch_cr_setup1(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* z,
+                      float* p, const CRScalars* sc, hipStream_t s) {
+  NODE_LAUNCH(k_cr_setup1, B, mass, flags, cols, stage, x, b, diag, dt, z, p, sc);
+}
+void launch_cr_setup2(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                      const float* stage, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
+                      BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_cr_setup2, B, mass, flags, cols, stage, z, diag, dt, Ap, sc, st, partials);
+}
+void launch_cr_update_x(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* p, const float* Ap,
+                        const float* diag, float* x, float* z, CRScalars* sc, BatchStatus* st, double* partials,
+                        double* hist, hipStream_t s) {
+  NODE_LAUNCH(k_cr_update_x, B, flags, cols, p, Ap, diag, x, z, sc, st, partials, hist);
+}
+void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
+                  const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* st, double* partials,
+                  hipStream_t s) {
+  NODE_LAUNCH(k_cr_ar, B, mass, flags, cols, stage, z, dt, Ar, sc, st, partials);
+}
+void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
+                        CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_cr_update_p, B, z, Ar, diag, p, Ap, sc, st, partials);
+}
+void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s) {
+  NODE_LAUNCH(k_residual_vector, B, z, diag, r);
+}
+void launch_finalize(const Batch& B, const float* x, const float* vc, float* vel, const CRScalars* sc,
+                     hipStream_t s) {
+  NODE_LAUNCH(k_finalize, B, x, vc, vel, sc);
+}
+void launch_plasticity(PView P, const int* labels, int skipLabel, int n, float thc, float ths, int* err,
+                       hipStream_t s) {
+  if (n > 0) LAUNCH(k_plasticity, grid_for(n, 16384), P, labels, skipLabel, n, thc, ths, err);
+}
+void launch_ballistic(PView P, const int* labels, int n, const float* grav, int ngrav, float dt,
+                      const ColliderSet* cols, int mode, hipStream_t s) {
+  if (n > 0) LAUNCH(k_ballistic, grid_for(n, 16384), P, labels, n, grav, ngrav, dt, cols, mode);
+}
+void launch_advect(PView P, const int* labels, int n, float dt, const ColliderSet* cols, hipStream_t s) {
+  if (n > 0) LAUNCH(k_advect, grid_for(n, 16384), P, labels, n, dt, cols);
 }
 void launch_probe(PView P, const int* slots, int n, int mode, const float* dF, float* out, hipStream_t s) {
   if (n > 0) LAUNCH(k_probe, grid_for(n), P, slots, n, mode, dF, out);
 }
-void launch_check_pvel(PView P, int n, int* err, hipStream_t s) {
-  if (n > 0) LAUNCH(k_check_pvel, grid_for(n), P, n, err);
+void launch_is_ballistic(const int* labels, int n, int* flag, hipStream_t s) {
+  if (n > 0) LAUNCH(k_is_ballistic, grid_for(n, 16384), labels, n, flag);
+}
+void launch_ballistic_compact(const int* labels, const int* pos, int n, int* out, int offset, hipStream_t s) {
+  if (n > 0) LAUNCH(k_ballistic_compact, grid_for(n, 16384), labels, pos, n, out, offset);
 }
 
 }  // namespace mpm

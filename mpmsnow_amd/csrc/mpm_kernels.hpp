@@ -8,7 +8,7 @@
 
 namespace mpm {
 
-// particles
+// ---- particle store
 void launch_set_column(PView P, int col0, int ncomp, const float* aos, const int* slot_of, int n, hipStream_t s);
 void launch_get_column(PView P, int col0, int ncomp, float* aos, const int* slot_of, int n, hipStream_t s);
 void launch_fill_column(PView P, int col, float value, int n, hipStream_t s);
@@ -16,83 +16,107 @@ void launch_identity_columns(PView P, int col0, int n, hipStream_t s);
 void launch_permute(PView src, PView dst, const int* new_to_old, int n, const int* orig_of_old, int* orig_of_new,
                     int* slot_of, hipStream_t s);
 void launch_gather_slots(const int* slot_of, const int* orig, int n, int* slots, hipStream_t s);
+void launch_fill_int(int* p, int value, int64_t n, hipStream_t s);
 
-// reductions over a slot list: bbox (ordered uint), sum m v and sum m (double)
-void launch_body_reduce(PView P, const int* slots, int base, int n, unsigned int* bbox6, double* mom4, hipStream_t s);
+// ---- body partition (Sim::calculateBodies) on the GPU
+// cell hash with cell edge slightly above h; counts need (mask + 1) ints, zeroed
+void launch_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count, hipStream_t s);
+void launch_cc_scatter(const int* bucket, const int* rank, const int* offset, int n, int* sorted, hipStream_t s);
+void launch_cc_link(PView P, int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                    const int* sorted, int* parent, int* hasNb, hipStream_t s);
+// isRoot[s] = 1 for the root of a component of >= 2 particles; minOrig[root] = smallest caller index
+void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
+                     hipStream_t s);
+void launch_cc_compact(int n, const int* isRoot, const int* rootIdx, const int* minOrig, int* rootMin, hipStream_t s);
+// labels[s] = body of s (rank of its root) or -1 for ballistic particles
+void launch_cc_assign(int n, const int* parent, const int* hasNb, const int* rootIdx, const int* rootRank,
+                      int* labels, hipStream_t s);
+void launch_check_pvel(PView P, int n, int* err, hipStream_t s);
 
-// binning / counting sort
-void launch_bin(PView P, const int* slots, int base, int n, const GridView& g, int* key, int* rank, int* count, int* err,
-                hipStream_t s);
-void launch_scatter(const int* slots, int base, const int* key, const int* rank, const int* offset, int n, int* perm,
+// ---- per-body reductions: bbox (ordered uint, 6/body), sum m v and sum m (double, 4/body), count
+void launch_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
+                        int* count, hipStream_t s);
+
+// ---- binning / counting sort into the batch
+void launch_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody, hipStream_t s);
+void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
+                int* count, int* err, hipStream_t s);
+void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
                     hipStream_t s);
-void launch_make_chunks(const int* count, const int* offset, const int* chunk_off, int NT, int4* chunks, hipStream_t s);
 void launch_chunk_counts(const int* count, int NT, int* nch, hipStream_t s);
+void launch_make_chunks(const int* count, const int* offset, const int* chunk_off, const int* tileBody, int NT,
+                        int4* chunks, hipStream_t s);
 // exclusive scan of n ints; *total (device) receives the sum. tmp needs scan_tmp_ints(n) ints.
-int64_t scan_tmp_ints(int n);
+int64_t scan_tmp_ints(int64_t n);
 void scan_exclusive(const int* in, int* out, int n, int* total, int* tmp, hipStream_t s);
 
-// grid <-> reference dense order
-void launch_to_dense_f(const float* src, int ncomp, float* dst, const GridView& g, hipStream_t s);
-void launch_from_dense_f(const float* src, int ncomp, float* dst, const GridView& g, hipStream_t s);
-void launch_to_dense_i8(const int8_t* src, int8_t* dst, const GridView& g, hipStream_t s);
-void launch_from_dense_i8(const int8_t* src, int8_t* dst, const GridView& g, hipStream_t s);
+// ---- batch grid <-> reference dense order of one body
+void launch_to_dense_f(const float* src, int ncomp, float* dst, const BodyGrid& G, hipStream_t s);
+void launch_from_dense_f(const float* src, int ncomp, float* dst, const BodyGrid& G, hipStream_t s);
+void launch_to_dense_i8(const int8_t* src, int8_t* dst, const BodyGrid& G, hipStream_t s);
+void launch_from_dense_i8(const int8_t* src, int8_t* dst, const BodyGrid& G, hipStream_t s);
 
-// P2G and friends
-void launch_p2g_mass(PView P, const GridView& g, float* mass, int* err, hipStream_t s);
-void launch_mass_fix(float* mass, const GridView& g, int* err, hipStream_t s);
-void launch_p2g_vel(PView P, const GridView& g, const float* mass, float* vel, int* err, hipStream_t s);
-void launch_check_vel(const float* vel, float* prev, const GridView& g, int* err, hipStream_t s);
-void launch_volumes(PView P, const GridView& g, const float* mass, int* err, hipStream_t s);
-
-// forces (internal only, into acc) and node epilogues
-void launch_forces(PView P, const GridView& g, float* acc, int* err, hipStream_t s);
-void launch_node_forces(const GridView& g, const float* mass, const float* acc, const float* grav, int ngrav, float* out,
-                        hipStream_t s);
-void launch_node_explicit(const GridView& g, const float* mass, const float* vel, const float* acc, const float* grav, int ngrav,
-                          float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc, CRScalars* sc,
-                          double* partials, hipStream_t s);
-void launch_collision_velocities(const GridView& g, const int8_t* flags, const ColliderSet* cols, float* vc,
-                                 hipStream_t s);
-
-// HVP: out += -sum_p V dP(dF(d)) F^T grad w   (d projected first when project = true)
-void launch_hvp(PView P, const GridView& g, const float* d, bool project, const int8_t* flags, const ColliderSet* cols,
-                float* out, const CRScalars* sc, int* err, hipStream_t s);
-// diagonal of the force differential (dForceidXi), raw (before -dt^2 and +m)
-void launch_diag(PView P, const GridView& g, float* out, int* err, hipStream_t s);
-
-// solver node kernels
-void launch_rhs(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* accvc,
-                const float* draw, float dt, float tol, float* b, float* x, float* diag, CRScalars* sc,
-                double* partials, hipStream_t s);
-void launch_cr_setup1(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                      const float* acc, const float* x, const float* b, const float* diag, float dt, float* z, float* p,
-                      const CRScalars* sc, hipStream_t s);
-void launch_cr_setup2(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                      const float* acc, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
-                      double* partials, hipStream_t s);
-void launch_cr_update_x(const GridView& g, const int8_t* flags, const ColliderSet* cols, const float* p,
-                        const float* Ap, const float* diag, float* x, float* z, CRScalars* sc, double* partials,
-                        double* hist, int hist_cap, hipStream_t s);
-void launch_cr_ar(const GridView& g, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* acc,
-                  const float* z, float dt, float* Ar, CRScalars* sc, double* partials, hipStream_t s);
-void launch_cr_update_p(const GridView& g, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                        CRScalars* sc, double* partials, hipStream_t s);
-void launch_residual_vector(const GridView& g, const float* z, const float* diag, float* r, hipStream_t s);
-void launch_finalize(const GridView& g, const float* x, const float* vc, float* vel, const CRScalars* sc,
-                     hipStream_t s);
-
+// ---- particle-side kernels: write per-chunk staged footprints
+void launch_p2g_mass(PView P, const Batch& B, float* stage, int* err, hipStream_t s);
+void launch_p2g_vel(PView P, const Batch& B, const float* mass, float* stage, int* err, hipStream_t s);
+void launch_volumes(PView P, const Batch& B, const float* mass, int* err, hipStream_t s);
+void launch_forces(PView P, const Batch& B, float* stage, int* err, hipStream_t s);
+// HVP: staged -sum_p V dP(dF(d)) F^T grad w (d projected first when project = true); sc may be null
+void launch_hvp(PView P, const Batch& B, const float* d, bool project, const int8_t* flags, const ColliderSet* cols,
+                float* stage, const CRScalars* sc, int* err, hipStream_t s);
+// raw diagonal of the force differential (dForceidXi)
+void launch_diag(PView P, const Batch& B, float* stage, int* err, hipStream_t s);
 // G2P: mode 0 = velocities only, 1 = F only, 2 = full step (v, F, plasticity, advection),
 // 3 = full step without advection
-void launch_g2p(PView P, const GridView& g, const float* vel, const float* prev, int mode, float dt, float thc,
-                float ths, const ColliderSet* cols, int* err, hipStream_t s);
-void launch_plasticity(PView P, const int* slots, int base, int n, float thc, float ths, int* err, hipStream_t s);
-// ballistic particles: mode bit 1 = velocity update (Sim.cpp:64-73), bit 2 = advection
-void launch_ballistic(PView P, const int* slots, int n, const float* grav, int ngrav, float dt, const ColliderSet* cols,
-                      int mode, hipStream_t s);
-// advection of a slot list (Sim.cpp:123-131)
-void launch_advect(PView P, const int* slots, int n, float dt, const ColliderSet* cols, hipStream_t s);
+void launch_g2p(PView P, const Batch& B, const float* vel, const float* prev, int mode, float dt, float thc, float ths,
+                const ColliderSet* cols, int* err, hipStream_t s);
+
+// ---- node kernels (one block per 4 tiles; per-body reductions)
+void launch_gather_mass(const Batch& B, const float* stage, float* mass, int* err, hipStream_t s);
+void launch_gather_vel(const Batch& B, const float* stage, float* vel, float* prev, int* err, hipStream_t s);
+void launch_gather3(const Batch& B, const float* stage, float* out, hipStream_t s);
+void launch_node_forces(const Batch& B, const float* mass, const float* stage, const float* grav, int ngrav,
+                        float* out, hipStream_t s);
+void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
+                          int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
+                          CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
+                                 hipStream_t s);
+void launch_rhs(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stageVc,
+                const float* stageDiag, float dt, float tol, float* b, float* x, float* diag, CRScalars* sc,
+                BatchStatus* st, double* partials, hipStream_t s);
+void launch_cr_setup1(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* z,
+                      float* p, const CRScalars* sc, hipStream_t s);
+void launch_cr_setup2(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                      const float* stage, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
+                      BatchStatus* st, double* partials, hipStream_t s);
+void launch_cr_update_x(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* p, const float* Ap,
+                        const float* diag, float* x, float* z, CRScalars* sc, BatchStatus* st, double* partials,
+                        double* hist, hipStream_t s);
+void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
+                  const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* st, double* partials,
+                  hipStream_t s);
+void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
+                        CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s);
+void launch_finalize(const Batch& B, const float* x, const float* vc, float* vel, const CRScalars* sc,
+                     hipStream_t s);
+
+// ---- particle updates outside the grids
+// plasticity of every slot whose label is not `skipLabel` (labels null: all slots)
+void launch_plasticity(PView P, const int* labels, int skipLabel, int n, float thc, float ths, int* err,
+                       hipStream_t s);
+// ballistic particles (label -1): mode bit 1 = velocity update (Sim.cpp:64-73), bit 2 = advection
+void launch_ballistic(PView P, const int* labels, int n, const float* grav, int ngrav, float dt,
+                      const ColliderSet* cols, int mode, hipStream_t s);
+// advection (Sim.cpp:123-131) of every slot with label >= 0
+void launch_advect(PView P, const int* labels, int n, float dt, const ColliderSet* cols, hipStream_t s);
 // model probes: mode 0 energy (out n), 1 stress (out 9n), 2 stress differential of dF (9n in, 9n out)
 void launch_probe(PView P, const int* slots, int n, int mode, const float* dF, float* out, hipStream_t s);
-void launch_check_pvel(PView P, int n, int* err, hipStream_t s);
+// 1 where labels[s] == -1 (for the ballistic compaction)
+void launch_is_ballistic(const int* labels, int n, int* flag, hipStream_t s);
+// out[offset + pos[s]] = s for ballistic slots
+void launch_ballistic_compact(const int* labels, const int* pos, int n, int* out, int offset, hipStream_t s);
 
 }  // namespace mpm

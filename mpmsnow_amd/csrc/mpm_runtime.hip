@@ -1,21 +1,23 @@
 // mpm_runtime.hip -- host runtime behind include/mpmsnow_gpu.h.
 //
 // Owns the device particle store (double-buffered SoA columns kept in body-
-// and tile-sorted slot order), per-body tiled grids, and the orchestration of
+// and tile-sorted slot order) and a batch of per-body grids, and sequences
 // Sim::advance (davidsminor/mpmsnow src/MpmSim/Sim.cpp:57-134) and
-// Grid::updateGridVelocities (src/MpmSim/Grid.cpp:889-1016) as sequences of
-// gfx950 kernel launches on one HIP stream. The conjugate-residual loop runs
-// device-resident: each kernel skips itself once the solve is done, and the
-// host checks the 4-byte done flag once per batch of iterations.
+// Grid::updateGridVelocities (src/MpmSim/Grid.cpp:889-1016) as gfx950 kernel
+// launches on one HIP stream. All bodies of a step share one batch, so every
+// kernel launch serves every body; each body keeps its own conjugate-residual
+// scalars and termination, and the host only polls a done-counter once per
+// batch of iterations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
-#include <cstdint>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -54,7 +56,7 @@ struct DevBuf {
     if (bytes <= cap) return;
     if (p) HIPCHECK(hipFree(p));
     p = nullptr;
-    size_t c = std::max<size_t>(bytes, 256);
+    size_t c = std::max<size_t>(bytes + bytes / 4, 256);  // headroom: grids grow slowly step to step
     HIPCHECK(hipMalloc(&p, c));
     cap = c;
   }
@@ -84,11 +86,11 @@ struct VarInfo {
 };
 const std::unordered_map<std::string, VarInfo>& var_table() {
   static const std::unordered_map<std::string, VarInfo> t = {
-      {"p", {C_X, 3, false}},      {"v", {C_V, 3, false}},     {"F", {C_F, 9, false}},
-      {"m", {C_M, 1, false}},      {"volume", {C_VOL, 1, false}}, {"Fp", {C_FP, 9, true}},
-      {"FinvTrans", {C_FI, 9, true}}, {"R", {C_R, 9, true}},   {"Ginv", {C_GI, 9, true}},
-      {"J", {C_J, 1, true}},       {"mu", {C_MU, 1, true}},    {"lambda", {C_LAM, 1, true}},
-      {"mu0", {C_MU0, 1, true}},   {"lambda0", {C_LAM0, 1, true}}, {"xi", {C_XI, 1, true}}};
+      {"p", {C_X, 3, false}},         {"v", {C_V, 3, false}},       {"F", {C_F, 9, false}},
+      {"m", {C_M, 1, false}},         {"volume", {C_VOL, 1, false}}, {"Fp", {C_FP, 9, true}},
+      {"FinvTrans", {C_FI, 9, true}}, {"R", {C_R, 9, true}},         {"Ginv", {C_GI, 9, true}},
+      {"J", {C_J, 1, true}},          {"mu", {C_MU, 1, true}},       {"lambda", {C_LAM, 1, true}},
+      {"mu0", {C_MU0, 1, true}},      {"lambda0", {C_LAM0, 1, true}}, {"xi", {C_XI, 1, true}}};
   return t;
 }
 
@@ -99,22 +101,84 @@ float decode_ord(unsigned int u) {
   return f;
 }
 
-}  // namespace
+void raise_errors(int e) {
+  if (e & ERR_NAN_MASS) throw std::runtime_error("nans in splatted masses!");
+  if (e & ERR_NAN_VEL) throw std::runtime_error("nans in splatted velocities!");
+  if (e & ERR_NAN_STRESS) throw std::runtime_error("nans in dEnergyDensitydF matrix!");
+  if (e & ERR_NAN_MOMENTA) throw std::runtime_error("nan in explicit momenta!");
+  if (e & ERR_HARDNESS) throw std::runtime_error("infinite hardness!!!");
+  if (e & ERR_NAN_PVEL) throw std::runtime_error("nans in particle velocity data!");
+  if (e & ERR_STENCIL) throw std::runtime_error("particle outside its grid tile (internal binning error)");
+}
 
-// Kernel timing for measurement: HIP events recorded on the library's stream around
-// each instrumented launch, resolved at the next synchronisation point. Algorithmic
-// bytes per launch = bytes per particle * particles + bytes per node * nodes with m > 0.
+// Kernel timing for measurement: HIP events recorded on the library's stream around each
+// instrumented launch, resolved at the next synchronisation point. Algorithmic bytes of a
+// launch = bytes/particle x particles + bytes/node x nodes with m > 0, over the bodies the
+// launch worked on (bodies whose solve had finished skip the launch).
 enum ProfKind { PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_COUNT };
 const char* const kProfNames[PK_COUNT] = {"p2g_mass", "p2g_vel", "forces", "hvp", "diag",
                                           "g2p", "cr_update_x", "cr_ar", "cr_update_p"};
 const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0};
 const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84};
 
+}  // namespace
+
 struct SolveRecord {
   int iterations = 0;
   int early_exit = 0;
   double rhs_norm2 = 0, residual_norm2 = 0;
+  int np = 0, active = 0;
   std::vector<double> history;
+};
+
+struct mpm_context;
+
+// A batch of per-body grids with all their device state.
+struct GridBatch {
+  mpm_context* ctx = nullptr;
+  std::vector<BodyGrid> hb;
+  int nb = 0, NT = 0, nblocks = 0, nchunks = 0, np = 0;
+  int64_t NN = 0;
+  DevBuf bodies, tileBody, blockBody, perm, key, rank, count, offset, nch, chunkOff, chunks, scantmp, totals;
+  DevBuf stageA, stageB;
+  DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, hist, dense,
+      dense2;
+  DevBuf bbox, mom, cnt;
+  std::vector<SolveRecord> rec;
+
+  Batch view() const {
+    Batch B;
+    B.bodies = bodies.as<BodyGrid>();
+    B.nbodies = nb;
+    B.blockBody = blockBody.as<int>();
+    B.tileBody = tileBody.as<int>();
+    B.perm = perm.as<int>();
+    B.chunks = chunks.as<int4>();
+    B.nchunks = nchunks;
+    B.tileChunk0 = chunkOff.as<int>();
+    B.tileChunkN = nch.as<int>();
+    B.NT = NT;
+    B.nblocks = nblocks;
+    return B;
+  }
+  float* f(DevBuf& d) { return d.as<float>(); }
+  size_t vecBytes() const { return sizeof(float) * 3 * (size_t)NN; }
+  CRScalars* sc() { return scal.as<CRScalars>(); }
+  BatchStatus* st() { return status.as<BatchStatus>(); }
+
+  void build(const int* slots, const int* labels, int n, int nbodies, float h, const float* frameV, bool computeFrame);
+  void forces_stage();
+  void hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter);
+  void solve(float dt, const mpm_solver_params* prm);
+  void to_dense(int body, const float* batch, int ncomp, float* host);
+  void from_dense(int body, const float* host, int ncomp, float* batch);
+  void fetch_history(int body, SolveRecord& r);
+};
+
+struct mpm_grid {
+  mpm_context* ctx = nullptr;
+  GridBatch batch;
+  DevBuf slots;
 };
 
 struct mpm_context {
@@ -128,7 +192,7 @@ struct mpm_context {
   DevBuf store[2];
   int cur = 0;
   DevBuf orig_of[2], slot_of;
-  DevBuf cols_dev, grav_dev, errflag, staging, idx_tmp, slots_tmp;
+  DevBuf cols_dev, grav_dev, errflag, staging, idx_tmp;
   ColliderSet hcols{};
   std::vector<float> hgrav;
   PinnedBuf pinned;
@@ -136,17 +200,24 @@ struct mpm_context {
   bool simInit = false;
   float h = 0;
   int dim = 3;
-  std::vector<std::vector<int>> bodies;  // original indices, ascending
-  std::vector<int> ballistic;
+  int nbodies = 0;
+  int nballistic = 0;
+  DevBuf labels;  // body of each slot (-1 ballistic)
+  GridBatch simBatch;
   std::vector<SolveRecord> lastSolves;
-  std::vector<std::unique_ptr<mpm_grid>> gridPool;
-  DevBuf new_to_old, ball_slots, other_slots;
+  bool bodyListsValid = false;
+  std::vector<std::vector<int>> bodyLists;
+  std::vector<int> ballisticList;
+  // body-partition scratch
+  DevBuf ccBucket, ccRank, ccCount, ccOffset, ccSorted, ccParent, ccHasNb, ccIsRoot, ccRootIdx, ccMinOrig, ccRootMin,
+      ccRootRank, ccTmp, ccTot, newToOld, ballPos;
   int iterHint = 8;
   // measurement
   struct ProfRec {
     int kind;
+    int iter;  // CR loop iteration (-1: once per solve)
     hipEvent_t a, b;
-    const mpm_grid* grid;
+    const GridBatch* batch;
   };
   bool profOn = false;
   std::vector<ProfRec> profPending;
@@ -154,35 +225,6 @@ struct mpm_context {
   double profMs[PK_COUNT] = {};
   int64_t profCount[PK_COUNT] = {};
   double profBytes[PK_COUNT] = {};
-  hipEvent_t new_event() {
-    if (!eventPool.empty()) {
-      hipEvent_t e = eventPool.back();
-      eventPool.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    HIPCHECK(hipEventCreate(&e));
-    return e;
-  }
-  int prof_begin(int kind, const mpm_grid* g) {
-    if (!profOn) return -1;
-    ProfRec r{kind, new_event(), new_event(), g};
-    HIPCHECK(hipEventRecord(r.a, stream));
-    profPending.push_back(r);
-    return (int)profPending.size() - 1;
-  }
-  void prof_end(int idx) {
-    if (idx < 0) return;
-    HIPCHECK(hipEventRecord(profPending[idx].b, stream));
-  }
-  void prof_resolve();
-  ~mpm_context() {
-    for (auto& r : profPending) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
-    }
-    for (auto e : eventPool) (void)hipEventDestroy(e);
-  }
 
   PView view(int which = -1) const {
     int w = which < 0 ? cur : which;
@@ -193,24 +235,17 @@ struct mpm_context {
   }
   void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
   const ColliderSet* cols() const { return cols_dev.as<ColliderSet>(); }
+  const float* grav() const { return grav_dev.as<float>(); }
+  int ngrav() const { return (int)(hgrav.size() / 3); }
   int* errp() const { return errflag.as<int>(); }
   void clear_err() { HIPCHECK(hipMemsetAsync(errflag.p, 0, sizeof(int), stream)); }
-  void check_errors(int mask = ~0) {
+  void check_errors() {
     int e = 0;
     HIPCHECK(hipMemcpyAsync(&e, errflag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     sync();
-    raise(e & mask);
+    raise_errors(e);
   }
-  static void raise(int e) {
-    if (e & ERR_NAN_MASS) throw std::runtime_error("nans in splatted masses!");
-    if (e & ERR_NAN_VEL) throw std::runtime_error("nans in splatted velocities!");
-    if (e & ERR_NAN_STRESS) throw std::runtime_error("nans in dEnergyDensitydF matrix!");
-    if (e & ERR_NAN_MOMENTA) throw std::runtime_error("nan in explicit momenta!");
-    if (e & ERR_HARDNESS) throw std::runtime_error("infinite hardness!!!");
-    if (e & ERR_NAN_PVEL) throw std::runtime_error("nans in particle velocity data!");
-    if (e & ERR_STENCIL) throw std::runtime_error("particle outside its grid tile (internal binning error)");
-  }
-  // upload original indices and map them to slots (device int array of count entries)
+  // upload caller indices and map them to slots
   int* slots_from_orig(const int32_t* orig, int64_t count, DevBuf& out) {
     out.ensure(sizeof(int) * std::max<int64_t>(count, 1));
     idx_tmp.ensure(sizeof(int) * std::max<int64_t>(count, 1));
@@ -222,281 +257,359 @@ struct mpm_context {
     }
     return out.as<int>();
   }
+  hipEvent_t new_event() {
+    if (!eventPool.empty()) {
+      hipEvent_t e = eventPool.back();
+      eventPool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  int prof_begin(int kind, const GridBatch* g, int iter = -1) {
+    if (!profOn) return -1;
+    ProfRec r{kind, iter, new_event(), new_event(), g};
+    HIPCHECK(hipEventRecord(r.a, stream));
+    profPending.push_back(r);
+    return (int)profPending.size() - 1;
+  }
+  void prof_end(int idx) {
+    if (idx < 0) return;
+    HIPCHECK(hipEventRecord(profPending[idx].b, stream));
+  }
+  void prof_resolve() {
+    if (profPending.empty()) return;
+    sync();
+    for (auto& r : profPending) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        double bytes = 0;
+        if (r.batch) {
+          for (const SolveRecord& s : r.batch->rec) {
+            // a body takes part in CR-loop launch `iter` if its solve had not finished
+            bool active = r.iter < 0 || (r.kind == PK_CR_X ? r.iter < s.iterations : r.iter + 1 < s.iterations);
+            if (active) bytes += kProfBytesPerParticle[r.kind] * s.np + kProfBytesPerNode[r.kind] * s.active;
+          }
+        }
+        profMs[r.kind] += ms;
+        profCount[r.kind] += 1;
+        profBytes[r.kind] += bytes;
+      }
+      eventPool.push_back(r.a);
+      eventPool.push_back(r.b);
+    }
+    profPending.clear();
+  }
+  ~mpm_context() {
+    for (auto& r : profPending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : eventPool) (void)hipEventDestroy(e);
+  }
 };
 
-struct mpm_grid {
-  mpm_context* ctx = nullptr;
-  GridView g{};
-  int np = 0;
-  int activeNodes = 0;
-  float fv[3] = {0, 0, 0};
-  DevBuf slots, perm, key, rank, count, offset, nch, chunk_off, chunks, scantmp, totals;
-  DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, acc, acc2, diag, draw, sc, partials, hist, dense, dense2;
-  DevBuf bbox, mom;
-  int64_t NN = 0;
-
-  float* f(DevBuf& d) { return d.as<float>(); }
-  size_t vecBytes() const { return sizeof(float) * 3 * (size_t)NN; }
-  CRScalars* scal() { return sc.as<CRScalars>(); }
-
-  // Grid::Grid (Grid.cpp:156-263) on a device slot list of np particles
-  void build(const int* slotList, int npart, float h, const float* frameV, bool computeFrame) {
-    mpm_context& c = *ctx;
-    hipStream_t s = c.stream;
-    np = npart;
-    if (npart <= 0) throw InvalidArg("grid needs at least one particle");
-    // body reductions: bbox and comoving frame (Sim.cpp:84-93)
-    bbox.ensure(6 * sizeof(unsigned int));
-    mom.ensure(4 * sizeof(double));
-    unsigned int init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    HIPCHECK(hipMemcpyAsync(bbox.p, init, sizeof(init), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemsetAsync(mom.p, 0, 4 * sizeof(double), s));
-    launch_body_reduce(c.view(), slotList, 0, npart, bbox.as<unsigned int>(), mom.as<double>(), s);
-    struct {
-      unsigned int bb[6];
-      unsigned int pad[2];
-      double mm[4];
-    } hb;
-    HIPCHECK(hipMemcpyAsync(hb.bb, bbox.p, 6 * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(hb.mm, mom.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-    c.sync();
-    if (computeFrame) {
-      for (int a = 0; a < 3; ++a) fv[a] = (float)(hb.mm[a] / hb.mm[3]);
-    } else {
-      for (int a = 0; a < 3; ++a) fv[a] = frameV ? frameV[a] : 0.0f;
-    }
-    // grid geometry (Grid.cpp:193-213), float arithmetic as the reference
-    GridView G{};
-    G.h = h;
+// ------------------------------------------------------------------ GridBatch
+// Grid::Grid (Grid.cpp:156-263) for every body of the batch: bbox and frame per body,
+// geometry, binning into the batch's tiles, mass and velocity splat.
+void GridBatch::build(const int* slotList, const int* labelList, int n, int nbodies, float h, const float* frameV,
+                      bool computeFrame) {
+  mpm_context& c = *ctx;
+  hipStream_t s = c.stream;
+  nb = nbodies;
+  if (nb <= 0) throw InvalidArg("a grid needs at least one body");
+  if (n <= 0) throw InvalidArg("grid needs at least one particle");
+  // per-body reductions
+  bbox.ensure(sizeof(unsigned int) * 6 * nb);
+  mom.ensure(sizeof(double) * 4 * nb);
+  cnt.ensure(sizeof(int) * nb);
+  std::vector<unsigned int> binit(6 * nb);
+  for (int b = 0; b < nb; ++b)
     for (int a = 0; a < 3; ++a) {
-      float mn = decode_ord(hb.bb[a]), mx = decode_ord(hb.bb[3 + a]);
+      binit[6 * b + a] = 0xFFFFFFFFu;
+      binit[6 * b + 3 + a] = 0u;
+    }
+  HIPCHECK(hipMemcpyAsync(bbox.p, binit.data(), sizeof(unsigned int) * 6 * nb, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemsetAsync(mom.p, 0, sizeof(double) * 4 * nb, s));
+  HIPCHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * nb, s));
+  launch_body_reduce(c.view(), slotList, labelList, n, bbox.as<unsigned int>(), mom.as<double>(), cnt.as<int>(), s);
+  std::vector<unsigned int> hbox(6 * nb);
+  std::vector<double> hmom(4 * nb);
+  std::vector<int> hcnt(nb);
+  HIPCHECK(hipMemcpyAsync(hbox.data(), bbox.p, sizeof(unsigned int) * 6 * nb, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(hmom.data(), mom.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(hcnt.data(), cnt.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+  c.sync();
+  // geometry (Grid.cpp:193-213) in the reference's float arithmetic
+  hb.assign(nb, BodyGrid{});
+  int tileBase = 0;
+  np = 0;
+  for (int b = 0; b < nb; ++b) {
+    BodyGrid& G = hb[b];
+    G.h = h;
+    G.np = hcnt[b];
+    np += hcnt[b];
+    if (hcnt[b] <= 0) throw std::runtime_error("empty body (internal error)");
+    for (int a = 0; a < 3; ++a) {
+      float mn = decode_ord(hbox[6 * b + a]), mx = decode_ord(hbox[6 * b + 3 + a]);
       int cellMin = (int)std::floor(mn / h) - 2 - 1;
       int cellMax = (int)std::ceil(mx / h) + 2 + 1;
       G.gmin[a] = (float)cellMin * h;
       G.n[a] = cellMax - cellMin;
-      G.fv[a] = fv[a];
+      G.fv[a] = computeFrame ? (float)(hmom[4 * b + a] / hmom[4 * b + 3]) : (frameV ? frameV[3 * b + a] : 0.0f);
     }
     long long ncells = (long long)G.n[0] * G.n[1] * G.n[2];
     if (ncells > 4000000000LL || ncells <= 0) throw std::runtime_error("grid is too big");
     for (int a = 0; a < 3; ++a) G.T[a] = (G.n[a] + 3) / 4;
     long long nt = (long long)G.T[0] * G.T[1] * G.T[2];
-    if (nt * 64 * 3 > (long long)INT32_MAX * 2) throw std::runtime_error("grid is too big");
+    nt = (nt + 3) / 4 * 4;
+    if ((long long)tileBase + nt > INT32_MAX / 256) throw std::runtime_error("grid is too big");
     G.NT = (int)nt;
-    NN = nt * 64;
-    // binning + counting sort by tile
-    slots.ensure(sizeof(int) * npart);
-    HIPCHECK(hipMemcpyAsync(slots.p, slotList, sizeof(int) * npart, hipMemcpyDeviceToDevice, s));
-    key.ensure(sizeof(int) * npart);
-    rank.ensure(sizeof(int) * npart);
-    perm.ensure(sizeof(int) * npart);
-    count.ensure(sizeof(int) * G.NT);
-    offset.ensure(sizeof(int) * G.NT);
-    nch.ensure(sizeof(int) * G.NT);
-    chunk_off.ensure(sizeof(int) * G.NT);
-    scantmp.ensure(sizeof(int) * scan_tmp_ints(G.NT));
-    totals.ensure(sizeof(int) * 4);
-    HIPCHECK(hipMemsetAsync(count.p, 0, sizeof(int) * G.NT, s));
-    c.clear_err();
-    launch_bin(c.view(), slots.as<int>(), 0, npart, G, key.as<int>(), rank.as<int>(), count.as<int>(), c.errp(), s);
-    scan_exclusive(count.as<int>(), offset.as<int>(), G.NT, totals.as<int>(), scantmp.as<int>(), s);
-    launch_chunk_counts(count.as<int>(), G.NT, nch.as<int>(), s);
-    scan_exclusive(nch.as<int>(), chunk_off.as<int>(), G.NT, totals.as<int>() + 1, scantmp.as<int>(), s);
-    launch_scatter(slots.as<int>(), 0, key.as<int>(), rank.as<int>(), offset.as<int>(), npart, perm.as<int>(), s);
-    int tot[2] = {0, 0};
-    int e = 0;
-    HIPCHECK(hipMemcpyAsync(tot, totals.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    c.sync();
-    mpm_context::raise(e);
-    if (tot[0] != npart) throw std::runtime_error("particle binning lost particles");
-    chunks.ensure(sizeof(int4) * std::max(tot[1], 1));
-    launch_make_chunks(count.as<int>(), offset.as<int>(), chunk_off.as<int>(), G.NT, chunks.as<int4>(), s);
-    G.perm = perm.as<int>();
-    G.chunks = chunks.as<int4>();
-    G.nchunks = tot[1];
-    g = G;
-    // node arrays
-    mass.ensure(sizeof(float) * NN);
-    flags.ensure(NN);
-    for (DevBuf* d : {&vel, &prev, &vc, &b, &x, &z, &p, &Ap, &Ar, &acc, &acc2, &diag, &draw}) d->ensure(vecBytes());
-    sc.ensure(sizeof(CRScalars));
-    partials.ensure(sizeof(double) * 8 * 4096);
-    HIPCHECK(hipMemsetAsync(mass.p, 0, sizeof(float) * NN, s));
-    HIPCHECK(hipMemsetAsync(vel.p, 0, vecBytes(), s));
-    HIPCHECK(hipMemsetAsync(flags.p, 0xFF, NN, s));  // -1: no collision until updateGridVelocities
-    HIPCHECK(hipMemsetAsync(sc.p, 0, sizeof(CRScalars), s));
-    // MassSplatter, clamp/NaN, VelocitySplatter, NaN, prevVelocities
-    int pr = c.prof_begin(PK_P2G_MASS, this);
-    launch_p2g_mass(c.view(), g, f(mass), c.errp(), s);
-    c.prof_end(pr);
-    launch_mass_fix(f(mass), g, c.errp(), s);
-    pr = c.prof_begin(PK_P2G_VEL, this);
-    launch_p2g_vel(c.view(), g, f(mass), f(vel), c.errp(), s);
-    c.prof_end(pr);
-    launch_check_vel(f(vel), f(prev), g, c.errp(), s);
+    G.tileBase = tileBase;
+    G.blockBase = tileBase / 4;
+    G.nblocks = G.NT / 4;
+    tileBase += G.NT;
   }
+  NT = tileBase;
+  nblocks = NT / 4;
+  NN = (int64_t)NT * 64;
+  bodies.ensure(sizeof(BodyGrid) * nb);
+  HIPCHECK(hipMemcpyAsync(bodies.p, hb.data(), sizeof(BodyGrid) * nb, hipMemcpyHostToDevice, s));
+  tileBody.ensure(sizeof(int) * NT);
+  blockBody.ensure(sizeof(int) * nblocks);
+  launch_tile_body(bodies.as<BodyGrid>(), nb, NT, tileBody.as<int>(), blockBody.as<int>(), s);
+  // binning + counting sort by global tile
+  key.ensure(sizeof(int) * n);
+  rank.ensure(sizeof(int) * n);
+  perm.ensure(sizeof(int) * np);
+  count.ensure(sizeof(int) * NT);
+  offset.ensure(sizeof(int) * NT);
+  nch.ensure(sizeof(int) * NT);
+  chunkOff.ensure(sizeof(int) * NT);
+  scantmp.ensure(sizeof(int) * scan_tmp_ints(NT));
+  totals.ensure(sizeof(int) * 4);
+  HIPCHECK(hipMemsetAsync(count.p, 0, sizeof(int) * NT, s));
+  c.clear_err();
+  launch_bin(c.view(), slotList, labelList, n, bodies.as<BodyGrid>(), key.as<int>(), rank.as<int>(), count.as<int>(),
+             c.errp(), s);
+  scan_exclusive(count.as<int>(), offset.as<int>(), NT, totals.as<int>(), scantmp.as<int>(), s);
+  launch_chunk_counts(count.as<int>(), NT, nch.as<int>(), s);
+  scan_exclusive(nch.as<int>(), chunkOff.as<int>(), NT, totals.as<int>() + 1, scantmp.as<int>(), s);
+  launch_scatter(slotList, key.as<int>(), rank.as<int>(), offset.as<int>(), n, perm.as<int>(), s);
+  int tot[2] = {0, 0};
+  int e = 0;
+  HIPCHECK(hipMemcpyAsync(tot, totals.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  c.sync();
+  raise_errors(e);
+  if (tot[0] != np) throw std::runtime_error("particle binning lost particles (internal error)");
+  nchunks = tot[1];
+  chunks.ensure(sizeof(int4) * std::max(nchunks, 1));
+  launch_make_chunks(count.as<int>(), offset.as<int>(), chunkOff.as<int>(), tileBody.as<int>(), NT, chunks.as<int4>(),
+                     s);
+  // node arrays and staging
+  stageA.ensure(sizeof(float) * (size_t)kStage * std::max(nchunks, 1));
+  stageB.ensure(sizeof(float) * (size_t)kStage * std::max(nchunks, 1));
+  mass.ensure(sizeof(float) * NN);
+  flags.ensure(NN);
+  flags2.ensure(NN);
+  for (DevBuf* d : {&vel, &prev, &vc, &b, &x, &z, &p, &Ap, &Ar, &diag, &tmp3}) d->ensure(vecBytes());
+  scal.ensure(sizeof(CRScalars) * nb);
+  status.ensure(sizeof(BatchStatus));
+  partials.ensure(sizeof(double) * 8 * std::max(nblocks, 1));
+  HIPCHECK(hipMemsetAsync(flags.p, 0xFF, NN, s));  // -1: no collision until updateGridVelocities
+  rec.assign(nb, SolveRecord{});
+  for (int bi = 0; bi < nb; ++bi) rec[bi].np = hb[bi].np;
+  Batch B = view();
+  int pr = c.prof_begin(PK_P2G_MASS, this);
+  launch_p2g_mass(c.view(), B, f(stageA), c.errp(), s);
+  c.prof_end(pr);
+  launch_gather_mass(B, f(stageA), f(mass), c.errp(), s);
+  pr = c.prof_begin(PK_P2G_VEL, this);
+  launch_p2g_vel(c.view(), B, f(mass), f(stageA), c.errp(), s);
+  c.prof_end(pr);
+  launch_gather_vel(B, f(stageA), f(vel), f(prev), c.errp(), s);
+}
 
-  void to_dense(const float* tiled, int ncomp, float* host) {
-    int64_t nd = (int64_t)g.n[0] * g.n[1] * g.n[2];
-    dense.ensure(sizeof(float) * ncomp * nd);
-    launch_to_dense_f(tiled, ncomp, dense.as<float>(), g, ctx->stream);
-    HIPCHECK(hipMemcpyAsync(host, dense.p, sizeof(float) * ncomp * nd, hipMemcpyDeviceToHost, ctx->stream));
+void GridBatch::to_dense(int body, const float* batch, int ncomp, float* host) {
+  const BodyGrid& G = hb.at(body);
+  int64_t nd = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  dense.ensure(sizeof(float) * ncomp * nd);
+  launch_to_dense_f(batch, ncomp, dense.as<float>(), G, ctx->stream);
+  HIPCHECK(hipMemcpyAsync(host, dense.p, sizeof(float) * ncomp * nd, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+}
+void GridBatch::from_dense(int body, const float* host, int ncomp, float* batch) {
+  const BodyGrid& G = hb.at(body);
+  int64_t nd = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  dense.ensure(sizeof(float) * ncomp * nd);
+  HIPCHECK(hipMemcpyAsync(dense.p, host, sizeof(float) * ncomp * nd, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(hipMemsetAsync(batch, 0, ncomp == 3 ? vecBytes() : sizeof(float) * NN, ctx->stream));
+  launch_from_dense_f(dense.as<float>(), ncomp, batch, G, ctx->stream);
+}
+
+void GridBatch::forces_stage() {
+  int pr = ctx->prof_begin(PK_FORCES, this);
+  launch_forces(ctx->view(), view(), f(stageA), ctx->errp(), ctx->stream);
+  ctx->prof_end(pr);
+}
+void GridBatch::hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter) {
+  int pr = ctx->prof_begin(PK_HVP, this, iter);
+  launch_hvp(ctx->view(), view(), d, project, flags.as<int8_t>(), ctx->cols(), stage, scp, ctx->errp(), ctx->stream);
+  ctx->prof_end(pr);
+}
+
+void GridBatch::fetch_history(int body, SolveRecord& r) {
+  int L = std::min(r.iterations, kHistCap);
+  r.history.resize(L);
+  if (L > 0) {
+    HIPCHECK(hipMemcpyAsync(r.history.data(), hist.as<double>() + (int64_t)body * kHistCap, sizeof(double) * L,
+                            hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
   }
-  void from_dense(const float* host, int ncomp, float* tiled) {
-    int64_t nd = (int64_t)g.n[0] * g.n[1] * g.n[2];
-    dense.ensure(sizeof(float) * ncomp * nd);
-    HIPCHECK(hipMemcpyAsync(dense.p, host, sizeof(float) * ncomp * nd, hipMemcpyHostToDevice, ctx->stream));
-    if (ncomp == 3) HIPCHECK(hipMemsetAsync(tiled, 0, vecBytes(), ctx->stream));
-    launch_from_dense_f(dense.as<float>(), ncomp, tiled, g, ctx->stream);
-  }
+}
 
-  const float* grav() const { return ctx->grav_dev.as<float>(); }
-  int ngrav() const { return (int)(ctx->hgrav.size() / 3); }
-
-  // internal forces into acc (zeroed first)
-  void forces_into_acc() {
-    HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), ctx->stream));
-    int pr = ctx->prof_begin(PK_FORCES, this);
-    launch_forces(ctx->view(), g, f(acc), ctx->errp(), ctx->stream);
-    ctx->prof_end(pr);
+// Grid::updateGridVelocities (Grid.cpp:889-1016) for every body of the batch
+void GridBatch::solve(float dt, const mpm_solver_params* prm) {
+  mpm_context& c = *ctx;
+  hipStream_t s = c.stream;
+  const bool hooks = prm && (prm->termination || prm->debug);
+  int maxIters = prm ? prm->max_iters : 0;
+  float tol = prm ? prm->tolerance : 0.0f;
+  if (prm && prm->termination) {
+    maxIters = INT32_MAX;
+    tol = 0.0f;
   }
-  void hvp(const float* d, bool project, float* out, const CRScalars* scp) {
-    int pr = ctx->prof_begin(PK_HVP, this);
-    launch_hvp(ctx->view(), g, d, project, flags.as<int8_t>(), ctx->cols(), out, scp, ctx->errp(), ctx->stream);
-    ctx->prof_end(pr);
+  Batch B = view();
+  // per-body scalars
+  size_t scb = sizeof(CRScalars) * nb;
+  c.pinned.ensure(scb + 4096);
+  CRScalars* hsc = c.pinned.as<CRScalars>();
+  std::memset(hsc, 0, scb);
+  for (int bi = 0; bi < nb; ++bi) hsc[bi].max_iters = maxIters;
+  HIPCHECK(hipMemcpyAsync(scal.p, hsc, scb, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemsetAsync(status.p, 0, sizeof(BatchStatus), s));
+  hist.ensure(sizeof(double) * kHistCap * nb);
+  c.clear_err();
+  // explicit momenta, flags, vc, guess sums
+  forces_stage();
+  launch_node_explicit(B, f(mass), f(vel), f(stageA), c.grav(), c.ngrav(), dt, c.cols(), f(b), flags.as<int8_t>(),
+                       f(vc), sc(), st(), partials.as<double>(), s);
+  // DF(vc) (no input projection) and the diagonal, then rhs / guess / diagonal finish
+  hvp(f(vc), false, f(stageA), nullptr, -1);
+  int pr = c.prof_begin(PK_DIAG, this);
+  launch_diag(c.view(), B, f(stageB), c.errp(), s);
+  c.prof_end(pr);
+  launch_rhs(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(stageB), dt, tol, f(b), f(x), f(diag), sc(), st(),
+             partials.as<double>(), s);
+  // CR setup (ConjugateResiduals.cpp:50-76)
+  hvp(f(x), true, f(stageA), sc(), -1);
+  launch_cr_setup1(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(), s);
+  hvp(f(z), true, f(stageA), sc(), -1);
+  launch_cr_setup2(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
+                   partials.as<double>(), s);
+  BatchStatus* hst = reinterpret_cast<BatchStatus*>(reinterpret_cast<char*>(c.pinned.p) + scb);
+  auto fetch_status = [&]() {
+    HIPCHECK(hipMemcpyAsync(hst, status.p, sizeof(BatchStatus), hipMemcpyDeviceToHost, s));
+    c.sync();
+  };
+  fetch_status();
+  {
+    int e = 0;
+    HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    c.sync();
+    raise_errors(e);
   }
-
-  // Grid::updateGridVelocities (Grid.cpp:889-1016)
-  void updateGridVelocities(float dt, const mpm_solver_params* prm, SolveRecord& rec) {
-    mpm_context& c = *ctx;
-    hipStream_t s = c.stream;
-    const bool hooks = prm && (prm->termination || prm->debug);
-    int maxIters = prm ? prm->max_iters : 0;
-    float tol = prm ? prm->tolerance : 0.0f;
-    if (prm && prm->termination) {
-      maxIters = INT32_MAX;
-      tol = 0.0f;
+  // one CR iteration (ConjugateResiduals.cpp:84-151) for every body still iterating
+  auto iterate_x = [&](int it) {
+    int px = c.prof_begin(PK_CR_X, this, it);
+    launch_cr_update_x(B, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
+                       partials.as<double>(), hist.as<double>(), s);
+    c.prof_end(px);
+  };
+  auto iterate_rest = [&](int it) {
+    hvp(f(z), true, f(stageA), sc(), it);
+    int pa = c.prof_begin(PK_CR_AR, this, it);
+    launch_cr_ar(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), sc(), st(),
+                 partials.as<double>(), s);
+    c.prof_end(pa);
+    int pp = c.prof_begin(PK_CR_P, this, it);
+    launch_cr_update_p(B, f(z), f(Ar), f(diag), f(p), f(Ap), sc(), st(), partials.as<double>(), s);
+    c.prof_end(pp);
+  };
+  auto set_done_flag = [&](int bi, int v) {
+    hsc[bi].done = v;
+    HIPCHECK(hipMemcpyAsync(&sc()[bi].done, &hsc[bi].done, sizeof(int), hipMemcpyHostToDevice, s));
+    c.sync();
+  };
+  const int64_t iterCap = (prm && prm->termination) ? 1000000 : (int64_t)maxIters + 2;
+  if (!hooks) {
+    // all bodies iterate in lockstep; the host polls the done counter once per batch of iterations
+    int it = 0;
+    while (hst->bodies_done < nb) {
+      if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
+      int batch = it == 0 ? std::max(1, c.iterHint) : 8;
+      for (int k = 0; k < batch; ++k, ++it) {
+        iterate_x(it);
+        iterate_rest(it);
+      }
+      fetch_status();
     }
-    // scalars: zero everything, set max_iters
-    CRScalars init{};
-    init.max_iters = maxIters;
-    c.pinned.ensure(4096);
-    std::memcpy(c.pinned.p, &init, sizeof(init));
-    HIPCHECK(hipMemcpyAsync(sc.p, c.pinned.p, sizeof(init), hipMemcpyHostToDevice, s));
+  } else {
+    // host hooks see the bodies one after another, as the reference solves them: bodies still
+    // iterating are parked (done = 2, which every kernel skips) and released one at a time
+    HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
     c.sync();
-    hist.ensure(sizeof(double) * 4096);
-    c.clear_err();
-    // explicit momenta, flags, vc, guess sums
-    forces_into_acc();
-    launch_node_explicit(g, f(mass), f(vel), f(acc), grav(), ngrav(), dt, c.cols(), f(b), flags.as<int8_t>(), f(vc),
-                         scal(), partials.as<double>(), s);
-    // DF(vc) (no input projection) and the diagonal
-    HIPCHECK(hipMemsetAsync(acc2.p, 0, vecBytes(), s));
-    hvp(f(vc), false, f(acc2), nullptr);
-    HIPCHECK(hipMemsetAsync(draw.p, 0, vecBytes(), s));
-    int pr = c.prof_begin(PK_DIAG, this);
-    launch_diag(c.view(), g, f(draw), c.errp(), s);
-    c.prof_end(pr);
-    launch_rhs(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc2), f(draw), dt, tol, f(b), f(x), f(diag), scal(),
-               partials.as<double>(), s);
-    // CR setup: A x, A z (ConjugateResiduals.cpp:50-76)
-    HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), s));
-    hvp(f(x), true, f(acc), scal());
-    launch_cr_setup1(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc), f(x), f(b), f(diag), dt, f(z), f(p), scal(), s);
-    HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), s));
-    hvp(f(z), true, f(acc), scal());
-    launch_cr_setup2(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc), f(z), f(diag), dt, f(Ap), scal(),
-                     partials.as<double>(), s);
-    CRScalars* hsc = c.pinned.as<CRScalars>();
-    auto fetch = [&]() {
-      HIPCHECK(hipMemcpyAsync(hsc, sc.p, sizeof(CRScalars), hipMemcpyDeviceToHost, s));
-      c.sync();
-    };
-    fetch();
-    activeNodes = hsc->active_nodes;
-    mpm_context::raise(hsc->errors);
-    int e0 = 0;
-    HIPCHECK(hipMemcpyAsync(&e0, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    c.sync();
-    mpm_context::raise(e0);
+    std::vector<int> pending;
+    for (int bi = 0; bi < nb; ++bi)
+      if (!hsc[bi].done) {
+        pending.push_back(bi);
+        set_done_flag(bi, 2);
+      }
     std::vector<float> hostvec;
-    int enqueued = 0;
-    const int histCap = 4096;
-    const int64_t iterCap = (prm && prm->termination) ? 1000000 : (int64_t)maxIters + 2;
-    while (!hsc->done) {
-      if (enqueued > iterCap) throw std::runtime_error("conjugate residual did not terminate");
-      int batch = hooks ? 1 : (enqueued == 0 ? std::max(1, c.iterHint) : 8);
-      for (int k = 0; k < batch; ++k) {
-        int px = c.prof_begin(PK_CR_X, this);
-        launch_cr_update_x(g, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), scal(),
-                           partials.as<double>(), hist.as<double>(), histCap, s);
-        c.prof_end(px);
-        if (hooks) {
-          fetch();
-          int it = hsc->iterations - 1;
+    for (int bi : pending) {
+      set_done_flag(bi, 0);
+      const BodyGrid& G = hb[bi];
+      const int64_t nd = 3LL * G.n[0] * G.n[1] * G.n[2];
+      hostvec.resize(nd);
+      for (int it = 0;; ++it) {
+        if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
+        iterate_x(it);
+        HIPCHECK(hipMemcpyAsync(&hsc[bi], &sc()[bi], sizeof(CRScalars), hipMemcpyDeviceToHost, s));
+        c.sync();
+        if (hsc[bi].iterations == it + 1) {  // this x update happened
           if (prm->debug) {
-            hostvec.resize((size_t)3 * g.n[0] * g.n[1] * g.n[2]);
-            to_dense(f(x), 3, hostvec.data());
-            prm->debug(prm->user, hostvec.data(), (int64_t)hostvec.size());
+            to_dense(bi, f(x), 3, hostvec.data());
+            prm->debug(prm->user, hostvec.data(), nd);
           }
-          if (prm->termination && !hsc->done) {
-            launch_residual_vector(g, f(z), f(diag), f(acc2), s);
-            hostvec.resize((size_t)3 * g.n[0] * g.n[1] * g.n[2]);
-            to_dense(f(acc2), 3, hostvec.data());
-            if (prm->termination(prm->user, hostvec.data(), (int64_t)hostvec.size(), it)) {
-              int one = 1;
-              HIPCHECK(hipMemcpyAsync(&scal()->done, &one, sizeof(int), hipMemcpyHostToDevice, s));
-              c.sync();
-            }
+          if (prm->termination && !hsc[bi].done) {
+            launch_residual_vector(B, f(z), f(diag), f(tmp3), s);
+            to_dense(bi, f(tmp3), 3, hostvec.data());
+            if (prm->termination(prm->user, hostvec.data(), nd, it)) set_done_flag(bi, 1);
           }
         }
-        HIPCHECK(hipMemsetAsync(acc.p, 0, vecBytes(), s));
-        hvp(f(z), true, f(acc), scal());
-        int pa = c.prof_begin(PK_CR_AR, this);
-        launch_cr_ar(g, f(mass), flags.as<int8_t>(), c.cols(), f(acc), f(z), dt, f(Ar), scal(), partials.as<double>(),
-                     s);
-        c.prof_end(pa);
-        int pp = c.prof_begin(PK_CR_P, this);
-        launch_cr_update_p(g, f(z), f(Ar), f(diag), f(p), f(Ap), scal(), partials.as<double>(), s);
-        c.prof_end(pp);
+        if (hsc[bi].done) break;
+        iterate_rest(it);
       }
-      enqueued += batch;
-      fetch();
     }
-    launch_finalize(g, f(x), f(vc), f(vel), scal(), s);
-    fetch();
-    rec.iterations = hsc->iterations;
-    rec.early_exit = hsc->early_exit;
-    rec.rhs_norm2 = hsc->bnorm2;
-    rec.residual_norm2 = hsc->rnorm2;
-    int L = std::min(hsc->iterations, histCap);
-    rec.history.resize(L);
-    if (L > 0) {
-      HIPCHECK(hipMemcpyAsync(rec.history.data(), hist.p, sizeof(double) * L, hipMemcpyDeviceToHost, s));
-      c.sync();
-    }
-    if (rec.iterations > 0) c.iterHint = rec.iterations;
   }
-};
-
-void mpm_context::prof_resolve() {
-  if (profPending.empty()) return;
-  sync();
-  for (auto& r : profPending) {
-    float ms = 0.0f;
-    hipError_t e = hipEventElapsedTime(&ms, r.a, r.b);
-    // launches skipped after the solve finished still count: they are part of the solve's cost
-    if (e == hipSuccess) {
-      profMs[r.kind] += ms;
-      profCount[r.kind] += 1;
-      profBytes[r.kind] += kProfBytesPerParticle[r.kind] * (r.grid ? r.grid->np : 0) +
-                           kProfBytesPerNode[r.kind] * (r.grid ? r.grid->activeNodes : 0);
-    }
-    eventPool.push_back(r.a);
-    eventPool.push_back(r.b);
+  launch_finalize(B, f(x), f(vc), f(vel), sc(), s);
+  HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
+  c.sync();
+  int maxIt = 0;
+  for (int bi = 0; bi < nb; ++bi) {
+    SolveRecord& r = rec[bi];
+    r.iterations = hsc[bi].iterations;
+    r.early_exit = hsc[bi].early_exit;
+    r.rhs_norm2 = hsc[bi].bnorm2;
+    r.residual_norm2 = hsc[bi].rnorm2;
+    r.active = hsc[bi].active_nodes;
+    r.history.clear();
+    maxIt = std::max(maxIt, r.iterations);
+    raise_errors(hsc[bi].errors);
   }
-  profPending.clear();
+  if (maxIt > 0) c.iterHint = maxIt;
 }
 
 namespace {
@@ -524,101 +637,113 @@ void require_particles(mpm_context* c) {
   if (c->n <= 0) throw InvalidArg("no particles (call mpm_particles_create first)");
 }
 
-// Sim::calculateBodies (Sim.cpp:136-203): connected components of the relation
-// |x_i - x_j|^2 < h^2 (float arithmetic of KdTree.inl:187-190); isolated particles are
-// ballistic; bodies are ordered by their smallest particle index.
+// Sim::calculateBodies (Sim.cpp:136-203) on the GPU: connected components of the relation
+// |x_i - x_j|^2 < h^2 (float arithmetic of KdTree.inl:187-190) by cell hashing and
+// union-find; isolated particles are ballistic; bodies are ordered by their smallest
+// particle index, as the reference's flood fill numbers them.
 void calculate_bodies(mpm_context& c) {
-  int n = (int)c.n;
-  std::vector<float> x((size_t)3 * n);
+  const int n = (int)c.n;
+  hipStream_t s = c.stream;
+  c.clear_err();
+  launch_check_pvel(c.view(), n, c.errp(), s);
+  uint32_t tsize = 1;
+  while (tsize < 2u * (uint32_t)n) tsize <<= 1;
+  const uint32_t mask = tsize - 1;
+  // any pair closer than h lies in adjacent cells: the 1% margin covers the rounding of
+  // x / cell for |x| / h up to ~1e5
+  const float cell = c.h * 1.01f;
+  c.ccBucket.ensure(sizeof(int) * n);
+  c.ccRank.ensure(sizeof(int) * n);
+  c.ccCount.ensure(sizeof(int) * tsize);
+  c.ccOffset.ensure(sizeof(int) * tsize);
+  c.ccSorted.ensure(sizeof(int) * n);
+  c.ccParent.ensure(sizeof(int) * n);
+  c.ccHasNb.ensure(sizeof(int) * n);
+  c.ccIsRoot.ensure(sizeof(int) * n);
+  c.ccRootIdx.ensure(sizeof(int) * n);
+  c.ccMinOrig.ensure(sizeof(int) * n);
+  c.ccTmp.ensure(sizeof(int) * scan_tmp_ints(std::max<int64_t>(tsize, n)));
+  c.ccTot.ensure(sizeof(int) * 4);
+  c.labels.ensure(sizeof(int) * n);
+  HIPCHECK(hipMemsetAsync(c.ccCount.p, 0, sizeof(int) * tsize, s));
+  launch_cc_hash(c.view(), n, cell, mask, c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccCount.as<int>(), s);
+  scan_exclusive(c.ccCount.as<int>(), c.ccOffset.as<int>(), (int)tsize, c.ccTot.as<int>(), c.ccTmp.as<int>(), s);
+  launch_cc_scatter(c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccOffset.as<int>(), n, c.ccSorted.as<int>(), s);
+  // parent = identity
   {
-    DevBuf tmp;
-    tmp.ensure(sizeof(float) * 3 * n);
-    // slot_of maps original -> slot; get_column gathers in original order
-    launch_get_column(c.view(), C_X, 3, tmp.as<float>(), c.slot_of.as<int>(), n, c.stream);
-    HIPCHECK(hipMemcpyAsync(x.data(), tmp.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c.stream));
-    c.clear_err();
-    launch_check_pvel(c.view(), n, c.errp(), c.stream);
-    c.check_errors();
+    std::vector<int> iota(n);
+    std::iota(iota.begin(), iota.end(), 0);
+    HIPCHECK(hipMemcpyAsync(c.ccParent.p, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    c.sync();
   }
-  const float h = c.h;
-  const float r2 = h * h;
-  // hash grid with cell size h
-  struct Key {
-    int64_t k;
-  };
-  auto cellOf = [&](int i, int a) { return (int64_t)std::floor(x[3 * i + a] / h); };
-  auto pack = [](int64_t i, int64_t j, int64_t k) {
-    return (uint64_t)((i & 0x1FFFFF) | ((j & 0x1FFFFF) << 21) | ((k & 0x1FFFFF) << 42));
-  };
-  std::vector<std::pair<uint64_t, int>> keys(n);
-  for (int i = 0; i < n; ++i) keys[i] = {pack(cellOf(i, 0), cellOf(i, 1), cellOf(i, 2)), i};
-  std::sort(keys.begin(), keys.end());
-  std::unordered_map<uint64_t, std::pair<int, int>> cells;
-  cells.reserve(keys.size() * 2);
-  for (int a = 0; a < n;) {
-    int b = a;
-    while (b < n && keys[b].first == keys[a].first) ++b;
-    cells[keys[a].first] = {a, b};
-    a = b;
+  launch_cc_link(c.view(), n, c.h, cell, mask, c.ccOffset.as<int>(), c.ccCount.as<int>(), c.ccSorted.as<int>(),
+                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), s);
+  HIPCHECK(hipMemsetAsync(c.ccIsRoot.p, 0, sizeof(int) * n, s));
+  launch_fill_int(c.ccMinOrig.as<int>(), INT_MAX, n, s);
+  launch_cc_roots(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.orig_of[c.cur].as<int>(), c.ccIsRoot.as<int>(),
+                  c.ccMinOrig.as<int>(), s);
+  scan_exclusive(c.ccIsRoot.as<int>(), c.ccRootIdx.as<int>(), n, c.ccTot.as<int>() + 1, c.ccTmp.as<int>(), s);
+  int nroots = 0;
+  HIPCHECK(hipMemcpyAsync(&nroots, c.ccTot.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  c.check_errors();
+  c.ccRootMin.ensure(sizeof(int) * std::max(nroots, 1));
+  c.ccRootRank.ensure(sizeof(int) * std::max(nroots, 1));
+  std::vector<int> rootMin(nroots), rank(nroots);
+  if (nroots > 0) {
+    launch_cc_compact(n, c.ccIsRoot.as<int>(), c.ccRootIdx.as<int>(), c.ccMinOrig.as<int>(), c.ccRootMin.as<int>(), s);
+    HIPCHECK(hipMemcpyAsync(rootMin.data(), c.ccRootMin.p, sizeof(int) * nroots, hipMemcpyDeviceToHost, s));
+    c.sync();
+    std::vector<int> order(nroots);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return rootMin[a] < rootMin[b]; });
+    for (int r = 0; r < nroots; ++r) rank[order[r]] = r;
+    HIPCHECK(hipMemcpyAsync(c.ccRootRank.p, rank.data(), sizeof(int) * nroots, hipMemcpyHostToDevice, s));
   }
-  std::vector<int> parent(n);
-  for (int i = 0; i < n; ++i) parent[i] = i;
-  auto find = [&](int i) {
-    while (parent[i] != i) {
-      parent[i] = parent[parent[i]];
-      i = parent[i];
-    }
-    return i;
-  };
-  std::vector<char> hasNb(n, 0);
-  for (int i = 0; i < n; ++i) {
-    int64_t ci = cellOf(i, 0), cj = cellOf(i, 1), ck = cellOf(i, 2);
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          auto it = cells.find(pack(ci + dx, cj + dy, ck + dz));
-          if (it == cells.end()) continue;
-          for (int q = it->second.first; q < it->second.second; ++q) {
-            int j = keys[q].second;
-            if (j <= i) continue;
-            float d0 = x[3 * i] - x[3 * j], d1 = x[3 * i + 1] - x[3 * j + 1], d2 = x[3 * i + 2] - x[3 * j + 2];
-            float d = d0 * d0 + d1 * d1 + d2 * d2;
-            if (d < r2) {
-              hasNb[i] = hasNb[j] = 1;
-              int a = find(i), b = find(j);
-              if (a != b) {
-                if (a < b) parent[b] = a;
-                else parent[a] = b;
-              }
-            }
-          }
-        }
-  }
-  c.bodies.clear();
-  c.ballistic.clear();
-  std::unordered_map<int, int> rootBody;
-  for (int i = 0; i < n; ++i) {
-    if (!hasNb[i]) {
-      c.ballistic.push_back(i);
-      continue;
-    }
-    int r = find(i);
-    auto it = rootBody.find(r);
-    if (it == rootBody.end()) {
-      rootBody[r] = (int)c.bodies.size();
-      c.bodies.push_back({i});
-    } else {
-      c.bodies[it->second].push_back(i);
-    }
-  }
+  launch_cc_assign(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccRootIdx.as<int>(), c.ccRootRank.as<int>(),
+                   c.labels.as<int>(), s);
+  c.nbodies = nroots;
+  c.bodyListsValid = false;
+  // ballistic count for statistics
+  c.ballPos.ensure(sizeof(int) * n);
+  launch_is_ballistic(c.labels.as<int>(), n, c.ccIsRoot.as<int>(), s);
+  scan_exclusive(c.ccIsRoot.as<int>(), c.ballPos.as<int>(), n, c.ccTot.as<int>() + 2, c.ccTmp.as<int>(), s);
+  HIPCHECK(hipMemcpyAsync(&c.nballistic, c.ccTot.as<int>() + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+  c.sync();
 }
 
-mpm_grid* pooled_grid(mpm_context& c, size_t b) {
-  while (c.gridPool.size() <= b) {
-    c.gridPool.emplace_back(new mpm_grid());
-    c.gridPool.back()->ctx = &c;
+void ensure_body_lists(const mpm_context& cc) {
+  mpm_context& c = const_cast<mpm_context&>(cc);
+  if (c.bodyListsValid) return;
+  const int n = (int)c.n;
+  std::vector<int> lab(n), orig(n);
+  HIPCHECK(hipMemcpyAsync(lab.data(), c.labels.p, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipMemcpyAsync(orig.data(), c.orig_of[c.cur].p, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  c.bodyLists.assign(c.nbodies, {});
+  c.ballisticList.clear();
+  for (int s = 0; s < n; ++s) {
+    if (lab[s] >= 0) c.bodyLists[lab[s]].push_back(orig[s]);
+    else c.ballisticList.push_back(orig[s]);
   }
-  return c.gridPool[b].get();
+  for (auto& b : c.bodyLists) std::sort(b.begin(), b.end());
+  std::sort(c.ballisticList.begin(), c.ballisticList.end());
+  c.bodyListsValid = true;
+}
+
+void fill_stats(const SolveRecord& r, mpm_solve_stats* st) {
+  if (!st) return;
+  st->iterations = r.iterations;
+  st->early_exit = r.early_exit;
+  st->rhs_norm2 = r.rhs_norm2;
+  st->residual_norm2 = r.residual_norm2;
+  int L = (int)r.history.size();
+  if (st->history && st->history_capacity > 0) {
+    int k = std::min(L, st->history_capacity);
+    std::memcpy(st->history, r.history.data(), sizeof(double) * k);
+    st->history_length = k;
+  } else {
+    st->history_length = 0;
+  }
 }
 
 }  // namespace
@@ -634,6 +759,7 @@ int mpm_create(mpm_context** out, int device) {
   if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return MPM_ERR_NO_DEVICE;
   std::unique_ptr<mpm_context> c(new mpm_context());
   c->device = device;
+  c->simBatch.ctx = c.get();
   int rc = guarded(c.get(), [&] {
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->errflag.ensure(sizeof(int));
@@ -651,7 +777,6 @@ int mpm_destroy(mpm_context* ctx) {
   if (!ctx) return MPM_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  ctx->gridPool.clear();
   hipStream_t s = ctx->stream;
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
@@ -675,7 +800,7 @@ int mpm_particles_create(mpm_context* ctx, int64_t n) {
     ctx->cur = 0;
     HIPCHECK(hipMemsetAsync(ctx->store[0].p, 0, sizeof(float) * NCOL * n, ctx->stream));
     std::vector<int> iota(n);
-    for (int64_t i = 0; i < n; ++i) iota[i] = (int)i;
+    std::iota(iota.begin(), iota.end(), 0);
     HIPCHECK(hipMemcpyAsync(ctx->orig_of[0].p, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipMemcpyAsync(ctx->slot_of.p, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
     launch_identity_columns(ctx->view(), C_F, (int)n, ctx->stream);
@@ -753,7 +878,7 @@ int mpm_model_update_particle_data(mpm_context* ctx) {
     require_particles(ctx);
     if (!ctx->model) throw InvalidArg("no constitutive model");
     ctx->clear_err();
-    launch_plasticity(ctx->view(), nullptr, 0, (int)ctx->n, ctx->thc, ctx->ths, ctx->errp(), ctx->stream);
+    launch_plasticity(ctx->view(), nullptr, INT_MIN, (int)ctx->n, ctx->thc, ctx->ths, ctx->errp(), ctx->stream);
     ctx->check_errors();
   });
 }
@@ -822,20 +947,21 @@ int mpm_set_gravity(mpm_context* ctx, const float* g, int n) {
   });
 }
 
-// ------------------------------------------------------------------ grid API
+// ------------------------------------------------------------------ grid API (one body)
 int mpm_grid_create(mpm_context* ctx, const int32_t* particles, int64_t count, float h, const float fv[3], int dim,
                     mpm_grid** out) {
   if (!out) return MPM_ERR_INVALID;
   *out = nullptr;
   std::unique_ptr<mpm_grid> g(new mpm_grid());
   g->ctx = ctx;
+  g->batch.ctx = ctx;
   int rc = guarded(ctx, [&] {
     require_particles(ctx);
     if (dim != 3) throw InvalidArg("only dimension 3 is supported on the GPU");
     if (!(h > 0)) throw InvalidArg("grid size must be positive");
-    DevBuf slots;
-    ctx->slots_from_orig(particles, count, slots);
-    g->build(slots.as<int>(), (int)count, h, fv, false);
+    ctx->slots_from_orig(particles, count, g->slots);
+    float z[3] = {0, 0, 0};
+    g->batch.build(g->slots.as<int>(), nullptr, (int)count, 1, h, fv ? fv : z, false);
     ctx->check_errors();
   });
   if (rc == MPM_OK) *out = g.release();
@@ -852,10 +978,10 @@ int mpm_grid_destroy(mpm_grid* grid) {
 }
 
 int mpm_grid_dims(const mpm_grid* grid, int n[3], float mn[3]) {
-  if (!grid) return MPM_ERR_INVALID;
+  if (!grid || grid->batch.hb.empty()) return MPM_ERR_INVALID;
   for (int a = 0; a < 3; ++a) {
-    if (n) n[a] = grid->g.n[a];
-    if (mn) mn[a] = grid->g.gmin[a];
+    if (n) n[a] = grid->batch.hb[0].n[a];
+    if (mn) mn[a] = grid->batch.hb[0].gmin[a];
   }
   return MPM_OK;
 }
@@ -863,15 +989,17 @@ int mpm_grid_dims(const mpm_grid* grid, int n[3], float mn[3]) {
 int mpm_grid_get(mpm_grid* grid, const char* name, void* out) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
+    GridBatch& B = grid->batch;
     std::string nm(name ? name : "");
-    if (nm == "masses") grid->to_dense(grid->f(grid->mass), 1, static_cast<float*>(out));
-    else if (nm == "velocities") grid->to_dense(grid->f(grid->vel), 3, static_cast<float*>(out));
-    else if (nm == "prev_velocities") grid->to_dense(grid->f(grid->prev), 3, static_cast<float*>(out));
+    if (nm == "masses") B.to_dense(0, B.f(B.mass), 1, static_cast<float*>(out));
+    else if (nm == "velocities") B.to_dense(0, B.f(B.vel), 3, static_cast<float*>(out));
+    else if (nm == "prev_velocities") B.to_dense(0, B.f(B.prev), 3, static_cast<float*>(out));
     else if (nm == "node_collided") {
-      int64_t nd = (int64_t)grid->g.n[0] * grid->g.n[1] * grid->g.n[2];
-      grid->dense2.ensure(nd);
-      launch_to_dense_i8(grid->flags.as<int8_t>(), grid->dense2.as<int8_t>(), grid->g, grid->ctx->stream);
-      HIPCHECK(hipMemcpyAsync(out, grid->dense2.p, nd, hipMemcpyDeviceToHost, grid->ctx->stream));
+      const BodyGrid& G = B.hb[0];
+      int64_t nd = (int64_t)G.n[0] * G.n[1] * G.n[2];
+      B.dense2.ensure(nd);
+      launch_to_dense_i8(B.flags.as<int8_t>(), B.dense2.as<int8_t>(), G, grid->ctx->stream);
+      HIPCHECK(hipMemcpyAsync(out, B.dense2.p, nd, hipMemcpyDeviceToHost, grid->ctx->stream));
       grid->ctx->sync();
     } else
       throw InvalidArg("unknown grid variable '" + nm + "'");
@@ -881,15 +1009,17 @@ int mpm_grid_get(mpm_grid* grid, const char* name, void* out) {
 int mpm_grid_set(mpm_grid* grid, const char* name, const void* in) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
+    GridBatch& B = grid->batch;
     std::string nm(name ? name : "");
-    if (nm == "masses") grid->from_dense(static_cast<const float*>(in), 1, grid->f(grid->mass));
-    else if (nm == "velocities") grid->from_dense(static_cast<const float*>(in), 3, grid->f(grid->vel));
-    else if (nm == "prev_velocities") grid->from_dense(static_cast<const float*>(in), 3, grid->f(grid->prev));
+    if (nm == "masses") B.from_dense(0, static_cast<const float*>(in), 1, B.f(B.mass));
+    else if (nm == "velocities") B.from_dense(0, static_cast<const float*>(in), 3, B.f(B.vel));
+    else if (nm == "prev_velocities") B.from_dense(0, static_cast<const float*>(in), 3, B.f(B.prev));
     else if (nm == "node_collided") {
-      int64_t nd = (int64_t)grid->g.n[0] * grid->g.n[1] * grid->g.n[2];
-      grid->dense2.ensure(nd);
-      HIPCHECK(hipMemcpyAsync(grid->dense2.p, in, nd, hipMemcpyHostToDevice, grid->ctx->stream));
-      launch_from_dense_i8(grid->dense2.as<int8_t>(), grid->flags.as<int8_t>(), grid->g, grid->ctx->stream);
+      const BodyGrid& G = B.hb[0];
+      int64_t nd = (int64_t)G.n[0] * G.n[1] * G.n[2];
+      B.dense2.ensure(nd);
+      HIPCHECK(hipMemcpyAsync(B.dense2.p, in, nd, hipMemcpyHostToDevice, grid->ctx->stream));
+      launch_from_dense_i8(B.dense2.as<int8_t>(), B.flags.as<int8_t>(), G, grid->ctx->stream);
     } else
       throw InvalidArg("unknown grid variable '" + nm + "'");
     grid->ctx->sync();
@@ -900,7 +1030,8 @@ int mpm_grid_compute_particle_volumes(mpm_grid* grid) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     grid->ctx->clear_err();
-    launch_volumes(grid->ctx->view(), grid->g, grid->f(grid->mass), grid->ctx->errp(), grid->ctx->stream);
+    launch_volumes(grid->ctx->view(), grid->batch.view(), grid->batch.f(grid->batch.mass), grid->ctx->errp(),
+                   grid->ctx->stream);
     grid->ctx->check_errors();
   });
 }
@@ -909,13 +1040,13 @@ int mpm_grid_calculate_forces(mpm_grid* grid, float* forces) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
     if (!c.model) throw InvalidArg("no constitutive model");
     c.clear_err();
-    grid->forces_into_acc();
-    launch_node_forces(grid->g, grid->f(grid->mass), grid->f(grid->acc), grid->grav(), grid->ngrav(),
-                       grid->f(grid->acc2), c.stream);
+    B.forces_stage();
+    launch_node_forces(B.view(), B.f(B.mass), B.f(B.stageA), c.grav(), c.ngrav(), B.f(B.tmp3), c.stream);
     c.check_errors();
-    grid->to_dense(grid->f(grid->acc2), 3, forces);
+    B.to_dense(0, B.f(B.tmp3), 3, forces);
   });
 }
 
@@ -923,14 +1054,14 @@ int mpm_grid_force_differentials(mpm_grid* grid, const float* dx, float* df) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
     if (!c.model) throw InvalidArg("no constitutive model");
     c.clear_err();
-    grid->from_dense(dx, 3, grid->f(grid->acc2));
-    HIPCHECK(hipMemsetAsync(grid->acc.p, 0, grid->vecBytes(), c.stream));
-    launch_hvp(c.view(), grid->g, grid->f(grid->acc2), false, grid->flags.as<int8_t>(), c.cols(), grid->f(grid->acc),
-               nullptr, c.errp(), c.stream);
+    B.from_dense(0, dx, 3, B.f(B.tmp3));
+    B.hvp(B.f(B.tmp3), false, B.f(B.stageA), nullptr, -1);
+    launch_gather3(B.view(), B.f(B.stageA), B.f(B.Ar), c.stream);
     c.check_errors();
-    grid->to_dense(grid->f(grid->acc), 3, df);
+    B.to_dense(0, B.f(B.Ar), 3, df);
   });
 }
 
@@ -938,12 +1069,13 @@ int mpm_grid_dforce_dxi(mpm_grid* grid, float* out) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
     if (!c.model) throw InvalidArg("no constitutive model");
     c.clear_err();
-    HIPCHECK(hipMemsetAsync(grid->acc.p, 0, grid->vecBytes(), c.stream));
-    launch_diag(c.view(), grid->g, grid->f(grid->acc), c.errp(), c.stream);
+    launch_diag(c.view(), B.view(), B.f(B.stageB), c.errp(), c.stream);
+    launch_gather3(B.view(), B.f(B.stageB), B.f(B.tmp3), c.stream);
     c.check_errors();
-    grid->to_dense(grid->f(grid->acc), 3, out);
+    B.to_dense(0, B.f(B.tmp3), 3, out);
   });
 }
 
@@ -951,27 +1083,27 @@ int mpm_grid_explicit_momenta(mpm_grid* grid, float dt, float* momenta, int8_t* 
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
     if (!c.model) throw InvalidArg("no constitutive model");
     c.clear_err();
-    HIPCHECK(hipMemsetAsync(grid->sc.p, 0, sizeof(CRScalars), c.stream));
-    grid->forces_into_acc();
-    // write into scratch buffers so the grid's own node flags (set only by
-    // updateGridVelocities in the reference) stay untouched
-    DevBuf fl;
-    fl.ensure(grid->NN);
-    launch_node_explicit(grid->g, grid->f(grid->mass), grid->f(grid->vel), grid->f(grid->acc), grid->grav(),
-                         grid->ngrav(), dt, c.cols(), grid->f(grid->acc2), fl.as<int8_t>(), grid->f(grid->draw),
-                         grid->scal(), grid->partials.as<double>(), c.stream);
+    HIPCHECK(hipMemsetAsync(B.scal.p, 0, sizeof(CRScalars) * B.nb, c.stream));
+    HIPCHECK(hipMemsetAsync(B.status.p, 0, sizeof(BatchStatus), c.stream));
+    B.forces_stage();
+    // scratch outputs: the grid's own node flags are set only by updateGridVelocities
+    launch_node_explicit(B.view(), B.f(B.mass), B.f(B.vel), B.f(B.stageA), c.grav(), c.ngrav(), dt, c.cols(),
+                         B.f(B.tmp3), B.flags2.as<int8_t>(), B.f(B.Ar), B.sc(), B.st(), B.partials.as<double>(),
+                         c.stream);
     CRScalars hs;
-    HIPCHECK(hipMemcpyAsync(&hs, grid->sc.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipMemcpyAsync(&hs, B.scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     c.sync();
     c.check_errors();
-    mpm_context::raise(hs.errors);
-    grid->to_dense(grid->f(grid->acc2), 3, momenta);
-    int64_t nd = (int64_t)grid->g.n[0] * grid->g.n[1] * grid->g.n[2];
-    grid->dense2.ensure(nd);
-    launch_to_dense_i8(fl.as<int8_t>(), grid->dense2.as<int8_t>(), grid->g, c.stream);
-    HIPCHECK(hipMemcpyAsync(collided, grid->dense2.p, nd, hipMemcpyDeviceToHost, c.stream));
+    raise_errors(hs.errors);
+    B.to_dense(0, B.f(B.tmp3), 3, momenta);
+    const BodyGrid& G = B.hb[0];
+    int64_t nd = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    B.dense2.ensure(nd);
+    launch_to_dense_i8(B.flags2.as<int8_t>(), B.dense2.as<int8_t>(), G, c.stream);
+    HIPCHECK(hipMemcpyAsync(collided, B.dense2.p, nd, hipMemcpyDeviceToHost, c.stream));
     c.sync();
   });
 }
@@ -980,34 +1112,18 @@ int mpm_grid_collision_velocities(mpm_grid* grid, const int8_t* collided, float*
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
-    int64_t nd = (int64_t)grid->g.n[0] * grid->g.n[1] * grid->g.n[2];
+    GridBatch& B = grid->batch;
+    const BodyGrid& G = B.hb[0];
+    int64_t nd = (int64_t)G.n[0] * G.n[1] * G.n[2];
     for (int64_t i = 0; i < nd; ++i)
       if (collided[i] >= c.hcols.n) throw InvalidArg("collision object index out of range");
-    DevBuf fl;
-    fl.ensure(grid->NN);
-    HIPCHECK(hipMemsetAsync(fl.p, 0xFF, grid->NN, c.stream));
-    grid->dense2.ensure(nd);
-    HIPCHECK(hipMemcpyAsync(grid->dense2.p, collided, nd, hipMemcpyHostToDevice, c.stream));
-    launch_from_dense_i8(grid->dense2.as<int8_t>(), fl.as<int8_t>(), grid->g, c.stream);
-    launch_collision_velocities(grid->g, fl.as<int8_t>(), c.cols(), grid->f(grid->acc2), c.stream);
-    grid->to_dense(grid->f(grid->acc2), 3, vc);
+    HIPCHECK(hipMemsetAsync(B.flags2.p, 0xFF, B.NN, c.stream));
+    B.dense2.ensure(nd);
+    HIPCHECK(hipMemcpyAsync(B.dense2.p, collided, nd, hipMemcpyHostToDevice, c.stream));
+    launch_from_dense_i8(B.dense2.as<int8_t>(), B.flags2.as<int8_t>(), G, c.stream);
+    launch_collision_velocities(B.view(), B.flags2.as<int8_t>(), c.cols(), B.f(B.tmp3), c.stream);
+    B.to_dense(0, B.f(B.tmp3), 3, vc);
   });
-}
-
-static void fill_stats(const SolveRecord& r, mpm_solve_stats* st) {
-  if (!st) return;
-  st->iterations = r.iterations;
-  st->early_exit = r.early_exit;
-  st->rhs_norm2 = r.rhs_norm2;
-  st->residual_norm2 = r.residual_norm2;
-  int L = (int)r.history.size();
-  if (st->history && st->history_capacity > 0) {
-    int k = std::min(L, st->history_capacity);
-    std::memcpy(st->history, r.history.data(), sizeof(double) * k);
-    st->history_length = k;
-  } else {
-    st->history_length = 0;
-  }
 }
 
 int mpm_grid_update_grid_velocities(mpm_grid* grid, float dt, const mpm_solver_params* params,
@@ -1016,9 +1132,11 @@ int mpm_grid_update_grid_velocities(mpm_grid* grid, float dt, const mpm_solver_p
   return guarded(grid->ctx, [&] {
     if (!grid->ctx->model) throw InvalidArg("no constitutive model");
     if (!params) throw InvalidArg("solver parameters required");
-    SolveRecord rec;
-    grid->updateGridVelocities(dt, params, rec);
-    fill_stats(rec, stats);
+    GridBatch& B = grid->batch;
+    B.solve(dt, params);
+    grid->ctx->prof_resolve();
+    B.fetch_history(0, B.rec[0]);
+    fill_stats(B.rec[0], stats);
   });
 }
 
@@ -1026,9 +1144,9 @@ int mpm_grid_update_particle_velocities(mpm_grid* grid) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
     c.clear_err();
-    launch_g2p(c.view(), grid->g, grid->f(grid->vel), grid->f(grid->prev), 0, 0.0f, 0.0f, 0.0f, c.cols(), c.errp(),
-               c.stream);
+    launch_g2p(c.view(), B.view(), B.f(B.vel), B.f(B.prev), 0, 0.0f, 0.0f, 0.0f, c.cols(), c.errp(), c.stream);
     c.check_errors();
   });
 }
@@ -1037,9 +1155,9 @@ int mpm_grid_update_deformation_gradients(mpm_grid* grid, float dt) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {
     mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
     c.clear_err();
-    launch_g2p(c.view(), grid->g, grid->f(grid->vel), grid->f(grid->prev), 1, dt, 0.0f, 0.0f, c.cols(), c.errp(),
-               c.stream);
+    launch_g2p(c.view(), B.view(), B.f(B.vel), B.f(B.prev), 1, dt, 0.0f, 0.0f, c.cols(), c.errp(), c.stream);
     c.check_errors();
   });
 }
@@ -1053,15 +1171,12 @@ int mpm_sim_init(mpm_context* ctx, float h, int dim) {
     ctx->h = h;
     ctx->dim = dim;
     calculate_bodies(*ctx);
-    // Sim::Sim: volumes of the initial bodies (Sim.cpp:44-49)
-    for (size_t b = 0; b < ctx->bodies.size(); ++b) {
-      const auto& B = ctx->bodies[b];
-      DevBuf slots;
-      ctx->slots_from_orig(B.data(), (int64_t)B.size(), slots);
-      mpm_grid* g = pooled_grid(*ctx, b);
-      float zero[3] = {0, 0, 0};
-      g->build(slots.as<int>(), (int)B.size(), h, zero, false);
-      launch_volumes(ctx->view(), g->g, g->f(g->mass), ctx->errp(), ctx->stream);
+    // Sim::Sim: volumes of the initial bodies (Sim.cpp:44-49), frames at rest
+    if (ctx->nbodies > 0) {
+      GridBatch& B = ctx->simBatch;
+      std::vector<float> zero(3 * ctx->nbodies, 0.0f);
+      B.build(nullptr, ctx->labels.as<int>(), (int)ctx->n, ctx->nbodies, h, zero.data(), false);
+      launch_volumes(ctx->view(), B.view(), B.f(B.mass), ctx->errp(), ctx->stream);
       ctx->check_errors();
     }
     ctx->simInit = true;
@@ -1080,93 +1195,76 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
     c.lastSolves.clear();
     if (stats) std::memset(stats, 0, sizeof(*stats));
     const bool cancelHook = params->cancelled != nullptr;
-    // body slot lists
-    std::vector<int> bodyStart;
-    size_t nb = c.bodies.size();
-    c.clear_err();
+    const int nb = c.nbodies;
+    const int* labels = c.labels.as<int>();
     bool cancelled = false;
-    for (size_t b = 0; b < nb; ++b) {
-      const auto& B = c.bodies[b];
-      DevBuf slots;
-      c.slots_from_orig(B.data(), (int64_t)B.size(), slots);
-      mpm_grid* g = pooled_grid(c, b);
-      g->build(slots.as<int>(), (int)B.size(), c.h, nullptr, true);
+    c.clear_err();
+    GridBatch& B = c.simBatch;
+    if (nb > 0) {
+      // updateParticleData after the first body runs over every particle (Sim.cpp:118): the
+      // particles outside body 0 (other bodies and ballistic ones) are brought up to date from
+      // their unchanged F before their own grids are built, which is where the reference's
+      // sequential loop would have done it
+      if (nb > 1 || c.nballistic > 0)
+        launch_plasticity(c.view(), labels, 0, n, c.thc, c.ths, c.errp(), s);
+      B.build(nullptr, labels, n, nb, c.h, nullptr, true);
       c.check_errors();
-      SolveRecord rec;
-      g->updateGridVelocities(dt, params, rec);
-      c.lastSolves.push_back(rec);
-      if (stats) {
-        stats->total_iterations += rec.iterations;
-        stats->max_iterations = std::max(stats->max_iterations, rec.iterations);
+      B.solve(dt, params);
+      // the reference polls cancellation after each body's solve and keeps the particle
+      // updates of the bodies before it (Sim.cpp:104-108); chunks are body-ordered, so those
+      // bodies are a prefix of the chunk list
+      int ncomplete = nb;
+      if (cancelHook)
+        for (int bi = 0; bi < nb; ++bi)
+          if (params->cancelled(params->user)) {
+            ncomplete = bi;
+            cancelled = true;
+            break;
+          }
+      Batch view = B.view();
+      if (cancelled) {
+        int prefix = 0;
+        if (ncomplete > 0) {
+          HIPCHECK(hipMemcpyAsync(&prefix, B.chunkOff.as<int>() + B.hb[ncomplete].tileBase, sizeof(int),
+                                  hipMemcpyDeviceToHost, s));
+          c.sync();
+        }
+        view.nchunks = prefix;
       }
-      if (cancelHook && params->cancelled(params->user)) {
-        cancelled = true;
-        break;
-      }
-      // G2P + F + plasticity (+ advection when no cancellation can intervene)
-      c.clear_err();
-      int pg = c.prof_begin(PK_G2P, g);
-      launch_g2p(c.view(), g->g, g->f(g->vel), g->f(g->prev), cancelHook ? 3 : 2, dt, c.thc, c.ths, c.cols(), c.errp(),
-                 s);
+      int pg = c.prof_begin(PK_G2P, &B);
+      launch_g2p(c.view(), view, B.f(B.vel), B.f(B.prev), cancelled ? 3 : 2, dt, c.thc, c.ths, c.cols(), c.errp(), s);
       c.prof_end(pg);
-      if (b == 0 && (int64_t)B.size() < c.n) {
-        // updateParticleData runs over all particles after each body (Sim.cpp:118); the other
-        // particles are brought up to date here once, from their unchanged F
-        std::vector<char> inBody(n, 0);
-        for (int i : B) inBody[i] = 1;
-        std::vector<int> others;
-        others.reserve(n - B.size());
-        for (int i = 0; i < n; ++i)
-          if (!inBody[i]) others.push_back(i);
-        c.slots_from_orig(others.data(), (int64_t)others.size(), c.other_slots);
-        launch_plasticity(c.view(), c.other_slots.as<int>(), 0, (int)others.size(), c.thc, c.ths, c.errp(), s);
-      }
-      c.check_errors();
+      c.lastSolves = B.rec;
+      if (stats)
+        for (const SolveRecord& r : B.rec) {
+          stats->total_iterations += r.iterations;
+          stats->max_iterations = std::max(stats->max_iterations, r.iterations);
+        }
     }
-    // ballistic particles: velocity (Sim.cpp:64-73) and advection
-    if (!c.ballistic.empty()) {
-      c.slots_from_orig(c.ballistic.data(), (int64_t)c.ballistic.size(), c.ball_slots);
-      launch_ballistic(c.view(), c.ball_slots.as<int>(), (int)c.ballistic.size(), c.grav_dev.as<float>(),
-                       (int)(c.hgrav.size() / 3), dt, c.cols(), cancelled ? 1 : (cancelHook ? 1 : 3), s);
-    }
-    if (cancelHook && !cancelled) {
-      // advection of all body particles (Sim.cpp:123-131)
-      for (size_t b = 0; b < nb; ++b) {
-        mpm_grid* g = pooled_grid(c, b);
-        launch_advect(c.view(), g->perm.as<int>(), g->np, dt, c.cols(), s);
-      }
-      if (!c.ballistic.empty())
-        launch_ballistic(c.view(), c.ball_slots.as<int>(), (int)c.ballistic.size(), c.grav_dev.as<float>(),
-                         (int)(c.hgrav.size() / 3), dt, c.cols(), 2, s);
-    }
+    // ballistic particles: velocity (Sim.cpp:64-73) and, unless cancelled, advection
+    if (c.nballistic > 0)
+      launch_ballistic(c.view(), labels, n, c.grav(), c.ngrav(), dt, c.cols(), cancelled ? 1 : 3, s);
+    c.check_errors();
     if (!cancelled && nb > 0) {
-      // keep the store body- and tile-sorted: new slot order = body perms, then the rest
-      std::vector<char> placed;
-      int64_t off = 0;
-      c.new_to_old.ensure(sizeof(int) * n);
-      for (size_t b = 0; b < nb; ++b) {
-        mpm_grid* g = pooled_grid(c, b);
-        HIPCHECK(hipMemcpyAsync(c.new_to_old.as<int>() + off, g->perm.p, sizeof(int) * g->np, hipMemcpyDeviceToDevice,
-                                s));
-        off += g->np;
-      }
-      if (off + (int64_t)c.ballistic.size() != n)
-        throw std::runtime_error("body partition does not cover every particle (internal error)");
-      if (off < n) {
-        HIPCHECK(hipMemcpyAsync(c.new_to_old.as<int>() + off, c.ball_slots.p, sizeof(int) * (n - off),
-                                hipMemcpyDeviceToDevice, s));
+      // keep the store body- and tile-sorted: body particles in batch order, then ballistic ones
+      c.newToOld.ensure(sizeof(int) * n);
+      HIPCHECK(hipMemcpyAsync(c.newToOld.p, B.perm.p, sizeof(int) * B.np, hipMemcpyDeviceToDevice, s));
+      if (B.np + c.nballistic != n) throw std::runtime_error("body partition does not cover every particle");
+      if (c.nballistic > 0) {
+        launch_is_ballistic(labels, n, c.ccIsRoot.as<int>(), s);
+        scan_exclusive(c.ccIsRoot.as<int>(), c.ballPos.as<int>(), n, c.ccTot.as<int>() + 3, c.ccTmp.as<int>(), s);
+        launch_ballistic_compact(labels, c.ballPos.as<int>(), n, c.newToOld.as<int>(), B.np, s);
       }
       int nxt = 1 - c.cur;
-      launch_permute(c.view(c.cur), c.view(nxt), c.new_to_old.as<int>(), n, c.orig_of[c.cur].as<int>(),
+      launch_permute(c.view(c.cur), c.view(nxt), c.newToOld.as<int>(), n, c.orig_of[c.cur].as<int>(),
                      c.orig_of[nxt].as<int>(), c.slot_of.as<int>(), s);
       c.cur = nxt;
     }
-    c.sync();
     c.prof_resolve();
     calculate_bodies(c);
     if (stats) {
-      stats->num_bodies = (int)c.bodies.size();
-      stats->num_ballistic = (int)c.ballistic.size();
+      stats->num_bodies = c.nbodies;
+      stats->num_ballistic = c.nballistic;
       stats->cancelled = cancelled ? 1 : 0;
     }
   });
@@ -1200,23 +1298,42 @@ int mpm_profile_query(mpm_context* ctx, const char* kernel, double* total_ms, in
   });
 }
 
-int mpm_sim_num_bodies(const mpm_context* ctx) { return ctx ? (int)ctx->bodies.size() : 0; }
+int mpm_sim_num_bodies(const mpm_context* ctx) { return ctx ? ctx->nbodies : 0; }
 
 int64_t mpm_sim_body(const mpm_context* ctx, int b, int32_t* out) {
-  if (!ctx || b < 0 || b >= (int)ctx->bodies.size()) return -1;
-  const auto& B = ctx->bodies[b];
+  if (!ctx || b < 0 || b >= ctx->nbodies) return -1;
+  try {
+    ensure_body_lists(*ctx);
+  } catch (...) {
+    return -1;
+  }
+  const auto& B = ctx->bodyLists[b];
   if (out) std::copy(B.begin(), B.end(), out);
   return (int64_t)B.size();
 }
 
 int64_t mpm_sim_ballistic(const mpm_context* ctx, int32_t* out) {
   if (!ctx) return -1;
-  if (out) std::copy(ctx->ballistic.begin(), ctx->ballistic.end(), out);
-  return (int64_t)ctx->ballistic.size();
+  try {
+    ensure_body_lists(*ctx);
+  } catch (...) {
+    return -1;
+  }
+  if (out) std::copy(ctx->ballisticList.begin(), ctx->ballisticList.end(), out);
+  return (int64_t)ctx->ballisticList.size();
 }
 
 int mpm_sim_body_solve_stats(const mpm_context* ctx, int b, mpm_solve_stats* st) {
   if (!ctx || b < 0 || b >= (int)ctx->lastSolves.size()) return MPM_ERR_INVALID;
+  mpm_context& c = const_cast<mpm_context&>(*ctx);
+  try {
+    HIPCHECK(hipSetDevice(c.device));
+    SolveRecord& r = c.lastSolves[b];
+    if (r.history.empty() && r.iterations > 0) c.simBatch.fetch_history(b, r);
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    return MPM_ERR_HIP;
+  }
   fill_stats(ctx->lastSolves[b], st);
   return MPM_OK;
 }
