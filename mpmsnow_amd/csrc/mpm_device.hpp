@@ -89,7 +89,18 @@ struct Batch {
   const int* tileChunkN;    // number of chunks of each global tile
   int NT;                   // global tiles
   int nblocks;              // global node blocks
+  unsigned int* tickets;    // reduction tickets: [body][set][1 + kSub][kTicketStride]
 };
+
+// Per-body reductions inside node kernels: every block stores its partial sums, then counts
+// in on one of kSub sub-tickets; the last block of each sub-ticket counts in on the body's
+// main ticket, and the last of those combines the partials in block order. Tickets sit on
+// separate 128-byte lines; spreading the arrivals keeps same-address atomics short.
+constexpr int kSub = 32;
+constexpr int kTicketStride = 32;
+constexpr int kTicketSets = 3;
+constexpr int kTicketsPerBody = kTicketSets * (kSub + 1) * kTicketStride;
+constexpr int kPartialStride = 16;  // doubles per block in the partials buffer
 
 // error bits raised by kernels (reference exception messages are mapped by the host)
 enum ErrorBits : int {
@@ -118,7 +129,6 @@ struct CRScalars {
   int max_iters;
   int errors;              // ErrorBits
   int active_nodes;        // nodes with m > 0 (measurement)
-  unsigned int ticket[4];  // last-block-done counters
 };
 
 // Counters shared by all bodies of a batch.

@@ -3,6 +3,8 @@
 // Layout and scatter scheme: see mpm_device.hpp. The arithmetic of each step
 // follows the reference (davidsminor/mpmsnow, paths relative to its checkout)
 // as cited per kernel.
+#include <climits>
+
 #include "mpm_kernels.hpp"
 
 namespace mpm {
@@ -122,14 +124,76 @@ __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, con
     sd[2][i] = v[2];
   }
 }
+// ------------------------------------------------------------ fixed-point footprint accumulation
+// LDS float atomics (ds_add_f32) run at a small fraction of the rate of integer ones on gfx950,
+// so each chunk accumulates its footprint scatter in fixed point with integer LDS atomics. A
+// chunk picks a power-of-two scale S from T, the sum over its particles of a bound on any one
+// contribution, so that every scaled contribution and every node sum stays below 2^43; each
+// scaled value y is split as y = h 2^22 + l (h = trunc(y 2^-22), |l| < 2^22) and the integer
+// parts of h and l go to two 32-bit accumulators (node sums of h stay below 2^21, of l below
+// 2^30 for at most 256 particles). The resolution is 2^-43 of T, far finer than float
+// rounding, and integer addition is order-independent: the staged footprints are bitwise
+// reproducible. A chunk with a non-finite T stages NaN.
+static_assert(kChunk == kBlock, "one particle per thread in chunk kernels");
+template <int NC> struct FxFoot {
+  int hi[NC][kFoot];
+  int lo[NC][kFoot];
+};
+template <int NC> __device__ __forceinline__ void fx_zero(FxFoot<NC>& a) {
+  for (int i = threadIdx.x; i < kFoot; i += kBlock)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      a.hi[c][i] = 0;
+      a.lo[c][i] = 0;
+    }
+}
+__device__ __forceinline__ void fx_add(int* hi, int* lo, float y) {
+  const float h = truncf(y * 2.384185791015625e-07f);  // 2^-22
+  const float l = fmaf(h, -4194304.0f, y);             // exact: y - h 2^22
+  atomicAdd(hi, (int)h);
+  atomicAdd(lo, (int)l);
+}
+struct FxScale {
+  float S, invS;
+  bool bad;
+};
+// T: per-thread contribution bounds, summed over the block (every thread calls)
+__device__ __forceinline__ FxScale fx_scale(float bound) {
+  __shared__ float red[kBlock / 64];
+  float v = bound;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const float T = ((red[0] + red[1]) + red[2]) + red[3];
+  FxScale r;
+  r.bad = !(T <= 3.0e38f);  // NaN or infinite
+  int k = 126;
+  if (T > 0.0f && !r.bad) {
+    k = 42 - ilogbf(T);  // T < 2^(ilogb + 1): sums stay below 2^43
+    k = k > 126 ? 126 : (k < -126 ? -126 : k);
+  }
+  r.S = __int_as_float((k + 127) << 23);
+  r.invS = __int_as_float((127 - k) << 23);
+  return r;
+}
 // staged layout per chunk: [footprint tile f][component c][64 local nodes]
-__device__ __forceinline__ void store_stage3(float (*so)[kFoot], float* stage) {
+template <int NC> __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc, float* stage) {
   float* dst = stage + (int64_t)blockIdx.x * kStage;
+  const double inv = (double)sc.invS;
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int f = i >> 6, l = i & 63;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dst[f * 192 + c * 64 + l] = so[c][i];
+    for (int c = 0; c < NC; ++c) {
+      double v = ((double)a.hi[c][i] * 4194304.0 + (double)a.lo[c][i]) * inv;
+      dst[f * 192 + c * 64 + l] = sc.bad ? __int_as_float(0x7fc00000) : (float)v;
+    }
   }
+}
+__device__ __forceinline__ float abs_sum9(const float m[9]) {
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s += fabsf(m[i]);
+  return s;
 }
 
 // gather of staged footprints for node (tile T, local l) of body G, component c
@@ -237,39 +301,58 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], dou
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = ((sm[0][k] + sm[1][k]) + sm[2][k]) + sm[3][k];
 }
+__device__ __forceinline__ unsigned int* body_tickets(const Batch& B, int body, int set) {
+  return B.tickets + (int64_t)body * kTicketsPerBody + (int64_t)set * (kSub + 1) * kTicketStride;
+}
 // Writes this block's partials; returns true in every thread of the last block of the body
-// to finish, with out[] = the fixed-order total over the body's blocks.
+// to finish, with out[] = the fixed-order total over the body's blocks. The partials are
+// written with agent-scope atomic stores and the block waits for their completion before it
+// takes a ticket: a release fence would write back the whole L2 of this XCD from every block.
 template <int NV>
-__device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, unsigned int* ticket,
+__device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, unsigned int* tk,
                          double (&out)[NV]) {
+  static_assert(NV <= kPartialStride, "partials stride");
   __shared__ double sm[kBlock / 64][NV];
   __shared__ bool amLast;
   block_sum<NV>(v, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partials[(int64_t)blockIdx.x * NV + k] = v[k];
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    amLast = (t == (unsigned int)G.nblocks - 1u);
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(partials + (int64_t)blockIdx.x * kPartialStride + k, v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const int local = blockIdx.x - G.blockBase;
+    const int sub = local % kSub;
+    const unsigned int arrivals = (unsigned int)((G.nblocks - sub + kSub - 1) / kSub);
+    const unsigned int nsubs = (unsigned int)(G.nblocks < kSub ? G.nblocks : kSub);
+    unsigned int* st = tk + (1 + sub) * kTicketStride;
+    bool last = false;
+    if (__hip_atomic_fetch_add(st, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == arrivals - 1u) {
+      __hip_atomic_store(st, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__hip_atomic_fetch_add(tk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nsubs - 1u) {
+        __hip_atomic_store(tk, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = true;
+      }
+    }
+    amLast = last;
   }
   __syncthreads();
   if (!amLast) return false;
-  __threadfence();
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
   for (int b = threadIdx.x; b < G.nblocks; b += kBlock)
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      acc[k] += __hip_atomic_load(partials + (int64_t)(G.blockBase + b) * NV + k, __ATOMIC_RELAXED,
+      acc[k] += __hip_atomic_load(partials + (int64_t)(G.blockBase + b) * kPartialStride + k, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);
   block_sum<NV>(acc, sm);
   __shared__ double res[NV];
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) res[k] = acc[k];
-    *ticket = 0u;
-  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = res[k];
@@ -300,6 +383,9 @@ __global__ void k_fill_column(PView P, int col, float value, int n) {
 }
 __global__ void k_fill_int(int* p, int value, int64_t n) {
   GRID_STRIDE(i, n) p[i] = value;
+}
+__global__ void k_iota(int* p, int64_t n) {
+  GRID_STRIDE(i, n) p[i] = (int)i;
 }
 __global__ void k_identity_columns(PView P, int col0, int n) {
   GRID_STRIDE(s, n) {
@@ -343,8 +429,12 @@ __global__ void k_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket
     rank[s] = atomicAdd(count + h, 1);
   }
 }
-__global__ void k_cc_scatter(const int* bucket, const int* rank, const int* offset, int n, int* sorted) {
-  GRID_STRIDE(s, n) sorted[offset[bucket[s]] + rank[s]] = (int)s;
+// cell-sorted copy of the positions, slot in .w, so candidate scans read contiguous memory
+__global__ void k_cc_scatter(PView P, const int* bucket, const int* rank, const int* offset, int n, float4* sorted) {
+  GRID_STRIDE(si, n) {
+    int s = (int)si;
+    sorted[offset[bucket[s]] + rank[s]] = make_float4(P.at(C_X, s), P.at(C_X + 1, s), P.at(C_X + 2, s), __int_as_float(s));
+  }
 }
 __device__ __forceinline__ int cc_find(int* parent, int x) {
   int p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -356,13 +446,24 @@ __device__ __forceinline__ int cc_find(int* parent, int x) {
   }
   return x;
 }
-// hook the larger root under the smaller one; parent pointers only ever decrease
+// Link priority: a bijective integer mix, so hooking follows a pseudo-random order of the
+// slots rather than their (spatially sorted) index order, which would build long chains.
+__device__ __forceinline__ uint32_t cc_priority(int x) {
+  uint32_t h = (uint32_t)x;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+// hook the root of higher priority under the other; priorities strictly fall towards a root
 __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
   while (true) {
     a = cc_find(parent, a);
     b = cc_find(parent, b);
     if (a == b) return;
-    if (a < b) {
+    if (cc_priority(a) < cc_priority(b)) {
       int t = a;
       a = b;
       b = t;
@@ -371,14 +472,15 @@ __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
     if (old == a) return;
   }
 }
-// neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190)
-__global__ void k_cc_link(PView P, int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                          const int* sorted, int* parent, int* hasNb) {
+// neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190);
+// threads walk the cell-sorted order so a wave's candidate cells mostly coincide
+__global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                          const float4* sorted, int* parent, int* hasNb) {
   const float r2 = h * h;
-  GRID_STRIDE(si, n) {
-    int s = (int)si;
-    float x[3];
-    load_x(P, s, x);
+  GRID_STRIDE(ti, n) {
+    float4 me = sorted[ti];
+    const int s = __float_as_int(me.w);
+    const float x[3] = {me.x, me.y, me.z};
     int c[3];
     cell_of(x, cell, c);
     bool nb = false;
@@ -388,12 +490,11 @@ __global__ void k_cc_link(PView P, int n, float h, float cell, uint32_t mask, co
           uint32_t hb = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, mask);
           int o = offset[hb], e = o + count[hb];
           for (int q = o; q < e; ++q) {
-            int j = sorted[q];
+            float4 y = sorted[q];
+            int j = __float_as_int(y.w);
             if (j == s) continue;
-            float y[3];
-            load_x(P, j, y);
             // (d0 d0 + d1 d1) + d2 d2 with separate roundings, as Eigen's squaredNorm on the host
-            float d0 = x[0] - y[0], d1 = x[1] - y[1], d2 = x[2] - y[2];
+            float d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
             float d = __fadd_rn(__fadd_rn(__fmul_rn(d0, d0), __fmul_rn(d1, d1)), __fmul_rn(d2, d2));
             if (d < r2) {
               nb = true;
@@ -404,14 +505,31 @@ __global__ void k_cc_link(PView P, int n, float h, float cell, uint32_t mask, co
     hasNb[s] = nb ? 1 : 0;
   }
 }
+// every slot points at its root; smallest caller index per component. The store is
+// body-sorted, so a wave's slots usually share one root: reduce in registers first.
 __global__ void k_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig) {
-  GRID_STRIDE(si, n) {
-    int s = (int)si;
-    int r = cc_find(parent, s);
-    parent[s] = r;
-    if (hasNb[s]) {
-      atomicMin(minOrig + r, orig_of[s]);
-      if (r == s) isRoot[s] = 1;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    int64_t k = base + threadIdx.x;
+    int r = -1, o = INT_MAX;
+    if (k < n) {
+      int s = (int)k;
+      int root = cc_find(parent, s);
+      parent[s] = root;
+      if (hasNb[s]) {
+        r = root;
+        o = orig_of[s];
+        if (root == s) isRoot[s] = 1;
+      }
+    }
+    unsigned long long have = __ballot(r >= 0);
+    if (have == 0ull) continue;
+    int first = __ffsll((long long)have) - 1;
+    int rr = __shfl(r, first, 64);
+    if (__all((r == rr) || (r < 0))) {
+      for (int off = 32; off > 0; off >>= 1) o = min(o, __shfl_down(o, off, 64));
+      if ((threadIdx.x & 63) == 0) atomicMin(minOrig + rr, o);
+    } else if (r >= 0) {
+      atomicMin(minOrig + r, o);
     }
   }
 }
@@ -437,71 +555,97 @@ __device__ __forceinline__ unsigned int ord_enc(float f) {
   unsigned int u = __float_as_uint(f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-// bbox (Grid.cpp:177-191) and comoving-frame sums (Sim.cpp:84-93) per body; waves whose
-// particles all belong to one body reduce in registers first
+// bbox (Grid.cpp:177-191) and comoving-frame sums (Sim.cpp:84-93) per body. Each block
+// takes a contiguous range of the (body-sorted) list and every thread keeps running sums for
+// its current body, flushing them when the body changes; blocks whose threads all end on one
+// body combine in LDS and flush once.
+struct BodyAcc {
+  unsigned int mn[3], mx[3];
+  double m[4];
+  int cnt;
+};
+__device__ __forceinline__ void body_acc_reset(BodyAcc& a) {
+  for (int q = 0; q < 3; ++q) {
+    a.mn[q] = 0xFFFFFFFFu;
+    a.mx[q] = 0u;
+  }
+  for (int q = 0; q < 4; ++q) a.m[q] = 0.0;
+  a.cnt = 0;
+}
+__device__ __forceinline__ void body_acc_flush(const BodyAcc& a, int b, unsigned int* bbox, double* mom, int* count) {
+  for (int q = 0; q < 3; ++q) {
+    atomicMin(bbox + 6 * b + q, a.mn[q]);
+    atomicMax(bbox + 6 * b + 3 + q, a.mx[q]);
+  }
+  for (int q = 0; q < 4; ++q) atomicAdd(mom + 4 * b + q, a.m[q]);
+  atomicAdd(count + b, a.cnt);
+}
 __global__ void k_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
                               int* count) {
-  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
-    int64_t k = base + threadIdx.x;
-    int b = -1;
-    int s = 0;
-    if (k < n) {
-      s = slots ? slots[k] : (int)k;
-      b = labels ? labels[s] : 0;
+  const int64_t per = ((int64_t)n + gridDim.x - 1) / gridDim.x;
+  const int64_t beg = (int64_t)blockIdx.x * per;
+  const int64_t end = beg + per < n ? beg + per : n;
+  BodyAcc a;
+  body_acc_reset(a);
+  int cb = -1;
+  for (int64_t k = beg + threadIdx.x; k < end; k += kBlock) {
+    int s = slots ? slots[k] : (int)k;
+    int b = labels ? labels[s] : 0;
+    if (b < 0) continue;
+    if (b != cb) {
+      if (cb >= 0) body_acc_flush(a, cb, bbox, mom, count);
+      body_acc_reset(a);
+      cb = b;
     }
-    unsigned int mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (b >= 0) {
-      float m = P.at(C_M, s);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        unsigned int e = ord_enc(P.at(C_X + a, s));
-        mn[a] = e;
-        mx[a] = e;
-        acc[a] = (double)(P.at(C_V + a, s) * m);
+    float m = P.at(C_M, s);
+    for (int q = 0; q < 3; ++q) {
+      unsigned int e = ord_enc(P.at(C_X + q, s));
+      a.mn[q] = min(a.mn[q], e);
+      a.mx[q] = max(a.mx[q], e);
+      a.m[q] += (double)(P.at(C_V + q, s) * m);
+    }
+    a.m[3] += (double)m;
+    a.cnt += 1;
+  }
+  __shared__ int blockBody;
+  __shared__ int uniform;
+  if (threadIdx.x == 0) {
+    blockBody = -1;
+    uniform = 1;
+  }
+  __syncthreads();
+  if (cb >= 0) atomicCAS(&blockBody, -1, cb);
+  __syncthreads();
+  if (cb >= 0 && cb != blockBody) uniform = 0;
+  __syncthreads();
+  if (!uniform) {
+    if (cb >= 0) body_acc_flush(a, cb, bbox, mom, count);
+    return;
+  }
+  if (blockBody < 0) return;
+  // one body in the block: combine in registers across the wave, then across waves in LDS
+  for (int q = 0; q < 3; ++q)
+    for (int off = 32; off > 0; off >>= 1) {
+      a.mn[q] = min(a.mn[q], (unsigned int)__shfl_xor((int)a.mn[q], off, 64));
+      a.mx[q] = max(a.mx[q], (unsigned int)__shfl_xor((int)a.mx[q], off, 64));
+    }
+  for (int q = 0; q < 4; ++q)
+    for (int off = 32; off > 0; off >>= 1) a.m[q] += __shfl_xor(a.m[q], off, 64);
+  for (int off = 32; off > 0; off >>= 1) a.cnt += __shfl_xor(a.cnt, off, 64);
+  __shared__ BodyAcc wacc[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) wacc[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    BodyAcc t = wacc[0];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      for (int q = 0; q < 3; ++q) {
+        t.mn[q] = min(t.mn[q], wacc[w].mn[q]);
+        t.mx[q] = max(t.mx[q], wacc[w].mx[q]);
       }
-      acc[3] = (double)m;
+      for (int q = 0; q < 4; ++q) t.m[q] += wacc[w].m[q];
+      t.cnt += wacc[w].cnt;
     }
-    // waves whose particles all belong to one body (the common case: the store is body-sorted)
-    // reduce in registers and issue one set of atomics
-    unsigned long long have = __ballot(b >= 0);
-    if (have == 0ull) continue;
-    int first = __ffsll((long long)have) - 1;
-    int bb = __shfl(b, first, 64);
-    if (__all((b == bb) || (b < 0))) {
-      int cnt = b >= 0 ? 1 : 0;
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-        for (int off = 32; off > 0; off >>= 1) {
-          mn[a] = min(mn[a], (unsigned int)__shfl_down((int)mn[a], off, 64));
-          mx[a] = max(mx[a], (unsigned int)__shfl_down((int)mx[a], off, 64));
-        }
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_down(acc[q], off, 64);
-      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
-      if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          atomicMin(bbox + 6 * bb + a, mn[a]);
-          atomicMax(bbox + 6 * bb + 3 + a, mx[a]);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) atomicAdd(mom + 4 * bb + q, acc[q]);
-        atomicAdd(count + bb, cnt);
-      }
-      continue;
-    }
-    if (b >= 0) {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        atomicMin(bbox + 6 * b + a, mn[a]);
-        atomicMax(bbox + 6 * b + 3 + a, mx[a]);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) atomicAdd(mom + 4 * b + q, acc[q]);
-      atomicAdd(count + b, 1);
-    }
+    body_acc_flush(t, blockBody, bbox, mom, count);
   }
 }
 
@@ -646,32 +790,40 @@ template <class T> __global__ void k_from_dense(const T* src, int ncomp, T* dst,
 // ------------------------------------------------------------ P2G
 // MassSplatter (Grid.cpp:62-99): m_i += m_p * w_ip
 __global__ __launch_bounds__(kBlock) void k_p2g_mass(PView P, Batch B, float* stage, int* err) {
-  __shared__ float acc[kFoot];
+  __shared__ FxFoot<1> acc;
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft);
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) acc[i] = 0.0f;
-  __syncthreads();
-  for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
+  fx_zero(acc);
+  const int k = threadIdx.x;
+  bool have = k < c.ch.z;
+  float m = 0.0f;
+  Stencil st;
+  int rx = 0, ry = 0, rz = 0;
+  if (have) {
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    float m = P.at(C_M, s);
-    Stencil st;
-    int rx, ry, rz;
+    m = P.at(C_M, s);
     if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
-      continue;
+      have = false;
     }
+  }
+  const FxScale sc = fx_scale(have ? fabsf(m) : 0.0f);  // w <= 1
+  if (have) {
+    const float ms = m * sc.S;
 #pragma unroll
     for (int cz = 0; cz < 4; ++cz)
 #pragma unroll
       for (int cy = 0; cy < 4; ++cy)
 #pragma unroll
-        for (int cx = 0; cx < 4; ++cx) atomicAdd(&acc[foot_index(rx + cx, ry + cy, rz + cz)], m * st_w(st, cx, cy, cz));
+        for (int cx = 0; cx < 4; ++cx) {
+          int idx = foot_index(rx + cx, ry + cy, rz + cz);
+          fx_add(&acc.hi[0][idx], &acc.lo[0][idx], ms * st_w(st, cx, cy, cz));
+        }
   }
   __syncthreads();
-  float* dst = stage + (int64_t)blockIdx.x * kStage;
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) dst[(i >> 6) * 192 + (i & 63)] = acc[i];
+  fx_store(acc, sc, stage);
 }
 // mass gather, clamp negative masses, NaN check (Grid.cpp:225-241)
 __global__ void k_gather_mass(Batch B, const float* stage, float* mass, int* err) {
@@ -683,28 +835,36 @@ __global__ void k_gather_mass(Batch B, const float* stage, float* mass, int* err
 // VelocitySplatter (Grid.cpp:102-150): v_i += w (m_p / m_i) (v_p - v_f) where m_i > 0
 __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const float* mass, float* stage, int* err) {
   __shared__ float sm[kFoot];
-  __shared__ float acc[3][kFoot];
+  __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft);
+  fx_zero(acc);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int gt = ft[i >> 6];
     sm[i] = gt >= 0 ? mass[(int64_t)gt * 64 + (i & 63)] : 0.0f;
-    acc[0][i] = acc[1][i] = acc[2][i] = 0.0f;
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
+  const int k = threadIdx.x;
+  bool have = k < c.ch.z;
+  float m = 0.0f, dv[3] = {0.0f, 0.0f, 0.0f};
+  Stencil st;
+  int rx = 0, ry = 0, rz = 0;
+  if (have) {
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    float m = P.at(C_M, s);
-    float dv[3] = {P.at(C_V, s) - c.G.fv[0], P.at(C_V + 1, s) - c.G.fv[1], P.at(C_V + 2, s) - c.G.fv[2]};
-    Stencil st;
-    int rx, ry, rz;
+    m = P.at(C_M, s);
+    dv[0] = P.at(C_V, s) - c.G.fv[0];
+    dv[1] = P.at(C_V + 1, s) - c.G.fv[1];
+    dv[2] = P.at(C_V + 2, s) - c.G.fv[2];
     if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
-      continue;
+      have = false;
     }
+  }
+  // w m_p / m_i <= 1 wherever m_i >= w m_p
+  const FxScale sc = fx_scale(have ? fabsf(dv[0]) + fabsf(dv[1]) + fabsf(dv[2]) : 0.0f);
+  if (have) {
 #pragma unroll
     for (int cz = 0; cz < 4; ++cz)
 #pragma unroll
@@ -714,15 +874,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const floa
           int idx = foot_index(rx + cx, ry + cy, rz + cz);
           float M = sm[idx];
           if (M > 0.0f) {
-            float sc = st_w(st, cx, cy, cz) * (m / M);
-            atomicAdd(&acc[0][idx], sc * dv[0]);
-            atomicAdd(&acc[1][idx], sc * dv[1]);
-            atomicAdd(&acc[2][idx], sc * dv[2]);
+            float f = st_w(st, cx, cy, cz) * (m / M);
+            fx_add(&acc.hi[0][idx], &acc.lo[0][idx], (f * dv[0]) * sc.S);
+            fx_add(&acc.hi[1][idx], &acc.lo[1][idx], (f * dv[1]) * sc.S);
+            fx_add(&acc.hi[2][idx], &acc.lo[2][idx], (f * dv[2]) * sc.S);
           }
         }
   }
   __syncthreads();
-  store_stage3(acc, stage);
+  fx_store(acc, sc, stage);
 }
 // velocity gather, NaN check (Grid.cpp:250-260) and m_prevVelocities = m_velocities (Grid.cpp:262)
 __global__ void k_gather_vel(Batch B, const float* stage, float* vel, float* prev, int* err) {
@@ -776,12 +936,16 @@ __global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const floa
 // ------------------------------------------------------------ forces
 // ForceSplatter (Grid.cpp:387-435): f_i -= V_p P_p F_p^T grad w_ip
 __global__ __launch_bounds__(kBlock) void k_forces(PView P, Batch B, float* stage, int* err) {
-  __shared__ float acc[3][kFoot];
+  __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft);
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) acc[0][i] = acc[1][i] = acc[2][i] = 0.0f;
-  __syncthreads();
-  for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
+  fx_zero(acc);
+  const int k = threadIdx.x;
+  bool have = k < c.ch.z;
+  float fm[9];
+  Stencil st;
+  int rx = 0, ry = 0, rz = 0;
+  if (have) {
     int s = B.perm[c.ch.y + k];
     float x[3], F[9], R[9], Fi[9], Pm[9];
     load_x(P, s, x);
@@ -796,14 +960,17 @@ __global__ __launch_bounds__(kBlock) void k_forces(PView P, Batch B, float* stag
     float vol = P.at(C_VOL, s);
 #pragma unroll
     for (int i = 0; i < 9; ++i) Pm[i] = vol * Pm[i];
-    float fm[9];
     mat_mul_bt(Pm, F, fm);
-    Stencil st;
-    int rx, ry, rz;
     if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
-      continue;
+      have = false;
     }
+  }
+  // |(fm grad w)_a| <= sum |fm| / h
+  const FxScale sc = fx_scale(have ? abs_sum9(fm) / c.G.h : 0.0f);
+  if (have) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fm[i] = -(fm[i] * sc.S);
 #pragma unroll
     for (int cz = 0; cz < 4; ++cz)
 #pragma unroll
@@ -814,13 +981,13 @@ __global__ __launch_bounds__(kBlock) void k_forces(PView P, Batch B, float* stag
           st_grad(st, cx, cy, cz, gw);
           mat_vec(fm, gw, t);
           int idx = foot_index(rx + cx, ry + cy, rz + cz);
-          atomicAdd(&acc[0][idx], -t[0]);
-          atomicAdd(&acc[1][idx], -t[1]);
-          atomicAdd(&acc[2][idx], -t[2]);
+          fx_add(&acc.hi[0][idx], &acc.lo[0][idx], t[0]);
+          fx_add(&acc.hi[1][idx], &acc.lo[1][idx], t[1]);
+          fx_add(&acc.hi[2][idx], &acc.lo[2][idx], t[2]);
         }
   }
   __syncthreads();
-  store_stage3(acc, stage);
+  fx_store(acc, sc, stage);
 }
 
 __device__ __forceinline__ void field_force(const float* grav, int ngrav, float m, float f[3]) {
@@ -899,11 +1066,11 @@ __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, co
   flags[c.i] = (int8_t)fl;
   CRScalars& S = sc[c.b];
   if (bad) atomicOr(&S.errors, ERR_NAN_MOMENTA);
-  for (int off = 32; off > 0; off >>= 1) active += __shfl_down(active, off, 64);
-  if ((threadIdx.x & 63) == 0 && active) atomicAdd(&S.active_nodes, active);
-  double tot[8];
-  if (body_sum<8>(s, c.G, partials, &S.ticket[0], tot) && threadIdx.x == 0) {
+  double sa[9] = {s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7], (double)active};
+  double tot[9];
+  if (body_sum<9>(sa, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0) {
     for (int k = 0; k < 8; ++k) S.sums[k] = tot[k];
+    S.active_nodes = (int)tot[8];
     float gv[3] = {0.0f, 0.0f, 0.0f};
     if (tot[0] == 0.0) {
     } else if (tot[4] == 0.0) {
@@ -936,54 +1103,63 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d
                                                 const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
   if (sc && sc[B.chunks[blockIdx.x].w].done) return;
   __shared__ float sd[3][kFoot];
-  __shared__ float so[3][kFoot];
+  __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft);
+  fx_zero(acc);
   __syncthreads();
   load_foot3(c, ft, d, sd, flags, cols, PROJ);
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) so[0][i] = so[1][i] = so[2][i] = 0.0f;
   __syncthreads();
-  for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
+  const int k = threadIdx.x;
+  bool have = k < c.ch.z;
+  float fm[9];
+  Stencil st;
+  int rx = 0, ry = 0, rz = 0;
+  if (have) {
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    Stencil st;
-    int rx, ry, rz;
     if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
-      continue;
-    }
-    float G[9];
+      have = false;
+    } else {
+      float G[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) G[i] = 0.0f;
+      for (int i = 0; i < 9; ++i) G[i] = 0.0f;
 #pragma unroll
-    for (int cz = 0; cz < 4; ++cz)
+      for (int cz = 0; cz < 4; ++cz)
 #pragma unroll
-      for (int cy = 0; cy < 4; ++cy)
+        for (int cy = 0; cy < 4; ++cy)
 #pragma unroll
-        for (int cx = 0; cx < 4; ++cx) {
-          float gw[3];
-          st_grad(st, cx, cy, cz, gw);
-          int idx = foot_index(rx + cx, ry + cy, rz + cz);
-          float d0 = sd[0][idx], d1 = sd[1][idx], d2 = sd[2][idx];
+          for (int cx = 0; cx < 4; ++cx) {
+            float gw[3];
+            st_grad(st, cx, cy, cz, gw);
+            int idx = foot_index(rx + cx, ry + cy, rz + cz);
+            float d0 = sd[0][idx], d1 = sd[1][idx], d2 = sd[2][idx];
 #pragma unroll
-          for (int cc = 0; cc < 3; ++cc) {
-            G[0 + 3 * cc] += d0 * gw[cc];
-            G[1 + 3 * cc] += d1 * gw[cc];
-            G[2 + 3 * cc] += d2 * gw[cc];
+            for (int cc = 0; cc < 3; ++cc) {
+              G[0 + 3 * cc] += d0 * gw[cc];
+              G[1 + 3 * cc] += d1 * gw[cc];
+              G[2 + 3 * cc] += d2 * gw[cc];
+            }
           }
-        }
-    float F[9], R[9], Fi[9], Gi[9], dF[9], dP[9], fm[9];
-    load_m9(P, C_F, s, F);
-    load_m9(P, C_R, s, R);
-    load_m9(P, C_FI, s, Fi);
-    load_m9(P, C_GI, s, Gi);
-    mat_mul(G, F, dF);
-    snow_stress_differential(dF, R, Fi, Gi, P.at(C_J, s), P.at(C_MU, s), P.at(C_LAM, s), dP);
-    float vol = P.at(C_VOL, s);
+      float F[9], R[9], Fi[9], Gi[9], dF[9], dP[9];
+      load_m9(P, C_F, s, F);
+      load_m9(P, C_R, s, R);
+      load_m9(P, C_FI, s, Fi);
+      load_m9(P, C_GI, s, Gi);
+      mat_mul(G, F, dF);
+      snow_stress_differential(dF, R, Fi, Gi, P.at(C_J, s), P.at(C_MU, s), P.at(C_LAM, s), dP);
+      float vol = P.at(C_VOL, s);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dP[i] = vol * dP[i];
-    mat_mul_bt(dP, F, fm);
+      for (int i = 0; i < 9; ++i) dP[i] = vol * dP[i];
+      mat_mul_bt(dP, F, fm);
+    }
+  }
+  const FxScale scl = fx_scale(have ? abs_sum9(fm) / c.G.h : 0.0f);
+  if (have) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fm[i] = -(fm[i] * scl.S);
 #pragma unroll
     for (int cz = 0; cz < 4; ++cz)
 #pragma unroll
@@ -994,35 +1170,40 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d
           st_grad(st, cx, cy, cz, gw);
           mat_vec(fm, gw, t);
           int idx = foot_index(rx + cx, ry + cy, rz + cz);
-          atomicAdd(&so[0][idx], -t[0]);
-          atomicAdd(&so[1][idx], -t[1]);
-          atomicAdd(&so[2][idx], -t[2]);
+          fx_add(&acc.hi[0][idx], &acc.lo[0][idx], t[0]);
+          fx_add(&acc.hi[1][idx], &acc.lo[1][idx], t[1]);
+          fx_add(&acc.hi[2][idx], &acc.lo[2][idx], t[2]);
         }
   }
   __syncthreads();
-  store_stage3(so, stage);
+  fx_store(acc, scl, stage);
 }
 
 // dFidXiSplatter (Grid.cpp:466-537) through per-particle quadratic forms: with g = F^T grad w,
 // the reference's entry [-V dP(e_a g^T) F^T grad w]_a equals -V g^T H_a g where
 // H_a(c, d) = dP(e_a e_d^T)(a, c) (dP is linear in dF).
 __global__ __launch_bounds__(kBlock) void k_diag(PView P, Batch B, float* stage, int* err) {
-  __shared__ float so[3][kFoot];
+  __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft);
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) so[0][i] = so[1][i] = so[2][i] = 0.0f;
-  __syncthreads();
-  for (int k = threadIdx.x; k < c.ch.z; k += kBlock) {
+  fx_zero(acc);
+  const int k = threadIdx.x;
+  bool have = k < c.ch.z;
+  float F[9], H[3][9];  // H[a][c + 3 d]
+  float bound = 0.0f;
+  Stencil st;
+  int rx = 0, ry = 0, rz = 0;
+  if (have) {
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    float F[9], R[9], Fi[9], Gi[9];
+    float R[9], Fi[9], Gi[9];
     load_m9(P, C_F, s, F);
     load_m9(P, C_R, s, R);
     load_m9(P, C_FI, s, Fi);
     load_m9(P, C_GI, s, Gi);
     float J = P.at(C_J, s), mu = P.at(C_MU, s), lam = P.at(C_LAM, s), vol = P.at(C_VOL, s);
-    float H[3][9];  // H[a][c + 3 d]
+    float hs = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1033,14 +1214,25 @@ __global__ __launch_bounds__(kBlock) void k_diag(PView P, Batch B, float* stage,
         E[a + 3 * dd] = 1.0f;
         snow_stress_differential(E, R, Fi, Gi, J, mu, lam, dP);
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) H[a][cc + 3 * dd] = dP[a + 3 * cc];
+        for (int cc = 0; cc < 3; ++cc) {
+          H[a][cc + 3 * dd] = -vol * dP[a + 3 * cc];
+          hs += fabsf(H[a][cc + 3 * dd]);
+        }
       }
-    Stencil st;
-    int rx, ry, rz;
+    // |g|_inf <= sum |F| / h, so |g^T H_a g| <= sum |H| (sum |F| / h)^2
+    const float gb = abs_sum9(F) / c.G.h;
+    bound = hs * gb * gb;
     if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
-      continue;
+      have = false;
     }
+  }
+  const FxScale sc = fx_scale(have ? bound : 0.0f);
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int i = 0; i < 9; ++i) H[a][i] *= sc.S;
 #pragma unroll
     for (int cz = 0; cz < 4; ++cz)
 #pragma unroll
@@ -1057,12 +1249,12 @@ __global__ __launch_bounds__(kBlock) void k_diag(PView P, Batch B, float* stage,
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
               q += gg[cc] * (H[a][cc] * gg[0] + H[a][cc + 3] * gg[1] + H[a][cc + 6] * gg[2]);
-            atomicAdd(&so[a][idx], -vol * q);
+            fx_add(&acc.hi[a][idx], &acc.lo[a][idx], q);
           }
         }
   }
   __syncthreads();
-  store_stage3(so, stage);
+  fx_store(acc, sc, stage);
 }
 
 // ------------------------------------------------------------ solver node kernels
@@ -1100,7 +1292,7 @@ __global__ void k_rhs(Batch B, const float* mass, const int8_t* flags, const Col
   s[0] = (double)bb[0] * bb[0] + (double)bb[1] * bb[1] + (double)bb[2] * bb[2];
   if (bad) atomicOr(&S.errors, ERR_NAN_MOMENTA);
   double tot[1];
-  if (body_sum<1>(s, c.G, partials, &S.ticket[1], tot) && threadIdx.x == 0) {
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
     S.bnorm2 = tot[0];
     S.threshold = (double)tol * (double)tol * tot[0];
     S.iter = 0;
@@ -1157,7 +1349,7 @@ __global__ void k_cr_setup2(Batch B, const float* mass, const int8_t* flags, con
     s[1] += (double)Ar[a] * (Ar[a] / dg[a]);
   }
   double tot[2];
-  if (body_sum<2>(s, c.G, partials, &S.ticket[2], tot) && threadIdx.x == 0) {
+  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0) {
     S.rz = tot[0];
     S.ApdPAp = tot[1];
     S.iter = 0;
@@ -1197,7 +1389,7 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
   }
   st3(z, c.i, zz);
   double tot[1];
-  if (body_sum<1>(s, c.G, partials, &S.ticket[0], tot) && threadIdx.x == 0) {
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0) {
     int it = S.iter;
     S.rnorm2 = tot[0];
     S.iterations = it + 1;
@@ -1220,7 +1412,7 @@ __global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const C
   st3(Ar, c.i, ar);
   s[0] = (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
   double tot[1];
-  if (body_sum<1>(s, c.G, partials, &S.ticket[1], tot) && threadIdx.x == 0) {
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
     double rz_old = S.rz;
     S.rz = tot[0];
     if (rz_old == 0.0) {
@@ -1254,7 +1446,7 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
   st3(p, c.i, pv);
   st3(Ap, c.i, ap);
   double tot[1];
-  if (body_sum<1>(s, c.G, partials, &S.ticket[2], tot) && threadIdx.x == 0) {
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0) {
     S.ApdPAp = tot[0];
     if (tot[0] == 0.0) {
       S.early_exit = 1;
@@ -1515,6 +1707,9 @@ void launch_fill_column(PView P, int col, float value, int n, hipStream_t s) {
 void launch_fill_int(int* p, int value, int64_t n, hipStream_t s) {
   if (n > 0) LAUNCH(k_fill_int, grid_for(n), p, value, n);
 }
+void launch_iota(int* p, int64_t n, hipStream_t s) {
+  if (n > 0) LAUNCH(k_iota, grid_for(n), p, n);
+}
 void launch_identity_columns(PView P, int col0, int n, hipStream_t s) {
   if (n > 0) LAUNCH(k_identity_columns, grid_for(n), P, col0, n);
 }
@@ -1528,12 +1723,13 @@ void launch_gather_slots(const int* slot_of, const int* orig, int n, int* slots,
 void launch_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count, hipStream_t s) {
   if (n > 0) LAUNCH(k_cc_hash, grid_for(n, 16384), P, n, cell, mask, bucket, rank, count);
 }
-void launch_cc_scatter(const int* bucket, const int* rank, const int* offset, int n, int* sorted, hipStream_t s) {
-  if (n > 0) LAUNCH(k_cc_scatter, grid_for(n, 16384), bucket, rank, offset, n, sorted);
+void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* offset, int n, float4* sorted,
+                       hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_scatter, grid_for(n, 16384), P, bucket, rank, offset, n, sorted);
 }
-void launch_cc_link(PView P, int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const int* sorted, int* parent, int* hasNb, hipStream_t s) {
-  if (n > 0) LAUNCH(k_cc_link, grid_for(n, 16384), P, n, h, cell, mask, offset, count, sorted, parent, hasNb);
+void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                    const float4* sorted, int* parent, int* hasNb, hipStream_t s) {
+  if (n > 0) LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb);
 }
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s) {
@@ -1551,7 +1747,7 @@ void launch_check_pvel(PView P, int n, int* err, hipStream_t s) {
 }
 void launch_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
                         int* count, hipStream_t s) {
-  if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 4096), P, slots, labels, n, bbox, mom, count);
+  if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 512), P, slots, labels, n, bbox, mom, count);
 }
 void launch_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody, hipStream_t s) {
   if (NT > 0) LAUNCH(k_tile_body, grid_for(NT), bodies, nb, NT, tileBody, blockBody);

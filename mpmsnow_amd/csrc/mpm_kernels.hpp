@@ -17,13 +17,16 @@ void launch_permute(PView src, PView dst, const int* new_to_old, int n, const in
                     int* slot_of, hipStream_t s);
 void launch_gather_slots(const int* slot_of, const int* orig, int n, int* slots, hipStream_t s);
 void launch_fill_int(int* p, int value, int64_t n, hipStream_t s);
+void launch_iota(int* p, int64_t n, hipStream_t s);
 
 // ---- body partition (Sim::calculateBodies) on the GPU
 // cell hash with cell edge slightly above h; counts need (mask + 1) ints, zeroed
 void launch_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count, hipStream_t s);
-void launch_cc_scatter(const int* bucket, const int* rank, const int* offset, int n, int* sorted, hipStream_t s);
-void launch_cc_link(PView P, int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const int* sorted, int* parent, int* hasNb, hipStream_t s);
+// sorted: cell-ordered float4 (x, y, z, slot bits)
+void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* offset, int n, float4* sorted,
+                       hipStream_t s);
+void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                    const float4* sorted, int* parent, int* hasNb, hipStream_t s);
 // isRoot[s] = 1 for the root of a component of >= 2 particles; minOrig[root] = smallest caller index
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s);

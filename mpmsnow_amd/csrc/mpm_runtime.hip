@@ -141,7 +141,7 @@ struct GridBatch {
   int64_t NN = 0;
   DevBuf bodies, tileBody, blockBody, perm, key, rank, count, offset, nch, chunkOff, chunks, scantmp, totals;
   DevBuf stageA, stageB;
-  DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, hist, dense,
+  DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, tickets, hist, dense,
       dense2;
   DevBuf bbox, mom, cnt;
   std::vector<SolveRecord> rec;
@@ -159,6 +159,7 @@ struct GridBatch {
     B.tileChunkN = nch.as<int>();
     B.NT = NT;
     B.nblocks = nblocks;
+    B.tickets = tickets.as<unsigned int>();
     return B;
   }
   float* f(DevBuf& d) { return d.as<float>(); }
@@ -417,7 +418,9 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   for (DevBuf* d : {&vel, &prev, &vc, &b, &x, &z, &p, &Ap, &Ar, &diag, &tmp3}) d->ensure(vecBytes());
   scal.ensure(sizeof(CRScalars) * nb);
   status.ensure(sizeof(BatchStatus));
-  partials.ensure(sizeof(double) * 8 * std::max(nblocks, 1));
+  partials.ensure(sizeof(double) * kPartialStride * std::max(nblocks, 1));
+  tickets.ensure(sizeof(unsigned int) * kTicketsPerBody * nb);
+  HIPCHECK(hipMemsetAsync(tickets.p, 0, sizeof(unsigned int) * kTicketsPerBody * nb, s));
   HIPCHECK(hipMemsetAsync(flags.p, 0xFF, NN, s));  // -1: no collision until updateGridVelocities
   rec.assign(nb, SolveRecord{});
   for (int bi = 0; bi < nb; ++bi) rec[bi].np = hb[bi].np;
@@ -656,7 +659,7 @@ void calculate_bodies(mpm_context& c) {
   c.ccRank.ensure(sizeof(int) * n);
   c.ccCount.ensure(sizeof(int) * tsize);
   c.ccOffset.ensure(sizeof(int) * tsize);
-  c.ccSorted.ensure(sizeof(int) * n);
+  c.ccSorted.ensure(sizeof(float4) * n);
   c.ccParent.ensure(sizeof(int) * n);
   c.ccHasNb.ensure(sizeof(int) * n);
   c.ccIsRoot.ensure(sizeof(int) * n);
@@ -668,15 +671,10 @@ void calculate_bodies(mpm_context& c) {
   HIPCHECK(hipMemsetAsync(c.ccCount.p, 0, sizeof(int) * tsize, s));
   launch_cc_hash(c.view(), n, cell, mask, c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccCount.as<int>(), s);
   scan_exclusive(c.ccCount.as<int>(), c.ccOffset.as<int>(), (int)tsize, c.ccTot.as<int>(), c.ccTmp.as<int>(), s);
-  launch_cc_scatter(c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccOffset.as<int>(), n, c.ccSorted.as<int>(), s);
-  // parent = identity
-  {
-    std::vector<int> iota(n);
-    std::iota(iota.begin(), iota.end(), 0);
-    HIPCHECK(hipMemcpyAsync(c.ccParent.p, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    c.sync();
-  }
-  launch_cc_link(c.view(), n, c.h, cell, mask, c.ccOffset.as<int>(), c.ccCount.as<int>(), c.ccSorted.as<int>(),
+  launch_cc_scatter(c.view(), c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccOffset.as<int>(), n,
+                    c.ccSorted.as<float4>(), s);
+  launch_iota(c.ccParent.as<int>(), n, s);
+  launch_cc_link(n, c.h, cell, mask, c.ccOffset.as<int>(), c.ccCount.as<int>(), c.ccSorted.as<float4>(),
                  c.ccParent.as<int>(), c.ccHasNb.as<int>(), s);
   HIPCHECK(hipMemsetAsync(c.ccIsRoot.p, 0, sizeof(int) * n, s));
   launch_fill_int(c.ccMinOrig.as<int>(), INT_MAX, n, s);
