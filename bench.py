@@ -4,20 +4,34 @@ Workload: two snowballs colliding (1M particles, 500k per sphere of r = 0.15 m,
 +/-3 m/s, h = 1/256 m, dt = 5e-4 s, snow E0 = 1.4e5 Pa, nu = 0.2, xi = 10,
 theta_c = 2.5e-2, theta_s = 7.5e-3, gravity, ground plane mu = 0.2, CR relative
 tolerance 1e-5), synthetic seeded particles. A step is one Sim::advance of the
-whole scene (body partition, per-body grid, implicit CR solve, G2P,
+whole scene (body partition, per-body grids, implicit CR solve, G2P,
 plasticity, advection). Prints one JSON line.
 
 value = particle-steps/s over the timed steps (inputs resident in HBM).
 roofline = the dominant kernel (the matrix-free HVP inside CR): algorithmic
-bytes per launch / mean launch time, both measured in this run with HIP events
-on the library's stream, against MI355X's 8 TB/s HBM peak.
+bytes per launch / mean launch time, both measured in the timed steps with HIP
+events on the library's stream (only HVP launches are instrumented there), against
+MI355X's 8 TB/s HBM peak. roofline.traffic = measured HBM bytes per HVP launch,
+read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+(profiles/hvp_two_snowballs_1M_traffic.md, made by `--pmc-summary`).
 cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the
 8-colour voxel partition like the reference's TBB splats) timed on this host
-for a bounded sample: one step of the same 1M-particle scene.
+for a bounded sample: one step of the same 1M-particle scene from the GPU run's
+state.
+
+Other modes:
+  --copy-calibration   also runs one 256 MiB device-to-device copy first (the
+                       known-size kernel that scales the PMC byte counters)
+  --pmc-summary FETCH_CSV WRITE_CSV [--pmc-out FILE]
+                       summarise two `rocprofv3 --pmc FETCH_SIZE` / `--pmc
+                       WRITE_SIZE` runs of `bench.py --copy-calibration` into the
+                       traffic summary file
 """
 import argparse
+import csv
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,6 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hvp_two_snowballs_1M_traffic.md")
+COPY_BYTES = 256 << 20
 
 
 def step_bytes(n_p, n_n, iters):
@@ -54,8 +70,73 @@ def cpu_baseline(sc, gpu_sim, steps=1):
     n = len(sc["m"])
     its = [s["iterations"] for s in sim.last_stats()]
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "%d step(s) of the same %d-particle scene from the GPU run's mid-run state "
-                      "(%.1f s; CR iterations of its largest bodies %s)" % (steps, n, dt, sorted(its)[-2:])}
+            "sample": "%d step(s) of the same %d-particle scene from the GPU run's state (%.1f s; CR "
+                      "iterations of its two largest bodies %s)" % (steps, n, dt, sorted(its)[-2:])}
+
+
+def _pmc_rows(path):
+    """(kernel name, grid size, counter value) per dispatch of a rocprofv3 counter_collection.csv."""
+    rows = []
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            rows.append((r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return rows
+
+
+def pmc_summary(fetch_csv, write_csv, out_path):
+    """HVP HBM traffic per launch from FETCH_SIZE / WRITE_SIZE runs, scaled by a known-size copy."""
+    res = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = _pmc_rows(path)
+        copies = [v for n, g, v in rows if "copyBuffer" in n]
+        if not copies:
+            raise SystemExit("no copy dispatch in %s (run bench.py with --copy-calibration)" % path)
+        copy_kib = max(copies)
+        factor = copy_kib * 1024.0 / COPY_BYTES
+        hvp = [(g, v) for n, g, v in rows if "k_hvp<true>" in n]
+        gmax = max(g for g, _ in hvp)
+        full = [v for g, v in hvp if g == gmax]
+        # full-size launches: every body still iterating (the large bodies' iterations)
+        full = sorted(full)[len(full) // 2:]
+        kib = sum(full) / len(full)
+        res[counter] = {"copy_kib": copy_kib, "factor": factor, "hvp_kib_per_launch": kib,
+                        "hvp_bytes_per_launch": kib * 1024.0 / factor, "launches": len(full)}
+    traffic = res["FETCH_SIZE"]["hvp_bytes_per_launch"] + res["WRITE_SIZE"]["hvp_bytes_per_launch"]
+    summary = {"kernel": "k_hvp<true>", "workload": "two_snowballs_1M", "traffic_bytes_per_launch": traffic,
+               "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
+    lines = [
+        "# HVP HBM traffic, two snowballs (1M particles)",
+        "",
+        "Commands (from the repository root on the GPU host):",
+        "",
+        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration",
+        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration",
+        "    python bench.py --pmc-summary <F>/run_counter_collection.csv <W>/run_counter_collection.csv",
+        "",
+        "The counters are scaled by what they read for a %d MiB device-to-device copy (`__amd_rocclr_copyBuffer`):"
+        % (COPY_BYTES >> 20),
+        "factor = counter KiB x 1024 / copied bytes. HVP figures average the upper half (by value) of the",
+        "full-grid `k_hvp<true>` dispatches (launches in which every body still iterates).",
+        "",
+        "| counter | copy KiB | factor | HVP KiB/launch | HVP bytes/launch (scaled) |",
+        "|---|---|---|---|---|",
+    ]
+    for k in ("FETCH_SIZE", "WRITE_SIZE"):
+        r = res[k]
+        lines.append("| %s | %.0f | %.4f | %.0f | %.4g |" % (k, r["copy_kib"], r["factor"], r["hvp_kib_per_launch"],
+                                                          r["hvp_bytes_per_launch"]))
+    lines += ["", "traffic (FETCH + WRITE) per HVP launch: %.4g bytes" % traffic, "",
+              "<!-- json: %s -->" % json.dumps(summary)]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(json.dumps(summary))
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    m = re.search(r"<!-- json: (.*) -->", open(TRAFFIC_FILE).read())
+    return json.loads(m.group(1))["traffic_bytes_per_launch"] if m else None
 
 
 def main():
@@ -66,7 +147,13 @@ def main():
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.pmc_out)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the step on a single MI355X "
               "(multi-GPU sharding is a design note, not implemented); use --gpus 1", file=sys.stderr)
@@ -77,12 +164,14 @@ def main():
     import mpmsnow_amd as G
     from mpmsnow_amd import scenes
 
-    try:
-        import torch
+    import torch
 
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync = lambda: None  # noqa: E731
+    if args.copy_calibration:
+        a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     sc = scenes.two_snowballs(n=args.particles, seed=1)
     n = len(sc["m"])
@@ -90,34 +179,40 @@ def main():
                 v=sc["v"])
     for _ in range(args.warmup):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    sync()
-    sim.profile(True)
-    iters = []
-    bodies = []
+    torch.cuda.synchronize()
+    # timed region: only HVP launches carry HIP events
+    sim.profile(["hvp"])
+    stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-        iters.append(st["total_iterations"])
-        bodies.append(st["num_bodies"])
-    sync()
+        stats.append(sim.advance(sc["dt"], sc["max_iters"], sc["tol"]))
+        nb = stats[-1]["num_bodies"]
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = sim.profile_query()
-    sim.profile(False)
+    hvp = sim.profile_query()["hvp"]
     elapsed = t1 - t0
     value = n * args.steps / elapsed
-    hvp = prof["hvp"]
     hvp_ms = hvp["ms"] / max(1, hvp["launches"])
     hvp_bytes = hvp["bytes"] / max(1, hvp["launches"])
     achieved = hvp_bytes / (hvp_ms * 1e-3) / 1e9 if hvp_ms > 0 else 0.0
+    # largest body's CR iterations in the last timed step
+    last = [sim.body_stats(b, hist_cap=0) for b in range(nb)]
+    big = max(last, key=lambda r: r["particles"]) if last else {"iterations": 0, "particles": 0, "active_nodes": 0}
+
+    # per-kernel table from two further steps with every kernel class timed (outside the timed region)
+    sim.profile(True)
+    for _ in range(2):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    prof = sim.profile_query()
+    sim.profile(False)
     kernels = {}
     for k, r in prof.items():
         if r["launches"]:
             ms = r["ms"] / r["launches"]
             by = r["bytes"] / r["launches"]
             kernels[k] = {"launches": r["launches"], "ms_per_launch": round(ms, 4),
-                          "gbs": round(by / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
-                          "share_of_step_time": round(r["ms"] / (elapsed * 1e3), 4)}
-    mean_iters = float(np.mean(iters))
+                          "gbs": round(by / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    traffic = read_traffic()
     out = {
         "metric": "particle-steps/s & CR iters/step at 1e-5 residual, learned-PC vs plain CR",
         "value": value,
@@ -133,11 +228,15 @@ def main():
         "data": "synthetic (seeded SplitMix64 particles)",
         "config": {"workload": "two snowballs colliding: 1M particles, h=1/256, dt=5e-4, CR tol 1e-5",
                    "particles": n, "preconditioner": "diagonal (plain CR)", "parallelism": "single GPU"},
-        "cr_iters_per_step": mean_iters,
-        "bodies_per_step": float(np.mean(bodies)),
+        "cr_iters_per_step": float(np.mean([s["total_iterations"] for s in stats])),
+        "cr_iters_note": "sum over the step's bodies; per body: largest body %d iterations (%d particles, %d "
+                         "active nodes) in the last timed step, max over bodies %d"
+                         % (big["iterations"], big["particles"], big["active_nodes"],
+                            max(s["max_iterations"] for s in stats)),
+        "bodies_per_step": float(np.mean([s["num_bodies"] for s in stats])),
         "roofline": {"bound": "hbm", "kernel": "hvp", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": hvp_bytes, "ms_per_launch": hvp_ms},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch": hvp_bytes, "ms_per_launch": hvp_ms, "launches": hvp["launches"]},
         "kernels": kernels,
     }
     if not args.no_cpu:

@@ -72,6 +72,8 @@ typedef struct {
   int history_length;      /* entries written to history */
   int history_capacity;    /* caller-provided capacity of history */
   double* history;         /* caller-provided: |r|^2 per iteration, may be NULL */
+  int particles;           /* particles of the body (grid API: of the grid) */
+  int active_nodes;        /* grid nodes with m > 0 */
 } mpm_solve_stats;
 
 typedef struct {
@@ -162,11 +164,13 @@ int64_t mpm_sim_ballistic(const mpm_context* ctx, int32_t* out);
 int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* stats);
 
 /* ---- measurement (no counterpart in the reference) ----------------------------
- * Kernel timing with HIP events on the context's stream. Kernel classes:
- * p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p.
+ * Kernel timing with HIP events on the context's stream. Kernel classes, in bit
+ * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p.
+ * mask selects the classes to time (bit k = class k; -1 = all, 0 = off); each
+ * timed launch costs two event records, so time only what is needed.
  * algorithmic_bytes sums, over the timed launches, bytes/particle * particles +
  * bytes/node * nodes with m > 0 of that kernel's minimal HBM traffic. */
-int mpm_profile_enable(mpm_context* ctx, int on); /* also resets the counters */
+int mpm_profile_enable(mpm_context* ctx, int mask); /* also resets the counters */
 int mpm_profile_query(mpm_context* ctx, const char* kernel, double* total_ms, int64_t* launches,
                       double* algorithmic_bytes);
 

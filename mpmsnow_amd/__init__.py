@@ -42,7 +42,7 @@ class SolverParams(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("early_exit", C.c_int), ("rhs_norm2", C.c_double),
                 ("residual_norm2", C.c_double), ("history_length", C.c_int), ("history_capacity", C.c_int),
-                ("history", _dp)]
+                ("history", _dp), ("particles", C.c_int), ("active_nodes", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -165,7 +165,9 @@ def make_params(max_iters, tol, termination=None, debug=None, cancelled=None):
 
 def _stats_dict(st, hist):
     return {"iterations": st.iterations, "early_exit": bool(st.early_exit), "bnorm2": st.rhs_norm2,
-            "residual_norm2_last": st.residual_norm2, "residual_norm2": hist[:st.history_length].copy()}
+            "residual_norm2_last": st.residual_norm2,
+            "residual_norm2": hist[:st.history_length].copy() if hist is not None else None,
+            "particles": st.particles, "active_nodes": st.active_nodes}
 
 
 class Context:
@@ -402,10 +404,11 @@ class Sim:
         return a
 
     def body_stats(self, b, hist_cap=4096):
-        hist = np.zeros(hist_cap)
+        """Solve statistics of body b in the last advance; hist_cap = 0 skips the residual history."""
+        hist = np.zeros(hist_cap) if hist_cap > 0 else None
         st = SolveStats()
         st.history_capacity = hist_cap
-        st.history = hist.ctypes.data_as(_dp)
+        st.history = hist.ctypes.data_as(_dp) if hist is not None else None
         rc = lib().mpm_sim_body_solve_stats(self.ctx.ptr, b, C.byref(st))
         if rc != MPM_OK:
             raise MpmError("no solve statistics for body %d" % b)
@@ -415,7 +418,16 @@ class Sim:
         return self.ctx.get(name)
 
     def profile(self, on=True):
-        self.ctx.check(lib().mpm_profile_enable(self.ctx.ptr, int(on)))
+        """Time kernel classes with HIP events: True = all, False = off, or an iterable of names."""
+        if on is True:
+            mask = -1
+        elif not on:
+            mask = 0
+        else:
+            mask = 0
+            for k in on:
+                mask |= 1 << PROFILE_KERNELS.index(k)
+        self.ctx.check(lib().mpm_profile_enable(self.ctx.ptr, mask))
 
     def profile_query(self):
         out = {}

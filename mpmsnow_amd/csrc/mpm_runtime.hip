@@ -220,7 +220,7 @@ struct mpm_context {
     hipEvent_t a, b;
     const GridBatch* batch;
   };
-  bool profOn = false;
+  unsigned int profMask = 0;  // kernel classes being timed
   std::vector<ProfRec> profPending;
   std::vector<hipEvent_t> eventPool;
   double profMs[PK_COUNT] = {};
@@ -269,7 +269,7 @@ struct mpm_context {
     return e;
   }
   int prof_begin(int kind, const GridBatch* g, int iter = -1) {
-    if (!profOn) return -1;
+    if (!(profMask & (1u << kind))) return -1;
     ProfRec r{kind, iter, new_event(), new_event(), g};
     HIPCHECK(hipEventRecord(r.a, stream));
     profPending.push_back(r);
@@ -730,6 +730,8 @@ void ensure_body_lists(const mpm_context& cc) {
 
 void fill_stats(const SolveRecord& r, mpm_solve_stats* st) {
   if (!st) return;
+  st->particles = r.np;
+  st->active_nodes = r.active;
   st->iterations = r.iterations;
   st->early_exit = r.early_exit;
   st->rhs_norm2 = r.rhs_norm2;
@@ -1268,10 +1270,10 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
   });
 }
 
-int mpm_profile_enable(mpm_context* ctx, int on) {
+int mpm_profile_enable(mpm_context* ctx, int mask) {
   return guarded(ctx, [&] {
     ctx->prof_resolve();
-    ctx->profOn = on != 0;
+    ctx->profMask = (unsigned int)mask;
     for (int k = 0; k < PK_COUNT; ++k) {
       ctx->profMs[k] = 0;
       ctx->profCount[k] = 0;
@@ -1327,7 +1329,8 @@ int mpm_sim_body_solve_stats(const mpm_context* ctx, int b, mpm_solve_stats* st)
   try {
     HIPCHECK(hipSetDevice(c.device));
     SolveRecord& r = c.lastSolves[b];
-    if (r.history.empty() && r.iterations > 0) c.simBatch.fetch_history(b, r);
+    if (st && st->history && st->history_capacity > 0 && r.history.empty() && r.iterations > 0)
+      c.simBatch.fetch_history(b, r);
   } catch (const std::exception& e) {
     c.err = e.what();
     return MPM_ERR_HIP;
