@@ -74,6 +74,7 @@ typedef struct {
   double* history;         /* caller-provided: |r|^2 per iteration, may be NULL */
   int particles;           /* particles of the body (grid API: of the grid) */
   int active_nodes;        /* grid nodes with m > 0 */
+  int fallbacks;           /* learned preconditioner: 1 if the body switched to the diagonal one */
 } mpm_solve_stats;
 
 typedef struct {
@@ -163,9 +164,34 @@ int64_t mpm_sim_ballistic(const mpm_context* ctx, int32_t* out);
 /* solve statistics of body b in the last advance */
 int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* stats);
 
+/* ---- learned preconditioner (no counterpart in the reference) ----------------
+ * A BF16 3-D convolutional network (MFMA, FP32 accumulation) maps a body's grid residual to
+ * a velocity correction inside a flexible conjugate-residual solve; it attaches where
+ * ConjugateResiduals takes a preconditioner (ConjugateResiduals.h:20, ProceduralMatrix.h:16).
+ * A body whose residual fails to fall in an iteration that used the network switches to the
+ * diagonal preconditioner for the rest of its solve; the termination test is unchanged. */
+int64_t mpm_lpc_num_params(void);
+/* random (He) initialisation; the output layer starts at zero */
+int mpm_lpc_init(mpm_context* ctx, uint64_t seed);
+int mpm_lpc_set_params(mpm_context* ctx, const float* params, int64_t n);
+int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n);
+/* 0 = diagonal-preconditioned CR (the reference's solver), 1 = learned flexible CR */
+int mpm_lpc_set_mode(mpm_context* ctx, int mode);
+/* the next mpm_sim_advance (diagonal CR) keeps up to `samples` (residual, exact correction)
+ * pairs of its solve: r_k = b - A x_k at chosen iterations k and e_k = x* - x_k */
+int mpm_lpc_capture(mpm_context* ctx, int samples);
+/* Adam over the captured pairs with the library's forward and backward kernels; loss = mean
+ * relative squared error of the last epoch */
+int mpm_lpc_train(mpm_context* ctx, int epochs, float learning_rate, double* loss);
+/* grid API (one body; dense arrays in node order): network output for residual r, diagonal
+ * diag and input scale, and the loss w |out - e / scale|^2 with its parameter gradient */
+int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float scale, float* out);
+int mpm_grid_lpc_gradients(mpm_grid* grid, const float* r, const float* diag, const float* e, float scale,
+                           float weight, float* grad, double* loss);
+
 /* ---- measurement (no counterpart in the reference) ----------------------------
  * Kernel timing with HIP events on the context's stream. Kernel classes, in bit
- * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p.
+ * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p lpc_forward.
  * mask selects the classes to time (bit k = class k; -1 = all, 0 = off); each
  * timed launch costs two event records, so time only what is needed.
  * algorithmic_bytes sums, over the timed launches, bytes/particle * particles +

@@ -42,7 +42,7 @@ class SolverParams(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("early_exit", C.c_int), ("rhs_norm2", C.c_double),
                 ("residual_norm2", C.c_double), ("history_length", C.c_int), ("history_capacity", C.c_int),
-                ("history", _dp), ("particles", C.c_int), ("active_nodes", C.c_int)]
+                ("history", _dp), ("particles", C.c_int), ("active_nodes", C.c_int), ("fallbacks", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -95,12 +95,23 @@ SIGNATURES = {
     "mpm_sim_body": (C.c_int64, [C.c_void_p, C.c_int, _ip]),
     "mpm_sim_ballistic": (C.c_int64, [C.c_void_p, _ip]),
     "mpm_sim_body_solve_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolveStats)]),
+    "mpm_lpc_num_params": (C.c_int64, []),
+    "mpm_lpc_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "mpm_lpc_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
+    "mpm_lpc_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
+    "mpm_lpc_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_lpc_capture": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_lpc_train": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.POINTER(C.c_double)]),
+    "mpm_grid_lpc_apply": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, _fp]),
+    "mpm_grid_lpc_gradients": (C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_float, C.c_float, _fp,
+                                         C.POINTER(C.c_double)]),
     "mpm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double)]),
 }
 
-PROFILE_KERNELS = ("p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p", "cr_update_x", "cr_ar", "cr_update_p")
+PROFILE_KERNELS = ("p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p", "cr_update_x", "cr_ar", "cr_update_p",
+                   "lpc_forward")
 
 
 def lib():
@@ -167,7 +178,7 @@ def _stats_dict(st, hist):
     return {"iterations": st.iterations, "early_exit": bool(st.early_exit), "bnorm2": st.rhs_norm2,
             "residual_norm2_last": st.residual_norm2,
             "residual_norm2": hist[:st.history_length].copy() if hist is not None else None,
-            "particles": st.particles, "active_nodes": st.active_nodes}
+            "particles": st.particles, "active_nodes": st.active_nodes, "fallbacks": st.fallbacks}
 
 
 class Context:
@@ -270,6 +281,31 @@ class Context:
         a, p = _f(g.reshape(-1))
         self.check(lib().mpm_set_gravity(self.ptr, p, len(g)))
 
+    # ---- learned preconditioner
+    def lpc_init(self, seed=1):
+        self.check(lib().mpm_lpc_init(self.ptr, seed))
+
+    def lpc_params(self):
+        n = lib().mpm_lpc_num_params()
+        out = np.zeros(n, dtype=np.float32)
+        self.check(lib().mpm_lpc_get_params(self.ptr, out.ctypes.data_as(_fp), n))
+        return out
+
+    def lpc_set_params(self, params):
+        a, p = _f(params)
+        self.check(lib().mpm_lpc_set_params(self.ptr, p, len(a)))
+
+    def lpc_mode(self, mode):
+        self.check(lib().mpm_lpc_set_mode(self.ptr, int(mode)))
+
+    def lpc_capture(self, samples):
+        self.check(lib().mpm_lpc_capture(self.ptr, int(samples)))
+
+    def lpc_train(self, epochs, lr):
+        loss = C.c_double()
+        self.check(lib().mpm_lpc_train(self.ptr, int(epochs), float(lr), C.byref(loss)))
+        return loss.value
+
 
 class Grid:
     """Grid of one body (reference include/MpmSim/Grid.h); construction splats mass and velocity."""
@@ -353,6 +389,25 @@ class Grid:
 
     def update_deformation_gradients(self, dt):
         self.ctx.check(lib().mpm_grid_update_deformation_gradients(self.ptr, dt))
+
+    def lpc_apply(self, r, diag, scale):
+        """Learned-preconditioner network output (dense (n, 3)) for residual r and diagonal diag."""
+        ra, rp = _f(r)
+        da, dp = _f(diag)
+        out = np.zeros(ra.size, dtype=np.float32)
+        self.ctx.check(lib().mpm_grid_lpc_apply(self.ptr, rp, dp, float(scale), out.ctypes.data_as(_fp)))
+        return out.reshape(-1, 3)
+
+    def lpc_gradients(self, r, diag, e, scale, weight):
+        """Loss weight |out - e / scale|^2 and its gradient with respect to the network parameters."""
+        ra, rp = _f(r)
+        da, dp = _f(diag)
+        ea, ep = _f(e)
+        g = np.zeros(lib().mpm_lpc_num_params(), dtype=np.float32)
+        loss = C.c_double()
+        self.ctx.check(lib().mpm_grid_lpc_gradients(self.ptr, rp, dp, ep, float(scale), float(weight),
+                                                    g.ctypes.data_as(_fp), C.byref(loss)))
+        return loss.value, g
 
 
 class Sim:

@@ -129,7 +129,22 @@ struct CRScalars {
   int max_iters;
   int errors;              // ErrorBits
   int active_nodes;        // nodes with m > 0 (measurement)
+  // flexible conjugate residual with the learned preconditioner (truncated GCR over kFcrM
+  // previous directions)
+  double gbeta[2];         // <A z, A p_j> / <A p_j, A p_j>
+  double ApAp[2];          // <A p_j, A p_j> of the stored directions
+  double rAp;              // <r, A p> of the new direction
+  double rr_prev;          // |r|^2 before the last update
+  double lpc_s2;           // sum over active nodes of (r / diag)^2, and their count (3 per node)
+  double lpc_cnt;
+  float lpc_scale;         // RMS of r / diag over active nodes: network input scale
+  int lpc_off;             // 1: this body uses the diagonal preconditioner
+  int fallbacks;           // iterations that switched this body to the diagonal preconditioner
+  int ndirs;               // directions stored (<= kFcrM)
+  int slot;                // ring slot of the most recent direction
+  int retry;               // 1: the network gave no direction; redo the iteration on the diagonal
 };
+constexpr int kFcrM = 2;
 
 // Counters shared by all bodies of a batch.
 struct BatchStatus {

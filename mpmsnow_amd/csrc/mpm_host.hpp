@@ -1,0 +1,62 @@
+// mpm_host.hpp -- host-side helpers shared by the runtime and the learned preconditioner:
+// error types, HIP status checks, growable device and pinned buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace mpmh {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+struct InvalidArg : std::runtime_error {
+  explicit InvalidArg(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define HIPCHECK(x)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) throw HipError(std::string("hip: ") + hipGetErrorName(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) HIPCHECK(hipFree(p));
+    p = nullptr;
+    size_t c = std::max<size_t>(bytes + bytes / 4, 256);  // headroom: grids grow slowly step to step
+    HIPCHECK(hipMalloc(&p, c));
+    cap = c;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) HIPCHECK(hipHostFree(p));
+    p = nullptr;
+    HIPCHECK(hipHostMalloc(&p, bytes));
+    cap = bytes;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace mpmh

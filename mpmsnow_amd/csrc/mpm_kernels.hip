@@ -1457,6 +1457,212 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
     S.iter = S.iter + 1;
   }
 }
+// ------------------------------------------------------------ flexible conjugate residual
+// The learned preconditioner changes from one iteration to the next, which breaks the
+// short recurrences of ConjugateResiduals.cpp:84-151. The solve then runs a truncated
+// generalised conjugate residual: each preconditioned vector z is A-orthogonalised against
+// the last kFcrM directions (in the Euclidean inner product of A p), and x, r take the step
+// that minimises |r|. The same termination (|r|^2 < tol^2 |b|^2 or the iteration cap,
+// SquareMagnitudeTermination.cpp:19-28) judges the result. A body whose residual fails to
+// fall by the factor kLpcFallback in an iteration that used the network switches to the
+// diagonal preconditioner z = r / diag for the rest of the solve.
+constexpr double kLpcFallback = 0.95;
+
+// r = b - A x0 ; network input scale ; iteration state
+__global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                           const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
+                           int netOn, CRScalars* sc, BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
+  const float dt2 = dt * dt;
+  float xv[3], df[3] = {0.0f, 0.0f, 0.0f}, Ax[3], bb[3], dg[3], rr[3];
+  ld3(x, c.i, xv);
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
+  const float m = mass[c.i];
+  apply_A(c, cols, flags[c.i], m, xv, df, dt2, Ax);
+  ld3(b, c.i, bb);
+  ld3(diag, c.i, dg);
+  double s[3] = {0.0, 0.0, m > 0.0f ? 3.0 : 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rr[a] = bb[a] - Ax[a];
+    s[0] += (double)rr[a] * rr[a];
+    if (m > 0.0f) {
+      const double q = (double)(rr[a] / dg[a]);
+      s[1] += q * q;
+    }
+  }
+  st3(r, c.i, rr);
+  double tot[3];
+  if (body_sum<3>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0) {
+    S.rnorm2 = tot[0];
+    S.rr_prev = tot[0];
+    S.lpc_s2 = tot[1];
+    S.lpc_cnt = tot[2];
+    S.lpc_scale = tot[2] > 0.0 ? (float)sqrt(tot[1] / tot[2]) : 0.0f;
+    S.lpc_off = netOn ? 0 : 1;
+    S.fallbacks = 0;
+    S.retry = 0;
+    S.ndirs = 0;
+    S.slot = kFcrM - 1;
+    for (int j = 0; j < kFcrM; ++j) {
+      S.ApAp[j] = 0.0;
+      S.gbeta[j] = 0.0;
+    }
+    S.iter = 0;
+    S.iterations = 0;
+    S.early_exit = 0;
+  }
+}
+// z = r / diag for bodies on the diagonal preconditioner
+__global__ void k_fcr_diag_z(Batch B, const float* r, const float* diag, float* z, const CRScalars* sc) {
+  NodeCtx c = node_begin(B);
+  const CRScalars& S = sc[c.b];
+  if (S.done || !S.lpc_off) return;
+  float rr[3], dg[3], zz[3];
+  ld3(r, c.i, rr);
+  ld3(diag, c.i, dg);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) zz[a] = rr[a] / dg[a];
+  st3(z, c.i, zz);
+}
+// A z and its projections on the stored directions' A p_j
+__global__ void k_fcr_az(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
+                         const float* z, float dt, float* Az, const float* Q0, const float* Q1, CRScalars* sc,
+                         BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
+  const float dt2 = dt * dt;
+  float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, az[3];
+  ld3(z, c.i, zz);
+  if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
+  apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, az);
+  st3(Az, c.i, az);
+  double s[2] = {0.0, 0.0};
+  const float* Q[2] = {Q0, Q1};
+#pragma unroll
+  for (int j = 0; j < kFcrM; ++j)
+    if (S.ApAp[j] != 0.0) {
+      float q[3];
+      ld3(Q[j], c.i, q);
+      s[j] = (double)az[0] * q[0] + (double)az[1] * q[1] + (double)az[2] * q[2];
+    }
+  double tot[2];
+  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
+    for (int j = 0; j < kFcrM; ++j) S.gbeta[j] = S.ApAp[j] != 0.0 ? tot[j] / S.ApAp[j] : 0.0;
+  }
+}
+// new direction p = z - sum beta_j p_j, A p = A z - sum beta_j A p_j into the ring's next slot;
+// alpha = <r, A p> / <A p, A p>
+__global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float* r, float* P0, float* P1, float* Q0,
+                           float* Q1, CRScalars* sc, BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
+  const int slot = (S.slot + 1) % kFcrM;
+  float* P[2] = {P0, P1};
+  float* Q[2] = {Q0, Q1};
+  float p[3], ap[3], rr[3];
+  ld3(z, c.i, p);
+  ld3(Az, c.i, ap);
+  ld3(r, c.i, rr);
+#pragma unroll
+  for (int j = 0; j < kFcrM; ++j) {
+    const float bj = (float)S.gbeta[j];
+    if (bj != 0.0f) {
+      float pj[3], qj[3];
+      ld3(P[j], c.i, pj);
+      ld3(Q[j], c.i, qj);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        p[a] -= bj * pj[a];
+        ap[a] -= bj * qj[a];
+      }
+    }
+  }
+  st3(P[slot], c.i, p);
+  st3(Q[slot], c.i, ap);
+  double s[2] = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    s[0] += (double)ap[a] * ap[a];
+    s[1] += (double)rr[a] * ap[a];
+  }
+  double tot[2];
+  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0) {
+    if (tot[0] == 0.0 && !S.lpc_off) {
+      // the network returned nothing usable: switch to the diagonal and redo the iteration
+      S.lpc_off = 1;
+      S.fallbacks += 1;
+      S.retry = 1;
+      S.ApAp[slot] = 0.0;  // this ring slot now holds no direction
+    } else if (tot[0] == 0.0) {
+      S.early_exit = 1;
+      set_done(S, bst);
+    } else {
+      S.retry = 0;
+      S.ApAp[slot] = tot[0];
+      S.rAp = tot[1];
+      S.alpha = tot[1] / tot[0];
+      S.slot = slot;
+      S.ndirs = S.ndirs < kFcrM ? S.ndirs + 1 : kFcrM;
+    }
+  }
+}
+// x = P(x + alpha p), r -= alpha A p; termination, fallback and the next network input scale
+__global__ void k_fcr_update(Batch B, const int8_t* flags, const ColliderSet* cols, const float* mass,
+                             const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
+                             float* x, float* r, CRScalars* sc, BatchStatus* bst, double* partials, double* hist) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done || S.retry) return;
+  const float alpha = (float)S.alpha;
+  const float* P[2] = {P0, P1};
+  const float* Q[2] = {Q0, Q1};
+  float xv[3], p[3], ap[3], rr[3], dg[3];
+  ld3(x, c.i, xv);
+  ld3(P[S.slot], c.i, p);
+  ld3(Q[S.slot], c.i, ap);
+  ld3(r, c.i, rr);
+  ld3(diag, c.i, dg);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) xv[a] = xv[a] + alpha * p[a];
+  node_project(c, cols, flags[c.i], xv);
+  st3(x, c.i, xv);
+  const float m = mass[c.i];
+  double s[3] = {0.0, 0.0, m > 0.0f ? 3.0 : 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rr[a] = rr[a] - alpha * ap[a];
+    s[0] += (double)rr[a] * rr[a];
+    if (m > 0.0f) {
+      const double q = (double)(rr[a] / dg[a]);
+      s[1] += q * q;
+    }
+  }
+  st3(r, c.i, rr);
+  double tot[3];
+  if (body_sum<3>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0) {
+    const int it = S.iter;
+    S.rnorm2 = tot[0];
+    S.iterations = it + 1;
+    if (hist && it < kHistCap) hist[(int64_t)c.b * kHistCap + it] = tot[0];
+    S.lpc_s2 = tot[1];
+    S.lpc_cnt = tot[2];
+    S.lpc_scale = tot[2] > 0.0 ? (float)sqrt(tot[1] / tot[2]) : 0.0f;
+    if (it >= S.max_iters || tot[0] < S.threshold) {
+      set_done(S, bst);
+    } else if (!S.lpc_off && tot[0] > kLpcFallback * S.rr_prev) {
+      S.lpc_off = 1;
+      S.fallbacks += 1;
+    }
+    S.rr_prev = tot[0];
+    S.iter = it + 1;
+  }
+}
+
 __global__ void k_residual_vector(Batch B, const float* z, const float* diag, float* r) {
   NodeCtx c = node_begin(B);
   float zz[3], dg[3];
@@ -1888,6 +2094,30 @@ void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const 
 void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
                         CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
   NODE_LAUNCH(k_cr_update_p, B, z, Ar, diag, p, Ap, sc, st, partials);
+}
+void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                     const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
+                     int netOn, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, sc, st, partials);
+}
+void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
+                       hipStream_t s) {
+  NODE_LAUNCH(k_fcr_diag_z, B, r, diag, z, sc);
+}
+void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                   const float* stage, const float* z, float dt, float* Az, const float* Q0, const float* Q1,
+                   CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_az, B, mass, flags, cols, stage, z, dt, Az, Q0, Q1, sc, st, partials);
+}
+void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, float* P0, float* P1,
+                     float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_orth, B, z, Az, r, P0, P1, Q0, Q1, sc, st, partials);
+}
+void launch_fcr_update(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* mass,
+                       const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
+                       float* x, float* r, CRScalars* sc, BatchStatus* st, double* partials, double* hist,
+                       hipStream_t s) {
+  NODE_LAUNCH(k_fcr_update, B, flags, cols, mass, diag, P0, P1, Q0, Q1, x, r, sc, st, partials, hist);
 }
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s) {
   NODE_LAUNCH(k_residual_vector, B, z, diag, r);

@@ -102,6 +102,21 @@ void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const 
                   hipStream_t s);
 void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
                         CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+// ---- flexible conjugate residual (learned preconditioner path)
+void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                     const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
+                     int netOn, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
+                       hipStream_t s);
+void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                   const float* stage, const float* z, float dt, float* Az, const float* Q0, const float* Q1,
+                   CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, float* P0, float* P1,
+                     float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+void launch_fcr_update(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* mass,
+                       const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
+                       float* x, float* r, CRScalars* sc, BatchStatus* st, double* partials, double* hist,
+                       hipStream_t s);
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s);
 void launch_finalize(const Batch& B, const float* x, const float* vc, float* vel, const CRScalars* sc,
                      hipStream_t s);

@@ -24,60 +24,14 @@
 #include <vector>
 
 #include "../../include/mpmsnow_gpu.h"
+#include "lpc_net.hpp"
+#include "mpm_host.hpp"
 #include "mpm_kernels.hpp"
 
 using namespace mpm;
+using namespace mpmh;
 
 namespace {
-
-struct HipError : std::runtime_error {
-  explicit HipError(const std::string& s) : std::runtime_error(s) {}
-};
-struct InvalidArg : std::runtime_error {
-  explicit InvalidArg(const std::string& s) : std::runtime_error(s) {}
-};
-
-#define HIPCHECK(x)                                                                   \
-  do {                                                                                \
-    hipError_t e_ = (x);                                                              \
-    if (e_ != hipSuccess) throw HipError(std::string("hip: ") + hipGetErrorName(e_)); \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  void ensure(size_t bytes) {
-    if (bytes <= cap) return;
-    if (p) HIPCHECK(hipFree(p));
-    p = nullptr;
-    size_t c = std::max<size_t>(bytes + bytes / 4, 256);  // headroom: grids grow slowly step to step
-    HIPCHECK(hipMalloc(&p, c));
-    cap = c;
-  }
-  template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-struct PinnedBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  ~PinnedBuf() {
-    if (p) (void)hipHostFree(p);
-  }
-  void ensure(size_t bytes) {
-    if (bytes <= cap) return;
-    if (p) HIPCHECK(hipHostFree(p));
-    p = nullptr;
-    HIPCHECK(hipHostMalloc(&p, bytes));
-    cap = bytes;
-  }
-  template <class T> T* as() const { return static_cast<T*>(p); }
-};
 
 struct VarInfo {
   int col;
@@ -115,17 +69,21 @@ void raise_errors(int e) {
 // instrumented launch, resolved at the next synchronisation point. Algorithmic bytes of a
 // launch = bytes/particle x particles + bytes/node x nodes with m > 0, over the bodies the
 // launch worked on (bodies whose solve had finished skip the launch).
-enum ProfKind { PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_COUNT };
+enum ProfKind {
+  PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_LPC, PK_COUNT
+};
 const char* const kProfNames[PK_COUNT] = {"p2g_mass", "p2g_vel", "forces", "hvp", "diag",
-                                          "g2p", "cr_update_x", "cr_ar", "cr_update_p"};
-const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0};
-const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84};
+                                          "g2p", "cr_update_x", "cr_ar", "cr_update_p", "lpc_forward"};
+// the network is MFMA work: its launches are timed but carry no HBM byte count
+const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0, 0};
+const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84, 0};
 
 }  // namespace
 
 struct SolveRecord {
   int iterations = 0;
   int early_exit = 0;
+  int fallbacks = 0;  // learned preconditioner: switched to the diagonal one
   double rhs_norm2 = 0, residual_norm2 = 0;
   int np = 0, active = 0;
   std::vector<double> history;
@@ -144,6 +102,10 @@ struct GridBatch {
   DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, tickets, hist, dense,
       dense2;
   DevBuf bbox, mom, cnt;
+  DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
+  LpcLevels levels;
+  bool levelsBuilt = false;
+  uint64_t version = 0;  // bumped by every build
   std::vector<SolveRecord> rec;
 
   Batch view() const {
@@ -171,6 +133,11 @@ struct GridBatch {
   void forces_stage();
   void hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter);
   void solve(float dt, const mpm_solver_params* prm);
+  void ensure_levels() {
+    if (!levelsBuilt) levels.build(hb, ctx_stream());
+    levelsBuilt = true;
+  }
+  hipStream_t ctx_stream() const;
   void to_dense(int body, const float* batch, int ncomp, float* host);
   void from_dense(int body, const float* host, int ncomp, float* batch);
   void fetch_history(int body, SolveRecord& r);
@@ -180,6 +147,20 @@ struct mpm_grid {
   mpm_context* ctx = nullptr;
   GridBatch batch;
   DevBuf slots;
+};
+
+// (residual, exact correction) pairs of one solve, for training the learned preconditioner
+struct LpcCapture {
+  int want = 0;                  // samples the next solve keeps
+  int n = 0;                     // samples kept
+  std::vector<int> iters;        // CR iteration of each sample
+  DevBuf data;                   // [sample][r, e][3 NN floats]
+  DevBuf scale, weight, sums;    // [sample][body]
+  const GridBatch* batch = nullptr;
+  uint64_t version = 0;
+  int64_t NN = 0;
+  float* r(int k) { return data.as<float>() + (int64_t)k * 6 * NN; }
+  float* e(int k) { return data.as<float>() + (int64_t)k * 6 * NN + 3 * NN; }
 };
 
 struct mpm_context {
@@ -213,6 +194,11 @@ struct mpm_context {
   DevBuf ccBucket, ccRank, ccCount, ccOffset, ccSorted, ccParent, ccHasNb, ccIsRoot, ccRootIdx, ccMinOrig, ccRootMin,
       ccRootRank, ccTmp, ccTot, newToOld, ballPos;
   int iterHint = 8;
+  int iterHintFcr = 8;
+  // learned preconditioner
+  LpcNet net;
+  int lpcMode = 0;  // 0: diagonal-preconditioned CR; 1: learned flexible CR
+  LpcCapture cap;
   // measurement
   struct ProfRec {
     int kind;
@@ -311,6 +297,8 @@ struct mpm_context {
   }
 };
 
+hipStream_t GridBatch::ctx_stream() const { return ctx->stream; }
+
 // ------------------------------------------------------------------ GridBatch
 // Grid::Grid (Grid.cpp:156-263) for every body of the batch: bbox and frame per body,
 // geometry, binning into the batch's tiles, mass and velocity splat.
@@ -375,6 +363,8 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   NT = tileBase;
   nblocks = NT / 4;
   NN = (int64_t)NT * 64;
+  levelsBuilt = false;
+  ++version;
   bodies.ensure(sizeof(BodyGrid) * nb);
   HIPCHECK(hipMemcpyAsync(bodies.p, hb.data(), sizeof(BodyGrid) * nb, hipMemcpyHostToDevice, s));
   tileBody.ensure(sizeof(int) * NT);
@@ -506,94 +496,191 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
   c.prof_end(pr);
   launch_rhs(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(stageB), dt, tol, f(b), f(x), f(diag), sc(), st(),
              partials.as<double>(), s);
-  // CR setup (ConjugateResiduals.cpp:50-76)
-  hvp(f(x), true, f(stageA), sc(), -1);
-  launch_cr_setup1(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(), s);
-  hvp(f(z), true, f(stageA), sc(), -1);
-  launch_cr_setup2(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
-                   partials.as<double>(), s);
   BatchStatus* hst = reinterpret_cast<BatchStatus*>(reinterpret_cast<char*>(c.pinned.p) + scb);
   auto fetch_status = [&]() {
     HIPCHECK(hipMemcpyAsync(hst, status.p, sizeof(BatchStatus), hipMemcpyDeviceToHost, s));
     c.sync();
   };
-  fetch_status();
-  {
+  auto check_errflag = [&]() {
     int e = 0;
     HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     c.sync();
     raise_errors(e);
-  }
-  // one CR iteration (ConjugateResiduals.cpp:84-151) for every body still iterating
-  auto iterate_x = [&](int it) {
-    int px = c.prof_begin(PK_CR_X, this, it);
-    launch_cr_update_x(B, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
-                       partials.as<double>(), hist.as<double>(), s);
-    c.prof_end(px);
-  };
-  auto iterate_rest = [&](int it) {
-    hvp(f(z), true, f(stageA), sc(), it);
-    int pa = c.prof_begin(PK_CR_AR, this, it);
-    launch_cr_ar(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), sc(), st(),
-                 partials.as<double>(), s);
-    c.prof_end(pa);
-    int pp = c.prof_begin(PK_CR_P, this, it);
-    launch_cr_update_p(B, f(z), f(Ar), f(diag), f(p), f(Ap), sc(), st(), partials.as<double>(), s);
-    c.prof_end(pp);
-  };
-  auto set_done_flag = [&](int bi, int v) {
-    hsc[bi].done = v;
-    HIPCHECK(hipMemcpyAsync(&sc()[bi].done, &hsc[bi].done, sizeof(int), hipMemcpyHostToDevice, s));
-    c.sync();
   };
   const int64_t iterCap = (prm && prm->termination) ? 1000000 : (int64_t)maxIters + 2;
-  if (!hooks) {
-    // all bodies iterate in lockstep; the host polls the done counter once per batch of iterations
+  const bool learned = c.lpcMode == 1 && !hooks;
+  if (learned) {
+    // flexible conjugate residual with the learned preconditioner (mpm_kernels.hip, k_fcr_*)
+    for (DevBuf* d : {&fr, &fp0, &fp1, &fq0, &fq1}) d->ensure(vecBytes());
+    const int netOn = c.net.initialised ? 1 : 0;
+    hvp(f(x), true, f(stageA), sc(), -1);
+    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, sc(),
+                    st(), partials.as<double>(), s);
+    fetch_status();
+    check_errflag();
+    bool netActive = netOn != 0;
+    if (netActive) {
+      ensure_levels();
+      c.net.alloc_acts(levels, false);
+    }
     int it = 0;
     while (hst->bodies_done < nb) {
       if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
-      int batch = it == 0 ? std::max(1, c.iterHint) : 8;
+      int batch = it == 0 ? std::max(1, c.iterHintFcr) : 8;
       for (int k = 0; k < batch; ++k, ++it) {
-        iterate_x(it);
-        iterate_rest(it);
+        if (netActive) {
+          int pn = c.prof_begin(PK_LPC, this, it);
+          c.net.forward(B, levels, f(fr), f(diag), f(mass), nullptr, sc(), nullptr, f(z), s);
+          c.prof_end(pn);
+        }
+        launch_fcr_diag_z(B, f(fr), f(diag), f(z), sc(), s);
+        hvp(f(z), true, f(stageA), sc(), it);
+        launch_fcr_az(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), f(fq0), f(fq1), sc(), st(),
+                      partials.as<double>(), s);
+        launch_fcr_orth(B, f(z), f(Ar), f(fr), f(fp0), f(fp1), f(fq0), f(fq1), sc(), st(), partials.as<double>(), s);
+        launch_fcr_update(B, flags.as<int8_t>(), c.cols(), f(mass), f(diag), f(fp0), f(fp1), f(fq0), f(fq1), f(x),
+                          f(fr), sc(), st(), partials.as<double>(), hist.as<double>(), s);
       }
       fetch_status();
+      if (netActive) {
+        // the network runs only while some unfinished body still uses it
+        HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
+        c.sync();
+        netActive = false;
+        for (int bi = 0; bi < nb; ++bi)
+          if (!hsc[bi].done && !hsc[bi].lpc_off) netActive = true;
+      }
     }
   } else {
-    // host hooks see the bodies one after another, as the reference solves them: bodies still
-    // iterating are parked (done = 2, which every kernel skips) and released one at a time
-    HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
-    c.sync();
-    std::vector<int> pending;
-    for (int bi = 0; bi < nb; ++bi)
-      if (!hsc[bi].done) {
-        pending.push_back(bi);
-        set_done_flag(bi, 2);
-      }
-    std::vector<float> hostvec;
-    for (int bi : pending) {
-      set_done_flag(bi, 0);
-      const BodyGrid& G = hb[bi];
-      const int64_t nd = 3LL * G.n[0] * G.n[1] * G.n[2];
-      hostvec.resize(nd);
-      for (int it = 0;; ++it) {
+    // CR setup (ConjugateResiduals.cpp:50-76)
+    hvp(f(x), true, f(stageA), sc(), -1);
+    launch_cr_setup1(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
+                     s);
+    hvp(f(z), true, f(stageA), sc(), -1);
+    launch_cr_setup2(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
+                     partials.as<double>(), s);
+    fetch_status();
+    check_errflag();
+    // training pairs: iterate x_k and residual r_k = M z_k at the iterations of kCaptureIters
+    static const int kCaptureIters[] = {0, 1, 2, 3, 4, 6, 8, 11, 15, 20, 26, 33, 41, 50, 60, 72};
+    LpcCapture& cap = c.cap;
+    const int capWant = hooks ? 0 : std::min<int>(cap.want, (int)(sizeof(kCaptureIters) / sizeof(int)));
+    if (capWant > 0) {
+      cap.NN = NN;
+      cap.data.ensure(sizeof(float) * 6 * (size_t)NN * capWant);
+      cap.iters.clear();
+      cap.n = 0;
+    }
+    auto snapshot = [&](int iterIndex) {
+      if (cap.n >= capWant || kCaptureIters[cap.n] != iterIndex) return;
+      HIPCHECK(hipMemcpyAsync(cap.e(cap.n), x.p, vecBytes(), hipMemcpyDeviceToDevice, s));
+      launch_residual_vector(B, f(z), f(diag), cap.r(cap.n), s);
+      cap.iters.push_back(iterIndex);
+      ++cap.n;
+    };
+    if (capWant > 0) snapshot(0);
+    // one CR iteration (ConjugateResiduals.cpp:84-151) for every body still iterating
+    auto iterate_x = [&](int it) {
+      int px = c.prof_begin(PK_CR_X, this, it);
+      launch_cr_update_x(B, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
+                         partials.as<double>(), hist.as<double>(), s);
+      c.prof_end(px);
+      if (capWant > 0) snapshot(it + 1);
+    };
+    auto iterate_rest = [&](int it) {
+      hvp(f(z), true, f(stageA), sc(), it);
+      int pa = c.prof_begin(PK_CR_AR, this, it);
+      launch_cr_ar(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), sc(), st(),
+                   partials.as<double>(), s);
+      c.prof_end(pa);
+      int pp = c.prof_begin(PK_CR_P, this, it);
+      launch_cr_update_p(B, f(z), f(Ar), f(diag), f(p), f(Ap), sc(), st(), partials.as<double>(), s);
+      c.prof_end(pp);
+    };
+    auto set_done_flag = [&](int bi, int v) {
+      hsc[bi].done = v;
+      HIPCHECK(hipMemcpyAsync(&sc()[bi].done, &hsc[bi].done, sizeof(int), hipMemcpyHostToDevice, s));
+      c.sync();
+    };
+    if (!hooks) {
+      // all bodies iterate in lockstep; the host polls the done counter once per batch of iterations
+      int it = 0;
+      while (hst->bodies_done < nb) {
         if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
-        iterate_x(it);
-        HIPCHECK(hipMemcpyAsync(&hsc[bi], &sc()[bi], sizeof(CRScalars), hipMemcpyDeviceToHost, s));
-        c.sync();
-        if (hsc[bi].iterations == it + 1) {  // this x update happened
-          if (prm->debug) {
-            to_dense(bi, f(x), 3, hostvec.data());
-            prm->debug(prm->user, hostvec.data(), nd);
-          }
-          if (prm->termination && !hsc[bi].done) {
-            launch_residual_vector(B, f(z), f(diag), f(tmp3), s);
-            to_dense(bi, f(tmp3), 3, hostvec.data());
-            if (prm->termination(prm->user, hostvec.data(), nd, it)) set_done_flag(bi, 1);
-          }
+        int batch = it == 0 ? std::max(1, c.iterHint) : 8;
+        for (int k = 0; k < batch; ++k, ++it) {
+          iterate_x(it);
+          iterate_rest(it);
         }
-        if (hsc[bi].done) break;
-        iterate_rest(it);
+        fetch_status();
+      }
+      if (capWant > 0) {
+        // e_k = x* - x_k (x* = the converged solution), per-body input scale and loss weight
+        for (int k = 0; k < cap.n; ++k) launch_sub_f32(f(x), cap.e(k), cap.e(k), 3 * NN, s);
+        cap.sums.ensure(sizeof(double) * 3 * nb * cap.n);
+        HIPCHECK(hipMemsetAsync(cap.sums.p, 0, sizeof(double) * 3 * nb * cap.n, s));
+        for (int k = 0; k < cap.n; ++k)
+          launch_lpc_sample_stats(B, cap.r(k), f(diag), f(mass), cap.e(k), cap.sums.as<double>() + 3 * nb * k, s);
+        std::vector<double> sums(3 * nb * cap.n);
+        HIPCHECK(hipMemcpyAsync(sums.data(), cap.sums.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
+        c.sync();
+        std::vector<float> hs(nb * cap.n), hw(nb * cap.n);
+        for (int k = 0; k < cap.n; ++k)
+          for (int bi = 0; bi < nb; ++bi) {
+            const double* q = &sums[3 * (nb * k + bi)];
+            const double sc2 = q[1] > 0.0 ? q[0] / q[1] : 0.0;
+            const double E = sc2 > 0.0 ? q[2] / sc2 : 0.0;
+            // bodies of at least 256 particles that were still iterating at this sample
+            const bool use = hb[bi].np >= 256 && hsc[bi].iterations > cap.iters[k] && E > 0.0 && !hsc[bi].zero_x;
+            hs[nb * k + bi] = (float)std::sqrt(sc2);
+            hw[nb * k + bi] = use ? (float)(1.0 / E) : 0.0f;
+          }
+        cap.scale.ensure(sizeof(float) * hs.size());
+        cap.weight.ensure(sizeof(float) * hw.size());
+        HIPCHECK(hipMemcpyAsync(cap.scale.p, hs.data(), sizeof(float) * hs.size(), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(cap.weight.p, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice, s));
+        c.sync();
+        cap.batch = this;
+        cap.version = version;
+        cap.want = 0;
+      }
+    } else {
+      // host hooks see the bodies one after another, as the reference solves them: bodies still
+      // iterating are parked (done = 2, which every kernel skips) and released one at a time
+      HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
+      c.sync();
+      std::vector<int> pending;
+      for (int bi = 0; bi < nb; ++bi)
+        if (!hsc[bi].done) {
+          pending.push_back(bi);
+          set_done_flag(bi, 2);
+        }
+      std::vector<float> hostvec;
+      for (int bi : pending) {
+        set_done_flag(bi, 0);
+        const BodyGrid& G = hb[bi];
+        const int64_t nd = 3LL * G.n[0] * G.n[1] * G.n[2];
+        hostvec.resize(nd);
+        for (int it = 0;; ++it) {
+          if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
+          iterate_x(it);
+          HIPCHECK(hipMemcpyAsync(&hsc[bi], &sc()[bi], sizeof(CRScalars), hipMemcpyDeviceToHost, s));
+          c.sync();
+          if (hsc[bi].iterations == it + 1) {  // this x update happened
+            if (prm->debug) {
+              to_dense(bi, f(x), 3, hostvec.data());
+              prm->debug(prm->user, hostvec.data(), nd);
+            }
+            if (prm->termination && !hsc[bi].done) {
+              launch_residual_vector(B, f(z), f(diag), f(tmp3), s);
+              to_dense(bi, f(tmp3), 3, hostvec.data());
+              if (prm->termination(prm->user, hostvec.data(), nd, it)) set_done_flag(bi, 1);
+            }
+          }
+          if (hsc[bi].done) break;
+          iterate_rest(it);
+        }
       }
     }
   }
@@ -608,11 +695,12 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     r.rhs_norm2 = hsc[bi].bnorm2;
     r.residual_norm2 = hsc[bi].rnorm2;
     r.active = hsc[bi].active_nodes;
+    r.fallbacks = learned ? hsc[bi].fallbacks : 0;
     r.history.clear();
     maxIt = std::max(maxIt, r.iterations);
     raise_errors(hsc[bi].errors);
   }
-  if (maxIt > 0) c.iterHint = maxIt;
+  if (maxIt > 0) (learned ? c.iterHintFcr : c.iterHint) = maxIt;
 }
 
 namespace {
@@ -730,6 +818,7 @@ void ensure_body_lists(const mpm_context& cc) {
 
 void fill_stats(const SolveRecord& r, mpm_solve_stats* st) {
   if (!st) return;
+  st->fallbacks = r.fallbacks;
   st->particles = r.np;
   st->active_nodes = r.active;
   st->iterations = r.iterations;
@@ -1266,6 +1355,120 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
       stats->num_bodies = c.nbodies;
       stats->num_ballistic = c.nballistic;
       stats->cancelled = cancelled ? 1 : 0;
+    }
+  });
+}
+
+// ------------------------------------------------------------------ learned preconditioner
+int64_t mpm_lpc_num_params(void) { return LpcNet().nparams; }
+
+int mpm_lpc_init(mpm_context* ctx, uint64_t seed) {
+  return guarded(ctx, [&] { ctx->net.init(seed, ctx->stream); });
+}
+
+int mpm_lpc_set_params(mpm_context* ctx, const float* params, int64_t n) {
+  return guarded(ctx, [&] {
+    if (!params || n != ctx->net.nparams) throw InvalidArg("parameter count mismatch");
+    ctx->net.set(params, ctx->stream);
+  });
+}
+
+int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n) {
+  return guarded(ctx, [&] {
+    if (!params || n != ctx->net.nparams) throw InvalidArg("parameter count mismatch");
+    ctx->net.get(params, ctx->stream);
+  });
+}
+
+int mpm_lpc_set_mode(mpm_context* ctx, int mode) {
+  return guarded(ctx, [&] {
+    if (mode != 0 && mode != 1) throw InvalidArg("mode must be 0 (diagonal CR) or 1 (learned flexible CR)");
+    ctx->lpcMode = mode;
+  });
+}
+
+int mpm_lpc_capture(mpm_context* ctx, int samples) {
+  return guarded(ctx, [&] {
+    if (samples < 0 || samples > 16) throw InvalidArg("at most 16 samples per solve");
+    ctx->cap.want = samples;
+  });
+}
+
+int mpm_lpc_train(mpm_context* ctx, int epochs, float learning_rate, double* loss) {
+  return guarded(ctx, [&] {
+    mpm_context& c = *ctx;
+    LpcCapture& cap = c.cap;
+    GridBatch& B = c.simBatch;
+    if (cap.n == 0 || cap.batch != &B || cap.version != B.version)
+      throw InvalidArg("no captured pairs (call mpm_lpc_capture before mpm_sim_advance)");
+    if (!c.net.initialised) c.net.init(1, c.stream);
+    B.ensure_levels();
+    const Batch v = B.view();
+    std::vector<float> hw((size_t)B.nb * cap.n);
+    HIPCHECK(hipMemcpyAsync(hw.data(), cap.weight.p, sizeof(float) * hw.size(), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    double last = 0.0;
+    for (int ep = 0; ep < epochs; ++ep) {
+      double sum = 0.0;
+      int used = 0;
+      for (int k = 0; k < cap.n; ++k) {
+        bool any = false;
+        for (int bi = 0; bi < B.nb; ++bi) any = any || hw[(size_t)B.nb * k + bi] > 0.0f;
+        if (!any) continue;
+        sum += c.net.gradients(v, B.levels, cap.r(k), B.f(B.diag), B.f(B.mass), cap.e(k),
+                               cap.scale.as<float>() + (size_t)B.nb * k, cap.weight.as<float>() + (size_t)B.nb * k,
+                               c.stream);
+        c.net.adam(learning_rate, c.stream);
+        ++used;
+      }
+      last = used ? sum / used : 0.0;
+    }
+    c.sync();
+    if (loss) *loss = last;
+  });
+}
+
+namespace {
+// dense single-body inputs of the grid API into the batch layout
+void grid_lpc_inputs(mpm_grid* grid, const float* r, const float* diag, const float* e, float scale, float weight) {
+  GridBatch& B = grid->batch;
+  for (DevBuf* d : {&B.fr, &B.fp0, &B.fp1}) d->ensure(B.vecBytes());
+  B.from_dense(0, r, 3, B.f(B.fr));
+  B.from_dense(0, diag, 3, B.f(B.fp0));
+  if (e) B.from_dense(0, e, 3, B.f(B.fp1));
+  float hv[2] = {scale, weight};
+  B.fq0.ensure(sizeof(float) * 2);
+  HIPCHECK(hipMemcpyAsync(B.fq0.p, hv, sizeof(hv), hipMemcpyHostToDevice, grid->ctx->stream));
+  B.ensure_levels();
+}
+}  // namespace
+
+int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float scale, float* out) {
+  if (!grid) return MPM_ERR_INVALID;
+  return guarded(grid->ctx, [&] {
+    mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
+    grid_lpc_inputs(grid, r, diag, nullptr, scale, 0.0f);
+    c.net.alloc_acts(B.levels, true);
+    c.net.forward(B.view(), B.levels, B.f(B.fr), B.f(B.fp0), B.f(B.mass), B.fq0.as<float>(), nullptr,
+                  c.net.raw.as<float>(), nullptr, c.stream);
+    B.to_dense(0, c.net.raw.as<float>(), 3, out);
+  });
+}
+
+int mpm_grid_lpc_gradients(mpm_grid* grid, const float* r, const float* diag, const float* e, float scale,
+                           float weight, float* grad, double* loss) {
+  if (!grid) return MPM_ERR_INVALID;
+  return guarded(grid->ctx, [&] {
+    mpm_context& c = *grid->ctx;
+    GridBatch& B = grid->batch;
+    grid_lpc_inputs(grid, r, diag, e, scale, weight);
+    const double l = c.net.gradients(B.view(), B.levels, B.f(B.fr), B.f(B.fp0), B.f(B.mass), B.f(B.fp1),
+                                     B.fq0.as<float>(), B.fq0.as<float>() + 1, c.stream);
+    if (loss) *loss = l;
+    if (grad) {
+      HIPCHECK(hipMemcpyAsync(grad, c.net.G.p, sizeof(float) * c.net.nparams, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
     }
   });
 }
