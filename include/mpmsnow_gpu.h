@@ -78,11 +78,12 @@ typedef struct {
 } mpm_solve_stats;
 
 typedef struct {
-  int num_bodies;
+  int num_bodies;          /* bodies for the next step (after the re-partition) */
   int num_ballistic;
   int total_iterations;
   int max_iterations;
   int cancelled;
+  int num_solved;          /* bodies solved in this step (valid indices of mpm_sim_body_solve_stats) */
 } mpm_step_stats;
 
 /* ---- context ------------------------------------------------------------ */
@@ -177,12 +178,15 @@ int mpm_lpc_set_params(mpm_context* ctx, const float* params, int64_t n);
 int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n);
 /* 0 = diagonal-preconditioned CR (the reference's solver), 1 = learned flexible CR */
 int mpm_lpc_set_mode(mpm_context* ctx, int mode);
-/* the next mpm_sim_advance (diagonal CR) keeps up to `samples` (residual, exact correction)
- * pairs of its solve: r_k = b - A x_k at chosen iterations k and e_k = x* - x_k */
-int mpm_lpc_capture(mpm_context* ctx, int samples);
-/* Adam over the captured pairs with the library's forward and backward kernels; loss = mean
- * relative squared error of the last epoch */
-int mpm_lpc_train(mpm_context* ctx, int epochs, float learning_rate, double* loss);
+/* training: the next mpm_sim_advance (diagonal CR) keeps up to `samples` residuals
+ * r_k = b - A x_k of its solve (at chosen iterations k) and, before the particle update, takes
+ * `epochs` Adam passes over them with the library's forward and backward kernels, minimising
+ * per body the residual left by the optimal step along the correction,
+ * 1 - <r, A z>_D^2 / (|r|_D^2 |A z|_D^2) (D = diagonal, z = r / diag + scale * network(r)):
+ * the exact correction A^-1 r gives 0. */
+int mpm_lpc_train_next(mpm_context* ctx, int samples, int epochs, float learning_rate);
+/* mean per-body loss of that training's first and last epoch, and of the diagonal step alone */
+int mpm_lpc_last_loss(mpm_context* ctx, double* first, double* last, double* diagonal);
 /* grid API (one body; dense arrays in node order): network output for residual r, diagonal
  * diag and input scale, and the loss w |out - e / scale|^2 with its parameter gradient */
 int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float scale, float* out);

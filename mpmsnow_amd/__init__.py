@@ -47,7 +47,7 @@ class SolveStats(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("num_bodies", C.c_int), ("num_ballistic", C.c_int), ("total_iterations", C.c_int),
-                ("max_iterations", C.c_int), ("cancelled", C.c_int)]
+                ("max_iterations", C.c_int), ("cancelled", C.c_int), ("num_solved", C.c_int)]
 
 
 class MpmError(RuntimeError):
@@ -100,8 +100,9 @@ SIGNATURES = {
     "mpm_lpc_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
     "mpm_lpc_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
     "mpm_lpc_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
-    "mpm_lpc_capture": (C.c_int, [C.c_void_p, C.c_int]),
-    "mpm_lpc_train": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.POINTER(C.c_double)]),
+    "mpm_lpc_train_next": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
+    "mpm_lpc_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
     "mpm_grid_lpc_apply": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, _fp]),
     "mpm_grid_lpc_gradients": (C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_float, C.c_float, _fp,
                                          C.POINTER(C.c_double)]),
@@ -298,13 +299,15 @@ class Context:
     def lpc_mode(self, mode):
         self.check(lib().mpm_lpc_set_mode(self.ptr, int(mode)))
 
-    def lpc_capture(self, samples):
-        self.check(lib().mpm_lpc_capture(self.ptr, int(samples)))
+    def lpc_train_next(self, samples, epochs, lr):
+        """The next advance trains the network on `samples` residuals of its solve."""
+        self.check(lib().mpm_lpc_train_next(self.ptr, int(samples), int(epochs), float(lr)))
 
-    def lpc_train(self, epochs, lr):
-        loss = C.c_double()
-        self.check(lib().mpm_lpc_train(self.ptr, int(epochs), float(lr), C.byref(loss)))
-        return loss.value
+    def lpc_last_loss(self):
+        """(first epoch, last epoch, diagonal step alone) mean residual loss of the last training."""
+        a, b, d = C.c_double(), C.c_double(), C.c_double()
+        self.check(lib().mpm_lpc_last_loss(self.ptr, C.byref(a), C.byref(b), C.byref(d)))
+        return a.value, b.value, d.value
 
 
 class Grid:
@@ -441,7 +444,7 @@ class Sim:
         self.ctx.check(lib().mpm_sim_advance(self.ctx.ptr, dt, C.byref(prm), C.byref(st)))
         return {"num_bodies": st.num_bodies, "num_ballistic": st.num_ballistic,
                 "total_iterations": st.total_iterations, "max_iterations": st.max_iterations,
-                "cancelled": bool(st.cancelled)}
+                "cancelled": bool(st.cancelled), "num_solved": st.num_solved}
 
     def bodies(self):
         out = []

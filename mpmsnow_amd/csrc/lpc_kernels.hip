@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_conv(LevelView V, const bf16* __re
         if (co < 3) {
           const int64_t zi = ((int64_t)tile * 3 + co) * 64 + vox;
           if (o.raw) o.raw[zi] = val;
-          if (o.z) o.z[zi] = val * o.sc[b].lpc_scale;
+          if (o.z) o.z[zi] = o.r[zi] / o.diag[zi] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * o.sc[b].lpc_scale;
         }
       } else {
         const bf16 hv = (bf16)val;
@@ -347,9 +347,9 @@ __global__ void k_transpose_flip(const bf16* W, bf16* WT, int cin, int cout) {
   }
 }
 __global__ void k_adam(float* p, const float* g, float* m, float* v, bf16* pb, int64_t n, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
+                       float b2, float eps, float c1, float c2, float gscale) {
   LPC_STRIDE(i, n) {
-    const float gi = g[i];
+    const float gi = g[i] * gscale;
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -382,8 +382,8 @@ __device__ __forceinline__ bool node_valid(const BodyGrid& G, int T, int l) {
   const int tx = lt % G.T[0], ty = (lt / G.T[0]) % G.T[1], tz = lt / (G.T[0] * G.T[1]);
   return tx * 4 + (l & 3) < G.n[0] && ty * 4 + ((l >> 2) & 3) < G.n[1] && tz * 4 + (l >> 4) < G.n[2];
 }
-__global__ void k_lpc_features(Batch B, const float* r, const float* diag, const float* mass, const float* scale,
-                               const CRScalars* sc, bf16* in) {
+__global__ void k_lpc_features(Batch B, const float* r, const float* diag, const float* mass, const int8_t* flags,
+                               const float* scale, const CRScalars* sc, bf16* in) {
   // one block per node block (4 tiles = 256 nodes); 16 channels per node
   const int b = B.blockBody[blockIdx.x];
   if (sc && (sc[b].done || sc[b].lpc_off)) return;
@@ -400,16 +400,23 @@ __global__ void k_lpc_features(Batch B, const float* r, const float* diag, const
   }
   if (node_valid(G, T, l)) {
     const int64_t vb = (int64_t)T * 192 + l;
+    const float m = mass[i];
+    const float col = flags && flags[i] != -1 ? 1.0f : 0.0f;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) lo[c] = (bf16)(r[vb + 64 * c] / diag[vb + 64 * c] * inv);
-    lo[3] = (bf16)(mass[i] > 0.0f ? 1.0f : 0.0f);
+    for (int c = 0; c < 3; ++c) {
+      const float q = fminf(fmaxf(r[vb + 64 * c] / diag[vb + 64 * c] * inv, -kLpcClamp), kLpcClamp);
+      lo[c] = (bf16)q;
+      lo[3 + c] = (bf16)(q * (m / diag[vb + 64 * c]));
+      if (c < 2) lo[6 + c] = (bf16)(q * col);
+      else hi[0] = (bf16)(q * col);
+    }
   }
   bf16x8* dst = reinterpret_cast<bf16x8*>(in + i * 16);
   dst[0] = lo;
   dst[1] = hi;
 }
-__global__ void k_lpc_out_grad(Batch B, const float* raw, const float* e, const float* scale, const float* weight,
-                               bf16* g, double* loss) {
+__global__ void k_lpc_out_grad(Batch B, const float* raw, const float* e, const float* r, const float* diag,
+                               const float* scale, const float* weight, bf16* g, double* loss) {
   const int b = B.blockBody[blockIdx.x];
   const BodyGrid G = B.bodies[b];
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -427,7 +434,8 @@ __global__ void k_lpc_out_grad(Batch B, const float* raw, const float* e, const 
     const int64_t vb = (int64_t)T * 192 + l;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float d = raw[vb + 64 * c] - e[vb + 64 * c] * inv;
+      const float t = fminf(fmaxf((e[vb + 64 * c] - r[vb + 64 * c] / diag[vb + 64 * c]) * inv, -kLpcClamp), kLpcClamp);
+      const float d = raw[vb + 64 * c] - t;
       lo[c] = (bf16)(2.0f * wb * d);
       ls += (double)wb * (double)d * (double)d;
     }
@@ -442,22 +450,22 @@ __global__ void k_lpc_out_grad(Batch B, const float* raw, const float* e, const 
 __global__ void k_sub_f32(const float* a, const float* b, float* out, int64_t n) {
   LPC_STRIDE(i, n) out[i] = a[i] - b[i];
 }
-// per-body sums over a captured pair: (r / diag)^2 and 3 x count over nodes with m > 0, and e^2
-__global__ void k_lpc_sample_stats(Batch B, const float* r, const float* diag, const float* mass, const float* e,
-                                   double* sums) {
+// per-body sums over a captured residual: (r / diag)^2 and 3 x count over nodes with m > 0, and
+// r^2 / diag over all nodes of the grid
+__global__ void k_lpc_sample_stats(Batch B, const float* r, const float* diag, const float* mass, double* sums) {
   const int b = B.blockBody[blockIdx.x];
   const BodyGrid G = B.bodies[b];
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const int T = (int)(i >> 6), l = (int)(i & 63);
   double s[3] = {0.0, 0.0, 0.0};
-  if (node_valid(G, T, l) && mass[i] > 0.0f) {
+  if (node_valid(G, T, l)) {
     const int64_t vb = (int64_t)T * 192 + l;
     for (int c = 0; c < 3; ++c) {
-      const double q = (double)(r[vb + 64 * c] / diag[vb + 64 * c]);
-      s[0] += q * q;
-      s[2] += (double)e[vb + 64 * c] * (double)e[vb + 64 * c];
+      const float qf = r[vb + 64 * c] / diag[vb + 64 * c];
+      s[2] += (double)r[vb + 64 * c] * (double)qf;
+      if (mass[i] > 0.0f) s[0] += (double)qf * (double)qf;
     }
-    s[1] = 3.0;
+    if (mass[i] > 0.0f) s[1] = 3.0;
   }
   __shared__ double red[3][kThreads / 64];
   for (int k = 0; k < 3; ++k) {
@@ -550,8 +558,10 @@ void launch_transpose_flip(const bf16* W, bf16* WT, int cin, int cout, hipStream
                      cout);
 }
 void launch_adam(float* p, const float* g, float* m, float* v, bf16* pb, int64_t n, float lr, float b1, float b2,
-                 float eps, float c1, float c2, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_adam, dim3(grid1d(n)), dim3(kThreads), 0, s, p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2);
+                 float eps, float c1, float c2, float gscale, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_adam, dim3(grid1d(n)), dim3(kThreads), 0, s, p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2,
+                       gscale);
 }
 void launch_f32_to_bf16(const float* in, bf16* out, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid1d(n)), dim3(kThreads), 0, s, in, out, n);
@@ -559,23 +569,33 @@ void launch_f32_to_bf16(const float* in, bf16* out, int64_t n, hipStream_t s) {
 void launch_level_tile_body(const LevelGrid* bodies, int nb, int NT, int* tileBody, hipStream_t s) {
   if (NT > 0) hipLaunchKernelGGL(k_level_tile_body, dim3(grid1d(NT)), dim3(kThreads), 0, s, bodies, nb, NT, tileBody);
 }
-void launch_lpc_features(const Batch& B, const float* r, const float* diag, const float* mass, const float* scale,
-                         const CRScalars* sc, bf16* in, hipStream_t s) {
+void launch_lpc_features(const Batch& B, const float* r, const float* diag, const float* mass, const int8_t* flags,
+                         const float* scale, const CRScalars* sc, bf16* in, hipStream_t s) {
   if (B.nblocks > 0)
-    hipLaunchKernelGGL(k_lpc_features, dim3(B.nblocks), dim3(kThreads), 0, s, B, r, diag, mass, scale, sc, in);
+    hipLaunchKernelGGL(k_lpc_features, dim3(B.nblocks), dim3(kThreads), 0, s, B, r, diag, mass, flags, scale, sc, in);
+}
+__global__ void k_sumsq(const float* g, int64_t n, double* out) {
+  double s = 0.0;
+  LPC_STRIDE(i, n) s += (double)g[i] * (double)g[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(out, s);
+}
+void launch_sumsq(const float* g, int64_t n, double* out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_sumsq, dim3(grid1d(n, 1024)), dim3(kThreads), 0, s, g, n, out);
 }
 void launch_sub_f32(const float* a, const float* b, float* out, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_sub_f32, dim3(grid1d(n)), dim3(kThreads), 0, s, a, b, out, n);
 }
-void launch_lpc_sample_stats(const Batch& B, const float* r, const float* diag, const float* mass, const float* e,
-                             double* sums, hipStream_t s) {
+void launch_lpc_sample_stats(const Batch& B, const float* r, const float* diag, const float* mass, double* sums,
+                             hipStream_t s) {
   if (B.nblocks > 0)
-    hipLaunchKernelGGL(k_lpc_sample_stats, dim3(B.nblocks), dim3(kThreads), 0, s, B, r, diag, mass, e, sums);
+    hipLaunchKernelGGL(k_lpc_sample_stats, dim3(B.nblocks), dim3(kThreads), 0, s, B, r, diag, mass, sums);
 }
-void launch_lpc_out_grad(const Batch& B, const float* raw, const float* e, const float* scale, const float* weight,
-                         bf16* g, double* loss, hipStream_t s) {
+void launch_lpc_out_grad(const Batch& B, const float* raw, const float* e, const float* r, const float* diag,
+                         const float* scale, const float* weight, bf16* g, double* loss, hipStream_t s) {
   if (B.nblocks > 0)
-    hipLaunchKernelGGL(k_lpc_out_grad, dim3(B.nblocks), dim3(kThreads), 0, s, B, raw, e, scale, weight, g, loss);
+    hipLaunchKernelGGL(k_lpc_out_grad, dim3(B.nblocks), dim3(kThreads), 0, s, B, raw, e, r, diag, scale, weight, g,
+                       loss);
 }
 
 }  // namespace mpm

@@ -44,8 +44,8 @@ struct ConvOut {
   const bf16* skip;    // added after the activation when out2 is set
   bf16* out2;          // out + skip
   float* raw;          // FINAL: network output channels 0..2, [tile][3][64] (may be null)
-  float* z;            // FINAL: preconditioned vector, [tile][3][64] (may be null)
-  const float* r;      // FINAL: residual, for bodies that use the diagonal preconditioner
+  float* z;            // FINAL: preconditioned vector r / diag + scale out, [tile][3][64] (may be null)
+  const float* r;      // FINAL: residual and diagonal (with z)
   const float* diag;
   const CRScalars* sc;  // per-body scalars: done bodies are skipped, scale and fallback read
 };
@@ -73,24 +73,30 @@ void launch_relu_mask(bf16* g, const bf16* a, int64_t n, hipStream_t s);
 void launch_add_bf16(const bf16* a, const bf16* b, bf16* out, int64_t n, hipStream_t s);
 // WT[ci][(26 - off) COUT + co] = W[co][off CIN + ci]  (weights of the transposed, flipped conv)
 void launch_transpose_flip(const bf16* W, bf16* WT, int cin, int cout, hipStream_t s);
-// Adam on fp32 parameters; writes the bf16 copy
+// Adam on fp32 parameters (gradient multiplied by gscale first); writes the bf16 copy
 void launch_adam(float* p, const float* g, float* m, float* v, bf16* pb, int64_t n, float lr, float b1, float b2,
-                 float eps, float c1, float c2, hipStream_t s);
+                 float eps, float c1, float c2, float gscale, hipStream_t s);
+// *out += sum g^2
+void launch_sumsq(const float* g, int64_t n, double* out, hipStream_t s);
 void launch_f32_to_bf16(const float* in, bf16* out, int64_t n, hipStream_t s);
 // per-level tile -> body table
 void launch_level_tile_body(const LevelGrid* bodies, int nb, int NT, int* tileBody, hipStream_t s);
-// network input at level 1: channels (r / diag) / scale (3), occupancy (1), zeros (12); the
-// per-body scale comes from `scale` when given, else from sc[b].lpc_scale (bodies that are done
-// or use the diagonal preconditioner are skipped when sc is given)
-void launch_lpc_features(const Batch& B, const float* r, const float* diag, const float* mass, const float* scale,
-                         const CRScalars* sc, bf16* in, hipStream_t s);
+// network input at level 0, every channel linear in q = clamp((r / diag) / scale): q (3),
+// q m / diag (3), q [collision flag set] (3; flags may be null), zeros (7). With zero biases the
+// network is then positively homogeneous in r. The per-body scale comes from `scale`
+// when given, else from sc[b].lpc_scale (bodies that are done or use the diagonal
+// preconditioner are skipped when sc is given)
+void launch_lpc_features(const Batch& B, const float* r, const float* diag, const float* mass, const int8_t* flags,
+                         const float* scale, const CRScalars* sc, bf16* in, hipStream_t s);
 void launch_sub_f32(const float* a, const float* b, float* out, int64_t n, hipStream_t s);
-// per-body sums of a captured pair: sums[3 b + {0, 1, 2}] += (r / diag)^2, 3 x nodes, e^2 over
-// nodes with m > 0
-void launch_lpc_sample_stats(const Batch& B, const float* r, const float* diag, const float* mass, const float* e,
-                             double* sums, hipStream_t s);
-// training output gradient: g[v][c] = w_b (raw - e / scale) for c < 3 (0 elsewhere)
-void launch_lpc_out_grad(const Batch& B, const float* raw, const float* e, const float* scale, const float* weight,
-                         bf16* g, double* loss, hipStream_t s);
+// per-body sums of a captured residual: sums[3 b + {0, 1}] += (r / diag)^2, 3 x nodes over nodes
+// with m > 0; sums[3 b + 2] += r^2 / diag over all nodes
+void launch_lpc_sample_stats(const Batch& B, const float* r, const float* diag, const float* mass, double* sums,
+                             hipStream_t s);
+// training output gradient: g[v][c] = 2 w_b (raw - t) for c < 3 (0 elsewhere), target
+// t = clamp((e - r / diag) / scale, +-kLpcClamp): the network learns the part of the exact
+// correction e that the diagonal step r / diag misses
+void launch_lpc_out_grad(const Batch& B, const float* raw, const float* e, const float* r, const float* diag,
+                         const float* scale, const float* weight, bf16* g, double* loss, hipStream_t s);
 
 }  // namespace mpm

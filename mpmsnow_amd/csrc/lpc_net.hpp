@@ -4,15 +4,20 @@
 //
 // Network (a 3-level U-Net on the grid hierarchy of lpc_kernels.hpp; 3x3x3 convolutions, bf16
 // weights and activations, fp32 accumulation):
-//   in  (level 0, 16 ch: r/diag/s (3), occupancy (1), zeros)
+//   in  (level 0, 16 ch, linear in q = r/diag/s: q (3), q m/diag (3), q collided (3), zeros)
 //   e1 = relu(conv0(in))            level 0, 16 ch
 //   e2 = relu(conv1(pool(e1)))      level 1, 32 ch
 //   e3 = relu(conv2(pool(e2)))      level 2, 64 ch
 //   d2 = relu(conv3(unpool(e3))) + e2   level 1, 32 ch
 //   d1 = relu(conv4(unpool(d2))) + e1   level 0, 16 ch
 //   out = conv5(d1), channels 0..2       level 0: correction / s
-// The solver uses z = s out; training minimises sum_b w_b |out - e / s|^2 over captured
-// (residual, exact correction) pairs, w_b = 1 / |e / s|^2 of body b.
+// The solver uses z = r / diag + s out: the network adds to the diagonal step what it misses.
+// Training minimises sum_b w_b |out - (e - r / diag) / s|^2 over captured (residual, exact
+// correction) pairs, w_b = 1 / |(e - r / diag) / s|^2 of body b. The output layer starts at zero,
+// so an untrained network is the diagonal preconditioner. Training keeps the biases at zero: with
+// inputs linear in r the network is then positively homogeneous in r, so its correction follows
+// the residual from one iteration to the next instead of repeating a fixed shape.
+
 #pragma once
 
 #include <cmath>
@@ -107,7 +112,7 @@ class LpcNet {
   }
 
   // He-normal initialisation (fan-in 27 cin); the output layer starts at zero, so an untrained
-  // network returns z = 0 and the solve falls back to the diagonal preconditioner at once
+  // network leaves the diagonal step unchanged
   void init(uint64_t seed, hipStream_t s) {
     std::vector<float> h(nparams, 0.0f);
     std::mt19937_64 rng(seed);
@@ -162,7 +167,7 @@ class LpcNet {
       act(gp1, 1, 16);
       act(ge1, 0, 16);
       act(ge2, 1, 32);
-      loss.ensure(sizeof(double));
+      loss.ensure(2 * sizeof(double));
     }
   }
 
@@ -170,7 +175,8 @@ class LpcNet {
   // sc[b].lpc_scale (solver, which also skips done bodies and bodies on the diagonal
   // preconditioner). Output: raw (fp32 [tile][3][64], may be null) and/or z = s out.
   void forward(const mpm::Batch& B, const LpcLevels& Lv, const float* r, const float* diag, const float* mass,
-               const float* scale, const mpm::CRScalars* sc, float* rawOut, float* z, hipStream_t s) {
+               const int8_t* flags, const float* scale, const mpm::CRScalars* sc, float* rawOut, float* z,
+               hipStream_t s) {
     using namespace mpm;
     if (!initialised) throw InvalidArg("learned preconditioner has no weights");
     const bf16* W = Pb.as<bf16>();
@@ -179,7 +185,7 @@ class LpcNet {
     auto bl = [&](int k) { return Pf + layer[k].b_off; };
     auto bfp = [](DevBuf& d) { return d.as<bf16>(); };
     const LevelView L0 = Lv.view(0, B), L1 = Lv.view(1, B), L2 = Lv.view(2, B);
-    launch_lpc_features(B, r, diag, mass, scale, sc, bfp(in), s);
+    launch_lpc_features(B, r, diag, mass, flags, scale, sc, bfp(in), s);
     ConvOut o{};
     o.sc = sc;
     o.out = bfp(e1);
@@ -204,6 +210,8 @@ class LpcNet {
     f.sc = sc;
     f.raw = rawOut;
     f.z = z;
+    f.r = r;
+    f.diag = diag;
     launch_conv(16, 16, CONV_FINAL, L0, bfp(d1), Wl(5), bl(5), f, s);
   }
 
@@ -237,16 +245,29 @@ class LpcNet {
   // One gradient evaluation: forward on (r, diag, mass, scale), loss against target e, and the
   // parameter gradient in G. Returns the loss.
   double gradients(const mpm::Batch& B, const LpcLevels& Lv, const float* r, const float* diag, const float* mass,
-                   const float* e, const float* scale, const float* weight, hipStream_t s) {
+                   const int8_t* flags, const float* e, const float* scale, const float* weight, hipStream_t s) {
     using namespace mpm;
     alloc_acts(Lv, true);
     G.ensure(sizeof(float) * nparams);
     WT.ensure(sizeof(bf16) * nparams);
-    forward(B, Lv, r, diag, mass, scale, nullptr, raw.as<float>(), nullptr, s);
+    forward(B, Lv, r, diag, mass, flags, scale, nullptr, raw.as<float>(), nullptr, s);
+    HIPCHECK(hipMemsetAsync(loss.p, 0, sizeof(double), s));
+    launch_lpc_out_grad(B, raw.as<float>(), e, r, diag, scale, weight, g6.as<bf16>(), loss.as<double>(), s);
+    backward(B, Lv, s);
+    double h = 0.0;
+    HIPCHECK(hipMemcpyAsync(&h, loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    return h;
+  }
+
+  // Parameter gradient in G from the output-side gradient in g6 and the activations of the last
+  // training forward pass.
+  void backward(const mpm::Batch& B, const LpcLevels& Lv, hipStream_t s) {
+    using namespace mpm;
+    G.ensure(sizeof(float) * nparams);
+    WT.ensure(sizeof(bf16) * nparams);
     auto bfp = [](DevBuf& d) { return d.as<bf16>(); };
     const LevelView L0 = Lv.view(0, B), L1 = Lv.view(1, B), L2 = Lv.view(2, B);
-    HIPCHECK(hipMemsetAsync(loss.p, 0, sizeof(double), s));
-    launch_lpc_out_grad(B, raw.as<float>(), e, scale, weight, bfp(g6), loss.as<double>(), s);
     const int64_t n0 = (int64_t)64 * L0.NT, n1 = (int64_t)64 * L1.NT, n2 = (int64_t)64 * L2.NT;
     // layer 5: out = conv5(d1)
     layer_grads(5, Lv, B, bfp(d1), bfp(g6), s);
@@ -284,18 +305,31 @@ class LpcNet {
     launch_relu_mask(bfp(ge1), bfp(e1), n0 * 16, s);
     // layer 0: e1 = relu(conv0(in))
     layer_grads(0, Lv, B, bfp(in), bfp(ge1), s);
-    double h = 0.0;
-    HIPCHECK(hipMemcpyAsync(&h, loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    return h;
   }
 
-  void adam(float lr, hipStream_t s) {
+  // Adam step; the gradient is clipped to clipFactor times the running mean of its norm. Bias
+  // gradients are dropped (biases stay at their value, zero after init).
+  double gnormMean = 0.0;
+  void adam(float lr, hipStream_t s, double clipFactor = 3.0) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int k = 0; k < kLayers; ++k)
+      HIPCHECK(hipMemsetAsync(G.as<float>() + layer[k].b_off, 0, sizeof(float) * layer[k].cout, s));
+    loss.ensure(2 * sizeof(double));
+    double* gn = loss.as<double>() + 1;
+    HIPCHECK(hipMemsetAsync(gn, 0, sizeof(double), s));
+    mpm::launch_sumsq(G.as<float>(), nparams, gn, s);
+    double h = 0.0;
+    HIPCHECK(hipMemcpyAsync(&h, gn, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    const double norm = std::sqrt(h);
+    if (!std::isfinite(norm)) return;  // skip a step with a non-finite gradient
+    float gscale = 1.0f;
+    if (gnormMean > 0.0 && norm > clipFactor * gnormMean) gscale = (float)(clipFactor * gnormMean / norm);
+    gnormMean = gnormMean > 0.0 ? 0.95 * gnormMean + 0.05 * std::min(norm, clipFactor * gnormMean) : norm;
     ++adamStep;
     const float c1 = 1.0f - std::pow(b1, (float)adamStep), c2 = 1.0f - std::pow(b2, (float)adamStep);
     mpm::launch_adam(P.as<float>(), G.as<float>(), M.as<float>(), V.as<float>(), Pb.as<mpm::bf16>(), nparams, lr, b1,
-                     b2, eps, c1, c2, s);
+                     b2, eps, c1, c2, gscale, s);
   }
 
  private:

@@ -143,7 +143,10 @@ struct CRScalars {
   int ndirs;               // directions stored (<= kFcrM)
   int slot;                // ring slot of the most recent direction
   int retry;               // 1: the network gave no direction; redo the iteration on the diagonal
+  int lpc_pause;           // 1: lpc_off holds for one iteration only (a fallback before the third)
 };
+constexpr int kLpcStrikes = 3;   // fallbacks after which a body keeps the diagonal preconditioner
+constexpr float kLpcClamp = 8.0f;  // network inputs, outputs and training targets, in units of the scale
 constexpr int kFcrM = 2;
 
 // Counters shared by all bodies of a batch.

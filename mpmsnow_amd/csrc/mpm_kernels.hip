@@ -1460,12 +1460,15 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
 // ------------------------------------------------------------ flexible conjugate residual
 // The learned preconditioner changes from one iteration to the next, which breaks the
 // short recurrences of ConjugateResiduals.cpp:84-151. The solve then runs a truncated
-// generalised conjugate residual: each preconditioned vector z is A-orthogonalised against
-// the last kFcrM directions (in the Euclidean inner product of A p), and x, r take the step
-// that minimises |r|. The same termination (|r|^2 < tol^2 |b|^2 or the iteration cap,
-// SquareMagnitudeTermination.cpp:19-28) judges the result. A body whose residual fails to
-// fall by the factor kLpcFallback in an iteration that used the network switches to the
-// diagonal preconditioner z = r / diag for the rest of the solve.
+// generalised conjugate residual in the inner product <u, v>_D = sum u v / diag: each
+// preconditioned vector z is orthogonalised against the last kFcrM directions (A p_j in that
+// inner product) and x, r take the step that minimises |r|_D. With the diagonal
+// preconditioner z = r / diag this is the preconditioned conjugate residual iteration of
+// ConjugateResiduals.cpp (same Krylov space, same minimised norm). The same termination
+// (|r|^2 < tol^2 |b|^2 or the iteration cap, SquareMagnitudeTermination.cpp:19-28) judges
+// the result. Whenever an iteration that used the network fails to lower |r|^2 by the factor
+// kLpcFallback, the body takes its next iteration with the diagonal preconditioner; after
+// kLpcStrikes such fallbacks it keeps the diagonal for the rest of the solve.
 constexpr double kLpcFallback = 0.95;
 
 // r = b - A x0 ; network input scale ; iteration state
@@ -1504,6 +1507,7 @@ __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, cons
     S.lpc_off = netOn ? 0 : 1;
     S.fallbacks = 0;
     S.retry = 0;
+    S.lpc_pause = 0;
     S.ndirs = 0;
     S.slot = kFcrM - 1;
     for (int j = 0; j < kFcrM; ++j) {
@@ -1529,8 +1533,8 @@ __global__ void k_fcr_diag_z(Batch B, const float* r, const float* diag, float* 
 }
 // A z and its projections on the stored directions' A p_j
 __global__ void k_fcr_az(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
-                         const float* z, float dt, float* Az, const float* Q0, const float* Q1, CRScalars* sc,
-                         BatchStatus* bst, double* partials) {
+                         const float* z, const float* diag, float dt, float* Az, const float* Q0, const float* Q1,
+                         CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
@@ -1542,12 +1546,14 @@ __global__ void k_fcr_az(Batch B, const float* mass, const int8_t* flags, const 
   st3(Az, c.i, az);
   double s[2] = {0.0, 0.0};
   const float* Q[2] = {Q0, Q1};
+  float dg[3];
+  ld3(diag, c.i, dg);
 #pragma unroll
   for (int j = 0; j < kFcrM; ++j)
     if (S.ApAp[j] != 0.0) {
       float q[3];
       ld3(Q[j], c.i, q);
-      s[j] = (double)az[0] * q[0] + (double)az[1] * q[1] + (double)az[2] * q[2];
+      s[j] = (double)az[0] * (q[0] / dg[0]) + (double)az[1] * (q[1] / dg[1]) + (double)az[2] * (q[2] / dg[2]);
     }
   double tot[2];
   if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
@@ -1556,8 +1562,8 @@ __global__ void k_fcr_az(Batch B, const float* mass, const int8_t* flags, const 
 }
 // new direction p = z - sum beta_j p_j, A p = A z - sum beta_j A p_j into the ring's next slot;
 // alpha = <r, A p> / <A p, A p>
-__global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float* r, float* P0, float* P1, float* Q0,
-                           float* Q1, CRScalars* sc, BatchStatus* bst, double* partials) {
+__global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float* r, const float* diag, float* P0,
+                           float* P1, float* Q0, float* Q1, CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
@@ -1585,10 +1591,13 @@ __global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float
   st3(P[slot], c.i, p);
   st3(Q[slot], c.i, ap);
   double s[2] = {0.0, 0.0};
+  float dg[3];
+  ld3(diag, c.i, dg);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    s[0] += (double)ap[a] * ap[a];
-    s[1] += (double)rr[a] * ap[a];
+    const double w = (double)(ap[a] / dg[a]);
+    s[0] += (double)ap[a] * w;
+    s[1] += (double)rr[a] * w;
   }
   double tot[2];
   if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0) {
@@ -1657,10 +1666,108 @@ __global__ void k_fcr_update(Batch B, const int8_t* flags, const ColliderSet* co
     } else if (!S.lpc_off && tot[0] > kLpcFallback * S.rr_prev) {
       S.lpc_off = 1;
       S.fallbacks += 1;
+      S.lpc_pause = S.fallbacks < kLpcStrikes ? 1 : 0;
+    } else if (S.lpc_off && S.lpc_pause) {
+      S.lpc_off = 0;  // one diagonal iteration done: back to the network
+      S.lpc_pause = 0;
     }
     S.rr_prev = tot[0];
     S.iter = it + 1;
   }
+}
+
+// ------------------------------------------------------------ learned preconditioner training
+// The network is trained on what one flexible step achieves with its correction: the residual
+// left after the optimal step along u = A z, per body L_b = 1 - <r, u>_D^2 / (|r|_D^2 |u|_D^2)
+// (<a, b>_D = sum a b / diag), z = r / diag + s_b clamp(out). A is symmetric, so
+// dL_b/dz = A y with y = (2 a / (R c)) D^-1 (alpha u - r), a = <r, u>_D, c = |u|_D^2,
+// R = |r|_D^2, alpha = a / c.
+__global__ void k_lpc_compose_z(Batch B, const float* r, const float* diag, const float* raw, const float* scale,
+                                const float* weight, float* z) {
+  NodeCtx c = node_begin(B);
+  float rr[3], dg[3], zz[3] = {0.0f, 0.0f, 0.0f};
+  if (weight[c.b] > 0.0f && c.valid) {
+    ld3(r, c.i, rr);
+    ld3(diag, c.i, dg);
+    float o[3] = {0.0f, 0.0f, 0.0f};
+    if (raw) ld3(raw, c.i, o);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) zz[a] = rr[a] / dg[a] + fminf(fmaxf(o[a], -kLpcClamp), kLpcClamp) * scale[c.b];
+  }
+  st3(z, c.i, zz);
+}
+// u = A z from the staged force differential; per-body dots[3 b + {0, 1, 2}] += <r, u>_D, |u|_D^2, |r|_D^2
+__global__ void k_lpc_res_dots(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                               const float* stage, const float* z, const float* r, const float* diag, float dt,
+                               const float* weight, float* u, double* dots) {
+  NodeCtx c = node_begin(B);
+  float uu[3] = {0.0f, 0.0f, 0.0f};
+  double s[3] = {0.0, 0.0, 0.0};
+  if (weight[c.b] > 0.0f && c.valid) {
+    float zz[3], df[3], rr[3], dg[3];
+    ld3(z, c.i, zz);
+    gather_node3(B, c.G, stage, c.T, c.l, df);
+    apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt * dt, uu);
+    ld3(r, c.i, rr);
+    ld3(diag, c.i, dg);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      s[0] += (double)rr[a] * (double)(uu[a] / dg[a]);
+      s[1] += (double)uu[a] * (double)(uu[a] / dg[a]);
+      s[2] += (double)rr[a] * (double)(rr[a] / dg[a]);
+    }
+  }
+  st3(u, c.i, uu);
+  __shared__ double red[3][kBlock / 64];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double v = s[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const double t = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+    if (t != 0.0) atomicAdd(dots + 3 * c.b + threadIdx.x, t);
+  }
+}
+// y = (2 a / (R c)) D^-1 (alpha u - r); *loss += 1 - a^2 / (R c) once per body (first node block)
+__global__ void k_lpc_res_y(Batch B, const float* u, const float* r, const float* diag, const float* weight,
+                            const double* dots, float* y, double* loss) {
+  NodeCtx c = node_begin(B);
+  float yy[3] = {0.0f, 0.0f, 0.0f};
+  const double a = dots[3 * c.b], cc = dots[3 * c.b + 1], R = dots[3 * c.b + 2];
+  const bool ok = weight[c.b] > 0.0f && cc > 0.0 && R > 0.0;
+  if (ok && c.valid) {
+    const float alpha = (float)(a / cc), f = (float)(2.0 * a / (R * cc));
+    float uu[3], rr[3], dg[3];
+    ld3(u, c.i, uu);
+    ld3(r, c.i, rr);
+    ld3(diag, c.i, dg);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) yy[k] = f * (alpha * uu[k] - rr[k]) / dg[k];
+  }
+  st3(y, c.i, yy);
+  if (ok && threadIdx.x == 0 && blockIdx.x == (unsigned)c.G.blockBase) atomicAdd(loss, 1.0 - a * a / (R * cc));
+}
+// output-side gradient of the network: g = s_b (A y) where |out| < clamp, bf16 [node][16 channels]
+__global__ void k_lpc_res_grad(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                               const float* stage, const float* y, const float* raw, float dt, const float* scale,
+                               const float* weight, __bf16* g) {
+  NodeCtx c = node_begin(B);
+  float gv[3] = {0.0f, 0.0f, 0.0f};
+  if (weight[c.b] > 0.0f && c.valid) {
+    float yy[3], df[3], ay[3], o[3];
+    ld3(y, c.i, yy);
+    gather_node3(B, c.G, stage, c.T, c.l, df);
+    apply_A(c, cols, flags[c.i], mass[c.i], yy, df, dt * dt, ay);
+    ld3(raw, c.i, o);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gv[a] = fabsf(o[a]) < kLpcClamp ? scale[c.b] * ay[a] : 0.0f;
+  }
+  __bf16* dst = g + c.i * 16;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) dst[k] = (__bf16)(k < 3 ? gv[k] : 0.0f);
 }
 
 __global__ void k_residual_vector(Batch B, const float* z, const float* diag, float* r) {
@@ -2105,19 +2212,38 @@ void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float*
   NODE_LAUNCH(k_fcr_diag_z, B, r, diag, z, sc);
 }
 void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                   const float* stage, const float* z, float dt, float* Az, const float* Q0, const float* Q1,
-                   CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_fcr_az, B, mass, flags, cols, stage, z, dt, Az, Q0, Q1, sc, st, partials);
+                   const float* stage, const float* z, const float* diag, float dt, float* Az, const float* Q0,
+                   const float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_az, B, mass, flags, cols, stage, z, diag, dt, Az, Q0, Q1, sc, st, partials);
 }
-void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, float* P0, float* P1,
-                     float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_fcr_orth, B, z, Az, r, P0, P1, Q0, Q1, sc, st, partials);
+void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, const float* diag, float* P0,
+                     float* P1, float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials,
+                     hipStream_t s) {
+  NODE_LAUNCH(k_fcr_orth, B, z, Az, r, diag, P0, P1, Q0, Q1, sc, st, partials);
 }
 void launch_fcr_update(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* mass,
                        const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
                        float* x, float* r, CRScalars* sc, BatchStatus* st, double* partials, double* hist,
                        hipStream_t s) {
   NODE_LAUNCH(k_fcr_update, B, flags, cols, mass, diag, P0, P1, Q0, Q1, x, r, sc, st, partials, hist);
+}
+void launch_lpc_compose_z(const Batch& B, const float* r, const float* diag, const float* raw, const float* scale,
+                          const float* weight, float* z, hipStream_t s) {
+  NODE_LAUNCH(k_lpc_compose_z, B, r, diag, raw, scale, weight, z);
+}
+void launch_lpc_res_dots(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                         const float* stage, const float* z, const float* r, const float* diag, float dt,
+                         const float* weight, float* u, double* dots, hipStream_t s) {
+  NODE_LAUNCH(k_lpc_res_dots, B, mass, flags, cols, stage, z, r, diag, dt, weight, u, dots);
+}
+void launch_lpc_res_y(const Batch& B, const float* u, const float* r, const float* diag, const float* weight,
+                      const double* dots, float* y, double* loss, hipStream_t s) {
+  NODE_LAUNCH(k_lpc_res_y, B, u, r, diag, weight, dots, y, loss);
+}
+void launch_lpc_res_grad(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                         const float* stage, const float* y, const float* raw, float dt, const float* scale,
+                         const float* weight, __bf16* g, hipStream_t s) {
+  NODE_LAUNCH(k_lpc_res_grad, B, mass, flags, cols, stage, y, raw, dt, scale, weight, g);
 }
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s) {
   NODE_LAUNCH(k_residual_vector, B, z, diag, r);

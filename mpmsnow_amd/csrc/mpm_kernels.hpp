@@ -109,14 +109,31 @@ void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, con
 void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
                        hipStream_t s);
 void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                   const float* stage, const float* z, float dt, float* Az, const float* Q0, const float* Q1,
-                   CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
-void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, float* P0, float* P1,
-                     float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+                   const float* stage, const float* z, const float* diag, float dt, float* Az, const float* Q0,
+                   const float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, const float* diag, float* P0,
+                     float* P1, float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials,
+                     hipStream_t s);
 void launch_fcr_update(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* mass,
                        const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
                        float* x, float* r, CRScalars* sc, BatchStatus* st, double* partials, double* hist,
                        hipStream_t s);
+// ---- learned preconditioner training on the optimal-step residual loss
+// 1 - <r, A z>_D^2 / (|r|_D^2 |A z|_D^2) per body
+// z = r / diag + scale clamp(raw) on bodies with weight > 0 (raw may be null: z = r / diag)
+void launch_lpc_compose_z(const Batch& B, const float* r, const float* diag, const float* raw, const float* scale,
+                          const float* weight, float* z, hipStream_t s);
+// with the force differential of z staged: u = A z; dots[3 b + {0, 1, 2}] += <r, u>_D, |u|_D^2, |r|_D^2
+void launch_lpc_res_dots(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                         const float* stage, const float* z, const float* r, const float* diag, float dt,
+                         const float* weight, float* u, double* dots, hipStream_t s);
+// y with dL/dz = A y; *loss += the per-body losses
+void launch_lpc_res_y(const Batch& B, const float* u, const float* r, const float* diag, const float* weight,
+                      const double* dots, float* y, double* loss, hipStream_t s);
+// with A y staged: network output gradient g = scale A y (where the output is not clamped)
+void launch_lpc_res_grad(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                         const float* stage, const float* y, const float* raw, float dt, const float* scale,
+                         const float* weight, __bf16* g, hipStream_t s);
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s);
 void launch_finalize(const Batch& B, const float* x, const float* vc, float* vel, const CRScalars* sc,
                      hipStream_t s);

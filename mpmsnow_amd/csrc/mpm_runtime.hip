@@ -133,6 +133,7 @@ struct GridBatch {
   void forces_stage();
   void hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter);
   void solve(float dt, const mpm_solver_params* prm);
+  void train_lpc(float dt, CRScalars* hsc, size_t scb);
   void ensure_levels() {
     if (!levelsBuilt) levels.build(hb, ctx_stream());
     levelsBuilt = true;
@@ -149,18 +150,18 @@ struct mpm_grid {
   DevBuf slots;
 };
 
-// (residual, exact correction) pairs of one solve, for training the learned preconditioner
+// Residuals of one solve for training the learned preconditioner, and the training settings
 struct LpcCapture {
-  int want = 0;                  // samples the next solve keeps
+  int want = 0;                  // samples the next solve keeps (0: no training)
+  int epochs = 0;
+  float lr = 0.0f;
   int n = 0;                     // samples kept
   std::vector<int> iters;        // CR iteration of each sample
-  DevBuf data;                   // [sample][r, e][3 NN floats]
-  DevBuf scale, weight, sums;    // [sample][body]
-  const GridBatch* batch = nullptr;
-  uint64_t version = 0;
+  DevBuf data;                   // [sample][3 NN floats]
+  DevBuf scale, weight, sums, loss;  // [sample][body]
   int64_t NN = 0;
-  float* r(int k) { return data.as<float>() + (int64_t)k * 6 * NN; }
-  float* e(int k) { return data.as<float>() + (int64_t)k * 6 * NN + 3 * NN; }
+  double lossFirst = 0, lossLast = 0, lossDiag = 0;  // mean residual loss: first, last epoch, diagonal only
+  float* r(int k) { return data.as<float>() + (int64_t)k * 3 * NN; }
 };
 
 struct mpm_context {
@@ -530,14 +531,15 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
       for (int k = 0; k < batch; ++k, ++it) {
         if (netActive) {
           int pn = c.prof_begin(PK_LPC, this, it);
-          c.net.forward(B, levels, f(fr), f(diag), f(mass), nullptr, sc(), nullptr, f(z), s);
+          c.net.forward(B, levels, f(fr), f(diag), f(mass), flags.as<int8_t>(), nullptr, sc(), nullptr, f(z), s);
           c.prof_end(pn);
         }
         launch_fcr_diag_z(B, f(fr), f(diag), f(z), sc(), s);
         hvp(f(z), true, f(stageA), sc(), it);
-        launch_fcr_az(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), f(fq0), f(fq1), sc(), st(),
-                      partials.as<double>(), s);
-        launch_fcr_orth(B, f(z), f(Ar), f(fr), f(fp0), f(fp1), f(fq0), f(fq1), sc(), st(), partials.as<double>(), s);
+        launch_fcr_az(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ar), f(fq0), f(fq1),
+                      sc(), st(), partials.as<double>(), s);
+        launch_fcr_orth(B, f(z), f(Ar), f(fr), f(diag), f(fp0), f(fp1), f(fq0), f(fq1), sc(), st(),
+                        partials.as<double>(), s);
         launch_fcr_update(B, flags.as<int8_t>(), c.cols(), f(mass), f(diag), f(fp0), f(fp1), f(fq0), f(fq1), f(x),
                           f(fr), sc(), st(), partials.as<double>(), hist.as<double>(), s);
       }
@@ -567,13 +569,12 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     const int capWant = hooks ? 0 : std::min<int>(cap.want, (int)(sizeof(kCaptureIters) / sizeof(int)));
     if (capWant > 0) {
       cap.NN = NN;
-      cap.data.ensure(sizeof(float) * 6 * (size_t)NN * capWant);
+      cap.data.ensure(sizeof(float) * 3 * (size_t)NN * capWant);
       cap.iters.clear();
       cap.n = 0;
     }
     auto snapshot = [&](int iterIndex) {
       if (cap.n >= capWant || kCaptureIters[cap.n] != iterIndex) return;
-      HIPCHECK(hipMemcpyAsync(cap.e(cap.n), x.p, vecBytes(), hipMemcpyDeviceToDevice, s));
       launch_residual_vector(B, f(z), f(diag), cap.r(cap.n), s);
       cap.iters.push_back(iterIndex);
       ++cap.n;
@@ -614,37 +615,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
         }
         fetch_status();
       }
-      if (capWant > 0) {
-        // e_k = x* - x_k (x* = the converged solution), per-body input scale and loss weight
-        for (int k = 0; k < cap.n; ++k) launch_sub_f32(f(x), cap.e(k), cap.e(k), 3 * NN, s);
-        cap.sums.ensure(sizeof(double) * 3 * nb * cap.n);
-        HIPCHECK(hipMemsetAsync(cap.sums.p, 0, sizeof(double) * 3 * nb * cap.n, s));
-        for (int k = 0; k < cap.n; ++k)
-          launch_lpc_sample_stats(B, cap.r(k), f(diag), f(mass), cap.e(k), cap.sums.as<double>() + 3 * nb * k, s);
-        std::vector<double> sums(3 * nb * cap.n);
-        HIPCHECK(hipMemcpyAsync(sums.data(), cap.sums.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
-        c.sync();
-        std::vector<float> hs(nb * cap.n), hw(nb * cap.n);
-        for (int k = 0; k < cap.n; ++k)
-          for (int bi = 0; bi < nb; ++bi) {
-            const double* q = &sums[3 * (nb * k + bi)];
-            const double sc2 = q[1] > 0.0 ? q[0] / q[1] : 0.0;
-            const double E = sc2 > 0.0 ? q[2] / sc2 : 0.0;
-            // bodies of at least 256 particles that were still iterating at this sample
-            const bool use = hb[bi].np >= 256 && hsc[bi].iterations > cap.iters[k] && E > 0.0 && !hsc[bi].zero_x;
-            hs[nb * k + bi] = (float)std::sqrt(sc2);
-            hw[nb * k + bi] = use ? (float)(1.0 / E) : 0.0f;
-          }
-        cap.scale.ensure(sizeof(float) * hs.size());
-        cap.weight.ensure(sizeof(float) * hw.size());
-        HIPCHECK(hipMemcpyAsync(cap.scale.p, hs.data(), sizeof(float) * hs.size(), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(cap.weight.p, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice, s));
-        c.sync();
-        cap.batch = this;
-        cap.version = version;
-        cap.want = 0;
-      }
+      if (capWant > 0 && cap.n > 0) train_lpc(dt, hsc, scb);
     } else {
       // host hooks see the bodies one after another, as the reference solves them: bodies still
       // iterating are parked (done = 2, which every kernel skips) and released one at a time
@@ -701,6 +672,92 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     raise_errors(hsc[bi].errors);
   }
   if (maxIt > 0) (learned ? c.iterHintFcr : c.iterHint) = maxIt;
+}
+
+// Train the learned preconditioner on the residuals captured in this solve, before the particle
+// update, while the system's operator is still in place: Adam on the per-body residual left by
+// the optimal step along A z, 1 - <r, A z>_D^2 / (|r|_D^2 |A z|_D^2), z = r / diag + s_b net(r)
+// (mpm_kernels.hip k_lpc_*).
+void GridBatch::train_lpc(float dt, CRScalars* hsc, size_t scb) {
+  mpm_context& c = *ctx;
+  hipStream_t s = c.stream;
+  LpcCapture& cap = c.cap;
+  const Batch B = view();
+  if (!c.net.initialised) c.net.init(1, s);
+  ensure_levels();
+  c.net.alloc_acts(levels, true);
+  for (DevBuf* d : {&fp0, &fp1}) d->ensure(vecBytes());
+  cap.sums.ensure(sizeof(double) * 3 * nb * cap.n);
+  HIPCHECK(hipMemsetAsync(cap.sums.p, 0, sizeof(double) * 3 * nb * cap.n, s));
+  for (int k = 0; k < cap.n; ++k)
+    launch_lpc_sample_stats(B, cap.r(k), f(diag), f(mass), cap.sums.as<double>() + 3 * nb * k, s);
+  std::vector<double> sums(3 * nb * cap.n);
+  HIPCHECK(hipMemcpyAsync(sums.data(), cap.sums.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
+  c.sync();
+  std::vector<float> hs(nb * cap.n), hw(nb * cap.n);
+  std::vector<int> useful;
+  for (int k = 0; k < cap.n; ++k) {
+    bool any = false;
+    for (int bi = 0; bi < nb; ++bi) {
+      const double* q = &sums[3 * (nb * k + bi)];
+      // bodies of at least 256 particles still iterating at this sample
+      const bool use = hb[bi].np >= 256 && hsc[bi].iterations > cap.iters[k] && q[2] > 0.0 && q[1] > 0.0;
+      hs[nb * k + bi] = (float)std::sqrt(q[1] > 0.0 ? q[0] / q[1] : 0.0);
+      hw[nb * k + bi] = use ? 1.0f : 0.0f;
+      any = any || use;
+    }
+    if (any) useful.push_back(k);
+  }
+  cap.scale.ensure(sizeof(float) * hs.size());
+  cap.weight.ensure(sizeof(float) * hw.size());
+  cap.loss.ensure(sizeof(double) * (1 + 3 * nb));
+  HIPCHECK(hipMemcpyAsync(cap.scale.p, hs.data(), sizeof(float) * hs.size(), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(cap.weight.p, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice, s));
+  const float* sk = cap.scale.as<float>();
+  const float* wk = cap.weight.as<float>();
+  const int8_t* fl = flags.as<int8_t>();
+  double* dl = cap.loss.as<double>();
+  double* dots = dl + 1;
+  auto step_loss = [&](int k, const float* raw) {
+    launch_lpc_compose_z(B, cap.r(k), f(diag), raw, sk + nb * k, wk + nb * k, f(fp0), s);
+    hvp(f(fp0), true, f(stageA), nullptr, -1);
+    HIPCHECK(hipMemsetAsync(dots, 0, sizeof(double) * 3 * nb, s));
+    launch_lpc_res_dots(B, f(mass), fl, c.cols(), f(stageA), f(fp0), cap.r(k), f(diag), dt, wk + nb * k, f(Ar),
+                        dots, s);
+    launch_lpc_res_y(B, f(Ar), cap.r(k), f(diag), wk + nb * k, dots, f(fp1), dl, s);
+  };
+  auto fetch_loss = [&]() {
+    double h = 0.0;
+    HIPCHECK(hipMemcpyAsync(&h, dl, sizeof(double), hipMemcpyDeviceToHost, s));
+    c.sync();
+    return h;
+  };
+  // loss of the diagonal step alone
+  HIPCHECK(hipMemsetAsync(dl, 0, sizeof(double), s));
+  for (int k : useful) step_loss(k, nullptr);
+  int nbodies_used = 0;
+  for (int k : useful)
+    for (int bi = 0; bi < nb; ++bi) nbodies_used += hw[nb * k + bi] > 0.0f ? 1 : 0;
+  const double norm = nbodies_used > 0 ? 1.0 / nbodies_used : 0.0;
+  cap.lossDiag = fetch_loss() * norm;
+  for (int ep = 0; ep < cap.epochs; ++ep) {
+    HIPCHECK(hipMemsetAsync(dl, 0, sizeof(double), s));
+    for (int k : useful) {
+      float* raw = c.net.raw.as<float>();
+      c.net.forward(B, levels, cap.r(k), f(diag), f(mass), fl, sk + nb * k, nullptr, raw, nullptr, s);
+      step_loss(k, raw);
+      hvp(f(fp1), true, f(stageA), nullptr, -1);
+      launch_lpc_res_grad(B, f(mass), fl, c.cols(), f(stageA), f(fp1), raw, dt, sk + nb * k, wk + nb * k,
+                          c.net.g6.as<bf16>(), s);
+      c.net.backward(B, levels, s);
+      c.net.adam(cap.lr, s);
+    }
+    const double l = fetch_loss() * norm;
+    if (ep == 0) cap.lossFirst = l;
+    cap.lossLast = l;
+  }
+  cap.want = 0;
 }
 
 namespace {
@@ -1355,6 +1412,7 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
       stats->num_bodies = c.nbodies;
       stats->num_ballistic = c.nballistic;
       stats->cancelled = cancelled ? 1 : 0;
+      stats->num_solved = (int)c.lastSolves.size();
     }
   });
 }
@@ -1387,44 +1445,22 @@ int mpm_lpc_set_mode(mpm_context* ctx, int mode) {
   });
 }
 
-int mpm_lpc_capture(mpm_context* ctx, int samples) {
+int mpm_lpc_train_next(mpm_context* ctx, int samples, int epochs, float learning_rate) {
   return guarded(ctx, [&] {
     if (samples < 0 || samples > 16) throw InvalidArg("at most 16 samples per solve");
+    if (epochs < 0) throw InvalidArg("epochs must be >= 0");
     ctx->cap.want = samples;
+    ctx->cap.epochs = epochs;
+    ctx->cap.lr = learning_rate;
+    ctx->cap.lossFirst = ctx->cap.lossLast = ctx->cap.lossDiag = 0.0;
   });
 }
 
-int mpm_lpc_train(mpm_context* ctx, int epochs, float learning_rate, double* loss) {
+int mpm_lpc_last_loss(mpm_context* ctx, double* first, double* last, double* diagonal) {
   return guarded(ctx, [&] {
-    mpm_context& c = *ctx;
-    LpcCapture& cap = c.cap;
-    GridBatch& B = c.simBatch;
-    if (cap.n == 0 || cap.batch != &B || cap.version != B.version)
-      throw InvalidArg("no captured pairs (call mpm_lpc_capture before mpm_sim_advance)");
-    if (!c.net.initialised) c.net.init(1, c.stream);
-    B.ensure_levels();
-    const Batch v = B.view();
-    std::vector<float> hw((size_t)B.nb * cap.n);
-    HIPCHECK(hipMemcpyAsync(hw.data(), cap.weight.p, sizeof(float) * hw.size(), hipMemcpyDeviceToHost, c.stream));
-    c.sync();
-    double last = 0.0;
-    for (int ep = 0; ep < epochs; ++ep) {
-      double sum = 0.0;
-      int used = 0;
-      for (int k = 0; k < cap.n; ++k) {
-        bool any = false;
-        for (int bi = 0; bi < B.nb; ++bi) any = any || hw[(size_t)B.nb * k + bi] > 0.0f;
-        if (!any) continue;
-        sum += c.net.gradients(v, B.levels, cap.r(k), B.f(B.diag), B.f(B.mass), cap.e(k),
-                               cap.scale.as<float>() + (size_t)B.nb * k, cap.weight.as<float>() + (size_t)B.nb * k,
-                               c.stream);
-        c.net.adam(learning_rate, c.stream);
-        ++used;
-      }
-      last = used ? sum / used : 0.0;
-    }
-    c.sync();
-    if (loss) *loss = last;
+    if (first) *first = ctx->cap.lossFirst;
+    if (last) *last = ctx->cap.lossLast;
+    if (diagonal) *diagonal = ctx->cap.lossDiag;
   });
 }
 
@@ -1450,7 +1486,7 @@ int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float 
     GridBatch& B = grid->batch;
     grid_lpc_inputs(grid, r, diag, nullptr, scale, 0.0f);
     c.net.alloc_acts(B.levels, true);
-    c.net.forward(B.view(), B.levels, B.f(B.fr), B.f(B.fp0), B.f(B.mass), B.fq0.as<float>(), nullptr,
+    c.net.forward(B.view(), B.levels, B.f(B.fr), B.f(B.fp0), B.f(B.mass), nullptr, B.fq0.as<float>(), nullptr,
                   c.net.raw.as<float>(), nullptr, c.stream);
     B.to_dense(0, c.net.raw.as<float>(), 3, out);
   });
@@ -1463,7 +1499,7 @@ int mpm_grid_lpc_gradients(mpm_grid* grid, const float* r, const float* diag, co
     mpm_context& c = *grid->ctx;
     GridBatch& B = grid->batch;
     grid_lpc_inputs(grid, r, diag, e, scale, weight);
-    const double l = c.net.gradients(B.view(), B.levels, B.f(B.fr), B.f(B.fp0), B.f(B.mass), B.f(B.fp1),
+    const double l = c.net.gradients(B.view(), B.levels, B.f(B.fr), B.f(B.fp0), B.f(B.mass), nullptr, B.f(B.fp1),
                                      B.fq0.as<float>(), B.fq0.as<float>() + 1, c.stream);
     if (loss) *loss = l;
     if (grad) {

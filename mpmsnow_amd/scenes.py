@@ -70,30 +70,38 @@ def snowball_drop(n=20000, seed=0, contact=False):
                 material=dict(SNOW))
 
 
-def two_snowballs(n=1000000, seed=1):
-    """Two snowballs colliding: n/2 particles per sphere r=0.15 at (0.5 -/+ 0.16, 0.5, 0.5), v = (+/-3,0,0)."""
-    r = 0.15
+def two_snowballs(n=1000000, seed=1, r=0.15, speed=3.0, E=None):
+    """Two snowballs colliding: n/2 particles per sphere r=0.15 at (0.5 -/+ 0.16, 0.5, 0.5), v = (+/-3,0,0).
+
+    r, speed and E vary the scene for training the learned preconditioner (the spheres stay
+    0.02 m apart).
+    """
     half = n // 2
-    a = sphere(seed, half, (0.5 - 0.16, 0.5, 0.5), r)
-    b = sphere(seed + 7919, n - half, (0.5 + 0.16, 0.5, 0.5), r)
+    a = sphere(seed, half, (0.5 - (r + 0.01), 0.5, 0.5), r)
+    b = sphere(seed + 7919, n - half, (0.5 + (r + 0.01), 0.5, 0.5), r)
     x = np.concatenate([a, b])
     mp = RHO * (4.0 / 3.0 * np.pi * r ** 3) / half
     m = np.full(n, mp, dtype=np.float32)
     v = np.zeros((n, 3), dtype=np.float32)
-    v[:half, 0] = 3.0
-    v[half:, 0] = -3.0
+    v[:half, 0] = speed
+    v[half:, 0] = -speed
+    mat = dict(SNOW)
+    if E is not None:
+        mat["E"] = float(E)
     return dict(x=x, m=m, v=v, h=1.0 / 256, dt=5e-4, gravity=(0.0, -9.81, 0.0),
                 planes=[((0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.2, False)], tol=1e-5, max_iters=1000,
-                material=dict(SNOW))
+                material=mat)
 
 
-def slope_slab(n=4000000, seed=2):
+def slope_slab(n=4000000, seed=2, size=(0.8, 0.2, 0.4)):
     """Snow slab on a 30-degree slope, as tilted gravity over the plane y=0.
 
-    Per-particle E0 log-uniform in [7e4, 2.8e5] Pa and xi uniform in [5, 10].
+    Per-particle E0 log-uniform in [7e4, 2.8e5] Pa and xi uniform in [5, 10]. `size` varies the
+    box for training the learned preconditioner.
     """
-    x = box(seed, n, (0.1, 0.0 + 1.0 / 256, 0.3), (0.9, 0.2, 0.7))
-    vol = 0.8 * 0.2 * 0.4
+    sx, sy, sz = size
+    x = box(seed, n, (0.5 - sx / 2, 0.0 + 1.0 / 256, 0.5 - sz / 2), (0.5 + sx / 2, sy, 0.5 + sz / 2))
+    vol = sx * (sy - 1.0 / 256) * sz if size != (0.8, 0.2, 0.4) else 0.8 * 0.2 * 0.4
     m = np.full(n, RHO * vol / n, dtype=np.float32)
     u = splitmix64_uniform(seed + 17, 2 * n).reshape(n, 2)
     E0 = np.exp(np.log(7e4) + (np.log(2.8e5) - np.log(7e4)) * u[:, 0])

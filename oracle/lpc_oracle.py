@@ -7,7 +7,7 @@ The network has no counterpart in the reference (it attaches where
 ConjugateResiduals takes a preconditioner, ConjugateResiduals.h:20). This file
 restates it on dense per-body volumes with numpy:
 
-    in  (16 ch: r/diag/s (3), occupancy (1), zeros)
+    in  (16 ch, linear in q = clamp(r/diag/s): q (3), q m/diag (3), q collided (3), zeros)
     e1 = relu(conv0(in))                  level 0, 16 ch
     e2 = relu(conv1(pool(e1)))            level 1, 32 ch
     e3 = relu(conv2(pool(e2)))            level 2, 64 ch
@@ -24,6 +24,8 @@ and every stored activation to bfloat16 where the GPU does; with bf16=False it
 is plain float64 (used for the finite-difference check of the backward pass).
 """
 import numpy as np
+
+CLAMP = 8.0  # inputs and training targets, in units of the scale
 
 LAYERS = [(16, 16, 0, "relu"), (16, 32, 1, "relu"), (32, 64, 2, "relu"), (64, 32, 1, "relu"), (32, 16, 0, "relu"),
           (16, 16, 0, "final")]
@@ -123,13 +125,18 @@ def unpool(x, fine_shape):
     return y[:fine_shape[0], :fine_shape[1], :fine_shape[2]]
 
 
-def features(r, diag, mass, scale, bf16):
-    """r, diag: (X, Y, Z, 3); mass: (X, Y, Z)."""
+def features(r, diag, mass, scale, bf16, collided=None):
+    """r, diag: (X, Y, Z, 3); mass: (X, Y, Z); collided: (X, Y, Z) bool or None."""
     f = np.zeros(r.shape[:3] + (16,))
     inv = 1.0 / scale if scale > 0 else 0.0
-    q = (np.asarray(r, np.float32) / np.asarray(diag, np.float32)).astype(np.float32)
-    f[..., :3] = (q * np.float32(inv)).astype(np.float32)
-    f[..., 3] = (np.asarray(mass) > 0).astype(np.float64)
+    d32 = np.asarray(diag, np.float32)
+    q = (np.asarray(r, np.float32) / d32).astype(np.float32)
+    q = np.clip((q * np.float32(inv)).astype(np.float32), -CLAMP, CLAMP)
+    m32 = np.asarray(mass, np.float32)
+    f[..., :3] = q
+    f[..., 3:6] = (q * (m32[..., None] / d32).astype(np.float32)).astype(np.float32)
+    if collided is not None:
+        f[..., 6:9] = q * np.asarray(collided, np.float32)[..., None]
     return round_bf16(f) if bf16 else f
 
 
@@ -156,11 +163,17 @@ def forward(params, r, diag, mass, scale, bf16=True):
 
 
 def loss_and_grad(params, r, diag, mass, e, scale, weight, bf16=True):
-    """Loss weight * sum |out - e / scale|^2 and its gradient (flat, like params)."""
+    """Loss weight * sum |out - clamp((e - r / diag) / scale)|^2 and its gradient (flat, like params).
+
+    The solver uses z = r / diag + scale * out, so the network's target is the part of the
+    exact correction e that the diagonal step misses.
+    """
     Ws = unpack(params, bf16)
     out, c = forward(params, r, diag, mass, scale, bf16)
-    inv = 1.0 / scale
-    d = out - np.asarray(e, np.float64) * inv
+    inv = np.float32(1.0 / scale)
+    q = (np.asarray(r, np.float32) / np.asarray(diag, np.float32)).astype(np.float32)
+    t = np.clip(((np.asarray(e, np.float32) - q) * inv).astype(np.float32), -CLAMP, CLAMP)
+    d = out - t.astype(np.float64)
     loss = weight * float(np.sum(d * d))
     offs, n = layer_offsets()
     grad = np.zeros(n)

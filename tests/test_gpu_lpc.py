@@ -104,36 +104,97 @@ def _small_scene(seed=3):
     return sc, sim
 
 
-def test_training_on_captured_pairs_reduces_loss():
-    say("train: contact snowball 20k, capture 8 pairs")
+def test_training_reduces_the_residual_loss_below_the_diagonal_step():
+    """In-solve training on 8 residuals of one solve: the residual left by the preconditioned
+    correction falls over the epochs and ends below that of the diagonal step alone."""
+    say("train: contact snowball 20k, 8 residuals, 30 epochs")
     sc, sim = _small_scene()
     for _ in range(3):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
     sim.ctx.lpc_init(5)
-    sim.ctx.lpc_capture(8)
+    sim.ctx.lpc_train_next(8, 30, 3e-4)
     sim.advance(sc["dt"], 400, 1e-7)
-    first = sim.ctx.lpc_train(1, 1e-3)
-    for _ in range(10):
-        last = sim.ctx.lpc_train(1, 1e-3)
-    say("loss %.4f -> %.4f" % (first, last))
-    assert np.isfinite(last) and last < first
+    first, last, diag = sim.ctx.lpc_last_loss()
+    say("residual loss %.4f -> %.4f (diagonal step %.4f)" % (first, last, diag))
+    assert np.isfinite(last) and last < first and last < diag
 
 
-def test_learned_flexible_cr_meets_the_residual_test():
-    """Untrained (zero output layer) network: every body falls back to the diagonal
-    preconditioner after the first iteration and still meets |r|^2 < tol^2 |b|^2."""
-    say("fcr: contact snowball 20k, untrained network")
-    sc, sim = _small_scene(seed=4)
+def _largest(sim, st):
+    rows = [sim.body_stats(b, hist_cap=0) for b in range(st["num_solved"])]
+    return max(rows, key=lambda r: r["particles"]), rows
+
+
+def test_untrained_network_is_diagonal_cr():
+    """Zero output layer: z = r / diag, so the flexible iteration in the diag^-1 inner product
+    is preconditioned CR; it meets |r|^2 < tol^2 |b|^2 in about the plain solver's iterations."""
+    say("fcr: contact snowball 20k, untrained network vs plain CR")
+    its = []
+    for mode in (0, 1):
+        sc, sim = _small_scene(seed=4)
+        for _ in range(2):
+            sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        sim.ctx.lpc_init(6)
+        sim.ctx.lpc_mode(mode)
+        st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        big, rows = _largest(sim, st)
+        tol2 = np.float64(sc["tol"]) ** 2
+        for s in rows:
+            if 0 < s["iterations"] <= sc["max_iters"]:
+                assert s["residual_norm2_last"] < tol2 * s["bnorm2"] or s["early_exit"], s
+        its.append(big["iterations"])
+    say("largest body iterations: plain %d, untrained learned %d" % tuple(its))
+    assert abs(its[1] - its[0]) <= max(2, its[0] // 10)
+
+
+def test_bad_network_falls_back_to_diagonal():
+    """A network with a large random output layer: the residual stops falling, the body switches
+    to the diagonal preconditioner (for good after three fallbacks) and still meets the
+    residual test."""
+    say("fcr: contact snowball 20k, random network")
+    sc, sim = _small_scene(seed=5)
     for _ in range(2):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    sim.ctx.lpc_init(6)
+    p = random_params(13)
+    offs, _ = L.layer_offsets()
+    w, b = offs[5]
+    p[w:w + 27 * 16 * 16] *= 50.0
+    sim.ctx.lpc_set_params(p)
     sim.ctx.lpc_mode(1)
     st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    assert st["num_bodies"] >= 1
-    tol2 = np.float64(sc["tol"]) ** 2
-    for b in range(st["num_bodies"]):
-        s = sim.body_stats(b, hist_cap=0)
-        if s["iterations"] == 0 or s["iterations"] > sc["max_iters"]:
-            continue
-        assert s["residual_norm2_last"] < tol2 * s["bnorm2"] or s["early_exit"], s
-        assert s["fallbacks"] == 1
+    big, _ = _largest(sim, st)
+    say("largest body: %d iterations, fallbacks %d" % (big["iterations"], big["fallbacks"]))
+    assert 1 <= big["fallbacks"] <= 3
+    assert big["residual_norm2_last"] < np.float64(sc["tol"]) ** 2 * big["bnorm2"]
+
+
+def test_ablation_trained_network_against_plain_cr():
+    """What the trained network gains: a stiffer contact snowball (E = 1.4e6 Pa) is run twice from
+    identical states; in one run the network is trained in-solve on two steps and the next steps
+    use the learned flexible CR, in the other they use plain CR. Both must meet the residual
+    test; the iteration counts of the largest body are reported (the gain is measured, not
+    assumed)."""
+    def scene():
+        sc = scenes.snowball_drop(n=20000, seed=7, contact=True)
+        sc["material"]["E"] = 1.4e6
+        sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
+                    gravity=[sc["gravity"]], v=sc["v"])
+        return sc, sim
+    its = {}
+    for mode in (0, 1):
+        sc, sim = scene()
+        sim.ctx.lpc_init(9)
+        for _ in range(2):
+            sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        for _ in range(2):  # train on both runs so that they take the same steps
+            sim.ctx.lpc_train_next(8, 30, 3e-4)
+            sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        sim.ctx.lpc_mode(mode)
+        counts = []
+        for _ in range(3):
+            st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+            big, _ = _largest(sim, st)
+            assert big["residual_norm2_last"] < np.float64(sc["tol"]) ** 2 * big["bnorm2"] or big["early_exit"]
+            counts.append(big["iterations"])
+        its[mode] = counts
+    say("ablation: largest-body CR iterations plain %s, learned %s" % (its[0], its[1]))
+    assert all(c > 0 for c in its[0] + its[1])
