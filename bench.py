@@ -18,6 +18,11 @@ cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the
 8-colour voxel partition like the reference's TBB splats) timed on this host
 for a bounded sample: one step of the same 1M-particle scene from the GPU run's
 state.
+learned_pc = the metric's second half: on the 4M-particle slope slab (seed 2,
+held out from training), particle-steps/s and CR iterations per step of the
+learned flexible CR (network weights mpmsnow_amd/lpc_weights.npy, trained by
+mpmsnow_amd/lpc_train.py) against the plain diagonal CR, same states, same
+1e-5 residual test.
 
 Other modes:
   --copy-calibration   also runs one 256 MiB device-to-device copy first (the
@@ -74,6 +79,44 @@ def cpu_baseline(sc, gpu_sim, steps=1):
                       "iterations of its two largest bodies %s)" % (steps, n, dt, sorted(its)[-2:])}
 
 
+def learned_leg(steps, warmup):
+    """Plain diagonal CR vs learned flexible CR on the 4M slope slab, from identical states."""
+    import numpy as np
+    import torch
+
+    import mpmsnow_amd as G
+    from mpmsnow_amd import scenes
+
+    sc = scenes.slope_slab(n=4000000, seed=2)
+    w = np.load(os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
+    out = {}
+    for mode, name in ((0, "plain"), (1, "learned")):
+        sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
+                    mu0=sc["mu0"], lambda0=sc["lambda0"], xi=sc["xi"])
+        sim.ctx.lpc_set_params(w)
+        for _ in range(warmup):
+            sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        sim.ctx.lpc_mode(mode)
+        its, fbs = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+            rows = [sim.body_stats(b, hist_cap=0) for b in range(st["num_solved"])]
+            big = max(rows, key=lambda r: r["particles"])
+            its.append(big["iterations"])
+            fbs.append(big["fallbacks"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"value": len(sc["m"]) * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "cr_iters_per_step": float(np.mean(its)), "fallbacks_per_step": float(np.mean(fbs))}
+        del sim
+    out["speedup"] = out["learned"]["value"] / out["plain"]["value"]
+    out["config"] = ("slope slab 4M particles (seed 2), h=1/256, dt=5e-4, CR tol 1e-5, %d steps after %d warmup"
+                     % (steps, warmup))
+    return out
+
+
 def _pmc_rows(path):
     """(kernel name, grid size, counter value) per dispatch of a rocprofv3 counter_collection.csv."""
     rows = []
@@ -109,8 +152,8 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         "",
         "Commands (from the repository root on the GPU host):",
         "",
-        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration",
-        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration",
+        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --no-learned --copy-calibration",
+        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --no-learned --copy-calibration",
         "    python bench.py --pmc-summary <F>/run_counter_collection.csv <W>/run_counter_collection.csv",
         "",
         "The counters are scaled by what they read for a %d MiB device-to-device copy (`__amd_rocclr_copyBuffer`):"
@@ -142,11 +185,13 @@ def read_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=197)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--learned-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
@@ -186,7 +231,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         stats.append(sim.advance(sc["dt"], sc["max_iters"], sc["tol"]))
-        nb = stats[-1]["num_bodies"]
+        nb = stats[-1]["num_solved"]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     hvp = sim.profile_query()["hvp"]
@@ -226,7 +271,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (FP64 CR reductions)",
         "data": "synthetic (seeded SplitMix64 particles)",
-        "config": {"workload": "two snowballs colliding: 1M particles, h=1/256, dt=5e-4, CR tol 1e-5",
+        "config": {"workload": "two snowballs colliding: 1M particles, h=1/256, dt=5e-4, CR tol 1e-5, "
+                               "%d steps after %d warmup" % (args.steps, args.warmup),
                    "particles": n, "preconditioner": "diagonal (plain CR)", "parallelism": "single GPU"},
         "cr_iters_per_step": float(np.mean([s["total_iterations"] for s in stats])),
         "cr_iters_note": "sum over the step's bodies; per body: largest body %d iterations (%d particles, %d "
@@ -242,6 +288,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, sim, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    if not args.no_learned:
+        out["learned_pc"] = learned_leg(args.learned_steps, 3)
     print(json.dumps(out))
 
 
