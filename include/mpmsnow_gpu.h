@@ -193,9 +193,19 @@ int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float 
 int mpm_grid_lpc_gradients(mpm_grid* grid, const float* r, const float* diag, const float* e, float scale,
                            float weight, float* grad, double* loss);
 
+/* ---- small bodies (no counterpart in the reference) -----------------------------
+ * Bodies of at most 8 node blocks (2048 grid nodes) and max_particles particles
+ * run their whole conjugate residual solve in one workgroup each, on a second
+ * stream, concurrently with the batched iteration of the larger bodies (default
+ * 2048; 0 = every body iterates in the batched loop). The iterates are bitwise
+ * the same either way; host termination/debug hooks and training captures
+ * always use the batched loop. */
+int mpm_set_small_bodies(mpm_context* ctx, int max_particles);
+
 /* ---- measurement (no counterpart in the reference) ----------------------------
- * Kernel timing with HIP events on the context's stream. Kernel classes, in bit
- * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p lpc_forward.
+ * Kernel timing with HIP events on the context's streams. Kernel classes, in bit
+ * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p lpc_forward
+ * cr_small (the small-body solver: one launch per solve, all its iterations).
  * mask selects the classes to time (bit k = class k; -1 = all, 0 = off); each
  * timed launch costs two event records, so time only what is needed.
  * algorithmic_bytes sums, over the timed launches, bytes/particle * particles +

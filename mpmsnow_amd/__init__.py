@@ -106,13 +106,14 @@ SIGNATURES = {
     "mpm_grid_lpc_apply": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, _fp]),
     "mpm_grid_lpc_gradients": (C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_float, C.c_float, _fp,
                                          C.POINTER(C.c_double)]),
+    "mpm_set_small_bodies": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double)]),
 }
 
 PROFILE_KERNELS = ("p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p", "cr_update_x", "cr_ar", "cr_update_p",
-                   "lpc_forward")
+                   "lpc_forward", "cr_small")
 
 
 def lib():
@@ -298,6 +299,11 @@ class Context:
 
     def lpc_mode(self, mode):
         self.check(lib().mpm_lpc_set_mode(self.ptr, int(mode)))
+
+    def small_bodies(self, max_particles):
+        """Bodies of at most 8 node blocks and this many particles are solved in one workgroup
+        each, concurrently with the batched loop (0: off). Same iterates either way."""
+        self.check(lib().mpm_set_small_bodies(self.ptr, int(max_particles)))
 
     def lpc_train_next(self, samples, epochs, lr):
         """The next advance trains the network on `samples` residuals of its solve."""

@@ -110,7 +110,8 @@ enum ErrorBits : int {
   ERR_NAN_MOMENTA = 8,     // "nan in explicit momenta!"
   ERR_HARDNESS = 16,       // "infinite hardness!!!"
   ERR_STENCIL = 32,        // internal: particle outside its tile's footprint
-  ERR_NAN_PVEL = 64        // "nans in particle velocity data!"
+  ERR_NAN_PVEL = 64,       // "nans in particle velocity data!"
+  ERR_INTERNAL = 128       // internal: a launch that does not fit the batch's layout
 };
 
 // Conjugate-residual scalars of one body, resident in device memory. Reductions are
@@ -121,7 +122,7 @@ struct CRScalars {
   double rz, ApdPAp, alpha, beta, rnorm2;
   double sums[8];          // totalMass, totalMomentum[3], collidedMass, collidedMomentum[3]
   float guess[3];
-  int done;                // 1 = solve finished
+  int done;                // 1 = solve finished; 2 = parked; kDoneSmall = small-body solver
   int zero_x;              // rhs was zero: x = 0 (ConjugateResiduals.cpp:34-38)
   int iter;                // current iteration index i
   int iterations;          // x updates performed
@@ -153,6 +154,10 @@ constexpr int kFcrM = 2;
 struct BatchStatus {
   int bodies_done;
   int errors;
+  int small_bodies;  // bodies handed to the one-workgroup solver (done = kDoneSmall)
 };
+// Small bodies: at most kSmallBlocks node blocks; their CR solve runs in one workgroup.
+constexpr int kSmallBlocks = 8;
+constexpr int kDoneSmall = 3;  // CRScalars::done of a body that the small-body solver owns
 
 }  // namespace mpm

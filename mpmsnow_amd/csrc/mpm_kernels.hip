@@ -74,9 +74,9 @@ struct ChunkCtx {
   int ox, oy, oz;  // node origin of the chunk's tile
 };
 // all threads: the chunk, its body, and (threads 0..7) the footprint's global tiles
-__device__ __forceinline__ ChunkCtx chunk_begin(const Batch& B, int* ft) {
+__device__ __forceinline__ ChunkCtx chunk_begin(const Batch& B, int* ft, int chunk) {
   ChunkCtx c;
-  c.ch = B.chunks[blockIdx.x];
+  c.ch = B.chunks[chunk];
   c.G = B.bodies[c.ch.w];
   int tx, ty, tz;
   tile_xyz(c.G, c.ch.x - c.G.tileBase, tx, ty, tz);
@@ -177,8 +177,9 @@ __device__ __forceinline__ FxScale fx_scale(float bound) {
   return r;
 }
 // staged layout per chunk: [footprint tile f][component c][64 local nodes]
-template <int NC> __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc, float* stage) {
-  float* dst = stage + (int64_t)blockIdx.x * kStage;
+template <int NC>
+__device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc, float* stage, int slot = -1) {
+  float* dst = stage + (int64_t)(slot < 0 ? (int)blockIdx.x : slot) * kStage;
   const double inv = (double)sc.invS;
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int f = i >> 6, l = i & 63;
@@ -242,11 +243,11 @@ struct NodeCtx {
   int ni, nj, nk;
   bool valid;   // inside the body's reference grid
 };
-__device__ __forceinline__ NodeCtx node_begin(const Batch& B) {
+__device__ __forceinline__ NodeCtx node_in(const BodyGrid& G, int body, int blk) {
   NodeCtx c;
-  c.b = B.blockBody[blockIdx.x];
-  c.G = B.bodies[c.b];
-  c.i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  c.b = body;
+  c.G = G;
+  c.i = (int64_t)blk * kBlock + threadIdx.x;
   c.T = (int)(c.i >> 6);
   c.l = (int)(c.i & 63);
   int tx, ty, tz;
@@ -256,6 +257,10 @@ __device__ __forceinline__ NodeCtx node_begin(const Batch& B) {
   c.nk = tz * 4 + (c.l >> 4);
   c.valid = c.ni < c.G.n[0] && c.nj < c.G.n[1] && c.nk < c.G.n[2] && (c.T - c.G.tileBase) < c.G.T[0] * c.G.T[1] * c.G.T[2];
   return c;
+}
+__device__ __forceinline__ NodeCtx node_begin(const Batch& B) {
+  const int b = B.blockBody[blockIdx.x];
+  return node_in(B.bodies[b], b, blockIdx.x);
 }
 __device__ __forceinline__ void ld3(const float* v, int64_t i, float o[3]) {
   int64_t b = (i >> 6) * 192 + (i & 63);
@@ -702,6 +707,50 @@ __global__ void k_scatter(const int* slots, const int* key, const int* rank, con
     perm[offset[t] + rank[k]] = slots ? slots[k] : (int)k;
   }
 }
+// Orders each tile's particles by slot. The binning's atomics leave them in arrival order, and
+// the split of a tile into chunks of kChunk particles fixes each chunk's fixed-point scale:
+// sorted tiles make every staged footprint, and so the whole step, bitwise reproducible.
+// Tiles up to kSortCap particles sort in LDS (bitonic); larger ones rank by counting in global
+// memory through tmp (a dead per-particle array of the binning).
+constexpr int kSortCap = 4096;
+__global__ __launch_bounds__(kBlock) void k_sort_tiles(const int* count, const int* offset, int* perm, int* tmp) {
+  __shared__ int sm[kSortCap];
+  const int t = blockIdx.x;
+  const int c = count[t];
+  if (c <= 1) return;
+  int* seg = perm + offset[t];
+  if (c <= kSortCap) {
+    int n2 = 2;
+    while (n2 < c) n2 <<= 1;
+    for (int i = threadIdx.x; i < n2; i += kBlock) sm[i] = i < c ? seg[i] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < n2; i += kBlock) {
+          const int l = i ^ j;
+          if (l > i) {
+            const int a = sm[i], b = sm[l];
+            if ((a > b) == ((i & k) == 0)) {
+              sm[i] = b;
+              sm[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (int i = threadIdx.x; i < c; i += kBlock) seg[i] = sm[i];
+  } else {
+    int* out = tmp + offset[t];
+    for (int i = threadIdx.x; i < c; i += kBlock) {
+      const int v = seg[i];
+      int r = 0;
+      for (int j = 0; j < c; ++j) r += seg[j] < v;
+      out[r] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += kBlock) seg[i] = out[i];
+  }
+}
 __global__ void k_chunk_counts(const int* count, int NT, int* nch) {
   GRID_STRIDE(t, NT) nch[t] = (count[t] + kChunk - 1) / kChunk;
 }
@@ -792,7 +841,7 @@ template <class T> __global__ void k_from_dense(const T* src, int ncomp, T* dst,
 __global__ __launch_bounds__(kBlock) void k_p2g_mass(PView P, Batch B, float* stage, int* err) {
   __shared__ FxFoot<1> acc;
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+  ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   fx_zero(acc);
   const int k = threadIdx.x;
   bool have = k < c.ch.z;
@@ -837,7 +886,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const floa
   __shared__ float sm[kFoot];
   __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+  ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   fx_zero(acc);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
@@ -903,7 +952,7 @@ __global__ void k_gather3(Batch B, const float* stage, float* out) {
 __global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const float* mass, int* err) {
   __shared__ float sm[kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+  ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int gt = ft[i >> 6];
@@ -938,7 +987,7 @@ __global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const floa
 __global__ __launch_bounds__(kBlock) void k_forces(PView P, Batch B, float* stage, int* err) {
   __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+  ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   fx_zero(acc);
   const int k = threadIdx.x;
   bool have = k < c.ch.z;
@@ -1098,14 +1147,12 @@ __global__ void k_collision_velocities(Batch B, const int8_t* flags, const Colli
 // ------------------------------------------------------------ HVP
 // ForceDifferentialSplatter (Grid.cpp:549-618), factorised:
 //   dF_p = (sum_j d_j grad w_jp^T) F_p ; df_i -= V_p dP(dF_p) F_p^T grad w_ip
+// One chunk's staged footprint of the HVP (LDS work areas passed in; every thread of the block calls).
 template <bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d, const int8_t* flags,
-                                                const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
-  if (sc && sc[B.chunks[blockIdx.x].w].done) return;
-  __shared__ float sd[3][kFoot];
-  __shared__ FxFoot<3> acc;
-  __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+__device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
+                                          const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
+                                          float (*sd)[kFoot], FxFoot<3>& acc, int* ft) {
+  ChunkCtx c = chunk_begin(B, ft, chunk);
   fx_zero(acc);
   __syncthreads();
   load_foot3(c, ft, d, sd, flags, cols, PROJ);
@@ -1176,7 +1223,16 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d
         }
   }
   __syncthreads();
-  fx_store(acc, scl, stage);
+  fx_store(acc, scl, stage, chunk);
+}
+template <bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d, const int8_t* flags,
+                                                const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
+  if (sc && sc[B.chunks[blockIdx.x].w].done) return;
+  __shared__ float sd[3][kFoot];
+  __shared__ FxFoot<3> acc;
+  __shared__ int ft[8];
+  hvp_chunk<PROJ>(P, B, blockIdx.x, d, flags, cols, stage, err, sd, acc, ft);
 }
 
 // dFidXiSplatter (Grid.cpp:466-537) through per-particle quadratic forms: with g = F^T grad w,
@@ -1185,7 +1241,7 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d
 __global__ __launch_bounds__(kBlock) void k_diag(PView P, Batch B, float* stage, int* err) {
   __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+  ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   fx_zero(acc);
   const int k = threadIdx.x;
   bool have = k < c.ch.z;
@@ -1309,13 +1365,14 @@ __global__ void k_rhs(Batch B, const float* mass, const int8_t* flags, const Col
   }
 }
 
+// The per-node work and the scalar updates of each CR kernel are device functions shared with
+// the one-workgroup solver of small bodies (k_small_cr), so that both compute the same bits.
+
 // CR setup part 1 (ConjugateResiduals.cpp:50-67): r = b - A x ; z = M^-1 r ; p = z
-__global__ void k_cr_setup1(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                            const float* stage, const float* x, const float* b, const float* diag, float dt, float* z,
-                            float* p, const CRScalars* sc) {
-  NodeCtx c = node_begin(B);
-  if (sc[c.b].done) return;
-  const float dt2 = dt * dt;
+__device__ __forceinline__ void cr_setup1_node(const Batch& B, const NodeCtx& c, const float* mass,
+                                               const int8_t* flags, const ColliderSet* cols, const float* stage,
+                                               const float* x, const float* b, const float* diag, float dt2, float* z,
+                                               float* p) {
   float xv[3], df[3] = {0.0f, 0.0f, 0.0f}, Ax[3], bb[3], dg[3], zz[3];
   ld3(x, c.i, xv);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
@@ -1327,16 +1384,19 @@ __global__ void k_cr_setup1(Batch B, const float* mass, const int8_t* flags, con
   st3(z, c.i, zz);
   st3(p, c.i, zz);
 }
+__global__ void k_cr_setup1(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                            const float* stage, const float* x, const float* b, const float* diag, float dt, float* z,
+                            float* p, const CRScalars* sc) {
+  NodeCtx c = node_begin(B);
+  if (sc[c.b].done) return;
+  cr_setup1_node(B, c, mass, flags, cols, stage, x, b, diag, dt * dt, z, p);
+}
 // CR setup part 2 (ConjugateResiduals.cpp:69-76 and the first alpha, :86-102):
 // Ap = Ar = A z ; rz = <z, Ar> ; alpha = rz / <Ap, M^-1 Ap>
-__global__ void k_cr_setup2(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                            const float* stage, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
-                            BatchStatus* bst, double* partials) {
-  NodeCtx c = node_begin(B);
-  CRScalars& S = sc[c.b];
-  if (S.done) return;
-  const float dt2 = dt * dt;
-  double s[2] = {0.0, 0.0};
+__device__ __forceinline__ void cr_setup2_node(const Batch& B, const NodeCtx& c, const float* mass,
+                                               const int8_t* flags, const ColliderSet* cols, const float* stage,
+                                               const float* z, const float* diag, float dt2, float* Ap,
+                                               double (&s)[2]) {
   float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, Ar[3], dg[3];
   ld3(z, c.i, zz);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
@@ -1348,29 +1408,36 @@ __global__ void k_cr_setup2(Batch B, const float* mass, const int8_t* flags, con
     s[0] += (double)zz[a] * Ar[a];
     s[1] += (double)Ar[a] * (Ar[a] / dg[a]);
   }
-  double tot[2];
-  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0) {
-    S.rz = tot[0];
-    S.ApdPAp = tot[1];
-    S.iter = 0;
-    if (tot[1] == 0.0) {
-      S.early_exit = 1;
-      set_done(S, bst);
-    } else {
-      S.alpha = tot[0] / tot[1];
-    }
-  }
 }
-// iteration part 1 (ConjugateResiduals.cpp:104-128): x = P(x + alpha p) ; z -= alpha M^-1 Ap ;
-// r = M z ; termination on |r|^2 (SquareMagnitudeTermination.cpp:19-28)
-__global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* cols, const float* p, const float* Ap,
-                              const float* diag, float* x, float* z, CRScalars* sc, BatchStatus* bst, double* partials,
-                              double* hist) {
+// returns true when the solve ends here
+__device__ __forceinline__ bool cr_setup2_finish(CRScalars& S, const double (&tot)[2]) {
+  S.rz = tot[0];
+  S.ApdPAp = tot[1];
+  S.iter = 0;
+  if (tot[1] == 0.0) {
+    S.early_exit = 1;
+    return true;
+  }
+  S.alpha = tot[0] / tot[1];
+  return false;
+}
+__global__ void k_cr_setup2(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                            const float* stage, const float* z, const float* diag, float dt, float* Ap, CRScalars* sc,
+                            BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
-  const float alpha = (float)S.alpha;
-  double s[1] = {0.0};
+  double s[2] = {0.0, 0.0};
+  cr_setup2_node(B, c, mass, flags, cols, stage, z, diag, dt * dt, Ap, s);
+  double tot[2];
+  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0)
+    if (cr_setup2_finish(S, tot)) set_done(S, bst);
+}
+// iteration part 1 (ConjugateResiduals.cpp:104-128): x = P(x + alpha p) ; z -= alpha M^-1 Ap ;
+// r = M z ; termination on |r|^2 (SquareMagnitudeTermination.cpp:19-28)
+__device__ __forceinline__ void cr_update_x_node(const NodeCtx& c, const int8_t* flags, const ColliderSet* cols,
+                                                 const float* p, const float* Ap, const float* diag, float alpha,
+                                                 float* x, float* z, double (&s)[1]) {
   float xv[3], pv[3], ap[3], dg[3], zz[3];
   ld3(x, c.i, xv);
   ld3(p, c.i, pv);
@@ -1388,49 +1455,61 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
     s[0] += (double)r * r;
   }
   st3(z, c.i, zz);
-  double tot[1];
-  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0) {
-    int it = S.iter;
-    S.rnorm2 = tot[0];
-    S.iterations = it + 1;
-    if (hist && it < kHistCap) hist[(int64_t)c.b * kHistCap + it] = tot[0];
-    if (it >= S.max_iters || tot[0] < S.threshold) set_done(S, bst);
-  }
 }
-// iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta
-__global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
-                        const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* bst, double* partials) {
+__device__ __forceinline__ bool cr_update_x_finish(CRScalars& S, const double (&tot)[1], int body, double* hist) {
+  int it = S.iter;
+  S.rnorm2 = tot[0];
+  S.iterations = it + 1;
+  if (hist && it < kHistCap) hist[(int64_t)body * kHistCap + it] = tot[0];
+  return it >= S.max_iters || tot[0] < S.threshold;
+}
+__global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* cols, const float* p, const float* Ap,
+                              const float* diag, float* x, float* z, CRScalars* sc, BatchStatus* bst, double* partials,
+                              double* hist) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
-  const float dt2 = dt * dt;
   double s[1] = {0.0};
+  cr_update_x_node(c, flags, cols, p, Ap, diag, (float)S.alpha, x, z, s);
+  double tot[1];
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
+    if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
+}
+// iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta
+__device__ __forceinline__ void cr_ar_node(const Batch& B, const NodeCtx& c, const float* mass, const int8_t* flags,
+                                           const ColliderSet* cols, const float* stage, const float* z, float dt2,
+                                           float* Ar, double (&s)[1]) {
   float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, ar[3];
   ld3(z, c.i, zz);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
   apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, ar);
   st3(Ar, c.i, ar);
   s[0] = (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
-  double tot[1];
-  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
-    double rz_old = S.rz;
-    S.rz = tot[0];
-    if (rz_old == 0.0) {
-      S.early_exit = 1;
-      set_done(S, bst);
-    } else {
-      S.beta = tot[0] / rz_old;
-    }
-  }
 }
-// iteration part 3 (ConjugateResiduals.cpp:147-151 and the next alpha, :86-102)
-__global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                              CRScalars* sc, BatchStatus* bst, double* partials) {
+__device__ __forceinline__ bool cr_ar_finish(CRScalars& S, const double (&tot)[1]) {
+  double rz_old = S.rz;
+  S.rz = tot[0];
+  if (rz_old == 0.0) {
+    S.early_exit = 1;
+    return true;
+  }
+  S.beta = tot[0] / rz_old;
+  return false;
+}
+__global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
+                        const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
-  const float beta = (float)S.beta;
   double s[1] = {0.0};
+  cr_ar_node(B, c, mass, flags, cols, stage, z, dt * dt, Ar, s);
+  double tot[1];
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0)
+    if (cr_ar_finish(S, tot)) set_done(S, bst);
+}
+// iteration part 3 (ConjugateResiduals.cpp:147-151 and the next alpha, :86-102)
+__device__ __forceinline__ void cr_update_p_node(const NodeCtx& c, const float* z, const float* Ar, const float* diag,
+                                                 float beta, float* p, float* Ap, double (&s)[1]) {
   float zz[3], pv[3], ar[3], ap[3], dg[3];
   ld3(z, c.i, zz);
   ld3(p, c.i, pv);
@@ -1445,16 +1524,157 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
   }
   st3(p, c.i, pv);
   st3(Ap, c.i, ap);
+}
+__device__ __forceinline__ bool cr_update_p_finish(CRScalars& S, const double (&tot)[1]) {
+  bool end = false;
+  S.ApdPAp = tot[0];
+  if (tot[0] == 0.0) {
+    S.early_exit = 1;
+    end = true;
+  } else {
+    S.alpha = S.rz / tot[0];
+  }
+  S.iter = S.iter + 1;
+  return end;
+}
+__global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
+                              CRScalars* sc, BatchStatus* bst, double* partials) {
+  NodeCtx c = node_begin(B);
+  CRScalars& S = sc[c.b];
+  if (S.done) return;
+  double s[1] = {0.0};
+  cr_update_p_node(c, z, Ar, diag, (float)S.beta, p, Ap, s);
   double tot[1];
-  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0) {
-    S.ApdPAp = tot[0];
-    if (tot[0] == 0.0) {
-      S.early_exit = 1;
-      set_done(S, bst);
-    } else {
-      S.alpha = S.rz / tot[0];
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0)
+    if (cr_update_p_finish(S, tot)) set_done(S, bst);
+}
+
+// ------------------------------------------------------------ small bodies
+// A body of at most kSmallBlocks node blocks runs its whole conjugate residual solve
+// (ConjugateResiduals.cpp:50-151) in one workgroup, concurrently with the batched iteration of
+// the large bodies: fragments of a few particles otherwise keep the batched loop launching
+// for tens of iterations after the large bodies converge. The host marks such bodies
+// kDoneSmall after the rhs, which every batched kernel skips, and counts them in
+// BatchStatus::small_bodies. The node work, the HVP chunk scatter, the reduction order (block
+// tree per node block, then block totals in block order, as body_sum) and the scalar updates
+// are those of the batched kernels, so the iterates are bitwise the same.
+template <int NV> struct SmallRed {
+  double w[kBlock / 64][kSmallBlocks][NV];
+  double sm[kBlock / 64][NV];
+  double res[NV];
+};
+// every thread of the workgroup calls; returns the body total in every thread
+template <int NV, class Fn>
+__device__ __forceinline__ void small_sweep(const BodyGrid& G, int body, SmallRed<NV>& R, Fn&& fn,
+                                            double (&tot)[NV]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j = 0; j < G.nblocks; ++j) {
+    NodeCtx c = node_in(G, body, G.blockBase + j);
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    fn(c, v);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+      if (lane == 0) R.w[wid][j][k] = v[k];
     }
-    S.iter = S.iter + 1;
+  }
+  __syncthreads();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    acc[k] = 0.0;
+    const int t = threadIdx.x;
+    if (t < G.nblocks) acc[k] += ((R.w[0][t][k] + R.w[1][t][k]) + R.w[2][t][k]) + R.w[3][t][k];
+  }
+  block_sum<NV>(acc, R.sm);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) R.res[k] = acc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = R.res[k];
+}
+__global__ void k_small_claim(const int* list, int n, CRScalars* sc, BatchStatus* bst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CRScalars& S = sc[list[i]];
+  if (S.done == 0) {
+    S.done = kDoneSmall;
+    atomicAdd(&bst->small_bodies, 1);
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int* list, const float* mass,
+                                                     const int8_t* flags, const ColliderSet* cols, float dt,
+                                                     float* stage, const float* bvec, const float* diag, float* x,
+                                                     float* z, float* p, float* Ap, float* Ar, CRScalars* sc,
+                                                     double* hist, int* err) {
+  __shared__ float sd[3][kFoot];
+  __shared__ FxFoot<3> acc;
+  __shared__ int ft[8];
+  __shared__ SmallRed<2> R2;
+  __shared__ SmallRed<1> R1;
+  __shared__ CRScalars L;
+  __shared__ int end;
+  const int body = list[blockIdx.x];
+  if (sc[body].done != kDoneSmall) return;  // finished at the rhs: not claimed
+  const BodyGrid G = B.bodies[body];
+  if (G.nblocks > kSmallBlocks) {  // the host only lists bodies that fit
+    if (threadIdx.x == 0) atomicOr(err, ERR_INTERNAL);
+    return;
+  }
+  if (threadIdx.x == 0) {
+    L = sc[body];
+    end = 0;
+  }
+  const float dt2 = dt * dt;
+  const int lastTile = G.tileBase + G.NT - 1;
+  const int c0 = B.tileChunk0[G.tileBase];
+  const int c1 = B.tileChunk0[lastTile] + B.tileChunkN[lastTile];
+  auto hvp = [&](const float* d) {
+    for (int ch = c0; ch < c1; ++ch) hvp_chunk<true>(P, B, ch, d, flags, cols, stage, err, sd, acc, ft);
+    __syncthreads();
+  };
+  __syncthreads();
+  // setup
+  hvp(x);
+  for (int j = 0; j < G.nblocks; ++j)
+    cr_setup1_node(B, node_in(G, body, G.blockBase + j), mass, flags, cols, stage, x, bvec, diag, dt2, z, p);
+  __syncthreads();
+  hvp(z);
+  double t2[2];
+  small_sweep<2>(G, body, R2, [&](const NodeCtx& c, double (&v)[2]) {
+    cr_setup2_node(B, c, mass, flags, cols, stage, z, diag, dt2, Ap, v);
+  }, t2);
+  if (threadIdx.x == 0) end = cr_setup2_finish(L, t2);
+  __syncthreads();
+  double t1[1];
+  while (!end) {
+    const float alpha = (float)L.alpha;
+    small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+      cr_update_x_node(c, flags, cols, p, Ap, diag, alpha, x, z, v);
+    }, t1);
+    if (threadIdx.x == 0) end = cr_update_x_finish(L, t1, body, hist);
+    __syncthreads();
+    if (end) break;
+    hvp(z);
+    small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+      cr_ar_node(B, c, mass, flags, cols, stage, z, dt2, Ar, v);
+    }, t1);
+    if (threadIdx.x == 0) end = cr_ar_finish(L, t1);
+    __syncthreads();
+    if (end) break;
+    const float beta = (float)L.beta;
+    small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+      cr_update_p_node(c, z, Ar, diag, beta, p, Ap, v);
+    }, t1);
+    if (threadIdx.x == 0) end = cr_update_p_finish(L, t1);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    L.done = 1;
+    sc[body] = L;
   }
 }
 // ------------------------------------------------------------ flexible conjugate residual
@@ -1806,7 +2026,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
   __shared__ float sv[3][kFoot];
   __shared__ float sp[DO_V ? 3 : 1][kFoot];
   __shared__ int ft[8];
-  ChunkCtx c = chunk_begin(B, ft);
+  ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   __syncthreads();
   for (int i = threadIdx.x; i < kFoot; i += kBlock) {
     int gt = ft[i >> 6];
@@ -2201,6 +2421,19 @@ void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const 
 void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
                         CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
   NODE_LAUNCH(k_cr_update_p, B, z, Ar, diag, p, Ap, sc, st, partials);
+}
+void launch_sort_tiles(const int* count, const int* offset, int* perm, int* tmp, int NT, hipStream_t s) {
+  if (NT > 0) hipLaunchKernelGGL(k_sort_tiles, dim3(NT), dim3(kBlock), 0, s, count, offset, perm, tmp);
+}
+void launch_small_claim(const int* list, int n, CRScalars* sc, BatchStatus* st, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_small_claim, dim3((n + 255) / 256), dim3(256), 0, s, list, n, sc, st);
+}
+void launch_small_cr(PView P, const Batch& B, const int* list, int n, const float* mass, const int8_t* flags,
+                     const ColliderSet* cols, float dt, float* stage, const float* b, const float* diag, float* x,
+                     float* z, float* p, float* Ap, float* Ar, CRScalars* sc, double* hist, int* err, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_small_cr, dim3(n), dim3(kBlock), 0, s, P, B, list, mass, flags, cols, dt, stage, b, diag, x,
+                       z, p, Ap, Ar, sc, hist, err);
 }
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,

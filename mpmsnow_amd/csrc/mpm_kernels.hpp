@@ -102,6 +102,14 @@ void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const 
                   hipStream_t s);
 void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
                         CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+// order each tile's particles by slot (tmp: n ints of scratch, n >= particles binned)
+void launch_sort_tiles(const int* count, const int* offset, int* perm, int* tmp, int NT, hipStream_t s);
+// small bodies: claim the listed bodies still iterating (done = kDoneSmall), then solve each in
+// one workgroup (setup and the whole CR loop)
+void launch_small_claim(const int* list, int n, CRScalars* sc, BatchStatus* st, hipStream_t s);
+void launch_small_cr(PView P, const Batch& B, const int* list, int n, const float* mass, const int8_t* flags,
+                     const ColliderSet* cols, float dt, float* stage, const float* b, const float* diag, float* x,
+                     float* z, float* p, float* Ap, float* Ar, CRScalars* sc, double* hist, int* err, hipStream_t s);
 // ---- flexible conjugate residual (learned preconditioner path)
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,

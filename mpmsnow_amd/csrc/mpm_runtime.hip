@@ -63,24 +63,33 @@ void raise_errors(int e) {
   if (e & ERR_HARDNESS) throw std::runtime_error("infinite hardness!!!");
   if (e & ERR_NAN_PVEL) throw std::runtime_error("nans in particle velocity data!");
   if (e & ERR_STENCIL) throw std::runtime_error("particle outside its grid tile (internal binning error)");
+  if (e & ERR_INTERNAL) throw std::runtime_error("launch does not fit the batch layout (internal error)");
 }
 
 // Kernel timing for measurement: HIP events recorded on the library's stream around each
 // instrumented launch, resolved at the next synchronisation point. Algorithmic bytes of a
 // launch = bytes/particle x particles + bytes/node x nodes with m > 0, over the bodies the
-// launch worked on (bodies whose solve had finished skip the launch).
+// launch worked on (bodies whose solve had finished skip the launch). The one-workgroup solver
+// of small bodies is one launch for all their iterations: its bytes are those of an HVP and
+// the three node kernels per iteration (plus the setup's two).
 enum ProfKind {
-  PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_LPC, PK_COUNT
+  PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_LPC, PK_CR_SMALL,
+  PK_COUNT
 };
-const char* const kProfNames[PK_COUNT] = {"p2g_mass", "p2g_vel", "forces", "hvp", "diag",
-                                          "g2p", "cr_update_x", "cr_ar", "cr_update_p", "lpc_forward"};
+const char* const kProfNames[PK_COUNT] = {"p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p",
+                                          "cr_update_x", "cr_ar", "cr_update_p", "lpc_forward", "cr_small"};
 // the network is MFMA work: its launches are timed but carry no HBM byte count
-const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0, 0};
-const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84, 0};
+const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0, 0, 172};
+const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84, 0, 85 + 41 + 84 + 29};
+// ProfRec::iter values other than CR-loop iterations
+constexpr int kProfOnce = -1;     // once per solve, every body of the batch
+constexpr int kProfSetup = -2;    // CR setup of the bodies the batched loop solves
+constexpr int kProfSmall = -3;    // the small-body solver
 
 }  // namespace
 
 struct SolveRecord {
+  bool small = false;  // solved by the one-workgroup small-body solver
   int iterations = 0;
   int early_exit = 0;
   int fallbacks = 0;  // learned preconditioner: switched to the diagonal one
@@ -103,6 +112,7 @@ struct GridBatch {
       dense2;
   DevBuf bbox, mom, cnt;
   DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
+  DevBuf smallList;               // bodies handed to the small-body solver
   LpcLevels levels;
   bool levelsBuilt = false;
   uint64_t version = 0;  // bumped by every build
@@ -167,6 +177,9 @@ struct LpcCapture {
 struct mpm_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // small-body solver, concurrent with the batched CR loop
+  hipEvent_t smallReady = nullptr, smallDone = nullptr;  // stream -> stream2 -> stream ordering
+  int smallMaxParticles = 2048;   // bodies up to this size and kSmallBlocks node blocks (0: off)
   std::string err;
   int64_t n = 0;
   bool model = false;
@@ -203,9 +216,10 @@ struct mpm_context {
   // measurement
   struct ProfRec {
     int kind;
-    int iter;  // CR loop iteration (-1: once per solve)
+    int iter;  // CR loop iteration, or kProfOnce / kProfSetup / kProfSmall
     hipEvent_t a, b;
     const GridBatch* batch;
+    hipStream_t stream;
   };
   unsigned int profMask = 0;  // kernel classes being timed
   std::vector<ProfRec> profPending;
@@ -255,16 +269,16 @@ struct mpm_context {
     HIPCHECK(hipEventCreate(&e));
     return e;
   }
-  int prof_begin(int kind, const GridBatch* g, int iter = -1) {
+  int prof_begin(int kind, const GridBatch* g, int iter = kProfOnce, hipStream_t on = nullptr) {
     if (!(profMask & (1u << kind))) return -1;
-    ProfRec r{kind, iter, new_event(), new_event(), g};
-    HIPCHECK(hipEventRecord(r.a, stream));
+    ProfRec r{kind, iter, new_event(), new_event(), g, on ? on : stream};
+    HIPCHECK(hipEventRecord(r.a, r.stream));
     profPending.push_back(r);
     return (int)profPending.size() - 1;
   }
   void prof_end(int idx) {
     if (idx < 0) return;
-    HIPCHECK(hipEventRecord(profPending[idx].b, stream));
+    HIPCHECK(hipEventRecord(profPending[idx].b, profPending[idx].stream));
   }
   void prof_resolve() {
     if (profPending.empty()) return;
@@ -276,8 +290,16 @@ struct mpm_context {
         if (r.batch) {
           for (const SolveRecord& s : r.batch->rec) {
             // a body takes part in CR-loop launch `iter` if its solve had not finished
-            bool active = r.iter < 0 || (r.kind == PK_CR_X ? r.iter < s.iterations : r.iter + 1 < s.iterations);
-            if (active) bytes += kProfBytesPerParticle[r.kind] * s.np + kProfBytesPerNode[r.kind] * s.active;
+            double times = 0;
+            if (r.iter == kProfOnce)
+              times = 1;
+            else if (r.iter == kProfSmall)
+              times = s.small ? s.iterations + 2 : 0;
+            else if (r.iter == kProfSetup)
+              times = s.small ? 0 : 1;
+            else if (!s.small)
+              times = (r.kind == PK_CR_X ? r.iter < s.iterations : r.iter + 1 < s.iterations) ? 1 : 0;
+            bytes += times * (kProfBytesPerParticle[r.kind] * s.np + kProfBytesPerNode[r.kind] * s.active);
           }
         }
         profMs[r.kind] += ms;
@@ -389,6 +411,7 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   launch_chunk_counts(count.as<int>(), NT, nch.as<int>(), s);
   scan_exclusive(nch.as<int>(), chunkOff.as<int>(), NT, totals.as<int>() + 1, scantmp.as<int>(), s);
   launch_scatter(slotList, key.as<int>(), rank.as<int>(), offset.as<int>(), n, perm.as<int>(), s);
+  launch_sort_tiles(count.as<int>(), offset.as<int>(), perm.as<int>(), rank.as<int>(), NT, s);
   int tot[2] = {0, 0};
   int e = 0;
   HIPCHECK(hipMemcpyAsync(tot, totals.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -478,10 +501,13 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
   Batch B = view();
   // per-body scalars
   size_t scb = sizeof(CRScalars) * nb;
-  c.pinned.ensure(scb + 4096);
+  c.pinned.ensure(scb + 4096 + sizeof(int) * nb);
   CRScalars* hsc = c.pinned.as<CRScalars>();
   std::memset(hsc, 0, scb);
-  for (int bi = 0; bi < nb; ++bi) hsc[bi].max_iters = maxIters;
+  for (int bi = 0; bi < nb; ++bi) {
+    hsc[bi].max_iters = maxIters;
+    rec[bi].small = false;
+  }
   HIPCHECK(hipMemcpyAsync(scal.p, hsc, scb, hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemsetAsync(status.p, 0, sizeof(BatchStatus), s));
   hist.ensure(sizeof(double) * kHistCap * nb);
@@ -554,11 +580,33 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
       }
     }
   } else {
+    // bodies of a few node blocks run their whole solve in one workgroup each, on the second
+    // stream, while the batched loop below iterates the rest
+    int nsmall = 0;
+    // (the iteration cap bounds the persistent kernel's run time)
+    if (!hooks && c.cap.want == 0 && c.smallMaxParticles > 0 && maxIters <= (1 << 16)) {
+      int* list = reinterpret_cast<int*>(reinterpret_cast<char*>(c.pinned.p) + scb + 4096);
+      for (int bi = 0; bi < nb; ++bi) {
+        rec[bi].small = hb[bi].nblocks <= kSmallBlocks && hb[bi].np <= c.smallMaxParticles;
+        if (rec[bi].small) list[nsmall++] = bi;
+      }
+      if (nsmall > 0) {
+        smallList.ensure(sizeof(int) * nsmall);
+        HIPCHECK(hipMemcpyAsync(smallList.p, list, sizeof(int) * nsmall, hipMemcpyHostToDevice, s));
+        launch_small_claim(smallList.as<int>(), nsmall, sc(), st(), s);
+        HIPCHECK(hipEventRecord(c.smallReady, s));
+        HIPCHECK(hipStreamWaitEvent(c.stream2, c.smallReady, 0));
+        int ps = c.prof_begin(PK_CR_SMALL, this, kProfSmall, c.stream2);
+        launch_small_cr(c.view(), B, smallList.as<int>(), nsmall, f(mass), flags.as<int8_t>(), c.cols(), dt, f(stageA),
+                        f(b), f(diag), f(x), f(z), f(p), f(Ap), f(Ar), sc(), hist.as<double>(), c.errp(), c.stream2);
+        c.prof_end(ps);
+      }
+    }
     // CR setup (ConjugateResiduals.cpp:50-76)
-    hvp(f(x), true, f(stageA), sc(), -1);
+    hvp(f(x), true, f(stageA), sc(), kProfSetup);
     launch_cr_setup1(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
                      s);
-    hvp(f(z), true, f(stageA), sc(), -1);
+    hvp(f(z), true, f(stageA), sc(), kProfSetup);
     launch_cr_setup2(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
                      partials.as<double>(), s);
     fetch_status();
@@ -606,7 +654,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     if (!hooks) {
       // all bodies iterate in lockstep; the host polls the done counter once per batch of iterations
       int it = 0;
-      while (hst->bodies_done < nb) {
+      while (hst->bodies_done + hst->small_bodies < nb) {
         if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
         int batch = it == 0 ? std::max(1, c.iterHint) : 8;
         for (int k = 0; k < batch; ++k, ++it) {
@@ -614,6 +662,10 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
           iterate_rest(it);
         }
         fetch_status();
+      }
+      if (nsmall > 0) {
+        HIPCHECK(hipEventRecord(c.smallDone, c.stream2));
+        HIPCHECK(hipStreamWaitEvent(s, c.smallDone, 0));
       }
       if (capWant > 0 && cap.n > 0) train_lpc(dt, hsc, scb);
     } else {
@@ -668,7 +720,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     r.active = hsc[bi].active_nodes;
     r.fallbacks = learned ? hsc[bi].fallbacks : 0;
     r.history.clear();
-    maxIt = std::max(maxIt, r.iterations);
+    if (!r.small) maxIt = std::max(maxIt, r.iterations);
     raise_errors(hsc[bi].errors);
   }
   if (maxIt > 0) (learned ? c.iterHintFcr : c.iterHint) = maxIt;
@@ -908,6 +960,9 @@ int mpm_create(mpm_context** out, int device) {
   c->simBatch.ctx = c.get();
   int rc = guarded(c.get(), [&] {
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&c->smallReady, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&c->smallDone, hipEventDisableTiming));
     c->errflag.ensure(sizeof(int));
     c->cols_dev.ensure(sizeof(ColliderSet));
     c->grav_dev.ensure(sizeof(float) * 3 * 64);
@@ -923,9 +978,13 @@ int mpm_destroy(mpm_context* ctx) {
   if (!ctx) return MPM_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  hipStream_t s = ctx->stream;
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  hipStream_t s = ctx->stream, s2 = ctx->stream2;
+  if (ctx->smallReady) (void)hipEventDestroy(ctx->smallReady);
+  if (ctx->smallDone) (void)hipEventDestroy(ctx->smallDone);
   delete ctx;
   if (s) (void)hipStreamDestroy(s);
+  if (s2) (void)hipStreamDestroy(s2);
   return MPM_OK;
 }
 
@@ -1506,6 +1565,13 @@ int mpm_grid_lpc_gradients(mpm_grid* grid, const float* r, const float* diag, co
       HIPCHECK(hipMemcpyAsync(grad, c.net.G.p, sizeof(float) * c.net.nparams, hipMemcpyDeviceToHost, c.stream));
       c.sync();
     }
+  });
+}
+
+int mpm_set_small_bodies(mpm_context* ctx, int max_particles) {
+  return guarded(ctx, [&] {
+    if (max_particles < 0) throw InvalidArg("max_particles must be >= 0");
+    ctx->smallMaxParticles = max_particles;
   });
 }
 
