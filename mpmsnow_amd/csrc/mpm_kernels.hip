@@ -30,9 +30,17 @@ __device__ __forceinline__ int tile_id(const BodyGrid& G, int tx, int ty, int tz
   if (tx < 0 || ty < 0 || tz < 0 || tx >= G.T[0] || ty >= G.T[1] || tz >= G.T[2]) return -1;
   return G.tileBase + tx + G.T[0] * (ty + G.T[1] * tz);
 }
-// footprint-relative node (rx, ry, rz) in [0, 8) -> LDS index in [0, 512): 64 * footprint tile + local
-__device__ __forceinline__ int foot_index(int rx, int ry, int rz) {
-  return ((((rz >> 2) << 2) | ((ry >> 2) << 1) | (rx >> 2)) << 6) | ((rz & 3) << 4) | ((ry & 3) << 2) | (rx & 3);
+// LDS footprints are linear 8x8x8 boxes: node (rx, ry, rz) in [0, 8) of a chunk's 2x2x2-tile
+// footprint sits at rx + 8 ry + 64 rz, so a particle's 64 stencil nodes are constant offsets
+// from its base node (immediate LDS offsets, no per-node address arithmetic).
+__device__ __forceinline__ int foot_index(int rx, int ry, int rz) { return rx + 8 * ry + 64 * rz; }
+// Loops that move a footprint between LDS and the tile-major global layout run over j = 64 f + l
+// (footprint tile f = (x >> 2) | (y >> 2) << 1 | (z >> 2) << 2, local node
+// l = (x & 3) | (y & 3) << 2 | (z & 3) << 4), so global accesses stay contiguous; this gives the
+// LDS index of j.
+__device__ __forceinline__ int foot_lds(int j) {
+  const int f = j >> 6, l = j & 63;
+  return ((f & 1) << 2 | (l & 3)) + 8 * (((f >> 1) & 1) << 2 | ((l >> 2) & 3)) + 64 * ((f >> 2) << 2 | (l >> 4));
 }
 __device__ __forceinline__ void node_pos(const BodyGrid& G, int i, int j, int k, float x[3]) {
   x[0] = G.h * (float)i + G.gmin[0];
@@ -100,9 +108,8 @@ __device__ __forceinline__ bool chunk_stencil(const ChunkCtx& c, const float x[3
 // load a batch 3-vector footprint into LDS (optionally projected)
 __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, const float* d, float (*sd)[kFoot],
                                            const int8_t* flags, const ColliderSet* cols, bool proj) {
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    int f = i >> 6, l = i & 63;
-    int gt = ft[f];
+  for (int j = threadIdx.x; j < kFoot; j += kBlock) {
+    const int gt = ft[j >> 6], l = j & 63, i = foot_lds(j);
     float v[3] = {0.0f, 0.0f, 0.0f};
     if (gt >= 0) {
       int64_t b = (int64_t)gt * 192 + l;
@@ -113,8 +120,7 @@ __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, con
         int fl = flags[(int64_t)gt * 64 + l];
         if (fl != -1) {
           float xn[3];
-          node_pos(c.G, c.ox + (f & 1) * 4 + (l & 3), c.oy + ((f >> 1) & 1) * 4 + ((l >> 2) & 3),
-                   c.oz + (f >> 2) * 4 + (l >> 4), xn);
+          node_pos(c.G, c.ox + (i & 7), c.oy + ((i >> 3) & 7), c.oz + (i >> 6), xn);
           project(cols, fl, xn, v);
         }
       }
@@ -157,8 +163,9 @@ struct FxScale {
   float S, invS;
   bool bad;
 };
-// T: per-thread contribution bounds, summed over the block (every thread calls)
-__device__ __forceinline__ FxScale fx_scale(float bound) {
+// T: per-thread contribution bounds, summed over the block (every thread calls); sums of scaled
+// contributions stay below 2^(top + 1)
+__device__ __forceinline__ FxScale fx_scale(float bound, int top = 42) {
   __shared__ float red[kBlock / 64];
   float v = bound;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -169,7 +176,7 @@ __device__ __forceinline__ FxScale fx_scale(float bound) {
   r.bad = !(T <= 3.0e38f);  // NaN or infinite
   int k = 126;
   if (T > 0.0f && !r.bad) {
-    k = 42 - ilogbf(T);  // T < 2^(ilogb + 1): sums stay below 2^43
+    k = top - ilogbf(T);  // T < 2^(ilogb + 1)
     k = k > 126 ? 126 : (k < -126 ? -126 : k);
   }
   r.S = __int_as_float((k + 127) << 23);
@@ -181,8 +188,8 @@ template <int NC>
 __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc, float* stage, int slot = -1) {
   float* dst = stage + (int64_t)(slot < 0 ? (int)blockIdx.x : slot) * kStage;
   const double inv = (double)sc.invS;
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    int f = i >> 6, l = i & 63;
+  for (int j = threadIdx.x; j < kFoot; j += kBlock) {
+    const int f = j >> 6, l = j & 63, i = foot_lds(j);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       double v = ((double)a.hi[c][i] * 4194304.0 + (double)a.lo[c][i]) * inv;
@@ -190,6 +197,107 @@ __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc,
     }
   }
 }
+// Single-word variant for the HVP, the solver's inner kernel: one integer per value and a
+// scale per component c from T_c, the chunk's sum of per-particle bounds on that component's
+// contributions (tight: the stencil's gradient weights are at most (2/3)^3 / h), so that sums
+// stay below 2^30; each contribution is rounded to nearest. Resolution 2^-30 of T_c: below the
+// float rounding of the node sums it feeds, with half the LDS atomics and a fifth of the
+// conversion work of the split form.
+template <int NC> struct FxFoot1 {
+  int v[NC][kFoot];
+};
+struct FxScale3 {
+  float S[3], invS[3];
+  bool bad;
+};
+__device__ __forceinline__ FxScale3 fx_scale3(const float bound[3], int top) {
+  __shared__ float red[3][kBlock / 64];
+  float v[3] = {bound[0], bound[1], bound[2]};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_xor(v[c], off, 64);
+    if ((threadIdx.x & 63) == 0) red[c][threadIdx.x >> 6] = v[c];
+  }
+  __syncthreads();
+  FxScale3 r;
+  r.bad = false;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float T = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
+    r.bad = r.bad || !(T <= 3.0e38f);
+    int k = 126;
+    if (T > 0.0f && T <= 3.0e38f) {
+      k = top - ilogbf(T);
+      k = k > 126 ? 126 : (k < -126 ? -126 : k);
+    }
+    r.S[c] = __int_as_float((k + 127) << 23);
+    r.invS[c] = __int_as_float((127 - k) << 23);
+  }
+  return r;
+}
+template <int NC> __device__ __forceinline__ void fx_zero(FxFoot1<NC>& a) {
+  for (int i = threadIdx.x; i < kFoot; i += kBlock)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a.v[c][i] = 0;
+}
+__device__ __forceinline__ void fx_store(const FxFoot1<3>& a, const FxScale3& sc, float* stage, int slot) {
+  float* dst = stage + (int64_t)slot * kStage;
+  for (int j = threadIdx.x; j < kFoot; j += kBlock) {
+    const int f = j >> 6, l = j & 63, i = foot_lds(j);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      dst[f * 192 + c * 64 + l] = sc.bad ? __int_as_float(0x7fc00000) : (float)a.v[c][i] * sc.invS[c];
+  }
+}
+constexpr float kGradWeightMax = 0.2963f;  // (2/3)^3 rounded up: |ghat_c| h over the stencil
+constexpr int kFx1Top = 29;
+
+// Separable stencil sums. The gradient weight of stencil node (i, j, k) is
+// ghat = (dwx_i wy_j wz_k, wx_i dwy_j wz_k, wx_i wy_j dwz_k) (st_grad returns -ghat); for a
+// footprint value v (LDS, linear layout, v[0] = the particle's base node) this returns
+// out[c] = sum_ijk v(i, j, k) ghat_c(i, j, k) by contracting one axis at a time (188 FMAs
+// instead of 15 operations per node).
+__device__ __forceinline__ void sep_gather(const float* v, const Stencil& st, float out[3]) {
+  float ex[4], fy[4], hz[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ex[k] = fy[k] = hz[k] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float d = v[i + 8 * j + 64 * k];
+        a0 = fmaf(d, st.w[0][i], a0);
+        a1 = fmaf(d, st.dw[0][i], a1);
+      }
+      ex[k] = fmaf(a1, st.w[1][j], ex[k]);
+      fy[k] = fmaf(a0, st.dw[1][j], fy[k]);
+      hz[k] = fmaf(a0, st.w[1][j], hz[k]);
+    }
+  }
+  out[0] = out[1] = out[2] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    out[0] = fmaf(ex[k], st.w[2][k], out[0]);
+    out[1] = fmaf(fy[k], st.w[2][k], out[1]);
+    out[2] = fmaf(hz[k], st.dw[2][k], out[2]);
+  }
+}
+// acc(i, j, k) += rint(f . ghat(i, j, k)) over the stencil, with f . ghat evaluated as
+// P(j, k) dwx_i + Q(j, k) wx_i, P = f0 wy_j wz_k, Q = f1 dwy_j wz_k + f2 wy_j dwz_k
+__device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f0, float f1, float f2) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float P = f0 * (st.w[1][j] * st.w[2][k]);
+      const float Q = fmaf(f1, st.dw[1][j] * st.w[2][k], f2 * (st.w[1][j] * st.dw[2][k]));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) atomicAdd(&acc[i + 8 * j + 64 * k], __float2int_rn(fmaf(P, st.dw[0][i], Q * st.w[0][i])));
+    }
+}
+
 __device__ __forceinline__ float abs_sum9(const float m[9]) {
   float s = 0.0f;
 #pragma unroll
@@ -889,9 +997,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const floa
   ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   fx_zero(acc);
   __syncthreads();
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    int gt = ft[i >> 6];
-    sm[i] = gt >= 0 ? mass[(int64_t)gt * 64 + (i & 63)] : 0.0f;
+  for (int j = threadIdx.x; j < kFoot; j += kBlock) {
+    int gt = ft[j >> 6];
+    sm[foot_lds(j)] = gt >= 0 ? mass[(int64_t)gt * 64 + (j & 63)] : 0.0f;
   }
   const int k = threadIdx.x;
   bool have = k < c.ch.z;
@@ -954,9 +1062,9 @@ __global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const floa
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   __syncthreads();
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    int gt = ft[i >> 6];
-    sm[i] = gt >= 0 ? mass[(int64_t)gt * 64 + (i & 63)] : 0.0f;
+  for (int j = threadIdx.x; j < kFoot; j += kBlock) {
+    int gt = ft[j >> 6];
+    sm[foot_lds(j)] = gt >= 0 ? mass[(int64_t)gt * 64 + (j & 63)] : 0.0f;
   }
   __syncthreads();
   const float cellVolume = c.G.h * c.G.h * c.G.h;
@@ -1151,7 +1259,7 @@ __global__ void k_collision_velocities(Batch B, const int8_t* flags, const Colli
 template <bool PROJ>
 __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
                                           const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
-                                          float (*sd)[kFoot], FxFoot<3>& acc, int* ft) {
+                                          float (*sd)[kFoot], FxFoot1<3>& acc, int* ft) {
   ChunkCtx c = chunk_begin(B, ft, chunk);
   fx_zero(acc);
   __syncthreads();
@@ -1161,35 +1269,28 @@ __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int ch
   bool have = k < c.ch.z;
   float fm[9];
   Stencil st;
-  int rx = 0, ry = 0, rz = 0;
+  int base = 0;
   if (have) {
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    if (!chunk_stencil(c, x, true, st, rx, ry, rz)) {
+    make_stencil_fast(x, c.G.gmin, c.G.h, 1.0f / c.G.h, st);
+    const int rx = st.base[0] - c.ox, ry = st.base[1] - c.oy, rz = st.base[2] - c.oz;
+    if (!(rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4)) {
       atomicOr(err, ERR_STENCIL);
       have = false;
     } else {
+      base = foot_index(rx, ry, rz);
+      // G[a + 3 c] = sum_i d_a(i) grad_c w_i = -sum_i d_a(i) ghat_c(i)
       float G[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) G[i] = 0.0f;
-#pragma unroll
-      for (int cz = 0; cz < 4; ++cz)
-#pragma unroll
-        for (int cy = 0; cy < 4; ++cy)
-#pragma unroll
-          for (int cx = 0; cx < 4; ++cx) {
-            float gw[3];
-            st_grad(st, cx, cy, cz, gw);
-            int idx = foot_index(rx + cx, ry + cy, rz + cz);
-            float d0 = sd[0][idx], d1 = sd[1][idx], d2 = sd[2][idx];
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-              G[0 + 3 * cc] += d0 * gw[cc];
-              G[1 + 3 * cc] += d1 * gw[cc];
-              G[2 + 3 * cc] += d2 * gw[cc];
-            }
-          }
+      for (int a = 0; a < 3; ++a) {
+        float g[3];
+        sep_gather(&sd[a][base], st, g);
+        G[a] = -g[0];
+        G[a + 3] = -g[1];
+        G[a + 6] = -g[2];
+      }
       float F[9], R[9], Fi[9], Gi[9], dF[9], dP[9];
       load_m9(P, C_F, s, F);
       load_m9(P, C_R, s, R);
@@ -1203,24 +1304,18 @@ __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int ch
       mat_mul_bt(dP, F, fm);
     }
   }
-  const FxScale scl = fx_scale(have ? abs_sum9(fm) / c.G.h : 0.0f);
+  // df_a(i) -= sum_c fm[a + 3 c] grad_c w_i = sum_c fm[a + 3 c] ghat_c(i)
+  float bound[3] = {0.0f, 0.0f, 0.0f};
+  if (have) {
+    const float gmax = kGradWeightMax / c.G.h;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bound[a] = (fabsf(fm[a]) + fabsf(fm[a + 3]) + fabsf(fm[a + 6])) * gmax;
+  }
+  const FxScale3 scl = fx_scale3(bound, kFx1Top);
   if (have) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) fm[i] = -(fm[i] * scl.S);
-#pragma unroll
-    for (int cz = 0; cz < 4; ++cz)
-#pragma unroll
-      for (int cy = 0; cy < 4; ++cy)
-#pragma unroll
-        for (int cx = 0; cx < 4; ++cx) {
-          float gw[3], t[3];
-          st_grad(st, cx, cy, cz, gw);
-          mat_vec(fm, gw, t);
-          int idx = foot_index(rx + cx, ry + cy, rz + cz);
-          fx_add(&acc.hi[0][idx], &acc.lo[0][idx], t[0]);
-          fx_add(&acc.hi[1][idx], &acc.lo[1][idx], t[1]);
-          fx_add(&acc.hi[2][idx], &acc.lo[2][idx], t[2]);
-        }
+    for (int a = 0; a < 3; ++a)
+      sep_scatter(&acc.v[a][base], st, fm[a] * scl.S[a], fm[a + 3] * scl.S[a], fm[a + 6] * scl.S[a]);
   }
   __syncthreads();
   fx_store(acc, scl, stage, chunk);
@@ -1230,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d
                                                 const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
   if (sc && sc[B.chunks[blockIdx.x].w].done) return;
   __shared__ float sd[3][kFoot];
-  __shared__ FxFoot<3> acc;
+  __shared__ FxFoot1<3> acc;
   __shared__ int ft[8];
   hvp_chunk<PROJ>(P, B, blockIdx.x, d, flags, cols, stage, err, sd, acc, ft);
 }
@@ -1611,7 +1706,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
                                                      float* z, float* p, float* Ap, float* Ar, CRScalars* sc,
                                                      double* hist, int* err) {
   __shared__ float sd[3][kFoot];
-  __shared__ FxFoot<3> acc;
+  __shared__ FxFoot1<3> acc;
   __shared__ int ft[8];
   __shared__ SmallRed<2> R2;
   __shared__ SmallRed<1> R1;
@@ -2028,9 +2123,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   __syncthreads();
-  for (int i = threadIdx.x; i < kFoot; i += kBlock) {
-    int gt = ft[i >> 6];
-    int64_t b = (int64_t)gt * 192 + (i & 63);
+  for (int j = threadIdx.x; j < kFoot; j += kBlock) {
+    int gt = ft[j >> 6], i = foot_lds(j);
+    int64_t b = (int64_t)gt * 192 + (j & 63);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       sv[a][i] = gt >= 0 ? vel[b + 64 * a] : 0.0f;
