@@ -30,17 +30,28 @@ __device__ __forceinline__ int tile_id(const BodyGrid& G, int tx, int ty, int tz
   if (tx < 0 || ty < 0 || tz < 0 || tx >= G.T[0] || ty >= G.T[1] || tz >= G.T[2]) return -1;
   return G.tileBase + tx + G.T[0] * (ty + G.T[1] * tz);
 }
-// LDS footprints are linear 8x8x8 boxes: node (rx, ry, rz) in [0, 8) of a chunk's 2x2x2-tile
-// footprint sits at rx + 8 ry + 64 rz, so a particle's 64 stencil nodes are constant offsets
-// from its base node (immediate LDS offsets, no per-node address arithmetic).
-__device__ __forceinline__ int foot_index(int rx, int ry, int rz) { return rx + 8 * ry + 64 * rz; }
+// LDS footprints are linear boxes: node (rx, ry, rz) in [0, 8) of a chunk's 2x2x2-tile footprint
+// sits at rx + 8 ry + kFootSZ rz, so a particle's 64 stencil nodes are constant offsets from its
+// base node (immediate LDS offsets, no per-node address arithmetic). The z stride 66 (not 64)
+// spreads the bases of a wave's particles over all 32 LDS banks: with 64 the bank ignores rz and
+// a wave's scatter and gather run at about twice the bank conflicts.
+constexpr int kFootSZ = 66;
+constexpr int kFootLds = 7 + 7 * 8 + 7 * kFootSZ + 1;  // LDS words of one footprint component
+__device__ __forceinline__ int foot_index(int rx, int ry, int rz) { return rx + 8 * ry + kFootSZ * rz; }
 // Loops that move a footprint between LDS and the tile-major global layout run over j = 64 f + l
 // (footprint tile f = (x >> 2) | (y >> 2) << 1 | (z >> 2) << 2, local node
 // l = (x & 3) | (y & 3) << 2 | (z & 3) << 4), so global accesses stay contiguous; this gives the
 // LDS index of j.
-__device__ __forceinline__ int foot_lds(int j) {
+__device__ __forceinline__ void foot_xyz(int j, int& x, int& y, int& z) {
   const int f = j >> 6, l = j & 63;
-  return ((f & 1) << 2 | (l & 3)) + 8 * (((f >> 1) & 1) << 2 | ((l >> 2) & 3)) + 64 * ((f >> 2) << 2 | (l >> 4));
+  x = (f & 1) << 2 | (l & 3);
+  y = ((f >> 1) & 1) << 2 | ((l >> 2) & 3);
+  z = (f >> 2) << 2 | (l >> 4);
+}
+__device__ __forceinline__ int foot_lds(int j) {
+  int x, y, z;
+  foot_xyz(j, x, y, z);
+  return foot_index(x, y, z);
 }
 __device__ __forceinline__ void node_pos(const BodyGrid& G, int i, int j, int k, float x[3]) {
   x[0] = G.h * (float)i + G.gmin[0];
@@ -106,7 +117,7 @@ __device__ __forceinline__ bool chunk_stencil(const ChunkCtx& c, const float x[3
   return rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4;
 }
 // load a batch 3-vector footprint into LDS (optionally projected)
-__device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, const float* d, float (*sd)[kFoot],
+__device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, const float* d, float (*sd)[kFootLds],
                                            const int8_t* flags, const ColliderSet* cols, bool proj) {
   for (int j = threadIdx.x; j < kFoot; j += kBlock) {
     const int gt = ft[j >> 6], l = j & 63, i = foot_lds(j);
@@ -120,7 +131,9 @@ __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, con
         int fl = flags[(int64_t)gt * 64 + l];
         if (fl != -1) {
           float xn[3];
-          node_pos(c.G, c.ox + (i & 7), c.oy + ((i >> 3) & 7), c.oz + (i >> 6), xn);
+          int fx, fy, fz;
+          foot_xyz(j, fx, fy, fz);
+          node_pos(c.G, c.ox + fx, c.oy + fy, c.oz + fz, xn);
           project(cols, fl, xn, v);
         }
       }
@@ -142,11 +155,11 @@ __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, con
 // reproducible. A chunk with a non-finite T stages NaN.
 static_assert(kChunk == kBlock, "one particle per thread in chunk kernels");
 template <int NC> struct FxFoot {
-  int hi[NC][kFoot];
-  int lo[NC][kFoot];
+  int hi[NC][kFootLds];
+  int lo[NC][kFootLds];
 };
 template <int NC> __device__ __forceinline__ void fx_zero(FxFoot<NC>& a) {
-  for (int i = threadIdx.x; i < kFoot; i += kBlock)
+  for (int i = threadIdx.x; i < kFootLds; i += kBlock)
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       a.hi[c][i] = 0;
@@ -204,7 +217,7 @@ __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc,
 // float rounding of the node sums it feeds, with half the LDS atomics and a fifth of the
 // conversion work of the split form.
 template <int NC> struct FxFoot1 {
-  int v[NC][kFoot];
+  int v[NC][kFootLds];
 };
 struct FxScale3 {
   float S[3], invS[3];
@@ -236,7 +249,7 @@ __device__ __forceinline__ FxScale3 fx_scale3(const float bound[3], int top) {
   return r;
 }
 template <int NC> __device__ __forceinline__ void fx_zero(FxFoot1<NC>& a) {
-  for (int i = threadIdx.x; i < kFoot; i += kBlock)
+  for (int i = threadIdx.x; i < kFootLds; i += kBlock)
 #pragma unroll
     for (int c = 0; c < NC; ++c) a.v[c][i] = 0;
 }
@@ -267,7 +280,7 @@ __device__ __forceinline__ void sep_gather(const float* v, const Stencil& st, fl
       float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float d = v[i + 8 * j + 64 * k];
+        const float d = v[i + 8 * j + kFootSZ * k];
         a0 = fmaf(d, st.w[0][i], a0);
         a1 = fmaf(d, st.dw[0][i], a1);
       }
@@ -294,7 +307,8 @@ __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f
       const float P = f0 * (st.w[1][j] * st.w[2][k]);
       const float Q = fmaf(f1, st.dw[1][j] * st.w[2][k], f2 * (st.w[1][j] * st.dw[2][k]));
 #pragma unroll
-      for (int i = 0; i < 4; ++i) atomicAdd(&acc[i + 8 * j + 64 * k], __float2int_rn(fmaf(P, st.dw[0][i], Q * st.w[0][i])));
+      for (int i = 0; i < 4; ++i)
+        atomicAdd(&acc[i + 8 * j + kFootSZ * k], __float2int_rn(fmaf(P, st.dw[0][i], Q * st.w[0][i])));
     }
 }
 
@@ -991,7 +1005,7 @@ __global__ void k_gather_mass(Batch B, const float* stage, float* mass, int* err
 }
 // VelocitySplatter (Grid.cpp:102-150): v_i += w (m_p / m_i) (v_p - v_f) where m_i > 0
 __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const float* mass, float* stage, int* err) {
-  __shared__ float sm[kFoot];
+  __shared__ float sm[kFootLds];
   __shared__ FxFoot<3> acc;
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
@@ -1058,7 +1072,7 @@ __global__ void k_gather3(Batch B, const float* stage, float* out) {
 }
 // computeParticleVolumes (Grid.cpp:265-292): V_p = m_p / sum_i w_ip m_i / h^3
 __global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const float* mass, int* err) {
-  __shared__ float sm[kFoot];
+  __shared__ float sm[kFootLds];
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   __syncthreads();
@@ -1259,7 +1273,7 @@ __global__ void k_collision_velocities(Batch B, const int8_t* flags, const Colli
 template <bool PROJ>
 __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
                                           const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
-                                          float (*sd)[kFoot], FxFoot1<3>& acc, int* ft) {
+                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
   ChunkCtx c = chunk_begin(B, ft, chunk);
   fx_zero(acc);
   __syncthreads();
@@ -1324,7 +1338,7 @@ template <bool PROJ>
 __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d, const int8_t* flags,
                                                 const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
   if (sc && sc[B.chunks[blockIdx.x].w].done) return;
-  __shared__ float sd[3][kFoot];
+  __shared__ float sd[3][kFootLds];
   __shared__ FxFoot1<3> acc;
   __shared__ int ft[8];
   hvp_chunk<PROJ>(P, B, blockIdx.x, d, flags, cols, stage, err, sd, acc, ft);
@@ -1705,7 +1719,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
                                                      float* stage, const float* bvec, const float* diag, float* x,
                                                      float* z, float* p, float* Ap, float* Ar, CRScalars* sc,
                                                      double* hist, int* err) {
-  __shared__ float sd[3][kFoot];
+  __shared__ float sd[3][kFootLds];
   __shared__ FxFoot1<3> acc;
   __shared__ int ft[8];
   __shared__ SmallRed<2> R2;
@@ -2118,8 +2132,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
                                                 float thc, float ths, const ColliderSet* cols, int* err) {
   constexpr bool DO_V = MODE != 1;
   constexpr bool DO_F = MODE != 0;
-  __shared__ float sv[3][kFoot];
-  __shared__ float sp[DO_V ? 3 : 1][kFoot];
+  __shared__ float sv[3][kFootLds];
+  __shared__ float sp[DO_V ? 3 : 1][kFootLds];
   __shared__ int ft[8];
   ChunkCtx c = chunk_begin(B, ft, blockIdx.x);
   __syncthreads();
