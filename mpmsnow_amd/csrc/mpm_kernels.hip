@@ -53,7 +53,9 @@ __device__ __forceinline__ int foot_lds(int j) {
   foot_xyz(j, x, y, z);
   return foot_index(x, y, z);
 }
+// grid node position h i + min with the reference's two roundings (no fused multiply-add)
 __device__ __forceinline__ void node_pos(const BodyGrid& G, int i, int j, int k, float x[3]) {
+#pragma clang fp contract(off)
   x[0] = G.h * (float)i + G.gmin[0];
   x[1] = G.h * (float)j + G.gmin[1];
   x[2] = G.h * (float)k + G.gmin[2];
@@ -540,6 +542,14 @@ __global__ void k_gather_slots(const int* slot_of, const int* orig, int n, int* 
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
   return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) & mask;
 }
+// |x - y|^2 as (d0 d0 + d1 d1) + d2 d2 with every product and sum rounded separately, like the
+// host's Vector3f::squaredNorm: the neighbour test d^2 < h^2 must match it bit for bit (the HIP
+// __fmul_rn / __fadd_rn helpers are plain operators that contraction would fuse)
+__device__ __forceinline__ float sqdist_ref(const float x[3], float y0, float y1, float y2) {
+#pragma clang fp contract(off)
+  const float d0 = x[0] - y0, d1 = x[1] - y1, d2 = x[2] - y2;
+  return (d0 * d0 + d1 * d1) + d2 * d2;
+}
 __device__ __forceinline__ void cell_of(const float x[3], float cell, int c[3]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) c[a] = (int)floorf(x[a] / cell);
@@ -584,8 +594,31 @@ __device__ __forceinline__ uint32_t cc_priority(int x) {
   h ^= h >> 16;
   return h;
 }
+// Parent pointers only ever point into their own set (hooking joins two sets, path halving
+// moves a pointer up its own tree), so any value a pointer ever held names a member of the
+// same set. Reads that only need that property go through the cache (workgroup-scope loads):
+// the device-coherent loads of cc_find are kept for the hooking loop, where progress needs the
+// current roots. Without this, every find of a large component ends on the device-coherent
+// line of its root, and the whole grid queues on that one address.
+__device__ __forceinline__ int cc_load_cached(const int* parent, int x) {
+  return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// root of x in kernels where no thread hooks roots (compression and root collection)
+__device__ __forceinline__ int cc_find_cached(const int* parent, int x) {
+  int p = cc_load_cached(parent, x);
+  while (p != x) {
+    x = p;
+    p = cc_load_cached(parent, x);
+  }
+  return x;
+}
 // hook the root of higher priority under the other; priorities strictly fall towards a root
 __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
+  {
+    // a cheap test that catches most pairs inside one already linked component
+    const int pa = cc_load_cached(parent, a), pb = cc_load_cached(parent, b);
+    if (pa == pb || pa == b || pb == a) return;
+  }
   while (true) {
     a = cc_find(parent, a);
     b = cc_find(parent, b);
@@ -599,37 +632,104 @@ __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
     if (old == a) return;
   }
 }
-// neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190);
-// threads walk the cell-sorted order so a wave's candidate cells mostly coincide
+// Neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190),
+// linked in two passes after the subgraph-sampling scheme of Sutton et al. (Afforest, IPDPS
+// 2018): pass 1 (links = kCcSampleLinks, skip = null) hooks every particle to its first few
+// neighbours, which already joins most of each body, and records whether it has any neighbour;
+// after compression and a sampled vote for the largest component (giant), pass 2 (links =
+// INT_MAX) hooks every remaining neighbour of the particles outside the giant component. A
+// particle inside it may skip its pairs: for a pair with one end outside, that end links it; a
+// pair with both ends inside is already joined. Threads walk the cell-sorted order so a wave's
+// candidate cells mostly coincide.
+constexpr int kCcSampleLinks = 2;
 __global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                          const float4* sorted, int* parent, int* hasNb) {
+                          const float4* sorted, int* parent, int* hasNb, int links, const int* giant) {
   const float r2 = h * h;
+  const int g = giant ? *giant : -1;
   GRID_STRIDE(ti, n) {
     float4 me = sorted[ti];
     const int s = __float_as_int(me.w);
+    if (g >= 0 && cc_load_cached(parent, s) == g) continue;
     const float x[3] = {me.x, me.y, me.z};
     int c[3];
     cell_of(x, cell, c);
-    bool nb = false;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
+    int found = 0;
+    for (int dz = -1; dz <= 1 && found < links; ++dz)
+      for (int dy = -1; dy <= 1 && found < links; ++dy)
+        for (int dx = -1; dx <= 1 && found < links; ++dx) {
           uint32_t hb = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, mask);
           int o = offset[hb], e = o + count[hb];
-          for (int q = o; q < e; ++q) {
+          for (int q = o; q < e && found < links; ++q) {
             float4 y = sorted[q];
             int j = __float_as_int(y.w);
             if (j == s) continue;
-            // (d0 d0 + d1 d1) + d2 d2 with separate roundings, as Eigen's squaredNorm on the host
-            float d0 = x[0] - y.x, d1 = x[1] - y.y, d2 = x[2] - y.z;
-            float d = __fadd_rn(__fadd_rn(__fmul_rn(d0, d0), __fmul_rn(d1, d1)), __fmul_rn(d2, d2));
-            if (d < r2) {
-              nb = true;
-              if (j > s) cc_union(parent, s, j);
+            if (sqdist_ref(x, y.x, y.y, y.z) < r2) {
+              ++found;
+              cc_union(parent, s, j);
             }
           }
         }
-    hasNb[s] = nb ? 1 : 0;
+    if (!giant) hasNb[s] = found > 0 ? 1 : 0;
+  }
+}
+// every pointer straight to its root
+__global__ void k_cc_compress(int n, int* parent) {
+  GRID_STRIDE(s, n) {
+    const int r = cc_find_cached(parent, (int)s);
+    __hip_atomic_store(parent + s, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+// The most frequent root among kCcVote evenly spaced particles (of compressed pointers), or -1
+// when no root occurs twice. One workgroup of kCcVote threads.
+constexpr int kCcVote = 1024;
+__global__ __launch_bounds__(kCcVote) void k_cc_vote(int n, const int* parent, int* giant) {
+  __shared__ int v[kCcVote];
+  __shared__ int bestCount[kCcVote / 64], bestRoot[kCcVote / 64];
+  const int t = threadIdx.x;
+  const int64_t stride = n / kCcVote > 0 ? n / kCcVote : 1;
+  const int64_t sIdx = (int64_t)t * stride;
+  v[t] = sIdx < n ? cc_load_cached(parent, (int)sIdx) : INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= kCcVote; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int l = t ^ j;
+      if (l > t) {
+        const int a = v[t], b = v[l];
+        if ((a > b) == ((t & k) == 0)) {
+          v[t] = b;
+          v[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+  int cnt = 0;
+  const int r = v[t];
+  if (r != INT_MAX && (t == 0 || v[t - 1] != r)) {
+    cnt = 1;
+    while (t + cnt < kCcVote && v[t + cnt] == r) ++cnt;
+  }
+  // longest run; ties to the smaller root
+  int bc = cnt, br = r;
+  for (int off = 32; off > 0; off >>= 1) {
+    const int oc = __shfl_down(bc, off, 64), orr = __shfl_down(br, off, 64);
+    if (oc > bc || (oc == bc && orr < br)) {
+      bc = oc;
+      br = orr;
+    }
+  }
+  if ((t & 63) == 0) {
+    bestCount[t >> 6] = bc;
+    bestRoot[t >> 6] = br;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int c = bestCount[0], rr = bestRoot[0];
+    for (int w = 1; w < kCcVote / 64; ++w)
+      if (bestCount[w] > c || (bestCount[w] == c && bestRoot[w] < rr)) {
+        c = bestCount[w];
+        rr = bestRoot[w];
+      }
+    *giant = c >= 2 ? rr : -1;
   }
 }
 // every slot points at its root; smallest caller index per component. The store is
@@ -640,7 +740,7 @@ __global__ void k_cc_roots(int n, int* parent, const int* hasNb, const int* orig
     int r = -1, o = INT_MAX;
     if (k < n) {
       int s = (int)k;
-      int root = cc_find(parent, s);
+      int root = cc_find_cached(parent, s);
       parent[s] = root;
       if (hasNb[s]) {
         r = root;
@@ -2370,8 +2470,14 @@ void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* o
   if (n > 0) LAUNCH(k_cc_scatter, grid_for(n, 16384), P, bucket, rank, offset, n, sorted);
 }
 void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const float4* sorted, int* parent, int* hasNb, hipStream_t s) {
-  if (n > 0) LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb);
+                    const float4* sorted, int* parent, int* hasNb, int* giant, hipStream_t s) {
+  if (n <= 0) return;
+  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb, kCcSampleLinks,
+         (const int*)nullptr);
+  LAUNCH(k_cc_compress, grid_for(n, 16384), n, parent);
+  hipLaunchKernelGGL(k_cc_vote, dim3(1), dim3(kCcVote), 0, s, n, (const int*)parent, giant);
+  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb, INT_MAX,
+         (const int*)giant);
 }
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s) {

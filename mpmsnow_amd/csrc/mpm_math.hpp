@@ -383,13 +383,18 @@ struct ColliderSet {
   Collider o[kMaxColliders];
 };
 
+// Collider tests decide the collided flag of a node, which must match the reference bit for bit:
+// they keep its separately rounded float products and sums (hipcc would otherwise fuse a*b + c
+// into one FMA, which rounds once).
 MPM_HD float col_phi(const Collider& o, const float x[3]) {
+#pragma clang fp contract(off)
   if (o.type == 0) return o.c[0] * x[0] + o.c[1] * x[1] + o.c[2] * x[2] + o.c[3];
   float d0 = x[0] - o.center[0], d1 = x[1] - o.center[1], d2 = x[2] - o.center[2];
   return sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o.radius;
 }
 // normalised gradient (CollisionObject::grad + Vector3f::normalize)
 MPM_HD void col_normal(const Collider& o, const float x[3], float n[3]) {
+#pragma clang fp contract(off)
   if (o.type == 0) {
     n[0] = o.c[0]; n[1] = o.c[1]; n[2] = o.c[2];
   } else {

@@ -839,7 +839,7 @@ void require_particles(mpm_context* c) {
 
 // Sim::calculateBodies (Sim.cpp:136-203) on the GPU: connected components of the relation
 // |x_i - x_j|^2 < h^2 (float arithmetic of KdTree.inl:187-190) by cell hashing and
-// union-find; isolated particles are ballistic; bodies are ordered by their smallest
+// sampled union-find; isolated particles are ballistic; bodies are ordered by their smallest
 // particle index, as the reference's flood fill numbers them.
 void calculate_bodies(mpm_context& c) {
   const int n = (int)c.n;
@@ -872,7 +872,7 @@ void calculate_bodies(mpm_context& c) {
                     c.ccSorted.as<float4>(), s);
   launch_iota(c.ccParent.as<int>(), n, s);
   launch_cc_link(n, c.h, cell, mask, c.ccOffset.as<int>(), c.ccCount.as<int>(), c.ccSorted.as<float4>(),
-                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), s);
+                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccTot.as<int>() + 3, s);
   HIPCHECK(hipMemsetAsync(c.ccIsRoot.p, 0, sizeof(int) * n, s));
   launch_fill_int(c.ccMinOrig.as<int>(), INT_MAX, n, s);
   launch_cc_roots(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.orig_of[c.cur].as<int>(), c.ccIsRoot.as<int>(),
