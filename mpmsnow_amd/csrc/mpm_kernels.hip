@@ -118,19 +118,21 @@ __device__ __forceinline__ bool chunk_stencil(const ChunkCtx& c, const float x[3
   rz = st.base[2] - c.oz;
   return rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4;
 }
-// load a batch 3-vector footprint into LDS (optionally projected)
+// load a batch 3-vector footprint into LDS (optionally projected); d and flags hold tiles from
+// global tile tileOff on (0: the whole batch)
 __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, const float* d, float (*sd)[kFootLds],
-                                           const int8_t* flags, const ColliderSet* cols, bool proj) {
+                                           const int8_t* flags, const ColliderSet* cols, bool proj,
+                                           int tileOff = 0) {
   for (int j = threadIdx.x; j < kFoot; j += kBlock) {
     const int gt = ft[j >> 6], l = j & 63, i = foot_lds(j);
     float v[3] = {0.0f, 0.0f, 0.0f};
     if (gt >= 0) {
-      int64_t b = (int64_t)gt * 192 + l;
+      int64_t b = (int64_t)(gt - tileOff) * 192 + l;
       v[0] = d[b];
       v[1] = d[b + 64];
       v[2] = d[b + 128];
       if (proj) {
-        int fl = flags[(int64_t)gt * 64 + l];
+        int fl = flags[(int64_t)(gt - tileOff) * 64 + l];
         if (fl != -1) {
           float xn[3];
           int fx, fy, fz;
@@ -336,8 +338,10 @@ __device__ __forceinline__ float gather_node(const Batch& B, const BodyGrid& G, 
   }
   return v;
 }
-__device__ __forceinline__ void gather_node3(const Batch& B, const BodyGrid& G, const float* stage, int T, int l,
-                                             float o[3]) {
+// tc0 / tcn: first chunk and chunk count of each tile from global tile tileOff on; stage holds
+// the chunks from the one numbered tc0 = 0 on
+__device__ __forceinline__ void gather3_tab(const BodyGrid& G, const int* tc0, const int* tcn, int tileOff,
+                                            const float* stage, int T, int l, float o[3]) {
   int tx, ty, tz;
   tile_xyz(G, T - G.tileBase, tx, ty, tz);
   float a = 0.0f, b = 0.0f, cc = 0.0f;
@@ -345,7 +349,7 @@ __device__ __forceinline__ void gather_node3(const Batch& B, const BodyGrid& G, 
   for (int f = 0; f < 8; ++f) {
     int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
     if (ts < 0) continue;
-    int c0 = B.tileChunk0[ts], cn = B.tileChunkN[ts];
+    int c0 = tc0[ts - tileOff], cn = tcn[ts - tileOff];
     for (int q = 0; q < cn; ++q) {
       const float* s = stage + (int64_t)(c0 + q) * kStage + f * 192 + l;
       a += s[0];
@@ -356,6 +360,10 @@ __device__ __forceinline__ void gather_node3(const Batch& B, const BodyGrid& G, 
   o[0] = a;
   o[1] = b;
   o[2] = cc;
+}
+__device__ __forceinline__ void gather_node3(const Batch& B, const BodyGrid& G, const float* stage, int T, int l,
+                                             float o[3]) {
+  gather3_tab(G, B.tileChunk0, B.tileChunkN, 0, stage, T, l, o);
 }
 
 // ------------------------------------------------------------ node block context
@@ -634,18 +642,20 @@ __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
 }
 // Neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190),
 // linked in two passes after the subgraph-sampling scheme of Sutton et al. (Afforest, IPDPS
-// 2018): pass 1 (links = kCcSampleLinks, skip = null) hooks every particle to its first few
+// 2018): pass 1 (SAMPLE) hooks every particle to its first kCcSampleLinks
 // neighbours, which already joins most of each body, and records whether it has any neighbour;
-// after compression and a sampled vote for the largest component (giant), pass 2 (links =
-// INT_MAX) hooks every remaining neighbour of the particles outside the giant component. A
+// after compression and a sampled vote for the largest component (giant), pass 2 hooks every
+// remaining neighbour of the particles outside the giant component. A
 // particle inside it may skip its pairs: for a pair with one end outside, that end links it; a
 // pair with both ends inside is already joined. Threads walk the cell-sorted order so a wave's
 // candidate cells mostly coincide.
 constexpr int kCcSampleLinks = 2;
+template <bool SAMPLE>
 __global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                          const float4* sorted, int* parent, int* hasNb, int links, const int* giant) {
+                          const float4* sorted, int* parent, int* hasNb, const int* giant) {
   const float r2 = h * h;
-  const int g = giant ? *giant : -1;
+  const int links = SAMPLE ? kCcSampleLinks : INT_MAX;
+  const int g = SAMPLE ? -1 : *giant;
   GRID_STRIDE(ti, n) {
     float4 me = sorted[ti];
     const int s = __float_as_int(me.w);
@@ -669,7 +679,7 @@ __global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* 
             }
           }
         }
-    if (!giant) hasNb[s] = found > 0 ? 1 : 0;
+    if (SAMPLE) hasNb[s] = found > 0 ? 1 : 0;
   }
 }
 // every pointer straight to its root
@@ -702,11 +712,16 @@ __global__ __launch_bounds__(kCcVote) void k_cc_vote(int n, const int* parent, i
       }
       __syncthreads();
     }
+  // run length at each run's last element: its first element by binary search
   int cnt = 0;
   const int r = v[t];
-  if (r != INT_MAX && (t == 0 || v[t - 1] != r)) {
-    cnt = 1;
-    while (t + cnt < kCcVote && v[t + cnt] == r) ++cnt;
+  if (r != INT_MAX && (t == kCcVote - 1 || v[t + 1] != r)) {
+    int lo = 0, hi = t;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (v[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    cnt = t - lo + 1;
   }
   // longest run; ties to the smaller root
   int bc = cnt, br = r;
@@ -1370,15 +1385,9 @@ __global__ void k_collision_velocities(Batch B, const int8_t* flags, const Colli
 // ForceDifferentialSplatter (Grid.cpp:549-618), factorised:
 //   dF_p = (sum_j d_j grad w_jp^T) F_p ; df_i -= V_p dP(dF_p) F_p^T grad w_ip
 // One chunk's staged footprint of the HVP (LDS work areas passed in; every thread of the block calls).
-template <bool PROJ>
-__device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
-                                          const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
-                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
-  ChunkCtx c = chunk_begin(B, ft, chunk);
-  fx_zero(acc);
-  __syncthreads();
-  load_foot3(c, ft, d, sd, flags, cols, PROJ);
-  __syncthreads();
+// particle part: footprint of d in sd (loaded), scatter into acc (zeroed); ends on a barrier
+__device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B, const ChunkCtx& c,
+                                                  float (*sd)[kFootLds], FxFoot1<3>& acc, int* err) {
   const int k = threadIdx.x;
   bool have = k < c.ch.z;
   float fm[9];
@@ -1432,6 +1441,18 @@ __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int ch
       sep_scatter(&acc.v[a][base], st, fm[a] * scl.S[a], fm[a + 3] * scl.S[a], fm[a + 6] * scl.S[a]);
   }
   __syncthreads();
+  return scl;
+}
+template <bool PROJ>
+__device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
+                                          const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
+                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
+  ChunkCtx c = chunk_begin(B, ft, chunk);
+  fx_zero(acc);
+  __syncthreads();
+  load_foot3(c, ft, d, sd, flags, cols, PROJ);
+  __syncthreads();
+  const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err);
   fx_store(acc, scl, stage, chunk);
 }
 template <bool PROJ>
@@ -1578,18 +1599,24 @@ __global__ void k_rhs(Batch B, const float* mass, const int8_t* flags, const Col
 // the one-workgroup solver of small bodies (k_small_cr), so that both compute the same bits.
 
 // CR setup part 1 (ConjugateResiduals.cpp:50-67): r = b - A x ; z = M^-1 r ; p = z
+__device__ __forceinline__ void cr_setup1_core(const NodeCtx& c, const ColliderSet* cols, int fl, float m,
+                                               const float xv[3], const float df[3], const float bb[3],
+                                               const float dg[3], float dt2, float zz[3]) {
+  float Ax[3];
+  apply_A(c, cols, fl, m, xv, df, dt2, Ax);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) zz[a] = (bb[a] - Ax[a]) / dg[a];
+}
 __device__ __forceinline__ void cr_setup1_node(const Batch& B, const NodeCtx& c, const float* mass,
                                                const int8_t* flags, const ColliderSet* cols, const float* stage,
                                                const float* x, const float* b, const float* diag, float dt2, float* z,
                                                float* p) {
-  float xv[3], df[3] = {0.0f, 0.0f, 0.0f}, Ax[3], bb[3], dg[3], zz[3];
+  float xv[3], df[3] = {0.0f, 0.0f, 0.0f}, bb[3], dg[3], zz[3];
   ld3(x, c.i, xv);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
-  apply_A(c, cols, flags[c.i], mass[c.i], xv, df, dt2, Ax);
   ld3(b, c.i, bb);
   ld3(diag, c.i, dg);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) zz[a] = (bb[a] - Ax[a]) / dg[a];
+  cr_setup1_core(c, cols, flags[c.i], mass[c.i], xv, df, bb, dg, dt2, zz);
   st3(z, c.i, zz);
   st3(p, c.i, zz);
 }
@@ -1602,6 +1629,16 @@ __global__ void k_cr_setup1(Batch B, const float* mass, const int8_t* flags, con
 }
 // CR setup part 2 (ConjugateResiduals.cpp:69-76 and the first alpha, :86-102):
 // Ap = Ar = A z ; rz = <z, Ar> ; alpha = rz / <Ap, M^-1 Ap>
+__device__ __forceinline__ void cr_setup2_core(const NodeCtx& c, const ColliderSet* cols, int fl, float m,
+                                               const float zz[3], const float df[3], const float dg[3], float dt2,
+                                               float Ar[3], double (&s)[2]) {
+  apply_A(c, cols, fl, m, zz, df, dt2, Ar);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    s[0] += (double)zz[a] * Ar[a];
+    s[1] += (double)Ar[a] * (Ar[a] / dg[a]);
+  }
+}
 __device__ __forceinline__ void cr_setup2_node(const Batch& B, const NodeCtx& c, const float* mass,
                                                const int8_t* flags, const ColliderSet* cols, const float* stage,
                                                const float* z, const float* diag, float dt2, float* Ap,
@@ -1609,14 +1646,9 @@ __device__ __forceinline__ void cr_setup2_node(const Batch& B, const NodeCtx& c,
   float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, Ar[3], dg[3];
   ld3(z, c.i, zz);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
-  apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, Ar);
-  st3(Ap, c.i, Ar);
   ld3(diag, c.i, dg);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    s[0] += (double)zz[a] * Ar[a];
-    s[1] += (double)Ar[a] * (Ar[a] / dg[a]);
-  }
+  cr_setup2_core(c, cols, flags[c.i], mass[c.i], zz, df, dg, dt2, Ar, s);
+  st3(Ap, c.i, Ar);
 }
 // returns true when the solve ends here
 __device__ __forceinline__ bool cr_setup2_finish(CRScalars& S, const double (&tot)[2]) {
@@ -1644,25 +1676,30 @@ __global__ void k_cr_setup2(Batch B, const float* mass, const int8_t* flags, con
 }
 // iteration part 1 (ConjugateResiduals.cpp:104-128): x = P(x + alpha p) ; z -= alpha M^-1 Ap ;
 // r = M z ; termination on |r|^2 (SquareMagnitudeTermination.cpp:19-28)
-__device__ __forceinline__ void cr_update_x_node(const NodeCtx& c, const int8_t* flags, const ColliderSet* cols,
-                                                 const float* p, const float* Ap, const float* diag, float alpha,
-                                                 float* x, float* z, double (&s)[1]) {
-  float xv[3], pv[3], ap[3], dg[3], zz[3];
-  ld3(x, c.i, xv);
-  ld3(p, c.i, pv);
+__device__ __forceinline__ void cr_update_x_core(const NodeCtx& c, const ColliderSet* cols, int fl, float alpha,
+                                                 float xv[3], const float pv[3], const float ap[3],
+                                                 const float dg[3], float zz[3], double (&s)[1]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) xv[a] = xv[a] + alpha * pv[a];
-  node_project(c, cols, flags[c.i], xv);
-  st3(x, c.i, xv);
-  ld3(Ap, c.i, ap);
-  ld3(diag, c.i, dg);
-  ld3(z, c.i, zz);
+  node_project(c, cols, fl, xv);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     zz[a] = zz[a] - alpha * (ap[a] / dg[a]);
     float r = zz[a] * dg[a];
     s[0] += (double)r * r;
   }
+}
+__device__ __forceinline__ void cr_update_x_node(const NodeCtx& c, const int8_t* flags, const ColliderSet* cols,
+                                                 const float* p, const float* Ap, const float* diag, float alpha,
+                                                 float* x, float* z, double (&s)[1]) {
+  float xv[3], pv[3], ap[3], dg[3], zz[3];
+  ld3(x, c.i, xv);
+  ld3(p, c.i, pv);
+  ld3(Ap, c.i, ap);
+  ld3(diag, c.i, dg);
+  ld3(z, c.i, zz);
+  cr_update_x_core(c, cols, flags[c.i], alpha, xv, pv, ap, dg, zz, s);
+  st3(x, c.i, xv);
   st3(z, c.i, zz);
 }
 __device__ __forceinline__ bool cr_update_x_finish(CRScalars& S, const double (&tot)[1], int body, double* hist) {
@@ -1685,15 +1722,20 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
     if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
 }
 // iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta
+__device__ __forceinline__ void cr_ar_core(const NodeCtx& c, const ColliderSet* cols, int fl, float m,
+                                           const float zz[3], const float df[3], float dt2, float ar[3],
+                                           double (&s)[1]) {
+  apply_A(c, cols, fl, m, zz, df, dt2, ar);
+  s[0] = (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
+}
 __device__ __forceinline__ void cr_ar_node(const Batch& B, const NodeCtx& c, const float* mass, const int8_t* flags,
                                            const ColliderSet* cols, const float* stage, const float* z, float dt2,
                                            float* Ar, double (&s)[1]) {
   float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, ar[3];
   ld3(z, c.i, zz);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
-  apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, ar);
+  cr_ar_core(c, cols, flags[c.i], mass[c.i], zz, df, dt2, ar, s);
   st3(Ar, c.i, ar);
-  s[0] = (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
 }
 __device__ __forceinline__ bool cr_ar_finish(CRScalars& S, const double (&tot)[1]) {
   double rz_old = S.rz;
@@ -1717,6 +1759,15 @@ __global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const C
     if (cr_ar_finish(S, tot)) set_done(S, bst);
 }
 // iteration part 3 (ConjugateResiduals.cpp:147-151 and the next alpha, :86-102)
+__device__ __forceinline__ void cr_update_p_core(const float zz[3], const float ar[3], const float dg[3], float beta,
+                                                 float pv[3], float ap[3], double (&s)[1]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    pv[a] = zz[a] + beta * pv[a];
+    ap[a] = ar[a] + beta * ap[a];
+    s[0] += (double)ap[a] * (ap[a] / dg[a]);
+  }
+}
 __device__ __forceinline__ void cr_update_p_node(const NodeCtx& c, const float* z, const float* Ar, const float* diag,
                                                  float beta, float* p, float* Ap, double (&s)[1]) {
   float zz[3], pv[3], ar[3], ap[3], dg[3];
@@ -1725,12 +1776,7 @@ __device__ __forceinline__ void cr_update_p_node(const NodeCtx& c, const float* 
   ld3(Ar, c.i, ar);
   ld3(Ap, c.i, ap);
   ld3(diag, c.i, dg);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    pv[a] = zz[a] + beta * pv[a];
-    ap[a] = ar[a] + beta * ap[a];
-    s[0] += (double)ap[a] * (ap[a] / dg[a]);
-  }
+  cr_update_p_core(zz, ar, dg, beta, pv, ap, s);
   st3(p, c.i, pv);
   st3(Ap, c.i, ap);
 }
@@ -1767,6 +1813,24 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
 // BatchStatus::small_bodies. The node work, the HVP chunk scatter, the reduction order (block
 // tree per node block, then block totals in block order, as body_sum) and the scalar updates
 // are those of the batched kernels, so the iterates are bitwise the same.
+//
+// The smallest bodies (at most kTinyBlocks node blocks and kTinyChunks particle chunks: the
+// fragments of a few particles that can take hundreds of iterations) keep their whole state
+// on chip: each thread holds its nodes' x, p, Ap, Ar, diag, mass and flags in registers, z and
+// the staged HVP footprints live in LDS, and an iteration touches global memory only for the
+// particles' constitutive data. The arithmetic is the same per-node cores and HVP pieces as
+// the batched kernels, on the same float values.
+constexpr int kTinyBlocks = 2;
+constexpr int kTinyChunks = 4;
+constexpr int kTinyTiles = kTinyBlocks * kBlock / 64;
+struct TinyLds {
+  float z[kTinyTiles * 192];            // the body's z (or x during setup), tile-major like the batch
+  float stage[kTinyChunks * kStage];    // staged footprints of the body's chunks
+  int8_t flags[kTinyTiles * 64];
+  int tc0[kTinyTiles], tcn[kTinyTiles];  // each tile's first chunk (relative to the body's) and count
+  int4 ch[kTinyChunks];
+  int ft[kTinyChunks][8];               // footprint tiles of each chunk (global ids, -1 outside)
+};
 template <int NV> struct SmallRed {
   double w[kBlock / 64][kSmallBlocks][NV];
   double sm[kBlock / 64][NV];
@@ -1805,6 +1869,163 @@ __device__ __forceinline__ void small_sweep(const BodyGrid& G, int body, SmallRe
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = R.res[k];
 }
+// tiny bodies: node j of this thread is G.blockBase + j; fn(j, v) adds this thread's terms
+template <int NV, class Fn>
+__device__ __forceinline__ void tiny_sweep(const BodyGrid& G, SmallRed<NV>& R, Fn&& fn, double (&tot)[NV]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kTinyBlocks; ++j) {
+    if (j >= G.nblocks) break;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    fn(j, v);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+      if (lane == 0) R.w[wid][j][k] = v[k];
+    }
+  }
+  __syncthreads();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    acc[k] = 0.0;
+    const int t = threadIdx.x;
+    if (t < G.nblocks) acc[k] += ((R.w[0][t][k] + R.w[1][t][k]) + R.w[2][t][k]) + R.w[3][t][k];
+  }
+  block_sum<NV>(acc, R.sm);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) R.res[k] = acc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = R.res[k];
+}
+// HVP of the LDS vector T.z over the body's chunks into T.stage (every thread calls)
+__device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const BodyGrid& G, int nch, TinyLds& T,
+                                         const ColliderSet* cols, int* err, float (*sd)[kFootLds],
+                                         FxFoot1<3>& acc) {
+  for (int q = 0; q < nch; ++q) {
+    ChunkCtx c;
+    c.ch = T.ch[q];
+    c.G = G;
+    int tx, ty, tz;
+    tile_xyz(G, c.ch.x - G.tileBase, tx, ty, tz);
+    c.ox = tx * 4;
+    c.oy = ty * 4;
+    c.oz = tz * 4;
+    fx_zero(acc);
+    load_foot3(c, T.ft[q], T.z, sd, T.flags, cols, true, G.tileBase);
+    __syncthreads();
+    const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err);
+    fx_store(acc, scl, T.stage, q);
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void tiny_st3(TinyLds& T, const BodyGrid& G, const NodeCtx& c, const float v[3]) {
+  const int lt = c.T - G.tileBase;
+  T.z[lt * 192 + c.l] = v[0];
+  T.z[lt * 192 + 64 + c.l] = v[1];
+  T.z[lt * 192 + 128 + c.l] = v[2];
+}
+// the tiny-body solve (every thread of the workgroup calls; L and end as in k_small_cr)
+__device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int body, const BodyGrid& G, int c0,
+                                           int nch, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                                           float dt, const float* bvec, const float* diag, float* x, double* hist,
+                                           int* err, TinyLds& T, float (*sd)[kFootLds], FxFoot1<3>& acc,
+                                           SmallRed<2>& R2, SmallRed<1>& R1, CRScalars& L, int& end) {
+  const float dt2 = dt * dt;
+  // chunk table, footprints and per-tile chunk ranges of the body
+  if (threadIdx.x < nch * 8) {
+    const int q = threadIdx.x >> 3, f = threadIdx.x & 7;
+    const int4 ch = B.chunks[c0 + q];
+    if (f == 0) T.ch[q] = ch;
+    int tx, ty, tz;
+    tile_xyz(G, ch.x - G.tileBase, tx, ty, tz);
+    T.ft[q][f] = tile_id(G, tx + (f & 1), ty + ((f >> 1) & 1), tz + (f >> 2));
+  }
+  if (threadIdx.x < G.NT) {
+    T.tc0[threadIdx.x] = B.tileChunk0[G.tileBase + threadIdx.x] - c0;
+    T.tcn[threadIdx.x] = B.tileChunkN[G.tileBase + threadIdx.x];
+  }
+  // this thread's nodes
+  NodeCtx cn[kTinyBlocks];
+  float xr[kTinyBlocks][3], pr[kTinyBlocks][3], apr[kTinyBlocks][3], arr[kTinyBlocks][3], dgr[kTinyBlocks][3],
+      zr[kTinyBlocks][3], mr[kTinyBlocks];
+  int flr[kTinyBlocks];
+#pragma unroll
+  for (int j = 0; j < kTinyBlocks; ++j) {
+    if (j >= G.nblocks) break;
+    cn[j] = node_in(G, body, G.blockBase + j);
+    ld3(x, cn[j].i, xr[j]);
+    ld3(diag, cn[j].i, dgr[j]);
+    mr[j] = mass[cn[j].i];
+    flr[j] = flags[cn[j].i];
+    tiny_st3(T, G, cn[j], xr[j]);
+    T.flags[(cn[j].T - G.tileBase) * 64 + cn[j].l] = (int8_t)flr[j];
+  }
+  __syncthreads();
+  // setup (ConjugateResiduals.cpp:50-76)
+  tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
+#pragma unroll
+  for (int j = 0; j < kTinyBlocks; ++j) {
+    if (j >= G.nblocks) break;
+    float df[3] = {0.0f, 0.0f, 0.0f}, bb[3];
+    if (cn[j].valid) gather3_tab(G, T.tc0, T.tcn, G.tileBase, T.stage, cn[j].T, cn[j].l, df);
+    ld3(bvec, cn[j].i, bb);
+    cr_setup1_core(cn[j], cols, flr[j], mr[j], xr[j], df, bb, dgr[j], dt2, zr[j]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pr[j][a] = zr[j][a];
+  }
+  __syncthreads();  // every gather of the x footprints is done before z replaces x
+#pragma unroll
+  for (int j = 0; j < kTinyBlocks; ++j) {
+    if (j >= G.nblocks) break;
+    tiny_st3(T, G, cn[j], zr[j]);
+  }
+  __syncthreads();
+  tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
+  double t2[2];
+  tiny_sweep<2>(G, R2, [&](int j, double (&v)[2]) {
+    float df[3] = {0.0f, 0.0f, 0.0f};
+    if (cn[j].valid) gather3_tab(G, T.tc0, T.tcn, G.tileBase, T.stage, cn[j].T, cn[j].l, df);
+    cr_setup2_core(cn[j], cols, flr[j], mr[j], zr[j], df, dgr[j], dt2, apr[j], v);
+  }, t2);
+  if (threadIdx.x == 0) end = cr_setup2_finish(L, t2);
+  __syncthreads();
+  double t1[1];
+  while (!end) {
+    const float alpha = (float)L.alpha;
+    tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+      cr_update_x_core(cn[j], cols, flr[j], alpha, xr[j], pr[j], apr[j], dgr[j], zr[j], v);
+      tiny_st3(T, G, cn[j], zr[j]);
+    }, t1);
+    if (threadIdx.x == 0) end = cr_update_x_finish(L, t1, body, hist);
+    __syncthreads();
+    if (end) break;
+    tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
+    tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+      float df[3] = {0.0f, 0.0f, 0.0f};
+      if (cn[j].valid) gather3_tab(G, T.tc0, T.tcn, G.tileBase, T.stage, cn[j].T, cn[j].l, df);
+      cr_ar_core(cn[j], cols, flr[j], mr[j], zr[j], df, dt2, arr[j], v);
+    }, t1);
+    if (threadIdx.x == 0) end = cr_ar_finish(L, t1);
+    __syncthreads();
+    if (end) break;
+    const float beta = (float)L.beta;
+    tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+      cr_update_p_core(zr[j], arr[j], dgr[j], beta, pr[j], apr[j], v);
+    }, t1);
+    if (threadIdx.x == 0) end = cr_update_p_finish(L, t1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kTinyBlocks; ++j) {
+    if (j >= G.nblocks) break;
+    st3(x, cn[j].i, xr[j]);
+  }
+}
 __global__ void k_small_claim(const int* list, int n, CRScalars* sc, BatchStatus* bst) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1826,6 +2047,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   __shared__ SmallRed<1> R1;
   __shared__ CRScalars L;
   __shared__ int end;
+  __shared__ TinyLds T;
   const int body = list[blockIdx.x];
   if (sc[body].done != kDoneSmall) return;  // finished at the rhs: not claimed
   const BodyGrid G = B.bodies[body];
@@ -1841,6 +2063,16 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   const int lastTile = G.tileBase + G.NT - 1;
   const int c0 = B.tileChunk0[G.tileBase];
   const int c1 = B.tileChunk0[lastTile] + B.tileChunkN[lastTile];
+  if (G.nblocks <= kTinyBlocks && c1 - c0 <= kTinyChunks) {
+    __syncthreads();
+    tiny_solve(P, B, body, G, c0, c1 - c0, mass, flags, cols, dt, bvec, diag, x, hist, err, T, sd, acc, R2, R1, L,
+               end);
+    if (threadIdx.x == 0) {
+      L.done = 1;
+      sc[body] = L;
+    }
+    return;
+  }
   auto hvp = [&](const float* d) {
     for (int ch = c0; ch < c1; ++ch) hvp_chunk<true>(P, B, ch, d, flags, cols, stage, err, sd, acc, ft);
     __syncthreads();
@@ -2472,11 +2704,11 @@ void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* o
 void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
                     const float4* sorted, int* parent, int* hasNb, int* giant, hipStream_t s) {
   if (n <= 0) return;
-  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb, kCcSampleLinks,
+  LAUNCH(k_cc_link<true>, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb,
          (const int*)nullptr);
   LAUNCH(k_cc_compress, grid_for(n, 16384), n, parent);
   hipLaunchKernelGGL(k_cc_vote, dim3(1), dim3(kCcVote), 0, s, n, (const int*)parent, giant);
-  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb, INT_MAX,
+  LAUNCH(k_cc_link<false>, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb,
          (const int*)giant);
 }
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
