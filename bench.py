@@ -31,6 +31,12 @@ Other modes:
                        summarise two `rocprofv3 --pmc FETCH_SIZE` / `--pmc
                        WRITE_SIZE` runs of `bench.py --copy-calibration` into the
                        traffic summary file
+  --trace-summary STATS_CSV BENCH_LOG [--trace-out FILE]
+                       summarise a `rocprofv3 --kernel-trace --stats` run of
+                       `bench.py --no-cpu --no-learned` (its kernel_stats.csv and
+                       the run's stdout): per-kernel table and the HVP mean
+                       launch time against the one bench.py measured with HIP
+                       events in the same run
 """
 import argparse
 import csv
@@ -45,7 +51,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hvp_two_snowballs_1M_traffic.md")
+STATS_FILE = os.path.join(ROOT, "profiles", "two_snowballs_1M_kernel_stats.md")
 COPY_BYTES = 256 << 20
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def step_bytes(n_p, n_n, iters):
@@ -68,10 +79,12 @@ def cpu_baseline(sc, gpu_sim, steps=1):
     sim = O.Sim(x, sc["m"], sc["h"], model, cols, f)
     for name in ("v", "F", "volume", "Fp", "FinvTrans", "R", "Ginv", "J", "mu", "lambda"):
         sim.pd.set(name, gpu_sim.get(name))
+    progress("cpu baseline: oracle set up, timing %d step(s)" % steps)
     t0 = time.perf_counter()
     for _ in range(steps):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
     dt = time.perf_counter() - t0
+    progress("cpu baseline: %.1f s" % dt)
     n = len(sc["m"])
     its = [s["iterations"] for s in sim.last_stats()]
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
@@ -94,6 +107,7 @@ def learned_leg(steps, warmup):
         sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
                     mu0=sc["mu0"], lambda0=sc["lambda0"], xi=sc["xi"])
         sim.ctx.lpc_set_params(w)
+        progress("learned leg: %s, 4M slab set up" % name)
         for _ in range(warmup):
             sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
         sim.ctx.lpc_mode(mode)
@@ -106,6 +120,7 @@ def learned_leg(steps, warmup):
             big = max(rows, key=lambda r: r["particles"])
             its.append(big["iterations"])
             fbs.append(big["fallbacks"])
+            progress("learned leg: %s step %d, largest body %d iterations" % (name, len(its), its[-1]))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         out[name] = {"value": len(sc["m"]) * steps / dt, "ms_per_step": dt / steps * 1e3,
@@ -137,11 +152,12 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         copy_kib = max(copies)
         factor = copy_kib * 1024.0 / COPY_BYTES
         hvp = [(g, v) for n, g, v in rows if "k_hvp<true>" in n]
+        # launches of the steps after the warm-up (the store is body- and tile-sorted from the
+        # first step on), full grid (every chunk launched): the median of those
+        hvp = hvp[len(hvp) * 3 // 5:]
         gmax = max(g for g, _ in hvp)
-        full = [v for g, v in hvp if g == gmax]
-        # full-size launches: every body still iterating (the large bodies' iterations)
-        full = sorted(full)[len(full) // 2:]
-        kib = sum(full) / len(full)
+        full = sorted(v for g, v in hvp if g == gmax)
+        kib = full[len(full) // 2]
         res[counter] = {"copy_kib": copy_kib, "factor": factor, "hvp_kib_per_launch": kib,
                         "hvp_bytes_per_launch": kib * 1024.0 / factor, "launches": len(full)}
     traffic = res["FETCH_SIZE"]["hvp_bytes_per_launch"] + res["WRITE_SIZE"]["hvp_bytes_per_launch"]
@@ -152,14 +168,14 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         "",
         "Commands (from the repository root on the GPU host):",
         "",
-        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --no-learned --copy-calibration",
-        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 1 --no-cpu --no-learned --copy-calibration",
+        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python bench.py --steps 2 --warmup 3 --no-cpu --no-learned --copy-calibration",
+        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 3 --no-cpu --no-learned --copy-calibration",
         "    python bench.py --pmc-summary <F>/run_counter_collection.csv <W>/run_counter_collection.csv",
         "",
         "The counters are scaled by what they read for a %d MiB device-to-device copy (`__amd_rocclr_copyBuffer`):"
         % (COPY_BYTES >> 20),
-        "factor = counter KiB x 1024 / copied bytes. HVP figures average the upper half (by value) of the",
-        "full-grid `k_hvp<true>` dispatches (launches in which every body still iterates).",
+        "factor = counter KiB x 1024 / copied bytes. HVP figures: median over the full-grid `k_hvp<true>`",
+        "dispatches (every chunk launched) of the last 40% of HVP dispatches (the timed and table steps).",
         "",
         "| counter | copy KiB | factor | HVP KiB/launch | HVP bytes/launch (scaled) |",
         "|---|---|---|---|---|",
@@ -170,6 +186,60 @@ def pmc_summary(fetch_csv, write_csv, out_path):
                                                           r["hvp_bytes_per_launch"]))
     lines += ["", "traffic (FETCH + WRITE) per HVP launch: %.4g bytes" % traffic, "",
               "<!-- json: %s -->" % json.dumps(summary)]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(json.dumps(summary))
+
+
+def _short(name):
+    name = name.replace("mpm::(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0]
+
+
+def trace_summary(stats_csv, bench_log, out_path):
+    """Kernel table of a rocprofv3 --kernel-trace --stats run of the bench, and the HVP check."""
+    rows = []
+    with open(stats_csv) as fh:
+        for r in csv.DictReader(fh):
+            rows.append((_short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]), float(r["Percentage"])))
+    rows.sort(key=lambda r: -r[2])
+    bench = None
+    for line in open(bench_log):
+        line = line.strip()
+        if line.startswith("{") and '"metric"' in line:
+            bench = json.loads(line)
+    if bench is None:
+        raise SystemExit("no bench JSON line in %s" % bench_log)
+    hvp_calls = sum(c for n, c, t, _ in rows if n.startswith("k_hvp<"))
+    hvp_ns = sum(t for n, c, t, _ in rows if n.startswith("k_hvp<"))
+    rp_ms = hvp_ns / hvp_calls / 1e6
+    ev_ms = bench["roofline"]["ms_per_launch"]
+    steps, warmup = bench["steps"], bench["warmup"]
+    lines = [
+        "# Kernel statistics, two snowballs (1M particles)",
+        "",
+        "Command (from the repository root on the GPU host):",
+        "",
+        "    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- python bench.py --steps %d "
+        "--warmup %d --no-cpu --no-learned > <LOG>" % (steps, warmup),
+        "    python bench.py --trace-summary <D>/run_kernel_stats.csv <LOG>",
+        "",
+        "The profiled run covers %d warm-up steps, %d timed steps and the 2 steps of the bench's per-kernel "
+        "table. bench.py line of that run: %.4g particle-steps/s, %.3f ms/step (under the profiler)."
+        % (warmup, steps, bench["value"], bench["ms_per_step"]),
+        "",
+        "HVP (`k_hvp<true>` and `k_hvp<false>`, every launch of the run): %d launches, mean %.4f ms "
+        "(rocprofv3); bench.py's HIP events over its timed steps: %d launches, mean %.4f ms (ratio %.3f)."
+        % (hvp_calls, rp_ms, bench["roofline"]["launches"], ev_ms, rp_ms / ev_ms if ev_ms else float("nan")),
+        "",
+        "| kernel | calls | total ms | mean us | share |",
+        "|---|---|---|---|---|",
+    ]
+    for n, c, t, pct in rows[:20]:
+        lines.append("| `%s` | %d | %.2f | %.1f | %.1f%% |" % (n, c, t / 1e6, t / c / 1e3, pct))
+    summary = {"workload": "two_snowballs_1M", "hvp_launches": hvp_calls, "hvp_ms_rocprof": rp_ms,
+               "hvp_ms_events": ev_ms, "steps": steps, "warmup": warmup}
+    lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
     with open(out_path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print(json.dumps(summary))
@@ -195,9 +265,14 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
+    ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_CSV", "BENCH_LOG"))
+    ap.add_argument("--trace-out", default=STATS_FILE)
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.pmc_out)
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary[0], args.trace_summary[1], args.trace_out)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the step on a single MI355X "
@@ -222,9 +297,11 @@ def main():
     n = len(sc["m"])
     sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
                 v=sc["v"])
+    progress("two snowballs %d particles set up" % n)
     for _ in range(args.warmup):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
     torch.cuda.synchronize()
+    progress("warm-up done")
     # timed region: only HVP launches carry HIP events
     sim.profile(["hvp"])
     stats = []
@@ -234,6 +311,7 @@ def main():
         nb = stats[-1]["num_solved"]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    progress("timed steps done: %.3f ms/step" % ((t1 - t0) / args.steps * 1e3))
     hvp = sim.profile_query()["hvp"]
     elapsed = t1 - t0
     value = n * args.steps / elapsed

@@ -61,7 +61,8 @@ constexpr int kFoot = 512;        // nodes in a tile's 2x2x2-tile footprint
 constexpr int kStage = 3 * kFoot; // floats per staged chunk footprint
 constexpr int kChunk = 256;       // max particles per workgroup chunk
 constexpr int kBlock = 256;       // threads per workgroup; node blocks cover 4 tiles
-constexpr int kHistCap = 4096;    // residual history entries kept per body
+constexpr int kHistCap = 4096;
+constexpr int kKeyCodeShift = 24;  // binning keys: global tile in the low bits, base-node code above    // residual history entries kept per body
 
 // Grid geometry of one body (Grid.cpp:193-213) and its place in the batch.
 struct BodyGrid {

@@ -917,7 +917,7 @@ __global__ void k_bin(PView P, const int* slots, const int* labels, int n, const
     const BodyGrid& G = bodies[b];
     float x[3];
     load_x(P, s, x);
-    int t3[3];
+    int t3[3], code = 0;
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -926,6 +926,7 @@ __global__ void k_bin(PView P, const int* slots, const int* labels, int n, const
       int t = bc >> 2;
       ok = ok && bc >= 0 && t < G.T[d];
       t3[d] = t;
+      code |= (bc & 3) << (2 * d);
     }
     if (!ok) {
       atomicOr(err, ERR_STENCIL);
@@ -933,59 +934,79 @@ __global__ void k_bin(PView P, const int* slots, const int* labels, int n, const
       continue;
     }
     int t = G.tileBase + t3[0] + G.T[0] * (t3[1] + G.T[1] * t3[2]);
-    key[k] = t;
+    key[k] = t | (code << kKeyCodeShift);  // the stencil base node within the tile, for k_sort_tiles
     rank[k] = atomicAdd(count + t, 1);
   }
 }
-__global__ void k_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm) {
+__global__ void k_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
+                          int8_t* pcode) {
   GRID_STRIDE(k, n) {
-    int t = key[k];
-    if (t < 0) continue;
-    perm[offset[t] + rank[k]] = slots ? slots[k] : (int)k;
+    const int kc = key[k];
+    if (kc < 0) continue;
+    const int t = kc & ((1 << kKeyCodeShift) - 1);
+    const int pos = offset[t] + rank[k];
+    perm[pos] = slots ? slots[k] : (int)k;
+    pcode[pos] = (int8_t)(kc >> kKeyCodeShift);
   }
 }
-// Orders each tile's particles by slot. The binning's atomics leave them in arrival order, and
-// the split of a tile into chunks of kChunk particles fixes each chunk's fixed-point scale:
-// sorted tiles make every staged footprint, and so the whole step, bitwise reproducible.
-// Tiles up to kSortCap particles sort in LDS (bitonic); larger ones rank by counting in global
-// memory through tmp (a dead per-particle array of the binning).
+// Orders each tile's particles deterministically and so that particles sharing a stencil base
+// node (their 64 scatter targets coincide, and same-address LDS atomics serialise) are spread
+// over different waves: by (r, b), b the base node within the tile (0..63) and r the particle's
+// rank by slot among those with base b. The binning's atomics leave a tile in arrival order,
+// and its split into chunks of kChunk particles fixes each chunk's fixed-point scale, so a fixed
+// order makes every staged footprint, and the whole step, bitwise reproducible. Positions come
+// from counting (each key is unique): tiles up to kSortCap particles count in LDS, larger ones
+// in global memory through tmpA and tmpB (the binning's per-particle key and rank arrays, dead
+// by now).
 constexpr int kSortCap = 4096;
-__global__ __launch_bounds__(kBlock) void k_sort_tiles(const int* count, const int* offset, int* perm, int* tmp) {
-  __shared__ int sm[kSortCap];
+__global__ __launch_bounds__(kBlock) void k_sort_tiles(const int* count, const int* offset, int* perm,
+                                                       const int8_t* pcode, int* tmpA, int* tmpB) {
+  __shared__ int sSlot[kSortCap], sKey[kSortCap];
+  __shared__ int8_t sCode[kSortCap];
   const int t = blockIdx.x;
   const int c = count[t];
   if (c <= 1) return;
   int* seg = perm + offset[t];
+  const int8_t* cseg = pcode + offset[t];
   if (c <= kSortCap) {
-    int n2 = 2;
-    while (n2 < c) n2 <<= 1;
-    for (int i = threadIdx.x; i < n2; i += kBlock) sm[i] = i < c ? seg[i] : INT_MAX;
-    __syncthreads();
-    for (int k = 2; k <= n2; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < n2; i += kBlock) {
-          const int l = i ^ j;
-          if (l > i) {
-            const int a = sm[i], b = sm[l];
-            if ((a > b) == ((i & k) == 0)) {
-              sm[i] = b;
-              sm[l] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    for (int i = threadIdx.x; i < c; i += kBlock) seg[i] = sm[i];
-  } else {
-    int* out = tmp + offset[t];
     for (int i = threadIdx.x; i < c; i += kBlock) {
-      const int v = seg[i];
-      int r = 0;
-      for (int j = 0; j < c; ++j) r += seg[j] < v;
-      out[r] = v;
+      sSlot[i] = seg[i];
+      sCode[i] = cseg[i];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < c; i += kBlock) seg[i] = out[i];
+    for (int i = threadIdx.x; i < c; i += kBlock) {
+      const int v = sSlot[i], b = sCode[i];
+      int r = 0;
+      for (int j = 0; j < c; ++j) r += (sCode[j] == b) & (sSlot[j] < v);
+      sKey[i] = r * 64 + b;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += kBlock) {
+      const int kk = sKey[i];
+      int pos = 0;
+      for (int j = 0; j < c; ++j) pos += sKey[j] < kk;
+      seg[pos] = sSlot[i];
+    }
+  } else {
+    int* key = tmpA + offset[t];
+    int* pos = tmpB + offset[t];
+    for (int i = threadIdx.x; i < c; i += kBlock) {
+      const int v = seg[i], b = cseg[i];
+      int r = 0;
+      for (int j = 0; j < c; ++j) r += (cseg[j] == b) & (seg[j] < v);
+      key[i] = r * 64 + b;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += kBlock) {
+      const int kk = key[i];
+      int p = 0;
+      for (int j = 0; j < c; ++j) p += key[j] < kk;
+      pos[i] = p;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += kBlock) key[pos[i]] = seg[i];  // keys are no longer read
+    __syncthreads();
+    for (int i = threadIdx.x; i < c; i += kBlock) seg[i] = key[i];
   }
 }
 __global__ void k_chunk_counts(const int* count, int NT, int* nch) {
@@ -2737,8 +2758,8 @@ void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyG
   if (n > 0) LAUNCH(k_bin, grid_for(n, 16384), P, slots, labels, n, bodies, key, rank, count, err);
 }
 void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
-                    hipStream_t s) {
-  if (n > 0) LAUNCH(k_scatter, grid_for(n, 16384), slots, key, rank, offset, n, perm);
+                    int8_t* pcode, hipStream_t s) {
+  if (n > 0) LAUNCH(k_scatter, grid_for(n, 16384), slots, key, rank, offset, n, perm, pcode);
 }
 void launch_chunk_counts(const int* count, int NT, int* nch, hipStream_t s) {
   if (NT > 0) LAUNCH(k_chunk_counts, grid_for(NT), count, NT, nch);
@@ -2869,8 +2890,9 @@ void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const f
                         CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
   NODE_LAUNCH(k_cr_update_p, B, z, Ar, diag, p, Ap, sc, st, partials);
 }
-void launch_sort_tiles(const int* count, const int* offset, int* perm, int* tmp, int NT, hipStream_t s) {
-  if (NT > 0) hipLaunchKernelGGL(k_sort_tiles, dim3(NT), dim3(kBlock), 0, s, count, offset, perm, tmp);
+void launch_sort_tiles(const int* count, const int* offset, int* perm, const int8_t* pcode, int* tmpA, int* tmpB,
+                       int NT, hipStream_t s) {
+  if (NT > 0) hipLaunchKernelGGL(k_sort_tiles, dim3(NT), dim3(kBlock), 0, s, count, offset, perm, pcode, tmpA, tmpB);
 }
 void launch_small_claim(const int* list, int n, CRScalars* sc, BatchStatus* st, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_small_claim, dim3((n + 255) / 256), dim3(256), 0, s, list, n, sc, st);

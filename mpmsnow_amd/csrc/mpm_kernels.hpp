@@ -47,7 +47,7 @@ void launch_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int
 void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
                 int* count, int* err, hipStream_t s);
 void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
-                    hipStream_t s);
+                    int8_t* pcode, hipStream_t s);
 void launch_chunk_counts(const int* count, int NT, int* nch, hipStream_t s);
 void launch_make_chunks(const int* count, const int* offset, const int* chunk_off, const int* tileBody, int NT,
                         int4* chunks, hipStream_t s);
@@ -104,8 +104,10 @@ void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const 
                   hipStream_t s);
 void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
                         CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
-// order each tile's particles by slot (tmp: n ints of scratch, n >= particles binned)
-void launch_sort_tiles(const int* count, const int* offset, int* perm, int* tmp, int NT, hipStream_t s);
+// order each tile's particles by (rank among equal stencil bases, base) (tmpA, tmpB: n ints of
+// scratch each, n >= particles binned)
+void launch_sort_tiles(const int* count, const int* offset, int* perm, const int8_t* pcode, int* tmpA, int* tmpB,
+                       int NT, hipStream_t s);
 // small bodies: claim the listed bodies still iterating (done = kDoneSmall), then solve each in
 // one workgroup (setup and the whole CR loop)
 void launch_small_claim(const int* list, int n, CRScalars* sc, BatchStatus* st, hipStream_t s);

@@ -106,7 +106,7 @@ struct GridBatch {
   std::vector<BodyGrid> hb;
   int nb = 0, NT = 0, nblocks = 0, nchunks = 0, np = 0;
   int64_t NN = 0;
-  DevBuf bodies, tileBody, blockBody, perm, key, rank, count, offset, nch, chunkOff, chunks, scantmp, totals;
+  DevBuf bodies, tileBody, blockBody, perm, pcode, key, rank, count, offset, nch, chunkOff, chunks, scantmp, totals;
   DevBuf stageA, stageB;
   DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, tickets, hist, dense,
       dense2;
@@ -384,6 +384,7 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
     tileBase += G.NT;
   }
   NT = tileBase;
+  if (NT >= (1 << kKeyCodeShift)) throw std::runtime_error("grid is too big");
   nblocks = NT / 4;
   NN = (int64_t)NT * 64;
   levelsBuilt = false;
@@ -397,6 +398,7 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   key.ensure(sizeof(int) * n);
   rank.ensure(sizeof(int) * n);
   perm.ensure(sizeof(int) * np);
+  pcode.ensure(np);
   count.ensure(sizeof(int) * NT);
   offset.ensure(sizeof(int) * NT);
   nch.ensure(sizeof(int) * NT);
@@ -410,8 +412,9 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   scan_exclusive(count.as<int>(), offset.as<int>(), NT, totals.as<int>(), scantmp.as<int>(), s);
   launch_chunk_counts(count.as<int>(), NT, nch.as<int>(), s);
   scan_exclusive(nch.as<int>(), chunkOff.as<int>(), NT, totals.as<int>() + 1, scantmp.as<int>(), s);
-  launch_scatter(slotList, key.as<int>(), rank.as<int>(), offset.as<int>(), n, perm.as<int>(), s);
-  launch_sort_tiles(count.as<int>(), offset.as<int>(), perm.as<int>(), rank.as<int>(), NT, s);
+  launch_scatter(slotList, key.as<int>(), rank.as<int>(), offset.as<int>(), n, perm.as<int>(), pcode.as<int8_t>(), s);
+  launch_sort_tiles(count.as<int>(), offset.as<int>(), perm.as<int>(), pcode.as<int8_t>(), key.as<int>(),
+                    rank.as<int>(), NT, s);
   int tot[2] = {0, 0};
   int e = 0;
   HIPCHECK(hipMemcpyAsync(tot, totals.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
