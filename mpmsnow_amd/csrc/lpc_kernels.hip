@@ -362,16 +362,17 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, bf16* pb, i
 __global__ void k_f32_to_bf16(const float* in, bf16* out, int64_t n) {
   LPC_STRIDE(i, n) out[i] = (bf16)in[i];
 }
-__global__ void k_level_tile_body(const LevelGrid* bodies, int nb, int NT, int* tileBody) {
+// order: the bodies by ascending tileBase
+__global__ void k_level_tile_body(const LevelGrid* bodies, const int* order, int nb, int NT, int* tileBody) {
   LPC_STRIDE(ti, NT) {
     const int t = (int)ti;
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (bodies[mid].tileBase <= t) lo = mid;
+      if (bodies[order[mid]].tileBase <= t) lo = mid;
       else hi = mid - 1;
     }
-    tileBody[t] = lo;
+    tileBody[t] = order[lo];
   }
 }
 
@@ -566,8 +567,9 @@ void launch_adam(float* p, const float* g, float* m, float* v, bf16* pb, int64_t
 void launch_f32_to_bf16(const float* in, bf16* out, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid1d(n)), dim3(kThreads), 0, s, in, out, n);
 }
-void launch_level_tile_body(const LevelGrid* bodies, int nb, int NT, int* tileBody, hipStream_t s) {
-  if (NT > 0) hipLaunchKernelGGL(k_level_tile_body, dim3(grid1d(NT)), dim3(kThreads), 0, s, bodies, nb, NT, tileBody);
+void launch_level_tile_body(const LevelGrid* bodies, const int* order, int nb, int NT, int* tileBody, hipStream_t s) {
+  if (NT > 0)
+    hipLaunchKernelGGL(k_level_tile_body, dim3(grid1d(NT)), dim3(kThreads), 0, s, bodies, order, nb, NT, tileBody);
 }
 void launch_lpc_features(const Batch& B, const float* r, const float* diag, const float* mass, const int8_t* flags,
                          const float* scale, const CRScalars* sc, bf16* in, hipStream_t s) {

@@ -80,7 +80,7 @@ void launch_adam(float* p, const float* g, float* m, float* v, bf16* pb, int64_t
 void launch_sumsq(const float* g, int64_t n, double* out, hipStream_t s);
 void launch_f32_to_bf16(const float* in, bf16* out, int64_t n, hipStream_t s);
 // per-level tile -> body table
-void launch_level_tile_body(const LevelGrid* bodies, int nb, int NT, int* tileBody, hipStream_t s);
+void launch_level_tile_body(const LevelGrid* bodies, const int* order, int nb, int NT, int* tileBody, hipStream_t s);
 // network input at level 0, every channel linear in q = clamp((r / diag) / scale): q (3),
 // q m / diag (3), q [collision flag set] (3; flags may be null), zeros (7). With zero biases the
 // network is then positively homogeneous in r. The per-body scale comes from `scale`

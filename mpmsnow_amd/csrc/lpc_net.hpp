@@ -46,12 +46,15 @@ struct LpcLevels {
     v.NT = NT[l];
     return v;
   }
-  // level 0 = the batch's node tiles; levels 1, 2 halve each axis
-  void build(const std::vector<mpm::BodyGrid>& hb, hipStream_t s) {
+  // level 0 = the batch's node tiles; levels 1, 2 halve each axis. order: the bodies in the
+  // batch's tile layout order (ascending tileBase), also on the device (orderDev); levels 1, 2 are
+  // laid out in the same order.
+  void build(const std::vector<mpm::BodyGrid>& hb, const std::vector<int>& order, const int* orderDev,
+             hipStream_t s) {
     const int nb = (int)hb.size();
     for (int l = 0; l < 3; ++l) host[l].assign(nb, mpm::LevelGrid{});
     int base[3] = {0, 0, 0};
-    for (int b = 0; b < nb; ++b) {
+    for (int b : order) {
       int n[3] = {hb[b].n[0], hb[b].n[1], hb[b].n[2]};
       for (int l = 0; l < 3; ++l) {
         mpm::LevelGrid& G = host[l][b];
@@ -71,7 +74,7 @@ struct LpcLevels {
       HIPCHECK(hipMemcpyAsync(bodies[l].p, host[l].data(), sizeof(mpm::LevelGrid) * nb, hipMemcpyHostToDevice, s));
       if (l > 0) {
         tileBody[l].ensure(sizeof(int) * std::max(NT[l], 1));
-        mpm::launch_level_tile_body(bodies[l].as<mpm::LevelGrid>(), nb, NT[l], tileBody[l].as<int>(), s);
+        mpm::launch_level_tile_body(bodies[l].as<mpm::LevelGrid>(), orderDev, nb, NT[l], tileBody[l].as<int>(), s);
       }
     }
     HIPCHECK(hipStreamSynchronize(s));

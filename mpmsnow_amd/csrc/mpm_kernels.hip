@@ -892,17 +892,18 @@ __global__ void k_body_reduce(PView P, const int* slots, const int* labels, int 
 }
 
 // ------------------------------------------------------------ binning
-__global__ void k_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody) {
+// order: the bodies by ascending tileBase (the batch's layout order)
+__global__ void k_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, int* tileBody, int* blockBody) {
   GRID_STRIDE(ti, NT) {
     int t = (int)ti;
     int lo = 0, hi = nb - 1;
-    while (lo < hi) {  // last body with tileBase <= t
+    while (lo < hi) {  // last body in layout order with tileBase <= t
       int mid = (lo + hi + 1) >> 1;
-      if (bodies[mid].tileBase <= t) lo = mid;
+      if (bodies[order[mid]].tileBase <= t) lo = mid;
       else hi = mid - 1;
     }
-    tileBody[t] = lo;
-    if ((t & 3) == 0) blockBody[t >> 2] = lo;
+    tileBody[t] = order[lo];
+    if ((t & 3) == 0) blockBody[t >> 2] = order[lo];
   }
 }
 __global__ void k_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
@@ -2750,8 +2751,9 @@ void launch_body_reduce(PView P, const int* slots, const int* labels, int n, uns
                         int* count, hipStream_t s) {
   if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 512), P, slots, labels, n, bbox, mom, count);
 }
-void launch_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody, hipStream_t s) {
-  if (NT > 0) LAUNCH(k_tile_body, grid_for(NT), bodies, nb, NT, tileBody, blockBody);
+void launch_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, int* tileBody, int* blockBody,
+                      hipStream_t s) {
+  if (NT > 0) LAUNCH(k_tile_body, grid_for(NT), bodies, order, nb, NT, tileBody, blockBody);
 }
 void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
                 int* count, int* err, hipStream_t s) {

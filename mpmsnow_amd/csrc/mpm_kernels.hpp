@@ -43,7 +43,8 @@ void launch_body_reduce(PView P, const int* slots, const int* labels, int n, uns
                         int* count, hipStream_t s);
 
 // ---- binning / counting sort into the batch
-void launch_tile_body(const BodyGrid* bodies, int nb, int NT, int* tileBody, int* blockBody, hipStream_t s);
+void launch_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, int* tileBody, int* blockBody,
+                      hipStream_t s);
 void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
                 int* count, int* err, hipStream_t s);
 void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,

@@ -106,13 +106,16 @@ struct GridBatch {
   std::vector<BodyGrid> hb;
   int nb = 0, NT = 0, nblocks = 0, nchunks = 0, np = 0;
   int64_t NN = 0;
-  DevBuf bodies, tileBody, blockBody, perm, pcode, key, rank, count, offset, nch, chunkOff, chunks, scantmp, totals;
+  DevBuf bodies, bodyOrder, tileBody, blockBody, perm, pcode, key, rank, count, offset, nch, chunkOff, chunks, scantmp, totals;
   DevBuf stageA, stageB;
   DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, tickets, hist, dense,
       dense2;
   DevBuf bbox, mom, cnt;
   DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
   DevBuf smallList;               // bodies handed to the small-body solver
+  std::vector<char> smallLayout;  // per body: laid out after the prefix (small-body solver candidate)
+  std::vector<int> layoutOrder;   // bodies by ascending tileBase
+  int prefixTiles = 0, prefixChunks = 0;  // tiles / chunks of the bodies laid out first
   LpcLevels levels;
   bool levelsBuilt = false;
   uint64_t version = 0;  // bumped by every build
@@ -139,13 +142,16 @@ struct GridBatch {
   CRScalars* sc() { return scal.as<CRScalars>(); }
   BatchStatus* st() { return status.as<BatchStatus>(); }
 
-  void build(const int* slots, const int* labels, int n, int nbodies, float h, const float* frameV, bool computeFrame);
+  // indexOrder: lay the bodies' tiles out in body order (no small-body suffix)
+  void build(const int* slots, const int* labels, int n, int nbodies, float h, const float* frameV, bool computeFrame,
+             bool indexOrder = false);
   void forces_stage();
   void hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter);
+  void hvp(const Batch& Bv, const float* d, bool project, float* stage, const CRScalars* scp, int iter);
   void solve(float dt, const mpm_solver_params* prm);
   void train_lpc(float dt, CRScalars* hsc, size_t scb);
   void ensure_levels() {
-    if (!levelsBuilt) levels.build(hb, ctx_stream());
+    if (!levelsBuilt) levels.build(hb, layoutOrder, bodyOrder.as<int>(), ctx_stream());
     levelsBuilt = true;
   }
   hipStream_t ctx_stream() const;
@@ -326,7 +332,7 @@ hipStream_t GridBatch::ctx_stream() const { return ctx->stream; }
 // Grid::Grid (Grid.cpp:156-263) for every body of the batch: bbox and frame per body,
 // geometry, binning into the batch's tiles, mass and velocity splat.
 void GridBatch::build(const int* slotList, const int* labelList, int n, int nbodies, float h, const float* frameV,
-                      bool computeFrame) {
+                      bool computeFrame, bool indexOrder) {
   mpm_context& c = *ctx;
   hipStream_t s = c.stream;
   nb = nbodies;
@@ -378,10 +384,28 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
     nt = (nt + 3) / 4 * 4;
     if ((long long)tileBase + nt > INT32_MAX / 256) throw std::runtime_error("grid is too big");
     G.NT = (int)nt;
-    G.tileBase = tileBase;
-    G.blockBase = tileBase / 4;
     G.nblocks = G.NT / 4;
     tileBase += G.NT;
+  }
+  // tile layout: the bodies the batched CR loop iterates first, the small-body solver's after, so
+  // that the loop's launches cover a prefix of the chunks and node blocks
+  smallLayout.assign(nb, 0);
+  std::vector<int>& order = layoutOrder;
+  order.clear();
+  order.reserve(nb);
+  tileBase = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int b = 0; b < nb; ++b) {
+      BodyGrid& G = hb[b];
+      const int small = !indexOrder && G.nblocks <= kSmallBlocks && G.np <= c.smallMaxParticles ? 1 : 0;
+      if (small != pass) continue;
+      smallLayout[b] = (char)small;
+      order.push_back(b);
+      G.tileBase = tileBase;
+      G.blockBase = tileBase / 4;
+      tileBase += G.NT;
+    }
+    if (pass == 0) prefixTiles = tileBase;
   }
   NT = tileBase;
   if (NT >= (1 << kKeyCodeShift)) throw std::runtime_error("grid is too big");
@@ -393,7 +417,9 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   HIPCHECK(hipMemcpyAsync(bodies.p, hb.data(), sizeof(BodyGrid) * nb, hipMemcpyHostToDevice, s));
   tileBody.ensure(sizeof(int) * NT);
   blockBody.ensure(sizeof(int) * nblocks);
-  launch_tile_body(bodies.as<BodyGrid>(), nb, NT, tileBody.as<int>(), blockBody.as<int>(), s);
+  bodyOrder.ensure(sizeof(int) * nb);
+  HIPCHECK(hipMemcpyAsync(bodyOrder.p, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+  launch_tile_body(bodies.as<BodyGrid>(), bodyOrder.as<int>(), nb, NT, tileBody.as<int>(), blockBody.as<int>(), s);
   // binning + counting sort by global tile
   key.ensure(sizeof(int) * n);
   rank.ensure(sizeof(int) * n);
@@ -416,13 +442,16 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   launch_sort_tiles(count.as<int>(), offset.as<int>(), perm.as<int>(), pcode.as<int8_t>(), key.as<int>(),
                     rank.as<int>(), NT, s);
   int tot[2] = {0, 0};
-  int e = 0;
+  int e = 0, pc = 0;
   HIPCHECK(hipMemcpyAsync(tot, totals.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (prefixTiles < NT)
+    HIPCHECK(hipMemcpyAsync(&pc, chunkOff.as<int>() + prefixTiles, sizeof(int), hipMemcpyDeviceToHost, s));
   c.sync();
   raise_errors(e);
   if (tot[0] != np) throw std::runtime_error("particle binning lost particles (internal error)");
   nchunks = tot[1];
+  prefixChunks = prefixTiles < NT ? pc : nchunks;
   chunks.ensure(sizeof(int4) * std::max(nchunks, 1));
   launch_make_chunks(count.as<int>(), offset.as<int>(), chunkOff.as<int>(), tileBody.as<int>(), NT, chunks.as<int4>(),
                      s);
@@ -475,8 +504,11 @@ void GridBatch::forces_stage() {
   ctx->prof_end(pr);
 }
 void GridBatch::hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter) {
+  hvp(view(), d, project, stage, scp, iter);
+}
+void GridBatch::hvp(const Batch& Bv, const float* d, bool project, float* stage, const CRScalars* scp, int iter) {
   int pr = ctx->prof_begin(PK_HVP, this, iter);
-  launch_hvp(ctx->view(), view(), d, project, flags.as<int8_t>(), ctx->cols(), stage, scp, ctx->errp(), ctx->stream);
+  launch_hvp(ctx->view(), Bv, d, project, flags.as<int8_t>(), ctx->cols(), stage, scp, ctx->errp(), ctx->stream);
   ctx->prof_end(pr);
 }
 
@@ -590,7 +622,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     if (!hooks && c.cap.want == 0 && c.smallMaxParticles > 0 && maxIters <= (1 << 16)) {
       int* list = reinterpret_cast<int*>(reinterpret_cast<char*>(c.pinned.p) + scb + 4096);
       for (int bi = 0; bi < nb; ++bi) {
-        rec[bi].small = hb[bi].nblocks <= kSmallBlocks && hb[bi].np <= c.smallMaxParticles;
+        rec[bi].small = smallLayout[bi] != 0;
         if (rec[bi].small) list[nsmall++] = bi;
       }
       if (nsmall > 0) {
@@ -605,12 +637,18 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
         c.prof_end(ps);
       }
     }
+    // with the small-body solver running, the batched launches cover the prefix of the layout
+    Batch Bb = B;
+    if (nsmall > 0) {
+      Bb.nblocks = prefixTiles / 4;
+      Bb.nchunks = prefixChunks;
+    }
     // CR setup (ConjugateResiduals.cpp:50-76)
-    hvp(f(x), true, f(stageA), sc(), kProfSetup);
-    launch_cr_setup1(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
+    hvp(Bb, f(x), true, f(stageA), sc(), kProfSetup);
+    launch_cr_setup1(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
                      s);
-    hvp(f(z), true, f(stageA), sc(), kProfSetup);
-    launch_cr_setup2(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
+    hvp(Bb, f(z), true, f(stageA), sc(), kProfSetup);
+    launch_cr_setup2(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
                      partials.as<double>(), s);
     fetch_status();
     check_errflag();
@@ -634,19 +672,19 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     // one CR iteration (ConjugateResiduals.cpp:84-151) for every body still iterating
     auto iterate_x = [&](int it) {
       int px = c.prof_begin(PK_CR_X, this, it);
-      launch_cr_update_x(B, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
+      launch_cr_update_x(Bb, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
                          partials.as<double>(), hist.as<double>(), s);
       c.prof_end(px);
       if (capWant > 0) snapshot(it + 1);
     };
     auto iterate_rest = [&](int it) {
-      hvp(f(z), true, f(stageA), sc(), it);
+      hvp(Bb, f(z), true, f(stageA), sc(), it);
       int pa = c.prof_begin(PK_CR_AR, this, it);
-      launch_cr_ar(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), sc(), st(),
+      launch_cr_ar(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), sc(), st(),
                    partials.as<double>(), s);
       c.prof_end(pa);
       int pp = c.prof_begin(PK_CR_P, this, it);
-      launch_cr_update_p(B, f(z), f(Ar), f(diag), f(p), f(Ap), sc(), st(), partials.as<double>(), s);
+      launch_cr_update_p(Bb, f(z), f(Ar), f(diag), f(p), f(Ap), sc(), st(), partials.as<double>(), s);
       c.prof_end(pp);
     };
     auto set_done_flag = [&](int bi, int v) {
@@ -1415,7 +1453,9 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
       // sequential loop would have done it
       if (nb > 1 || c.nballistic > 0)
         launch_plasticity(c.view(), labels, 0, n, c.thc, c.ths, c.errp(), s);
-      B.build(nullptr, labels, n, nb, c.h, nullptr, true);
+      // with a cancellation hook the bodies' chunks stay in body order: a cancelled step keeps the
+      // updates of a prefix of the bodies
+      B.build(nullptr, labels, n, nb, c.h, nullptr, true, params->cancelled != nullptr);
       c.check_errors();
       B.solve(dt, params);
       // the reference polls cancellation after each body's solve and keeps the particle
