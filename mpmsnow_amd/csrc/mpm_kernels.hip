@@ -1325,7 +1325,7 @@ __global__ void k_node_forces(Batch B, const float* mass, const float* stage, co
 // + the sums of the initial guess (Grid.cpp:911-947)
 __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, const float* stage, const float* grav,
                                 int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                                CRScalars* sc, BatchStatus* bst, double* partials) {
+                                CRScalars* sc, BatchStatus* bst, double* partials, unsigned char* blockActive) {
   NodeCtx c = node_begin(B);
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int bad = 0, active = 0;
@@ -1372,6 +1372,11 @@ __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, co
   st3(em, c.i, e);
   st3(vc, c.i, cv);
   flags[c.i] = (int8_t)fl;
+  // A node block with no mass and no collider flag keeps x = guess and z = p = Ap = 0 through the
+  // whole CR loop (b, A x and every HVP contribution vanish there): the loop kernels skip its
+  // loads and stores and add a zero partial in its place.
+  const int act = __syncthreads_or(active || fl != -1);
+  if (threadIdx.x == 0) blockActive[c.i >> 8] = act ? 1 : 0;
   CRScalars& S = sc[c.b];
   if (bad) atomicOr(&S.errors, ERR_NAN_MOMENTA);
   double sa[9] = {s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7], (double)active};
@@ -1738,7 +1743,8 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
-  cr_update_x_node(c, flags, cols, p, Ap, diag, (float)S.alpha, x, z, s);
+  if (!B.blockActive || B.blockActive[c.i >> 8])
+    cr_update_x_node(c, flags, cols, p, Ap, diag, (float)S.alpha, x, z, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
     if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
@@ -1775,7 +1781,8 @@ __global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const C
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
-  cr_ar_node(B, c, mass, flags, cols, stage, z, dt * dt, Ar, s);
+  if (!B.blockActive || B.blockActive[c.i >> 8])
+    cr_ar_node(B, c, mass, flags, cols, stage, z, dt * dt, Ar, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0)
     if (cr_ar_finish(S, tot)) set_done(S, bst);
@@ -1820,7 +1827,8 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
-  cr_update_p_node(c, z, Ar, diag, (float)S.beta, p, Ap, s);
+  if (!B.blockActive || B.blockActive[c.i >> 8])
+    cr_update_p_node(c, z, Ar, diag, (float)S.beta, p, Ap, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0)
     if (cr_update_p_finish(S, tot)) set_done(S, bst);
@@ -2856,8 +2864,10 @@ void launch_node_forces(const Batch& B, const float* mass, const float* stage, c
 }
 void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
                           int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                          CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_node_explicit, B, mass, vel, stage, grav, ngrav, dt, cols, em, flags, vc, sc, st, partials);
+                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* blockActive,
+                          hipStream_t s) {
+  NODE_LAUNCH(k_node_explicit, B, mass, vel, stage, grav, ngrav, dt, cols, em, flags, vc, sc, st, partials,
+              blockActive);
 }
 void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
                                  hipStream_t s) {
