@@ -64,8 +64,15 @@ def step_bytes(n_p, n_n, iters):
     return n_p * (692 + 172 * (iters + 3)) + n_n * (117 + 29 * (iters + 3) + 120 * (iters + 1))
 
 
-def cpu_baseline(sc, gpu_sim, steps=1):
-    """Time the reference-faithful CPU oracle for `steps` step(s) from the GPU run's current state."""
+STATE_FIELDS = ("p", "v", "F", "volume", "Fp", "FinvTrans", "R", "Ginv", "J", "mu", "lambda")
+
+
+def snapshot(sim):
+    return {k: sim.get(k) for k in STATE_FIELDS}
+
+
+def cpu_baseline(sc, state, steps=1):
+    """Time the reference-faithful CPU oracle for `steps` step(s) from a state of the GPU run."""
     from oracle import pyoracle as O
 
     mat = sc["material"]
@@ -75,10 +82,9 @@ def cpu_baseline(sc, gpu_sim, steps=1):
     f = O.Fields.create()
     f.add_gravity(sc["gravity"])
     model = O.Model.snow(mat["E"], mat["nu"], mat["xi"], mat["theta_c"], mat["theta_s"])
-    x = gpu_sim.get("p")
-    sim = O.Sim(x, sc["m"], sc["h"], model, cols, f)
-    for name in ("v", "F", "volume", "Fp", "FinvTrans", "R", "Ginv", "J", "mu", "lambda"):
-        sim.pd.set(name, gpu_sim.get(name))
+    sim = O.Sim(state["p"], sc["m"], sc["h"], model, cols, f)
+    for name in STATE_FIELDS[1:]:
+        sim.pd.set(name, state[name])
     progress("cpu baseline: oracle set up, timing %d step(s)" % steps)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -88,8 +94,9 @@ def cpu_baseline(sc, gpu_sim, steps=1):
     n = len(sc["m"])
     its = [s["iterations"] for s in sim.last_stats()]
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "%d step(s) of the same %d-particle scene from the GPU run's state (%.1f s; CR "
-                      "iterations of its two largest bodies %s)" % (steps, n, dt, sorted(its)[-2:])}
+            "sample": "%d step(s) of the same %d-particle scene from the GPU run's state at the start of its "
+                      "timed steps (%.1f s; CR iterations of its two largest bodies %s; later steps of the "
+                      "timed run take more iterations)" % (steps, n, dt, sorted(its)[-2:])}
 
 
 def learned_leg(steps, warmup):
@@ -146,9 +153,12 @@ def pmc_summary(fetch_csv, write_csv, out_path):
     res = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = _pmc_rows(path)
-        copies = [v for n, g, v in rows if "copyBuffer" in n]
+        # calibration kernel of known traffic: the 256 MiB copy writes COPY_BYTES; the reduction
+        # over its destination reads COPY_BYTES
+        calib = "reduce_kernel" if counter == "FETCH_SIZE" else "copyBuffer"
+        copies = [v for n, g, v in rows if calib in n]
         if not copies:
-            raise SystemExit("no copy dispatch in %s (run bench.py with --copy-calibration)" % path)
+            raise SystemExit("no %s dispatch in %s (run bench.py with --copy-calibration)" % (calib, path))
         copy_kib = max(copies)
         factor = copy_kib * 1024.0 / COPY_BYTES
         hvp = [(g, v) for n, g, v in rows if "k_hvp<true>" in n]
@@ -172,12 +182,14 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 3 --no-cpu --no-learned --copy-calibration",
         "    python bench.py --pmc-summary <F>/run_counter_collection.csv <W>/run_counter_collection.csv",
         "",
-        "The counters are scaled by what they read for a %d MiB device-to-device copy (`__amd_rocclr_copyBuffer`):"
+        "The counters are scaled by what they read for kernels of known traffic on a %d MiB buffer: FETCH_SIZE by"
         % (COPY_BYTES >> 20),
-        "factor = counter KiB x 1024 / copied bytes. HVP figures: median over the full-grid `k_hvp<true>`",
+        "torch's sum over it (reads it once), WRITE_SIZE by the device-to-device copy into it",
+        "(`__amd_rocclr_copyBuffer`, writes it once): factor = counter KiB x 1024 / buffer bytes.",
+        "HVP figures: median over the full-grid `k_hvp<true>`",
         "dispatches (every chunk launched) of the last 40% of HVP dispatches (the timed and table steps).",
         "",
-        "| counter | copy KiB | factor | HVP KiB/launch | HVP bytes/launch (scaled) |",
+        "| counter | calibration KiB | factor | HVP KiB/launch | HVP bytes/launch (scaled) |",
         "|---|---|---|---|---|",
     ]
     for k in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -291,6 +303,7 @@ def main():
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
+        float(b.sum())  # a read-only pass over the copy's destination
         del a, b
 
     sc = scenes.two_snowballs(n=args.particles, seed=1)
@@ -302,6 +315,7 @@ def main():
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
     torch.cuda.synchronize()
     progress("warm-up done")
+    start_state = None if args.no_cpu else snapshot(sim)
     # timed region: only HVP launches carry HIP events
     sim.profile(["hvp"])
     stats = []
@@ -364,7 +378,7 @@ def main():
         "kernels": kernels,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, sim, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(sc, start_state, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     if not args.no_learned:
         out["learned_pc"] = learned_leg(args.learned_steps, 3)

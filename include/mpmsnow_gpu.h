@@ -159,6 +159,10 @@ int mpm_sim_init(mpm_context* ctx, float grid_size, int dimension);
 /* Sim::advance (Sim.cpp:57-134) */
 int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params, mpm_step_stats* stats);
 int mpm_sim_num_bodies(const mpm_context* ctx);
+/* layout of the last step's batched grids (no reference counterpart; measurement): out[7] =
+ * bodies, tiles, particle chunks, node slots, particles in bodies, tiles and chunks of the
+ * bodies the batched CR loop iterates (laid out first) */
+int mpm_sim_grid_layout(const mpm_context* ctx, int64_t* out);
 /* particle indices of body b (ascending); returns the count (out may be NULL) */
 int64_t mpm_sim_body(const mpm_context* ctx, int body, int32_t* out);
 int64_t mpm_sim_ballistic(const mpm_context* ctx, int32_t* out);

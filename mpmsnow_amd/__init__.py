@@ -92,6 +92,7 @@ SIGNATURES = {
     "mpm_sim_init": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "mpm_sim_advance": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(SolverParams), C.POINTER(StepStats)]),
     "mpm_sim_num_bodies": (C.c_int, [C.c_void_p]),
+    "mpm_sim_grid_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "mpm_sim_body": (C.c_int64, [C.c_void_p, C.c_int, _ip]),
     "mpm_sim_ballistic": (C.c_int64, [C.c_void_p, _ip]),
     "mpm_sim_body_solve_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolveStats)]),
@@ -451,6 +452,13 @@ class Sim:
         return {"num_bodies": st.num_bodies, "num_ballistic": st.num_ballistic,
                 "total_iterations": st.total_iterations, "max_iterations": st.max_iterations,
                 "cancelled": bool(st.cancelled), "num_solved": st.num_solved}
+
+    def grid_layout(self):
+        """Layout of the last step's batched grids (measurement)."""
+        a = (C.c_int64 * 7)()
+        self.ctx.check(lib().mpm_sim_grid_layout(self.ctx.ptr, a))
+        keys = ("bodies", "tiles", "chunks", "node_slots", "particles", "loop_tiles", "loop_chunks")
+        return dict(zip(keys, [int(v) for v in a]))
 
     def bodies(self):
         out = []

@@ -1652,6 +1652,19 @@ int mpm_profile_query(mpm_context* ctx, const char* kernel, double* total_ms, in
 
 int mpm_sim_num_bodies(const mpm_context* ctx) { return ctx ? ctx->nbodies : 0; }
 
+int mpm_sim_grid_layout(const mpm_context* ctx, int64_t* out) {
+  if (!ctx || !out) return MPM_ERR_INVALID;
+  const GridBatch& B = ctx->simBatch;
+  out[0] = B.nb;
+  out[1] = B.NT;
+  out[2] = B.nchunks;
+  out[3] = B.NN;
+  out[4] = B.np;
+  out[5] = B.prefixTiles;
+  out[6] = B.prefixChunks;
+  return MPM_OK;
+}
+
 int64_t mpm_sim_body(const mpm_context* ctx, int b, int32_t* out) {
   if (!ctx || b < 0 || b >= ctx->nbodies) return -1;
   try {
