@@ -91,7 +91,7 @@ struct Batch {
   int NT;                   // global tiles
   int nblocks;              // global node blocks
   unsigned int* tickets;    // reduction tickets: [body][set][1 + kSub][kTicketStride]
-  const unsigned char* blockActive;  // per node block: some node has mass or a collider flag
+  const unsigned char* tileActive;  // per tile: some node has mass or a collider flag
 };
 
 // Per-body reductions inside node kernels: every block stores its partial sums, then counts

@@ -1325,7 +1325,7 @@ __global__ void k_node_forces(Batch B, const float* mass, const float* stage, co
 // + the sums of the initial guess (Grid.cpp:911-947)
 __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, const float* stage, const float* grav,
                                 int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                                CRScalars* sc, BatchStatus* bst, double* partials, unsigned char* blockActive) {
+                                CRScalars* sc, BatchStatus* bst, double* partials, unsigned char* tileActive) {
   NodeCtx c = node_begin(B);
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int bad = 0, active = 0;
@@ -1372,11 +1372,24 @@ __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, co
   st3(em, c.i, e);
   st3(vc, c.i, cv);
   flags[c.i] = (int8_t)fl;
-  // A node block with no mass and no collider flag keeps x = guess and z = p = Ap = 0 through the
-  // whole CR loop (b, A x and every HVP contribution vanish there): the loop kernels skip its
-  // loads and stores and add a zero partial in its place.
-  const int act = __syncthreads_or(active || fl != -1);
-  if (threadIdx.x == 0) blockActive[c.i >> 8] = act ? 1 : 0;
+  // A tile outside every chunk's footprint (no particle in it or in its -x/-y/-z neighbour tiles)
+  // and with no collider flag keeps x = guess and z = p = Ap = 0 through the whole CR loop: b,
+  // A x and every staged HVP contribution are exactly zero there. The loop kernels skip its
+  // loads and stores and add zeros in its place. (Zero mass alone is not enough: a tiny edge
+  // weight can round to zero in the mass scatter and not in the HVP's.) A tile is one wave of a
+  // node block.
+  int src = 0;
+  {
+    int tx, ty, tz;
+    tile_xyz(c.G, c.T - c.G.tileBase, tx, ty, tz);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      const int ts = tile_id(c.G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
+      if (ts >= 0) src += B.tileChunkN[ts];
+    }
+  }
+  const bool act = __ballot(src > 0 || fl != -1) != 0ull;
+  if ((threadIdx.x & 63) == 0) tileActive[c.T] = act ? 1 : 0;
   CRScalars& S = sc[c.b];
   if (bad) atomicOr(&S.errors, ERR_NAN_MOMENTA);
   double sa[9] = {s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7], (double)active};
@@ -1743,7 +1756,7 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
-  if (!B.blockActive || B.blockActive[c.i >> 8])
+  if (!B.tileActive || B.tileActive[c.T])
     cr_update_x_node(c, flags, cols, p, Ap, diag, (float)S.alpha, x, z, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
@@ -1781,7 +1794,7 @@ __global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const C
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
-  if (!B.blockActive || B.blockActive[c.i >> 8])
+  if (!B.tileActive || B.tileActive[c.T])
     cr_ar_node(B, c, mass, flags, cols, stage, z, dt * dt, Ar, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0)
@@ -1827,7 +1840,7 @@ __global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const fl
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
-  if (!B.blockActive || B.blockActive[c.i >> 8])
+  if (!B.tileActive || B.tileActive[c.T])
     cr_update_p_node(c, z, Ar, diag, (float)S.beta, p, Ap, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0)
@@ -1937,6 +1950,8 @@ __device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const B
                                          const ColliderSet* cols, int* err, float (*sd)[kFootLds],
                                          FxFoot1<3>& acc) {
   for (int q = 0; q < nch; ++q) {
+    // the previous chunk's fx_store reads acc in another thread mapping than fx_zero writes it
+    __syncthreads();
     ChunkCtx c;
     c.ch = T.ch[q];
     c.G = G;
@@ -2104,7 +2119,10 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
     return;
   }
   auto hvp = [&](const float* d) {
-    for (int ch = c0; ch < c1; ++ch) hvp_chunk<true>(P, B, ch, d, flags, cols, stage, err, sd, acc, ft);
+    for (int ch = c0; ch < c1; ++ch) {
+      __syncthreads();  // the previous chunk's fx_store reads acc in another mapping than fx_zero writes it
+      hvp_chunk<true>(P, B, ch, d, flags, cols, stage, err, sd, acc, ft);
+    }
     __syncthreads();
   };
   __syncthreads();
@@ -2864,10 +2882,10 @@ void launch_node_forces(const Batch& B, const float* mass, const float* stage, c
 }
 void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
                           int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* blockActive,
+                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* tileActive,
                           hipStream_t s) {
   NODE_LAUNCH(k_node_explicit, B, mass, vel, stage, grav, ngrav, dt, cols, em, flags, vc, sc, st, partials,
-              blockActive);
+              tileActive);
 }
 void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
                                  hipStream_t s) {

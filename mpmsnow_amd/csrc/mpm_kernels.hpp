@@ -83,10 +83,10 @@ void launch_gather_vel(const Batch& B, const float* stage, float* vel, float* pr
 void launch_gather3(const Batch& B, const float* stage, float* out, hipStream_t s);
 void launch_node_forces(const Batch& B, const float* mass, const float* stage, const float* grav, int ngrav,
                         float* out, hipStream_t s);
-// also marks each node block active (some node with mass or a collider flag) in blockActive
+// also marks each tile active (some node with mass or a collider flag) in tileActive
 void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
                           int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* blockActive,
+                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* tileActive,
                           hipStream_t s);
 void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
                                  hipStream_t s);

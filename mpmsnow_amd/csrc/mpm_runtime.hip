@@ -111,7 +111,7 @@ struct GridBatch {
   DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, tickets, hist, dense,
       dense2;
   DevBuf bbox, mom, cnt;
-  DevBuf blockAct;  // per node block: active in the CR loop (set by k_node_explicit)
+  DevBuf tileAct;  // per tile: active in the CR loop (set by k_node_explicit)
   DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
   DevBuf smallList;               // bodies handed to the small-body solver
   std::vector<char> smallLayout;  // per body: laid out after the prefix (small-body solver candidate)
@@ -136,7 +136,7 @@ struct GridBatch {
     B.NT = NT;
     B.nblocks = nblocks;
     B.tickets = tickets.as<unsigned int>();
-    B.blockActive = blockAct.as<unsigned char>();
+    B.tileActive = tileAct.as<unsigned char>();
     return B;
   }
   float* f(DevBuf& d) { return d.as<float>(); }
@@ -467,8 +467,8 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   scal.ensure(sizeof(CRScalars) * nb);
   status.ensure(sizeof(BatchStatus));
   partials.ensure(sizeof(double) * kPartialStride * std::max(nblocks, 1));
-  blockAct.ensure(std::max(nblocks, 1));
-  HIPCHECK(hipMemsetAsync(blockAct.p, 1, std::max(nblocks, 1), s));
+  tileAct.ensure(std::max(NT, 1));
+  HIPCHECK(hipMemsetAsync(tileAct.p, 1, std::max(NT, 1), s));
   tickets.ensure(sizeof(unsigned int) * kTicketsPerBody * nb);
   HIPCHECK(hipMemsetAsync(tickets.p, 0, sizeof(unsigned int) * kTicketsPerBody * nb, s));
   HIPCHECK(hipMemsetAsync(flags.p, 0xFF, NN, s));  // -1: no collision until updateGridVelocities
@@ -554,7 +554,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
   // explicit momenta, flags, vc, guess sums
   forces_stage();
   launch_node_explicit(B, f(mass), f(vel), f(stageA), c.grav(), c.ngrav(), dt, c.cols(), f(b), flags.as<int8_t>(),
-                       f(vc), sc(), st(), partials.as<double>(), blockAct.as<unsigned char>(), s);
+                       f(vc), sc(), st(), partials.as<double>(), tileAct.as<unsigned char>(), s);
   // DF(vc) (no input projection) and the diagonal, then rhs / guess / diagonal finish
   hvp(f(vc), false, f(stageA), nullptr, -1);
   int pr = c.prof_begin(PK_DIAG, this);
@@ -1342,7 +1342,7 @@ int mpm_grid_explicit_momenta(mpm_grid* grid, float dt, float* momenta, int8_t* 
     // scratch outputs: the grid's own node flags are set only by updateGridVelocities
     launch_node_explicit(B.view(), B.f(B.mass), B.f(B.vel), B.f(B.stageA), c.grav(), c.ngrav(), dt, c.cols(),
                          B.f(B.tmp3), B.flags2.as<int8_t>(), B.f(B.Ar), B.sc(), B.st(), B.partials.as<double>(),
-                         B.blockAct.as<unsigned char>(), c.stream);
+                         B.tileAct.as<unsigned char>(), c.stream);
     CRScalars hs;
     HIPCHECK(hipMemcpyAsync(&hs, B.scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     c.sync();

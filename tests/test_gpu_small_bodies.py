@@ -90,3 +90,27 @@ def test_small_body_solver_is_bitwise_the_batched_loop():
     same_runs(sb, sc)
     for name in ("p", "v", "F"):
         assert np.array_equal(b.get(name), c.get(name)), name
+
+
+def test_fragmenting_collision_small_solver_bitwise():
+    """Two colliding snowballs (100k particles) shatter into fragments, many of them small bodies
+    with several chunks; 90 steps with the small-body solver on and off must agree bit for bit
+    (this caught a race between consecutive chunks of one workgroup's HVP)."""
+    def run_collision(limit):
+        sc = scenes.two_snowballs(n=100000, seed=3, r=0.07, speed=3.0)
+        sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
+                    gravity=[sc["gravity"]], v=sc["v"])
+        sim.ctx.small_bodies(limit)
+        its = []
+        for _ in range(90):
+            st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+            its.append((st["num_bodies"], st["total_iterations"]))
+        return sim, its
+    say("collision 100k, small-body solver off")
+    a, ia = run_collision(0)
+    say("collision 100k, small-body solver on")
+    b, ib = run_collision(2048)
+    say("bodies / CR iterations at steps 30, 60, 89: %s" % [ib[k] for k in (30, 60, 89)])
+    assert ia == ib
+    for name in ("p", "v", "F"):
+        assert np.array_equal(a.get(name), b.get(name)), name
