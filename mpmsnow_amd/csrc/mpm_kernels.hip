@@ -2505,8 +2505,9 @@ __global__ void k_finalize(Batch B, const float* x, const float* vc, float* vel,
 
 // ------------------------------------------------------------ G2P
 // updateParticleVelocities (Grid.cpp:1049-1081), updateDeformationGradients (Grid.cpp:1018-1047),
-// and for the full step SnowConstitutiveModel::updateParticleData (SnowConstitutiveModel.cpp:27-121)
-// plus the advection of Sim.cpp:123-131.
+// and for the full step (MODE 2) SnowConstitutiveModel::updateParticleData
+// (SnowConstitutiveModel.cpp:27-121). The step's advection (Sim.cpp:123-131) is k_advect: fused
+// here it raised the kernel to 256 VGPRs with spills.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* vel, const float* prev, float dt,
                                                 float thc, float ths, const ColliderSet* cols, int* err) {
@@ -2579,10 +2580,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
     if (DO_V) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) v[a] = alpha * vf[a] + beta * vp[a];
-      if (MODE != 2) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) P.at(C_V + a, s) = v[a];
-      }
+      for (int a = 0; a < 3; ++a) P.at(C_V + a, s) = v[a];
     }
     if (DO_F) {
       float F[9], A[9], Fn[9];
@@ -2605,16 +2604,6 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
         P.at(C_J, s) = J;
         P.at(C_MU, s) = mu;
         P.at(C_LAM, s) = lam;
-        if (MODE == 2) {
-          // advection (Sim.cpp:123-131)
-          const float zero[3] = {0.0f, 0.0f, 0.0f};
-          collide(*cols, v, x, zero, true);
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            P.at(C_V + a, s) = v[a];
-            P.at(C_X + a, s) = x[a] + v[a] * dt;
-          }
-        }
       }
     }
   }
@@ -2860,10 +2849,8 @@ void launch_g2p(PView P, const Batch& B, const float* vel, const float* prev, in
     LAUNCH(k_g2p<0>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
   else if (mode == 1)
     LAUNCH(k_g2p<1>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
-  else if (mode == 2)
-    LAUNCH(k_g2p<2>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
   else
-    LAUNCH(k_g2p<3>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
+    LAUNCH(k_g2p<2>, B.nchunks, P, B, vel, prev, dt, thc, ths, cols, err);
 }
 #define NODE_LAUNCH(kernel, ...) \
   if (B.nblocks > 0) hipLaunchKernelGGL(kernel, dim3(B.nblocks), dim3(kBlock), 0, s, __VA_ARGS__)

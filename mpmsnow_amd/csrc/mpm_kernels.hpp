@@ -72,8 +72,8 @@ void launch_hvp(PView P, const Batch& B, const float* d, bool project, const int
                 float* stage, const CRScalars* sc, int* err, hipStream_t s);
 // raw diagonal of the force differential (dForceidXi)
 void launch_diag(PView P, const Batch& B, float* stage, int* err, hipStream_t s);
-// G2P: mode 0 = velocities only, 1 = F only, 2 = full step (v, F, plasticity, advection),
-// 3 = full step without advection
+// G2P: mode 0 = velocities only, 1 = F only, 2 = full step (v, F, plasticity; advection is
+// launch_advect)
 void launch_g2p(PView P, const Batch& B, const float* vel, const float* prev, int mode, float dt, float thc, float ths,
                 const ColliderSet* cols, int* err, hipStream_t s);
 

@@ -1484,7 +1484,9 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
         view.nchunks = prefix;
       }
       int pg = c.prof_begin(PK_G2P, &B);
-      launch_g2p(c.view(), view, B.f(B.vel), B.f(B.prev), cancelled ? 3 : 2, dt, c.thc, c.ths, c.cols(), c.errp(), s);
+      launch_g2p(c.view(), view, B.f(B.vel), B.f(B.prev), 2, dt, c.thc, c.ths, c.cols(), c.errp(), s);
+      // advection of the body particles (Sim.cpp:123-131), not after a cancellation (Sim.cpp:107-111)
+      if (!cancelled) launch_advect(c.view(), labels, n, dt, c.cols(), s);
       c.prof_end(pg);
       c.lastSolves = B.rec;
       if (stats)
