@@ -215,6 +215,10 @@ int mpm_set_small_bodies(mpm_context* ctx, int max_particles);
  * algorithmic_bytes sums, over the timed launches, bytes/particle * particles +
  * bytes/node * nodes with m > 0 of that kernel's minimal HBM traffic. */
 int mpm_profile_enable(mpm_context* ctx, int mask); /* also resets the counters */
+/* Time one launch in `period` of each enabled class (1: every launch; the default): the
+ * event records around a timed launch idle the stream for a few microseconds, so sampling
+ * keeps the measurement from slowing what it measures. Resets the sampling counters. */
+int mpm_profile_period(mpm_context* ctx, int period);
 int mpm_profile_query(mpm_context* ctx, const char* kernel, double* total_ms, int64_t* launches,
                       double* algorithmic_bytes);
 

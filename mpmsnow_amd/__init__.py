@@ -109,6 +109,7 @@ SIGNATURES = {
                                          C.POINTER(C.c_double)]),
     "mpm_set_small_bodies": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_profile_period": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double)]),
 }
@@ -489,8 +490,9 @@ class Sim:
     def get(self, name):
         return self.ctx.get(name)
 
-    def profile(self, on=True):
-        """Time kernel classes with HIP events: True = all, False = off, or an iterable of names."""
+    def profile(self, on=True, period=1):
+        """Time kernel classes with HIP events: True = all, False = off, or an iterable of names;
+        one launch in `period` of each class is timed."""
         if on is True:
             mask = -1
         elif not on:
@@ -500,6 +502,7 @@ class Sim:
             for k in on:
                 mask |= 1 << PROFILE_KERNELS.index(k)
         self.ctx.check(lib().mpm_profile_enable(self.ctx.ptr, mask))
+        self.ctx.check(lib().mpm_profile_period(self.ctx.ptr, int(period)))
 
     def profile_query(self):
         out = {}

@@ -55,14 +55,24 @@ struct PView {
   float* base;
   int64_t stride;
   __device__ __forceinline__ float& at(int c, int s) const { return base[(int64_t)c * stride + s]; }
+  // at() for a column c that is the same in every lane of the wave: the column base is held in
+  // scalar registers and the slot is a 32-bit byte offset (slots stay below 2^30, which
+  // mpm_particles_create checks), the form the loads take as their vector operand
+  __device__ __forceinline__ float& atu(int c, int s) const {
+    const uint64_t o = (uint64_t)((int64_t)c * stride) * 4u;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(o >> 32));
+    char* col = reinterpret_cast<char*>(base) + ((hi << 32) | lo);
+    return *reinterpret_cast<float*>(col + (uint32_t)s * 4u);
+  }
 };
 
 constexpr int kFoot = 512;        // nodes in a tile's 2x2x2-tile footprint
 constexpr int kStage = 3 * kFoot; // floats per staged chunk footprint
 constexpr int kChunk = 256;       // max particles per workgroup chunk
 constexpr int kBlock = 256;       // threads per workgroup; node blocks cover 4 tiles
-constexpr int kHistCap = 4096;
-constexpr int kKeyCodeShift = 24;  // binning keys: global tile in the low bits, base-node code above    // residual history entries kept per body
+constexpr int kHistCap = 4096;     // residual history entries kept per body
+constexpr int kKeyCodeShift = 24;  // binning keys: global tile in the low bits, base-node code above
 
 // Grid geometry of one body (Grid.cpp:193-213) and its place in the batch.
 struct BodyGrid {

@@ -49,12 +49,15 @@ struct PinnedBuf {
   ~PinnedBuf() {
     if (p) (void)hipHostFree(p);
   }
+  // grows with headroom: hipHostFree waits for the device and a fresh hipHostMalloc costs
+  // hundreds of microseconds, while per-body host arrays grow a little most steps as bodies fragment
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
     if (p) HIPCHECK(hipHostFree(p));
     p = nullptr;
-    HIPCHECK(hipHostMalloc(&p, bytes));
-    cap = bytes;
+    size_t c = std::max<size_t>(bytes + bytes / 2, 1 << 16);
+    HIPCHECK(hipHostMalloc(&p, c));
+    cap = c;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };

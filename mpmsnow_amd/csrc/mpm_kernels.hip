@@ -75,17 +75,17 @@ __device__ __forceinline__ void project(const ColliderSet* cols, int flag, const
   v[2] -= nd * n[2];
 }
 __device__ __forceinline__ void load_x(const PView& P, int s, float x[3]) {
-  x[0] = P.at(C_X, s);
-  x[1] = P.at(C_X + 1, s);
-  x[2] = P.at(C_X + 2, s);
+  x[0] = P.atu(C_X, s);
+  x[1] = P.atu(C_X + 1, s);
+  x[2] = P.atu(C_X + 2, s);
 }
 __device__ __forceinline__ void load_m9(const PView& P, int col, int s, float m[9]) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) m[i] = P.at(col + i, s);
+  for (int i = 0; i < 9; ++i) m[i] = P.atu(col + i, s);
 }
 __device__ __forceinline__ void store_m9(const PView& P, int col, int s, const float m[9]) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) P.at(col + i, s) = m[i];
+  for (int i = 0; i < 9; ++i) P.atu(col + i, s) = m[i];
 }
 
 // ------------------------------------------------------------ chunk context
@@ -95,9 +95,9 @@ struct ChunkCtx {
   int ox, oy, oz;  // node origin of the chunk's tile
 };
 // all threads: the chunk, its body, and (threads 0..7) the footprint's global tiles
-__device__ __forceinline__ ChunkCtx chunk_begin(const Batch& B, int* ft, int chunk) {
+__device__ __forceinline__ ChunkCtx chunk_begin_at(const Batch& B, int* ft, int4 ch) {
   ChunkCtx c;
-  c.ch = B.chunks[chunk];
+  c.ch = ch;
   c.G = B.bodies[c.ch.w];
   int tx, ty, tz;
   tile_xyz(c.G, c.ch.x - c.G.tileBase, tx, ty, tz);
@@ -109,6 +109,9 @@ __device__ __forceinline__ ChunkCtx chunk_begin(const Batch& B, int* ft, int chu
     ft[f] = tile_id(c.G, tx + (f & 1), ty + ((f >> 1) & 1), tz + (f >> 2));
   }
   return c;
+}
+__device__ __forceinline__ ChunkCtx chunk_begin(const Batch& B, int* ft, int chunk) {
+  return chunk_begin_at(B, ft, B.chunks[chunk]);
 }
 __device__ __forceinline__ bool chunk_stencil(const ChunkCtx& c, const float x[3], bool derivs, Stencil& st, int& rx,
                                               int& ry, int& rz) {
@@ -217,7 +220,7 @@ __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc,
 // Single-word variant for the HVP, the solver's inner kernel: one integer per value and a
 // scale per component c from T_c, the chunk's sum of per-particle bounds on that component's
 // contributions (tight: the stencil's gradient weights are at most (2/3)^3 / h), so that sums
-// stay below 2^30; each contribution is rounded to nearest. Resolution 2^-30 of T_c: below the
+// stay below 2^30; each contribution is truncated toward zero. Resolution 2^-30 of T_c: below the
 // float rounding of the node sums it feeds, with half the LDS atomics and a fifth of the
 // conversion work of the split form.
 template <int NC> struct FxFoot1 {
@@ -301,8 +304,10 @@ __device__ __forceinline__ void sep_gather(const float* v, const Stencil& st, fl
     out[2] = fmaf(hz[k], st.dw[2][k], out[2]);
   }
 }
-// acc(i, j, k) += rint(f . ghat(i, j, k)) over the stencil, with f . ghat evaluated as
-// P(j, k) dwx_i + Q(j, k) wx_i, P = f0 wy_j wz_k, Q = f1 dwy_j wz_k + f2 wy_j dwz_k
+// acc(i, j, k) += trunc(f . ghat(i, j, k)) over the stencil, with f . ghat evaluated as
+// P(j, k) dwx_i + Q(j, k) wx_i, P = f0 wy_j wz_k, Q = f1 dwy_j wz_k + f2 wy_j dwz_k. Truncation
+// is one instruction (round-to-nearest is two); its error, under one fixed-point unit, is far
+// below the float rounding of the scaled contribution itself (2^-24 of values up to 2^29).
 __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f0, float f1, float f2) {
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -312,7 +317,7 @@ __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f
       const float Q = fmaf(f1, st.dw[1][j] * st.w[2][k], f2 * (st.w[1][j] * st.dw[2][k]));
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        atomicAdd(&acc[i + 8 * j + kFootSZ * k], __float2int_rn(fmaf(P, st.dw[0][i], Q * st.w[0][i])));
+        atomicAdd(&acc[i + 8 * j + kFootSZ * k], (int)fmaf(P, st.dw[0][i], Q * st.w[0][i]));
     }
 }
 
@@ -480,11 +485,28 @@ __device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, u
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < G.nblocks; b += kBlock)
+  // thread t adds the partials of blocks t, t + kBlock, ... in that order; U of them are loaded
+  // at a time so that the last block's wait is a few round trips, not one per partial. Past the
+  // end it adds +0.0, which leaves a sum that started at +0.0 unchanged.
+  constexpr int U = NV <= 2 ? 8 : (NV <= 4 ? 4 : 2);
+  for (int b0 = threadIdx.x; b0 < G.nblocks; b0 += U * kBlock) {
+    double pv[U][NV];
+    bool on[U];
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      acc[k] += __hip_atomic_load(partials + (int64_t)(G.blockBase + b) * kPartialStride + k, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * kBlock;
+      on[u] = b < G.nblocks;
+      const int64_t row = G.blockBase + (on[u] ? b : 0);
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        pv[u][k] =
+            __hip_atomic_load(partials + row * kPartialStride + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) acc[k] += on[u] ? pv[u][k] : 0.0;
+  }
   block_sum<NV>(acc, sm);
   __shared__ double res[NV];
   if (threadIdx.x == 0)
@@ -516,7 +538,7 @@ __global__ void k_get_column(PView P, int col0, int ncomp, float* aos, const int
   }
 }
 __global__ void k_fill_column(PView P, int col, float value, int n) {
-  GRID_STRIDE(s, n) P.at(col, (int)s) = value;
+  GRID_STRIDE(s, n) P.atu(col, (int)s) = value;
 }
 __global__ void k_fill_int(int* p, int value, int64_t n) {
   GRID_STRIDE(i, n) p[i] = value;
@@ -527,7 +549,7 @@ __global__ void k_iota(int* p, int64_t n) {
 __global__ void k_identity_columns(PView P, int col0, int n) {
   GRID_STRIDE(s, n) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) P.at(col0 + i, (int)s) = (i == 0 || i == 4 || i == 8) ? 1.0f : 0.0f;
+    for (int i = 0; i < 9; ++i) P.atu(col0 + i, (int)s) = (i == 0 || i == 4 || i == 8) ? 1.0f : 0.0f;
   }
 }
 __global__ void k_permute(PView src, PView dst, const int* new_to_old, int n, const int* orig_old, int* orig_new,
@@ -536,7 +558,7 @@ __global__ void k_permute(PView src, PView dst, const int* new_to_old, int n, co
     int s = (int)si;
     int o = new_to_old[s];
 #pragma unroll 4
-    for (int c = 0; c < NCOL; ++c) dst.at(c, s) = src.at(c, o);
+    for (int c = 0; c < NCOL; ++c) dst.atu(c, s) = src.atu(c, o);
     int orig = orig_old[o];
     orig_new[s] = orig;
     slot_of[orig] = s;
@@ -578,7 +600,7 @@ __global__ void k_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket
 __global__ void k_cc_scatter(PView P, const int* bucket, const int* rank, const int* offset, int n, float4* sorted) {
   GRID_STRIDE(si, n) {
     int s = (int)si;
-    sorted[offset[bucket[s]] + rank[s]] = make_float4(P.at(C_X, s), P.at(C_X + 1, s), P.at(C_X + 2, s), __int_as_float(s));
+    sorted[offset[bucket[s]] + rank[s]] = make_float4(P.atu(C_X, s), P.atu(C_X + 1, s), P.atu(C_X + 2, s), __int_as_float(s));
   }
 }
 __device__ __forceinline__ int cc_find(int* parent, int x) {
@@ -787,7 +809,7 @@ __global__ void k_check_pvel(PView P, int n, int* err) {
   int bad = 0;
   GRID_STRIDE(si, n) {
     int s = (int)si;
-    if (!isfinite(P.at(C_V, s) * P.at(C_V + 1, s) * P.at(C_V + 2, s))) bad = 1;
+    if (!isfinite(P.atu(C_V, s) * P.atu(C_V + 1, s) * P.atu(C_V + 2, s))) bad = 1;
   }
   if (bad) atomicOr(err, ERR_NAN_PVEL);
 }
@@ -839,12 +861,12 @@ __global__ void k_body_reduce(PView P, const int* slots, const int* labels, int 
       body_acc_reset(a);
       cb = b;
     }
-    float m = P.at(C_M, s);
+    float m = P.atu(C_M, s);
     for (int q = 0; q < 3; ++q) {
-      unsigned int e = ord_enc(P.at(C_X + q, s));
+      unsigned int e = ord_enc(P.atu(C_X + q, s));
       a.mn[q] = min(a.mn[q], e);
       a.mx[q] = max(a.mx[q], e);
-      a.m[q] += (double)(P.at(C_V + q, s) * m);
+      a.m[q] += (double)(P.atu(C_V + q, s) * m);
     }
     a.m[3] += (double)m;
     a.cnt += 1;
@@ -1111,7 +1133,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_mass(PView P, Batch B, float* st
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    m = P.at(C_M, s);
+    m = P.atu(C_M, s);
     if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       have = false;
@@ -1161,10 +1183,10 @@ __global__ __launch_bounds__(kBlock) void k_p2g_vel(PView P, Batch B, const floa
     int s = B.perm[c.ch.y + k];
     float x[3];
     load_x(P, s, x);
-    m = P.at(C_M, s);
-    dv[0] = P.at(C_V, s) - c.G.fv[0];
-    dv[1] = P.at(C_V + 1, s) - c.G.fv[1];
-    dv[2] = P.at(C_V + 2, s) - c.G.fv[2];
+    m = P.atu(C_M, s);
+    dv[0] = P.atu(C_V, s) - c.G.fv[0];
+    dv[1] = P.atu(C_V + 1, s) - c.G.fv[1];
+    dv[2] = P.atu(C_V + 2, s) - c.G.fv[2];
     if (!chunk_stencil(c, x, false, st, rx, ry, rz)) {
       atomicOr(err, ERR_STENCIL);
       have = false;
@@ -1237,7 +1259,7 @@ __global__ __launch_bounds__(kBlock) void k_volumes(PView P, Batch B, const floa
 #pragma unroll
         for (int cx = 0; cx < 4; ++cx)
           density += st_w(st, cx, cy, cz) * sm[foot_index(rx + cx, ry + cy, rz + cz)] / cellVolume;
-    P.at(C_VOL, s) = P.at(C_M, s) / density;
+    P.atu(C_VOL, s) = P.atu(C_M, s) / density;
   }
 }
 
@@ -1260,12 +1282,12 @@ __global__ __launch_bounds__(kBlock) void k_forces(PView P, Batch B, float* stag
     load_m9(P, C_F, s, F);
     load_m9(P, C_R, s, R);
     load_m9(P, C_FI, s, Fi);
-    snow_stress(F, R, Fi, P.at(C_J, s), P.at(C_MU, s), P.at(C_LAM, s), Pm);
+    snow_stress(F, R, Fi, P.atu(C_J, s), P.atu(C_MU, s), P.atu(C_LAM, s), Pm);
     float n2 = 0.0f;
 #pragma unroll
     for (int i = 0; i < 9; ++i) n2 += Pm[i] * Pm[i];
     if (!isfinite(sqrtf(n2))) atomicOr(err, ERR_NAN_STRESS);
-    float vol = P.at(C_VOL, s);
+    float vol = P.atu(C_VOL, s);
 #pragma unroll
     for (int i = 0; i < 9; ++i) Pm[i] = vol * Pm[i];
     mat_mul_bt(Pm, F, fm);
@@ -1426,17 +1448,33 @@ __global__ void k_collision_velocities(Batch B, const int8_t* flags, const Colli
 //   dF_p = (sum_j d_j grad w_jp^T) F_p ; df_i -= V_p dP(dF_p) F_p^T grad w_ip
 // One chunk's staged footprint of the HVP (LDS work areas passed in; every thread of the block calls).
 // particle part: footprint of d in sd (loaded), scatter into acc (zeroed); ends on a barrier
+// The slot and position of this thread's particle of chunk ch, loaded ahead of the footprint
+// phase so that their latency overlaps it (thread k holds particle k of the chunk, if any).
+struct HvpPre {
+  int s;
+  float x[3];
+};
+__device__ __forceinline__ HvpPre hvp_prefetch(const PView& P, const Batch& B, int4 ch) {
+  HvpPre q;
+  q.s = -1;
+  q.x[0] = q.x[1] = q.x[2] = 0.0f;
+  if ((int)threadIdx.x < ch.z) {
+    q.s = B.perm[ch.y + threadIdx.x];
+    load_x(P, q.s, q.x);
+  }
+  return q;
+}
 __device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B, const ChunkCtx& c,
-                                                  float (*sd)[kFootLds], FxFoot1<3>& acc, int* err) {
+                                                  float (*sd)[kFootLds], FxFoot1<3>& acc, int* err,
+                                                  const HvpPre& pre) {
   const int k = threadIdx.x;
   bool have = k < c.ch.z;
   float fm[9];
   Stencil st;
   int base = 0;
   if (have) {
-    int s = B.perm[c.ch.y + k];
-    float x[3];
-    load_x(P, s, x);
+    const int s = pre.s;
+    const float x[3] = {pre.x[0], pre.x[1], pre.x[2]};
     make_stencil_fast(x, c.G.gmin, c.G.h, 1.0f / c.G.h, st);
     const int rx = st.base[0] - c.ox, ry = st.base[1] - c.oy, rz = st.base[2] - c.oz;
     if (!(rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4)) {
@@ -1460,8 +1498,8 @@ __device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B
       load_m9(P, C_FI, s, Fi);
       load_m9(P, C_GI, s, Gi);
       mat_mul(G, F, dF);
-      snow_stress_differential(dF, R, Fi, Gi, P.at(C_J, s), P.at(C_MU, s), P.at(C_LAM, s), dP);
-      float vol = P.at(C_VOL, s);
+      snow_stress_differential(dF, R, Fi, Gi, P.atu(C_J, s), P.atu(C_MU, s), P.atu(C_LAM, s), dP);
+      float vol = P.atu(C_VOL, s);
 #pragma unroll
       for (int i = 0; i < 9; ++i) dP[i] = vol * dP[i];
       mat_mul_bt(dP, F, fm);
@@ -1483,26 +1521,39 @@ __device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B
   __syncthreads();
   return scl;
 }
+// ch = B.chunks[chunk], pre = hvp_prefetch(P, B, ch)
 template <bool PROJ>
-__device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
-                                          const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
-                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
-  ChunkCtx c = chunk_begin(B, ft, chunk);
+__device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, int4 ch, const HvpPre& pre,
+                                          const float* d, const int8_t* flags, const ColliderSet* cols, float* stage,
+                                          int* err, float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
+  ChunkCtx c = chunk_begin_at(B, ft, ch);
   fx_zero(acc);
   __syncthreads();
   load_foot3(c, ft, d, sd, flags, cols, PROJ);
   __syncthreads();
-  const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err);
+  const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err, pre);
   fx_store(acc, scl, stage, chunk);
 }
 template <bool PROJ>
+__device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
+                                          const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
+                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
+  const int4 ch = B.chunks[chunk];
+  hvp_chunk<PROJ>(P, B, chunk, ch, hvp_prefetch(P, B, ch), d, flags, cols, stage, err, sd, acc, ft);
+}
+// The chunk's particle slots and positions are requested first, beside the done flag of its
+// body: the chain of dependent loads before the stencil work is the chunk, then the slot, body
+// and flag together, then position and footprint together.
+template <bool PROJ>
 __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d, const int8_t* flags,
                                                 const ColliderSet* cols, float* stage, const CRScalars* sc, int* err) {
-  if (sc && sc[B.chunks[blockIdx.x].w].done) return;
   __shared__ float sd[3][kFootLds];
   __shared__ FxFoot1<3> acc;
   __shared__ int ft[8];
-  hvp_chunk<PROJ>(P, B, blockIdx.x, d, flags, cols, stage, err, sd, acc, ft);
+  const int4 ch = B.chunks[blockIdx.x];
+  const HvpPre pre = hvp_prefetch(P, B, ch);
+  if (sc && sc[ch.w].done) return;
+  hvp_chunk<PROJ>(P, B, blockIdx.x, ch, pre, d, flags, cols, stage, err, sd, acc, ft);
 }
 
 // dFidXiSplatter (Grid.cpp:466-537) through per-particle quadratic forms: with g = F^T grad w,
@@ -1528,7 +1579,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(PView P, Batch B, float* stage,
     load_m9(P, C_R, s, R);
     load_m9(P, C_FI, s, Fi);
     load_m9(P, C_GI, s, Gi);
-    float J = P.at(C_J, s), mu = P.at(C_MU, s), lam = P.at(C_LAM, s), vol = P.at(C_VOL, s);
+    float J = P.atu(C_J, s), mu = P.atu(C_MU, s), lam = P.atu(C_LAM, s), vol = P.atu(C_VOL, s);
     float hs = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -1963,7 +2014,7 @@ __device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const B
     fx_zero(acc);
     load_foot3(c, T.ft[q], T.z, sd, T.flags, cols, true, G.tileBase);
     __syncthreads();
-    const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err);
+    const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err, hvp_prefetch(P, B, c.ch));
     fx_store(acc, scl, T.stage, q);
   }
   __syncthreads();
@@ -2542,9 +2593,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
     }
     float vf[3] = {0.0f, 0.0f, 0.0f}, vp[3] = {0.0f, 0.0f, 0.0f};
     if (DO_V) {
-      vf[0] = P.at(C_V, s);
-      vf[1] = P.at(C_V + 1, s);
-      vf[2] = P.at(C_V + 2, s);
+      vf[0] = P.atu(C_V, s);
+      vf[1] = P.atu(C_V + 1, s);
+      vf[2] = P.atu(C_V + 2, s);
     }
     float dV[9];
 #pragma unroll
@@ -2581,7 +2632,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
 #pragma unroll
       for (int a = 0; a < 3; ++a) v[a] = alpha * vf[a] + beta * vp[a];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) P.at(C_V + a, s) = v[a];
+      for (int a = 0; a < 3; ++a) P.atu(C_V + a, s) = v[a];
     }
     if (DO_F) {
       float F[9], A[9], Fn[9];
@@ -2594,16 +2645,16 @@ __global__ __launch_bounds__(kBlock) void k_g2p(PView P, Batch B, const float* v
       } else {
         float Fp[9], Fi[9], R[9], Gi[9], J, mu, lam;
         load_m9(P, C_FP, s, Fp);
-        bool ok = snow_update(Fn, Fp, Fi, R, Gi, J, mu, lam, P.at(C_MU0, s), P.at(C_LAM0, s), P.at(C_XI, s), thc, ths);
+        bool ok = snow_update(Fn, Fp, Fi, R, Gi, J, mu, lam, P.atu(C_MU0, s), P.atu(C_LAM0, s), P.atu(C_XI, s), thc, ths);
         if (!ok) atomicOr(err, ERR_HARDNESS);
         store_m9(P, C_F, s, Fn);
         store_m9(P, C_FP, s, Fp);
         store_m9(P, C_FI, s, Fi);
         store_m9(P, C_R, s, R);
         store_m9(P, C_GI, s, Gi);
-        P.at(C_J, s) = J;
-        P.at(C_MU, s) = mu;
-        P.at(C_LAM, s) = lam;
+        P.atu(C_J, s) = J;
+        P.atu(C_MU, s) = mu;
+        P.atu(C_LAM, s) = lam;
       }
     }
   }
@@ -2616,16 +2667,16 @@ __global__ void k_plasticity(PView P, const int* labels, int skipLabel, int n, f
     float F[9], Fp[9], Fi[9], R[9], Gi[9], J, mu, lam;
     load_m9(P, C_F, s, F);
     load_m9(P, C_FP, s, Fp);
-    bool ok = snow_update(F, Fp, Fi, R, Gi, J, mu, lam, P.at(C_MU0, s), P.at(C_LAM0, s), P.at(C_XI, s), thc, ths);
+    bool ok = snow_update(F, Fp, Fi, R, Gi, J, mu, lam, P.atu(C_MU0, s), P.atu(C_LAM0, s), P.atu(C_XI, s), thc, ths);
     if (!ok) atomicOr(err, ERR_HARDNESS);
     store_m9(P, C_F, s, F);
     store_m9(P, C_FP, s, Fp);
     store_m9(P, C_FI, s, Fi);
     store_m9(P, C_R, s, R);
     store_m9(P, C_GI, s, Gi);
-    P.at(C_J, s) = J;
-    P.at(C_MU, s) = mu;
-    P.at(C_LAM, s) = lam;
+    P.atu(C_J, s) = J;
+    P.atu(C_MU, s) = mu;
+    P.atu(C_LAM, s) = lam;
   }
 }
 
@@ -2638,9 +2689,9 @@ __global__ void k_ballistic(PView P, const int* labels, int n, const float* grav
     float v[3], x[3];
     load_x(P, s, x);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) v[q] = P.at(C_V + q, s);
+    for (int q = 0; q < 3; ++q) v[q] = P.atu(C_V + q, s);
     if (mode & 1) {
-      float m = P.at(C_M, s);
+      float m = P.atu(C_M, s);
       float a[3] = {0.0f, 0.0f, 0.0f};
       field_force(grav, ngrav, m, a);
 #pragma unroll
@@ -2653,10 +2704,10 @@ __global__ void k_ballistic(PView P, const int* labels, int n, const float* grav
       const float zero[3] = {0.0f, 0.0f, 0.0f};
       collide(*cols, v, x, zero, true);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) P.at(C_X + q, s) = x[q] + v[q] * dt;
+      for (int q = 0; q < 3; ++q) P.atu(C_X + q, s) = x[q] + v[q] * dt;
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) P.at(C_V + q, s) = v[q];
+    for (int q = 0; q < 3; ++q) P.atu(C_V + q, s) = v[q];
   }
 }
 __global__ void k_advect(PView P, const int* labels, int n, float dt, const ColliderSet* cols) {
@@ -2666,13 +2717,13 @@ __global__ void k_advect(PView P, const int* labels, int n, float dt, const Coll
     float v[3], x[3];
     load_x(P, s, x);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) v[q] = P.at(C_V + q, s);
+    for (int q = 0; q < 3; ++q) v[q] = P.atu(C_V + q, s);
     const float zero[3] = {0.0f, 0.0f, 0.0f};
     collide(*cols, v, x, zero, true);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      P.at(C_V + q, s) = v[q];
-      P.at(C_X + q, s) = x[q] + v[q] * dt;
+      P.atu(C_V + q, s) = v[q];
+      P.atu(C_X + q, s) = x[q] + v[q] * dt;
     }
   }
 }
@@ -2684,7 +2735,7 @@ __global__ void k_probe(PView P, const int* slots, int n, int mode, const float*
     load_m9(P, C_R, s, R);
     load_m9(P, C_FI, s, Fi);
     load_m9(P, C_GI, s, Gi);
-    float J = P.at(C_J, s), mu = P.at(C_MU, s), lam = P.at(C_LAM, s);
+    float J = P.atu(C_J, s), mu = P.atu(C_MU, s), lam = P.atu(C_LAM, s);
     if (mode == 0) {
       out[k] = snow_energy(F, R, J, mu, lam);
     } else if (mode == 1) {

@@ -230,6 +230,8 @@ struct mpm_context {
     hipStream_t stream;
   };
   unsigned int profMask = 0;  // kernel classes being timed
+  int profPeriod = 1;         // time one launch in profPeriod of each class
+  int64_t profSeen[PK_COUNT] = {};
   std::vector<ProfRec> profPending;
   std::vector<hipEvent_t> eventPool;
   double profMs[PK_COUNT] = {};
@@ -279,6 +281,7 @@ struct mpm_context {
   }
   int prof_begin(int kind, const GridBatch* g, int iter = kProfOnce, hipStream_t on = nullptr) {
     if (!(profMask & (1u << kind))) return -1;
+    if (profPeriod > 1 && profSeen[kind]++ % profPeriod != 0) return -1;
     ProfRec r{kind, iter, new_event(), new_event(), g, on ? on : stream};
     HIPCHECK(hipEventRecord(r.a, r.stream));
     profPending.push_back(r);
@@ -291,28 +294,74 @@ struct mpm_context {
   void prof_resolve() {
     if (profPending.empty()) return;
     sync();
+    // per batch: bodies of the batched loop by iteration count, with suffix sums of their
+    // particles and active nodes, so that a CR-loop launch's bytes cost a binary search
+    struct LoopSums {
+      std::vector<int> its;              // ascending
+      std::vector<double> np, active;    // sums over bodies k.. of the ascending order
+      double npAll = 0, activeAll = 0;   // every body (kProfOnce)
+      double npSetup = 0, activeSetup = 0, npSmall = 0, activeSmall = 0;  // batched / small bodies
+    };
+    std::vector<std::pair<const GridBatch*, LoopSums>> sums;
+    auto sums_of = [&](const GridBatch* g) -> const LoopSums& {
+      for (auto& e : sums)
+        if (e.first == g) return e.second;
+      LoopSums L;
+      std::vector<std::pair<int, int>> order;  // (iterations, body)
+      for (int bi = 0; bi < (int)g->rec.size(); ++bi) {
+        const SolveRecord& r = g->rec[bi];
+        L.npAll += r.np;
+        L.activeAll += r.active;
+        if (r.small) {
+          L.npSmall += (double)r.np * (r.iterations + 2);
+          L.activeSmall += (double)r.active * (r.iterations + 2);
+        } else {
+          L.npSetup += r.np;
+          L.activeSetup += r.active;
+          order.push_back({r.iterations, bi});
+        }
+      }
+      std::sort(order.begin(), order.end());
+      const size_t m = order.size();
+      L.its.resize(m);
+      L.np.assign(m + 1, 0.0);
+      L.active.assign(m + 1, 0.0);
+      for (size_t k = m; k-- > 0;) {
+        const SolveRecord& r = g->rec[order[k].second];
+        L.its[k] = order[k].first;
+        L.np[k] = L.np[k + 1] + r.np;
+        L.active[k] = L.active[k + 1] + r.active;
+      }
+      sums.push_back({g, std::move(L)});
+      return sums.back().second;
+    };
     for (auto& r : profPending) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-        double bytes = 0;
+        double np = 0, active = 0;
         if (r.batch) {
-          for (const SolveRecord& s : r.batch->rec) {
-            // a body takes part in CR-loop launch `iter` if its solve had not finished
-            double times = 0;
-            if (r.iter == kProfOnce)
-              times = 1;
-            else if (r.iter == kProfSmall)
-              times = s.small ? s.iterations + 2 : 0;
-            else if (r.iter == kProfSetup)
-              times = s.small ? 0 : 1;
-            else if (!s.small)
-              times = (r.kind == PK_CR_X ? r.iter < s.iterations : r.iter + 1 < s.iterations) ? 1 : 0;
-            bytes += times * (kProfBytesPerParticle[r.kind] * s.np + kProfBytesPerNode[r.kind] * s.active);
+          const LoopSums& L = sums_of(r.batch);
+          if (r.iter == kProfOnce) {
+            np = L.npAll;
+            active = L.activeAll;
+          } else if (r.iter == kProfSmall) {
+            np = L.npSmall;
+            active = L.activeSmall;
+          } else if (r.iter == kProfSetup) {
+            np = L.npSetup;
+            active = L.activeSetup;
+          } else {
+            // a batched body takes part in CR-loop launch `iter` if its solve had not finished:
+            // iter < iterations for the x update, iter + 1 < iterations for the others
+            const int lim = r.kind == PK_CR_X ? r.iter : r.iter + 1;
+            const size_t k = std::upper_bound(L.its.begin(), L.its.end(), lim) - L.its.begin();
+            np = L.np[k];
+            active = L.active[k];
           }
         }
         profMs[r.kind] += ms;
         profCount[r.kind] += 1;
-        profBytes[r.kind] += bytes;
+        profBytes[r.kind] += kProfBytesPerParticle[r.kind] * np + kProfBytesPerNode[r.kind] * active;
       }
       eventPool.push_back(r.a);
       eventPool.push_back(r.b);
@@ -1632,7 +1681,17 @@ int mpm_profile_enable(mpm_context* ctx, int mask) {
       ctx->profMs[k] = 0;
       ctx->profCount[k] = 0;
       ctx->profBytes[k] = 0;
+      ctx->profSeen[k] = 0;
     }
+  });
+}
+
+int mpm_profile_period(mpm_context* ctx, int period) {
+  return guarded(ctx, [&] {
+    if (period < 1) throw InvalidArg("profile period must be >= 1");
+    ctx->prof_resolve();
+    ctx->profPeriod = period;
+    for (int k = 0; k < PK_COUNT; ++k) ctx->profSeen[k] = 0;
   });
 }
 
