@@ -208,8 +208,9 @@ int mpm_set_small_bodies(mpm_context* ctx, int max_particles);
 
 /* ---- measurement (no counterpart in the reference) ----------------------------
  * Kernel timing with HIP events on the context's streams. Kernel classes, in bit
- * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_p lpc_forward
- * cr_small (the small-body solver: one launch per solve, all its iterations).
+ * order: p2g_mass p2g_vel forces hvp diag g2p cr_update_x cr_ar cr_update_px lpc_forward
+ * cr_small (cr_update_x: a solve's first x update; cr_update_px: each later iteration's
+ * p and x updates in one pass; cr_small: the small-body solver, one launch per solve).
  * mask selects the classes to time (bit k = class k; -1 = all, 0 = off); each
  * timed launch costs two event records, so time only what is needed.
  * algorithmic_bytes sums, over the timed launches, bytes/particle * particles +

@@ -114,7 +114,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_double)]),
 }
 
-PROFILE_KERNELS = ("p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p", "cr_update_x", "cr_ar", "cr_update_p",
+PROFILE_KERNELS = ("p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p", "cr_update_x", "cr_ar", "cr_update_px",
                    "lpc_forward", "cr_small")
 
 

@@ -104,15 +104,21 @@ struct Batch {
   const unsigned char* tileActive;  // per tile: some node has mass or a collider flag
 };
 
-// Per-body reductions inside node kernels: every block stores its partial sums, then counts
-// in on one of kSub sub-tickets; the last block of each sub-ticket counts in on the body's
-// main ticket, and the last of those combines the partials in block order. Tickets sit on
-// separate 128-byte lines; spreading the arrivals keeps same-address atomics short.
+// Per-body reductions inside node kernels: the body's blocks form kSub sub-groups (local
+// block index mod kSub). Every block stores its partial sums and counts in on its sub-group's
+// ticket; the last block of a sub-group adds the sub-group's partials in member order, stores
+// that total and counts in on the body's main ticket; the last of those adds the sub-group
+// totals in sub-group order. The last block's wait is then two short reads instead of one over
+// every block of a large body. Tickets sit on separate 128-byte lines; spreading the arrivals
+// keeps same-address atomics short. Per body and set: kSetTickets unsigned ints of tickets,
+// then kSub sub-group totals of kPartialStride doubles.
 constexpr int kSub = 32;
 constexpr int kTicketStride = 32;
 constexpr int kTicketSets = 3;
-constexpr int kTicketsPerBody = kTicketSets * (kSub + 1) * kTicketStride;
 constexpr int kPartialStride = 16;  // doubles per block in the partials buffer
+constexpr int kSetTickets = (kSub + 1) * kTicketStride;
+constexpr int kSetStride = kSetTickets + kSub * kPartialStride * 2;  // unsigned ints
+constexpr int kTicketsPerBody = kTicketSets * kSetStride;
 
 // error bits raised by kernels (reference exception messages are mapped by the host)
 enum ErrorBits : int {

@@ -444,63 +444,28 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], dou
     for (int k = 0; k < NV; ++k) v[k] = ((sm[0][k] + sm[1][k]) + sm[2][k]) + sm[3][k];
 }
 __device__ __forceinline__ unsigned int* body_tickets(const Batch& B, int body, int set) {
-  return B.tickets + (int64_t)body * kTicketsPerBody + (int64_t)set * (kSub + 1) * kTicketStride;
+  return B.tickets + (int64_t)body * kTicketsPerBody + (int64_t)set * kSetStride;
 }
-// Writes this block's partials; returns true in every thread of the last block of the body
-// to finish, with out[] = the fixed-order total over the body's blocks. The partials are
-// written with agent-scope atomic stores and the block waits for their completion before it
-// takes a ticket: a release fence would write back the whole L2 of this XCD from every block.
+// Adds, in thread t, the NV-vectors rows[t], rows[t + kBlock], ... (row r at src + r * stride,
+// n rows) in that order, loading U rows at a time, then sums over the block (block_sum); the
+// total is in thread 0. Past the end it adds +0.0, which leaves a sum that started at +0.0
+// unchanged.
 template <int NV>
-__device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, unsigned int* tk,
-                         double (&out)[NV]) {
-  static_assert(NV <= kPartialStride, "partials stride");
-  __shared__ double sm[kBlock / 64][NV];
-  __shared__ bool amLast;
-  block_sum<NV>(v, sm);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(partials + (int64_t)blockIdx.x * kPartialStride + k, v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const int local = blockIdx.x - G.blockBase;
-    const int sub = local % kSub;
-    const unsigned int arrivals = (unsigned int)((G.nblocks - sub + kSub - 1) / kSub);
-    const unsigned int nsubs = (unsigned int)(G.nblocks < kSub ? G.nblocks : kSub);
-    unsigned int* st = tk + (1 + sub) * kTicketStride;
-    bool last = false;
-    if (__hip_atomic_fetch_add(st, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == arrivals - 1u) {
-      __hip_atomic_store(st, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (__hip_atomic_fetch_add(tk, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nsubs - 1u) {
-        __hip_atomic_store(tk, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = true;
-      }
-    }
-    amLast = last;
-  }
-  __syncthreads();
-  if (!amLast) return false;
-  double acc[NV];
+__device__ __forceinline__ void rows_sum(const double* src, int64_t stride, int n, double (&acc)[NV],
+                                         double (*sm)[NV]) {
+  constexpr int U = NV <= 2 ? 8 : (NV <= 4 ? 4 : 2);
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  // thread t adds the partials of blocks t, t + kBlock, ... in that order; U of them are loaded
-  // at a time so that the last block's wait is a few round trips, not one per partial. Past the
-  // end it adds +0.0, which leaves a sum that started at +0.0 unchanged.
-  constexpr int U = NV <= 2 ? 8 : (NV <= 4 ? 4 : 2);
-  for (int b0 = threadIdx.x; b0 < G.nblocks; b0 += U * kBlock) {
+  for (int r0 = threadIdx.x; r0 < n; r0 += U * kBlock) {
     double pv[U][NV];
     bool on[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * kBlock;
-      on[u] = b < G.nblocks;
-      const int64_t row = G.blockBase + (on[u] ? b : 0);
+      const int r = r0 + u * kBlock;
+      on[u] = r < n;
+      const double* row = src + (int64_t)(on[u] ? r : 0) * stride;
 #pragma unroll
-      for (int k = 0; k < NV; ++k)
-        pv[u][k] =
-            __hip_atomic_load(partials + row * kPartialStride + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int k = 0; k < NV; ++k) pv[u][k] = __hip_atomic_load(row + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -508,6 +473,52 @@ __device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, u
       for (int k = 0; k < NV; ++k) acc[k] += on[u] ? pv[u][k] : 0.0;
   }
   block_sum<NV>(acc, sm);
+}
+// Stores v (thread 0) at dst with agent-scope atomic stores, waits for their completion, then
+// takes a ticket: true (thread 0) for the arrival that completes `arrivals` (the counter is
+// reset for the next use). A release fence would write back the whole L2 of this XCD instead.
+template <int NV>
+__device__ __forceinline__ bool store_and_count(const double (&v)[NV], double* dst, unsigned int* ticket,
+                                                unsigned int arrivals) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) __hip_atomic_store(dst + k, v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  if (__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == arrivals - 1u) {
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  return false;
+}
+// Returns true in every thread of the last block of the body to finish, with out[] = the
+// body's total: sub-group totals (members in block order) added in sub-group order. For a
+// body of at most kSub blocks every sub-group is one block and this is the block-order sum.
+template <int NV>
+__device__ bool body_sum(double (&v)[NV], const BodyGrid& G, double* partials, unsigned int* tk,
+                         double (&out)[NV]) {
+  static_assert(NV <= kPartialStride, "partials stride");
+  __shared__ double sm[kBlock / 64][NV];
+  __shared__ bool amLast;
+  double* subTot = reinterpret_cast<double*>(tk + kSetTickets);
+  block_sum<NV>(v, sm);
+  const int local = blockIdx.x - G.blockBase;
+  const int sub = local % kSub;
+  const int members = (G.nblocks - sub + kSub - 1) / kSub;
+  const int nsubs = G.nblocks < kSub ? G.nblocks : kSub;
+  if (threadIdx.x == 0)
+    amLast = store_and_count<NV>(v, partials + (int64_t)blockIdx.x * kPartialStride, tk + (1 + sub) * kTicketStride,
+                                 (unsigned int)members);
+  __syncthreads();
+  if (!amLast) return false;
+  double acc[NV];
+  rows_sum<NV>(partials + (int64_t)(G.blockBase + sub) * kPartialStride, (int64_t)kSub * kPartialStride, members,
+               acc, sm);
+  __syncthreads();
+  if (threadIdx.x == 0) amLast = store_and_count<NV>(acc, subTot + sub * kPartialStride, tk, (unsigned int)nsubs);
+  __syncthreads();
+  if (!amLast) return false;
+  rows_sum<NV>(subTot, kPartialStride, nsubs, acc, sm);
   __shared__ double res[NV];
   if (threadIdx.x == 0)
 #pragma unroll
@@ -1813,89 +1824,109 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
     if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
 }
-// iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta
+// iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta. The
+// next iteration's denominator <Ap', M^-1 Ap'> with Ap' = Ar + beta Ap (:147-151, :86-102) is
+// taken from the same pass by expanding it: <Ar, M^-1 Ar> + 2 beta <Ar, M^-1 Ap> + beta^2
+// <Ap, M^-1 Ap>, the last term being the current denominator. The p / Ap update then runs in
+// the next x update's pass (k_cr_update_px) instead of a pass of its own.
 __device__ __forceinline__ void cr_ar_core(const NodeCtx& c, const ColliderSet* cols, int fl, float m,
-                                           const float zz[3], const float df[3], float dt2, float ar[3],
-                                           double (&s)[1]) {
+                                           const float zz[3], const float df[3], const float ap[3],
+                                           const float dg[3], float dt2, float ar[3], double (&s)[3]) {
   apply_A(c, cols, fl, m, zz, df, dt2, ar);
   s[0] = (double)zz[0] * ar[0] + (double)zz[1] * ar[1] + (double)zz[2] * ar[2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    s[1] += (double)ar[a] * (ar[a] / dg[a]);
+    s[2] += (double)ar[a] * (ap[a] / dg[a]);
+  }
 }
 __device__ __forceinline__ void cr_ar_node(const Batch& B, const NodeCtx& c, const float* mass, const int8_t* flags,
-                                           const ColliderSet* cols, const float* stage, const float* z, float dt2,
-                                           float* Ar, double (&s)[1]) {
-  float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, ar[3];
+                                           const ColliderSet* cols, const float* stage, const float* z,
+                                           const float* Ap, const float* diag, float dt2, float* Ar,
+                                           double (&s)[3]) {
+  float zz[3], df[3] = {0.0f, 0.0f, 0.0f}, ap[3], dg[3], ar[3];
   ld3(z, c.i, zz);
+  ld3(Ap, c.i, ap);
+  ld3(diag, c.i, dg);
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
-  cr_ar_core(c, cols, flags[c.i], mass[c.i], zz, df, dt2, ar, s);
+  cr_ar_core(c, cols, flags[c.i], mass[c.i], zz, df, ap, dg, dt2, ar, s);
   st3(Ar, c.i, ar);
 }
-__device__ __forceinline__ bool cr_ar_finish(CRScalars& S, const double (&tot)[1]) {
+// beta, and alpha for the next x update; returns true when the solve ends here (rz of the
+// previous iteration or the next denominator is zero: ConjugateResiduals.cpp:86-102, :140-145)
+__device__ __forceinline__ bool cr_ar_finish(CRScalars& S, const double (&tot)[3]) {
   double rz_old = S.rz;
   S.rz = tot[0];
   if (rz_old == 0.0) {
     S.early_exit = 1;
     return true;
   }
-  S.beta = tot[0] / rz_old;
+  const double beta = tot[0] / rz_old;
+  S.beta = beta;
+  const double den = tot[1] + 2.0 * beta * tot[2] + beta * beta * S.ApdPAp;
+  S.ApdPAp = den;
+  S.iter = S.iter + 1;
+  if (den == 0.0) {
+    S.early_exit = 1;
+    return true;
+  }
+  S.alpha = S.rz / den;
   return false;
 }
 __global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
-                        const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* bst, double* partials) {
+                        const float* z, const float* Ap, const float* diag, float dt, float* Ar, CRScalars* sc,
+                        BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
-  double s[1] = {0.0};
+  double s[3] = {0.0, 0.0, 0.0};
   if (!B.tileActive || B.tileActive[c.T])
-    cr_ar_node(B, c, mass, flags, cols, stage, z, dt * dt, Ar, s);
-  double tot[1];
-  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0)
+    cr_ar_node(B, c, mass, flags, cols, stage, z, Ap, diag, dt * dt, Ar, s);
+  double tot[3];
+  if (body_sum<3>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0)
     if (cr_ar_finish(S, tot)) set_done(S, bst);
 }
-// iteration part 3 (ConjugateResiduals.cpp:147-151 and the next alpha, :86-102)
-__device__ __forceinline__ void cr_update_p_core(const float zz[3], const float ar[3], const float dg[3], float beta,
-                                                 float pv[3], float ap[3], double (&s)[1]) {
+// iteration part 3 of the previous iteration (ConjugateResiduals.cpp:147-151: p = z + beta p,
+// Ap = Ar + beta Ap) and part 1 of this one (x update, termination), in one pass
+__device__ __forceinline__ void cr_update_px_core(const NodeCtx& c, const ColliderSet* cols, int fl, float beta,
+                                                  float alpha, float xv[3], float pv[3], float ap[3],
+                                                  const float ar[3], const float dg[3], float zz[3],
+                                                  double (&s)[1]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     pv[a] = zz[a] + beta * pv[a];
     ap[a] = ar[a] + beta * ap[a];
-    s[0] += (double)ap[a] * (ap[a] / dg[a]);
   }
+  cr_update_x_core(c, cols, fl, alpha, xv, pv, ap, dg, zz, s);
 }
-__device__ __forceinline__ void cr_update_p_node(const NodeCtx& c, const float* z, const float* Ar, const float* diag,
-                                                 float beta, float* p, float* Ap, double (&s)[1]) {
-  float zz[3], pv[3], ar[3], ap[3], dg[3];
-  ld3(z, c.i, zz);
+__device__ __forceinline__ void cr_update_px_node(const NodeCtx& c, const int8_t* flags, const ColliderSet* cols,
+                                                  const float* Ar, const float* diag, float beta, float alpha,
+                                                  float* p, float* Ap, float* x, float* z, double (&s)[1]) {
+  float xv[3], pv[3], ap[3], ar[3], dg[3], zz[3];
+  ld3(x, c.i, xv);
   ld3(p, c.i, pv);
-  ld3(Ar, c.i, ar);
   ld3(Ap, c.i, ap);
+  ld3(Ar, c.i, ar);
   ld3(diag, c.i, dg);
-  cr_update_p_core(zz, ar, dg, beta, pv, ap, s);
+  ld3(z, c.i, zz);
+  cr_update_px_core(c, cols, flags[c.i], beta, alpha, xv, pv, ap, ar, dg, zz, s);
   st3(p, c.i, pv);
   st3(Ap, c.i, ap);
+  st3(x, c.i, xv);
+  st3(z, c.i, zz);
 }
-__device__ __forceinline__ bool cr_update_p_finish(CRScalars& S, const double (&tot)[1]) {
-  bool end = false;
-  S.ApdPAp = tot[0];
-  if (tot[0] == 0.0) {
-    S.early_exit = 1;
-    end = true;
-  } else {
-    S.alpha = S.rz / tot[0];
-  }
-  S.iter = S.iter + 1;
-  return end;
-}
-__global__ void k_cr_update_p(Batch B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                              CRScalars* sc, BatchStatus* bst, double* partials) {
+__global__ void k_cr_update_px(Batch B, const int8_t* flags, const ColliderSet* cols, const float* Ar,
+                               const float* diag, float* p, float* Ap, float* x, float* z, CRScalars* sc,
+                               BatchStatus* bst, double* partials, double* hist) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
   double s[1] = {0.0};
   if (!B.tileActive || B.tileActive[c.T])
-    cr_update_p_node(c, z, Ar, diag, (float)S.beta, p, Ap, s);
+    cr_update_px_node(c, flags, cols, Ar, diag, (float)S.beta, (float)S.alpha, p, Ap, x, z, s);
   double tot[1];
-  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 2), tot) && threadIdx.x == 0)
-    if (cr_update_p_finish(S, tot)) set_done(S, bst);
+  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
+    if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
 }
 
 // ------------------------------------------------------------ small bodies
@@ -2030,7 +2061,8 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
                                            int nch, const float* mass, const int8_t* flags, const ColliderSet* cols,
                                            float dt, const float* bvec, const float* diag, float* x, double* hist,
                                            int* err, TinyLds& T, float (*sd)[kFootLds], FxFoot1<3>& acc,
-                                           SmallRed<2>& R2, SmallRed<1>& R1, CRScalars& L, int& end) {
+                                           SmallRed<2>& R2, SmallRed<1>& R1, SmallRed<3>& R3, CRScalars& L,
+                                           int& end) {
   const float dt2 = dt * dt;
   // chunk table, footprints and per-tile chunk ranges of the body
   if (threadIdx.x < nch * 8) {
@@ -2090,30 +2122,31 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
   }, t2);
   if (threadIdx.x == 0) end = cr_setup2_finish(L, t2);
   __syncthreads();
-  double t1[1];
-  while (!end) {
+  double t1[1], t3[3];
+  for (int it = 0; !end; ++it) {
     const float alpha = (float)L.alpha;
-    tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
-      cr_update_x_core(cn[j], cols, flr[j], alpha, xr[j], pr[j], apr[j], dgr[j], zr[j], v);
-      tiny_st3(T, G, cn[j], zr[j]);
-    }, t1);
+    if (it == 0) {
+      tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+        cr_update_x_core(cn[j], cols, flr[j], alpha, xr[j], pr[j], apr[j], dgr[j], zr[j], v);
+        tiny_st3(T, G, cn[j], zr[j]);
+      }, t1);
+    } else {
+      const float beta = (float)L.beta;
+      tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+        cr_update_px_core(cn[j], cols, flr[j], beta, alpha, xr[j], pr[j], apr[j], arr[j], dgr[j], zr[j], v);
+        tiny_st3(T, G, cn[j], zr[j]);
+      }, t1);
+    }
     if (threadIdx.x == 0) end = cr_update_x_finish(L, t1, body, hist);
     __syncthreads();
     if (end) break;
     tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
-    tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+    tiny_sweep<3>(G, R3, [&](int j, double (&v)[3]) {
       float df[3] = {0.0f, 0.0f, 0.0f};
       if (cn[j].valid) gather3_tab(G, T.tc0, T.tcn, G.tileBase, T.stage, cn[j].T, cn[j].l, df);
-      cr_ar_core(cn[j], cols, flr[j], mr[j], zr[j], df, dt2, arr[j], v);
-    }, t1);
-    if (threadIdx.x == 0) end = cr_ar_finish(L, t1);
-    __syncthreads();
-    if (end) break;
-    const float beta = (float)L.beta;
-    tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
-      cr_update_p_core(zr[j], arr[j], dgr[j], beta, pr[j], apr[j], v);
-    }, t1);
-    if (threadIdx.x == 0) end = cr_update_p_finish(L, t1);
+      cr_ar_core(cn[j], cols, flr[j], mr[j], zr[j], df, apr[j], dgr[j], dt2, arr[j], v);
+    }, t3);
+    if (threadIdx.x == 0) end = cr_ar_finish(L, t3);
     __syncthreads();
   }
 #pragma unroll
@@ -2141,6 +2174,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   __shared__ int ft[8];
   __shared__ SmallRed<2> R2;
   __shared__ SmallRed<1> R1;
+  __shared__ SmallRed<3> R3;
   __shared__ CRScalars L;
   __shared__ int end;
   __shared__ TinyLds T;
@@ -2161,8 +2195,8 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   const int c1 = B.tileChunk0[lastTile] + B.tileChunkN[lastTile];
   if (G.nblocks <= kTinyBlocks && c1 - c0 <= kTinyChunks) {
     __syncthreads();
-    tiny_solve(P, B, body, G, c0, c1 - c0, mass, flags, cols, dt, bvec, diag, x, hist, err, T, sd, acc, R2, R1, L,
-               end);
+    tiny_solve(P, B, body, G, c0, c1 - c0, mass, flags, cols, dt, bvec, diag, x, hist, err, T, sd, acc, R2, R1, R3,
+               L, end);
     if (threadIdx.x == 0) {
       L.done = 1;
       sc[body] = L;
@@ -2189,27 +2223,27 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   }, t2);
   if (threadIdx.x == 0) end = cr_setup2_finish(L, t2);
   __syncthreads();
-  double t1[1];
-  while (!end) {
+  double t1[1], t3[3];
+  for (int it = 0; !end; ++it) {
     const float alpha = (float)L.alpha;
-    small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
-      cr_update_x_node(c, flags, cols, p, Ap, diag, alpha, x, z, v);
-    }, t1);
+    if (it == 0) {
+      small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+        cr_update_x_node(c, flags, cols, p, Ap, diag, alpha, x, z, v);
+      }, t1);
+    } else {
+      const float beta = (float)L.beta;
+      small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+        cr_update_px_node(c, flags, cols, Ar, diag, beta, alpha, p, Ap, x, z, v);
+      }, t1);
+    }
     if (threadIdx.x == 0) end = cr_update_x_finish(L, t1, body, hist);
     __syncthreads();
     if (end) break;
     hvp(z);
-    small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
-      cr_ar_node(B, c, mass, flags, cols, stage, z, dt2, Ar, v);
-    }, t1);
-    if (threadIdx.x == 0) end = cr_ar_finish(L, t1);
-    __syncthreads();
-    if (end) break;
-    const float beta = (float)L.beta;
-    small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
-      cr_update_p_node(c, z, Ar, diag, beta, p, Ap, v);
-    }, t1);
-    if (threadIdx.x == 0) end = cr_update_p_finish(L, t1);
+    small_sweep<3>(G, body, R3, [&](const NodeCtx& c, double (&v)[3]) {
+      cr_ar_node(B, c, mass, flags, cols, stage, z, Ap, diag, dt2, Ar, v);
+    }, t3);
+    if (threadIdx.x == 0) end = cr_ar_finish(L, t3);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -2950,13 +2984,14 @@ void launch_cr_update_x(const Batch& B, const int8_t* flags, const ColliderSet* 
   NODE_LAUNCH(k_cr_update_x, B, flags, cols, p, Ap, diag, x, z, sc, st, partials, hist);
 }
 void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
-                  const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* st, double* partials,
-                  hipStream_t s) {
-  NODE_LAUNCH(k_cr_ar, B, mass, flags, cols, stage, z, dt, Ar, sc, st, partials);
+                  const float* z, const float* Ap, const float* diag, float dt, float* Ar, CRScalars* sc,
+                  BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_cr_ar, B, mass, flags, cols, stage, z, Ap, diag, dt, Ar, sc, st, partials);
 }
-void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                        CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_cr_update_p, B, z, Ar, diag, p, Ap, sc, st, partials);
+void launch_cr_update_px(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* Ar,
+                         const float* diag, float* p, float* Ap, float* x, float* z, CRScalars* sc, BatchStatus* st,
+                         double* partials, double* hist, hipStream_t s) {
+  NODE_LAUNCH(k_cr_update_px, B, flags, cols, Ar, diag, p, Ap, x, z, sc, st, partials, hist);
 }
 void launch_sort_tiles(const int* count, const int* offset, int* perm, const int8_t* pcode, int* tmpA, int* tmpB,
                        int NT, hipStream_t s) {

@@ -102,11 +102,14 @@ void launch_cr_setup2(const Batch& B, const float* mass, const int8_t* flags, co
 void launch_cr_update_x(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* p, const float* Ap,
                         const float* diag, float* x, float* z, CRScalars* sc, BatchStatus* st, double* partials,
                         double* hist, hipStream_t s);
+// Ar = A z, beta and the next alpha (the next denominator by expansion, from Ap and diag)
 void launch_cr_ar(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
-                  const float* z, float dt, float* Ar, CRScalars* sc, BatchStatus* st, double* partials,
-                  hipStream_t s);
-void launch_cr_update_p(const Batch& B, const float* z, const float* Ar, const float* diag, float* p, float* Ap,
-                        CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+                  const float* z, const float* Ap, const float* diag, float dt, float* Ar, CRScalars* sc,
+                  BatchStatus* st, double* partials, hipStream_t s);
+// p = z + beta p, Ap = Ar + beta Ap, then the x update of the next iteration (as launch_cr_update_x)
+void launch_cr_update_px(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* Ar,
+                         const float* diag, float* p, float* Ap, float* x, float* z, CRScalars* sc, BatchStatus* st,
+                         double* partials, double* hist, hipStream_t s);
 // order each tile's particles by (rank among equal stencil bases, base) (tmpA, tmpB: n ints of
 // scratch each, n >= particles binned)
 void launch_sort_tiles(const int* count, const int* offset, int* perm, const int8_t* pcode, int* tmpA, int* tmpB,

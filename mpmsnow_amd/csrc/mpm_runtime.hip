@@ -73,14 +73,14 @@ void raise_errors(int e) {
 // of small bodies is one launch for all their iterations: its bytes are those of an HVP and
 // the three node kernels per iteration (plus the setup's two).
 enum ProfKind {
-  PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_P, PK_LPC, PK_CR_SMALL,
+  PK_P2G_MASS, PK_P2G_VEL, PK_FORCES, PK_HVP, PK_DIAG, PK_G2P, PK_CR_X, PK_CR_AR, PK_CR_PX, PK_LPC, PK_CR_SMALL,
   PK_COUNT
 };
 const char* const kProfNames[PK_COUNT] = {"p2g_mass", "p2g_vel", "forces", "hvp", "diag", "g2p",
-                                          "cr_update_x", "cr_ar", "cr_update_p", "lpc_forward", "cr_small"};
+                                          "cr_update_x", "cr_ar", "cr_update_px", "lpc_forward", "cr_small"};
 // the network is MFMA work: its launches are timed but carry no HBM byte count
 const double kProfBytesPerParticle[PK_COUNT] = {16, 28, 136, 172, 172, 324, 0, 0, 0, 0, 172};
-const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 41, 84, 0, 85 + 41 + 84 + 29};
+const double kProfBytesPerNode[PK_COUNT] = {4, 16, 12, 29, 12, 24, 85, 65, 121, 0, 121 + 65 + 29};
 // ProfRec::iter values other than CR-loop iterations
 constexpr int kProfOnce = -1;     // once per solve, every body of the batch
 constexpr int kProfSetup = -2;    // CR setup of the bodies the batched loop solves
@@ -723,22 +723,28 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     };
     if (capWant > 0) snapshot(0);
     // one CR iteration (ConjugateResiduals.cpp:84-151) for every body still iterating
+    // iteration `it` from its x update on; from the second iteration on, the previous
+    // iteration's p / Ap update runs in the same pass (k_cr_update_px)
     auto iterate_x = [&](int it) {
-      int px = c.prof_begin(PK_CR_X, this, it);
-      launch_cr_update_x(Bb, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
-                         partials.as<double>(), hist.as<double>(), s);
-      c.prof_end(px);
+      if (it == 0) {
+        int px = c.prof_begin(PK_CR_X, this, it);
+        launch_cr_update_x(Bb, flags.as<int8_t>(), c.cols(), f(p), f(Ap), f(diag), f(x), f(z), sc(), st(),
+                           partials.as<double>(), hist.as<double>(), s);
+        c.prof_end(px);
+      } else {
+        int pp = c.prof_begin(PK_CR_PX, this, it);
+        launch_cr_update_px(Bb, flags.as<int8_t>(), c.cols(), f(Ar), f(diag), f(p), f(Ap), f(x), f(z), sc(), st(),
+                            partials.as<double>(), hist.as<double>(), s);
+        c.prof_end(pp);
+      }
       if (capWant > 0) snapshot(it + 1);
     };
     auto iterate_rest = [&](int it) {
       hvp(Bb, f(z), true, f(stageA), sc(), it);
       int pa = c.prof_begin(PK_CR_AR, this, it);
-      launch_cr_ar(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), dt, f(Ar), sc(), st(),
+      launch_cr_ar(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(Ap), f(diag), dt, f(Ar), sc(), st(),
                    partials.as<double>(), s);
       c.prof_end(pa);
-      int pp = c.prof_begin(PK_CR_P, this, it);
-      launch_cr_update_p(Bb, f(z), f(Ar), f(diag), f(p), f(Ap), sc(), st(), partials.as<double>(), s);
-      c.prof_end(pp);
     };
     auto set_done_flag = [&](int bi, int v) {
       hsc[bi].done = v;
