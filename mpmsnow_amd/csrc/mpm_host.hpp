@@ -36,7 +36,9 @@ struct DevBuf {
     if (bytes <= cap) return;
     if (p) HIPCHECK(hipFree(p));
     p = nullptr;
-    size_t c = std::max<size_t>(bytes + bytes / 4, 256);  // headroom: grids grow slowly step to step
+    // headroom: grids grow step to step as bodies fragment and spread, and every regrowth costs
+    // a device-wide wait in hipFree plus a fresh hipMalloc (milliseconds for large buffers)
+    size_t c = std::max<size_t>(2 * bytes, 256);
     HIPCHECK(hipMalloc(&p, c));
     cap = c;
   }

@@ -808,12 +808,14 @@ __global__ void k_cc_roots(int n, int* parent, const int* hasNb, const int* orig
     }
   }
 }
-__global__ void k_cc_compact(int n, const int* isRoot, const int* rootIdx, const int* minOrig, int* rootMin) {
-  GRID_STRIDE(s, n) if (isRoot[s]) rootMin[rootIdx[s]] = minOrig[s];
+// Bodies are numbered by their smallest original particle index (distinct per body): mark those
+// indices, and after an exclusive scan of the marks a body's number is its index's scan value.
+__global__ void k_cc_mark(int n, const int* isRoot, const int* minOrig, int* mark) {
+  GRID_STRIDE(s, n) if (isRoot[s]) mark[minOrig[s]] = 1;
 }
-__global__ void k_cc_assign(int n, const int* parent, const int* hasNb, const int* rootIdx, const int* rootRank,
+__global__ void k_cc_assign(int n, const int* parent, const int* hasNb, const int* minOrig, const int* markPos,
                             int* labels) {
-  GRID_STRIDE(s, n) labels[s] = hasNb[s] ? rootRank[rootIdx[parent[s]]] : -1;
+  GRID_STRIDE(s, n) labels[s] = hasNb[s] ? markPos[minOrig[parent[s]]] : -1;
 }
 // velocity finiteness (Sim.cpp:151-159)
 __global__ void k_check_pvel(PView P, int n, int* err) {
@@ -2837,12 +2839,13 @@ void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, i
                      hipStream_t s) {
   if (n > 0) LAUNCH(k_cc_roots, grid_for(n, 16384), n, parent, hasNb, orig_of, isRoot, minOrig);
 }
-void launch_cc_compact(int n, const int* isRoot, const int* rootIdx, const int* minOrig, int* rootMin, hipStream_t s) {
-  if (n > 0) LAUNCH(k_cc_compact, grid_for(n, 16384), n, isRoot, rootIdx, minOrig, rootMin);
-}
-void launch_cc_assign(int n, const int* parent, const int* hasNb, const int* rootIdx, const int* rootRank,
-                      int* labels, hipStream_t s) {
-  if (n > 0) LAUNCH(k_cc_assign, grid_for(n, 16384), n, parent, hasNb, rootIdx, rootRank, labels);
+void launch_cc_number(int n, const int* parent, const int* hasNb, const int* isRoot, const int* minOrig, int* mark,
+                      int* markPos, int* nbodies, int* tmp, int* labels, hipStream_t s) {
+  if (n <= 0) return;
+  LAUNCH(k_fill_int, grid_for(n, 16384), mark, 0, (int64_t)n);
+  LAUNCH(k_cc_mark, grid_for(n, 16384), n, isRoot, minOrig, mark);
+  scan_exclusive(mark, markPos, n, nbodies, tmp, s);
+  LAUNCH(k_cc_assign, grid_for(n, 16384), n, parent, hasNb, minOrig, (const int*)markPos, labels);
 }
 void launch_check_pvel(PView P, int n, int* err, hipStream_t s) {
   if (n > 0) LAUNCH(k_check_pvel, grid_for(n), P, n, err);

@@ -32,10 +32,10 @@ void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset
 // isRoot[s] = 1 for the root of a component of >= 2 particles; minOrig[root] = smallest caller index
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s);
-void launch_cc_compact(int n, const int* isRoot, const int* rootIdx, const int* minOrig, int* rootMin, hipStream_t s);
-// labels[s] = body of s (rank of its root) or -1 for ballistic particles
-void launch_cc_assign(int n, const int* parent, const int* hasNb, const int* rootIdx, const int* rootRank,
-                      int* labels, hipStream_t s);
+// labels[s] = number of the body of particle s (bodies numbered by smallest original index),
+// -1 for particles without neighbours; *nbodies (device) = the count. mark, markPos: n ints.
+void launch_cc_number(int n, const int* parent, const int* hasNb, const int* isRoot, const int* minOrig, int* mark,
+                      int* markPos, int* nbodies, int* tmp, int* labels, hipStream_t s);
 void launch_check_pvel(PView P, int n, int* err, hipStream_t s);
 
 // ---- per-body reductions: bbox (ordered uint, 6/body), sum m v and sum m (double, 4/body), count

@@ -213,8 +213,8 @@ struct mpm_context {
   std::vector<std::vector<int>> bodyLists;
   std::vector<int> ballisticList;
   // body-partition scratch
-  DevBuf ccBucket, ccRank, ccCount, ccOffset, ccSorted, ccParent, ccHasNb, ccIsRoot, ccRootIdx, ccMinOrig, ccRootMin,
-      ccRootRank, ccTmp, ccTot, newToOld, ballPos;
+  DevBuf ccBucket, ccRank, ccCount, ccOffset, ccSorted, ccParent, ccHasNb, ccIsRoot, ccRootIdx, ccMinOrig, ccTmp,
+      ccTot, newToOld, ballPos;
   int iterHint = 8;
   int iterHintFcr = 8;
   // learned preconditioner
@@ -518,8 +518,11 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   partials.ensure(sizeof(double) * kPartialStride * std::max(nblocks, 1));
   tileAct.ensure(std::max(NT, 1));
   HIPCHECK(hipMemsetAsync(tileAct.p, 1, std::max(NT, 1), s));
+  // reduction tickets return to zero after every use (the last arrival resets each counter), so
+  // they are cleared only when the buffer is (re)allocated
+  const void* oldTickets = tickets.p;
   tickets.ensure(sizeof(unsigned int) * kTicketsPerBody * nb);
-  HIPCHECK(hipMemsetAsync(tickets.p, 0, sizeof(unsigned int) * kTicketsPerBody * nb, s));
+  if (tickets.p != oldTickets) HIPCHECK(hipMemsetAsync(tickets.p, 0, tickets.cap, s));
   HIPCHECK(hipMemsetAsync(flags.p, 0xFF, NN, s));  // -1: no collision until updateGridVelocities
   rec.assign(nb, SolveRecord{});
   for (int bi = 0; bi < nb; ++bi) rec[bi].np = hb[bi].np;
@@ -616,11 +619,13 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     HIPCHECK(hipMemcpyAsync(hst, status.p, sizeof(BatchStatus), hipMemcpyDeviceToHost, s));
     c.sync();
   };
-  auto check_errflag = [&]() {
-    int e = 0;
-    HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  // status and error flag in one wait
+  auto fetch_status_errors = [&]() {
+    int* e = reinterpret_cast<int*>(reinterpret_cast<char*>(hst) + sizeof(BatchStatus));
+    HIPCHECK(hipMemcpyAsync(hst, status.p, sizeof(BatchStatus), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     c.sync();
-    raise_errors(e);
+    raise_errors(*e);
   };
   const int64_t iterCap = (prm && prm->termination) ? 1000000 : (int64_t)maxIters + 2;
   const bool learned = c.lpcMode == 1 && !hooks;
@@ -631,8 +636,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     hvp(f(x), true, f(stageA), sc(), -1);
     launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, sc(),
                     st(), partials.as<double>(), s);
-    fetch_status();
-    check_errflag();
+    fetch_status_errors();
     bool netActive = netOn != 0;
     if (netActive) {
       ensure_levels();
@@ -703,8 +707,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     hvp(Bb, f(z), true, f(stageA), sc(), kProfSetup);
     launch_cr_setup2(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
                      partials.as<double>(), s);
-    fetch_status();
-    check_errflag();
+    fetch_status_errors();
     // training pairs: iterate x_k and residual r_k = M z_k at the iterations of kCaptureIters
     static const int kCaptureIters[] = {0, 1, 2, 3, 4, 6, 8, 11, 15, 20, 26, 33, 41, 50, 60, 72};
     LpcCapture& cap = c.cap;
@@ -941,10 +944,11 @@ void require_particles(mpm_context* c) {
 // |x_i - x_j|^2 < h^2 (float arithmetic of KdTree.inl:187-190) by cell hashing and
 // sampled union-find; isolated particles are ballistic; bodies are ordered by their smallest
 // particle index, as the reference's flood fill numbers them.
-void calculate_bodies(mpm_context& c) {
+// keepErrors: the error flag still holds the step's earlier errors, raised by this wait
+void calculate_bodies(mpm_context& c, bool keepErrors = false) {
   const int n = (int)c.n;
   hipStream_t s = c.stream;
-  c.clear_err();
+  if (!keepErrors) c.clear_err();
   launch_check_pvel(c.view(), n, c.errp(), s);
   uint32_t tsize = 1;
   while (tsize < 2u * (uint32_t)n) tsize <<= 1;
@@ -977,33 +981,23 @@ void calculate_bodies(mpm_context& c) {
   launch_fill_int(c.ccMinOrig.as<int>(), INT_MAX, n, s);
   launch_cc_roots(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.orig_of[c.cur].as<int>(), c.ccIsRoot.as<int>(),
                   c.ccMinOrig.as<int>(), s);
-  scan_exclusive(c.ccIsRoot.as<int>(), c.ccRootIdx.as<int>(), n, c.ccTot.as<int>() + 1, c.ccTmp.as<int>(), s);
-  int nroots = 0;
-  HIPCHECK(hipMemcpyAsync(&nroots, c.ccTot.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-  c.check_errors();
-  c.ccRootMin.ensure(sizeof(int) * std::max(nroots, 1));
-  c.ccRootRank.ensure(sizeof(int) * std::max(nroots, 1));
-  std::vector<int> rootMin(nroots), rank(nroots);
-  if (nroots > 0) {
-    launch_cc_compact(n, c.ccIsRoot.as<int>(), c.ccRootIdx.as<int>(), c.ccMinOrig.as<int>(), c.ccRootMin.as<int>(), s);
-    HIPCHECK(hipMemcpyAsync(rootMin.data(), c.ccRootMin.p, sizeof(int) * nroots, hipMemcpyDeviceToHost, s));
-    c.sync();
-    std::vector<int> order(nroots);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return rootMin[a] < rootMin[b]; });
-    for (int r = 0; r < nroots; ++r) rank[order[r]] = r;
-    HIPCHECK(hipMemcpyAsync(c.ccRootRank.p, rank.data(), sizeof(int) * nroots, hipMemcpyHostToDevice, s));
-  }
-  launch_cc_assign(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccRootIdx.as<int>(), c.ccRootRank.as<int>(),
+  // bodies numbered by smallest original particle index (the reference's flood order)
+  launch_cc_number(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccIsRoot.as<int>(), c.ccMinOrig.as<int>(),
+                   c.ccRootIdx.as<int>(), c.ccRank.as<int>(), c.ccTot.as<int>() + 1, c.ccTmp.as<int>(),
                    c.labels.as<int>(), s);
-  c.nbodies = nroots;
-  c.bodyListsValid = false;
   // ballistic count for statistics
   c.ballPos.ensure(sizeof(int) * n);
   launch_is_ballistic(c.labels.as<int>(), n, c.ccIsRoot.as<int>(), s);
   scan_exclusive(c.ccIsRoot.as<int>(), c.ballPos.as<int>(), n, c.ccTot.as<int>() + 2, c.ccTmp.as<int>(), s);
-  HIPCHECK(hipMemcpyAsync(&c.nballistic, c.ccTot.as<int>() + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+  // one wait: body and ballistic counts and the velocity check
+  int tot[2] = {0, 0}, e = 0;
+  HIPCHECK(hipMemcpyAsync(tot, c.ccTot.as<int>() + 1, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(&e, c.errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
   c.sync();
+  raise_errors(e);
+  c.nbodies = tot[0];
+  c.nballistic = tot[1];
+  c.bodyListsValid = false;
 }
 
 void ensure_body_lists(const mpm_context& cc) {
@@ -1553,7 +1547,7 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
     // ballistic particles: velocity (Sim.cpp:64-73) and, unless cancelled, advection
     if (c.nballistic > 0)
       launch_ballistic(c.view(), labels, n, c.grav(), c.ngrav(), dt, c.cols(), cancelled ? 1 : 3, s);
-    c.check_errors();
+    // errors of the particle update are raised with the body partition's wait below
     if (!cancelled && nb > 0) {
       // keep the store body- and tile-sorted: body particles in batch order, then ballistic ones
       c.newToOld.ensure(sizeof(int) * n);
@@ -1570,7 +1564,7 @@ int mpm_sim_advance(mpm_context* ctx, float dt, const mpm_solver_params* params,
       c.cur = nxt;
     }
     c.prof_resolve();
-    calculate_bodies(c);
+    calculate_bodies(c, true);
     if (stats) {
       stats->num_bodies = c.nbodies;
       stats->num_ballistic = c.nballistic;
