@@ -36,7 +36,14 @@ typedef struct mpm_grid mpm_grid;
 
 /* Collision object (reference include/MpmSim/CollisionObject.h:10-45):
  * type 0 = CollisionPlane (phi = c0 x + c1 y + c2 z + c3, CollisionPlane.h:14-40),
- * type 1 = analytic sphere (phi = |x - center| - radius). */
+ * type 1 = analytic sphere (phi = |x - center| - radius),
+ * type 2 = level-set volume, the reference's VDBCollisionObject
+ *   (houdiniPlugin/VDBCollisionObject.cpp:11-37, which samples OpenVDB grids; that object uses
+ *   friction 0.5 and is not sticky): phi at the points origin + (i, j, k) voxel, i < dims[0],
+ *   j < dims[1], k < dims[2], point (i, j, k) at phi[i + dims[0] (j + dims[1] k)], sampled
+ *   trilinearly (clamped to the volume); grad = central difference of the sampled phi over one
+ *   voxel per axis; velocity = phi_velocity sampled the same way (3 floats per point), or the
+ *   uniform `velocity` when phi_velocity is null. The arrays are copied by mpm_set_colliders. */
 typedef struct {
   int type;
   float coeffs[4];
@@ -45,6 +52,11 @@ typedef struct {
   float velocity[3];
   float friction; /* coulombFriction() */
   int sticky;     /* sticky() */
+  int dims[3];              /* type 2 */
+  float origin[3];          /* type 2 */
+  float voxel;              /* type 2 */
+  const float* phi;         /* type 2: dims[0] dims[1] dims[2] floats */
+  const float* phi_velocity; /* type 2, optional: 3 dims[0] dims[1] dims[2] floats */
 } mpm_collider;
 
 /* Solver settings: SquareMagnitudeTermination(maxIters, tolError)

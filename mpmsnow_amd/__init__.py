@@ -31,7 +31,8 @@ CANCEL_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 class Collider(C.Structure):
     _fields_ = [("type", C.c_int), ("coeffs", C.c_float * 4), ("center", C.c_float * 3), ("radius", C.c_float),
-                ("velocity", C.c_float * 3), ("friction", C.c_float), ("sticky", C.c_int)]
+                ("velocity", C.c_float * 3), ("friction", C.c_float), ("sticky", C.c_int), ("dims", C.c_int * 3),
+                ("origin", C.c_float * 3), ("voxel", C.c_float), ("phi", _fp), ("phi_velocity", _fp)]
 
 
 class SolverParams(C.Structure):
@@ -257,9 +258,13 @@ class Context:
         return from_colmajor(out)
 
     # CollisionObjectSet / ForceFieldSet
-    def set_colliders(self, planes=(), spheres=()):
-        """planes: (coeffs4, velocity3, friction, sticky); spheres: (center3, radius, velocity3, friction, sticky)."""
+    def set_colliders(self, planes=(), spheres=(), volumes=()):
+        """planes: (coeffs4, velocity3, friction, sticky); spheres: (center3, radius, velocity3, friction, sticky);
+        volumes (level sets, the reference's VDBCollisionObject): (phi, origin3, voxel, velocity, friction,
+        sticky) with phi[k, j, i] the value at origin + (i, j, k) voxel and velocity a 3-vector or an array
+        of shape phi.shape + (3,). Order in the set: planes, spheres, volumes."""
         items = []
+        keep = []
         for coeffs, v, mu, sticky in planes:
             c = Collider()
             c.type = 0
@@ -274,6 +279,27 @@ class Context:
             c.center[:] = list(map(float, center))
             c.radius = r
             c.velocity[:] = list(map(float, v))
+            c.friction = mu
+            c.sticky = int(sticky)
+            items.append(c)
+        for phi, origin, voxel, v, mu, sticky in volumes:
+            ph = np.ascontiguousarray(phi, dtype=np.float32)
+            if ph.ndim != 3:
+                raise ValueError("phi must be a 3-D array indexed [k, j, i]")
+            c = Collider()
+            c.type = 2
+            c.dims[:] = [ph.shape[2], ph.shape[1], ph.shape[0]]
+            c.origin[:] = list(map(float, origin))
+            c.voxel = float(voxel)
+            c.phi = ph.ctypes.data_as(_fp)
+            keep.append(ph)
+            va = np.asarray(v, dtype=np.float32)
+            if va.shape == (3,):
+                c.velocity[:] = list(map(float, va))
+            else:
+                va = np.ascontiguousarray(va.reshape(ph.shape + (3,)))
+                c.phi_velocity = va.ctypes.data_as(_fp)
+                keep.append(va)
             c.friction = mu
             c.sticky = int(sticky)
             items.append(c)
@@ -425,7 +451,7 @@ class Sim:
     """Reference Sim (include/MpmSim/Sim.h) on the GPU: particles, snow model, colliders, gravity."""
 
     def __init__(self, x, m, h, material=None, planes=(), spheres=(), gravity=(), device=0, mu0=None, lambda0=None,
-                 xi=None, v=None):
+                 xi=None, v=None, volumes=()):
         self.ctx = Context(device)
         x = np.asarray(x, dtype=np.float32).reshape(-1, 3)
         self.ctx.particles_create(len(x))
@@ -440,7 +466,7 @@ class Sim:
                 self.ctx.set("mu0", mu0)
                 self.ctx.set("lambda0", lambda0)
                 self.ctx.set("xi", xi)
-        self.ctx.set_colliders(planes, spheres)
+        self.ctx.set_colliders(planes, spheres, volumes)
         if len(gravity):
             self.ctx.set_gravity(gravity)
         self.ctx.check(lib().mpm_sim_init(self.ctx.ptr, h, 3))

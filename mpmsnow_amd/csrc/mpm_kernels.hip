@@ -1390,10 +1390,10 @@ __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, co
     e[2] = ev[2] * m;
     if (!isfinite(e[0] * e[1] * e[2])) bad = 1;
     if (fl >= 0) {
-      const Collider& o = cols->o[fl];
-      cv[0] = o.v[0] - c.G.fv[0];
-      cv[1] = o.v[1] - c.G.fv[1];
-      cv[2] = o.v[2] - c.G.fv[2];
+      col_velocity(cols->o[fl], x, cv);
+      cv[0] -= c.G.fv[0];
+      cv[1] -= c.G.fv[1];
+      cv[2] -= c.G.fv[2];
       s[4] = (double)m;
       s[5] = (double)(m * cv[0]);
       s[6] = (double)(m * cv[1]);
@@ -1450,8 +1450,10 @@ __global__ void k_collision_velocities(Batch B, const int8_t* flags, const Colli
   int fl = flags[c.i];
   float v[3] = {0.0f, 0.0f, 0.0f};
   if (fl >= 0 && c.valid) {
-    const Collider& o = cols->o[fl];
-    for (int a = 0; a < 3; ++a) v[a] = o.v[a] - c.G.fv[a];
+    float x[3];
+    node_pos(c.G, c.ni, c.nj, c.nk, x);
+    col_velocity(cols->o[fl], x, v);
+    for (int a = 0; a < 3; ++a) v[a] -= c.G.fv[a];
   }
   st3(vc, c.i, v);
 }

@@ -370,13 +370,20 @@ MPM_HD void snow_stress_differential(const float* dF, const float* R, const floa
 
 // ------------------------------------------------------------ colliders
 struct Collider {
-  int type;  // 0 plane, 1 sphere
+  int type;  // 0 plane, 1 sphere, 2 level-set volume
   float c[4];
   float center[3];
   float radius;
   float v[3];
   float friction;
   int sticky;
+  // type 2: phi at the points origin + (i, j, k) voxel (point (i, j, k) at i + dims0 (j + dims1 k)),
+  // and optionally a velocity per point (3 floats; null: the uniform v), in device memory
+  int dims[3];
+  float origin[3];
+  float voxel;
+  const float* phi;
+  const float* vel;
 };
 struct ColliderSet {
   int n;
@@ -386,22 +393,79 @@ struct ColliderSet {
 // Collider tests decide the collided flag of a node, which must match the reference bit for bit:
 // they keep its separately rounded float products and sums (hipcc would otherwise fuse a*b + c
 // into one FMA, which rounds once).
+// Level-set volume sampling (the reference's VDBCollisionObject, houdiniPlugin/VDBCollisionObject.cpp:
+// 11-37, reads phi, its gradient and a velocity field from OpenVDB grids): trilinear
+// interpolation of the point values, clamped to the volume like a 3-D texture with clamp
+// addressing; each lerp is a + t (b - a) with separately rounded operations.
+MPM_HD float vol_lerp(float a, float b, float t) {
+#pragma clang fp contract(off)
+  return a + t * (b - a);
+}
+// nc components per point, component c of point p at f[p * nc + c]
+MPM_HD void vol_sample(const Collider& o, const float* f, int nc, const float x[3], float* out) {
+#pragma clang fp contract(off)
+  int i0[3];
+  float t[3];
+  for (int a = 0; a < 3; ++a) {
+    float u = (x[a] - o.origin[a]) / o.voxel;
+    const float hi = (float)(o.dims[a] - 1);
+    u = u < 0.0f ? 0.0f : (u > hi ? hi : u);  // also maps NaN to 0 (u < 0 and u > hi are false)
+    int i = (int)floorf(u);
+    if (i > o.dims[a] - 2) i = o.dims[a] - 2;
+    i0[a] = i;
+    t[a] = u - (float)i;
+  }
+  const int64_t sx = nc, sy = (int64_t)nc * o.dims[0], sz = sy * o.dims[1];
+  const float* b = f + i0[0] * sx + i0[1] * sy + i0[2] * sz;
+  for (int c = 0; c < nc; ++c) {
+    const float* q = b + c;
+    const float e00 = vol_lerp(q[0], q[sx], t[0]);
+    const float e10 = vol_lerp(q[sy], q[sy + sx], t[0]);
+    const float e01 = vol_lerp(q[sz], q[sz + sx], t[0]);
+    const float e11 = vol_lerp(q[sz + sy], q[sz + sy + sx], t[0]);
+    out[c] = vol_lerp(vol_lerp(e00, e10, t[1]), vol_lerp(e01, e11, t[1]), t[2]);
+  }
+}
 MPM_HD float col_phi(const Collider& o, const float x[3]) {
 #pragma clang fp contract(off)
   if (o.type == 0) return o.c[0] * x[0] + o.c[1] * x[1] + o.c[2] * x[2] + o.c[3];
+  if (o.type == 2) {
+    float p;
+    vol_sample(o, o.phi, 1, x, &p);
+    return p;
+  }
   float d0 = x[0] - o.center[0], d1 = x[1] - o.center[1], d2 = x[2] - o.center[2];
   return sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o.radius;
 }
-// normalised gradient (CollisionObject::grad + Vector3f::normalize)
+// normalised gradient (CollisionObject::grad + Vector3f::normalize); a volume's gradient is the
+// central difference of the sampled phi over one voxel on each axis
 MPM_HD void col_normal(const Collider& o, const float x[3], float n[3]) {
 #pragma clang fp contract(off)
   if (o.type == 0) {
     n[0] = o.c[0]; n[1] = o.c[1]; n[2] = o.c[2];
+  } else if (o.type == 2) {
+    for (int a = 0; a < 3; ++a) {
+      float xp[3] = {x[0], x[1], x[2]}, xm[3] = {x[0], x[1], x[2]};
+      xp[a] = x[a] + o.voxel;
+      xm[a] = x[a] - o.voxel;
+      float pp, pm;
+      vol_sample(o, o.phi, 1, xp, &pp);
+      vol_sample(o, o.phi, 1, xm, &pm);
+      n[a] = (pp - pm) / (2.0f * o.voxel);
+    }
   } else {
     n[0] = x[0] - o.center[0]; n[1] = x[1] - o.center[1]; n[2] = x[2] - o.center[2];
   }
   float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   n[0] /= nn; n[1] /= nn; n[2] /= nn;
+}
+// CollisionObject::velocity: uniform, or sampled from a volume's velocity field
+MPM_HD void col_velocity(const Collider& o, const float x[3], float v[3]) {
+  if (o.type == 2 && o.vel) {
+    vol_sample(o, o.vel, 3, x, v);
+  } else {
+    v[0] = o.v[0]; v[1] = o.v[1]; v[2] = o.v[2];
+  }
 }
 
 // CollisionObjectSet::collide (src/CollisionObject.cpp:19-96)
@@ -417,7 +481,9 @@ MPM_HD int collide(const ColliderSet& cs, float v[3], const float x[3], const fl
         break;
       }
       already = true;
-      float vo[3] = {o.v[0] - fv[0], o.v[1] - fv[1], o.v[2] - fv[2]};
+      float vo[3];
+      col_velocity(o, x, vo);
+      vo[0] -= fv[0]; vo[1] -= fv[1]; vo[2] -= fv[2];
       v[0] -= vo[0]; v[1] -= vo[1]; v[2] -= vo[2];
       float n[3];
       col_normal(o, x, n);

@@ -198,6 +198,7 @@ struct mpm_context {
   DevBuf orig_of[2], slot_of;
   DevBuf cols_dev, grav_dev, errflag, staging, idx_tmp;
   ColliderSet hcols{};
+  DevBuf sdfPhi[kMaxColliders], sdfVel[kMaxColliders];  // level-set collider volumes
   std::vector<float> hgrav;
   PinnedBuf pinned;
   // simulation state
@@ -1219,7 +1220,7 @@ int mpm_set_colliders(mpm_context* ctx, const mpm_collider* cols, int n) {
     for (int i = 0; i < n; ++i) {
       Collider& o = cs.o[i];
       o.type = cols[i].type;
-      if (o.type != 0 && o.type != 1) throw InvalidArg("unknown collider type");
+      if (o.type != 0 && o.type != 1 && o.type != 2) throw InvalidArg("unknown collider type");
       for (int a = 0; a < 4; ++a) o.c[a] = cols[i].coeffs[a];
       for (int a = 0; a < 3; ++a) {
         o.center[a] = cols[i].center[a];
@@ -1228,6 +1229,30 @@ int mpm_set_colliders(mpm_context* ctx, const mpm_collider* cols, int n) {
       o.radius = cols[i].radius;
       o.friction = cols[i].friction;
       o.sticky = cols[i].sticky;
+      o.phi = nullptr;
+      o.vel = nullptr;
+      if (o.type == 2) {
+        const mpm_collider& q = cols[i];
+        if (!q.phi) throw InvalidArg("level-set collider without phi");
+        if (!(q.voxel > 0.0f)) throw InvalidArg("level-set voxel size must be positive");
+        int64_t np = 1;
+        for (int a = 0; a < 3; ++a) {
+          if (q.dims[a] < 2 || q.dims[a] > (1 << 12)) throw InvalidArg("level-set dims must be in [2, 4096]");
+          np *= q.dims[a];
+          o.dims[a] = q.dims[a];
+          o.origin[a] = q.origin[a];
+        }
+        o.voxel = q.voxel;
+        ctx->sdfPhi[i].ensure(sizeof(float) * np);
+        HIPCHECK(hipMemcpyAsync(ctx->sdfPhi[i].p, q.phi, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+        o.phi = ctx->sdfPhi[i].as<float>();
+        if (q.phi_velocity) {
+          ctx->sdfVel[i].ensure(sizeof(float) * 3 * np);
+          HIPCHECK(hipMemcpyAsync(ctx->sdfVel[i].p, q.phi_velocity, sizeof(float) * 3 * np, hipMemcpyHostToDevice,
+                                  ctx->stream));
+          o.vel = ctx->sdfVel[i].as<float>();
+        }
+      }
     }
     ctx->hcols = cs;
     HIPCHECK(hipMemcpyAsync(ctx->cols_dev.p, &ctx->hcols, sizeof(ColliderSet), hipMemcpyHostToDevice, ctx->stream));

@@ -377,23 +377,82 @@ template <class S> struct SnowModel : ConstitutiveModel<S> {
 // ------------------------------------------------------------ colliders
 // reference include/MpmSim/CollisionObject.h:10-45, CollisionPlane.cpp:23-56;
 // the sphere is the analytic sphere collider extension (phi = |x-c| - r).
+// Type 2 is a level-set volume standing in for the reference's VDBCollisionObject
+// (src/houdiniPlugin/VDBCollisionObject.cpp:11-37: phi, gradient and velocity read from OpenVDB
+// grids through the Houdini HDK, which is absent here, so the sampling rule is this build's own):
+// phi and an optional velocity per point at origin + (i, j, k) voxel, trilinear interpolation
+// clamped to the volume, each lerp a + t (b - a); gradient = central difference of the sampled
+// phi over one voxel per axis.
 template <class S> struct Collider {
-  int type = 0;  // 0 plane, 1 sphere
+  int type = 0;  // 0 plane, 1 sphere, 2 level-set volume
   S coeffs[4] = {0, 0, 0, 0};
   V3<S> center;
   S radius = 0;
   V3<S> vel;
   S friction = S(0.5);
   bool sticky = false;
+  int dims[3] = {0, 0, 0};
+  V3<S> origin;
+  S voxel = S(1);
+  std::vector<S> phiv, velv;  // velv: 3 values per point, or empty (uniform vel)
+  void sample(const std::vector<S>& f, int nc, const V3<S>& x, S* out) const {
+    int i0[3];
+    S t[3];
+    for (int a = 0; a < 3; ++a) {
+      S u = (x[a] - origin[a]) / voxel;
+      const S hi = S(dims[a] - 1);
+      u = u < S(0) ? S(0) : (u > hi ? hi : u);
+      int i = int(std::floor(u));
+      if (i > dims[a] - 2) i = dims[a] - 2;
+      i0[a] = i;
+      t[a] = u - S(i);
+    }
+    auto lerp = [](S a, S b, S tt) { return a + tt * (b - a); };
+    const size_t sx = size_t(nc), sy = sx * size_t(dims[0]), sz = sy * size_t(dims[1]);
+    const size_t b = size_t(i0[0]) * sx + size_t(i0[1]) * sy + size_t(i0[2]) * sz;
+    for (int c = 0; c < nc; ++c) {
+      const S* q = f.data() + b + size_t(c);
+      const S e00 = lerp(q[0], q[sx], t[0]);
+      const S e10 = lerp(q[sy], q[sy + sx], t[0]);
+      const S e01 = lerp(q[sz], q[sz + sx], t[0]);
+      const S e11 = lerp(q[sz + sy], q[sz + sy + sx], t[0]);
+      out[c] = lerp(lerp(e00, e10, t[1]), lerp(e01, e11, t[1]), t[2]);
+    }
+  }
   S phi(const V3<S>& x) const {
     if (type == 0) return coeffs[0] * x[0] + coeffs[1] * x[1] + coeffs[2] * x[2] + coeffs[3];
+    if (type == 2) {
+      S p;
+      sample(phiv, 1, x, &p);
+      return p;
+    }
     return norm(x - center) - radius;
   }
   void grad(const V3<S>& x, V3<S>& g) const {
     if (type == 0) { g = V3<S>(coeffs[0], coeffs[1], coeffs[2]); return; }
+    if (type == 2) {
+      for (int a = 0; a < 3; ++a) {
+        V3<S> xp = x, xm = x;
+        xp[a] = x[a] + voxel;
+        xm[a] = x[a] - voxel;
+        S pp, pm;
+        sample(phiv, 1, xp, &pp);
+        sample(phiv, 1, xm, &pm);
+        g[a] = (pp - pm) / (S(2) * voxel);
+      }
+      return;
+    }
     g = x - center;
   }
-  void velocity(const V3<S>&, V3<S>& v) const { v = vel; }
+  void velocity(const V3<S>& x, V3<S>& v) const {
+    if (type == 2 && !velv.empty()) {
+      S o[3];
+      sample(velv, 3, x, o);
+      v = V3<S>(o[0], o[1], o[2]);
+      return;
+    }
+    v = vel;
+  }
 };
 
 template <class S> struct CollisionSet {

@@ -297,6 +297,30 @@ int orc_cols_add_sphere(void* h, const double* center, double radius, const doub
     });
   });
 }
+// phi: dims0 dims1 dims2 values, point (i, j, k) at i + dims0 (j + dims1 k); pvel: 3 per point or null
+int orc_cols_add_volume(void* h, const double* phi, const int* dims, const double* origin, double voxel,
+                        const double* pvel, const double* v, double mu, int sticky) {
+  return guard([&] {
+    DISPATCH(h, {
+      Collider<S> c;
+      c.type = 2;
+      size_t np = 1;
+      for (int a = 0; a < 3; ++a) {
+        if (dims[a] < 2) throw std::runtime_error("level-set dims must be >= 2");
+        c.dims[a] = dims[a];
+        np *= size_t(dims[a]);
+      }
+      c.origin = v3of<S>(origin);
+      c.voxel = S(voxel);
+      c.phiv.assign(phi, phi + np);
+      if (pvel) c.velv.assign(pvel, pvel + 3 * np);
+      c.vel = v3of<S>(v);
+      c.friction = S(mu);
+      c.sticky = sticky != 0;
+      static_cast<ColsH<S>*>(h)->c.objects.push_back(c);
+    });
+  });
+}
 int orc_cols_collide(void* h, double* v, const double* x, const double* fv, int addVel) {
   int r = 0;
   DISPATCH(h, {

@@ -57,6 +57,8 @@ def lib():
             "orc_cols_create": (C.c_void_p, [C.c_int]),
             "orc_cols_add_plane": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, C.c_int]),
             "orc_cols_add_sphere": (C.c_int, [C.c_void_p, _dp, C.c_double, _dp, C.c_double, C.c_int]),
+            "orc_cols_add_volume": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int), _dp, C.c_double, _dp, _dp,
+                                              C.c_double, C.c_int]),
             "orc_cols_collide": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int]),
             "orc_fields_create": (C.c_void_p, [C.c_int]),
             "orc_fields_add_gravity": (C.c_int, [C.c_void_p, _dp]),
@@ -231,6 +233,23 @@ class Colliders(_H):
         c, pc = _d(center)
         vv, pv = _d(v)
         _check(lib().orc_cols_add_sphere(self.ptr, pc, radius, pv, friction, int(sticky)))
+
+    def add_volume(self, phi, origin, voxel, v=(0, 0, 0), friction=0.5, sticky=False):
+        """Level-set volume: phi[k, j, i] at origin + (i, j, k) voxel; v a 3-vector (uniform) or an
+        array of shape phi.shape + (3,)."""
+        ph = np.ascontiguousarray(phi, dtype=np.float64)
+        dims = (C.c_int * 3)(ph.shape[2], ph.shape[1], ph.shape[0])
+        oo, po = _d(origin)
+        va = np.asarray(v, dtype=np.float64)
+        if va.shape == (3,):
+            uv, pu = _d(va)
+            pvel = None
+        else:
+            vf = np.ascontiguousarray(va.reshape(ph.shape + (3,)))
+            pvel = vf.ctypes.data_as(_dp)
+            uv, pu = _d(np.zeros(3))
+        _check(lib().orc_cols_add_volume(self.ptr, ph.ctypes.data_as(_dp), dims, po, float(voxel), pvel, pu,
+                                         friction, int(sticky)))
 
     def collide(self, v, x, frame_v=(0, 0, 0), add_collision_velocity=False):
         vv, pv = _d(np.array(v, dtype=np.float64))
