@@ -102,6 +102,7 @@ struct Batch {
   int nblocks;              // global node blocks
   unsigned int* tickets;    // reduction tickets: [body][set][1 + kSub][kTicketStride]
   const unsigned char* tileActive;  // per tile: some node has mass or a collider flag
+  const float* nrm;         // per node with a collider flag >= 0: the collider's unit normal there
 };
 
 // Per-body reductions inside node kernels: the body's blocks form kSub sub-groups (local

@@ -60,15 +60,17 @@ __device__ __forceinline__ void node_pos(const BodyGrid& G, int i, int j, int k,
   x[1] = G.h * (float)j + G.gmin[1];
   x[2] = G.h * (float)k + G.gmin[2];
 }
-// ImplicitUpdateMatrix::subspaceProject for one node (Grid.cpp:778-818)
-__device__ __forceinline__ void project(const ColliderSet* cols, int flag, const float x[3], float v[3]) {
+// ImplicitUpdateMatrix::subspaceProject for one node (Grid.cpp:778-818). The collider's normal
+// at the node (CollisionObject::grad, normalised) is evaluated once per solve, where the flags are
+// set (k_node_explicit, k_node_normals), and read here from nrm at the node's vector offset b:
+// the solver's kernels then carry no collider evaluation.
+__device__ __forceinline__ void project(const float* nrm, int64_t b, int flag, float v[3]) {
   if (flag == -1) return;
   if (flag == -2) {
     v[0] = v[1] = v[2] = 0.0f;
     return;
   }
-  float n[3];
-  col_normal(cols->o[flag], x, n);
+  const float n[3] = {nrm[b], nrm[b + 64], nrm[b + 128]};
   float nd = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
   v[0] -= nd * n[0];
   v[1] -= nd * n[1];
@@ -93,11 +95,13 @@ struct ChunkCtx {
   int4 ch;       // (global tile, start in perm, count, body)
   BodyGrid G;
   int ox, oy, oz;  // node origin of the chunk's tile
+  const float* nrm;  // Batch::nrm
 };
 // all threads: the chunk, its body, and (threads 0..7) the footprint's global tiles
 __device__ __forceinline__ ChunkCtx chunk_begin_at(const Batch& B, int* ft, int4 ch) {
   ChunkCtx c;
   c.ch = ch;
+  c.nrm = B.nrm;
   c.G = B.bodies[c.ch.w];
   int tx, ty, tz;
   tile_xyz(c.G, c.ch.x - c.G.tileBase, tx, ty, tz);
@@ -136,13 +140,7 @@ __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, con
       v[2] = d[b + 128];
       if (proj) {
         int fl = flags[(int64_t)(gt - tileOff) * 64 + l];
-        if (fl != -1) {
-          float xn[3];
-          int fx, fy, fz;
-          foot_xyz(j, fx, fy, fz);
-          node_pos(c.G, c.ox + fx, c.oy + fy, c.oz + fz, xn);
-          project(cols, fl, xn, v);
-        }
+        if (fl != -1) project(c.nrm, (int64_t)gt * 192 + l, fl, v);
       }
     }
     sd[0][i] = v[0];
@@ -379,11 +377,13 @@ struct NodeCtx {
   int T, l;     // global tile, local node
   int ni, nj, nk;
   bool valid;   // inside the body's reference grid
+  const float* nrm;  // Batch::nrm
 };
-__device__ __forceinline__ NodeCtx node_in(const BodyGrid& G, int body, int blk) {
+__device__ __forceinline__ NodeCtx node_in(const BodyGrid& G, int body, int blk, const float* nrm) {
   NodeCtx c;
   c.b = body;
   c.G = G;
+  c.nrm = nrm;
   c.i = (int64_t)blk * kBlock + threadIdx.x;
   c.T = (int)(c.i >> 6);
   c.l = (int)(c.i & 63);
@@ -397,7 +397,7 @@ __device__ __forceinline__ NodeCtx node_in(const BodyGrid& G, int body, int blk)
 }
 __device__ __forceinline__ NodeCtx node_begin(const Batch& B) {
   const int b = B.blockBody[blockIdx.x];
-  return node_in(B.bodies[b], b, blockIdx.x);
+  return node_in(B.bodies[b], b, blockIdx.x, B.nrm);
 }
 __device__ __forceinline__ void ld3(const float* v, int64_t i, float o[3]) {
   int64_t b = (i >> 6) * 192 + (i & 63);
@@ -411,11 +411,9 @@ __device__ __forceinline__ void st3(float* v, int64_t i, const float o[3]) {
   v[b + 64] = o[1];
   v[b + 128] = o[2];
 }
-__device__ __forceinline__ void node_project(const NodeCtx& c, const ColliderSet* cols, int fl, float v[3]) {
+__device__ __forceinline__ void node_project(const NodeCtx& c, const ColliderSet*, int fl, float v[3]) {
   if (fl == -1) return;
-  float x[3];
-  node_pos(c.G, c.ni, c.nj, c.nk, x);
-  project(cols, fl, x, v);
+  project(c.nrm, (c.i >> 6) * 192 + (c.i & 63), fl, v);
 }
 // ImplicitUpdateMatrix::multVector epilogue (Grid.cpp:764-770): P(m P d - dt^2 df)
 __device__ __forceinline__ void apply_A(const NodeCtx& c, const ColliderSet* cols, int fl, float m, const float d[3],
@@ -1360,7 +1358,8 @@ __global__ void k_node_forces(Batch B, const float* mass, const float* stage, co
 // + the sums of the initial guess (Grid.cpp:911-947)
 __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, const float* stage, const float* grav,
                                 int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                                CRScalars* sc, BatchStatus* bst, double* partials, unsigned char* tileActive) {
+                                CRScalars* sc, BatchStatus* bst, double* partials, unsigned char* tileActive,
+                                float* nrm) {
   NodeCtx c = node_begin(B);
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int bad = 0, active = 0;
@@ -1390,6 +1389,9 @@ __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, co
     e[2] = ev[2] * m;
     if (!isfinite(e[0] * e[1] * e[2])) bad = 1;
     if (fl >= 0) {
+      float n[3];
+      col_normal(cols->o[fl], x, n);
+      st3(nrm, c.i, n);
       col_velocity(cols->o[fl], x, cv);
       cv[0] -= c.G.fv[0];
       cv[1] -= c.G.fv[1];
@@ -1445,6 +1447,16 @@ __global__ void k_node_explicit(Batch B, const float* mass, const float* vel, co
   }
 }
 
+// the collider normals of nodes whose flags the caller set (Grid::m_nodeCollided written directly)
+__global__ void k_node_normals(Batch B, const int8_t* flags, const ColliderSet* cols, float* nrm) {
+  NodeCtx c = node_begin(B);
+  const int fl = flags[c.i];
+  if (fl < 0 || fl >= cols->n || !c.valid) return;
+  float x[3], n[3];
+  node_pos(c.G, c.ni, c.nj, c.nk, x);
+  col_normal(cols->o[fl], x, n);
+  st3(nrm, c.i, n);
+}
 __global__ void k_collision_velocities(Batch B, const int8_t* flags, const ColliderSet* cols, float* vc) {
   NodeCtx c = node_begin(B);
   int fl = flags[c.i];
@@ -1967,11 +1979,11 @@ template <int NV> struct SmallRed {
 };
 // every thread of the workgroup calls; returns the body total in every thread
 template <int NV, class Fn>
-__device__ __forceinline__ void small_sweep(const BodyGrid& G, int body, SmallRed<NV>& R, Fn&& fn,
+__device__ __forceinline__ void small_sweep(const BodyGrid& G, int body, const float* nrm, SmallRed<NV>& R, Fn&& fn,
                                             double (&tot)[NV]) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int j = 0; j < G.nblocks; ++j) {
-    NodeCtx c = node_in(G, body, G.blockBase + j);
+    NodeCtx c = node_in(G, body, G.blockBase + j, nrm);
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = 0.0;
@@ -2041,6 +2053,7 @@ __device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const B
     ChunkCtx c;
     c.ch = T.ch[q];
     c.G = G;
+    c.nrm = B.nrm;
     int tx, ty, tz;
     tile_xyz(G, c.ch.x - G.tileBase, tx, ty, tz);
     c.ox = tx * 4;
@@ -2089,7 +2102,7 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
 #pragma unroll
   for (int j = 0; j < kTinyBlocks; ++j) {
     if (j >= G.nblocks) break;
-    cn[j] = node_in(G, body, G.blockBase + j);
+    cn[j] = node_in(G, body, G.blockBase + j, B.nrm);
     ld3(x, cn[j].i, xr[j]);
     ld3(diag, cn[j].i, dgr[j]);
     mr[j] = mass[cn[j].i];
@@ -2218,11 +2231,12 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   // setup
   hvp(x);
   for (int j = 0; j < G.nblocks; ++j)
-    cr_setup1_node(B, node_in(G, body, G.blockBase + j), mass, flags, cols, stage, x, bvec, diag, dt2, z, p);
+    cr_setup1_node(B, node_in(G, body, G.blockBase + j, B.nrm), mass, flags, cols, stage, x, bvec, diag, dt2, z,
+                   p);
   __syncthreads();
   hvp(z);
   double t2[2];
-  small_sweep<2>(G, body, R2, [&](const NodeCtx& c, double (&v)[2]) {
+  small_sweep<2>(G, body, B.nrm, R2, [&](const NodeCtx& c, double (&v)[2]) {
     cr_setup2_node(B, c, mass, flags, cols, stage, z, diag, dt2, Ap, v);
   }, t2);
   if (threadIdx.x == 0) end = cr_setup2_finish(L, t2);
@@ -2231,12 +2245,12 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   for (int it = 0; !end; ++it) {
     const float alpha = (float)L.alpha;
     if (it == 0) {
-      small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+      small_sweep<1>(G, body, B.nrm, R1, [&](const NodeCtx& c, double (&v)[1]) {
         cr_update_x_node(c, flags, cols, p, Ap, diag, alpha, x, z, v);
       }, t1);
     } else {
       const float beta = (float)L.beta;
-      small_sweep<1>(G, body, R1, [&](const NodeCtx& c, double (&v)[1]) {
+      small_sweep<1>(G, body, B.nrm, R1, [&](const NodeCtx& c, double (&v)[1]) {
         cr_update_px_node(c, flags, cols, Ar, diag, beta, alpha, p, Ap, x, z, v);
       }, t1);
     }
@@ -2244,7 +2258,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
     __syncthreads();
     if (end) break;
     hvp(z);
-    small_sweep<3>(G, body, R3, [&](const NodeCtx& c, double (&v)[3]) {
+    small_sweep<3>(G, body, B.nrm, R3, [&](const NodeCtx& c, double (&v)[3]) {
       cr_ar_node(B, c, mass, flags, cols, stage, z, Ap, diag, dt2, Ar, v);
     }, t3);
     if (threadIdx.x == 0) end = cr_ar_finish(L, t3);
@@ -2959,10 +2973,13 @@ void launch_node_forces(const Batch& B, const float* mass, const float* stage, c
 }
 void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
                           int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* tileActive,
+                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* tileActive, float* nrm,
                           hipStream_t s) {
   NODE_LAUNCH(k_node_explicit, B, mass, vel, stage, grav, ngrav, dt, cols, em, flags, vc, sc, st, partials,
-              tileActive);
+              tileActive, nrm);
+}
+void launch_node_normals(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* nrm, hipStream_t s) {
+  NODE_LAUNCH(k_node_normals, B, flags, cols, nrm);
 }
 void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
                                  hipStream_t s) {

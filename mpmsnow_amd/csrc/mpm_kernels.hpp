@@ -84,10 +84,13 @@ void launch_gather3(const Batch& B, const float* stage, float* out, hipStream_t 
 void launch_node_forces(const Batch& B, const float* mass, const float* stage, const float* grav, int ngrav,
                         float* out, hipStream_t s);
 // also marks each tile active (some node with mass or a collider flag) in tileActive
+// also writes nrm (the collider's unit normal, 3-vector layout) at every node with a flag >= 0
 void launch_node_explicit(const Batch& B, const float* mass, const float* vel, const float* stage, const float* grav,
                           int ngrav, float dt, const ColliderSet* cols, float* em, int8_t* flags, float* vc,
-                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* tileActive,
+                          CRScalars* sc, BatchStatus* st, double* partials, unsigned char* tileActive, float* nrm,
                           hipStream_t s);
+// nrm at every node with a flag >= 0, for flags set by the caller
+void launch_node_normals(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* nrm, hipStream_t s);
 void launch_collision_velocities(const Batch& B, const int8_t* flags, const ColliderSet* cols, float* vc,
                                  hipStream_t s);
 void launch_rhs(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stageVc,

@@ -112,6 +112,7 @@ struct GridBatch {
       dense2;
   DevBuf bbox, mom, cnt;
   DevBuf tileAct;  // per tile: active in the CR loop (set by k_node_explicit)
+  DevBuf nrm;      // collider unit normal at nodes with a flag >= 0 (set with the flags)
   DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
   DevBuf smallList;               // bodies handed to the small-body solver
   std::vector<char> smallLayout;  // per body: laid out after the prefix (small-body solver candidate)
@@ -137,6 +138,7 @@ struct GridBatch {
     B.nblocks = nblocks;
     B.tickets = tickets.as<unsigned int>();
     B.tileActive = tileAct.as<unsigned char>();
+    B.nrm = nrm.as<float>();
     return B;
   }
   float* f(DevBuf& d) { return d.as<float>(); }
@@ -513,7 +515,7 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   mass.ensure(sizeof(float) * NN);
   flags.ensure(NN);
   flags2.ensure(NN);
-  for (DevBuf* d : {&vel, &prev, &vc, &b, &x, &z, &p, &Ap, &Ar, &diag, &tmp3}) d->ensure(vecBytes());
+  for (DevBuf* d : {&vel, &prev, &vc, &b, &x, &z, &p, &Ap, &Ar, &diag, &tmp3, &nrm}) d->ensure(vecBytes());
   scal.ensure(sizeof(CRScalars) * nb);
   status.ensure(sizeof(BatchStatus));
   partials.ensure(sizeof(double) * kPartialStride * std::max(nblocks, 1));
@@ -607,7 +609,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
   // explicit momenta, flags, vc, guess sums
   forces_stage();
   launch_node_explicit(B, f(mass), f(vel), f(stageA), c.grav(), c.ngrav(), dt, c.cols(), f(b), flags.as<int8_t>(),
-                       f(vc), sc(), st(), partials.as<double>(), tileAct.as<unsigned char>(), s);
+                       f(vc), sc(), st(), partials.as<double>(), tileAct.as<unsigned char>(), f(nrm), s);
   // DF(vc) (no input projection) and the diagonal, then rhs / guess / diagonal finish
   hvp(f(vc), false, f(stageA), nullptr, -1);
   int pr = c.prof_begin(PK_DIAG, this);
@@ -1344,6 +1346,7 @@ int mpm_grid_set(mpm_grid* grid, const char* name, const void* in) {
       B.dense2.ensure(nd);
       HIPCHECK(hipMemcpyAsync(B.dense2.p, in, nd, hipMemcpyHostToDevice, grid->ctx->stream));
       launch_from_dense_i8(B.dense2.as<int8_t>(), B.flags.as<int8_t>(), G, grid->ctx->stream);
+      launch_node_normals(B.view(), B.flags.as<int8_t>(), grid->ctx->cols(), B.f(B.nrm), grid->ctx->stream);
     } else
       throw InvalidArg("unknown grid variable '" + nm + "'");
     grid->ctx->sync();
@@ -1416,7 +1419,7 @@ int mpm_grid_explicit_momenta(mpm_grid* grid, float dt, float* momenta, int8_t* 
     // scratch outputs: the grid's own node flags are set only by updateGridVelocities
     launch_node_explicit(B.view(), B.f(B.mass), B.f(B.vel), B.f(B.stageA), c.grav(), c.ngrav(), dt, c.cols(),
                          B.f(B.tmp3), B.flags2.as<int8_t>(), B.f(B.Ar), B.sc(), B.st(), B.partials.as<double>(),
-                         B.tileAct.as<unsigned char>(), c.stream);
+                         B.tileAct.as<unsigned char>(), B.f(B.p), c.stream);  // normals: scratch (p)
     CRScalars hs;
     HIPCHECK(hipMemcpyAsync(&hs, B.scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     c.sync();
