@@ -798,11 +798,15 @@ __global__ void k_cc_roots(int n, int* parent, const int* hasNb, const int* orig
     if (have == 0ull) continue;
     int first = __ffsll((long long)have) - 1;
     int rr = __shfl(r, first, 64);
+    // minOrig only decreases: an atomic that a plain read shows cannot lower it is skipped (a
+    // stale read only costs an unneeded atomic), which keeps the thousands of waves of a large
+    // body from queueing on its root's word
     if (__all((r == rr) || (r < 0))) {
       for (int off = 32; off > 0; off >>= 1) o = min(o, __shfl_down(o, off, 64));
-      if ((threadIdx.x & 63) == 0) atomicMin(minOrig + rr, o);
+      if ((threadIdx.x & 63) == 0 && o < __hip_atomic_load(minOrig + rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(minOrig + rr, o);
     } else if (r >= 0) {
-      atomicMin(minOrig + r, o);
+      if (o < __hip_atomic_load(minOrig + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(minOrig + r, o);
     }
   }
 }
@@ -849,14 +853,25 @@ __device__ __forceinline__ void body_acc_reset(BodyAcc& a) {
 }
 __device__ __forceinline__ void body_acc_flush(const BodyAcc& a, int b, unsigned int* bbox, double* mom, int* count) {
   for (int q = 0; q < 3; ++q) {
-    atomicMin(bbox + 6 * b + q, a.mn[q]);
-    atomicMax(bbox + 6 * b + 3 + q, a.mx[q]);
+    // skipped when a plain read shows the bound already covers this block's
+    if (a.mn[q] < __hip_atomic_load(bbox + 6 * b + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(bbox + 6 * b + q, a.mn[q]);
+    if (a.mx[q] > __hip_atomic_load(bbox + 6 * b + 3 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(bbox + 6 * b + 3 + q, a.mx[q]);
   }
   for (int q = 0; q < 4; ++q) atomicAdd(mom + 4 * b + q, a.m[q]);
   atomicAdd(count + b, a.cnt);
 }
-__global__ void k_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
-                              int* count) {
+// Each block adds into replica blockIdx % kBodyReplicas of the accumulators (bbox, mom, count
+// point at replica 0, replica r at + r nb bodies): the hundreds of blocks of a large body would
+// otherwise queue their atomics on one cache line. k_body_combine folds the replicas in order.
+constexpr int kBodyReplicas = 16;
+__global__ void k_body_reduce(PView P, const int* slots, const int* labels, int n, int nb, unsigned int* bboxR,
+                              double* momR, int* countR) {
+  const int64_t rep = (int64_t)(blockIdx.x % kBodyReplicas) * nb;
+  unsigned int* bbox = bboxR + 6 * rep;
+  double* mom = momR + 4 * rep;
+  int* count = countR + rep;
   const int64_t per = ((int64_t)n + gridDim.x - 1) / gridDim.x;
   const int64_t beg = (int64_t)blockIdx.x * per;
   const int64_t end = beg + per < n ? beg + per : n;
@@ -921,6 +936,41 @@ __global__ void k_body_reduce(PView P, const int* slots, const int* labels, int 
       t.cnt += wacc[w].cnt;
     }
     body_acc_flush(t, blockBody, bbox, mom, count);
+  }
+}
+
+__global__ void k_body_init(int nrep, unsigned int* bboxR, double* momR, int* countR) {
+  GRID_STRIDE(i, nrep) {
+    for (int a = 0; a < 3; ++a) {
+      bboxR[6 * i + a] = 0xFFFFFFFFu;
+      bboxR[6 * i + 3 + a] = 0u;
+    }
+    for (int a = 0; a < 4; ++a) momR[4 * i + a] = 0.0;
+    countR[i] = 0;
+  }
+}
+__global__ void k_body_combine(int nb, const unsigned int* bboxR, const double* momR, const int* countR,
+                               unsigned int* bbox, double* mom, int* count) {
+  GRID_STRIDE(bi, nb) {
+    const int b = (int)bi;
+    unsigned int mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    int c = 0;
+    for (int r = 0; r < kBodyReplicas; ++r) {
+      const int64_t i = (int64_t)r * nb + b;
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = min(mn[a], bboxR[6 * i + a]);
+        mx[a] = max(mx[a], bboxR[6 * i + 3 + a]);
+      }
+      for (int a = 0; a < 4; ++a) m[a] += momR[4 * i + a];
+      c += countR[i];
+    }
+    for (int a = 0; a < 3; ++a) {
+      bbox[6 * b + a] = mn[a];
+      bbox[6 * b + 3 + a] = mx[a];
+    }
+    for (int a = 0; a < 4; ++a) mom[4 * b + a] = m[a];
+    count[b] = c;
   }
 }
 
@@ -2866,10 +2916,16 @@ void launch_cc_number(int n, const int* parent, const int* hasNb, const int* isR
 void launch_check_pvel(PView P, int n, int* err, hipStream_t s) {
   if (n > 0) LAUNCH(k_check_pvel, grid_for(n), P, n, err);
 }
-void launch_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
-                        int* count, hipStream_t s) {
-  if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 512), P, slots, labels, n, bbox, mom, count);
+void launch_body_reduce(PView P, const int* slots, const int* labels, int n, int nb, unsigned int* bbox, double* mom,
+                        int* count, unsigned int* bboxR, double* momR, int* countR, hipStream_t s) {
+  if (nb <= 0) return;
+  const int nrep = kBodyReplicas * nb;
+  LAUNCH(k_body_init, grid_for(nrep), nrep, bboxR, momR, countR);
+  if (n > 0) LAUNCH(k_body_reduce, grid_for(n, 512), P, slots, labels, n, nb, bboxR, momR, countR);
+  LAUNCH(k_body_combine, grid_for(nb), nb, (const unsigned int*)bboxR, (const double*)momR, (const int*)countR, bbox,
+         mom, count);
 }
+int body_replicas() { return kBodyReplicas; }
 void launch_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, int* tileBody, int* blockBody,
                       hipStream_t s) {
   if (NT > 0) LAUNCH(k_tile_body, grid_for(NT), bodies, order, nb, NT, tileBody, blockBody);

@@ -39,8 +39,11 @@ void launch_cc_number(int n, const int* parent, const int* hasNb, const int* isR
 void launch_check_pvel(PView P, int n, int* err, hipStream_t s);
 
 // ---- per-body reductions: bbox (ordered uint, 6/body), sum m v and sum m (double, 4/body), count
-void launch_body_reduce(PView P, const int* slots, const int* labels, int n, unsigned int* bbox, double* mom,
-                        int* count, hipStream_t s);
+// per body: bbox (ordered-int encoded min/max), FP64 sums of m v and m, particle count; bboxR,
+// momR, countR: body_replicas() copies of the same (scratch)
+void launch_body_reduce(PView P, const int* slots, const int* labels, int n, int nb, unsigned int* bbox, double* mom,
+                        int* count, unsigned int* bboxR, double* momR, int* countR, hipStream_t s);
+int body_replicas();
 
 // ---- binning / counting sort into the batch
 void launch_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, int* tileBody, int* blockBody,

@@ -110,7 +110,7 @@ struct GridBatch {
   DevBuf stageA, stageB;
   DevBuf mass, vel, prev, flags, vc, b, x, z, p, Ap, Ar, diag, tmp3, flags2, scal, status, partials, tickets, hist, dense,
       dense2;
-  DevBuf bbox, mom, cnt;
+  DevBuf bbox, mom, cnt, bboxR, momR, cntR;
   DevBuf tileAct;  // per tile: active in the CR loop (set by k_node_explicit)
   DevBuf nrm;      // collider unit normal at nodes with a flag >= 0 (set with the flags)
   DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
@@ -396,16 +396,12 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   bbox.ensure(sizeof(unsigned int) * 6 * nb);
   mom.ensure(sizeof(double) * 4 * nb);
   cnt.ensure(sizeof(int) * nb);
-  std::vector<unsigned int> binit(6 * nb);
-  for (int b = 0; b < nb; ++b)
-    for (int a = 0; a < 3; ++a) {
-      binit[6 * b + a] = 0xFFFFFFFFu;
-      binit[6 * b + 3 + a] = 0u;
-    }
-  HIPCHECK(hipMemcpyAsync(bbox.p, binit.data(), sizeof(unsigned int) * 6 * nb, hipMemcpyHostToDevice, s));
-  HIPCHECK(hipMemsetAsync(mom.p, 0, sizeof(double) * 4 * nb, s));
-  HIPCHECK(hipMemsetAsync(cnt.p, 0, sizeof(int) * nb, s));
-  launch_body_reduce(c.view(), slotList, labelList, n, bbox.as<unsigned int>(), mom.as<double>(), cnt.as<int>(), s);
+  const size_t nrep = (size_t)body_replicas() * nb;
+  bboxR.ensure(sizeof(unsigned int) * 6 * nrep);
+  momR.ensure(sizeof(double) * 4 * nrep);
+  cntR.ensure(sizeof(int) * nrep);
+  launch_body_reduce(c.view(), slotList, labelList, n, nb, bbox.as<unsigned int>(), mom.as<double>(), cnt.as<int>(),
+                     bboxR.as<unsigned int>(), momR.as<double>(), cntR.as<int>(), s);
   std::vector<unsigned int> hbox(6 * nb);
   std::vector<double> hmom(4 * nb);
   std::vector<int> hcnt(nb);
