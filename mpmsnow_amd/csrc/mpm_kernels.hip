@@ -654,9 +654,12 @@ __device__ __forceinline__ int cc_find_cached(const int* parent, int x) {
 // hook the root of higher priority under the other; priorities strictly fall towards a root
 __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
   {
-    // a cheap test that catches most pairs inside one already linked component
+    // cheap tests that catch most pairs inside one already linked component: direct parents,
+    // then the roots as the cache shows them (any pointer value ever held names a member of the
+    // same set, so equal cached roots prove one set; unequal ones fall through to the hook)
     const int pa = cc_load_cached(parent, a), pb = cc_load_cached(parent, b);
     if (pa == pb || pa == b || pb == a) return;
+    if (cc_find_cached(parent, pa) == cc_find_cached(parent, pb)) return;
   }
   while (true) {
     a = cc_find(parent, a);
