@@ -593,6 +593,27 @@ __device__ __forceinline__ void cell_of(const float x[3], float cell, int c[3]) 
 #pragma unroll
   for (int a = 0; a < 3; ++a) c[a] = (int)floorf(x[a] / cell);
 }
+// count[t]++ returning the old value, one atomic per group of lanes with the same t (particles in
+// storage order mostly share a tile with their neighbours, whose lanes would otherwise queue on
+// one counter); every active lane calls. Ranks within a tile follow arrival order either way:
+// k_sort_tiles fixes the final order.
+__device__ __forceinline__ int wave_aggregated_inc(int* count, int t) {
+  const int lane = threadIdx.x & 63;
+  int rank = 0;
+  for (;;) {
+    const int leader = __ffsll((long long)__ballot(1)) - 1;
+    const int lt = __shfl(t, leader, 64);
+    if (t == lt) {
+      const unsigned long long peers = __ballot(1);
+      int base = 0;
+      if (lane == leader) base = atomicAdd(count + t, __popcll(peers));
+      base = __shfl(base, leader, 64);
+      rank = base + __popcll(peers & ((1ull << lane) - 1ull));
+      break;
+    }
+  }
+  return rank;
+}
 __global__ void k_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count) {
   GRID_STRIDE(si, n) {
     int s = (int)si;
@@ -602,7 +623,7 @@ __global__ void k_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket
     cell_of(x, cell, c);
     uint32_t h = cell_hash(c[0], c[1], c[2], mask);
     bucket[s] = (int)h;
-    rank[s] = atomicAdd(count + h, 1);
+    rank[s] = wave_aggregated_inc(count, (int)h);  // the partition does not depend on this order
   }
 }
 // cell-sorted copy of the positions, slot in .w, so candidate scans read contiguous memory
@@ -1022,7 +1043,7 @@ __global__ void k_bin(PView P, const int* slots, const int* labels, int n, const
     }
     int t = G.tileBase + t3[0] + G.T[0] * (t3[1] + G.T[1] * t3[2]);
     key[k] = t | (code << kKeyCodeShift);  // the stencil base node within the tile, for k_sort_tiles
-    rank[k] = atomicAdd(count + t, 1);
+    rank[k] = wave_aggregated_inc(count, t);
   }
 }
 __global__ void k_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
