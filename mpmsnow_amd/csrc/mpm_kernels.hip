@@ -1565,14 +1565,11 @@ __device__ __forceinline__ HvpPre hvp_prefetch(const PView& P, const Batch& B, i
   }
   return q;
 }
-__device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B, const ChunkCtx& c,
-                                                  float (*sd)[kFootLds], FxFoot1<3>& acc, int* err,
-                                                  const HvpPre& pre) {
-  const int k = threadIdx.x;
-  bool have = k < c.ch.z;
-  float fm[9];
-  Stencil st;
-  int base = 0;
+// One particle's part of the HVP: stencil, gathered gradient G of the footprint sd, and
+// fm = V dP(G F) F^T (the 3x3 map whose rows scatter against the gradient weights); have is
+// cleared when the particle's stencil leaves the chunk's footprint.
+__device__ __forceinline__ void hvp_particle(const PView& P, const ChunkCtx& c, float (*sd)[kFootLds], int* err,
+                                             const HvpPre& pre, bool& have, Stencil& st, int& base, float fm[9]) {
   if (have) {
     const int s = pre.s;
     const float x[3] = {pre.x[0], pre.x[1], pre.x[2]};
@@ -1606,13 +1603,27 @@ __device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B
       mat_mul_bt(dP, F, fm);
     }
   }
-  // df_a(i) -= sum_c fm[a + 3 c] grad_c w_i = sum_c fm[a + 3 c] ghat_c(i)
-  float bound[3] = {0.0f, 0.0f, 0.0f};
+}
+// per-component bound on the particle's scatter contributions (the fixed-point scale's input)
+__device__ __forceinline__ void hvp_bound(const ChunkCtx& c, bool have, const float fm[9], float bound[3]) {
+  bound[0] = bound[1] = bound[2] = 0.0f;
   if (have) {
     const float gmax = kGradWeightMax / c.G.h;
 #pragma unroll
     for (int a = 0; a < 3; ++a) bound[a] = (fabsf(fm[a]) + fabsf(fm[a + 3]) + fabsf(fm[a + 6])) * gmax;
   }
+}
+__device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B, const ChunkCtx& c,
+                                                  float (*sd)[kFootLds], FxFoot1<3>& acc, int* err,
+                                                  const HvpPre& pre) {
+  bool have = (int)threadIdx.x < c.ch.z;
+  float fm[9];
+  Stencil st;
+  int base = 0;
+  hvp_particle(P, c, sd, err, pre, have, st, base, fm);
+  // df_a(i) -= sum_c fm[a + 3 c] grad_c w_i = sum_c fm[a + 3 c] ghat_c(i)
+  float bound[3];
+  hvp_bound(c, have, fm, bound);
   const FxScale3 scl = fx_scale3(bound, kFx1Top);
   if (have) {
 #pragma unroll
@@ -1655,6 +1666,98 @@ __global__ __launch_bounds__(kBlock) void k_hvp(PView P, Batch B, const float* d
   const HvpPre pre = hvp_prefetch(P, B, ch);
   if (sc && sc[ch.w].done) return;
   hvp_chunk<PROJ>(P, B, blockIdx.x, ch, pre, d, flags, cols, stage, err, sd, acc, ft);
+}
+
+// Paired HVP: a workgroup takes two consecutive chunks whose particles fit in one (at most
+// kChunk together), chunk A's particles on threads 0..nA-1 and chunk B's on nA..nA+nB-1, each
+// scattering into its own footprint. Sparse chunks (surface tiles, fragments) otherwise run
+// their particles in mostly idle waves: the kernel is bound by VALU issue. Each chunk's
+// fixed-point scale is computed from its particles' bounds placed at their index within the
+// chunk, in the same reduction tree as a chunk of its own, so every staged footprint is bitwise
+// that of k_hvp (and of the small-body solver's per-chunk HVP).
+// groups[i] = (chunk A, chunk B or -1); blocks from *ngroups on exit.
+__global__ void k_pair_chunks(const int4* chunks, int nch, int* cnt) {
+  GRID_STRIDE(i, (nch + 1) / 2) {
+    const int a = 2 * (int)i, b = a + 1;
+    cnt[i] = (b < nch && chunks[a].z + chunks[b].z <= kChunk) || b >= nch ? 1 : 2;
+  }
+}
+__global__ void k_pair_groups(const int4* chunks, int nch, const int* cnt, const int* pos, int2* groups) {
+  GRID_STRIDE(i, (nch + 1) / 2) {
+    const int a = 2 * (int)i, b = a + 1, o = pos[i];
+    if (cnt[i] == 1) {
+      groups[o] = make_int2(a, b < nch ? b : -1);
+    } else {
+      groups[o] = make_int2(a, -1);
+      groups[o + 1] = make_int2(b, -1);
+    }
+  }
+}
+template <bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_hvp_pair(PView P, Batch B, const int2* groups, const int* ngroups,
+                                                     const float* d, const int8_t* flags, const ColliderSet* cols,
+                                                     float* stage, const CRScalars* sc, int* err) {
+  if ((int)blockIdx.x >= *ngroups) return;
+  __shared__ float sd[2][3][kFootLds];
+  __shared__ FxFoot1<3> acc[2];
+  __shared__ int ft[2][8];
+  __shared__ float bnd[2][3][kBlock];
+  const int2 g = groups[blockIdx.x];
+  const int4 chA = B.chunks[g.x];
+  const bool pair = g.y >= 0;
+  const int4 chB = pair ? B.chunks[g.y] : make_int4(chA.x, chA.y, 0, chA.w);
+  // the threads' particles: chunk A on 0..nA-1, chunk B on nA..nA+nB-1
+  const int k = threadIdx.x;
+  const int cid = k < chA.z ? 0 : 1;
+  const int kk = cid ? k - chA.z : k;
+  HvpPre pre;
+  pre.s = -1;
+  pre.x[0] = pre.x[1] = pre.x[2] = 0.0f;
+  const int4 chM = cid ? chB : chA;
+  if (kk < chM.z) {
+    pre.s = B.perm[chM.y + kk];
+    load_x(P, pre.s, pre.x);
+  }
+  const bool doneA = sc && sc[chA.w].done;
+  const bool doneB = !pair || (sc && sc[chB.w].done);
+  if (doneA && doneB) return;
+  const ChunkCtx cA = chunk_begin_at(B, ft[0], chA);
+  const ChunkCtx cB = chunk_begin_at(B, ft[1], chB);
+  fx_zero(acc[0]);
+  if (pair) fx_zero(acc[1]);
+  __syncthreads();
+  if (!doneA) load_foot3(cA, ft[0], d, sd[0], flags, cols, PROJ);
+  if (!doneB) load_foot3(cB, ft[1], d, sd[1], flags, cols, PROJ);
+  __syncthreads();
+  const ChunkCtx& c = cid ? cB : cA;
+  bool have = kk < chM.z && !(cid ? doneB : doneA);
+  float fm[9];
+  Stencil st;
+  int base = 0;
+  hvp_particle(P, c, sd[cid], err, pre, have, st, base, fm);
+  float bound[3];
+  hvp_bound(c, have, fm, bound);
+  // bounds at each particle's index within its chunk; the rest zero
+  for (int i = k; i < 2 * 3 * kBlock; i += kBlock) (&bnd[0][0][0])[i] = 0.0f;
+  __syncthreads();
+  if (have)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bnd[cid][a][kk] = bound[a];
+  __syncthreads();
+  float bA[3] = {bnd[0][0][k], bnd[0][1][k], bnd[0][2][k]};
+  const FxScale3 sclA = fx_scale3(bA, kFx1Top);
+  __syncthreads();  // fx_scale3's reduction buffer is reused
+  float bB[3] = {bnd[1][0][k], bnd[1][1][k], bnd[1][2][k]};
+  const FxScale3 sclB = fx_scale3(bB, kFx1Top);
+  if (have) {
+    const FxScale3& scl = cid ? sclB : sclA;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      sep_scatter(&acc[cid].v[a][base], st, fm[a] * scl.S[a], fm[a + 3] * scl.S[a], fm[a + 6] * scl.S[a]);
+  }
+  __syncthreads();
+  if (!doneA) fx_store(acc[0], sclA, stage, g.x);
+  if (!doneB) fx_store(acc[1], sclB, stage, g.y);
 }
 
 // dFidXiSplatter (Grid.cpp:466-537) through per-particle quadratic forms: with g = F^T grad w,
@@ -3022,6 +3125,22 @@ void launch_hvp(PView P, const Batch& B, const float* d, bool project, const int
     LAUNCH(k_hvp<true>, B.nchunks, P, B, d, flags, cols, stage, sc, err);
   else
     LAUNCH(k_hvp<false>, B.nchunks, P, B, d, flags, cols, stage, sc, err);
+}
+void launch_pair_chunks(const Batch& B, int* cnt, int* pos, int2* groups, int* ngroups, int* tmp, hipStream_t s) {
+  const int npairs = (B.nchunks + 1) / 2;
+  if (npairs <= 0) return;
+  LAUNCH(k_pair_chunks, grid_for(npairs), B.chunks, B.nchunks, cnt);
+  scan_exclusive(cnt, pos, npairs, ngroups, tmp, s);
+  LAUNCH(k_pair_groups, grid_for(npairs), B.chunks, B.nchunks, (const int*)cnt, (const int*)pos, groups);
+}
+void launch_hvp_pair(PView P, const Batch& B, const int2* groups, const int* ngroups, const float* d, bool project,
+                     const int8_t* flags, const ColliderSet* cols, float* stage, const CRScalars* sc, int* err,
+                     hipStream_t s) {
+  if (B.nchunks <= 0) return;
+  if (project)
+    LAUNCH(k_hvp_pair<true>, B.nchunks, P, B, groups, ngroups, d, flags, cols, stage, sc, err);
+  else
+    LAUNCH(k_hvp_pair<false>, B.nchunks, P, B, groups, ngroups, d, flags, cols, stage, sc, err);
 }
 void launch_diag(PView P, const Batch& B, float* stage, int* err, hipStream_t s) {
   if (B.nchunks > 0) LAUNCH(k_diag, B.nchunks, P, B, stage, err);

@@ -73,6 +73,13 @@ void launch_forces(PView P, const Batch& B, float* stage, int* err, hipStream_t 
 // HVP: staged -sum_p V dP(dF(d)) F^T grad w (d projected first when project = true); sc may be null
 void launch_hvp(PView P, const Batch& B, const float* d, bool project, const int8_t* flags, const ColliderSet* cols,
                 float* stage, const CRScalars* sc, int* err, hipStream_t s);
+// pairs of consecutive chunks of B that fit one workgroup (groups: up to B.nchunks int2, *ngroups
+// device; cnt, pos: (B.nchunks + 1) / 2 ints; tmp: scan_tmp_ints of that), and the HVP over them
+// (grid B.nchunks; blocks past *ngroups exit): the same staged footprints as launch_hvp
+void launch_pair_chunks(const Batch& B, int* cnt, int* pos, int2* groups, int* ngroups, int* tmp, hipStream_t s);
+void launch_hvp_pair(PView P, const Batch& B, const int2* groups, const int* ngroups, const float* d, bool project,
+                     const int8_t* flags, const ColliderSet* cols, float* stage, const CRScalars* sc, int* err,
+                     hipStream_t s);
 // raw diagonal of the force differential (dForceidXi)
 void launch_diag(PView P, const Batch& B, float* stage, int* err, hipStream_t s);
 // G2P: mode 0 = velocities only, 1 = F only, 2 = full step (v, F, plasticity; advection is

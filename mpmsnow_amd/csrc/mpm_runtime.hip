@@ -112,6 +112,7 @@ struct GridBatch {
       dense2;
   DevBuf bbox, mom, cnt, bboxR, momR, cntR;
   DevBuf tileAct;  // per tile: active in the CR loop (set by k_node_explicit)
+  DevBuf pairCnt, pairPos, groups;  // chunk pairs of the CR loop's HVP
   DevBuf nrm;      // collider unit normal at nodes with a flag >= 0 (set with the flags)
   DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
   DevBuf smallList;               // bodies handed to the small-body solver
@@ -151,7 +152,10 @@ struct GridBatch {
              bool indexOrder = false);
   void forces_stage();
   void hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter);
-  void hvp(const Batch& Bv, const float* d, bool project, float* stage, const CRScalars* scp, int iter);
+  // paired: over the chunk pairs of the last pair_chunks (built for this Bv)
+  void hvp(const Batch& Bv, const float* d, bool project, float* stage, const CRScalars* scp, int iter,
+           bool paired = false);
+  void pair_chunks(const Batch& Bv);
   void solve(float dt, const mpm_solver_params* prm);
   void train_lpc(float dt, CRScalars* hsc, size_t scb);
   void ensure_levels() {
@@ -561,10 +565,24 @@ void GridBatch::forces_stage() {
 void GridBatch::hvp(const float* d, bool project, float* stage, const CRScalars* scp, int iter) {
   hvp(view(), d, project, stage, scp, iter);
 }
-void GridBatch::hvp(const Batch& Bv, const float* d, bool project, float* stage, const CRScalars* scp, int iter) {
+void GridBatch::hvp(const Batch& Bv, const float* d, bool project, float* stage, const CRScalars* scp, int iter,
+                    bool paired) {
   int pr = ctx->prof_begin(PK_HVP, this, iter);
-  launch_hvp(ctx->view(), Bv, d, project, flags.as<int8_t>(), ctx->cols(), stage, scp, ctx->errp(), ctx->stream);
+  if (paired)
+    launch_hvp_pair(ctx->view(), Bv, groups.as<int2>(), totals.as<int>() + 2, d, project, flags.as<int8_t>(),
+                    ctx->cols(), stage, scp, ctx->errp(), ctx->stream);
+  else
+    launch_hvp(ctx->view(), Bv, d, project, flags.as<int8_t>(), ctx->cols(), stage, scp, ctx->errp(), ctx->stream);
   ctx->prof_end(pr);
+}
+void GridBatch::pair_chunks(const Batch& Bv) {
+  const int npairs = std::max(1, (Bv.nchunks + 1) / 2);
+  pairCnt.ensure(sizeof(int) * npairs);
+  pairPos.ensure(sizeof(int) * npairs);
+  groups.ensure(sizeof(int2) * std::max(1, Bv.nchunks));
+  scantmp.ensure(sizeof(int) * scan_tmp_ints(npairs));
+  launch_pair_chunks(Bv, pairCnt.as<int>(), pairPos.as<int>(), groups.as<int2>(), totals.as<int>() + 2,
+                     scantmp.as<int>(), ctx->stream);
 }
 
 void GridBatch::fetch_history(int body, SolveRecord& r) {
@@ -699,11 +717,12 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
       Bb.nblocks = prefixTiles / 4;
       Bb.nchunks = prefixChunks;
     }
-    // CR setup (ConjugateResiduals.cpp:50-76)
-    hvp(Bb, f(x), true, f(stageA), sc(), kProfSetup);
+    // CR setup (ConjugateResiduals.cpp:50-76); the HVPs of the solve run over chunk pairs
+    pair_chunks(Bb);
+    hvp(Bb, f(x), true, f(stageA), sc(), kProfSetup, true);
     launch_cr_setup1(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
                      s);
-    hvp(Bb, f(z), true, f(stageA), sc(), kProfSetup);
+    hvp(Bb, f(z), true, f(stageA), sc(), kProfSetup, true);
     launch_cr_setup2(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
                      partials.as<double>(), s);
     fetch_status_errors();
@@ -742,7 +761,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
       if (capWant > 0) snapshot(it + 1);
     };
     auto iterate_rest = [&](int it) {
-      hvp(Bb, f(z), true, f(stageA), sc(), it);
+      hvp(Bb, f(z), true, f(stageA), sc(), it, true);
       int pa = c.prof_begin(PK_CR_AR, this, it);
       launch_cr_ar(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(Ap), f(diag), dt, f(Ar), sc(), st(),
                    partials.as<double>(), s);
