@@ -162,7 +162,7 @@ def pmc_summary(fetch_csv, write_csv, out_path):
             raise SystemExit("no %s dispatch in %s (run bench.py with --copy-calibration)" % (calib, path))
         copy_kib = max(copies)
         factor = copy_kib * 1024.0 / COPY_BYTES
-        hvp = [(g, v) for n, g, v in rows if "k_hvp<true>" in n]
+        hvp = [(g, v) for n, g, v in rows if "k_hvp_pair<true>" in n]
         # launches of the steps after the warm-up (the store is body- and tile-sorted from the
         # first step on), full grid (every chunk launched): the median of those
         hvp = hvp[len(hvp) * 3 // 5:]
@@ -172,7 +172,7 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         res[counter] = {"copy_kib": copy_kib, "factor": factor, "hvp_kib_per_launch": kib,
                         "hvp_bytes_per_launch": kib * 1024.0 / factor, "launches": len(full)}
     traffic = res["FETCH_SIZE"]["hvp_bytes_per_launch"] + res["WRITE_SIZE"]["hvp_bytes_per_launch"]
-    summary = {"kernel": "k_hvp<true>", "workload": "two_snowballs_1M", "traffic_bytes_per_launch": traffic,
+    summary = {"kernel": "k_hvp_pair<true>", "workload": "two_snowballs_1M", "traffic_bytes_per_launch": traffic,
                "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
     lines = [
         "# HVP HBM traffic, two snowballs (1M particles)",
@@ -187,7 +187,7 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         % (COPY_BYTES >> 20),
         "torch's sum over it (reads it once), WRITE_SIZE by the device-to-device copy into it",
         "(`__amd_rocclr_copyBuffer`, writes it once): factor = counter KiB x 1024 / buffer bytes.",
-        "HVP figures: median over the full-grid `k_hvp<true>`",
+        "HVP figures: median over the full-grid `k_hvp_pair<true>` (the CR loop's HVP)",
         "dispatches (every chunk launched) of the last 40% of HVP dispatches (the timed and table steps).",
         "",
         "| counter | calibration KiB | factor | HVP KiB/launch | HVP bytes/launch (scaled) |",
@@ -223,8 +223,8 @@ def trace_summary(stats_csv, bench_log, out_path):
             bench = json.loads(line)
     if bench is None:
         raise SystemExit("no bench JSON line in %s" % bench_log)
-    hvp_calls = sum(c for n, c, t, _ in rows if n.startswith("k_hvp<"))
-    hvp_ns = sum(t for n, c, t, _ in rows if n.startswith("k_hvp<"))
+    hvp_calls = sum(c for n, c, t, _ in rows if n.startswith("k_hvp"))
+    hvp_ns = sum(t for n, c, t, _ in rows if n.startswith("k_hvp"))
     rp_ms = hvp_ns / hvp_calls / 1e6
     ev_ms = bench["roofline"]["ms_per_launch"]
     steps, warmup = bench["steps"], bench["warmup"]
@@ -241,7 +241,8 @@ def trace_summary(stats_csv, bench_log, out_path):
         "table. bench.py line of that run: %.4g particle-steps/s, %.3f ms/step (under the profiler)."
         % (warmup, steps, bench["value"], bench["ms_per_step"]),
         "",
-        "HVP (`k_hvp<true>` and `k_hvp<false>`, every launch of the run): %d launches, mean %.4f ms "
+        "HVP (`k_hvp_pair<true>`, the CR loop's, and `k_hvp<false>`, every launch of the run): %d launches, "
+        "mean %.4f ms "
         "(rocprofv3); bench.py's HIP events over its timed steps (one HVP launch in %d): %d launches, mean "
         "%.4f ms (ratio %.3f)."
         % (hvp_calls, rp_ms, HVP_SAMPLE, bench["roofline"]["launches"], ev_ms,
