@@ -409,7 +409,8 @@ MPM_HD void vol_sample(const Collider& o, const float* f, int nc, const float x[
   for (int a = 0; a < 3; ++a) {
     float u = (x[a] - o.origin[a]) / o.voxel;
     const float hi = (float)(o.dims[a] - 1);
-    u = u < 0.0f ? 0.0f : (u > hi ? hi : u);  // also maps NaN to 0 (u < 0 and u > hi are false)
+    if (!(u >= 0.0f)) u = 0.0f;  // below the volume, or NaN
+    if (u > hi) u = hi;
     int i = (int)floorf(u);
     if (i > o.dims[a] - 2) i = o.dims[a] - 2;
     i0[a] = i;

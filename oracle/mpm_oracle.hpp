@@ -401,7 +401,8 @@ template <class S> struct Collider {
     for (int a = 0; a < 3; ++a) {
       S u = (x[a] - origin[a]) / voxel;
       const S hi = S(dims[a] - 1);
-      u = u < S(0) ? S(0) : (u > hi ? hi : u);
+      if (!(u >= S(0))) u = S(0);  // below the volume, or NaN
+      if (u > hi) u = hi;
       int i = int(std::floor(u));
       if (i > dims[a] - 2) i = dims[a] - 2;
       i0[a] = i;
