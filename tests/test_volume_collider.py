@@ -107,3 +107,14 @@ def test_sphere_volume_close_to_the_sphere(oracle):
             # normals differ by the central difference error, O(voxel^2 / r^2)
             assert np.abs(v1 - v2).max() <= 1e-2 * max(1.0, np.abs(v2).max())
     assert agree == total
+
+
+def test_nan_position_samples_inside_the_volume(oracle):
+    """A NaN position reads the volume's first point (never outside the array): with phi > 0
+    there, no collision."""
+    phi = np.ones((3, 3, 3))
+    for prec in ("float", "double"):
+        vol = oracle.Colliders.create(prec)
+        vol.add_volume(phi, (0, 0, 0), 0.5)
+        flag, _ = vol.collide(np.array([0.0, -1.0, 0.0]), np.array([np.nan, 0.2, np.nan]))
+        assert flag == -1
