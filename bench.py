@@ -72,20 +72,30 @@ def snapshot(sim):
     return {k: sim.get(k) for k in STATE_FIELDS}
 
 
-def cpu_baseline(sc, state, steps=1):
-    """Time the reference-faithful CPU oracle for `steps` step(s) from a state of the GPU run."""
+def oracle_sim(sc, state, prec="float"):
+    """The CPU oracle's Sim on the scene's colliders and gravity, at a state of the GPU run."""
     from oracle import pyoracle as O
 
     mat = sc["material"]
-    cols = O.Colliders.create()
+    cols = O.Colliders.create(prec)
     for c, v, mu, st in sc["planes"]:
         cols.add_plane(c, v=v, friction=mu, sticky=st)
-    f = O.Fields.create()
+    f = O.Fields.create(prec)
     f.add_gravity(sc["gravity"])
-    model = O.Model.snow(mat["E"], mat["nu"], mat["xi"], mat["theta_c"], mat["theta_s"])
-    sim = O.Sim(state["p"], sc["m"], sc["h"], model, cols, f)
+    model = O.Model.snow(mat["E"], mat["nu"], mat["xi"], mat["theta_c"], mat["theta_s"], prec)
+    sim = O.Sim(state["p"], sc["m"], sc["h"], model, cols, f, prec=prec)
     for name in STATE_FIELDS[1:]:
         sim.pd.set(name, state[name])
+    sim._keep = (cols, f, model)
+    return sim
+
+
+def cpu_baseline(sc, state, steps=1):
+    """Time the reference-faithful CPU oracle for `steps` step(s) from a state of the GPU run;
+    returns the result line and the oracle's particle state after those steps."""
+    from oracle import pyoracle as O
+
+    sim = oracle_sim(sc, state)
     progress("cpu baseline: oracle set up, timing %d step(s)" % steps)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -94,10 +104,41 @@ def cpu_baseline(sc, state, steps=1):
     progress("cpu baseline: %.1f s" % dt)
     n = len(sc["m"])
     its = [s["iterations"] for s in sim.last_stats()]
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
+    line = {"value": n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
             "sample": "%d step(s) of the same %d-particle scene from the GPU run's state at the start of its "
                       "timed steps (%.1f s; CR iterations of its two largest bodies %s; later steps of the "
                       "timed run take more iterations)" % (steps, n, dt, sorted(its)[-2:])}
+    return line, {"p": sim.pd.get("p"), "F": sim.pd.get("F")}
+
+
+def error_vs_fp64(sc, state, o32, steps):
+    """Part of the CPU-baseline leg, the oracle as checker: one GPU step and one FP64 oracle step
+    from the CPU sample's state; position and F deviations of the GPU and of the FP32 oracle
+    (the sample's own result) from FP64."""
+    import numpy as np
+
+    import mpmsnow_amd as G
+
+    progress("error check: FP64 oracle, %d step(s)" % steps)
+    o64 = oracle_sim(sc, state, "double")
+    for _ in range(steps):
+        o64.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    ref = {"p": o64.pd.get("p"), "F": o64.pd.get("F")}
+    sim = G.Sim(state["p"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
+                gravity=[sc["gravity"]], v=state["v"])
+    for name in STATE_FIELDS[1:]:
+        sim.ctx.set(name, state[name])
+    for _ in range(steps):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    gpu = {"p": sim.get("p"), "F": sim.get("F")}
+    n = len(sc["m"])
+    out = {"steps": steps, "note": "from the CPU sample's state; positions in units of h"}
+    for who, st in (("gpu", gpu), ("fp32_oracle", o32)):
+        dx = np.abs(st["p"].reshape(n, 3) - ref["p"].reshape(n, 3)).max(1) / sc["h"]
+        dF = np.abs(st["F"].reshape(n, 9) - ref["F"].reshape(n, 9)).max(1)
+        out[who] = {"max_dx_over_h": float(dx.max()), "p99_dx_over_h": float(np.percentile(dx, 99)),
+                    "max_dF": float(dF.max()), "p99_dF": float(np.percentile(dF, 99))}
+    return out
 
 
 def learned_leg(steps, warmup):
@@ -384,8 +425,9 @@ def main():
         "kernels": kernels,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, start_state, args.cpu_steps)
+        out["cpu_baseline"], o32 = cpu_baseline(sc, start_state, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["error_vs_fp64"] = error_vs_fp64(sc, start_state, o32, args.cpu_steps)
     if not args.no_learned:
         out["learned_pc"] = learned_leg(args.learned_steps, 3)
     print(json.dumps(out))
