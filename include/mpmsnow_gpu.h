@@ -1,7 +1,7 @@
 /* mpmsnow_gpu.h -- C-ABI of the MI355X (gfx950) implicit MPM snow step.
  *
  * This is the drop-in boundary under the MpmSim library interfaces of
- * davidsminor/mpmsnow (include/MpmSim/*.h in the reference checkout): every
+ * davidsminor/mpmsnow (the headers in include/MpmSim of the reference): every
  * entry point below carries out one member function of those classes on the
  * GPU, and is cited to the reference declaration it replaces. Plain C types
  * only: float/int arrays in the reference's host layouts (Vector3f = 3
