@@ -342,19 +342,46 @@ __device__ __forceinline__ float gather_node(const Batch& B, const BodyGrid& G, 
   return v;
 }
 // tc0 / tcn: first chunk and chunk count of each tile from global tile tileOff on; stage holds
-// the chunks from the one numbered tc0 = 0 on
+// the chunks from the one numbered tc0 = 0 on. The chunk tables of all 8 source tiles and the
+// first chunk of each are loaded before any sum (most tiles have one chunk), so a node waits for
+// two rounds of memory latency instead of sixteen; the sums still run source tile by source
+// tile and chunk by chunk, in the same order as a plain double loop.
 __device__ __forceinline__ void gather3_tab(const BodyGrid& G, const int* tc0, const int* tcn, int tileOff,
                                             const float* stage, int T, int l, float o[3]) {
   int tx, ty, tz;
   tile_xyz(G, T - G.tileBase, tx, ty, tz);
+  int c0[8], cn[8];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    const int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
+    c0[f] = 0;
+    cn[f] = 0;
+    if (ts >= 0) {
+      c0[f] = tc0[ts - tileOff];
+      cn[f] = tcn[ts - tileOff];
+    }
+  }
+  float v[8][3];
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    v[f][0] = v[f][1] = v[f][2] = 0.0f;
+    if (cn[f] > 0) {
+      const float* s = stage + (int64_t)c0[f] * kStage + f * 192 + l;
+      v[f][0] = s[0];
+      v[f][1] = s[64];
+      v[f][2] = s[128];
+    }
+  }
   float a = 0.0f, b = 0.0f, cc = 0.0f;
 #pragma unroll
   for (int f = 0; f < 8; ++f) {
-    int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
-    if (ts < 0) continue;
-    int c0 = tc0[ts - tileOff], cn = tcn[ts - tileOff];
-    for (int q = 0; q < cn; ++q) {
-      const float* s = stage + (int64_t)(c0 + q) * kStage + f * 192 + l;
+    if (cn[f] > 0) {
+      a += v[f][0];
+      b += v[f][1];
+      cc += v[f][2];
+    }
+    for (int q = 1; q < cn[f]; ++q) {
+      const float* s = stage + (int64_t)(c0[f] + q) * kStage + f * 192 + l;
       a += s[0];
       b += s[64];
       cc += s[128];
