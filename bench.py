@@ -25,6 +25,12 @@ mpmsnow_amd/lpc_train.py) against the plain diagonal CR, same states, same
 1e-5 residual test.
 
 Other modes:
+  --stiff              only the stiff stress case (16M particles in 8 seeded boxes and
+                       spheres dropped onto the ground plane and an analytic sphere
+                       collider, E0 = 4.8e5 Pa, h = 1/512, dt = 2e-3): plain vs learned CR
+                       from the same warmed-up state, per-step CR iterations, fallback
+                       triggers and capped solves; one JSON line (--stiff-particles,
+                       --stiff-steps, --stiff-warmup)
   --copy-calibration   also runs one 256 MiB device-to-device copy first (the
                        known-size kernel that scales the PMC byte counters)
   --pmc-summary FETCH_CSV WRITE_CSV [--pmc-out FILE]
@@ -181,6 +187,62 @@ def learned_leg(steps, warmup):
     return out
 
 
+def stiff_leg(n, steps, warmup, seed=4):
+    """Plain diagonal CR vs learned flexible CR on the stiff stress case (8 seeded boxes and spheres
+    dropped onto the ground plane and an analytic sphere collider, E0 = 4.8e5 Pa, h = 1/512,
+    dt = 2e-3), both from the state after `warmup` plain steps. Reports per-step CR iterations (sum
+    and max over the step's bodies) and the learned solver's fallback triggers (iterations redone
+    with the diagonal, summed over bodies)."""
+    import numpy as np
+    import torch
+
+    import mpmsnow_amd as G
+    from mpmsnow_amd import scenes
+
+    sc = scenes.stiff_drop(n=n, seed=seed)
+    w = np.load(os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
+    out = {}
+    for mode, name in ((0, "plain"), (1, "learned")):
+        sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
+                    spheres=sc["spheres"], gravity=[sc["gravity"]], v=sc["v"])
+        sim.ctx.lpc_set_params(w)
+        progress("stiff leg: %s, %d particles set up" % (name, n))
+        for k in range(warmup):
+            st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+            progress("stiff leg: %s warm-up step %d, %d bodies, CR iterations sum %d max %d"
+                     % (name, k, st["num_bodies"], st["total_iterations"], st["max_iterations"]))
+        sim.ctx.lpc_mode(mode)
+        rows = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+            torch.cuda.synchronize()
+            te = time.perf_counter()
+            bs = [sim.body_stats(b, hist_cap=0) for b in range(st["num_solved"])]
+            rows.append({"ms": (te - ts) * 1e3, "bodies": st["num_bodies"], "iters_sum": st["total_iterations"],
+                         "iters_max": st["max_iterations"], "fallbacks": int(sum(b["fallbacks"] for b in bs)),
+                         "capped": int(sum(b["iterations"] >= sc["max_iters"] for b in bs))})
+            progress("stiff leg: %s step %d: %.1f ms, %d bodies, CR iterations sum %d max %d, fallbacks %d"
+                     % (name, len(rows), rows[-1]["ms"], st["num_bodies"], st["total_iterations"],
+                        st["max_iterations"], rows[-1]["fallbacks"]))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"value": n * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "cr_iters_per_step_sum": float(np.mean([r["iters_sum"] for r in rows])),
+                     "cr_iters_per_step_max": float(np.mean([r["iters_max"] for r in rows])),
+                     "fallbacks_per_step": float(np.mean([r["fallbacks"] for r in rows])),
+                     "capped_solves": int(sum(r["capped"] for r in rows)), "steps": rows}
+        del sim
+    out["speedup"] = out["learned"]["value"] / out["plain"]["value"]
+    out["config"] = ("stiff stress case: %d particles in 8 seeded boxes and spheres (seed %d), plane + analytic "
+                     "sphere collider, E0=4.8e5, xi=10, h=1/512, dt=2e-3, CR tol 1e-5, %d steps after %d plain "
+                     "warm-up steps" % (n, seed, steps, warmup))
+    out["objects"] = sc["objects"]
+    return out
+
+
 def _pmc_rows(path):
     """(kernel name, grid size, counter value) per dispatch of a rocprofv3 counter_collection.csv."""
     rows = []
@@ -319,6 +381,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-steps", type=int, default=5)
+    ap.add_argument("--stiff", action="store_true",
+                    help="run only the stiff stress case (plain vs learned CR) and print its JSON line")
+    ap.add_argument("--stiff-particles", type=int, default=16000000)
+    ap.add_argument("--stiff-steps", type=int, default=5)
+    ap.add_argument("--stiff-warmup", type=int, default=12)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
@@ -335,6 +402,10 @@ def main():
         print("bench.py measures one GPU only: this build runs the step on a single MI355X "
               "(multi-GPU sharding is a design note, not implemented); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.stiff:
+        print(json.dumps(stiff_leg(args.stiff_particles, args.stiff_steps, args.stiff_warmup)))
+        return
 
     import numpy as np
 

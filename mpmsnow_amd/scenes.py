@@ -115,3 +115,61 @@ def slope_slab(n=4000000, seed=2, size=(0.8, 0.2, 0.4)):
     return dict(x=x, m=m, v=np.zeros((n, 3), dtype=np.float32), h=1.0 / 256, dt=5e-4,
                 gravity=(g * s30, -g * c30, 0.0), planes=[((0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.2, False)],
                 tol=1e-5, max_iters=1000, material=dict(SNOW), mu0=mu0, lambda0=lam0, xi=xi.astype(np.float32))
+
+
+def stiff_drop(n=16000000, seed=4, h=1.0 / 512, v0=1.0):
+    """Stiff stress case: 8 seeded boxes and spheres dropped onto the plane y=0 and an analytic sphere.
+
+    Objects sit on a 4 x 2 layout (x centres 0.15 + i * 0.2333, z centres 0.33 / 0.67, +/- 5 mm
+    jitter); each slot is a box (sides uniform in [0.10, 0.17] m) or a sphere (radius uniform in
+    [0.06, 0.095] m), chosen by the seed. The sphere collider (centre (0.5, -0.05, 0.5), r = 0.25,
+    friction 0.2) lies under the four inner slots; the outer four land on the ground plane
+    (friction 0.2). Each object's lowest point starts 5-20 mm above the highest support point under
+    its footprint and moves down at v0 (1 m/s), so every object is in contact within the first
+    dozen steps of the 100. Particles per object are proportional to its volume (uniform density
+    400 kg/m^3, about 6 particles per cell at h = 1/512). Snow with E0 = 4.8e5 Pa, xi = 10;
+    h = 1/512, dt = 2e-3 (c dt / h = 35: stiff). Returns the scene dict plus `objects` (kind,
+    centre, size, count per object).
+    """
+    u = splitmix64_uniform(seed, 64)
+    sc_c, sc_r = np.array([0.5, -0.05, 0.5]), 0.25
+    specs = []
+    for s in range(8):
+        i, kz = s % 4, s // 4
+        cx = 0.15 + i * (0.7 / 3.0) + 0.01 * (u[8 * s] - 0.5)
+        cz = (0.33 if kz == 0 else 0.67) + 0.01 * (u[8 * s + 1] - 0.5)
+        if u[8 * s + 2] < 0.5:
+            half = 0.5 * (0.10 + 0.07 * u[8 * s + 3:8 * s + 6])
+            kind, vol, hx, hz = "box", float(np.prod(2 * half)), half[0], half[2]
+        else:
+            r = 0.06 + 0.035 * u[8 * s + 3]
+            half = np.array([r, r, r])
+            kind, vol, hx, hz = "sphere", 4.0 / 3.0 * np.pi * r ** 3, r, r
+        # highest support under the footprint: the collider sphere at the footprint point nearest its axis
+        qx = min(max(sc_c[0], cx - hx), cx + hx)
+        qz = min(max(sc_c[2], cz - hz), cz + hz)
+        d2 = (qx - sc_c[0]) ** 2 + (qz - sc_c[2]) ** 2
+        support = max(0.0, sc_c[1] + np.sqrt(sc_r ** 2 - d2)) if d2 < sc_r ** 2 else 0.0
+        cy = support + 0.005 + 0.015 * u[8 * s + 6] + half[1]
+        specs.append((kind, (cx, cy, cz), half, vol))
+    vols = np.array([sp[3] for sp in specs])
+    counts = np.floor(n * vols / vols.sum()).astype(np.int64)
+    counts[np.argmax(vols)] += n - counts.sum()
+    xs, objects = [], []
+    for s, ((kind, c, half, vol), k) in enumerate(zip(specs, counts)):
+        if kind == "box":
+            xs.append(box(seed * 100 + s, int(k), np.array(c) - half, np.array(c) + half))
+        else:
+            xs.append(sphere(seed * 100 + s, int(k), c, float(half[0])))
+        objects.append(dict(kind=kind, center=tuple(float(t) for t in c), size=tuple(float(t) for t in 2 * half),
+                            count=int(k)))
+    x = np.concatenate(xs)
+    m = np.concatenate([np.full(int(k), RHO * sp[3] / k, dtype=np.float32) for sp, k in zip(specs, counts)])
+    v = np.zeros((n, 3), dtype=np.float32)
+    v[:, 1] = -v0
+    mat = dict(SNOW)
+    mat["E"] = 4.8e5
+    return dict(x=x, m=m, v=v, h=h, dt=2e-3, gravity=(0.0, -9.81, 0.0),
+                planes=[((0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.2, False)],
+                spheres=[(tuple(sc_c), sc_r, (0.0, 0.0, 0.0), 0.2, False)], tol=1e-5, max_iters=1000,
+                material=mat, objects=objects)
