@@ -147,7 +147,7 @@ def error_vs_fp64(sc, state, o32, steps):
     return out
 
 
-def learned_leg(steps, warmup):
+def learned_leg(steps, warmup, weights=None, fallback=None):
     """Plain diagonal CR vs learned flexible CR on the 4M slope slab, from identical states."""
     import numpy as np
     import torch
@@ -156,12 +156,14 @@ def learned_leg(steps, warmup):
     from mpmsnow_amd import scenes
 
     sc = scenes.slope_slab(n=4000000, seed=2)
-    w = np.load(os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
+    w = np.load(weights or os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
     out = {}
     for mode, name in ((0, "plain"), (1, "learned")):
         sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
                     mu0=sc["mu0"], lambda0=sc["lambda0"], xi=sc["xi"])
         sim.ctx.lpc_set_params(w)
+        if fallback is not None:
+            sim.ctx.lpc_fallback(fallback)
         progress("learned leg: %s, 4M slab set up" % name)
         for _ in range(warmup):
             sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
@@ -187,7 +189,7 @@ def learned_leg(steps, warmup):
     return out
 
 
-def stiff_leg(n, steps, warmup, seed=4):
+def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None):
     """Plain diagonal CR vs learned flexible CR on the stiff stress case (8 seeded boxes and spheres
     dropped onto the ground plane and an analytic sphere collider, E0 = 4.8e5 Pa, h = 1/512,
     dt = 2e-3), both from the state after `warmup` plain steps. Reports per-step CR iterations (sum
@@ -200,12 +202,14 @@ def stiff_leg(n, steps, warmup, seed=4):
     from mpmsnow_amd import scenes
 
     sc = scenes.stiff_drop(n=n, seed=seed)
-    w = np.load(os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
+    w = np.load(weights or os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
     out = {}
     for mode, name in ((0, "plain"), (1, "learned")):
         sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
                     spheres=sc["spheres"], gravity=[sc["gravity"]], v=sc["v"])
         sim.ctx.lpc_set_params(w)
+        if fallback is not None:
+            sim.ctx.lpc_fallback(fallback)
         progress("stiff leg: %s, %d particles set up" % (name, n))
         for k in range(warmup):
             st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
@@ -386,6 +390,8 @@ def main():
     ap.add_argument("--stiff-particles", type=int, default=16000000)
     ap.add_argument("--stiff-steps", type=int, default=5)
     ap.add_argument("--stiff-warmup", type=int, default=12)
+    ap.add_argument("--lpc-weights", default=None, help="network parameters (.npy) for the learned legs")
+    ap.add_argument("--lpc-fallback", type=float, default=None, help="fallback ratio of the learned legs")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
@@ -404,7 +410,8 @@ def main():
         sys.exit(2)
 
     if args.stiff:
-        print(json.dumps(stiff_leg(args.stiff_particles, args.stiff_steps, args.stiff_warmup)))
+        print(json.dumps(stiff_leg(args.stiff_particles, args.stiff_steps, args.stiff_warmup,
+                                   weights=args.lpc_weights, fallback=args.lpc_fallback)))
         return
 
     import numpy as np
@@ -500,7 +507,7 @@ def main():
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
         out["error_vs_fp64"] = error_vs_fp64(sc, start_state, o32, args.cpu_steps)
     if not args.no_learned:
-        out["learned_pc"] = learned_leg(args.learned_steps, 3)
+        out["learned_pc"] = learned_leg(args.learned_steps, 3, args.lpc_weights, args.lpc_fallback)
     print(json.dumps(out))
 
 

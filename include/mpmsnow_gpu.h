@@ -185,8 +185,10 @@ int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* 
  * A BF16 3-D convolutional network (MFMA, FP32 accumulation) maps a body's grid residual to
  * a velocity correction inside a flexible conjugate-residual solve; it attaches where
  * ConjugateResiduals takes a preconditioner (ConjugateResiduals.h:20, ProceduralMatrix.h:16).
- * A body whose residual fails to fall in an iteration that used the network switches to the
- * diagonal preconditioner for the rest of its solve; the termination test is unchanged. */
+ * An iteration that used the network and did not lower |r|^2 below `ratio` times its previous
+ * value (mpm_lpc_set_fallback; 0.95 by default) is a fallback: the body takes its next iteration
+ * with the diagonal preconditioner, and after three fallbacks keeps the diagonal for the rest of
+ * its solve; the termination test is unchanged. */
 int64_t mpm_lpc_num_params(void);
 /* random (He) initialisation; the output layer starts at zero */
 int mpm_lpc_init(mpm_context* ctx, uint64_t seed);
@@ -194,6 +196,8 @@ int mpm_lpc_set_params(mpm_context* ctx, const float* params, int64_t n);
 int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n);
 /* 0 = diagonal-preconditioned CR (the reference's solver), 1 = learned flexible CR */
 int mpm_lpc_set_mode(mpm_context* ctx, int mode);
+/* fallback ratio in (0, 1]: 1 = a fallback only when |r|^2 rises */
+int mpm_lpc_set_fallback(mpm_context* ctx, double ratio);
 /* training: the next mpm_sim_advance (diagonal CR) keeps up to `samples` residuals
  * r_k = b - A x_k of its solve (at chosen iterations k) and, before the particle update, takes
  * `epochs` Adam passes over them with the library's forward and backward kernels, minimising

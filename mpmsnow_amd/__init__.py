@@ -102,6 +102,7 @@ SIGNATURES = {
     "mpm_lpc_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
     "mpm_lpc_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
     "mpm_lpc_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_lpc_set_fallback": (C.c_int, [C.c_void_p, C.c_double]),
     "mpm_lpc_train_next": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
     "mpm_lpc_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
@@ -327,6 +328,9 @@ class Context:
 
     def lpc_mode(self, mode):
         self.check(lib().mpm_lpc_set_mode(self.ptr, int(mode)))
+
+    def lpc_fallback(self, ratio):
+        self.check(lib().mpm_lpc_set_fallback(self.ptr, float(ratio)))
 
     def small_bodies(self, max_particles):
         """Bodies of at most 8 node blocks and this many particles are solved in one workgroup

@@ -155,6 +155,7 @@ struct CRScalars {
   double ApAp[2];          // <A p_j, A p_j> of the stored directions
   double rAp;              // <r, A p> of the new direction
   double rr_prev;          // |r|^2 before the last update
+  double lpc_ratio;        // a network iteration whose |r|^2 exceeds lpc_ratio * rr_prev is a fallback
   double lpc_s2;           // sum over active nodes of (r / diag)^2, and their count (3 per node)
   double lpc_cnt;
   float lpc_scale;         // RMS of r / diag over active nodes: network input scale

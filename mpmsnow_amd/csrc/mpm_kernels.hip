@@ -2482,15 +2482,15 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
 // preconditioner z = r / diag this is the preconditioned conjugate residual iteration of
 // ConjugateResiduals.cpp (same Krylov space, same minimised norm). The same termination
 // (|r|^2 < tol^2 |b|^2 or the iteration cap, SquareMagnitudeTermination.cpp:19-28) judges
-// the result. Whenever an iteration that used the network fails to lower |r|^2 by the factor
-// kLpcFallback, the body takes its next iteration with the diagonal preconditioner; after
-// kLpcStrikes such fallbacks it keeps the diagonal for the rest of the solve.
-constexpr double kLpcFallback = 0.95;
+// the result. Whenever an iteration that used the network fails to lower |r|^2 by the body's
+// fallback ratio (the context's setting: 0.95 by default, 1 = any rise), the body takes its
+// next iteration with the diagonal preconditioner; after kLpcStrikes such fallbacks it keeps
+// the diagonal for the rest of the solve.
 
 // r = b - A x0 ; network input scale ; iteration state
 __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                            const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                           int netOn, CRScalars* sc, BatchStatus* bst, double* partials) {
+                           int netOn, double ratio, CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
@@ -2517,6 +2517,7 @@ __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, cons
   if (body_sum<3>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0) {
     S.rnorm2 = tot[0];
     S.rr_prev = tot[0];
+    S.lpc_ratio = ratio;
     S.lpc_s2 = tot[1];
     S.lpc_cnt = tot[2];
     S.lpc_scale = tot[2] > 0.0 ? (float)sqrt(tot[1] / tot[2]) : 0.0f;
@@ -2679,7 +2680,7 @@ __global__ void k_fcr_update(Batch B, const int8_t* flags, const ColliderSet* co
     S.lpc_scale = tot[2] > 0.0 ? (float)sqrt(tot[1] / tot[2]) : 0.0f;
     if (it >= S.max_iters || tot[0] < S.threshold) {
       set_done(S, bst);
-    } else if (!S.lpc_off && tot[0] > kLpcFallback * S.rr_prev) {
+    } else if (!S.lpc_off && tot[0] > S.lpc_ratio * S.rr_prev) {
       S.lpc_off = 1;
       S.fallbacks += 1;
       S.lpc_pause = S.fallbacks < kLpcStrikes ? 1 : 0;
@@ -3257,8 +3258,8 @@ void launch_small_cr(PView P, const Batch& B, const int* list, int n, const floa
 }
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                     int netOn, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, sc, st, partials);
+                     int netOn, double ratio, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, ratio, sc, st, partials);
 }
 void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
                        hipStream_t s) {

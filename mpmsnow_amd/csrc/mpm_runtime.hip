@@ -227,6 +227,7 @@ struct mpm_context {
   // learned preconditioner
   LpcNet net;
   int lpcMode = 0;  // 0: diagonal-preconditioned CR; 1: learned flexible CR
+  double lpcFallback = 0.95;  // fallback ratio of a network iteration's |r|^2 to the previous one
   LpcCapture cap;
   // measurement
   struct ProfRec {
@@ -651,7 +652,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     for (DevBuf* d : {&fr, &fp0, &fp1, &fq0, &fq1}) d->ensure(vecBytes());
     const int netOn = c.net.initialised ? 1 : 0;
     hvp(f(x), true, f(stageA), sc(), -1);
-    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, sc(),
+    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, c.lpcFallback, sc(),
                     st(), partials.as<double>(), s);
     fetch_status_errors();
     bool netActive = netOn != 0;
@@ -1642,6 +1643,13 @@ int mpm_lpc_set_mode(mpm_context* ctx, int mode) {
   return guarded(ctx, [&] {
     if (mode != 0 && mode != 1) throw InvalidArg("mode must be 0 (diagonal CR) or 1 (learned flexible CR)");
     ctx->lpcMode = mode;
+  });
+}
+
+int mpm_lpc_set_fallback(mpm_context* ctx, double ratio) {
+  return guarded(ctx, [&] {
+    if (!(ratio > 0.0 && ratio <= 1.0)) throw InvalidArg("fallback ratio must lie in (0, 1]");
+    ctx->lpcFallback = ratio;
   });
 }
 

@@ -31,9 +31,17 @@ from mpmsnow_amd import scenes  # noqa: E402
 DEFAULT_OUT = os.path.join(HERE, "lpc_weights.npy")
 
 
-def make_scene(k, rng):
-    """Scene k of the training curriculum."""
+def make_scene(k, rng, curriculum="mixed"):
+    """Scene k of the training curriculum: "mixed" alternates snowball pairs and slope slabs,
+    "stiff" draws stiff drops (8 boxes and spheres on the plane and the sphere collider,
+    E0 = 4.8e5 Pa, h = 1/512, dt = 2e-3) of 6-10M particles."""
     seed = 1000 + k
+    if curriculum == "stiff":
+        n = int(rng.integers(6, 11)) * 1000000
+        sc = scenes.stiff_drop(n=n, seed=seed)
+        sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], spheres=sc["spheres"],
+                    gravity=[sc["gravity"]], v=sc["v"])
+        return "stiff drop n=%d" % n, sc, sim, int(rng.integers(8, 16))
     if k % 2 == 0:
         n = int(rng.integers(5, 11)) * 100000
         sc = scenes.two_snowballs(n=n, seed=seed, r=float(rng.uniform(0.1, 0.16)), speed=float(rng.uniform(1.5, 4.0)),
@@ -49,13 +57,14 @@ def make_scene(k, rng):
     return "slab n=%d size=%s" % (n, tuple(round(v, 2) for v in size)), sc, sim, 2
 
 
-def train(out=DEFAULT_OUT, scenes_n=24, captures=4, epochs=8, lr=2e-4, seed=0, init=None):
+def train(out=DEFAULT_OUT, scenes_n=24, captures=4, epochs=8, lr=2e-4, seed=0, init=None, curriculum="mixed",
+          decay=40.0, samples=12):
     rng = np.random.default_rng(seed)
     params = None if init is None else np.load(init, allow_pickle=False)
     step = 0
     t0 = time.time()
     for k in range(scenes_n):
-        name, sc, sim, warm = make_scene(k, rng)
+        name, sc, sim, warm = make_scene(k, rng, curriculum)
         if params is None:
             sim.ctx.lpc_init(seed + 1)
         else:
@@ -63,8 +72,8 @@ def train(out=DEFAULT_OUT, scenes_n=24, captures=4, epochs=8, lr=2e-4, seed=0, i
         for _ in range(warm):
             sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
         for c in range(captures):
-            rate = lr * (0.5 ** (step / 40.0))
-            sim.ctx.lpc_train_next(12, epochs, rate)
+            rate = lr * (0.5 ** (step / decay))
+            sim.ctx.lpc_train_next(samples, epochs, rate)
             sim.advance(sc["dt"], 300, 1e-7)
             first, last, diag = sim.ctx.lpc_last_loss()
             step += 1
@@ -87,8 +96,11 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
+    ap.add_argument("--curriculum", choices=("mixed", "stiff"), default="mixed")
+    ap.add_argument("--decay", type=float, default=40.0, help="learning-rate half-life in training steps")
+    ap.add_argument("--samples", type=int, default=12, help="residuals kept per captured solve (at most 16)")
     a = ap.parse_args()
-    train(a.out, a.scenes, a.captures, a.epochs, a.lr, a.seed, a.init)
+    train(a.out, a.scenes, a.captures, a.epochs, a.lr, a.seed, a.init, a.curriculum, a.decay, a.samples)
 
 
 if __name__ == "__main__":

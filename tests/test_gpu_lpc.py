@@ -167,6 +167,30 @@ def test_bad_network_falls_back_to_diagonal():
     assert big["residual_norm2_last"] < np.float64(sc["tol"]) ** 2 * big["bnorm2"]
 
 
+def test_fallback_ratio_setting():
+    """The fallback ratio is validated (0 < ratio <= 1); at ratio 1 (a fallback only when |r|^2
+    rises) a bad network still ends on the diagonal within three fallbacks and meets the test."""
+    say("fcr: fallback ratio 1.0, contact snowball 20k, random network")
+    sc, sim = _small_scene(seed=5)
+    for bad in (0.0, 1.5):
+        with pytest.raises(G.MpmError):
+            sim.ctx.lpc_fallback(bad)
+    sim.ctx.lpc_fallback(1.0)
+    for _ in range(2):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    p = random_params(13)
+    offs, _ = L.layer_offsets()
+    w, b = offs[5]
+    p[w:w + 27 * 16 * 16] *= 50.0
+    sim.ctx.lpc_set_params(p)
+    sim.ctx.lpc_mode(1)
+    st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    big, _ = _largest(sim, st)
+    say("largest body: %d iterations, fallbacks %d" % (big["iterations"], big["fallbacks"]))
+    assert 1 <= big["fallbacks"] <= 3
+    assert big["residual_norm2_last"] < np.float64(sc["tol"]) ** 2 * big["bnorm2"]
+
+
 def test_ablation_trained_network_against_plain_cr():
     """What the trained network gains: a stiffer contact snowball (E = 1.4e6 Pa) is run twice from
     identical states; in one run the network is trained in-solve on two steps and the next steps
