@@ -1,6 +1,6 @@
 """GPU parity: the gfx950 library (through its C-ABI) against the CPU oracle on identical seeded inputs.
 
-Bars (north-star): cell binning and node indexing bit-exact; per-phase grid
+Bars: cell binning and node indexing bit-exact; per-phase grid
 quantities, CR residual histories and particle F within 1e-5 relative (FP32
 bulk against the FP64 oracle, scaled by the array's magnitude); positions
 within 1e-4 h after multi-step runs. Printed case names precede every GPU step.
@@ -352,7 +352,7 @@ def oracle_sim(oracle, prec, x, m, v, h, planes, gravity, mat):
     return sim
 
 
-def test_snowball_drop_100_steps_north_star(oracle):
+def test_snowball_drop_100_steps_against_fp64(oracle):
     """Snowball drop (config 0 geometry at 4000 particles, no ground contact within 100 steps):
     positions within 1e-4 h and F within 1e-5 of the FP64 oracle after 100 steps."""
     from mpmsnow_amd import scenes
