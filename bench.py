@@ -14,6 +14,9 @@ events on the library's stream (only HVP launches are instrumented there), again
 MI355X's 8 TB/s HBM peak. roofline.traffic = measured HBM bytes per HVP launch,
 read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
 (profiles/hvp_two_snowballs_1M_traffic.md, made by `--pmc-summary`).
+hvp_issue = what binds the HVP instead: its VALU-issue and LDS-active fractions of
+the kernel's clocks, read from the committed rocprofv3 SQ counter summary
+(profiles/hvp_two_snowballs_1M_sq_counters.md).
 cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the
 8-colour voxel partition like the reference's TBB splats) timed on this host
 for a bounded sample: one step of the same 1M-particle scene from the GPU run's
@@ -311,6 +314,71 @@ def pmc_summary(fetch_csv, write_csv, out_path):
     print(json.dumps(summary))
 
 
+SQ_PASSES = (
+    "SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES SQ_INSTS_VALU SQ_INSTS_LDS SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS "
+    "SQ_LDS_BANK_CONFLICT GRBM_GUI_ACTIVE",
+    "SQ_LDS_ADDR_CONFLICT SQ_WAIT_INST_LDS SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
+    "SQ_ACTIVE_INST_ANY SQ_INSTS_SALU GRBM_GUI_ACTIVE",
+    "SQ_LDS_UNALIGNED_STALL SQ_LDS_IDX_ACTIVE SQ_LDS_DATA_FIFO_FULL SQ_LDS_CMD_FIFO_FULL SQ_INST_LEVEL_LDS "
+    "SQ_ACCUM_PREV_HIRES GRBM_GUI_ACTIVE",
+)
+
+
+def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>"):
+    """Mean per-dispatch SQ/LDS counters of the CR loop's HVP from the three `rocprofv3 --pmc` passes
+    of SQ_PASSES, and what they imply on MI355X (256 CUs, 1024 SIMDs, 32 shader engines; a wave64
+    VALU instruction occupies its SIMD for 4 clocks)."""
+    vals = {}
+    for path in csv_paths:
+        per = {}
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if kernel not in r["Kernel_Name"]:
+                    continue
+                d = int(r.get("Dispatch_Id") or r["Correlation_Id"])
+                per.setdefault(d, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+        ids = sorted(per)[len(per) * 3 // 5:]  # the steps after the warm-up
+        if not ids:
+            raise SystemExit("no %s dispatch in %s" % (kernel, path))
+        for name in per[ids[0]]:
+            if name not in vals:
+                vals[name] = sum(per[d].get(name, 0.0) for d in ids) / len(ids)
+    clocks = vals["SQ_BUSY_CYCLES"] / 32.0
+    valu = vals["SQ_INSTS_VALU"] * 4.0 / 1024.0
+    lds = vals["SQ_LDS_IDX_ACTIVE"] / 256.0
+    bank = vals["SQ_LDS_BANK_CONFLICT"] / 256.0
+    addr = vals["SQ_LDS_ADDR_CONFLICT"] / 256.0
+    resident = vals["SQ_WAVE_CYCLES"] * 4.0 / 256.0 / clocks
+    summary = {"kernel": kernel, "workload": "two_snowballs_1M", "valu_issue_frac": round(valu / clocks, 3),
+               "lds_active_frac": round(lds / clocks, 3), "resident_waves_per_cu": round(resident, 1)}
+    lines = ["# HVP SQ / LDS counters, two snowballs (1M particles)", "",
+             "Commands (from the repository root on the GPU host), one counter pass each:", ""]
+    for k, p in enumerate(SQ_PASSES):
+        lines.append("    rocprofv3 --pmc %s --output-format csv -d <P%d> -o run -- python bench.py --steps 2 "
+                     "--warmup 1 --no-cpu --no-learned" % (p, k))
+    lines += ["    python bench.py --sq-summary <P0>/run_counter_collection.csv <P1>/run_counter_collection.csv "
+              "<P2>/run_counter_collection.csv", "",
+              "Values: mean per `%s` dispatch over the last 40%% of the run's dispatches of that kernel, "
+              "summed over the device as rocprofv3 reports them." % kernel, "",
+              "| counter | per dispatch |", "|---|---|"]
+    for name in sorted(vals):
+        lines.append("| %s | %.4g |" % (name, vals[name]))
+    lines += ["", "Derived (256 CUs, 1024 SIMDs, 32 shader engines; a wave64 VALU instruction occupies its SIMD "
+              "for 4 clocks; SQ_WAVE_CYCLES read as counting 4-clock units):", "",
+              "* kernel length = SQ_BUSY_CYCLES / 32 SEs = %.0f k clocks" % (clocks / 1e3),
+              "* VALU issue: SQ_INSTS_VALU x 4 clocks / 1024 SIMDs = %.0f k clocks = %.0f%% of the kernel"
+              % (valu / 1e3, 100 * valu / clocks),
+              "* LDS active per CU: SQ_LDS_IDX_ACTIVE / 256 = %.0f k clocks = %.0f%%; of it bank conflicts %.0f k "
+              "(%.0f%%), address conflicts %.0f k (%.0f%%)" % (lds / 1e3, 100 * lds / clocks, bank / 1e3,
+                                                              100 * bank / clocks, addr / 1e3, 100 * addr / clocks),
+              "* resident waves per CU: SQ_WAVE_CYCLES x 4 / 256 / kernel clocks = %.1f" % resident,
+              "* VALU instructions per wave: %.0f" % (vals["SQ_INSTS_VALU"] / vals["SQ_WAVES"]),
+              "", "<!-- json: %s -->" % json.dumps(summary)]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(json.dumps(summary))
+
+
 def _short(name):
     name = name.replace("mpm::(anonymous namespace)::", "").replace("void ", "")
     return name.split("(")[0]
@@ -368,6 +436,17 @@ def trace_summary(stats_csv, bench_log, out_path):
     print(json.dumps(summary))
 
 
+SQ_FILE = os.path.join(ROOT, "profiles", "hvp_two_snowballs_1M_sq_counters.md")
+
+
+def read_sq():
+    """VALU-issue and LDS-active fractions of the HVP's clocks from the committed SQ counter summary."""
+    if not os.path.exists(SQ_FILE):
+        return None
+    m = re.search(r"<!-- json: (.*) -->", open(SQ_FILE).read())
+    return json.loads(m.group(1)) if m else None
+
+
 def read_traffic():
     if not os.path.exists(TRAFFIC_FILE):
         return None
@@ -396,10 +475,15 @@ def main():
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_CSV", "BENCH_LOG"))
+    ap.add_argument("--sq-summary", nargs=3, metavar=("PASS0_CSV", "PASS1_CSV", "PASS2_CSV"))
+    ap.add_argument("--sq-out", default=SQ_FILE)
     ap.add_argument("--trace-out", default=STATS_FILE)
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.pmc_out)
+        return
+    if args.sq_summary:
+        sq_summary(args.sq_summary, args.sq_out)
         return
     if args.trace_summary:
         trace_summary(args.trace_summary[0], args.trace_summary[1], args.trace_out)
@@ -500,6 +584,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "hvp", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": hvp_bytes, "ms_per_launch": hvp_ms, "launches": hvp["launches"]},
+        "hvp_issue": read_sq(),
         "kernels": kernels,
     }
     if not args.no_cpu:
