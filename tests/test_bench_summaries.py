@@ -1,0 +1,62 @@
+"""bench.py's rocprofv3 summarizers on synthetic counter files (CPU): the traffic summary scales the
+HVP's FETCH_SIZE / WRITE_SIZE by the known-size calibration kernels, and the SQ summary averages the
+HVP's dispatches of the last 40% and derives the issue fractions with the stated formulas."""
+import csv
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import bench  # noqa: E402
+
+
+def write_csv(path, rows):
+    with open(path, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(["Correlation_Id", "Dispatch_Id", "Kernel_Name", "Grid_Size", "Counter_Name", "Counter_Value"])
+        for r in rows:
+            w.writerow(r)
+
+
+def footer(path):
+    return json.loads(re.search(r"<!-- json: (.*) -->", open(path).read()).group(1))
+
+
+def test_pmc_summary_scales_by_calibration(tmp_path):
+    mib = bench.COPY_BYTES / 1024.0
+    for counter, calib, factor in (("FETCH_SIZE", "reduce_kernel", 0.5), ("WRITE_SIZE", "copyBuffer", 1.0)):
+        rows = [(0, 0, "at::native::" + calib, 1, counter, factor * mib)]
+        rows += [(d, d, "void mpm::k_hvp_pair<true>(x)", 4096, counter, 1000.0 * factor) for d in range(1, 11)]
+        write_csv(str(tmp_path / (counter + ".csv")), rows)
+    out = str(tmp_path / "traffic.md")
+    bench.pmc_summary(str(tmp_path / "FETCH_SIZE.csv"), str(tmp_path / "WRITE_SIZE.csv"), out)
+    s = footer(out)
+    # 1000 KiB true traffic for each counter after scaling
+    assert abs(s["fetch"]["hvp_bytes_per_launch"] - 1000.0 * 1024) < 1e-6
+    assert abs(s["traffic_bytes_per_launch"] - 2000.0 * 1024) < 1e-6
+
+
+def test_sq_summary_fractions(tmp_path):
+    paths = []
+    vals = {"SQ_BUSY_CYCLES": 32 * 1000.0, "SQ_INSTS_VALU": 1024 * 600 / 4.0, "SQ_LDS_IDX_ACTIVE": 256 * 500.0,
+            "SQ_LDS_BANK_CONFLICT": 256 * 100.0, "SQ_LDS_ADDR_CONFLICT": 256 * 50.0,
+            "SQ_WAVE_CYCLES": 256 * 1000.0 * 18 / 4.0, "SQ_WAVES": 100.0}
+    for k, p in enumerate(bench.SQ_PASSES):
+        rows = []
+        for d in range(10):
+            # the early dispatches (first 60%) carry different values and must be ignored
+            scale = 3.0 if d < 6 else 1.0
+            for name in p.split():
+                rows.append((d, d, "void mpm::k_hvp_pair<true>(x)", 1000, name, scale * vals.get(name, 1.0)))
+        path = str(tmp_path / ("p%d.csv" % k))
+        write_csv(path, rows)
+        paths.append(path)
+    out = str(tmp_path / "sq.md")
+    bench.sq_summary(paths, out)
+    s = footer(out)
+    assert s["valu_issue_frac"] == 0.6
+    assert s["lds_active_frac"] == 0.5
+    assert s["resident_waves_per_cu"] == 18.0
