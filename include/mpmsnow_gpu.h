@@ -198,6 +198,9 @@ int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n);
 int mpm_lpc_set_mode(mpm_context* ctx, int mode);
 /* fallback ratio in (0, 1]: 1 = a fallback only when |r|^2 rises */
 int mpm_lpc_set_fallback(mpm_context* ctx, double ratio);
+/* CR iterations at which training solves keep their residuals: 1-16 increasing indices
+ * (default 0 1 2 3 4 6 8 11 15 20 26 33 41 50 60 72); a solve keeps the first `samples` of them */
+int mpm_lpc_set_capture_iterations(mpm_context* ctx, const int32_t* iters, int n);
 /* training: the next mpm_sim_advance (diagonal CR) keeps up to `samples` residuals
  * r_k = b - A x_k of its solve (at chosen iterations k) and, before the particle update, takes
  * `epochs` Adam passes over them with the library's forward and backward kernels, minimising

@@ -181,6 +181,7 @@ struct LpcCapture {
   float lr = 0.0f;
   int n = 0;                     // samples kept
   std::vector<int> iters;        // CR iteration of each sample
+  std::vector<int> schedule = {0, 1, 2, 3, 4, 6, 8, 11, 15, 20, 26, 33, 41, 50, 60, 72};  // iterations to keep
   DevBuf data;                   // [sample][3 NN floats]
   DevBuf scale, weight, sums, loss;  // [sample][body]
   int64_t NN = 0;
@@ -727,10 +728,10 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     launch_cr_setup2(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ap), sc(), st(),
                      partials.as<double>(), s);
     fetch_status_errors();
-    // training pairs: iterate x_k and residual r_k = M z_k at the iterations of kCaptureIters
-    static const int kCaptureIters[] = {0, 1, 2, 3, 4, 6, 8, 11, 15, 20, 26, 33, 41, 50, 60, 72};
+    // training pairs: iterate x_k and residual r_k = M z_k at the iterations of the capture schedule
     LpcCapture& cap = c.cap;
-    const int capWant = hooks ? 0 : std::min<int>(cap.want, (int)(sizeof(kCaptureIters) / sizeof(int)));
+    const std::vector<int>& kCaptureIters = cap.schedule;
+    const int capWant = hooks ? 0 : std::min<int>(cap.want, (int)kCaptureIters.size());
     if (capWant > 0) {
       cap.NN = NN;
       cap.data.ensure(sizeof(float) * 3 * (size_t)NN * capWant);
@@ -1650,6 +1651,16 @@ int mpm_lpc_set_fallback(mpm_context* ctx, double ratio) {
   return guarded(ctx, [&] {
     if (!(ratio > 0.0 && ratio <= 1.0)) throw InvalidArg("fallback ratio must lie in (0, 1]");
     ctx->lpcFallback = ratio;
+  });
+}
+
+int mpm_lpc_set_capture_iterations(mpm_context* ctx, const int32_t* iters, int n) {
+  return guarded(ctx, [&] {
+    if (!iters || n < 1 || n > 16) throw InvalidArg("1 to 16 capture iterations");
+    for (int k = 0; k < n; ++k)
+      if (iters[k] < 0 || (k > 0 && iters[k] <= iters[k - 1]))
+        throw InvalidArg("capture iterations must be non-negative and increasing");
+    ctx->cap.schedule.assign(iters, iters + n);
   });
 }
 

@@ -58,13 +58,15 @@ def make_scene(k, rng, curriculum="mixed"):
 
 
 def train(out=DEFAULT_OUT, scenes_n=24, captures=4, epochs=8, lr=2e-4, seed=0, init=None, curriculum="mixed",
-          decay=40.0, samples=12):
+          decay=40.0, samples=12, capture_iters=None):
     rng = np.random.default_rng(seed)
     params = None if init is None else np.load(init, allow_pickle=False)
     step = 0
     t0 = time.time()
     for k in range(scenes_n):
         name, sc, sim, warm = make_scene(k, rng, curriculum)
+        if capture_iters:
+            sim.ctx.lpc_capture_iterations(capture_iters)
         if params is None:
             sim.ctx.lpc_init(seed + 1)
         else:
@@ -99,8 +101,11 @@ def main():
     ap.add_argument("--curriculum", choices=("mixed", "stiff"), default="mixed")
     ap.add_argument("--decay", type=float, default=40.0, help="learning-rate half-life in training steps")
     ap.add_argument("--samples", type=int, default=12, help="residuals kept per captured solve (at most 16)")
+    ap.add_argument("--capture-iters", default=None,
+                    help="comma-separated CR iterations whose residuals a training solve keeps")
     a = ap.parse_args()
-    train(a.out, a.scenes, a.captures, a.epochs, a.lr, a.seed, a.init, a.curriculum, a.decay, a.samples)
+    its = [int(t) for t in a.capture_iters.split(",")] if a.capture_iters else None
+    train(a.out, a.scenes, a.captures, a.epochs, a.lr, a.seed, a.init, a.curriculum, a.decay, a.samples, its)
 
 
 if __name__ == "__main__":

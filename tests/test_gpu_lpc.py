@@ -119,6 +119,25 @@ def test_training_reduces_the_residual_loss_below_the_diagonal_step():
     assert np.isfinite(last) and last < first and last < diag
 
 
+def test_capture_iterations_setting():
+    """The capture schedule is validated (1-16 increasing non-negative iterations); a training solve
+    with a late schedule keeps residuals only at iterations it reached and trains on them."""
+    say("train: capture schedule, contact snowball 20k")
+    sc, sim = _small_scene()
+    for bad in ([], [3, 3], [-1, 2], list(range(17))):
+        with pytest.raises(G.MpmError):
+            sim.ctx.lpc_capture_iterations(bad)
+    for _ in range(3):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    sim.ctx.lpc_capture_iterations([0, 5, 10, 20, 40, 80, 160])
+    sim.ctx.lpc_init(5)
+    sim.ctx.lpc_train_next(7, 4, 1e-4)
+    sim.advance(sc["dt"], 400, 1e-7)
+    first, last, diag = sim.ctx.lpc_last_loss()
+    say("residual loss %.4f -> %.4f (diagonal step %.4f)" % (first, last, diag))
+    assert np.isfinite(first) and np.isfinite(last) and 0.0 <= diag <= 1.0
+
+
 def _largest(sim, st):
     rows = [sim.body_stats(b, hist_cap=0) for b in range(st["num_solved"])]
     return max(rows, key=lambda r: r["particles"]), rows
