@@ -1,0 +1,74 @@
+"""The reference's own grid tests with their own assertions, run on the GPU library through the C-ABI:
+TestGrid::testDeformationGradients (TestGrid.cpp:216-278) and the invariants of
+TestGrid::testImplicitUpdate (TestGrid.cpp:456-708): collided nodes move only tangentially to their
+collider, multi-contact nodes are at rest, and the solved velocities satisfy the projected system
+P(M(v - vc) - dt^2 DF v) = p*, within the reference test's own bound (1e-6, absolute max-norm,
+TestGrid.cpp:698-707), computed in FP64 from the GPU's FP32 outputs."""
+import numpy as np
+import pytest
+
+import mpmsnow_amd as G
+from test_gpu_parity import implicit_pair, say
+
+pytestmark = pytest.mark.gpu
+
+
+def test_deformation_gradients_linear_field():
+    """Linear velocity field (x, -y, 0.5 z), h = 0.2, dt = 0.01, 7 steps: F of the particle at the
+    origin stays diag(1 + 0.01 i, 1 - 0.01 i, 1 + 0.005 i) within 1e-3."""
+    say("reference testDeformationGradients: 1001 particles, 7 steps")
+    pts = [[0, 0, 0]] + [[i / 9 - 0.5, j / 9 - 0.5, k / 9 - 0.5] for i in range(10) for j in range(10) for k in range(10)]
+    x = np.array(pts, dtype=np.float32)
+    n = len(x)
+    ctx = G.Context()
+    ctx.particles_create(n)
+    ctx.set("p", x)
+    ctx.set("m", np.ones(n, np.float32))
+    ctx.set("volume", np.ones(n, np.float32))
+    v = np.stack([x[:, 0], -x[:, 1], 0.5 * x[:, 2]], 1).astype(np.float32)
+    v[0] = 0
+    ctx.set("v", v)
+    for i in range(1, 8):
+        g = G.Grid(ctx, np.arange(n), 0.2)
+        g.update_deformation_gradients(0.01)
+        g.close()
+        x = (x + v * np.float32(0.01)).astype(np.float32)
+        ctx.set("p", x)
+        F0 = ctx.get("F")[0]
+        assert abs(F0[0, 0] - (1 + 0.01 * i)) < 1e-3
+        assert abs(F0[1, 1] - (1 - 0.01 * i)) < 1e-3
+        assert abs(F0[2, 2] - (1 + 0.005 * i)) < 1e-3
+
+
+def test_implicit_update_solves_projected_system(oracle):
+    say("reference testImplicitUpdate invariants: 64 particles, two moving planes, maxIters=40 tol=0")
+    P, h, cols = implicit_pair(oracle)
+    dt = 0.002
+    g = G.Grid(P.ctx, np.arange(P.n), h)
+    g.compute_particle_volumes()
+    em, col = g.explicit_momenta(dt)
+    g.update_grid_velocities(dt, 40, 0.0)
+    v = g.get("velocities").astype(np.float64).reshape(-1, 3)
+    col = np.asarray(col)
+    vc = g.collision_velocities(col).astype(np.float64).reshape(-1, 3)
+    normals = [np.array([-1, -1, -1.0]) / np.sqrt(3), np.array([-1.0, 0, 0])]
+    objv = [np.array([-0.1, -0.2, -0.3]), np.array([-0.2, -0.1, -0.4])]
+    assert (col >= 0).any() and (col == -1).any()
+    for idx in np.nonzero(col >= 0)[0]:
+        o = col[idx]
+        assert np.abs(vc[idx] - objv[o]).max() < 1e-7
+        assert abs(normals[o] @ (v[idx] - objv[o])) < 1e-5
+    for idx in np.nonzero(col == -2)[0]:
+        assert np.linalg.norm(v[idx]) == 0
+    df = g.force_differentials(g.get("velocities")).astype(np.float64).reshape(-1, 3)
+    M = g.get("masses").astype(np.float64)
+    lhs = M[:, None] * (v - vc) - df * dt * dt
+    for idx in range(len(M)):
+        o = col[idx]
+        if o >= 0:
+            lhs[idx] -= (normals[o] @ lhs[idx]) * normals[o]
+        elif o == -2:
+            lhs[idx] = 0
+    err = np.linalg.norm(lhs - em.astype(np.float64).reshape(-1, 3), axis=1).max()
+    say("max-norm of P(M(v - vc) - dt^2 DF v) - p*: %.3e (reference test bound 1e-6)" % err)
+    assert err < 1e-6
