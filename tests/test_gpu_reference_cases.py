@@ -1,5 +1,6 @@
 """The reference's own grid tests with their own assertions, run on the GPU library through the C-ABI:
-TestGrid::testDeformationGradients (TestGrid.cpp:216-278) and the invariants of
+TestGrid::testDeformationGradients (TestGrid.cpp:216-278), TestGrid::testForces (TestGrid.cpp:281-396)
+and the invariants of
 TestGrid::testImplicitUpdate (TestGrid.cpp:456-708): collided nodes move only tangentially to their
 collider, multi-contact nodes are at rest, and the solved velocities satisfy the projected system
 P(M(v - vc) - dt^2 DF v) = p*, within the reference test's own bound (1e-6, absolute max-norm,
@@ -72,3 +73,60 @@ def test_implicit_update_solves_projected_system(oracle):
     err = np.linalg.norm(lhs - em.astype(np.float64).reshape(-1, 3), axis=1).max()
     say("max-norm of P(M(v - vc) - dt^2 DF v) - p*: %.3e (reference test bound 1e-6)" % err)
     assert err < 1e-6
+
+
+def test_forces_and_hvp_against_finite_differences():
+    """TestGrid::testForces (TestGrid.cpp:281-396) on the GPU: one particle (volume 0.5, E = 1.4e3,
+    h = 1): zero force at a pure rotation (1e-4); then F = rotation + 0.01 random: each force
+    component against -dE/dx by central differences of the energy when the grid moves by +-0.01
+    along it (1%), and the HVP against the central difference of the forces (5%: the HVP keeps the
+    force's F^T factor fixed while the difference recomputes it, as the reference's own comment on
+    its 2.5% bound says; the oracle pins the HVP's assembly exactly)."""
+    from test_oracle_pins import ROT, rand_mat
+
+    say("reference testForces: single particle, FD of energy and forces")
+    rng = np.random.default_rng(23)
+    ctx = G.Context()
+    ctx.particles_create(1)
+    ctx.set("p", np.zeros((1, 3), np.float32))
+    ctx.set("m", np.ones(1, np.float32))
+    ctx.set("volume", np.full(1, 0.5, np.float32))
+    ctx.set("F", ROT.reshape(1, 3, 3).astype(np.float32))
+    ctx.model_snow(1.4e3, 0.2, 0.0, 1e5, 1e5)
+    ctx.update_particle_data()
+    g = G.Grid(ctx, np.arange(1), 1.0)
+    f = g.calculate_forces()
+    assert np.abs(f).max() < 1e-4
+    Forig = (ROT + 0.01 * rand_mat(rng)).reshape(1, 3, 3).astype(np.float32)
+
+    def reset():
+        ctx.set("F", Forig)
+        ctx.update_particle_data()
+
+    reset()
+    f = g.calculate_forces()
+    dx = 0.01
+    nv = g.num_nodes * 3
+    worst_f = worst_df = 0.0
+    for i in range(nv):
+        if f[i] == 0:
+            continue
+        reset()
+        vel = np.zeros(nv, np.float32)
+        vel[i] = dx
+        df = g.force_differentials(vel).astype(np.float64)
+        ends = []
+        for sgn in (1.0, -1.0):
+            reset()
+            vel[i] = sgn * dx
+            g.set("velocities", vel)
+            g.update_deformation_gradients(1.0)
+            ctx.update_particle_data()
+            ends.append((float(ctx.energy_density(np.arange(1))[0]) * 0.5, g.calculate_forces().astype(np.float64)))
+        reset()
+        (ep, fp), (em, fm) = ends
+        worst_f = max(worst_f, abs((f[i] + (ep - em) / (2 * dx)) / f[i]))
+        worst_df = max(worst_df, np.linalg.norm(0.5 * (fp - fm) - df) / np.linalg.norm(df))
+    say("worst relative force error %.3e, worst HVP error %.3e" % (worst_f, worst_df))
+    assert worst_f < 0.01
+    assert worst_df < 0.05
