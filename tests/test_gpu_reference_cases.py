@@ -1,6 +1,6 @@
 """The reference's own grid tests with their own assertions, run on the GPU library through the C-ABI:
-TestGrid::testDeformationGradients (TestGrid.cpp:216-278), TestGrid::testForces (TestGrid.cpp:281-396)
-and the invariants of
+TestGrid::testDeformationGradients (TestGrid.cpp:216-278), TestGrid::testForces (TestGrid.cpp:281-396),
+TestSnowConstitutiveModel (TestSnowConstitutiveModel.cpp:17-158) and the invariants of
 TestGrid::testImplicitUpdate (TestGrid.cpp:456-708): collided nodes move only tangentially to their
 collider, multi-contact nodes are at rest, and the solved velocities satisfy the projected system
 P(M(v - vc) - dt^2 DF v) = p*, within the reference test's own bound (1e-6, absolute max-norm,
@@ -130,3 +130,75 @@ def test_forces_and_hvp_against_finite_differences():
     say("worst relative force error %.3e, worst HVP error %.3e" % (worst_f, worst_df))
     assert worst_f < 0.01
     assert worst_df < 0.05
+
+
+def gpu_model(F, Fp=None, E=1.0, nu=0.25, xi=10.0, thc=0.1, ths=0.2):
+    """One particle with the reference test's snow model (TestSnowConstitutiveModel.cpp:30-36)."""
+    ctx = G.Context()
+    ctx.particles_create(1)
+    ctx.set("m", np.ones(1, np.float32))
+    ctx.model_snow(E, nu, xi, thc, ths)
+    ctx.set("F", np.asarray(F, np.float32).reshape(1, 3, 3))
+    if Fp is not None:
+        ctx.set("Fp", np.asarray(Fp, np.float32).reshape(1, 3, 3))
+    ctx.update_particle_data()
+    return ctx
+
+
+def test_snow_constitutive_model_reference_assertions():
+    """TestSnowConstitutiveModel (TestSnowConstitutiveModel.cpp:17-158) on the GPU, with its own
+    bounds: updateParticleData keeps F Fp and gives F^-T and J = det F (1e-4); the clamp known
+    answer sigma 1.3 -> 1.2, 0.8 -> 0.9; zero energy and stress at the identity and at a pure
+    rotation (1e-6); dEnergyDensitydF against central differences of the energy and
+    dEdFDifferential against the change of dEnergyDensitydF (1e-5)."""
+    from test_oracle_pins import ROT, rand_mat
+
+    say("reference TestSnowConstitutiveModel on the GPU")
+    rng = np.random.default_rng(11)
+    P = rand_mat(rng)
+    P = P @ P
+    Mm = rand_mat(rng)
+    Mm = Mm @ Mm
+    ctx = gpu_model(Mm, P)
+    F = ctx.get("F")[0].astype(np.float64)
+    Fp = ctx.get("Fp")[0].astype(np.float64)
+    assert np.linalg.norm(F @ Fp - Mm @ P) < 1e-4 * max(1, np.linalg.norm(Mm @ P))
+    assert np.linalg.norm(np.linalg.inv(F).T - ctx.get("FinvTrans")[0]) < 1e-3 * max(1, np.linalg.norm(np.linalg.inv(F)))
+    assert abs(ctx.get("J")[0] - abs(np.linalg.det(F))) < 1e-3
+    ctx = gpu_model(np.diag([1.3, 0.8, 1.0]), np.eye(3))
+    F = ctx.get("F")[0]
+    assert abs(F[0, 0] - 1.2) < 1e-6 and abs(F[1, 1] - 0.9) < 1e-6
+    assert np.abs(F.astype(np.float64) @ ctx.get("Fp")[0] - np.diag([1.3, 0.8, 1.0])).max() < 1e-6
+    for R in (np.eye(3), ROT):
+        ctx = gpu_model(R, np.eye(3))
+        assert abs(ctx.energy_density(np.arange(1))[0]) < 1e-6
+        assert np.linalg.norm(ctx.stress(np.arange(1))[0]) < 1e-6
+    # stress against central differences of the energy
+    F0 = np.diag([1.02, 0.95, 1.0]) @ ROT
+    ctx = gpu_model(F0, np.eye(3))
+    Pst = ctx.stress(np.arange(1))[0].astype(np.float64)
+    delta = 0.01
+    fd = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            e = []
+            for sgn in (1.0, -1.0):
+                Fq = F0.copy()
+                Fq[i, j] += sgn * delta
+                ctx.set("F", Fq.reshape(1, 3, 3).astype(np.float32))
+                ctx.update_particle_data()
+                e.append(float(ctx.energy_density(np.arange(1))[0]))
+            fd[i, j] = (e[0] - e[1]) / (2 * delta)
+    say("stress vs FD of the energy: %.3e" % np.abs(Pst - fd).max())
+    assert np.abs(Pst - fd).max() < 1e-5
+    # differential against the change of the stress
+    rng = np.random.default_rng(21)
+    ctx = gpu_model(F0, np.eye(3))
+    P0 = ctx.stress(np.arange(1))[0].astype(np.float64)
+    dF = 0.001 * rand_mat(rng)
+    dP = ctx.stress_differential(np.arange(1), dF.reshape(1, 3, 3))[0].astype(np.float64)
+    ctx.set("F", (F0 + dF).reshape(1, 3, 3).astype(np.float32))
+    ctx.update_particle_data()
+    fd = ctx.stress(np.arange(1))[0].astype(np.float64) - P0
+    say("stress differential vs change of the stress: %.3e" % np.abs(fd - dP).max())
+    assert np.abs(fd - dP).max() < 1e-5
