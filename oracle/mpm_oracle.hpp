@@ -567,6 +567,15 @@ template <class S> struct Termination {
   S tol;
   S threshold = 0;
   bool cancelled = false;
+  // TerminationCriterion::cancelled() as a user interrupt arriving during a solve: from its
+  // cancelOnCall-th call on it returns true (Sim::advance calls it once per body, Sim.cpp:107)
+  int cancelOnCall = -1;
+  int cancelCalls = 0;
+  bool pollCancelled() {
+    ++cancelCalls;
+    if (cancelOnCall > 0 && cancelCalls >= cancelOnCall) cancelled = true;
+    return cancelled;
+  }
   std::vector<S> history;  // ||r||^2 per call (the reference prints sqrt of it)
   Termination(int maxIt, S tolError) : maxIters(float(maxIt)), tol(tolError) {}
   void init(const Vec<S>& b) { S bn = vsqnorm(b); threshold = tol * tol * bn; history.clear(); }
@@ -1234,7 +1243,7 @@ template <class S> struct Sim {
       g.updateGridVelocities(dt, model, cols, fields, term, &bs.solve);
       bs.solve.residualNorm2 = term.history;
       lastStats.push_back(bs);
-      if (term.cancelled) { calculateBodies(); return; }
+      if (term.pollCancelled()) { calculateBodies(); return; }
       g.updateParticleVelocities();
       g.updateDeformationGradients(dt);
       model.updateParticleData();

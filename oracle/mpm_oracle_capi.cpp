@@ -593,6 +593,18 @@ int orc_sim_advance(void* sh, double dt, int maxIters, double tol) {
     });
   });
 }
+// the same with a termination whose cancelled() turns true on its cancelOnCall-th call; returns 1
+// if the step was cancelled
+int orc_sim_advance_cancel(void* sh, double dt, int maxIters, double tol, int cancelOnCall, int* wasCancelled) {
+  return guard([&] {
+    DISPATCH(sh, {
+      Termination<S> t(maxIters, S(tol));
+      t.cancelOnCall = cancelOnCall;
+      static_cast<SimH<S>*>(sh)->s->advance(S(dt), t);
+      *wasCancelled = t.cancelled ? 1 : 0;
+    });
+  });
+}
 // out: [iterations, bNorm2, earlyExit, fv0, fv1, fv2, n0, n1, n2, particles]; returns residual history length
 int orc_sim_body_stats(void* sh, long body, double* out, double* hist, int cap) {
   int L = -1;

@@ -86,6 +86,8 @@ def lib():
                                             C.c_void_p, C.c_int, _dp, _dp, _dp]),
             "orc_sim_pd": (C.c_void_p, [C.c_void_p]),
             "orc_sim_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double]),
+            "orc_sim_advance_cancel": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
+                                                 C.POINTER(C.c_int)]),
             "orc_sim_body_stats": (C.c_int, [C.c_void_p, C.c_long, _dp, _dp, C.c_int]),
             "orc_sim_num_stats": (C.c_long, [C.c_void_p]),
             "orc_sim_num_bodies": (C.c_long, [C.c_void_p]),
@@ -436,8 +438,15 @@ class Sim(_H):
         self.pd = ParticleData(lib().orc_sim_pd(self.ptr), owner=False)
         self.pd._sim = self
 
-    def advance(self, dt, max_iters, tol):
-        _check(lib().orc_sim_advance(self.ptr, dt, max_iters, tol))
+    def advance(self, dt, max_iters, tol, cancel_on_call=None):
+        """One Sim::advance; cancel_on_call = n makes the termination's cancelled() true from its n-th
+        call on (one call after each body's solve); returns whether the step was cancelled."""
+        if cancel_on_call is None:
+            _check(lib().orc_sim_advance(self.ptr, dt, max_iters, tol))
+            return False
+        was = C.c_int(0)
+        _check(lib().orc_sim_advance_cancel(self.ptr, dt, max_iters, tol, int(cancel_on_call), C.byref(was)))
+        return bool(was.value)
 
     def bodies(self):
         nb = lib().orc_sim_num_bodies(self.ptr)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import mpmsnow_amd as G
-from test_gpu_parity import implicit_pair, say
+from test_gpu_parity import implicit_pair, relerr, say
 
 pytestmark = pytest.mark.gpu
 
@@ -202,3 +202,43 @@ def test_snow_constitutive_model_reference_assertions():
     fd = ctx.stress(np.arange(1))[0].astype(np.float64) - P0
     say("stress differential vs change of the stress: %.3e" % np.abs(fd - dP).max())
     assert np.abs(fd - dP).max() < 1e-5
+
+
+def test_sim_cancellation_against_oracle(oracle):
+    """Sim::advance cancelled after the second body's solve (Sim.cpp:107-111), on the GPU against
+    the FP64 oracle with the same interrupt: cancelled() is called once per body until it returns
+    true; no particle moves; body 0 gets its G2P, F update and plasticity; bodies 1 and 2 keep their
+    velocities and F bit for bit; the ballistic particles keep their gravity update. The next,
+    uncancelled step then matches the oracle (positions within 1e-4 h, F within 1e-5)."""
+    from test_oracle_pins import cancel_oracle_sim, cancel_scene
+
+    say("Sim cancellation on the second poll: 3 snowballs + 5 ballistic particles")
+    x, m, v, h = cancel_scene()
+    planes = [((0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.2, False)]
+    mat = dict(E=1.4e5, nu=0.2, xi=10.0, theta_c=2.5e-2, theta_s=7.5e-3)
+    sim = G.Sim(x, m, h, material=mat, planes=planes, gravity=[(0.0, -9.81, 0.0)], v=v)
+    o = cancel_oracle_sim(oracle, "double", x, m, v, h)
+    F0 = sim.get("F").copy()
+    calls = []
+
+    def cancelled():
+        calls.append(1)
+        return len(calls) >= 2
+
+    st = sim.advance(1e-3, 1000, 1e-5, cancelled=cancelled)
+    assert o.advance(1e-3, 1000, 1e-5, cancel_on_call=2)
+    assert st["cancelled"] and len(calls) == 2
+    assert np.array_equal(sim.get("p"), x)
+    vg, vo = sim.get("v"), o.pd.get("v")
+    assert relerr(vg[:2500], vo[:2500]) < 1e-5
+    assert np.array_equal(vg[2500:7500], v[2500:7500])
+    assert np.abs(vg[7500:] - vo[7500:]).max() < 1e-7
+    Fg = sim.get("F")
+    assert np.abs(Fg[:2500] - o.pd.get("F")[:2500]).max() < 1e-5
+    assert np.array_equal(Fg[2500:], F0[2500:])
+    sim.advance(1e-3, 1000, 1e-5)
+    o.advance(1e-3, 1000, 1e-5)
+    dx = np.abs(sim.get("p") - o.pd.get("p")).max()
+    say("after the next step: max |x - x_oracle| = %.3e h" % (dx / h))
+    assert dx < 1e-4 * h
+    assert np.abs(sim.get("F") - o.pd.get("F")).max() < 1e-5
