@@ -242,3 +242,73 @@ def test_sim_cancellation_against_oracle(oracle):
     say("after the next step: max |x - x_oracle| = %.3e h" % (dx / h))
     assert dx < 1e-4 * h
     assert np.abs(sim.get("F") - o.pd.get("F")).max() < 1e-5
+
+
+def lattice_ctx(oracle, v):
+    """6^3 particles at spacing 0.05 (TestGrid-style lattice) with F = I (stress-free), on the GPU and
+    in the FP64 oracle."""
+    pos = np.array([[0.05 * i, 0.05 * j - 0.02, 0.05 * k] for i in range(6) for j in range(6) for k in range(6)],
+                   dtype=np.float32)
+    n = len(pos)
+    ctx = G.Context()
+    ctx.particles_create(n)
+    ctx.set("p", pos)
+    ctx.set("v", np.tile(np.asarray(v, np.float32), (n, 1)))
+    ctx.set("m", np.ones(n, np.float32))
+    ctx.set("volume", np.full(n, 1e-4, np.float32))
+    ctx.model_snow(1.4e5, 0.2, 10.0, 2.5e-2, 7.5e-3)
+    ctx.set_gravity([(0.0, -9.81, 0.0)])
+    pd = oracle.ParticleData.create(n, "double")
+    pd.set("p", pos)
+    pd.set("v", np.tile(np.asarray(v, np.float64), (n, 1)))
+    pd.set("m", np.ones(n))
+    pd.set("volume", np.full(n, 1e-4, np.float32))
+    model = oracle.Model.snow(1.4e5, 0.2, 10.0, 2.5e-2, 7.5e-3, "double")
+    model.set_particles(pd)
+    fields = oracle.Fields.create("double")
+    fields.add_gravity([0, -9.81, 0])
+    return ctx, pd, model, fields, n
+
+
+def test_explicit_momenta_sticky_plane_sphere_and_multi_contact(oracle):
+    """calculateExplicitMomenta and collisionVelocities (Grid.cpp:634-724) with the ground plane
+    (friction 0.2), a moving sticky wall and a moving analytic sphere (friction 0.5), against the FP64
+    oracle: collision flags exact (including multi-contact -2 and sticky nodes), explicit momenta
+    within 1e-5, collision velocities within 1e-7 (src/CollisionObject.cpp:19-96)."""
+    say("explicit momenta: ground plane, sticky moving wall, moving sphere")
+    ctx, pd, model, fields, n = lattice_ctx(oracle, [0.5, -1.0, 0.3])
+    planes = [((0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.2, False),
+              ((-1.0, 0.0, 0.0, 0.2), (0.1, 0.0, 0.0), 0.0, True)]
+    spheres = [((0.05, 0.1, 0.1), 0.06, (0.0, 0.2, 0.0), 0.5, False)]
+    ctx.set_colliders(planes, spheres)
+    cols = oracle.Colliders.create("double")
+    for c, vv, mu, st in planes:
+        cols.add_plane(c, v=vv, friction=mu, sticky=st)
+    for c, r, vv, mu, st in spheres:
+        cols.add_sphere(c, r, v=vv, friction=mu, sticky=st)
+    h, dt = 0.1, 1e-3
+    g = G.Grid(ctx, np.arange(n), h)
+    og = oracle.Grid(pd, np.arange(n), h)
+    em, col = g.explicit_momenta(dt)
+    emo, colo = og.explicit_momenta(dt, model, cols, fields)
+    col, colo = np.asarray(col), np.asarray(colo)
+    say("flags: %s" % {int(k): int((colo == k).sum()) for k in np.unique(colo)})
+    for k in (-2, 0, 1, 2):
+        assert (colo == k).any(), k
+    assert (col == colo).all()
+    assert relerr(em, emo) < 1e-5
+    vc = g.collision_velocities(col)
+    vco = og.collision_velocities(cols, colo)
+    assert np.abs(vc - vco).max() < 1e-7
+
+
+def test_explicit_momenta_nan_error_from_frictionless_contact(oracle):
+    """The reference's zero-friction quirk (src/CollisionObject.cpp:70-84): a node pressed straight
+    into a frictionless plane has zero tangential velocity, 0 * (x / 0) is NaN, and the explicit
+    momentum check raises "nan in explicit momenta!" (Grid.cpp:677-687); the GPU raises the same."""
+    say("explicit momenta: frictionless plane, normal approach -> nan error")
+    ctx, pd, model, fields, n = lattice_ctx(oracle, [0.0, -1.0, 0.0])
+    ctx.set_colliders([((0.0, 1.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.0, False)])
+    g = G.Grid(ctx, np.arange(n), 0.1)
+    with pytest.raises(G.MpmError, match="nan in explicit momenta!"):
+        g.explicit_momenta(1e-3)
