@@ -31,11 +31,11 @@ from mpmsnow_amd import scenes  # noqa: E402
 DEFAULT_OUT = os.path.join(HERE, "lpc_weights.npy")
 
 
-def make_scene(k, rng, curriculum="mixed"):
+def make_scene(k, rng, curriculum="mixed", offset=0):
     """Scene k of the training curriculum: "mixed" alternates snowball pairs and slope slabs,
     "stiff" draws stiff drops (8 boxes and spheres on the plane and the sphere collider,
     E0 = 4.8e5 Pa, h = 1/512, dt = 2e-3) of 6-10M particles."""
-    seed = 1000 + k
+    seed = 1000 + offset + k
     if curriculum == "stiff":
         n = int(rng.integers(6, 11)) * 1000000
         sc = scenes.stiff_drop(n=n, seed=seed)
@@ -58,13 +58,13 @@ def make_scene(k, rng, curriculum="mixed"):
 
 
 def train(out=DEFAULT_OUT, scenes_n=24, captures=4, epochs=8, lr=2e-4, seed=0, init=None, curriculum="mixed",
-          decay=40.0, samples=12, capture_iters=None):
+          decay=40.0, samples=12, capture_iters=None, offset=0):
     rng = np.random.default_rng(seed)
     params = None if init is None else np.load(init, allow_pickle=False)
     step = 0
     t0 = time.time()
     for k in range(scenes_n):
-        name, sc, sim, warm = make_scene(k, rng, curriculum)
+        name, sc, sim, warm = make_scene(k, rng, curriculum, offset)
         if capture_iters:
             sim.ctx.lpc_capture_iterations(capture_iters)
         if params is None:
@@ -103,9 +103,11 @@ def main():
     ap.add_argument("--samples", type=int, default=12, help="residuals kept per captured solve (at most 16)")
     ap.add_argument("--capture-iters", default=None,
                     help="comma-separated CR iterations whose residuals a training solve keeps")
+    ap.add_argument("--scene-offset", type=int, default=0, help="scene k uses seed 1000 + offset + k")
     a = ap.parse_args()
     its = [int(t) for t in a.capture_iters.split(",")] if a.capture_iters else None
-    train(a.out, a.scenes, a.captures, a.epochs, a.lr, a.seed, a.init, a.curriculum, a.decay, a.samples, its)
+    train(a.out, a.scenes, a.captures, a.epochs, a.lr, a.seed, a.init, a.curriculum, a.decay, a.samples, its,
+          a.scene_offset)
 
 
 if __name__ == "__main__":
