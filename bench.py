@@ -150,7 +150,7 @@ def error_vs_fp64(sc, state, o32, steps):
     return out
 
 
-def learned_leg(steps, warmup, weights=None, fallback=None):
+def learned_leg(steps, warmup, weights=None, fallback=None, dirs=None):
     """Plain diagonal CR vs learned flexible CR on the 4M slope slab, from identical states."""
     import numpy as np
     import torch
@@ -167,6 +167,8 @@ def learned_leg(steps, warmup, weights=None, fallback=None):
         sim.ctx.lpc_set_params(w)
         if fallback is not None:
             sim.ctx.lpc_fallback(fallback)
+        if dirs is not None:
+            sim.ctx.lpc_directions(dirs)
         progress("learned leg: %s, 4M slab set up" % name)
         for _ in range(warmup):
             sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
@@ -192,7 +194,7 @@ def learned_leg(steps, warmup, weights=None, fallback=None):
     return out
 
 
-def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None):
+def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None):
     """Plain diagonal CR vs learned flexible CR on the stiff stress case (8 seeded boxes and spheres
     dropped onto the ground plane and an analytic sphere collider, E0 = 4.8e5 Pa, h = 1/512,
     dt = 2e-3), both from the state after `warmup` plain steps. Reports per-step CR iterations (sum
@@ -213,6 +215,8 @@ def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None):
         sim.ctx.lpc_set_params(w)
         if fallback is not None:
             sim.ctx.lpc_fallback(fallback)
+        if dirs is not None:
+            sim.ctx.lpc_directions(dirs)
         progress("stiff leg: %s, %d particles set up" % (name, n))
         for k in range(warmup):
             st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
@@ -471,6 +475,7 @@ def main():
     ap.add_argument("--stiff-warmup", type=int, default=12)
     ap.add_argument("--lpc-weights", default=None, help="network parameters (.npy) for the learned legs")
     ap.add_argument("--lpc-fallback", type=float, default=None, help="fallback ratio of the learned legs")
+    ap.add_argument("--lpc-dirs", type=int, default=None, help="directions the learned legs' flexible CR keeps")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
@@ -495,7 +500,7 @@ def main():
 
     if args.stiff:
         print(json.dumps(stiff_leg(args.stiff_particles, args.stiff_steps, args.stiff_warmup,
-                                   weights=args.lpc_weights, fallback=args.lpc_fallback)))
+                                   weights=args.lpc_weights, fallback=args.lpc_fallback, dirs=args.lpc_dirs)))
         return
 
     import numpy as np
@@ -592,7 +597,7 @@ def main():
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
         out["error_vs_fp64"] = error_vs_fp64(sc, start_state, o32, args.cpu_steps)
     if not args.no_learned:
-        out["learned_pc"] = learned_leg(args.learned_steps, 3, args.lpc_weights, args.lpc_fallback)
+        out["learned_pc"] = learned_leg(args.learned_steps, 3, args.lpc_weights, args.lpc_fallback, args.lpc_dirs)
     print(json.dumps(out))
 
 

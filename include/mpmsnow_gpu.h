@@ -198,6 +198,9 @@ int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n);
 int mpm_lpc_set_mode(mpm_context* ctx, int mode);
 /* fallback ratio in (0, 1]: 1 = a fallback only when |r|^2 rises */
 int mpm_lpc_set_fallback(mpm_context* ctx, double ratio);
+/* directions the flexible CR orthogonalises against, 1-8 (default 2; with the diagonal
+ * preconditioner every m >= 1 gives the same Krylov iteration in exact arithmetic) */
+int mpm_lpc_set_directions(mpm_context* ctx, int m);
 /* CR iterations at which training solves keep their residuals: 1-16 increasing indices
  * (default 0 1 2 3 4 6 8 11 15 20 26 33 41 50 60 72); a solve keeps the first `samples` of them */
 int mpm_lpc_set_capture_iterations(mpm_context* ctx, const int32_t* iters, int n);

@@ -103,6 +103,7 @@ SIGNATURES = {
     "mpm_lpc_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
     "mpm_lpc_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_lpc_set_fallback": (C.c_int, [C.c_void_p, C.c_double]),
+    "mpm_lpc_set_directions": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_lpc_set_capture_iterations": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "mpm_lpc_train_next": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
     "mpm_lpc_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -332,6 +333,9 @@ class Context:
 
     def lpc_fallback(self, ratio):
         self.check(lib().mpm_lpc_set_fallback(self.ptr, float(ratio)))
+
+    def lpc_directions(self, m):
+        self.check(lib().mpm_lpc_set_directions(self.ptr, int(m)))
 
     def lpc_capture_iterations(self, iters):
         a = np.ascontiguousarray(iters, dtype=np.int32)

@@ -133,6 +133,8 @@ enum ErrorBits : int {
   ERR_INTERNAL = 128       // internal: a launch that does not fit the batch's layout
 };
 
+constexpr int kFcrMax = 8;  // flexible CR: most stored directions (the context's setting, default 2)
+
 // Conjugate-residual scalars of one body, resident in device memory. Reductions are
 // FP64 and deterministic (per-block partials combined in a fixed order by the last
 // block of the body to finish).
@@ -149,10 +151,10 @@ struct CRScalars {
   int max_iters;
   int errors;              // ErrorBits
   int active_nodes;        // nodes with m > 0 (measurement)
-  // flexible conjugate residual with the learned preconditioner (truncated GCR over kFcrM
-  // previous directions)
-  double gbeta[2];         // <A z, A p_j> / <A p_j, A p_j>
-  double ApAp[2];          // <A p_j, A p_j> of the stored directions
+  // flexible conjugate residual with the learned preconditioner (truncated GCR over the last
+  // fcr_m directions, fcr_m <= kFcrMax)
+  double gbeta[kFcrMax];   // <A z, A p_j> / <A p_j, A p_j>
+  double ApAp[kFcrMax];    // <A p_j, A p_j> of the stored directions
   double rAp;              // <r, A p> of the new direction
   double rr_prev;          // |r|^2 before the last update
   double lpc_ratio;        // a network iteration whose |r|^2 exceeds lpc_ratio * rr_prev is a fallback
@@ -161,14 +163,14 @@ struct CRScalars {
   float lpc_scale;         // RMS of r / diag over active nodes: network input scale
   int lpc_off;             // 1: this body uses the diagonal preconditioner
   int fallbacks;           // iterations that switched this body to the diagonal preconditioner
-  int ndirs;               // directions stored (<= kFcrM)
+  int ndirs;               // directions stored (<= fcr_m)
+  int fcr_m;               // directions kept
   int slot;                // ring slot of the most recent direction
   int retry;               // 1: the network gave no direction; redo the iteration on the diagonal
   int lpc_pause;           // 1: lpc_off holds for one iteration only (a fallback before the third)
 };
 constexpr int kLpcStrikes = 3;   // fallbacks after which a body keeps the diagonal preconditioner
 constexpr float kLpcClamp = 8.0f;  // network inputs, outputs and training targets, in units of the scale
-constexpr int kFcrM = 2;
 
 // Counters shared by all bodies of a batch.
 struct BatchStatus {

@@ -2477,7 +2477,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
 // The learned preconditioner changes from one iteration to the next, which breaks the
 // short recurrences of ConjugateResiduals.cpp:84-151. The solve then runs a truncated
 // generalised conjugate residual in the inner product <u, v>_D = sum u v / diag: each
-// preconditioned vector z is orthogonalised against the last kFcrM directions (A p_j in that
+// preconditioned vector z is orthogonalised against the last fcr_m directions (A p_j in that
 // inner product) and x, r take the step that minimises |r|_D. With the diagonal
 // preconditioner z = r / diag this is the preconditioned conjugate residual iteration of
 // ConjugateResiduals.cpp (same Krylov space, same minimised norm). The same termination
@@ -2490,7 +2490,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
 // r = b - A x0 ; network input scale ; iteration state
 __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                            const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                           int netOn, double ratio, CRScalars* sc, BatchStatus* bst, double* partials) {
+                           int netOn, double ratio, int mdirs, CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
@@ -2526,8 +2526,9 @@ __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, cons
     S.retry = 0;
     S.lpc_pause = 0;
     S.ndirs = 0;
-    S.slot = kFcrM - 1;
-    for (int j = 0; j < kFcrM; ++j) {
+    S.fcr_m = mdirs;
+    S.slot = mdirs - 1;
+    for (int j = 0; j < kFcrMax; ++j) {
       S.ApAp[j] = 0.0;
       S.gbeta[j] = 0.0;
     }
@@ -2550,7 +2551,7 @@ __global__ void k_fcr_diag_z(Batch B, const float* r, const float* diag, float* 
 }
 // A z and its projections on the stored directions' A p_j
 __global__ void k_fcr_az(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols, const float* stage,
-                         const float* z, const float* diag, float dt, float* Az, const float* Q0, const float* Q1,
+                         const float* z, const float* diag, float dt, float* Az, const float* Q, int64_t vs,
                          CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
@@ -2561,43 +2562,42 @@ __global__ void k_fcr_az(Batch B, const float* mass, const int8_t* flags, const 
   if (c.valid) gather_node3(B, c.G, stage, c.T, c.l, df);
   apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt2, az);
   st3(Az, c.i, az);
-  double s[2] = {0.0, 0.0};
-  const float* Q[2] = {Q0, Q1};
+  double s[kFcrMax];
   float dg[3];
   ld3(diag, c.i, dg);
 #pragma unroll
-  for (int j = 0; j < kFcrM; ++j)
-    if (S.ApAp[j] != 0.0) {
+  for (int j = 0; j < kFcrMax; ++j) {
+    s[j] = 0.0;
+    if (j < S.fcr_m && S.ApAp[j] != 0.0) {
       float q[3];
-      ld3(Q[j], c.i, q);
+      ld3(Q + j * vs, c.i, q);
       s[j] = (double)az[0] * (q[0] / dg[0]) + (double)az[1] * (q[1] / dg[1]) + (double)az[2] * (q[2] / dg[2]);
     }
-  double tot[2];
-  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
-    for (int j = 0; j < kFcrM; ++j) S.gbeta[j] = S.ApAp[j] != 0.0 ? tot[j] / S.ApAp[j] : 0.0;
+  }
+  double tot[kFcrMax];
+  if (body_sum<kFcrMax>(s, c.G, partials, body_tickets(B, c.b, 1), tot) && threadIdx.x == 0) {
+    for (int j = 0; j < kFcrMax; ++j) S.gbeta[j] = S.ApAp[j] != 0.0 ? tot[j] / S.ApAp[j] : 0.0;
   }
 }
 // new direction p = z - sum beta_j p_j, A p = A z - sum beta_j A p_j into the ring's next slot;
 // alpha = <r, A p> / <A p, A p>
-__global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float* r, const float* diag, float* P0,
-                           float* P1, float* Q0, float* Q1, CRScalars* sc, BatchStatus* bst, double* partials) {
+__global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float* r, const float* diag, float* P,
+                           float* Q, int64_t vs, CRScalars* sc, BatchStatus* bst, double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
-  const int slot = (S.slot + 1) % kFcrM;
-  float* P[2] = {P0, P1};
-  float* Q[2] = {Q0, Q1};
+  const int slot = (S.slot + 1) % S.fcr_m;
   float p[3], ap[3], rr[3];
   ld3(z, c.i, p);
   ld3(Az, c.i, ap);
   ld3(r, c.i, rr);
 #pragma unroll
-  for (int j = 0; j < kFcrM; ++j) {
-    const float bj = (float)S.gbeta[j];
+  for (int j = 0; j < kFcrMax; ++j) {
+    const float bj = j < S.fcr_m ? (float)S.gbeta[j] : 0.0f;
     if (bj != 0.0f) {
       float pj[3], qj[3];
-      ld3(P[j], c.i, pj);
-      ld3(Q[j], c.i, qj);
+      ld3(P + j * vs, c.i, pj);
+      ld3(Q + j * vs, c.i, qj);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         p[a] -= bj * pj[a];
@@ -2605,8 +2605,8 @@ __global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float
       }
     }
   }
-  st3(P[slot], c.i, p);
-  st3(Q[slot], c.i, ap);
+  st3(P + slot * vs, c.i, p);
+  st3(Q + slot * vs, c.i, ap);
   double s[2] = {0.0, 0.0};
   float dg[3];
   ld3(diag, c.i, dg);
@@ -2633,24 +2633,22 @@ __global__ void k_fcr_orth(Batch B, const float* z, const float* Az, const float
       S.rAp = tot[1];
       S.alpha = tot[1] / tot[0];
       S.slot = slot;
-      S.ndirs = S.ndirs < kFcrM ? S.ndirs + 1 : kFcrM;
+      S.ndirs = S.ndirs < S.fcr_m ? S.ndirs + 1 : S.fcr_m;
     }
   }
 }
 // x = P(x + alpha p), r -= alpha A p; termination, fallback and the next network input scale
 __global__ void k_fcr_update(Batch B, const int8_t* flags, const ColliderSet* cols, const float* mass,
-                             const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
-                             float* x, float* r, CRScalars* sc, BatchStatus* bst, double* partials, double* hist) {
+                             const float* diag, const float* P, const float* Q, int64_t vs, float* x, float* r,
+                             CRScalars* sc, BatchStatus* bst, double* partials, double* hist) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done || S.retry) return;
   const float alpha = (float)S.alpha;
-  const float* P[2] = {P0, P1};
-  const float* Q[2] = {Q0, Q1};
   float xv[3], p[3], ap[3], rr[3], dg[3];
   ld3(x, c.i, xv);
-  ld3(P[S.slot], c.i, p);
-  ld3(Q[S.slot], c.i, ap);
+  ld3(P + S.slot * vs, c.i, p);
+  ld3(Q + S.slot * vs, c.i, ap);
   ld3(r, c.i, rr);
   ld3(diag, c.i, dg);
 #pragma unroll
@@ -3258,28 +3256,27 @@ void launch_small_cr(PView P, const Batch& B, const int* list, int n, const floa
 }
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                     int netOn, double ratio, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, ratio, sc, st, partials);
+                     int netOn, double ratio, int mdirs, CRScalars* sc, BatchStatus* st, double* partials,
+                     hipStream_t s) {
+  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, ratio, mdirs, sc, st, partials);
 }
 void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
                        hipStream_t s) {
   NODE_LAUNCH(k_fcr_diag_z, B, r, diag, z, sc);
 }
 void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                   const float* stage, const float* z, const float* diag, float dt, float* Az, const float* Q0,
-                   const float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
-  NODE_LAUNCH(k_fcr_az, B, mass, flags, cols, stage, z, diag, dt, Az, Q0, Q1, sc, st, partials);
+                   const float* stage, const float* z, const float* diag, float dt, float* Az, const float* Q,
+                   int64_t vs, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_az, B, mass, flags, cols, stage, z, diag, dt, Az, Q, vs, sc, st, partials);
 }
-void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, const float* diag, float* P0,
-                     float* P1, float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials,
-                     hipStream_t s) {
-  NODE_LAUNCH(k_fcr_orth, B, z, Az, r, diag, P0, P1, Q0, Q1, sc, st, partials);
+void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, const float* diag, float* P,
+                     float* Q, int64_t vs, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_orth, B, z, Az, r, diag, P, Q, vs, sc, st, partials);
 }
 void launch_fcr_update(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* mass,
-                       const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
-                       float* x, float* r, CRScalars* sc, BatchStatus* st, double* partials, double* hist,
-                       hipStream_t s) {
-  NODE_LAUNCH(k_fcr_update, B, flags, cols, mass, diag, P0, P1, Q0, Q1, x, r, sc, st, partials, hist);
+                       const float* diag, const float* P, const float* Q, int64_t vs, float* x, float* r,
+                       CRScalars* sc, BatchStatus* st, double* partials, double* hist, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_update, B, flags, cols, mass, diag, P, Q, vs, x, r, sc, st, partials, hist);
 }
 void launch_lpc_compose_z(const Batch& B, const float* r, const float* diag, const float* raw, const float* scale,
                           const float* weight, float* z, hipStream_t s) {

@@ -136,19 +136,19 @@ void launch_small_cr(PView P, const Batch& B, const int* list, int n, const floa
 // ---- flexible conjugate residual (learned preconditioner path)
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                     int netOn, double ratio, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+                     int netOn, double ratio, int mdirs, CRScalars* sc, BatchStatus* st, double* partials,
+                     hipStream_t s);
 void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
                        hipStream_t s);
 void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
-                   const float* stage, const float* z, const float* diag, float dt, float* Az, const float* Q0,
-                   const float* Q1, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
-void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, const float* diag, float* P0,
-                     float* P1, float* Q0, float* Q1, CRScalars* sc, BatchStatus* st, double* partials,
-                     hipStream_t s);
+                   const float* stage, const float* z, const float* diag, float dt, float* Az, const float* Q,
+                   int64_t vs, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
+// P, Q: fcr_m direction vectors each, vs floats apart
+void launch_fcr_orth(const Batch& B, const float* z, const float* Az, const float* r, const float* diag, float* P,
+                     float* Q, int64_t vs, CRScalars* sc, BatchStatus* st, double* partials, hipStream_t s);
 void launch_fcr_update(const Batch& B, const int8_t* flags, const ColliderSet* cols, const float* mass,
-                       const float* diag, const float* P0, const float* P1, const float* Q0, const float* Q1,
-                       float* x, float* r, CRScalars* sc, BatchStatus* st, double* partials, double* hist,
-                       hipStream_t s);
+                       const float* diag, const float* P, const float* Q, int64_t vs, float* x, float* r,
+                       CRScalars* sc, BatchStatus* st, double* partials, double* hist, hipStream_t s);
 // ---- learned preconditioner training on the optimal-step residual loss
 // 1 - <r, A z>_D^2 / (|r|_D^2 |A z|_D^2) per body
 // z = r / diag + scale clamp(raw) on bodies with weight > 0 (raw may be null: z = r / diag)

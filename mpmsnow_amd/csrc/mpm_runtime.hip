@@ -114,7 +114,8 @@ struct GridBatch {
   DevBuf tileAct;  // per tile: active in the CR loop (set by k_node_explicit)
   DevBuf pairCnt, pairPos, groups;  // chunk pairs of the CR loop's HVP
   DevBuf nrm;      // collider unit normal at nodes with a flag >= 0 (set with the flags)
-  DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR: residual and the ring of directions
+  DevBuf fr, fp0, fp1, fq0, fq1;  // flexible CR residual; training and grid-API scratch vectors
+  DevBuf fP, fQ;                   // flexible CR: the ring of directions p_j and A p_j (fcr_m vectors each)
   DevBuf smallList;               // bodies handed to the small-body solver
   std::vector<char> smallLayout;  // per body: laid out after the prefix (small-body solver candidate)
   std::vector<int> layoutOrder;   // bodies by ascending tileBase
@@ -229,6 +230,7 @@ struct mpm_context {
   LpcNet net;
   int lpcMode = 0;  // 0: diagonal-preconditioned CR; 1: learned flexible CR
   double lpcFallback = 0.95;  // fallback ratio of a network iteration's |r|^2 to the previous one
+  int lpcDirs = 2;            // directions the flexible CR keeps (1..kFcrMax)
   LpcCapture cap;
   // measurement
   struct ProfRec {
@@ -651,9 +653,13 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
   if (learned) {
     // flexible conjugate residual with the learned preconditioner (mpm_kernels.hip, k_fcr_*)
     for (DevBuf* d : {&fr, &fp0, &fp1, &fq0, &fq1}) d->ensure(vecBytes());
+    const int mdirs = c.lpcDirs;
+    const int64_t vs = 3 * (int64_t)NN;  // floats between stored direction vectors
+    fP.ensure(vecBytes() * mdirs);
+    fQ.ensure(vecBytes() * mdirs);
     const int netOn = c.net.initialised ? 1 : 0;
     hvp(f(x), true, f(stageA), sc(), -1);
-    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, c.lpcFallback, sc(),
+    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, c.lpcFallback, mdirs, sc(),
                     st(), partials.as<double>(), s);
     fetch_status_errors();
     bool netActive = netOn != 0;
@@ -673,12 +679,11 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
         }
         launch_fcr_diag_z(B, f(fr), f(diag), f(z), sc(), s);
         hvp(f(z), true, f(stageA), sc(), it);
-        launch_fcr_az(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ar), f(fq0), f(fq1),
+        launch_fcr_az(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ar), f(fQ), vs,
                       sc(), st(), partials.as<double>(), s);
-        launch_fcr_orth(B, f(z), f(Ar), f(fr), f(diag), f(fp0), f(fp1), f(fq0), f(fq1), sc(), st(),
-                        partials.as<double>(), s);
-        launch_fcr_update(B, flags.as<int8_t>(), c.cols(), f(mass), f(diag), f(fp0), f(fp1), f(fq0), f(fq1), f(x),
-                          f(fr), sc(), st(), partials.as<double>(), hist.as<double>(), s);
+        launch_fcr_orth(B, f(z), f(Ar), f(fr), f(diag), f(fP), f(fQ), vs, sc(), st(), partials.as<double>(), s);
+        launch_fcr_update(B, flags.as<int8_t>(), c.cols(), f(mass), f(diag), f(fP), f(fQ), vs, f(x), f(fr), sc(),
+                          st(), partials.as<double>(), hist.as<double>(), s);
       }
       fetch_status();
       if (netActive) {
@@ -1651,6 +1656,13 @@ int mpm_lpc_set_fallback(mpm_context* ctx, double ratio) {
   return guarded(ctx, [&] {
     if (!(ratio > 0.0 && ratio <= 1.0)) throw InvalidArg("fallback ratio must lie in (0, 1]");
     ctx->lpcFallback = ratio;
+  });
+}
+
+int mpm_lpc_set_directions(mpm_context* ctx, int m) {
+  return guarded(ctx, [&] {
+    if (m < 1 || m > kFcrMax) throw InvalidArg("the flexible CR keeps 1 to 8 directions");
+    ctx->lpcDirs = m;
   });
 }
 

@@ -143,13 +143,19 @@ def _largest(sim, st):
     return max(rows, key=lambda r: r["particles"]), rows
 
 
-def test_untrained_network_is_diagonal_cr():
+@pytest.mark.parametrize("dirs", [2, 8])
+def test_untrained_network_is_diagonal_cr(dirs):
     """Zero output layer: z = r / diag, so the flexible iteration in the diag^-1 inner product
-    is preconditioned CR; it meets |r|^2 < tol^2 |b|^2 in about the plain solver's iterations."""
-    say("fcr: contact snowball 20k, untrained network vs plain CR")
+    is preconditioned CR whatever the number of directions it keeps; it meets |r|^2 < tol^2 |b|^2
+    in about the plain solver's iterations."""
+    say("fcr: contact snowball 20k, untrained network (%d directions) vs plain CR" % dirs)
     its = []
     for mode in (0, 1):
         sc, sim = _small_scene(seed=4)
+        for bad in (0, 9):
+            with pytest.raises(G.MpmError):
+                sim.ctx.lpc_directions(bad)
+        sim.ctx.lpc_directions(dirs)
         for _ in range(2):
             sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
         sim.ctx.lpc_init(6)
