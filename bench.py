@@ -34,6 +34,10 @@ Other modes:
                        from the same warmed-up state, per-step CR iterations, fallback
                        triggers and capped solves; one JSON line (--stiff-particles,
                        --stiff-steps, --stiff-warmup)
+  --lpc-weights W, --lpc-fallback R, --lpc-dirs M
+                       network parameters (.npy), fallback ratio and flexible-CR
+                       direction count of the learned legs (defaults: the committed
+                       weights, 0.95, 2)
   --copy-calibration   also runs one 256 MiB device-to-device copy first (the
                        known-size kernel that scales the PMC byte counters)
   --pmc-summary FETCH_CSV WRITE_CSV [--pmc-out FILE]
