@@ -198,7 +198,7 @@ def learned_leg(steps, warmup, weights=None, fallback=None, dirs=None):
     return out
 
 
-def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None):
+def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None, plain_only=False):
     """Plain diagonal CR vs learned flexible CR on the stiff stress case (8 seeded boxes and spheres
     dropped onto the ground plane and an analytic sphere collider, E0 = 4.8e5 Pa, h = 1/512,
     dt = 2e-3), both from the state after `warmup` plain steps. Reports per-step CR iterations (sum
@@ -213,7 +213,8 @@ def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None):
     sc = scenes.stiff_drop(n=n, seed=seed)
     w = np.load(weights or os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
     out = {}
-    for mode, name in ((0, "plain"), (1, "learned")):
+    modes = ((0, "plain"),) if plain_only else ((0, "plain"), (1, "learned"))
+    for mode, name in modes:
         sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
                     spheres=sc["spheres"], gravity=[sc["gravity"]], v=sc["v"])
         sim.ctx.lpc_set_params(w)
@@ -227,6 +228,7 @@ def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None):
             progress("stiff leg: %s warm-up step %d, %d bodies, CR iterations sum %d max %d"
                      % (name, k, st["num_bodies"], st["total_iterations"], st["max_iterations"]))
         sim.ctx.lpc_mode(mode)
+        sim.profile(["hvp"], period=HVP_SAMPLE)
         rows = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -244,17 +246,25 @@ def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None):
                         st["max_iterations"], rows[-1]["fallbacks"]))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        hv = sim.profile_query()["hvp"]
+        sim.profile(False)
+        hms = hv["ms"] / max(1, hv["launches"])
+        hby = hv["bytes"] / max(1, hv["launches"])
         out[name] = {"value": n * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "hvp": {"launches_timed": hv["launches"], "ms_per_launch": hms, "bytes_per_launch": hby,
+                             "achieved_gbs": hby / (hms * 1e-3) / 1e9 if hms > 0 else 0.0},
                      "cr_iters_per_step_sum": float(np.mean([r["iters_sum"] for r in rows])),
                      "cr_iters_per_step_max": float(np.mean([r["iters_max"] for r in rows])),
                      "fallbacks_per_step": float(np.mean([r["fallbacks"] for r in rows])),
                      "capped_solves": int(sum(r["capped"] for r in rows)), "steps": rows}
         del sim
-    out["speedup"] = out["learned"]["value"] / out["plain"]["value"]
+    if not plain_only:
+        out["speedup"] = out["learned"]["value"] / out["plain"]["value"]
     out["config"] = ("stiff stress case: %d particles in 8 seeded boxes and spheres (seed %d), plane + analytic "
                      "sphere collider, E0=4.8e5, xi=10, h=1/512, dt=2e-3, CR tol 1e-5, %d steps after %d plain "
                      "warm-up steps" % (n, seed, steps, warmup))
     out["objects"] = sc["objects"]
+    out["warmup"] = warmup
     return out
 
 
@@ -445,6 +455,47 @@ def trace_summary(stats_csv, bench_log, out_path):
 
 
 SQ_FILE = os.path.join(ROOT, "profiles", "hvp_two_snowballs_1M_sq_counters.md")
+STIFF_STATS_FILE = os.path.join(ROOT, "profiles", "stiff_drop_16M_kernel_stats.md")
+
+
+def stiff_trace_summary(stats_csv, stiff_json, out_path):
+    """Kernel table of a rocprofv3 --kernel-trace --stats run of `bench.py --stiff --stiff-plain-only`, with
+    the HVP's algorithmic rate from the run's own HIP-event sample."""
+    rows = []
+    with open(stats_csv) as fh:
+        for r in csv.DictReader(fh):
+            rows.append((_short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]), float(r["Percentage"])))
+    rows.sort(key=lambda r: -r[2])
+    d = None
+    for line in open(stiff_json):
+        if line.strip().startswith("{"):
+            d = json.loads(line)
+    if d is None:
+        raise SystemExit("no JSON line in %s" % stiff_json)
+    p = d["plain"]
+    hvp_calls = sum(c for n, c, t, _ in rows if n.startswith("k_hvp"))
+    hvp_ns = sum(t for n, c, t, _ in rows if n.startswith("k_hvp"))
+    lines = ["# Kernel statistics, stiff stress case (16M particles, plain CR)", "",
+             "Command (from the repository root on the GPU host):", "",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- python bench.py --stiff "
+             "--stiff-plain-only --stiff-steps %d --stiff-warmup %d > <JSON>" % (len(p["steps"]), d["warmup"]),
+             "    python bench.py --stiff-trace-summary <D>/run_kernel_stats.csv <JSON>", "",
+             "Workload: " + d["config"] + ".", "",
+             "Timed steps under the profiler: %.1f ms/step, %.3g particle-steps/s, CR iterations per step %.0f "
+             "(sum over bodies), %.0f (slowest body)." % (p["ms_per_step"], p["value"], p["cr_iters_per_step_sum"],
+                                                        p["cr_iters_per_step_max"]), "",
+             "HVP (`k_hvp*`, every launch of the run, warm-up included): %d launches, mean %.1f us (rocprofv3). "
+             "bench.py's HIP events over the timed steps (one launch in %d): %d launches, mean %.1f us, %.3g "
+             "algorithmic bytes per launch (172 B per particle + 29 B per active node of the bodies still "
+             "iterating), %.0f GB/s = %.1f%% of 8 TB/s."
+             % (hvp_calls, hvp_ns / max(1, hvp_calls) / 1e3, HVP_SAMPLE, p["hvp"]["launches_timed"],
+                p["hvp"]["ms_per_launch"] * 1e3, p["hvp"]["bytes_per_launch"], p["hvp"]["achieved_gbs"],
+                100 * p["hvp"]["achieved_gbs"] / HBM_PEAK_GBS), "",
+             "| kernel | calls | total ms | mean us | share |", "|---|---|---|---|---|"]
+    for n, c, t, pct in rows[:20]:
+        lines.append("| `%s` | %d | %.2f | %.1f | %.1f%% |" % (n, c, t / 1e6, t / c / 1e3, pct))
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
 
 
 def read_sq():
@@ -477,6 +528,8 @@ def main():
     ap.add_argument("--stiff-particles", type=int, default=16000000)
     ap.add_argument("--stiff-steps", type=int, default=5)
     ap.add_argument("--stiff-warmup", type=int, default=12)
+    ap.add_argument("--stiff-plain-only", action="store_true", help="stiff case: the plain CR leg only")
+    ap.add_argument("--stiff-trace-summary", nargs=2, metavar=("STATS_CSV", "STIFF_JSON"))
     ap.add_argument("--lpc-weights", default=None, help="network parameters (.npy) for the learned legs")
     ap.add_argument("--lpc-fallback", type=float, default=None, help="fallback ratio of the learned legs")
     ap.add_argument("--lpc-dirs", type=int, default=None, help="directions the learned legs' flexible CR keeps")
@@ -491,6 +544,9 @@ def main():
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.pmc_out)
         return
+    if args.stiff_trace_summary:
+        stiff_trace_summary(args.stiff_trace_summary[0], args.stiff_trace_summary[1], STIFF_STATS_FILE)
+        return
     if args.sq_summary:
         sq_summary(args.sq_summary, args.sq_out)
         return
@@ -504,7 +560,8 @@ def main():
 
     if args.stiff:
         print(json.dumps(stiff_leg(args.stiff_particles, args.stiff_steps, args.stiff_warmup,
-                                   weights=args.lpc_weights, fallback=args.lpc_fallback, dirs=args.lpc_dirs)))
+                                   weights=args.lpc_weights, fallback=args.lpc_fallback, dirs=args.lpc_dirs,
+                                   plain_only=args.stiff_plain_only)))
         return
 
     import numpy as np
