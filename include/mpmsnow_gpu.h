@@ -87,6 +87,7 @@ typedef struct {
   int particles;           /* particles of the body (grid API: of the grid) */
   int active_nodes;        /* grid nodes with m > 0 */
   int fallbacks;           /* learned preconditioner: 1 if the body switched to the diagonal one */
+  int restarts;            /* CR restarts (p = z) where the expanded denominator lost its significance */
 } mpm_solve_stats;
 
 typedef struct {
@@ -142,7 +143,10 @@ int mpm_grid_destroy(mpm_grid* grid);
 /* m_n, m_min */
 int mpm_grid_dims(const mpm_grid* grid, int n[3], float min[3]);
 /* names: "masses" (n floats), "velocities", "prev_velocities" (3n floats),
- * "node_collided" (n int8: -1 none, -2 several, >= 0 object) */
+ * "node_collided" (n int8: -1 none, -2 several, >= 0 object);
+ * get only: "base_cells" (3 int32 per particle, in the order of the particle list given to
+ * mpm_grid_create): the stencil base node floor((x - m_min) / h) - 1 of each particle as the
+ * binning kernel computed it (ShapeFunctionIterator::initialize, Grid.cpp:1105-1136) */
 int mpm_grid_get(mpm_grid* grid, const char* name, void* out);
 int mpm_grid_set(mpm_grid* grid, const char* name, const void* in);
 /* computeParticleVolumes (Grid.cpp:265-292) */

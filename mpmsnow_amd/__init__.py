@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libmpmsnow_hip.so")
 
 MPM_OK = 0
+MPM_ERR_INVALID = 1
 
 VEC3 = {"p", "v"}
 MAT3 = {"F", "Fp", "FinvTrans", "R", "Ginv"}
@@ -43,7 +44,8 @@ class SolverParams(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("early_exit", C.c_int), ("rhs_norm2", C.c_double),
                 ("residual_norm2", C.c_double), ("history_length", C.c_int), ("history_capacity", C.c_int),
-                ("history", _dp), ("particles", C.c_int), ("active_nodes", C.c_int), ("fallbacks", C.c_int)]
+                ("history", _dp), ("particles", C.c_int), ("active_nodes", C.c_int), ("fallbacks", C.c_int),
+                ("restarts", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -186,7 +188,8 @@ def _stats_dict(st, hist):
     return {"iterations": st.iterations, "early_exit": bool(st.early_exit), "bnorm2": st.rhs_norm2,
             "residual_norm2_last": st.residual_norm2,
             "residual_norm2": hist[:st.history_length].copy() if hist is not None else None,
-            "particles": st.particles, "active_nodes": st.active_nodes, "fallbacks": st.fallbacks}
+            "particles": st.particles, "active_nodes": st.active_nodes, "fallbacks": st.fallbacks,
+            "restarts": st.restarts}
 
 
 class Context:
@@ -367,6 +370,7 @@ class Grid:
         g = C.c_void_p()
         ctx.check(lib().mpm_grid_create(ctx.ptr, pi, ii.size, h, pf, dim, C.byref(g)))
         self.ptr = g.value
+        self.num_particles = int(ii.size)
         n = np.zeros(3, dtype=np.int32)
         mn = np.zeros(3, dtype=np.float32)
         lib().mpm_grid_dims(self.ptr, n.ctypes.data_as(_ip), mn.ctypes.data_as(_fp))
@@ -383,6 +387,13 @@ class Grid:
     __del__ = close
 
     def get(self, name):
+        """Grid variables in the reference's dense node order; "base_cells" gives each particle's
+        stencil base node (the binning kernel's, Grid.cpp:1105-1136) as an (n, 3) int array in the
+        order of the particle list the grid was made from."""
+        if name == "base_cells":
+            out = np.zeros((self.num_particles, 3), dtype=np.int32)
+            self.ctx.check(lib().mpm_grid_get(self.ptr, name.encode(), out.ctypes.data))
+            return out
         if name == "node_collided":
             out = np.zeros(self.num_nodes, dtype=np.int8)
         else:

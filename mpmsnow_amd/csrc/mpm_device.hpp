@@ -148,6 +148,7 @@ struct CRScalars {
   int iter;                // current iteration index i
   int iterations;          // x updates performed
   int early_exit;          // zero denominator
+  int restarts;            // CR recurrence restarts (beta = 0) after a cancelled denominator expansion
   int max_iters;
   int errors;              // ErrorBits
   int active_nodes;        // nodes with m > 0 (measurement)

@@ -43,6 +43,10 @@ struct DevBuf {
     cap = c;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
 };
 
 struct PinnedBuf {

@@ -218,9 +218,8 @@ __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc,
 // Single-word variant for the HVP, the solver's inner kernel: one integer per value and a
 // scale per component c from T_c, the chunk's sum of per-particle bounds on that component's
 // contributions (tight: the stencil's gradient weights are at most (2/3)^3 / h), so that sums
-// stay below 2^30; each contribution is truncated toward zero. Resolution 2^-30 of T_c: below the
-// float rounding of the node sums it feeds, with half the LDS atomics and a fifth of the
-// conversion work of the split form.
+// stay below 2^30; each contribution is rounded to the nearest unit. Resolution 2^-30 of T_c,
+// with half the LDS atomics and a fifth of the conversion work of the split form.
 template <int NC> struct FxFoot1 {
   int v[NC][kFootLds];
 };
@@ -302,10 +301,19 @@ __device__ __forceinline__ void sep_gather(const float* v, const Stencil& st, fl
     out[2] = fmaf(hz[k], st.dw[2][k], out[2]);
   }
 }
-// acc(i, j, k) += trunc(f . ghat(i, j, k)) over the stencil, with f . ghat evaluated as
-// P(j, k) dwx_i + Q(j, k) wx_i, P = f0 wy_j wz_k, Q = f1 dwy_j wz_k + f2 wy_j dwz_k. Truncation
-// is one instruction (round-to-nearest is two); its error, under one fixed-point unit, is far
-// below the float rounding of the scaled contribution itself (2^-24 of values up to 2^29).
+// floor(x + 0.5) as an int in one instruction (v_cvt_rpi_i32_f32; the compiler's forms of a
+// rounding conversion take two or three)
+__device__ __forceinline__ int cvt_round(float x) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+// acc(i, j, k) += round(f . ghat(i, j, k)) over the stencil, with f . ghat evaluated as
+// P(j, k) dwx_i + Q(j, k) wx_i, P = f0 wy_j wz_k, Q = f1 dwy_j wz_k + f2 wy_j dwz_k. The chunk's
+// scale is shared by all its nodes, so a node that few or light particles reach holds a value of
+// few fixed-point units: rounding to nearest keeps each contribution's error within half a unit
+// and unbiased (truncation toward zero, the same instruction count, erred by up to a unit, always
+// toward zero, which left such nodes' HVP values biased by up to 1e-3 of themselves).
 __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f0, float f1, float f2) {
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -315,7 +323,7 @@ __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f
       const float Q = fmaf(f1, st.dw[1][j] * st.w[2][k], f2 * (st.w[1][j] * st.dw[2][k]));
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        atomicAdd(&acc[i + 8 * j + kFootSZ * k], (int)fmaf(P, st.dw[0][i], Q * st.w[0][i]));
+        atomicAdd(&acc[i + 8 * j + kFootSZ * k], cvt_round(fmaf(P, st.dw[0][i], Q * st.w[0][i])));
     }
 }
 
@@ -603,7 +611,6 @@ __global__ void k_permute(PView src, PView dst, const int* new_to_old, int n, co
 __global__ void k_gather_slots(const int* slot_of, const int* orig, int n, int* slots) {
   GRID_STRIDE(k, n) slots[k] = slot_of[orig[k]];
 }
-
 // ------------------------------------------------------------ body partition (Sim.cpp:136-203)
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
   return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) & mask;
@@ -1071,6 +1078,27 @@ __global__ void k_bin(PView P, const int* slots, const int* labels, int n, const
     int t = G.tileBase + t3[0] + G.T[0] * (t3[1] + G.T[1] * t3[2]);
     key[k] = t | (code << kKeyCodeShift);  // the stencil base node within the tile, for k_sort_tiles
     rank[k] = wave_aggregated_inc(count, t);
+  }
+}
+// a binning key back to the reference's stencil base node (i, j, k) in its body's grid:
+// 4 x the tile's coordinates plus the base node's position within the tile; -1 for an unbinned key
+__global__ void k_decode_bin(const int* key, int n, const BodyGrid* bodies, const int* tileBody, int* cells) {
+  GRID_STRIDE(k, n) {
+    const int kc = key[k];
+    int c[3] = {-1, -1, -1};
+    if (kc >= 0) {
+      const int t = kc & ((1 << kKeyCodeShift) - 1);
+      const int code = kc >> kKeyCodeShift;
+      const BodyGrid& G = bodies[tileBody[t]];
+      int tx, ty, tz;
+      tile_xyz(G, t - G.tileBase, tx, ty, tz);
+      c[0] = 4 * tx + (code & 3);
+      c[1] = 4 * ty + ((code >> 2) & 3);
+      c[2] = 4 * tz + ((code >> 4) & 3);
+    }
+    cells[3 * k] = c[0];
+    cells[3 * k + 1] = c[1];
+    cells[3 * k + 2] = c[2];
   }
 }
 __global__ void k_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
@@ -2024,10 +2052,14 @@ __device__ __forceinline__ void cr_update_x_node(const NodeCtx& c, const int8_t*
   st3(x, c.i, xv);
   st3(z, c.i, zz);
 }
-__device__ __forceinline__ bool cr_update_x_finish(CRScalars& S, const double (&tot)[1], int body, double* hist) {
+template <int NV>
+__device__ __forceinline__ bool cr_update_x_finish(CRScalars& S, const double (&tot)[NV], int body, double* hist) {
   int it = S.iter;
   S.rnorm2 = tot[0];
   S.iterations = it + 1;
+  // after a p / Ap update (k_cr_update_px) the pass also summed <Ap, M^-1 Ap> of the Ap it
+  // stored: the next denominator's expansion starts from the stored vector's own value
+  if constexpr (NV > 1) S.ApdPAp = tot[1];
   if (hist && it < kHistCap) hist[(int64_t)body * kHistCap + it] = tot[0];
   return it >= S.max_iters || tot[0] < S.threshold;
 }
@@ -2042,13 +2074,19 @@ __global__ void k_cr_update_x(Batch B, const int8_t* flags, const ColliderSet* c
     cr_update_x_node(c, flags, cols, p, Ap, diag, (float)S.alpha, x, z, s);
   double tot[1];
   if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
-    if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
+    if (cr_update_x_finish<1>(S, tot, c.b, hist)) set_done(S, bst);
 }
 // iteration part 2 (ConjugateResiduals.cpp:130-145): Ar = A z ; rz = <z, Ar> ; beta. The
 // next iteration's denominator <Ap', M^-1 Ap'> with Ap' = Ar + beta Ap (:147-151, :86-102) is
 // taken from the same pass by expanding it: <Ar, M^-1 Ar> + 2 beta <Ar, M^-1 Ap> + beta^2
-// <Ap, M^-1 Ap>, the last term being the current denominator. The p / Ap update then runs in
-// the next x update's pass (k_cr_update_px) instead of a pass of its own.
+// <Ap, M^-1 Ap>, the last term summed over the stored Ap by the pass that wrote it. The p / Ap
+// update then runs in the next x update's pass (k_cr_update_px) instead of a pass of its own.
+// The expansion is a difference of nearly equal terms when Ar is nearly parallel to -beta Ap
+// (a stalling or indefinite solve); summing its last term over the Ap actually stored keeps it
+// in step with the direct form. If it still comes out below kCrDenKeep of the terms' size
+// (rounding noise, or not positive), the recurrence restarts with beta = 0 (p = z, Ap = Ar),
+// whose denominator <Ar, M^-1 Ar> is summed directly.
+constexpr double kCrDenKeep = 1e-10;
 __device__ __forceinline__ void cr_ar_core(const NodeCtx& c, const ColliderSet* cols, int fl, float m,
                                            const float zz[3], const float df[3], const float ap[3],
                                            const float dg[3], float dt2, float ar[3], double (&s)[3]) {
@@ -2081,9 +2119,14 @@ __device__ __forceinline__ bool cr_ar_finish(CRScalars& S, const double (&tot)[3
     S.early_exit = 1;
     return true;
   }
-  const double beta = tot[0] / rz_old;
+  double beta = tot[0] / rz_old;
+  double den = tot[1] + 2.0 * beta * tot[2] + beta * beta * S.ApdPAp;
+  if (!(den > kCrDenKeep * (tot[1] + beta * beta * S.ApdPAp))) {
+    beta = 0.0;
+    den = tot[1];
+    S.restarts = S.restarts + 1;
+  }
   S.beta = beta;
-  const double den = tot[1] + 2.0 * beta * tot[2] + beta * beta * S.ApdPAp;
   S.ApdPAp = den;
   S.iter = S.iter + 1;
   if (den == 0.0) {
@@ -2108,20 +2151,24 @@ __global__ void k_cr_ar(Batch B, const float* mass, const int8_t* flags, const C
 }
 // iteration part 3 of the previous iteration (ConjugateResiduals.cpp:147-151: p = z + beta p,
 // Ap = Ar + beta Ap) and part 1 of this one (x update, termination), in one pass
+// s[1] sums <Ap, M^-1 Ap> of the updated Ap as stored (the next expansion's last term)
 __device__ __forceinline__ void cr_update_px_core(const NodeCtx& c, const ColliderSet* cols, int fl, float beta,
                                                   float alpha, float xv[3], float pv[3], float ap[3],
                                                   const float ar[3], const float dg[3], float zz[3],
-                                                  double (&s)[1]) {
+                                                  double (&s)[2]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     pv[a] = zz[a] + beta * pv[a];
     ap[a] = ar[a] + beta * ap[a];
+    s[1] += (double)ap[a] * (ap[a] / dg[a]);
   }
-  cr_update_x_core(c, cols, fl, alpha, xv, pv, ap, dg, zz, s);
+  double sx[1] = {s[0]};
+  cr_update_x_core(c, cols, fl, alpha, xv, pv, ap, dg, zz, sx);
+  s[0] = sx[0];
 }
 __device__ __forceinline__ void cr_update_px_node(const NodeCtx& c, const int8_t* flags, const ColliderSet* cols,
                                                   const float* Ar, const float* diag, float beta, float alpha,
-                                                  float* p, float* Ap, float* x, float* z, double (&s)[1]) {
+                                                  float* p, float* Ap, float* x, float* z, double (&s)[2]) {
   float xv[3], pv[3], ap[3], ar[3], dg[3], zz[3];
   ld3(x, c.i, xv);
   ld3(p, c.i, pv);
@@ -2141,12 +2188,12 @@ __global__ void k_cr_update_px(Batch B, const int8_t* flags, const ColliderSet* 
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
-  double s[1] = {0.0};
+  double s[2] = {0.0, 0.0};
   if (!B.tileActive || B.tileActive[c.T])
     cr_update_px_node(c, flags, cols, Ar, diag, (float)S.beta, (float)S.alpha, p, Ap, x, z, s);
-  double tot[1];
-  if (body_sum<1>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
-    if (cr_update_x_finish(S, tot, c.b, hist)) set_done(S, bst);
+  double tot[2];
+  if (body_sum<2>(s, c.G, partials, body_tickets(B, c.b, 0), tot) && threadIdx.x == 0)
+    if (cr_update_x_finish<2>(S, tot, c.b, hist)) set_done(S, bst);
 }
 
 // ------------------------------------------------------------ small bodies
@@ -2353,12 +2400,14 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
       }, t1);
     } else {
       const float beta = (float)L.beta;
-      tiny_sweep<1>(G, R1, [&](int j, double (&v)[1]) {
+      double t2p[2];
+      tiny_sweep<2>(G, R2, [&](int j, double (&v)[2]) {
         cr_update_px_core(cn[j], cols, flr[j], beta, alpha, xr[j], pr[j], apr[j], arr[j], dgr[j], zr[j], v);
         tiny_st3(T, G, cn[j], zr[j]);
-      }, t1);
+      }, t2p);
+      if (threadIdx.x == 0) end = cr_update_x_finish<2>(L, t2p, body, hist);
     }
-    if (threadIdx.x == 0) end = cr_update_x_finish(L, t1, body, hist);
+    if (it == 0 && threadIdx.x == 0) end = cr_update_x_finish<1>(L, t1, body, hist);
     __syncthreads();
     if (end) break;
     tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
@@ -2454,11 +2503,13 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
       }, t1);
     } else {
       const float beta = (float)L.beta;
-      small_sweep<1>(G, body, B.nrm, R1, [&](const NodeCtx& c, double (&v)[1]) {
+      double t2p[2];
+      small_sweep<2>(G, body, B.nrm, R2, [&](const NodeCtx& c, double (&v)[2]) {
         cr_update_px_node(c, flags, cols, Ar, diag, beta, alpha, p, Ap, x, z, v);
-      }, t1);
+      }, t2p);
+      if (threadIdx.x == 0) end = cr_update_x_finish<2>(L, t2p, body, hist);
     }
-    if (threadIdx.x == 0) end = cr_update_x_finish(L, t1, body, hist);
+    if (it == 0 && threadIdx.x == 0) end = cr_update_x_finish<1>(L, t1, body, hist);
     __syncthreads();
     if (end) break;
     hvp(z);
@@ -3086,6 +3137,9 @@ void launch_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, 
 void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
                 int* count, int* err, hipStream_t s) {
   if (n > 0) LAUNCH(k_bin, grid_for(n, 16384), P, slots, labels, n, bodies, key, rank, count, err);
+}
+void launch_decode_bin(const int* key, int n, const BodyGrid* bodies, const int* tileBody, int* cells, hipStream_t s) {
+  if (n > 0) LAUNCH(k_decode_bin, grid_for(n), key, n, bodies, tileBody, cells);
 }
 void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
                     int8_t* pcode, hipStream_t s) {

@@ -50,6 +50,8 @@ void launch_tile_body(const BodyGrid* bodies, const int* order, int nb, int NT, 
                       hipStream_t s);
 void launch_bin(PView P, const int* slots, const int* labels, int n, const BodyGrid* bodies, int* key, int* rank,
                 int* count, int* err, hipStream_t s);
+// binning keys (k_bin) -> the reference's stencil base node, 3 ints per key (-1 when unbinned)
+void launch_decode_bin(const int* key, int n, const BodyGrid* bodies, const int* tileBody, int* cells, hipStream_t s);
 void launch_scatter(const int* slots, const int* key, const int* rank, const int* offset, int n, int* perm,
                     int8_t* pcode, hipStream_t s);
 void launch_chunk_counts(const int* count, int NT, int* nch, hipStream_t s);

@@ -162,6 +162,12 @@ template <class T> MPM_HD T ddot(const T* a, const T* b) {
 // descending; U, V orthogonal (the conventions of Eigen's JacobiSVD). All the
 // model derives from it (R, F^-T, S, J, the clamp) is independent of the
 // sign/order freedom of the decomposition for nonsingular F.
+// A pair of columns is rotated only while their inner product exceeds twice the unit roundoff of
+// T relative to their norms (|g| > 2 eps sqrt(a b), the threshold of Eigen's JacobiSVD): below it
+// they are orthogonal to working precision. Without it an F that is the identity up to rounding
+// (equal column norms, off-diagonal entries of 1e-16) takes 45-degree rotations whose own
+// rounding leaves R off by ~1e-7, i.e. a spurious stress on undeformed snow.
+template <class T> MPM_HD T jacobi_skip_tol() { return sizeof(T) == 4 ? T(1.1920929e-7) : T(2.220446049250313e-16); }
 template <class T, int SWEEPS> MPM_HD void svd3_jacobi(const T* A, T* U, T* s, T* V) {
   T B[9], W[9] = {T(1), T(0), T(0), T(0), T(1), T(0), T(0), T(0), T(1)};
 #pragma unroll
@@ -175,7 +181,7 @@ template <class T, int SWEEPS> MPM_HD void svd3_jacobi(const T* A, T* U, T* s, T
       T a = B[3 * p] * B[3 * p] + B[3 * p + 1] * B[3 * p + 1] + B[3 * p + 2] * B[3 * p + 2];
       T b = B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] + B[3 * q + 2] * B[3 * q + 2];
       T g = B[3 * p] * B[3 * q] + B[3 * p + 1] * B[3 * q + 1] + B[3 * p + 2] * B[3 * q + 2];
-      if (g != T(0)) {
+      if (mabs(g) > jacobi_skip_tol<T>() * msqrt(a * b)) {
         T zeta = (b - a) / (T(2) * g);
         T t = (zeta >= T(0) ? T(1) : T(-1)) / (mabs(zeta) + msqrt(T(1) + zeta * zeta));
         T cs = T(1) / msqrt(T(1) + t * t);

@@ -31,6 +31,7 @@
 using namespace mpm;
 using namespace mpmh;
 
+
 namespace {
 
 struct VarInfo {
@@ -93,6 +94,7 @@ struct SolveRecord {
   int iterations = 0;
   int early_exit = 0;
   int fallbacks = 0;  // learned preconditioner: switched to the diagonal one
+  int restarts = 0;   // CR recurrence restarts after a cancelled denominator expansion
   double rhs_norm2 = 0, residual_norm2 = 0;
   int np = 0, active = 0;
   std::vector<double> history;
@@ -120,6 +122,7 @@ struct GridBatch {
   std::vector<char> smallLayout;  // per body: laid out after the prefix (small-body solver candidate)
   std::vector<int> layoutOrder;   // bodies by ascending tileBase
   int prefixTiles = 0, prefixChunks = 0;  // tiles / chunks of the bodies laid out first
+  DevBuf* keepKey = nullptr;  // when set, build() keeps a copy of the binning's keys here (caller order)
   LpcLevels levels;
   bool levelsBuilt = false;
   uint64_t version = 0;  // bumped by every build
@@ -173,6 +176,8 @@ struct mpm_grid {
   mpm_context* ctx = nullptr;
   GridBatch batch;
   DevBuf slots;
+  DevBuf binKey, cells;  // the binning's key per particle of the grid (caller order) and its decoding
+  int64_t count = 0;
 };
 
 // Residuals of one solve for training the learned preconditioner, and the training settings
@@ -493,6 +498,10 @@ void GridBatch::build(const int* slotList, const int* labelList, int n, int nbod
   c.clear_err();
   launch_bin(c.view(), slotList, labelList, n, bodies.as<BodyGrid>(), key.as<int>(), rank.as<int>(), count.as<int>(),
              c.errp(), s);
+  if (keepKey) {  // k_sort_tiles reuses key as scratch below
+    keepKey->ensure(sizeof(int) * n);
+    HIPCHECK(hipMemcpyAsync(keepKey->p, key.p, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+  }
   scan_exclusive(count.as<int>(), offset.as<int>(), NT, totals.as<int>(), scantmp.as<int>(), s);
   launch_chunk_counts(count.as<int>(), NT, nch.as<int>(), s);
   scan_exclusive(nch.as<int>(), chunkOff.as<int>(), NT, totals.as<int>() + 1, scantmp.as<int>(), s);
@@ -847,6 +856,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     r.residual_norm2 = hsc[bi].rnorm2;
     r.active = hsc[bi].active_nodes;
     r.fallbacks = learned ? hsc[bi].fallbacks : 0;
+    r.restarts = learned ? 0 : hsc[bi].restarts;
     r.history.clear();
     if (!r.small) maxIt = std::max(maxIt, r.iterations);
     raise_errors(hsc[bi].errors);
@@ -1047,6 +1057,7 @@ void ensure_body_lists(const mpm_context& cc) {
 void fill_stats(const SolveRecord& r, mpm_solve_stats* st) {
   if (!st) return;
   st->fallbacks = r.fallbacks;
+  st->restarts = r.restarts;
   st->particles = r.np;
   st->active_nodes = r.active;
   st->iterations = r.iterations;
@@ -1239,8 +1250,12 @@ int mpm_model_stress_differential(mpm_context* ctx, const int32_t* p, const floa
 int mpm_set_colliders(mpm_context* ctx, const mpm_collider* cols, int n) {
   return guarded(ctx, [&] {
     if (n < 0 || n > kMaxColliders) throw InvalidArg("at most 16 colliders are supported");
+    if (n > 0 && !cols) throw InvalidArg("null collider array");
+    // pass 1: validate every collider before any device memory is touched, so that a rejected
+    // call leaves the previous collider set (and the volumes its device copy points into) intact
     ColliderSet cs{};
     cs.n = n;
+    int64_t npts[kMaxColliders] = {};
     for (int i = 0; i < n; ++i) {
       Collider& o = cs.o[i];
       o.type = cols[i].type;
@@ -1267,20 +1282,32 @@ int mpm_set_colliders(mpm_context* ctx, const mpm_collider* cols, int n) {
           o.origin[a] = q.origin[a];
         }
         o.voxel = q.voxel;
-        ctx->sdfPhi[i].ensure(sizeof(float) * np);
-        HIPCHECK(hipMemcpyAsync(ctx->sdfPhi[i].p, q.phi, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
-        o.phi = ctx->sdfPhi[i].as<float>();
-        if (q.phi_velocity) {
-          ctx->sdfVel[i].ensure(sizeof(float) * 3 * np);
-          HIPCHECK(hipMemcpyAsync(ctx->sdfVel[i].p, q.phi_velocity, sizeof(float) * 3 * np, hipMemcpyHostToDevice,
-                                  ctx->stream));
-          o.vel = ctx->sdfVel[i].as<float>();
-        }
+        npts[i] = np;
       }
     }
-    ctx->hcols = cs;
-    HIPCHECK(hipMemcpyAsync(ctx->cols_dev.p, &ctx->hcols, sizeof(ColliderSet), hipMemcpyHostToDevice, ctx->stream));
+    // pass 2: the volumes go to fresh buffers; the context's buffers are replaced only once the
+    // new set is on the device (the old ones are freed on leaving this scope, after the wait)
+    DevBuf phi[kMaxColliders], vel[kMaxColliders];
+    for (int i = 0; i < n; ++i) {
+      if (cs.o[i].type != 2) continue;
+      const mpm_collider& q = cols[i];
+      phi[i].ensure(sizeof(float) * npts[i]);
+      HIPCHECK(hipMemcpyAsync(phi[i].p, q.phi, sizeof(float) * npts[i], hipMemcpyHostToDevice, ctx->stream));
+      cs.o[i].phi = phi[i].as<float>();
+      if (q.phi_velocity) {
+        vel[i].ensure(sizeof(float) * 3 * npts[i]);
+        HIPCHECK(hipMemcpyAsync(vel[i].p, q.phi_velocity, sizeof(float) * 3 * npts[i], hipMemcpyHostToDevice,
+                                ctx->stream));
+        cs.o[i].vel = vel[i].as<float>();
+      }
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->cols_dev.p, &cs, sizeof(ColliderSet), hipMemcpyHostToDevice, ctx->stream));
     ctx->sync();
+    ctx->hcols = cs;
+    for (int i = 0; i < kMaxColliders; ++i) {
+      ctx->sdfPhi[i].swap(phi[i]);
+      ctx->sdfVel[i].swap(vel[i]);
+    }
   });
 }
 
@@ -1309,6 +1336,8 @@ int mpm_grid_create(mpm_context* ctx, const int32_t* particles, int64_t count, f
     if (!(h > 0)) throw InvalidArg("grid size must be positive");
     ctx->slots_from_orig(particles, count, g->slots);
     float z[3] = {0, 0, 0};
+    g->count = count;
+    g->batch.keepKey = &g->binKey;
     g->batch.build(g->slots.as<int>(), nullptr, (int)count, 1, h, fv ? fv : z, false);
     ctx->check_errors();
   });
@@ -1348,6 +1377,14 @@ int mpm_grid_get(mpm_grid* grid, const char* name, void* out) {
       B.dense2.ensure(nd);
       launch_to_dense_i8(B.flags.as<int8_t>(), B.dense2.as<int8_t>(), G, grid->ctx->stream);
       HIPCHECK(hipMemcpyAsync(out, B.dense2.p, nd, hipMemcpyDeviceToHost, grid->ctx->stream));
+      grid->ctx->sync();
+    } else if (nm == "base_cells") {
+      // the stencil base node of each particle as the binning kernel computed it
+      const int64_t n = grid->count;
+      grid->cells.ensure(sizeof(int) * 3 * std::max<int64_t>(n, 1));
+      launch_decode_bin(grid->binKey.as<int>(), (int)n, B.bodies.as<BodyGrid>(), B.tileBody.as<int>(),
+                        grid->cells.as<int>(), grid->ctx->stream);
+      HIPCHECK(hipMemcpyAsync(out, grid->cells.p, sizeof(int) * 3 * n, hipMemcpyDeviceToHost, grid->ctx->stream));
       grid->ctx->sync();
     } else
       throw InvalidArg("unknown grid variable '" + nm + "'");

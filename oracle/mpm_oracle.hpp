@@ -1025,7 +1025,8 @@ template <class S> struct Grid {
   // Grid.cpp:889-1016
   void updateGridVelocities(S dt, const ConstitutiveModel<S>& model, const CollisionSet<S>& cols, const ForceSet<S>& fields,
                             Termination<S>& term, SolveStats<S>* stats = nullptr,
-                            const std::function<void(Vec<S>&)>* debug = nullptr, Vec<S>* rhsOut = nullptr) {
+                            const std::function<void(Vec<S>&)>* debug = nullptr, Vec<S>* rhsOut = nullptr,
+                            V3<S>* guessOut = nullptr) {
     Vec<S> em;
     calculateExplicitMomenta(em, nodeCollided, dt, model, cols, fields);
     Vec<S> vc;
@@ -1044,6 +1045,7 @@ template <class S> struct Grid {
     if (totalMass == 0) guess = V3<S>();
     else if (collidedMass == 0) guess = totalMomentum / totalMass;
     else guess = collidedMomentum / collidedMass;
+    if (guessOut) *guessOut = guess;
     for (size_t idx = 0; idx < masses.size(); ++idx)
       for (int c = 0; c < 3; ++c) velocities[3 * idx + c] = guess[c];
     ImplicitMatrix A(*this, model, cols, dt);

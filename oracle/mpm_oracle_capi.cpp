@@ -483,20 +483,46 @@ int orc_grid_collision_velocities(void* gh, void* ch, const signed char* collide
 }
 // stats: [iterations, bNorm2, earlyExit]; hist: ||r||^2 per termination call
 int orc_grid_update_velocities(void* gh, double dt, void* mh, void* ch, void* fh, int maxIters, double tol,
-                               double* stats, double* hist, int histCap, int* histLen, double* rhsOut) {
+                               double* stats, double* hist, int histCap, int* histLen, double* rhsOut,
+                               double* guessOut) {
   return guard([&] {
     DISPATCH(gh, {
       Grid<S>& g = *static_cast<GridH<S>*>(gh)->g;
       Termination<S> t(maxIters, S(tol));
       SolveStats<S> st;
       Vec<S> rhs;
+      V3<S> guess;
       g.updateGridVelocities(S(dt), *static_cast<ModelH<S>*>(mh)->m, static_cast<ColsH<S>*>(ch)->c,
-                             static_cast<FieldsH<S>*>(fh)->f, t, &st, nullptr, rhsOut ? &rhs : nullptr);
+                             static_cast<FieldsH<S>*>(fh)->f, t, &st, nullptr, rhsOut ? &rhs : nullptr, &guess);
+      if (guessOut) for (int c = 0; c < 3; ++c) guessOut[c] = double(guess[c]);
       if (stats) { stats[0] = st.iterations; stats[1] = double(st.bNorm2); stats[2] = st.earlyExit; }
       int L = int(t.history.size());
       if (histLen) *histLen = L;
       if (hist) for (int i = 0; i < L && i < histCap; ++i) hist[i] = double(t.history[i]);
       if (rhsOut) fromS(rhs, rhsOut);
+    });
+  });
+}
+// updateGridVelocities with the reference's LinearSolver::Debug hook (TestGrid.cpp:398-453):
+// every CR iterate x (velocities relative to the colliders, after each x update) is kept, the
+// first iterCap of them in iters (3 n doubles each); *count = the number of hook calls
+int orc_grid_update_velocities_iterates(void* gh, double dt, void* mh, void* ch, void* fh, int maxIters, double tol,
+                                        double* iters, int iterCap, int* count) {
+  return guard([&] {
+    DISPATCH(gh, {
+      Grid<S>& g = *static_cast<GridH<S>*>(gh)->g;
+      Termination<S> t(maxIters, S(tol));
+      SolveStats<S> st;
+      const size_t nv = g.velocities.size();
+      int calls = 0;
+      std::function<void(Vec<S>&)> hook = [&](Vec<S>& x) {
+        if (calls < iterCap)
+          for (size_t i = 0; i < nv; ++i) iters[(size_t)calls * nv + i] = double(x[i]);
+        ++calls;
+      };
+      g.updateGridVelocities(S(dt), *static_cast<ModelH<S>*>(mh)->m, static_cast<ColsH<S>*>(ch)->c,
+                             static_cast<FieldsH<S>*>(fh)->f, t, &st, &hook, nullptr);
+      if (count) *count = calls;
     });
   });
 }

@@ -76,7 +76,9 @@ def lib():
             "orc_grid_explicit_momenta": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, _dp, _bp]),
             "orc_grid_collision_velocities": (C.c_int, [C.c_void_p, C.c_void_p, _bp, _dp]),
             "orc_grid_update_velocities": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
-                                                     C.c_int, C.c_double, _dp, _dp, C.c_int, _ip, _dp]),
+                                                     C.c_int, C.c_double, _dp, _dp, C.c_int, _ip, _dp, _dp]),
+            "orc_grid_update_velocities_iterates": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                                              C.c_void_p, C.c_int, C.c_double, _dp, C.c_int, _ip]),
             "orc_grid_update_F": (C.c_int, [C.c_void_p, C.c_double]),
             "orc_grid_update_pv": (C.c_int, [C.c_void_p]),
             "orc_voxel_sort": (C.c_int, [C.c_int, _ip, C.c_long, C.c_double, _dp]),
@@ -355,14 +357,25 @@ class Grid(_H):
         hist = np.zeros(hist_cap)
         hl = C.c_int()
         rhs = np.zeros(3 * self.num_nodes) if want_rhs else None
+        guess = np.zeros(3)
         _check(lib().orc_grid_update_velocities(self.ptr, dt, model.ptr, cols.ptr, fields.ptr, max_iters, tol,
                                                 stats.ctypes.data_as(_dp), hist.ctypes.data_as(_dp), hist_cap,
-                                                C.byref(hl), rhs.ctypes.data_as(_dp) if want_rhs else None))
+                                                C.byref(hl), rhs.ctypes.data_as(_dp) if want_rhs else None,
+                                                guess.ctypes.data_as(_dp)))
         out = {"iterations": int(stats[0]), "bnorm2": stats[1], "early_exit": bool(stats[2]),
-               "residual_norm2": hist[:min(hl.value, hist_cap)].copy()}
+               "residual_norm2": hist[:min(hl.value, hist_cap)].copy(), "guess": guess}
         if want_rhs:
             out["rhs"] = rhs
         return out
+
+    def update_grid_velocities_iterates(self, dt, model, cols, fields, max_iters, tol, cap=512):
+        """The solve of update_grid_velocities with the reference's debug hook: the CR iterates x
+        (relative to the colliders, one per x update), as a (calls, 3 n) array."""
+        its = np.zeros((cap, 3 * self.num_nodes))
+        cnt = C.c_int()
+        _check(lib().orc_grid_update_velocities_iterates(self.ptr, dt, model.ptr, cols.ptr, fields.ptr, max_iters,
+                                                         tol, its.ctypes.data_as(_dp), cap, C.byref(cnt)))
+        return its[:min(cnt.value, cap)].copy()
 
     def update_deformation_gradients(self, dt):
         _check(lib().orc_grid_update_F(self.ptr, dt))

@@ -98,7 +98,10 @@ def test_network_gradients_match_oracle():
 
 
 def _small_scene(seed=3):
+    """The contact snowball lowered to 5 mm above the ground, so that it is in contact from its
+    third step on (in free fall the uniform guess solves the system at once)."""
     sc = scenes.snowball_drop(n=20000, seed=seed, contact=True)
+    sc["x"] = (sc["x"] - np.float32([0.0, 0.095, 0.0])).astype(np.float32)
     sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
                 v=sc["v"])
     return sc, sim
@@ -224,6 +227,7 @@ def test_ablation_trained_network_against_plain_cr():
     assumed)."""
     def scene():
         sc = scenes.snowball_drop(n=20000, seed=7, contact=True)
+        sc["x"] = (sc["x"] - np.float32([0.0, 0.095, 0.0])).astype(np.float32)
         sc["material"]["E"] = 1.4e6
         sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
                     gravity=[sc["gravity"]], v=sc["v"])

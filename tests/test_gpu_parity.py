@@ -94,20 +94,64 @@ def test_binning_and_p2g(oracle):
     # bit-exact geometry: n and m_min (float) equal the reference formula
     assert (g.n == og.n).all()
     assert (g.min == og.min.astype(np.float32)).all()
-    # bit-exact base cells: compare each particle's stencil base via the oracle iterator
+    # bit-exact base cells: the binning kernel's stencil base of every particle against the
+    # oracle's ShapeFunctionIterator (Grid.cpp:1105-1136)
     x = P.ctx.get("p")
-    base_gpu = np.floor((x - g.min) / np.float32(h)).astype(np.int64) - 1
-    for k in range(0, P.n, 97):
-        nodes, w, dw = og.stencil(x[k].astype(np.float64))
-        assert (nodes[0] == base_gpu[k]).all()
+    base_gpu = g.get("base_cells")
+    base_orc = np.array([og.stencil(x[k].astype(np.float64))[0][0] for k in range(P.n)])
+    assert (base_gpu == base_orc).all()
     M = g.get("masses")
     V = g.get("velocities")
     assert relerr(M, og64.get("masses")) < REL
     assert relerr(M, og.get("masses")) < REL
+    # grid momenta m_i v_i (the velocity splat divides per contribution, Grid.cpp:135-139)
     Mo = og64.get("masses")
+    mom = np.repeat(M, 3) * V
+    mom64 = np.repeat(Mo, 3) * og64.get("velocities")
+    say("P2G momenta: rel err %.2e vs FP64" % relerr(mom, mom64))
+    assert relerr(mom, mom64) < REL
     heavy = np.repeat(Mo > 1e-3 * Mo.max(), 3)
     assert relerr(V[heavy], og64.get("velocities")[heavy]) < 1e-4
     assert np.abs(g.get("prev_velocities") - V).max() == 0
+
+
+def test_binning_at_cell_boundaries(oracle):
+    """Positions within a few ulps of cell boundaries on every axis: the binning kernel's base
+    cells equal the oracle iterator's, particle by particle, so the float subtraction, division
+    and floor of Grid.cpp:1111-1113 round as on the host."""
+    say("binning_at_cell_boundaries")
+    rng = np.random.default_rng(5)
+    h = np.float32(1.0 / 64)
+    core = rng.uniform(0.3, 0.7, (400, 3)).astype(np.float32)
+    lo = core.min(0)
+    m_min = (np.floor(lo / h).astype(np.float32) - np.float32(3)) * h  # Grid.cpp:194-207
+    pts, groups = [], []
+    for k in range(int(np.ceil((0.35 - m_min.min()) / h)), int((0.65 - m_min.max()) / h)):
+        for axis in range(3):
+            groups.append((len(pts), axis))
+            b = (m_min[axis] + np.float32(k) * h).astype(np.float32)
+            for step in range(-3, 4):
+                v = b
+                for _ in range(abs(step)):
+                    v = np.nextafter(v, np.float32(np.inf if step > 0 else -np.inf), dtype=np.float32)
+                p = rng.uniform(0.4, 0.6, 3).astype(np.float32)
+                p[axis] = v
+                pts.append(p)
+    x = np.concatenate([core, np.array(pts, dtype=np.float32)])
+    n = len(x)
+    P = Pair(oracle, x, np.zeros((n, 3), np.float32), np.full(n, 1e-3, np.float32))
+    inds = np.arange(n)
+    g = G.Grid(P.ctx, inds, h)
+    og = oracle.Grid(P.o[F32], inds, float(h))
+    assert (g.min == m_min).all() and (og.min.astype(np.float32) == m_min).all()
+    base_gpu = g.get("base_cells")
+    base_orc = np.array([og.stencil(x[k].astype(np.float64))[0][0] for k in range(n)])
+    closed = np.floor((x - m_min) / h).astype(np.int64) - 1
+    # the cases do straddle the boundaries: each group of 7 ulp steps covers two cells
+    c0 = len(core)
+    assert all(len(set(closed[c0 + i:c0 + i + 7, a])) == 2 for i, a in groups)
+    assert (base_orc == closed).all()
+    assert (base_gpu == base_orc).all()
 
 
 def test_model_probes_and_plasticity(oracle):

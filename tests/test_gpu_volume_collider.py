@@ -108,3 +108,27 @@ def test_ground_volume_matches_ground_plane():
     d = np.abs(runs[0] - runs[1]).max()
     say("max position difference %.3g (h = %.3g)" % (d, h))
     assert d < 1e-3 * h
+
+
+def test_rejected_collider_set_keeps_the_previous_one(oracle):
+    """mpm_set_colliders validates the whole set before it touches device memory: a call whose
+    second volume is malformed (voxel size 0) fails and leaves the previous set, and the volume
+    buffers its device copy points into, as they were, although the rejected set's first volume
+    is larger than the buffers the previous one filled."""
+    say("rejected collider set")
+    P, h, cols = pair_with_volume(oracle)
+    inds = np.arange(P.n)
+    g = G.Grid(P.ctx, inds, h)
+    g.compute_particle_volumes()
+    em, col = g.explicit_momenta(0.002)
+    phi, lo, voxel, vel = sphere_volume()
+    big = np.ascontiguousarray(np.tile(phi, (3, 1, 1)))
+    bigv = np.ascontiguousarray(np.tile(vel, (3, 1, 1, 1)))
+    with pytest.raises(G.MpmError):
+        P.ctx.set_colliders([((-1, 0, 0, 1.5), (0.0, 0.0, 0.0), 0.5, False)],
+                            volumes=[(big, lo, voxel, bigv, 0.4, False), (phi, lo, 0.0, vel, 0.4, False)])
+    g2 = G.Grid(P.ctx, inds, h)
+    g2.compute_particle_volumes()
+    em2, col2 = g2.explicit_momenta(0.002)
+    assert (col2 == col).all()
+    assert np.array_equal(em2, em)

@@ -769,6 +769,61 @@ def test_implicit_update_solves_projected_system(oracle):
         assert err < tol
 
 
+def test_initial_guess_closed_form(oracle):
+    """Grid.cpp:911-947: the solver starts from one velocity at every node: the centre-of-mass
+    velocity of the explicit momenta (sum p* / sum m, momenta not velocities) when no node
+    collided, else the collided-mass-weighted collision velocity sum m vc / sum m over nodes with
+    a flag >= 0. Both branches against the closed form, in both precisions; the mistakes it has to
+    catch (node velocities averaged instead of momenta, the collided branch dropped) move the
+    guess far more than the tolerance."""
+    for prec, tol in ((F64, 1e-12), (F32, 2e-6)):
+        # collided branch: two moving planes (TestGrid.cpp:456-708)
+        pd, model, g, cols, fields = implicit_scene(oracle, prec)
+        dt = 0.002
+        em, collided = g.explicit_momenta(dt, model, cols, fields)
+        vc = g.collision_velocities(cols, collided).reshape(-1, 3)
+        M = g.get("masses")
+        res = g.update_grid_velocities(dt, model, cols, fields, 40, 0.0)
+        hit = collided >= 0
+        assert hit.any()
+        want = (M[hit, None] * vc[hit]).sum(0) / M[hit].sum()
+        assert np.abs(res["guess"] - want).max() <= tol * np.abs(want).max()
+        com = em.reshape(-1, 3).sum(0) / M.sum()
+        assert np.abs(com - want).max() > 1e3 * tol * np.abs(want).max()
+        # centre-of-mass branch: no colliders, a non-uniform velocity field
+        pd, model, g, cols, fields = implicit_scene(oracle, prec, planes=[])
+        fields.add_gravity([0, -9.81, 0])
+        em, collided = g.explicit_momenta(dt, model, cols, fields)
+        assert (collided == -1).all()
+        M = g.get("masses")
+        res = g.update_grid_velocities(dt, model, cols, fields, 40, 0.0)
+        want = em.reshape(-1, 3).sum(0) / M.sum()
+        assert np.abs(res["guess"] - want).max() <= tol * np.abs(want).max()
+        heavy = M > 0
+        vmean = (em.reshape(-1, 3)[heavy] / M[heavy, None]).mean(0)
+        assert np.abs(vmean - want).max() > 1e3 * tol * np.abs(want).max()
+
+
+def test_initial_guess_is_exact_in_free_fall(oracle):
+    """The consequence the reference's comment at Grid.cpp:947-954 names: for a stress-free body
+    moving as a whole (F = I, one velocity, gravity only), A x0 = M x0 for the uniform guess and
+    the centre-of-mass velocity of p* = m (v + dt g) is the exact solution, so the solve ends at
+    once (at most one x update) in FP32 and FP64."""
+    for prec in (F64, F32):
+        pd, model, g, cols, fields = implicit_scene(oracle, prec, vel_fn=lambda p: np.array([0.3, -0.2, 0.1]),
+                                                    planes=[])
+        pd.set("F", np.tile(np.eye(3), (pd.size(), 1, 1)))
+        model.update_particle_data()
+        fields.add_gravity([0, -9.81, 0])
+        dt = 0.002
+        res = g.update_grid_velocities(dt, model, cols, fields, 40, 1e-5)
+        assert np.abs(res["guess"] - np.array([0.3, -0.2 - 9.81 * dt, 0.1])).max() < 1e-6
+        assert res["iterations"] <= 1
+        M = g.get("masses")
+        v = g.get("velocities").reshape(-1, 3)[M > 0]
+        assert np.abs(v - res["guess"]).max() < 1e-6
+
+
 def test_explicit_momenta_gravity_closed_form(oracle):
     """calculateExplicitMomenta with a stress-free body: p* = m (v + dt g), flags from the plane."""
     h = 0.1
