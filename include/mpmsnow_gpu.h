@@ -129,6 +129,14 @@ int mpm_model_energy_density(mpm_context* ctx, const int32_t* particles, int64_t
 int mpm_model_stress(mpm_context* ctx, const int32_t* particles, int64_t n, float* out);
 int mpm_model_stress_differential(mpm_context* ctx, const int32_t* particles, const float* dF, int64_t n,
                                   float* out);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): mpm_model_stress_differential with particles (n
+ * int32), dF (9n floats) and out (9n floats) in device memory of the context's device, calling no
+ * hipMalloc/hipFree. Its scratch (the slot list) is a fixed per-GPU device array, so n is at most
+ * 1048576 (larger n: MPM_ERR_INVALID, nothing computed) and calls that overlap on one GPU race.
+ * An index outside [0, particles) fails the call (MPM_ERR_INVALID, its rows of out are NaN). */
+int mpm_model_stress_differential_noalloc(mpm_context* ctx, const int32_t* particles, const float* dF, int64_t n,
+                                          float* out);
 
 /* ---- colliders and force fields (CollisionObjectSet, ForceFieldSet) -------- */
 int mpm_set_colliders(mpm_context* ctx, const mpm_collider* colliders, int n);

@@ -75,6 +75,9 @@ SIGNATURES = {
     "mpm_model_energy_density": (C.c_int, [C.c_void_p, _ip, C.c_int64, _fp]),
     "mpm_model_stress": (C.c_int, [C.c_void_p, _ip, C.c_int64, _fp]),
     "mpm_model_stress_differential": (C.c_int, [C.c_void_p, _ip, _fp, C.c_int64, _fp]),
+    # device pointers (particles, dF, out)
+    "mpm_model_stress_differential_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                                        C.c_void_p]),
     "mpm_set_colliders": (C.c_int, [C.c_void_p, C.POINTER(Collider), C.c_int]),
     "mpm_set_gravity": (C.c_int, [C.c_void_p, _fp, C.c_int]),
     "mpm_grid_create": (C.c_int, [C.c_void_p, _ip, C.c_int64, C.c_float, _fp, C.c_int, C.POINTER(C.c_void_p)]),

@@ -130,7 +130,8 @@ enum ErrorBits : int {
   ERR_HARDNESS = 16,       // "infinite hardness!!!"
   ERR_STENCIL = 32,        // internal: particle outside its tile's footprint
   ERR_NAN_PVEL = 64,       // "nans in particle velocity data!"
-  ERR_INTERNAL = 128       // internal: a launch that does not fit the batch's layout
+  ERR_INTERNAL = 128,      // internal: a launch that does not fit the batch's layout
+  ERR_INDEX = 256          // a caller particle index out of range (device-resident index lists)
 };
 
 constexpr int kFcrMax = 8;  // flexible CR: most stored directions (the context's setting, default 2)

@@ -611,6 +611,17 @@ __global__ void k_permute(PView src, PView dst, const int* new_to_old, int n, co
 __global__ void k_gather_slots(const int* slot_of, const int* orig, int n, int* slots) {
   GRID_STRIDE(k, n) slots[k] = slot_of[orig[k]];
 }
+// the same for a device-resident index list that the host has not checked: an index outside
+// [0, np) gives slot -1 and raises ERR_INDEX
+__global__ void k_gather_slots_checked(const int* slot_of, const int* orig, int n, int np, int* slots, int* err) {
+  GRID_STRIDE(k, n) {
+    const int o = orig[k];
+    const bool ok = o >= 0 && o < np;
+    if (!ok) atomicOr(err, ERR_INDEX);
+    slots[k] = ok ? slot_of[o] : -1;
+  }
+}
+
 // ------------------------------------------------------------ body partition (Sim.cpp:136-203)
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
   return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) & mask;
@@ -3036,6 +3047,11 @@ __global__ void k_advect(PView P, const int* labels, int n, float dt, const Coll
 __global__ void k_probe(PView P, const int* slots, int n, int mode, const float* dF, float* out) {
   GRID_STRIDE(k, n) {
     int s = slots[k];
+    if (s < 0) {  // rejected index (the call fails): the output stays defined
+      const int w = mode == 0 ? 1 : 9;
+      for (int c = 0; c < w; ++c) out[w * (int64_t)k + c] = __int_as_float(0x7fc00000);
+      continue;
+    }
     float F[9], R[9], Fi[9], Gi[9];
     load_m9(P, C_F, s, F);
     load_m9(P, C_R, s, R);
@@ -3087,6 +3103,10 @@ void launch_permute(PView src, PView dst, const int* new_to_old, int n, const in
 }
 void launch_gather_slots(const int* slot_of, const int* orig, int n, int* slots, hipStream_t s) {
   if (n > 0) LAUNCH(k_gather_slots, grid_for(n), slot_of, orig, n, slots);
+}
+void launch_gather_slots_checked(const int* slot_of, const int* orig, int n, int np, int* slots, int* err,
+                                 hipStream_t s) {
+  if (n > 0) LAUNCH(k_gather_slots_checked, grid_for(n), slot_of, orig, n, np, slots, err);
 }
 void launch_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count, hipStream_t s) {
   if (n > 0) LAUNCH(k_cc_hash, grid_for(n, 16384), P, n, cell, mask, bucket, rank, count);

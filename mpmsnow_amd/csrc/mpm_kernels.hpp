@@ -16,6 +16,9 @@ void launch_identity_columns(PView P, int col0, int n, hipStream_t s);
 void launch_permute(PView src, PView dst, const int* new_to_old, int n, const int* orig_of_old, int* orig_of_new,
                     int* slot_of, hipStream_t s);
 void launch_gather_slots(const int* slot_of, const int* orig, int n, int* slots, hipStream_t s);
+// device-resident caller indices, range-checked on the device (ERR_INDEX, slot -1)
+void launch_gather_slots_checked(const int* slot_of, const int* orig, int n, int np, int* slots, int* err,
+                                 hipStream_t s);
 void launch_fill_int(int* p, int value, int64_t n, hipStream_t s);
 void launch_iota(int* p, int64_t n, hipStream_t s);
 

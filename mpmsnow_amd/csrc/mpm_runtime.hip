@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -31,6 +32,13 @@
 using namespace mpm;
 using namespace mpmh;
 
+// Scratch of the allocation-free entry point mpm_model_stress_differential_noalloc: the slot list
+// of one call's particles. Each GPU holds its own copy of the array (4 MiB) until the process
+// ends; host code reaches it only through the per-device address that hipGetSymbolAddress returns
+// (taken once per device, at the first mpm_create on it). Every call on a GPU shares it, so calls
+// that overlap on one GPU race.
+constexpr int64_t kNoAllocMaxParticles = 1 << 20;
+static __device__ __align__(256) int scratch[kNoAllocMaxParticles];
 
 namespace {
 
@@ -65,6 +73,7 @@ void raise_errors(int e) {
   if (e & ERR_NAN_PVEL) throw std::runtime_error("nans in particle velocity data!");
   if (e & ERR_STENCIL) throw std::runtime_error("particle outside its grid tile (internal binning error)");
   if (e & ERR_INTERNAL) throw std::runtime_error("launch does not fit the batch layout (internal error)");
+  if (e & ERR_INDEX) throw InvalidArg("particle index out of range");
 }
 
 // Kernel timing for measurement: HIP events recorded on the library's stream around each
@@ -87,6 +96,22 @@ constexpr int kProfOnce = -1;     // once per solve, every body of the batch
 constexpr int kProfSetup = -2;    // CR setup of the bodies the batched loop solves
 constexpr int kProfSmall = -3;    // the small-body solver
 
+}  // namespace
+
+namespace {
+constexpr int kMaxDevices = 256;
+std::mutex gScratchMutex;
+int* gScratch[kMaxDevices] = {};  // device address of `scratch` on each device (null: not taken yet)
+// the calling thread's current device must be `device`
+int* scratch_on(int device) {
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratch[device]) {
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    gScratch[device] = static_cast<int*>(p);
+  }
+  return gScratch[device];
+}
 }  // namespace
 
 struct SolveRecord {
@@ -197,6 +222,7 @@ struct LpcCapture {
 
 struct mpm_context {
   int device = 0;
+  int* scratch = nullptr;  // this device's copy of the file-scope `scratch` array
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // small-body solver, concurrent with the batched CR loop
   hipEvent_t smallReady = nullptr, smallDone = nullptr;  // stream -> stream2 -> stream ordering
@@ -1085,6 +1111,7 @@ int mpm_create(mpm_context** out, int device) {
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return MPM_ERR_NO_DEVICE;
+  if (device >= kMaxDevices) return MPM_ERR_INVALID;
   std::unique_ptr<mpm_context> c(new mpm_context());
   c->device = device;
   c->simBatch.ctx = c.get();
@@ -1098,6 +1125,7 @@ int mpm_create(mpm_context** out, int device) {
     c->grav_dev.ensure(sizeof(float) * 3 * 64);
     ColliderSet empty{};
     HIPCHECK(hipMemcpy(c->cols_dev.p, &empty, sizeof(empty), hipMemcpyHostToDevice));
+    c->scratch = scratch_on(device);  // guarded() made `device` current
   });
   if (rc != MPM_OK) return rc;
   *out = c.release();
@@ -1245,6 +1273,26 @@ int mpm_model_stress(mpm_context* ctx, const int32_t* p, int64_t n, float* out) 
 int mpm_model_stress_differential(mpm_context* ctx, const int32_t* p, const float* dF, int64_t n, float* out) {
   if (!dF) return MPM_ERR_INVALID;
   return model_probe(ctx, p, dF, n, 2, out);
+}
+
+// The variant of mpm_model_stress_differential that allocates nothing: device-resident inputs
+// and output, the slot list in `scratch` (this device's address, taken at mpm_create). The index
+// check runs on the device; the call waits for its result like the others.
+int mpm_model_stress_differential_noalloc(mpm_context* ctx, const int32_t* particles, const float* dF, int64_t n,
+                                          float* out) {
+  return guarded(ctx, [&] {
+    require_particles(ctx);
+    if (!ctx->model) throw InvalidArg("no constitutive model");
+    if (n < 0 || n > kNoAllocMaxParticles)
+      throw InvalidArg("mpm_model_stress_differential_noalloc takes at most 1048576 particles per call");
+    if (n == 0) return;
+    if (!particles || !dF || !out) throw InvalidArg("null device pointer");
+    launch_fill_int(ctx->errp(), 0, 1, ctx->stream);  // a fill kernel, not a memset node
+    launch_gather_slots_checked(ctx->slot_of.as<int>(), particles, (int)n, (int)ctx->n, ctx->scratch, ctx->errp(),
+                                ctx->stream);
+    launch_probe(ctx->view(), ctx->scratch, (int)n, 2, dF, out, ctx->stream);
+    ctx->check_errors();
+  });
 }
 
 int mpm_set_colliders(mpm_context* ctx, const mpm_collider* cols, int n) {
