@@ -1,38 +1,37 @@
-"""Benchmark: implicit MPM snow step on one MI355X.
+"""Benchmark: implicit MPM snow step on one MI355X, learned-preconditioned CR against plain CR.
 
-Workload: two snowballs colliding (1M particles, 500k per sphere of r = 0.15 m,
-+/-3 m/s, h = 1/256 m, dt = 5e-4 s, snow E0 = 1.4e5 Pa, nu = 0.2, xi = 10,
-theta_c = 2.5e-2, theta_s = 7.5e-3, gravity, ground plane mu = 0.2, CR relative
-tolerance 1e-5), synthetic seeded particles. A step is one Sim::advance of the
-whole scene (body partition, per-body grids, implicit CR solve, G2P,
-plasticity, advection). Prints one JSON line.
+Workload (the metric's configuration): snow slab on a 30-degree slope, 4M particles seeded
+uniform in a 0.8 x 0.2 x 0.4 m box, per-particle E0 log-uniform in [7e4, 2.8e5] Pa and xi uniform
+in [5, 10] (seed 2, held out from the preconditioner's training scenes), h = 1/256 m, dt = 5e-4 s,
+nu = 0.2, theta_c = 2.5e-2, theta_s = 7.5e-3, tilted gravity over the ground plane (mu = 0.2), CR
+relative tolerance 1e-5; synthetic seeded particles. A step is one Sim::advance of the whole
+scene (body partition, per-body grids, implicit CR solve, G2P, plasticity, advection). After
+--warmup plain steps the particle state is kept; --steps steps are timed with the learned
+flexible CR (BF16 MFMA network, weights mpmsnow_amd/lpc_weights.npy) and, from the same kept
+state, --steps steps with plain diagonal CR. Prints one JSON line.
 
-value = particle-steps/s over the timed steps (inputs resident in HBM).
-roofline = the dominant kernel (the matrix-free HVP inside CR): algorithmic
-bytes per launch / mean launch time, both measured in the timed steps with HIP
-events on the library's stream (only HVP launches are instrumented there), against
-MI355X's 8 TB/s HBM peak. roofline.traffic = measured HBM bytes per HVP launch,
-read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-(profiles/hvp_two_snowballs_1M_traffic.md, made by `--pmc-summary`).
-hvp_issue = what binds the HVP instead: its VALU-issue and LDS-active fractions of
-the kernel's clocks, read from the committed rocprofv3 SQ counter summary
-(profiles/hvp_two_snowballs_1M_sq_counters.md).
-cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the
-8-colour voxel partition like the reference's TBB splats) timed on this host
-for a bounded sample: one step of the same 1M-particle scene from the GPU run's
-state.
-learned_pc = the metric's second half: on the 4M-particle slope slab (seed 2,
-held out from training), particle-steps/s and CR iterations per step of the
-learned flexible CR (network weights mpmsnow_amd/lpc_weights.npy, trained by
-mpmsnow_amd/lpc_train.py) against the plain diagonal CR, same states, same
-1e-5 residual test.
+value = particle-steps/s of the learned-preconditioned step (inputs resident in HBM); `plain`
+and `speedup` beside it; CR iterations per step of the largest body for both.
+roofline = the dominant kernel (the matrix-free HVP inside CR) in the learned leg: algorithmic
+bytes per launch (172 B per particle + 29 B per active node of the bodies still iterating) /
+mean launch time, both measured in the timed steps with HIP events on the library's stream
+(one HVP launch in 7), against MI355X's 8 TB/s HBM peak. roofline.traffic = measured HBM bytes
+per HVP launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload.
+network = the preconditioner's forward pass (MFMA BF16): FLOPs per forward from the layer shapes
+over the active tiles, mean time per forward (HIP events), fraction of 2.5 PFLOP/s.
+cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the 8-colour voxel
+partition like the reference's TBB splats) timed on this host for a bounded sample: one step of
+a 1M-particle slab of the same density, materials, h and dt (0.4 x 0.2 x 0.2 m), from the GPU's
+state after the same warm-up. error_vs_fp64 steps that sample's state once more on the GPU and in
+the FP64 oracle.
+two_snowballs_1M = the two-snowballs scene (1M particles, +/-3 m/s, h = 1/256, dt = 5e-4, plain
+CR), particle-steps/s over --snowball-steps steps.
 
 Other modes:
   --stiff              only the stiff stress case (16M particles in 8 seeded boxes and
                        spheres dropped onto the ground plane and an analytic sphere
                        collider, E0 = 4.8e5 Pa, h = 1/512, dt = 2e-3): plain vs learned CR
-                       from the same warmed-up state, per-step CR iterations, fallback
-                       triggers and capped solves; one JSON line (--stiff-particles,
+                       from the same warmed-up state; one JSON line (--stiff-particles,
                        --stiff-steps, --stiff-warmup)
   --lpc-weights W, --lpc-fallback R, --lpc-dirs M
                        network parameters (.npy), fallback ratio and flexible-CR
@@ -40,16 +39,10 @@ Other modes:
                        weights, 0.95, 2)
   --copy-calibration   also runs one 256 MiB device-to-device copy first (the
                        known-size kernel that scales the PMC byte counters)
-  --pmc-summary FETCH_CSV WRITE_CSV [--pmc-out FILE]
-                       summarise two `rocprofv3 --pmc FETCH_SIZE` / `--pmc
-                       WRITE_SIZE` runs of `bench.py --copy-calibration` into the
-                       traffic summary file
-  --trace-summary STATS_CSV BENCH_LOG [--trace-out FILE]
-                       summarise a `rocprofv3 --kernel-trace --stats` run of
-                       `bench.py --no-cpu --no-learned` (its kernel_stats.csv and
-                       the run's stdout): per-kernel table and the HVP mean
-                       launch time against the one bench.py measured with HIP
-                       events in the same run
+  --pmc-summary FETCH_CSV WRITE_CSV, --sq-summary P0 P1 P2, --trace-summary STATS_CSV LOG
+                       summarise rocprofv3 counter / kernel-trace runs of bench.py into the
+                       profiles/ files of the --workload they ran (slab, or snowballs for the
+                       two-snowballs scene run alone with --workload snowballs)
 """
 import argparse
 import csv
@@ -63,10 +56,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hvp_two_snowballs_1M_traffic.md")
-STATS_FILE = os.path.join(ROOT, "profiles", "two_snowballs_1M_kernel_stats.md")
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 HVP_SAMPLE = 7  # HVP launches timed with HIP events in the timed region: one in HVP_SAMPLE
 COPY_BYTES = 256 << 20
+# rocprofv3 summaries per workload (bench.py --workload W runs W alone)
+WORKLOAD_NAMES = {"slab": "slope_slab_4M", "snowballs": "two_snowballs_1M"}
+WORKLOAD_TITLES = {"slab": "slope slab (4M particles, learned and plain CR legs)",
+                   "snowballs": "two snowballs (1M particles, plain CR)"}
+
+
+def profile_file(kind, workload):
+    name = WORKLOAD_NAMES[workload]
+    return os.path.join(ROOT, "profiles", {"traffic": "hvp_%s_traffic.md", "sq": "hvp_%s_sq_counters.md",
+                                           "stats": "%s_kernel_stats.md"}[kind] % name)
 
 
 def progress(msg):
@@ -85,8 +87,14 @@ def snapshot(sim):
     return {k: sim.get(k) for k in STATE_FIELDS}
 
 
+def restore(sim, state):
+    for k in STATE_FIELDS:
+        sim.ctx.set(k, state[k])
+
+
 def oracle_sim(sc, state, prec="float"):
-    """The CPU oracle's Sim on the scene's colliders and gravity, at a state of the GPU run."""
+    """The CPU oracle's Sim on the scene's colliders, gravity and per-particle material, at a state
+    of the GPU run."""
     from oracle import pyoracle as O
 
     mat = sc["material"]
@@ -96,88 +104,104 @@ def oracle_sim(sc, state, prec="float"):
     f = O.Fields.create(prec)
     f.add_gravity(sc["gravity"])
     model = O.Model.snow(mat["E"], mat["nu"], mat["xi"], mat["theta_c"], mat["theta_s"], prec)
-    sim = O.Sim(state["p"], sc["m"], sc["h"], model, cols, f, prec=prec)
+    extra = {}
+    if "mu0" in sc:
+        extra = dict(mu0=sc["mu0"], lambda0=sc["lambda0"], xi=sc["xi"])
+    sim = O.Sim(state["p"], sc["m"], sc["h"], model, cols, f, prec=prec, **extra)
     for name in STATE_FIELDS[1:]:
         sim.pd.set(name, state[name])
     sim._keep = (cols, f, model)
     return sim
 
 
-def cpu_baseline(sc, state, steps=1):
-    """Time the reference-faithful CPU oracle for `steps` step(s) from a state of the GPU run;
-    returns the result line and the oracle's particle state after those steps."""
-    from oracle import pyoracle as O
-
-    sim = oracle_sim(sc, state)
-    progress("cpu baseline: oracle set up, timing %d step(s)" % steps)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    dt = time.perf_counter() - t0
-    progress("cpu baseline: %.1f s" % dt)
-    n = len(sc["m"])
-    its = [s["iterations"] for s in sim.last_stats()]
-    line = {"value": n * steps / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "%d step(s) of the same %d-particle scene from the GPU run's state at the start of its "
-                      "timed steps (%.1f s; CR iterations of its two largest bodies %s; later steps of the "
-                      "timed run take more iterations)" % (steps, n, dt, sorted(its)[-2:])}
-    return line, {"p": sim.pd.get("p"), "F": sim.pd.get("F")}
-
-
-def error_vs_fp64(sc, state, o32, steps):
-    """Part of the CPU-baseline leg, the oracle as checker: one GPU step and one FP64 oracle step
-    from the CPU sample's state; position and F deviations of the GPU and of the FP32 oracle
-    (the sample's own result) from FP64."""
-    import numpy as np
-
+def gpu_sim(sc):
     import mpmsnow_amd as G
 
-    progress("error check: FP64 oracle, %d step(s)" % steps)
-    o64 = oracle_sim(sc, state, "double")
-    for _ in range(steps):
-        o64.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    ref = {"p": o64.pd.get("p"), "F": o64.pd.get("F")}
-    sim = G.Sim(state["p"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"],
-                gravity=[sc["gravity"]], v=state["v"])
-    for name in STATE_FIELDS[1:]:
-        sim.ctx.set(name, state[name])
-    for _ in range(steps):
-        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    gpu = {"p": sim.get("p"), "F": sim.get("F")}
+    extra = {}
+    if "mu0" in sc:
+        extra = dict(mu0=sc["mu0"], lambda0=sc["lambda0"], xi=sc["xi"])
+    return G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
+                 v=sc.get("v"), **extra)
+
+
+def cpu_leg(warmup):
+    """cpu_baseline and error_vs_fp64 on a bounded sample of the workload: a 1M-particle slab of the
+    slope-slab scene's density, materials, h and dt (box 0.4 x 0.2 x 0.2 m, seed 2), stepped
+    `warmup` times on the GPU; then one step of the FP32 oracle (timed), one of the FP64 oracle
+    and one of the GPU from that state."""
+    import numpy as np
+
+    from mpmsnow_amd import scenes
+    from oracle import pyoracle as O
+
+    sc = scenes.slope_slab(n=1000000, seed=2, size=(0.4, 0.2, 0.2))
     n = len(sc["m"])
-    out = {"steps": steps, "note": "from the CPU sample's state; positions in units of h"}
-    for who, st in (("gpu", gpu), ("fp32_oracle", o32)):
-        dx = np.abs(st["p"].reshape(n, 3) - ref["p"].reshape(n, 3)).max(1) / sc["h"]
-        dF = np.abs(st["F"].reshape(n, 9) - ref["F"].reshape(n, 9)).max(1)
+    sim = gpu_sim(sc)
+    for _ in range(warmup):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    state = snapshot(sim)
+    o32 = oracle_sim(sc, state)
+    progress("cpu baseline: 1M slab sample, oracle set up")
+    t0 = time.perf_counter()
+    o32.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    dt = time.perf_counter() - t0
+    progress("cpu baseline: %.1f s" % dt)
+    its = max(s["iterations"] for s in o32.last_stats())
+    line = {"value": n / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "1 step of a 1M-particle slab of the workload's density, materials, h and dt (0.4 x 0.2 x "
+                      "0.2 m, seed 2) from the GPU's state after %d steps (%.1f s; %d CR iterations for its "
+                      "largest body)" % (warmup, dt, its)}
+    progress("error check: FP64 oracle, 1 step")
+    o64 = oracle_sim(sc, state, "double")
+    o64.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    out = {"steps": 1, "note": "the CPU sample's step on the GPU, in the FP32 and in the FP64 oracle; positions "
+                               "in units of h"}
+    ref = {"p": o64.pd.get("p").reshape(n, 3), "F": o64.pd.get("F").reshape(n, 9)}
+    for who, get in (("gpu", sim.get), ("fp32_oracle", o32.pd.get)):
+        dx = np.abs(get("p").reshape(n, 3) - ref["p"]).max(1) / sc["h"]
+        dF = np.abs(get("F").reshape(n, 9) - ref["F"]).max(1)
         out[who] = {"max_dx_over_h": float(dx.max()), "p99_dx_over_h": float(np.percentile(dx, 99)),
                     "max_dF": float(dF.max()), "p99_dF": float(np.percentile(dF, 99))}
-    return out
+    return line, out
 
 
-def learned_leg(steps, warmup, weights=None, fallback=None, dirs=None):
-    """Plain diagonal CR vs learned flexible CR on the 4M slope slab, from identical states."""
+def lpc_forward_flops(sim):
+    """FLOPs of one network forward over the batch's active tiles: 2 x voxels x 27 x cin x cout per
+    convolution (tile levels 0, 1, 2 of 64, 8 and 1 voxels per level-0 tile)."""
+    lay = sim.grid_layout()
+    t0 = lay["loop_tiles"]
+    vox = (64 * t0, 8 * t0, t0)  # upper bounds at levels 1, 2 (their tiles cover 8 x 8 x 8 nodes)
+    layers = ((0, 16, 16), (1, 16, 32), (2, 32, 64), (1, 64, 32), (0, 32, 16), (0, 16, 16))
+    return sum(2.0 * vox[lv] * 27 * ci * co for lv, ci, co in layers)
+
+
+def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
+    """Learned flexible CR and plain diagonal CR on the slope slab, --steps each, from one warmed-up
+    state; HVP and network launches timed with HIP events."""
     import numpy as np
     import torch
 
-    import mpmsnow_amd as G
     from mpmsnow_amd import scenes
 
-    sc = scenes.slope_slab(n=4000000, seed=2)
+    sc = scenes.slope_slab(n=n, seed=2)
     w = np.load(weights or os.path.join(ROOT, "mpmsnow_amd", "lpc_weights.npy"), allow_pickle=False)
+    sim = gpu_sim(sc)
+    sim.ctx.lpc_set_params(w)
+    if fallback is not None:
+        sim.ctx.lpc_fallback(fallback)
+    if dirs is not None:
+        sim.ctx.lpc_directions(dirs)
+    progress("slab: %d particles set up" % n)
+    for _ in range(warmup):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    state = snapshot(sim)
     out = {}
-    for mode, name in ((0, "plain"), (1, "learned")):
-        sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
-                    mu0=sc["mu0"], lambda0=sc["lambda0"], xi=sc["xi"])
-        sim.ctx.lpc_set_params(w)
-        if fallback is not None:
-            sim.ctx.lpc_fallback(fallback)
-        if dirs is not None:
-            sim.ctx.lpc_directions(dirs)
-        progress("learned leg: %s, 4M slab set up" % name)
-        for _ in range(warmup):
-            sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    for mode, name in ((1, "learned"), (0, "plain")):
+        restore(sim, state)
         sim.ctx.lpc_mode(mode)
-        its, fbs = [], []
+        sim.profile(["hvp", "lpc_forward"], period=HVP_SAMPLE)
+        its, tot, fbs = [], [], []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -186,16 +210,64 @@ def learned_leg(steps, warmup, weights=None, fallback=None, dirs=None):
             big = max(rows, key=lambda r: r["particles"])
             its.append(big["iterations"])
             fbs.append(big["fallbacks"])
-            progress("learned leg: %s step %d, largest body %d iterations" % (name, len(its), its[-1]))
+            tot.append(st["total_iterations"])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        out[name] = {"value": len(sc["m"]) * steps / dt, "ms_per_step": dt / steps * 1e3,
-                     "cr_iters_per_step": float(np.mean(its)), "fallbacks_per_step": float(np.mean(fbs))}
-        del sim
-    out["speedup"] = out["learned"]["value"] / out["plain"]["value"]
-    out["config"] = ("slope slab 4M particles (seed 2), h=1/256, dt=5e-4, CR tol 1e-5, %d steps after %d warmup"
-                     % (steps, warmup))
-    return out
+        prof = sim.profile_query()
+        sim.profile(False)
+        hv = prof["hvp"]
+        hms = hv["ms"] / max(1, hv["launches"])
+        hby = hv["bytes"] / max(1, hv["launches"])
+        leg = {"value": n * steps / dt, "ms_per_step": dt / steps * 1e3, "cr_iters_per_step": float(np.mean(its)),
+               "cr_iters_per_step_all_bodies": float(np.mean(tot)), "fallbacks_per_step": float(np.mean(fbs)),
+               "hvp": {"launches_timed": hv["launches"], "ms_per_launch": hms, "bytes_per_launch": hby,
+                       "achieved_gbs": hby / (hms * 1e-3) / 1e9 if hms > 0 else 0.0}}
+        if mode == 1:
+            lp = prof["lpc_forward"]
+            if lp["launches"]:
+                fms = lp["ms"] / lp["launches"]
+                fl = lpc_forward_flops(sim)
+                leg["network"] = {"forwards_timed": lp["launches"], "ms_per_forward": fms,
+                                  "flops_per_forward": fl, "tflops": fl / (fms * 1e-3) / 1e12,
+                                  "frac_of_bf16_peak": fl / (fms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS,
+                                  "note": "FLOPs over the loop's tiles at every level (upper bound)"}
+        progress("slab %s: %.2f ms/step, largest body %.1f CR iterations/step, fallbacks %.2f/step"
+                 % (name, leg["ms_per_step"], leg["cr_iters_per_step"], leg["fallbacks_per_step"]))
+        out[name] = leg
+    return out, sc
+
+
+def snowballs_leg(steps, warmup, n=1000000):
+    """Two snowballs colliding, plain CR: particle-steps/s and the HVP's HIP-event rate."""
+    import numpy as np
+    import torch
+
+    from mpmsnow_amd import scenes
+
+    sc = scenes.two_snowballs(n=n, seed=1)
+    sim = gpu_sim(sc)
+    progress("two snowballs %d particles set up" % n)
+    for _ in range(warmup):
+        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+    sim.profile(["hvp"], period=HVP_SAMPLE)
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(sim.advance(sc["dt"], sc["max_iters"], sc["tol"]))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    hv = sim.profile_query()["hvp"]
+    sim.profile(False)
+    hms = hv["ms"] / max(1, hv["launches"])
+    hby = hv["bytes"] / max(1, hv["launches"])
+    return {"value": n * steps / dt, "ms_per_step": dt / steps * 1e3, "steps": steps, "warmup": warmup,
+            "cr_iters_per_step_all_bodies": float(np.mean([s["total_iterations"] for s in stats])),
+            "bodies_per_step": float(np.mean([s["num_bodies"] for s in stats])),
+            "hvp": {"launches_timed": hv["launches"], "ms_per_launch": hms, "bytes_per_launch": hby,
+                    "achieved_gbs": hby / (hms * 1e-3) / 1e9 if hms > 0 else 0.0},
+            "config": "two snowballs colliding: 1M particles, +/-3 m/s, h=1/256, dt=5e-4, CR tol 1e-5, plain CR, "
+                      "%d steps after %d warm-up" % (steps, warmup)}
 
 
 def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None, plain_only=False):
@@ -277,7 +349,13 @@ def _pmc_rows(path):
     return rows
 
 
-def pmc_summary(fetch_csv, write_csv, out_path):
+def bench_command(workload, steps, warmup, extra=""):
+    if workload == "slab":
+        return "python bench.py --steps %d --warmup %d --no-cpu --no-snowballs%s" % (steps, warmup, extra)
+    return "python bench.py --workload snowballs --steps %d --warmup %d%s" % (steps, warmup, extra)
+
+
+def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
     """HVP HBM traffic per launch from FETCH_SIZE / WRITE_SIZE runs, scaled by a known-size copy."""
     res = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
@@ -300,16 +378,18 @@ def pmc_summary(fetch_csv, write_csv, out_path):
         res[counter] = {"copy_kib": copy_kib, "factor": factor, "hvp_kib_per_launch": kib,
                         "hvp_bytes_per_launch": kib * 1024.0 / factor, "launches": len(full)}
     traffic = res["FETCH_SIZE"]["hvp_bytes_per_launch"] + res["WRITE_SIZE"]["hvp_bytes_per_launch"]
-    summary = {"kernel": "k_hvp_pair<true>", "workload": "two_snowballs_1M", "traffic_bytes_per_launch": traffic,
-               "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
+    summary = {"kernel": "k_hvp_pair<true>", "workload": WORKLOAD_NAMES[workload],
+               "traffic_bytes_per_launch": traffic, "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
+    cmd = bench_command(workload, 2, 3, " --copy-calibration")
     lines = [
-        "# HVP HBM traffic, two snowballs (1M particles)",
+        "# HVP HBM traffic, " + WORKLOAD_TITLES[workload],
         "",
         "Commands (from the repository root on the GPU host):",
         "",
-        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- python bench.py --steps 2 --warmup 3 --no-cpu --no-learned --copy-calibration",
-        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- python bench.py --steps 2 --warmup 3 --no-cpu --no-learned --copy-calibration",
-        "    python bench.py --pmc-summary <F>/run_counter_collection.csv <W>/run_counter_collection.csv",
+        "    rocprofv3 --pmc FETCH_SIZE --output-format csv -d <F> -o run -- " + cmd,
+        "    rocprofv3 --pmc WRITE_SIZE --output-format csv -d <W> -o run -- " + cmd,
+        "    python bench.py --workload %s --pmc-summary <F>/run_counter_collection.csv "
+        "<W>/run_counter_collection.csv" % workload,
         "",
         "The counters are scaled by what they read for kernels of known traffic on a %d MiB buffer: FETCH_SIZE by"
         % (COPY_BYTES >> 20),
@@ -342,7 +422,7 @@ SQ_PASSES = (
 )
 
 
-def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>"):
+def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>", workload="slab"):
     """Mean per-dispatch SQ/LDS counters of the CR loop's HVP from the three `rocprofv3 --pmc` passes
     of SQ_PASSES, and what they imply on MI355X (256 CUs, 1024 SIMDs, 32 shader engines; a wave64
     VALU instruction occupies its SIMD for 4 clocks)."""
@@ -367,15 +447,15 @@ def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>"):
     bank = vals["SQ_LDS_BANK_CONFLICT"] / 256.0
     addr = vals["SQ_LDS_ADDR_CONFLICT"] / 256.0
     resident = vals["SQ_WAVE_CYCLES"] * 4.0 / 256.0 / clocks
-    summary = {"kernel": kernel, "workload": "two_snowballs_1M", "valu_issue_frac": round(valu / clocks, 3),
+    summary = {"kernel": kernel, "workload": WORKLOAD_NAMES[workload], "valu_issue_frac": round(valu / clocks, 3),
                "lds_active_frac": round(lds / clocks, 3), "resident_waves_per_cu": round(resident, 1)}
-    lines = ["# HVP SQ / LDS counters, two snowballs (1M particles)", "",
+    lines = ["# HVP SQ / LDS counters, " + WORKLOAD_TITLES[workload], "",
              "Commands (from the repository root on the GPU host), one counter pass each:", ""]
     for k, p in enumerate(SQ_PASSES):
-        lines.append("    rocprofv3 --pmc %s --output-format csv -d <P%d> -o run -- python bench.py --steps 2 "
-                     "--warmup 1 --no-cpu --no-learned" % (p, k))
-    lines += ["    python bench.py --sq-summary <P0>/run_counter_collection.csv <P1>/run_counter_collection.csv "
-              "<P2>/run_counter_collection.csv", "",
+        lines.append("    rocprofv3 --pmc %s --output-format csv -d <P%d> -o run -- %s"
+                     % (p, k, bench_command(workload, 2, 1)))
+    lines += ["    python bench.py --workload %s --sq-summary <P0>/run_counter_collection.csv "
+              "<P1>/run_counter_collection.csv <P2>/run_counter_collection.csv" % workload, "",
               "Values: mean per `%s` dispatch over the last 40%% of the run's dispatches of that kernel, "
               "summed over the device as rocprofv3 reports them." % kernel, "",
               "| counter | per dispatch |", "|---|---|"]
@@ -402,7 +482,7 @@ def _short(name):
     return name.split("(")[0]
 
 
-def trace_summary(stats_csv, bench_log, out_path):
+def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
     """Kernel table of a rocprofv3 --kernel-trace --stats run of the bench, and the HVP check."""
     rows = []
     with open(stats_csv) as fh:
@@ -421,23 +501,23 @@ def trace_summary(stats_csv, bench_log, out_path):
     rp_ms = hvp_ns / hvp_calls / 1e6
     ev_ms = bench["roofline"]["ms_per_launch"]
     steps, warmup = bench["steps"], bench["warmup"]
+    legs = ("the learned and the plain leg (each --steps steps from the kept state)" if workload == "slab"
+            else "the timed steps")
     lines = [
-        "# Kernel statistics, two snowballs (1M particles)",
+        "# Kernel statistics, " + WORKLOAD_TITLES[workload],
         "",
         "Command (from the repository root on the GPU host):",
         "",
-        "    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- python bench.py --steps %d "
-        "--warmup %d --no-cpu --no-learned > <LOG>" % (steps, warmup),
-        "    python bench.py --trace-summary <D>/run_kernel_stats.csv <LOG>",
+        "    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- %s > <LOG>"
+        % bench_command(workload, steps, warmup),
+        "    python bench.py --workload %s --trace-summary <D>/run_kernel_stats.csv <LOG>" % workload,
         "",
-        "The profiled run covers %d warm-up steps, %d timed steps and the 2 steps of the bench's per-kernel "
-        "table. bench.py line of that run: %.4g particle-steps/s, %.3f ms/step (under the profiler)."
-        % (warmup, steps, bench["value"], bench["ms_per_step"]),
+        "The profiled run covers %d warm-up steps and %s. bench.py line of that run: %.4g particle-steps/s, "
+        "%.3f ms/step (under the profiler)." % (warmup, legs, bench["value"], bench["ms_per_step"]),
         "",
         "HVP (`k_hvp_pair<true>`, the CR loop's, and `k_hvp<false>`, every launch of the run): %d launches, "
-        "mean %.4f ms "
-        "(rocprofv3); bench.py's HIP events over its timed steps (one HVP launch in %d): %d launches, mean "
-        "%.4f ms (ratio %.3f)."
+        "mean %.4f ms (rocprofv3); bench.py's HIP events over its timed steps (one HVP launch in %d): %d "
+        "launches, mean %.4f ms (ratio %.3f)."
         % (hvp_calls, rp_ms, HVP_SAMPLE, bench["roofline"]["launches"], ev_ms,
            rp_ms / ev_ms if ev_ms else float("nan")),
         "",
@@ -446,7 +526,7 @@ def trace_summary(stats_csv, bench_log, out_path):
     ]
     for n, c, t, pct in rows[:20]:
         lines.append("| `%s` | %d | %.2f | %.1f | %.1f%% |" % (n, c, t / 1e6, t / c / 1e3, pct))
-    summary = {"workload": "two_snowballs_1M", "hvp_launches": hvp_calls, "hvp_ms_rocprof": rp_ms,
+    summary = {"workload": WORKLOAD_NAMES[workload], "hvp_launches": hvp_calls, "hvp_ms_rocprof": rp_ms,
                "hvp_ms_events": ev_ms, "steps": steps, "warmup": warmup}
     lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
     with open(out_path, "w") as fh:
@@ -454,7 +534,6 @@ def trace_summary(stats_csv, bench_log, out_path):
     print(json.dumps(summary))
 
 
-SQ_FILE = os.path.join(ROOT, "profiles", "hvp_two_snowballs_1M_sq_counters.md")
 STIFF_STATS_FILE = os.path.join(ROOT, "profiles", "stiff_drop_16M_kernel_stats.md")
 
 
@@ -498,31 +577,35 @@ def stiff_trace_summary(stats_csv, stiff_json, out_path):
         fh.write("\n".join(lines) + "\n")
 
 
-def read_sq():
+def read_sq(workload="slab"):
     """VALU-issue and LDS-active fractions of the HVP's clocks from the committed SQ counter summary."""
-    if not os.path.exists(SQ_FILE):
+    path = profile_file("sq", workload)
+    if not os.path.exists(path):
         return None
-    m = re.search(r"<!-- json: (.*) -->", open(SQ_FILE).read())
+    m = re.search(r"<!-- json: (.*) -->", open(path).read())
     return json.loads(m.group(1)) if m else None
 
 
-def read_traffic():
-    if not os.path.exists(TRAFFIC_FILE):
+def read_traffic(workload="slab"):
+    path = profile_file("traffic", workload)
+    if not os.path.exists(path):
         return None
-    m = re.search(r"<!-- json: (.*) -->", open(TRAFFIC_FILE).read())
+    m = re.search(r"<!-- json: (.*) -->", open(path).read())
     return json.loads(m.group(1))["traffic_bytes_per_launch"] if m else None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=197)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--particles", type=int, default=1000000)
+    ap.add_argument("--steps", type=int, default=40, help="timed steps of each leg")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", choices=("slab", "snowballs"), default="slab",
+                    help="slab: the headline line (learned vs plain on the 4M slope slab); snowballs: the 1M "
+                         "two-snowball scene alone")
+    ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
-    ap.add_argument("--learned-steps", type=int, default=5)
+    ap.add_argument("--no-snowballs", action="store_true", help="skip the two-snowballs line")
+    ap.add_argument("--snowball-steps", type=int, default=60)
     ap.add_argument("--stiff", action="store_true",
                     help="run only the stiff stress case (plain vs learned CR) and print its JSON line")
     ap.add_argument("--stiff-particles", type=int, default=16000000)
@@ -535,23 +618,21 @@ def main():
     ap.add_argument("--lpc-dirs", type=int, default=None, help="directions the learned legs' flexible CR keeps")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
-    ap.add_argument("--pmc-out", default=TRAFFIC_FILE)
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_CSV", "BENCH_LOG"))
     ap.add_argument("--sq-summary", nargs=3, metavar=("PASS0_CSV", "PASS1_CSV", "PASS2_CSV"))
-    ap.add_argument("--sq-out", default=SQ_FILE)
-    ap.add_argument("--trace-out", default=STATS_FILE)
     args = ap.parse_args()
+    wl = args.workload
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.pmc_out)
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], profile_file("traffic", wl), wl)
         return
     if args.stiff_trace_summary:
         stiff_trace_summary(args.stiff_trace_summary[0], args.stiff_trace_summary[1], STIFF_STATS_FILE)
         return
     if args.sq_summary:
-        sq_summary(args.sq_summary, args.sq_out)
+        sq_summary(args.sq_summary, profile_file("sq", wl), workload=wl)
         return
     if args.trace_summary:
-        trace_summary(args.trace_summary[0], args.trace_summary[1], args.trace_out)
+        trace_summary(args.trace_summary[0], args.trace_summary[1], profile_file("stats", wl), wl)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the step on a single MI355X "
@@ -564,11 +645,6 @@ def main():
                                    plain_only=args.stiff_plain_only)))
         return
 
-    import numpy as np
-
-    import mpmsnow_amd as G
-    from mpmsnow_amd import scenes
-
     import torch
 
     if args.copy_calibration:
@@ -579,86 +655,53 @@ def main():
         float(b.sum())  # a read-only pass over the copy's destination
         del a, b
 
-    sc = scenes.two_snowballs(n=args.particles, seed=1)
-    n = len(sc["m"])
-    sim = G.Sim(sc["x"], sc["m"], sc["h"], material=sc["material"], planes=sc["planes"], gravity=[sc["gravity"]],
-                v=sc["v"])
-    progress("two snowballs %d particles set up" % n)
-    for _ in range(args.warmup):
-        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    torch.cuda.synchronize()
-    progress("warm-up done")
-    start_state = None if args.no_cpu else snapshot(sim)
-    # timed region: only HVP launches carry HIP events, one launch in HVP_SAMPLE (the event
-    # records idle the stream for a few microseconds each; the period is coprime with the
-    # solver's 8-iteration polling batches, so the sample walks every iteration index)
-    sim.profile(["hvp"], period=HVP_SAMPLE)
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(sim.advance(sc["dt"], sc["max_iters"], sc["tol"]))
-        nb = stats[-1]["num_solved"]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    progress("timed steps done: %.3f ms/step" % ((t1 - t0) / args.steps * 1e3))
-    hvp = sim.profile_query()["hvp"]
-    elapsed = t1 - t0
-    value = n * args.steps / elapsed
-    hvp_ms = hvp["ms"] / max(1, hvp["launches"])
-    hvp_bytes = hvp["bytes"] / max(1, hvp["launches"])
-    achieved = hvp_bytes / (hvp_ms * 1e-3) / 1e9 if hvp_ms > 0 else 0.0
-    # largest body's CR iterations in the last timed step
-    last = [sim.body_stats(b, hist_cap=0) for b in range(nb)]
-    big = max(last, key=lambda r: r["particles"]) if last else {"iterations": 0, "particles": 0, "active_nodes": 0}
+    base = {"metric": "particle-steps/s & CR iters/step at 1e-5 residual, learned-PC vs plain CR",
+            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic (seeded SplitMix64 particles)"}
+    if wl == "snowballs":
+        n = args.particles or 1000000
+        sb = snowballs_leg(args.steps, args.warmup, n)
+        out = dict(base, value=sb["value"], ms_per_step=sb["ms_per_step"], dtype="f32 (FP64 CR reductions)",
+                   config={"workload": sb["config"], "particles": n, "preconditioner": "diagonal (plain CR)",
+                           "parallelism": "single GPU"})
+        h = sb["hvp"]
+        out["roofline"] = {"bound": "hbm", "kernel": "hvp", "achieved": round(h["achieved_gbs"], 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(h["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                           "traffic": read_traffic(wl), "bytes_per_launch": h["bytes_per_launch"],
+                           "ms_per_launch": h["ms_per_launch"], "launches": h["launches_timed"]}
+        out["two_snowballs_1M"] = sb
+        print(json.dumps(out))
+        return
 
-    # per-kernel table from two further steps with every kernel class timed (outside the timed region)
-    sim.profile(True)
-    for _ in range(2):
-        sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    prof = sim.profile_query()
-    sim.profile(False)
-    kernels = {}
-    for k, r in prof.items():
-        if r["launches"]:
-            ms = r["ms"] / r["launches"]
-            by = r["bytes"] / r["launches"]
-            kernels[k] = {"launches": r["launches"], "ms_per_launch": round(ms, 4),
-                          "gbs": round(by / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-    traffic = read_traffic()
-    out = {
-        "metric": "particle-steps/s & CR iters/step at 1e-5 residual, learned-PC vs plain CR",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (FP64 CR reductions)",
-        "data": "synthetic (seeded SplitMix64 particles)",
-        "config": {"workload": "two snowballs colliding: 1M particles, h=1/256, dt=5e-4, CR tol 1e-5, "
-                               "%d steps after %d warmup" % (args.steps, args.warmup),
-                   "particles": n, "preconditioner": "diagonal (plain CR)", "parallelism": "single GPU"},
-        "cr_iters_per_step": float(np.mean([s["total_iterations"] for s in stats])),
-        "cr_iters_note": "sum over the step's bodies; per body: largest body %d iterations (%d particles, %d "
-                         "active nodes) in the last timed step, max over bodies %d"
-                         % (big["iterations"], big["particles"], big["active_nodes"],
-                            max(s["max_iterations"] for s in stats)),
-        "bodies_per_step": float(np.mean([s["num_bodies"] for s in stats])),
-        "roofline": {"bound": "hbm", "kernel": "hvp", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": hvp_bytes, "ms_per_launch": hvp_ms, "launches": hvp["launches"]},
-        "hvp_issue": read_sq(),
-        "kernels": kernels,
-    }
+    n = args.particles or 4000000
+    legs, sc = slab_legs(args.steps, args.warmup, n, args.lpc_weights, args.lpc_fallback, args.lpc_dirs)
+    lr, pl = legs["learned"], legs["plain"]
+    h = lr["hvp"]
+    out = dict(base, value=lr["value"], ms_per_step=lr["ms_per_step"],
+               dtype="f32 (FP64 CR reductions; BF16 MFMA network, FP32 accumulation)",
+               config={"workload": "slope slab: %d particles, 0.8 x 0.2 x 0.4 m, per-particle E0 in [7e4, 2.8e5] Pa "
+                                   "and xi in [5, 10] (seed 2, held out from training), 30-degree slope, h=1/256, "
+                                   "dt=5e-4, CR tol 1e-5; learned flexible CR and plain diagonal CR, %d steps each "
+                                   "from the state after %d plain steps" % (n, args.steps, args.warmup),
+                       "particles": n, "preconditioner": "learned (flexible CR), against plain diagonal CR",
+                       "parallelism": "single GPU"})
+    out["value_is"] = "particle-steps/s of the learned-preconditioned step"
+    out["cr_iters_per_step"] = lr["cr_iters_per_step"]
+    out["learned"] = lr
+    out["plain"] = pl
+    out["speedup"] = lr["value"] / pl["value"]
+    out["iteration_ratio"] = pl["cr_iters_per_step"] / max(lr["cr_iters_per_step"], 1e-9)
+    out["roofline"] = {"bound": "hbm", "kernel": "hvp", "achieved": round(h["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(h["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                       "traffic": read_traffic(wl), "bytes_per_launch": h["bytes_per_launch"],
+                       "ms_per_launch": h["ms_per_launch"], "launches": h["launches_timed"]}
+    out["hvp_issue"] = read_sq(wl)
     if not args.no_cpu:
-        out["cpu_baseline"], o32 = cpu_baseline(sc, start_state, args.cpu_steps)
-        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
-        out["error_vs_fp64"] = error_vs_fp64(sc, start_state, o32, args.cpu_steps)
-    if not args.no_learned:
-        out["learned_pc"] = learned_leg(args.learned_steps, 3, args.lpc_weights, args.lpc_fallback, args.lpc_dirs)
+        out["cpu_baseline"], out["error_vs_fp64"] = cpu_leg(args.warmup)
+        out["cpu_baseline"]["gpu_over_cpu"] = pl["value"] / out["cpu_baseline"]["value"]
+    if not args.no_snowballs:
+        out["two_snowballs_1M"] = snowballs_leg(args.snowball_steps, 3)
     print(json.dumps(out))
 
 

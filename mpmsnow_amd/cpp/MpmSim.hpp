@@ -426,6 +426,11 @@ class CollisionObject {
       for (const CollisionObject* o : m_objects) delete o;
     }
     void add(CollisionObject* o) { m_objects.push_back(o); }
+    // deletes the objects (the reference's SOP rebuilds its set at every cook by assigning a fresh one)
+    void clear() {
+      for (const CollisionObject* o : m_objects) delete o;
+      m_objects.clear();
+    }
     const CollisionObject* object(size_t i) const { return m_objects[i]; }
     size_t numObjects() const { return m_objects.size(); }
     // the objects' device descriptions, in order
@@ -541,6 +546,10 @@ class LevelSetCollisionObject : public CollisionObject {
   float coulombFriction() const override { return m_coulombFriction; }
   bool sticky() const override { return m_sticky; }
   void setV(const Vector3f& v) { m_v = v; }
+  const Vector3f& origin() const { return m_origin; }
+  float voxel() const { return m_voxel; }
+  const int* dims() const { return m_dims; }
+  bool hasVelocityField() const { return !m_vel.empty(); }
   bool gpuCollider(mpm_collider& c) const override {
     c.type = 2;
     for (int i = 0; i < 3; ++i) c.dims[i] = m_dims[i], c.origin[i] = m_origin[i], c.velocity[i] = m_v[i];
@@ -606,6 +615,9 @@ class SquareMagnitudeTermination : public TerminationCriterion {
     return s < m_threshold;
   }
   bool cancelled() const override { return false; }
+  // true when this object's residual test is the base class's, which the device then runs; a
+  // subclass that keeps the test and only adds cancelled() overrides this to return true
+  virtual bool deviceTest() const { return typeid(*this) == typeid(SquareMagnitudeTermination); }
   int maxIters() const { return (int)m_maxIters; }
   float tolError() const { return m_tolError; }
 
@@ -647,10 +659,13 @@ inline mpm_solver_params solver_params(TerminationCriterion& t, LinearSolver::De
   h.term = &t;
   h.debug = d;
   p.user = &h;
-  if (typeid(t) == typeid(SquareMagnitudeTermination)) {
-    const SquareMagnitudeTermination& s = static_cast<const SquareMagnitudeTermination&>(t);
-    p.max_iters = s.maxIters();
-    p.tolerance = s.tolError();
+  const SquareMagnitudeTermination* sq = dynamic_cast<const SquareMagnitudeTermination*>(&t);
+  if (sq && sq->deviceTest()) {
+    p.max_iters = sq->maxIters();
+    p.tolerance = sq->tolError();
+    // a subclass may be cancelled: poll it after each body's solve (the base never is, and a
+    // cancellation hook keeps the batch in body order, which a plain solve has no need of)
+    if (typeid(t) != typeid(SquareMagnitudeTermination)) p.cancelled = cancel_cb;
   } else {
     p.termination = term_cb;
     p.cancelled = cancel_cb;

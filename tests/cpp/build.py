@@ -1,4 +1,4 @@
-"""Build recipe of the MpmSim C++ layer test (tests/cpp/_build/test_mpmsim).
+"""Build recipe of the C++ layer test programs (tests/cpp/_build/test_mpmsim, test_cook).
 
 g++ compiles tests/cpp/test_mpmsim.cpp against the header-only layer
 (mpmsnow_amd/cpp/MpmSim.hpp) and links the GPU library
@@ -15,20 +15,29 @@ BIN = os.path.join(OUT_DIR, "test_mpmsim")
 LIB_DIR = os.path.join(ROOT, "mpmsnow_amd", "_build")
 
 
-def build():
-    if not os.path.exists(os.path.join(LIB_DIR, "libmpmsnow_hip.so")):
-        raise RuntimeError("build the GPU library first (mpmsnow_amd/build.py)")
-    os.makedirs(OUT_DIR, exist_ok=True)
+COOK_BIN = os.path.join(OUT_DIR, "test_cook")
+
+
+def _compile(src, out):
     rpath = "$ORIGIN/" + os.path.relpath(LIB_DIR, OUT_DIR)
-    tmp = BIN + ".tmp"
+    tmp = out + ".tmp"
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "mpmsnow_amd", "cpp"),
-           os.path.join(HERE, "test_mpmsim.cpp"), "-L", LIB_DIR, "-lmpmsnow_hip",
+           os.path.join(HERE, src), "-L", LIB_DIR, "-lmpmsnow_hip",
            "-Wl,-rpath," + rpath, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("g++ failed:\n" + r.stderr)
-    os.replace(tmp, BIN)
+        raise RuntimeError("g++ failed for %s:\n%s" % (src, r.stderr))
+    os.replace(tmp, out)
+
+
+def build():
+    """Both test programs: the MpmSim layer's (test_mpmsim) and the SOP cook driver's (test_cook)."""
+    if not os.path.exists(os.path.join(LIB_DIR, "libmpmsnow_hip.so")):
+        raise RuntimeError("build the GPU library first (mpmsnow_amd/build.py)")
+    os.makedirs(OUT_DIR, exist_ok=True)
+    _compile("test_mpmsim.cpp", BIN)
+    _compile("test_cook.cpp", COOK_BIN)
     return BIN
 
 
