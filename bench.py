@@ -341,11 +341,13 @@ def stiff_leg(n, steps, warmup, seed=4, weights=None, fallback=None, dirs=None, 
 
 
 def _pmc_rows(path):
-    """(kernel name, grid size, counter value) per dispatch of a rocprofv3 counter_collection.csv."""
+    """(kernel name, grid size, counter value, start timestamp) per dispatch of a rocprofv3
+    counter_collection.csv."""
     rows = []
     with open(path) as fh:
         for r in csv.DictReader(fh):
-            rows.append((r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+            rows.append((r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"]),
+                         int(r.get("Start_Timestamp") or 0)))
     return rows
 
 
@@ -363,23 +365,46 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
         # calibration kernel of known traffic: the 256 MiB copy writes COPY_BYTES; the reduction
         # over its destination reads COPY_BYTES
         calib = "reduce_kernel" if counter == "FETCH_SIZE" else "copyBuffer"
-        copies = [v for n, g, v in rows if calib in n]
+        copies = [v for n, g, v, _ in rows if calib in n]
         if not copies:
             raise SystemExit("no %s dispatch in %s (run bench.py with --copy-calibration)" % (calib, path))
         copy_kib = max(copies)
         factor = copy_kib * 1024.0 / COPY_BYTES
-        hvp = [(g, v) for n, g, v in rows if "k_hvp_pair<true>" in n]
+        hvp = [(g, v, t) for n, g, v, t in rows if "k_hvp_pair<true>" in n]
         # launches of the steps after the warm-up (the store is body- and tile-sorted from the
         # first step on), full grid (every chunk launched): the median of those
-        hvp = hvp[len(hvp) * 3 // 5:]
-        gmax = max(g for g, _ in hvp)
-        full = sorted(v for g, v in hvp if g == gmax)
+        late = hvp[len(hvp) * 3 // 5:]
+        gmax = max(g for g, _, _ in late)
+        full = sorted(v for g, v, _ in late if g == gmax)
         kib = full[len(full) // 2]
         res[counter] = {"copy_kib": copy_kib, "factor": factor, "hvp_kib_per_launch": kib,
                         "hvp_bytes_per_launch": kib * 1024.0 / factor, "launches": len(full)}
-    traffic = res["FETCH_SIZE"]["hvp_bytes_per_launch"] + res["WRITE_SIZE"]["hvp_bytes_per_launch"]
-    summary = {"kernel": "k_hvp_pair<true>", "workload": WORKLOAD_NAMES[workload],
-               "traffic_bytes_per_launch": traffic, "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
+        # every HVP dispatch of the learned leg (between its first and last network dispatch; the
+        # warm-up and the plain leg run no network): the population whose algorithmic bytes the
+        # bench line's roofline averages
+        net = [t for n, _, _, t in rows if "k_conv" in n]
+        if net:
+            lo, hi = min(net), max(net)
+            leg = [v for n, _, v, t in rows if "k_hvp" in n and lo <= t <= hi]
+            if leg:
+                res[counter]["leg_launches"] = len(leg)
+                res[counter]["leg_bytes_per_launch"] = sum(leg) / len(leg) * 1024.0 / factor
+    full_traffic = res["FETCH_SIZE"]["hvp_bytes_per_launch"] + res["WRITE_SIZE"]["hvp_bytes_per_launch"]
+    leg = all("leg_bytes_per_launch" in res[k] for k in res)
+    traffic = (res["FETCH_SIZE"]["leg_bytes_per_launch"] + res["WRITE_SIZE"]["leg_bytes_per_launch"] if leg
+               else full_traffic)
+    # algorithmic bytes per HVP launch of the profiled run itself (its bench line's HIP-event sample)
+    algo = None
+    log = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(fetch_csv))),
+                       os.path.basename(os.path.dirname(os.path.abspath(fetch_csv))) + ".log")
+    if os.path.exists(log):
+        m = re.search(r'"roofline": (\{[^}]*\})', open(log).read())
+        if m:
+            algo = json.loads(m.group(1))["bytes_per_launch"]
+    summary = {"kernel": "k_hvp_pair<true>" if not leg else "HVP (k_hvp_pair<true>, k_hvp<true>)",
+               "workload": WORKLOAD_NAMES[workload], "traffic_bytes_per_launch": traffic,
+               "traffic_full_grid_bytes_per_launch": full_traffic, "algorithmic_bytes_per_launch": algo,
+               "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
     cmd = bench_command(workload, 2, 3, " --copy-calibration")
     lines = [
         "# HVP HBM traffic, " + WORKLOAD_TITLES[workload],
@@ -405,8 +430,17 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
         r = res[k]
         lines.append("| %s | %.0f | %.4f | %.0f | %.4g |" % (k, r["copy_kib"], r["factor"], r["hvp_kib_per_launch"],
                                                           r["hvp_bytes_per_launch"]))
-    lines += ["", "traffic (FETCH + WRITE) per HVP launch: %.4g bytes" % traffic, "",
-              "<!-- json: %s -->" % json.dumps(summary)]
+    lines += ["", "traffic (FETCH + WRITE) per full-grid HVP launch: %.4g bytes" % full_traffic]
+    if leg:
+        lines += ["", "Every HVP dispatch of the learned leg (between its first and last network dispatch; the "
+                  "population the bench line's roofline averages): %d dispatches, FETCH %.4g + WRITE %.4g = "
+                  "**%.4g bytes per launch**." % (res["FETCH_SIZE"]["leg_launches"],
+                                                  res["FETCH_SIZE"]["leg_bytes_per_launch"],
+                                                  res["WRITE_SIZE"]["leg_bytes_per_launch"], traffic)]
+    if algo:
+        lines += ["", "Algorithmic bytes per HVP launch of the same run (its HIP-event sample of the learned "
+                  "leg): %.4g; traffic / algorithmic = %.2f." % (algo, traffic / algo)]
+    lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
     with open(out_path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print(json.dumps(summary))
@@ -501,6 +535,31 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
     rp_ms = hvp_ns / hvp_calls / 1e6
     ev_ms = bench["roofline"]["ms_per_launch"]
     steps, warmup = bench["steps"], bench["warmup"]
+    # the HVP dispatches of the learned leg alone, whose launches bench.py's HIP events sample: from
+    # the kernel trace, those between the leg's first and last network dispatch (the warm-up and
+    # the plain leg run no network)
+    timed = None
+    trace_csv = os.path.join(os.path.dirname(stats_csv), os.path.basename(stats_csv).replace("kernel_stats",
+                                                                                            "kernel_trace"))
+    if os.path.exists(trace_csv):
+        hv, net = [], []
+        with open(trace_csv) as fh:
+            for r in csv.DictReader(fh):
+                nm = _short(r["Kernel_Name"])
+                be = (int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                if nm.startswith("k_hvp"):
+                    hv.append(be)
+                elif nm.startswith("k_conv"):
+                    net.append(be)
+        if net:
+            lo, hi = min(b for b, _ in net), max(e for _, e in net)
+            hv = [(b, e) for b, e in hv if lo <= b <= hi]
+        else:
+            hv.sort()
+            hv = hv[-HVP_SAMPLE * bench["roofline"]["launches"]:]
+        k = len(hv)
+        if k:
+            timed = {"launches": k, "ms": sum(e - b for b, e in hv[-k:]) / k / 1e6}
     legs = ("the learned and the plain leg (each --steps steps from the kept state)" if workload == "slab"
             else "the timed steps")
     lines = [
@@ -521,13 +580,22 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
         % (hvp_calls, rp_ms, HVP_SAMPLE, bench["roofline"]["launches"], ev_ms,
            rp_ms / ev_ms if ev_ms else float("nan")),
         "",
+    ]
+    if timed:
+        lines += ["HVP dispatches of the learned leg alone (%d, those between its first and last network "
+                  "dispatch in `run_kernel_trace.csv`): mean %.4f ms (rocprofv3) against the HIP events' %.4f ms "
+                  "(ratio %.3f). The run's mean above also covers the warm-up, whose first step solves with the "
+                  "particle store still in input order and runs to the iteration cap, and the plain leg." % (timed["launches"], timed["ms"], ev_ms,
+                                                       timed["ms"] / ev_ms if ev_ms else float("nan")), ""]
+    lines += [
         "| kernel | calls | total ms | mean us | share |",
         "|---|---|---|---|---|",
     ]
     for n, c, t, pct in rows[:20]:
         lines.append("| `%s` | %d | %.2f | %.1f | %.1f%% |" % (n, c, t / 1e6, t / c / 1e3, pct))
     summary = {"workload": WORKLOAD_NAMES[workload], "hvp_launches": hvp_calls, "hvp_ms_rocprof": rp_ms,
-               "hvp_ms_events": ev_ms, "steps": steps, "warmup": warmup}
+               "hvp_ms_events": ev_ms, "steps": steps, "warmup": warmup,
+               "hvp_ms_rocprof_timed": timed["ms"] if timed else None}
     lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
     with open(out_path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
