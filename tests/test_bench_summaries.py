@@ -60,3 +60,37 @@ def test_sq_summary_fractions(tmp_path):
     assert s["valu_issue_frac"] == 0.6
     assert s["lds_active_frac"] == 0.5
     assert s["resident_waves_per_cu"] == 18.0
+
+
+def test_pmc_summary_learned_leg_population(tmp_path):
+    """With network dispatches present, the traffic figure is the mean over every HVP dispatch
+    between the first and last of them (the learned leg), not the full-grid median."""
+    mib = bench.COPY_BYTES / 1024.0
+    for counter, calib, factor in (("FETCH_SIZE", "reduce_kernel", 0.5), ("WRITE_SIZE", "copyBuffer", 1.0)):
+        rows = [("at::native::" + calib, 1, factor * mib, 1)]
+        t = 10
+        # warm-up: large launches, no network
+        for _ in range(5):
+            rows.append(("void mpm::k_hvp_pair<true>(x)", 4096, 5000.0 * factor, t))
+            t += 1
+        # learned leg: network + HVPs of 1000 and 3000 KiB
+        for v in (1000.0, 3000.0, 1000.0, 3000.0):
+            rows.append(("void mpm::k_conv<16, 16, 0>(x)", 64, 1.0, t))
+            rows.append(("void mpm::k_hvp_pair<true>(x)", 4096 if v > 2000 else 1024, v * factor, t + 1))
+            t += 2
+        rows.append(("void mpm::k_conv<16, 16, 0>(x)", 64, 1.0, t))
+        # plain leg
+        for _ in range(5):
+            t += 1
+            rows.append(("void mpm::k_hvp_pair<true>(x)", 4096, 7000.0 * factor, t))
+        with open(str(tmp_path / (counter + ".csv")), "w", newline="") as fh:
+            w = csv.writer(fh)
+            w.writerow(["Kernel_Name", "Grid_Size", "Counter_Value", "Start_Timestamp"])
+            for r in rows:
+                w.writerow(r)
+    out = str(tmp_path / "traffic.md")
+    bench.pmc_summary(str(tmp_path / "FETCH_SIZE.csv"), str(tmp_path / "WRITE_SIZE.csv"), out)
+    s = footer(out)
+    assert s["fetch"]["leg_launches"] == 4
+    assert abs(s["fetch"]["leg_bytes_per_launch"] - 2000.0 * 1024) < 1e-6
+    assert abs(s["traffic_bytes_per_launch"] - 4000.0 * 1024) < 1e-6
