@@ -39,7 +39,8 @@ Other modes:
                        weights, 0.95, 2)
   --copy-calibration   also runs one 256 MiB device-to-device copy first (the
                        known-size kernel that scales the PMC byte counters)
-  --pmc-summary FETCH_CSV WRITE_CSV, --sq-summary P0 P1 P2, --trace-summary STATS_CSV LOG
+  --pmc-summary FETCH_CSV WRITE_CSV, --sq-summary P0 P1 P2, --trace-summary STATS_CSV LOG,
+  --mfma-summary PMC_CSV TRACE_CSV
                        summarise rocprofv3 counter / kernel-trace runs of bench.py into the
                        profiles/ files of the --workload they ran (slab, or snowballs for the
                        two-snowballs scene run alone with --workload snowballs)
@@ -401,7 +402,7 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
         m = re.search(r'"roofline": (\{[^}]*\})', open(log).read())
         if m:
             algo = json.loads(m.group(1))["bytes_per_launch"]
-    summary = {"kernel": "k_hvp_pair<true>" if not leg else "HVP (k_hvp_pair<true>, k_hvp<true>)",
+    summary = {"kernel": "k_hvp_pair<true>" if not leg else "k_hvp<true> (the flexible CR's HVP)",
                "workload": WORKLOAD_NAMES[workload], "traffic_bytes_per_launch": traffic,
                "traffic_full_grid_bytes_per_launch": full_traffic, "algorithmic_bytes_per_launch": algo,
                "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
@@ -430,10 +431,11 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
         r = res[k]
         lines.append("| %s | %.0f | %.4f | %.0f | %.4g |" % (k, r["copy_kib"], r["factor"], r["hvp_kib_per_launch"],
                                                           r["hvp_bytes_per_launch"]))
-    lines += ["", "traffic (FETCH + WRITE) per full-grid HVP launch: %.4g bytes" % full_traffic]
+    lines += ["", "traffic (FETCH + WRITE) per full-grid `k_hvp_pair<true>` launch: %.4g bytes" % full_traffic]
     if leg:
-        lines += ["", "Every HVP dispatch of the learned leg (between its first and last network dispatch; the "
-                  "population the bench line's roofline averages): %d dispatches, FETCH %.4g + WRITE %.4g = "
+        lines += ["", "Every HVP dispatch of the learned leg (`k_hvp<true>`, the flexible CR's HVP, one workgroup per "
+                  "chunk; between the leg's first and last network dispatch; the population the bench line's "
+                  "roofline averages): %d dispatches, FETCH %.4g + WRITE %.4g = "
                   "**%.4g bytes per launch**." % (res["FETCH_SIZE"]["leg_launches"],
                                                   res["FETCH_SIZE"]["leg_bytes_per_launch"],
                                                   res["WRITE_SIZE"]["leg_bytes_per_launch"], traffic)]
@@ -461,15 +463,31 @@ def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>", workload="slab"):
     of SQ_PASSES, and what they imply on MI355X (256 CUs, 1024 SIMDs, 32 shader engines; a wave64
     VALU instruction occupies its SIMD for 4 clocks)."""
     vals = {}
+    selection = "the last 40%% of the run's dispatches of that kernel"
     for path in csv_paths:
-        per = {}
+        per, start, net = {}, {}, []
         with open(path) as fh:
             for r in csv.DictReader(fh):
+                if "k_conv" in r["Kernel_Name"] and r.get("Start_Timestamp"):
+                    net.append(int(r["Start_Timestamp"]))
                 if kernel not in r["Kernel_Name"]:
                     continue
                 d = int(r.get("Dispatch_Id") or r["Correlation_Id"])
                 per.setdefault(d, {})[r["Counter_Name"]] = float(r["Counter_Value"])
-        ids = sorted(per)[len(per) * 3 // 5:]  # the steps after the warm-up
+                if r.get("Start_Timestamp"):
+                    start[d] = int(r["Start_Timestamp"])
+        if net and start:
+            # the learned leg (between its first and last network dispatch), dispatches that found
+            # work: a launch after every body of the batch has converged lasts a few percent of a
+            # full one (its waves exit at the done flag)
+            lo, hi = min(net), max(net)
+            ids = [d for d in sorted(per) if lo <= start.get(d, -1) <= hi]
+            top = max((per[d].get("GRBM_GUI_ACTIVE", 0.0) for d in ids), default=0.0)
+            ids = [d for d in ids if per[d].get("GRBM_GUI_ACTIVE", 0.0) >= 0.25 * top]
+            selection = ("the learned leg's dispatches (between its first and last network dispatch) that found work "
+                         "(GRBM_GUI_ACTIVE, in every pass, at least 25%% of the leg's longest)")
+        else:
+            ids = sorted(per)[len(per) * 3 // 5:]  # the steps after the warm-up
         if not ids:
             raise SystemExit("no %s dispatch in %s" % (kernel, path))
         for name in per[ids[0]]:
@@ -490,8 +508,8 @@ def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>", workload="slab"):
                      % (p, k, bench_command(workload, 2, 1)))
     lines += ["    python bench.py --workload %s --sq-summary <P0>/run_counter_collection.csv "
               "<P1>/run_counter_collection.csv <P2>/run_counter_collection.csv" % workload, "",
-              "Values: mean per `%s` dispatch over the last 40%% of the run's dispatches of that kernel, "
-              "summed over the device as rocprofv3 reports them." % kernel, "",
+              ("Values: mean per `%s` dispatch over " + selection + ", summed over the device as rocprofv3 reports "
+               "them.") % kernel, "",
               "| counter | per dispatch |", "|---|---|"]
     for name in sorted(vals):
         lines.append("| %s | %.4g |" % (name, vals[name]))
@@ -506,6 +524,77 @@ def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>", workload="slab"):
               "* resident waves per CU: SQ_WAVE_CYCLES x 4 / 256 / kernel clocks = %.1f" % resident,
               "* VALU instructions per wave: %.0f" % (vals["SQ_INSTS_VALU"] / vals["SQ_WAVES"]),
               "", "<!-- json: %s -->" % json.dumps(summary)]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(json.dumps(summary))
+
+
+MFMA_PASS = ("SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_INSTS_VALU_MFMA_BF16 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES SQ_WAVES "
+             "GRBM_GUI_ACTIVE")
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def mfma_summary(pmc_csv, trace_csv, out_path):
+    """The network's BF16 MFMA convolutions (`k_conv<CIN, COUT, MODE>`) on the slope slab's learned
+    leg: per kernel, MFMA FLOPs per dispatch from SQ_INSTS_VALU_MFMA_MOPS_BF16 (units of 512 FLOPs)
+    and, with each kernel's mean duration from a kernel trace of the same workload, the FLOP rate
+    against 2.5 PFLOP/s and the MFMA pipe's busy fraction: SQ_VALU_MFMA_BUSY_CYCLES (summed over the
+    1024 SIMDs; a 16x16x32 BF16 MFMA counts 16) / 1024 / (duration x 2.4 GHz). The counter run's
+    own kernels run about 2x longer than in the trace (GRBM_GUI_ACTIVE, summed over the 8 XCDs, / 8
+    is their length in clocks), so its clocks are not used."""
+    per = {}
+    with open(pmc_csv) as fh:
+        for r in csv.DictReader(fh):
+            nm = _short(r["Kernel_Name"])
+            if not nm.startswith("k_conv"):
+                continue
+            d = int(r.get("Dispatch_Id") or r["Correlation_Id"])
+            per.setdefault(nm, {}).setdefault(d, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    if not per:
+        raise SystemExit("no k_conv dispatch in %s" % pmc_csv)
+    dur = {}
+    with open(trace_csv) as fh:
+        for r in csv.DictReader(fh):
+            nm = _short(r["Kernel_Name"])
+            if nm.startswith("k_conv"):
+                dur.setdefault(nm, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows, tot_fl, tot_ns = [], 0.0, 0.0
+    for nm in sorted(per):
+        ds = per[nm].values()
+        mean = {k: sum(d.get(k, 0.0) for d in ds) / len(ds) for k in next(iter(ds))}
+        fl = mean.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0) * 512.0
+        ns = sum(dur.get(nm, [0])) / max(1, len(dur.get(nm, [])))
+        # MFMA busy cycles per SIMD over the kernel's clocks in the trace run (2.4 GHz)
+        busy = mean.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / 1024.0 / max(1.0, ns * 2.4)
+        rate = fl / (ns * 1e-9) / 1e12 if ns else 0.0
+        rows.append((nm, len(ds), fl, busy, ns, rate))
+        tot_fl += fl
+        tot_ns += ns
+    lines = ["# BF16 MFMA convolutions of the learned preconditioner, " + WORKLOAD_TITLES["slab"], "",
+             "Commands (from the repository root on the GPU host):", "",
+             "    rocprofv3 --pmc %s --output-format csv -d <M> -o run -- %s" % (MFMA_PASS,
+                                                                              bench_command("slab", 2, 3)),
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <D> -o run -- %s" % bench_command("slab", 10, 5),
+             "    python bench.py --mfma-summary <M>/run_counter_collection.csv <D>/run_kernel_trace.csv", "",
+             "Per kernel: MFMA FLOPs per dispatch = SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 (mean over the counter run's "
+             "dispatches); mean duration from the kernel trace (every dispatch of the kernel); rate = FLOPs / "
+             "duration against %.0f TFLOP/s dense BF16; MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / 1024 SIMDs / "
+             "(duration x 2.4 GHz) (the counter sums the SIMDs' busy cycles, 16 per 16x16x32 BF16 MFMA, so it "
+             "matches the rate's fraction). The counter run's kernels take about 2x their trace duration (its "
+             "GRBM_GUI_ACTIVE, summed over the 8 XCDs, / 8), so its clocks are not used." % BF16_PEAK_TFLOPS, "",
+             "| kernel | dispatches (counters) | MFMA GFLOP / dispatch | MFMA busy | mean us | TFLOP/s | of peak |",
+             "|---|---|---|---|---|---|---|"]
+    for nm, nd, fl, busy, ns, rate in rows:
+        lines.append("| `%s` | %d | %.2f | %.1f%% | %.1f | %.1f | %.1f%% |" % (
+            nm, nd, fl / 1e9, 100 * busy, ns / 1e3, rate, 100 * rate / BF16_PEAK_TFLOPS))
+    agg = tot_fl / (tot_ns * 1e-9) / 1e12 if tot_ns else 0.0
+    lines += ["", "All convolutions of one forward: %.1f GFLOP of MFMA work in %.1f us of kernel time: %.1f TFLOP/s, "
+              "%.1f%% of the dense BF16 peak." % (tot_fl / 1e9, tot_ns / 1e3, agg, 100 * agg / BF16_PEAK_TFLOPS)]
+    summary = {"workload": WORKLOAD_NAMES["slab"], "conv_gflop_per_forward": tot_fl / 1e9,
+               "conv_us_per_forward": tot_ns / 1e3, "tflops": agg, "frac_of_bf16_peak": agg / BF16_PEAK_TFLOPS,
+               "kernels": {nm: {"gflop": fl / 1e9, "mfma_busy": busy, "us": ns / 1e3, "tflops": rate}
+                           for nm, nd, fl, busy, ns, rate in rows}}
+    lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
     with open(out_path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print(json.dumps(summary))
@@ -603,6 +692,8 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
 
 
 STIFF_STATS_FILE = os.path.join(ROOT, "profiles", "stiff_drop_16M_kernel_stats.md")
+MFMA_FILE = os.path.join(ROOT, "profiles", "lpc_conv_mfma_slope_slab_4M.md")
+PRECISION_FILE = os.path.join(ROOT, "profiles", "slope_slab_precision_300_steps.md")
 
 
 def stiff_trace_summary(stats_csv, stiff_json, out_path):
@@ -643,6 +734,14 @@ def stiff_trace_summary(stats_csv, stiff_json, out_path):
         lines.append("| `%s` | %d | %.2f | %.1f | %.1f%% |" % (n, c, t / 1e6, t / c / 1e3, pct))
     with open(out_path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
+
+
+def read_footer(path):
+    """The JSON footer of a committed profiles/ summary, or None."""
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"<!-- json: (.*) -->", open(path).read())
+    return json.loads(m.group(1)) if m else None
 
 
 def read_sq(workload="slab"):
@@ -688,6 +787,7 @@ def main():
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_CSV", "BENCH_LOG"))
     ap.add_argument("--sq-summary", nargs=3, metavar=("PASS0_CSV", "PASS1_CSV", "PASS2_CSV"))
+    ap.add_argument("--mfma-summary", nargs=2, metavar=("PMC_CSV", "TRACE_CSV"))
     args = ap.parse_args()
     wl = args.workload
     if args.pmc_summary:
@@ -697,7 +797,11 @@ def main():
         stiff_trace_summary(args.stiff_trace_summary[0], args.stiff_trace_summary[1], STIFF_STATS_FILE)
         return
     if args.sq_summary:
-        sq_summary(args.sq_summary, profile_file("sq", wl), workload=wl)
+        sq_summary(args.sq_summary, profile_file("sq", wl), kernel="k_hvp<true>" if wl == "slab" else "k_hvp_pair<true>",
+                   workload=wl)
+        return
+    if args.mfma_summary:
+        mfma_summary(args.mfma_summary[0], args.mfma_summary[1], MFMA_FILE)
         return
     if args.trace_summary:
         trace_summary(args.trace_summary[0], args.trace_summary[1], profile_file("stats", wl), wl)
@@ -765,6 +869,18 @@ def main():
                        "traffic": read_traffic(wl), "bytes_per_launch": h["bytes_per_launch"],
                        "ms_per_launch": h["ms_per_launch"], "launches": h["launches_timed"]}
     out["hvp_issue"] = read_sq(wl)
+    mf = read_footer(MFMA_FILE)
+    if mf and "network" in lr:
+        lr["network"]["mfma_counters"] = {
+            "profile": os.path.relpath(MFMA_FILE, ROOT), "tflops": round(mf["tflops"], 1),
+            "frac_of_bf16_peak": round(mf["frac_of_bf16_peak"], 4),
+            "conv_gflop_per_forward": round(mf["conv_gflop_per_forward"], 2)}
+    ps = read_footer(PRECISION_FILE)
+    if ps:
+        out["precision_study"] = {"profile": os.path.relpath(PRECISION_FILE, ROOT), "particles": ps["particles"],
+                                  "steps": ps["steps"], "max_over_steps": ps["max_over_steps"],
+                                  "note": "FP32-bulk GPU step vs the FP64 oracle (and the FP32 oracle) over the "
+                                          "same steps at the slab's density, h and dt"}
     if not args.no_cpu:
         out["cpu_baseline"], out["error_vs_fp64"] = cpu_leg(args.warmup)
         out["cpu_baseline"]["gpu_over_cpu"] = pl["value"] / out["cpu_baseline"]["value"]

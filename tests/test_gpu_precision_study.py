@@ -1,0 +1,28 @@
+"""The slope-slab precision study (tests/precision_study.py) at a small size: 6000 particles of the
+4M slab's density, materials, h and dt, 12 steps on the GPU and in the FP32 and FP64 oracles. Every
+step, the GPU's momentum, position and F errors against FP64 stay within 3x the FP32 oracle's (plus
+floors of 1e-6 relative, 1e-6 h, 1e-6) at the maximum and the 99th percentile, and the summary
+file carries the per-step table."""
+import json
+import re
+
+import pytest
+
+import precision_study as PS
+
+pytestmark = pytest.mark.gpu
+
+
+def test_precision_study_small_slab(tmp_path):
+    print("[gpu-case] precision study: slope slab 6000 particles, 12 steps", flush=True)
+    recs, meta = PS.run_study(6000, 12, str(tmp_path / "steps.csv"), log=lambda s: print(s, flush=True))
+    floors = {"mom_rel": 1e-6, "x_max": 1e-6, "x_p99": 1e-6, "F_max": 1e-6, "F_p99": 1e-6}
+    for r in recs:
+        for q, fl in floors.items():
+            assert r["gpu_" + q] <= 3.0 * r["fp32_" + q] + fl, (r["step"], q, r["gpu_" + q], r["fp32_" + q])
+    out = tmp_path / "study.md"
+    s = PS.summarize(recs, meta, str(out), "python tests/precision_study.py --n 6000 --steps 12")
+    foot = json.loads(re.search(r"<!-- json: (.*) -->", out.read_text()).group(1))
+    assert foot["steps"] == 12 and foot["particles"] == meta["particles"]
+    assert s["max_over_steps"]["gpu"]["x_max"] == max(r["gpu_x_max"] for r in recs)
+    assert len((tmp_path / "steps.csv").read_text().splitlines()) == 13
