@@ -17,13 +17,15 @@ bytes per launch (172 B per particle + 29 B per active node of the bodies still 
 mean launch time, both measured in the timed steps with HIP events on the library's stream
 (one HVP launch in 7), against MI355X's 8 TB/s HBM peak. roofline.traffic = measured HBM bytes
 per HVP launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload.
-network = the preconditioner's forward pass (MFMA BF16): FLOPs per forward from the layer shapes
-over the active tiles, mean time per forward (HIP events), fraction of 2.5 PFLOP/s.
+network = the preconditioner's forward pass (MFMA BF16): mean time per forward (HIP events), an
+upper bound of its FLOPs from the layer shapes, and the MFMA FLOPs that rocprofv3 counted
+(committed summary). precision_study = the committed 300-step FP32-vs-FP64 study of the slab.
 cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the 8-colour voxel
-partition like the reference's TBB splats) timed on this host for a bounded sample: one step of
-a 1M-particle slab of the same density, materials, h and dt (0.4 x 0.2 x 0.2 m), from the GPU's
-state after the same warm-up. error_vs_fp64 steps that sample's state once more on the GPU and in
-the FP64 oracle.
+partition like the reference's TBB splats) timed on this host for a bounded sample: a 500k-
+particle slab of the same density, materials, h and dt (0.2 m cube) from the GPU's state after
+the same warm-up, 2 warm-up steps excluded, the median of 3 timed steps. error_vs_fp64 steps
+that sample's state once on the GPU and in the FP64 oracle and compares with the FP32 oracle's
+first step.
 two_snowballs_1M = the two-snowballs scene (1M particles, +/-3 m/s, h = 1/256, dt = 5e-4, plain
 CR), particle-steps/s over --snowball-steps steps.
 
@@ -125,43 +127,57 @@ def gpu_sim(sc):
                  v=sc.get("v"), **extra)
 
 
-def cpu_leg(warmup):
-    """cpu_baseline and error_vs_fp64 on a bounded sample of the workload: a 1M-particle slab of the
-    slope-slab scene's density, materials, h and dt (box 0.4 x 0.2 x 0.2 m, seed 2), stepped
-    `warmup` times on the GPU; then one step of the FP32 oracle (timed), one of the FP64 oracle
-    and one of the GPU from that state."""
+def cpu_leg(warmup, n=500000, cpu_warmup=2, cpu_timed=3):
+    """cpu_baseline and error_vs_fp64 on a bounded sample of the workload: a slab of the slope-slab
+    scene's density, materials, h and dt (n = 500k: box 0.2 x 0.2 x 0.2 m, seed 2), stepped
+    `warmup` times on the GPU. From that state the FP32 oracle takes cpu_warmup untimed steps and
+    cpu_timed timed ones (the median is the baseline); its first step is compared, with the FP64
+    oracle's and the GPU's first step from the same state, against FP64."""
     import numpy as np
 
     from mpmsnow_amd import scenes
     from oracle import pyoracle as O
 
-    sc = scenes.slope_slab(n=1000000, seed=2, size=(0.4, 0.2, 0.2))
+    side = 0.2 * (n / 500000.0) ** (1.0 / 3.0)
+    sc = scenes.slope_slab(n=n, seed=2, size=(side, side, side))
     n = len(sc["m"])
     sim = gpu_sim(sc)
     for _ in range(warmup):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
     state = snapshot(sim)
     o32 = oracle_sim(sc, state)
-    progress("cpu baseline: 1M slab sample, oracle set up")
-    t0 = time.perf_counter()
-    o32.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    dt = time.perf_counter() - t0
-    progress("cpu baseline: %.1f s" % dt)
-    its = max(s["iterations"] for s in o32.last_stats())
-    line = {"value": n / dt, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "1 step of a 1M-particle slab of the workload's density, materials, h and dt (0.4 x 0.2 x "
-                      "0.2 m, seed 2) from the GPU's state after %d steps (%.1f s; %d CR iterations for its "
-                      "largest body)" % (warmup, dt, its)}
+    progress("cpu baseline: %d-particle slab sample, oracle set up" % n)
+    times, its = [], []
+    first32 = None
+    for k in range(cpu_warmup + cpu_timed):
+        t0 = time.perf_counter()
+        o32.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        dt = time.perf_counter() - t0
+        if k == 0:
+            first32 = {"p": o32.pd.get("p").reshape(n, 3).copy(), "F": o32.pd.get("F").reshape(n, 9).copy()}
+        if k >= cpu_warmup:
+            times.append(dt)
+            its.append(max(s["iterations"] for s in o32.last_stats()))
+        progress("cpu baseline: oracle step %d %.1f s%s" % (k, dt, "" if k >= cpu_warmup else " (warm-up)"))
+    med = float(np.median(times))
+    line = {"value": n / med, "unit": "particle-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "FP32 oracle on a %d-particle slab of the workload's density, materials, h and dt (%.3g m cube, "
+                      "seed 2) from the GPU's state after %d steps: %d warm-up steps excluded, median of %d timed "
+                      "steps (%s s; %s CR iterations for the largest body)"
+                      % (n, side, warmup, cpu_warmup, cpu_timed, ", ".join("%.1f" % t for t in times),
+                         ", ".join(str(i) for i in its))}
     progress("error check: FP64 oracle, 1 step")
     o64 = oracle_sim(sc, state, "double")
     o64.advance(sc["dt"], sc["max_iters"], sc["tol"])
     sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
-    out = {"steps": 1, "note": "the CPU sample's step on the GPU, in the FP32 and in the FP64 oracle; positions "
-                               "in units of h"}
+    out = {"steps": 1, "particles": n,
+           "note": "one step from the CPU sample's state on the GPU, in the FP32 and in the FP64 oracle; "
+                   "positions in units of h"}
     ref = {"p": o64.pd.get("p").reshape(n, 3), "F": o64.pd.get("F").reshape(n, 9)}
-    for who, get in (("gpu", sim.get), ("fp32_oracle", o32.pd.get)):
-        dx = np.abs(get("p").reshape(n, 3) - ref["p"]).max(1) / sc["h"]
-        dF = np.abs(get("F").reshape(n, 9) - ref["F"]).max(1)
+    for who, got in (("gpu", {"p": sim.get("p").reshape(n, 3), "F": sim.get("F").reshape(n, 9)}),
+                     ("fp32_oracle", first32)):
+        dx = np.abs(got["p"] - ref["p"]).max(1) / sc["h"]
+        dF = np.abs(got["F"] - ref["F"]).max(1)
         out[who] = {"max_dx_over_h": float(dx.max()), "p99_dx_over_h": float(np.percentile(dx, 99)),
                     "max_dF": float(dF.max()), "p99_dF": float(np.percentile(dF, 99))}
     return line, out
@@ -229,9 +245,12 @@ def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
                 fms = lp["ms"] / lp["launches"]
                 fl = lpc_forward_flops(sim)
                 leg["network"] = {"forwards_timed": lp["launches"], "ms_per_forward": fms,
-                                  "flops_per_forward": fl, "tflops": fl / (fms * 1e-3) / 1e12,
-                                  "frac_of_bf16_peak": fl / (fms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS,
-                                  "note": "FLOPs over the loop's tiles at every level (upper bound)"}
+                                  "flops_per_forward_upper_bound": fl,
+                                  "tflops_upper_bound": fl / (fms * 1e-3) / 1e12,
+                                  "note": "upper bound: 2 x voxels x 27 x cin x cout over every voxel of the "
+                                          "loop's tiles at each level, over the whole forward (convolutions, "
+                                          "pooling, features); the MFMA work counted by rocprofv3 is in "
+                                          "mfma_counters"}
         progress("slab %s: %.2f ms/step, largest body %.1f CR iterations/step, fallbacks %.2f/step"
                  % (name, leg["ms_per_step"], leg["cr_iters_per_step"], leg["fallbacks_per_step"]))
         out[name] = leg
@@ -531,7 +550,6 @@ def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>", workload="slab"):
 
 MFMA_PASS = ("SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_INSTS_VALU_MFMA_BF16 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES SQ_WAVES "
              "GRBM_GUI_ACTIVE")
-BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 
 
 def mfma_summary(pmc_csv, trace_csv, out_path):
@@ -581,17 +599,17 @@ def mfma_summary(pmc_csv, trace_csv, out_path):
              "duration against %.0f TFLOP/s dense BF16; MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / 1024 SIMDs / "
              "(duration x 2.4 GHz) (the counter sums the SIMDs' busy cycles, 16 per 16x16x32 BF16 MFMA, so it "
              "matches the rate's fraction). The counter run's kernels take about 2x their trace duration (its "
-             "GRBM_GUI_ACTIVE, summed over the 8 XCDs, / 8), so its clocks are not used." % BF16_PEAK_TFLOPS, "",
+             "GRBM_GUI_ACTIVE, summed over the 8 XCDs, / 8), so its clocks are not used." % MFMA_BF16_PEAK_TFLOPS, "",
              "| kernel | dispatches (counters) | MFMA GFLOP / dispatch | MFMA busy | mean us | TFLOP/s | of peak |",
              "|---|---|---|---|---|---|---|"]
     for nm, nd, fl, busy, ns, rate in rows:
         lines.append("| `%s` | %d | %.2f | %.1f%% | %.1f | %.1f | %.1f%% |" % (
-            nm, nd, fl / 1e9, 100 * busy, ns / 1e3, rate, 100 * rate / BF16_PEAK_TFLOPS))
+            nm, nd, fl / 1e9, 100 * busy, ns / 1e3, rate, 100 * rate / MFMA_BF16_PEAK_TFLOPS))
     agg = tot_fl / (tot_ns * 1e-9) / 1e12 if tot_ns else 0.0
     lines += ["", "All convolutions of one forward: %.1f GFLOP of MFMA work in %.1f us of kernel time: %.1f TFLOP/s, "
-              "%.1f%% of the dense BF16 peak." % (tot_fl / 1e9, tot_ns / 1e3, agg, 100 * agg / BF16_PEAK_TFLOPS)]
+              "%.1f%% of the dense BF16 peak." % (tot_fl / 1e9, tot_ns / 1e3, agg, 100 * agg / MFMA_BF16_PEAK_TFLOPS)]
     summary = {"workload": WORKLOAD_NAMES["slab"], "conv_gflop_per_forward": tot_fl / 1e9,
-               "conv_us_per_forward": tot_ns / 1e3, "tflops": agg, "frac_of_bf16_peak": agg / BF16_PEAK_TFLOPS,
+               "conv_us_per_forward": tot_ns / 1e3, "tflops": agg, "frac_of_bf16_peak": agg / MFMA_BF16_PEAK_TFLOPS,
                "kernels": {nm: {"gflop": fl / 1e9, "mfma_busy": busy, "us": ns / 1e3, "tflops": rate}
                            for nm, nd, fl, busy, ns, rate in rows}}
     lines += ["", "<!-- json: %s -->" % json.dumps(summary)]

@@ -174,7 +174,7 @@ def summarize(recs, meta, out_path, command):
     if fp32:
         ratio = {q: (worst["gpu"][q] / worst["fp32"][q] if worst["fp32"][q] > 0 else None) for q in QUANTITIES}
         lines += ["", "GPU / FP32-oracle, max over steps: " + ", ".join(
-            "%s %.2f" % (q, v) for q, v in ratio.items() if v is not None) + "."]
+            "%s %.3g" % (q, v) for q, v in ratio.items() if v is not None) + "."]
     its = np.array([[r["its_gpu"], r["its_fp64"]] for r in recs], dtype=float)
     lines += ["", "CR iterations of the largest body, mean over steps: GPU %.1f, FP64 %.1f%s." % (
         its[:, 0].mean(), its[:, 1].mean(),

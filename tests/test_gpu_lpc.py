@@ -219,6 +219,44 @@ def test_fallback_ratio_setting():
     assert big["residual_norm2_last"] < np.float64(sc["tol"]) ** 2 * big["bnorm2"]
 
 
+def test_handoff_to_cr_after_the_third_fallback():
+    """A network with a large random output layer sends the body to the diagonal for good
+    within a few iterations. With the handoff (the default) the rest of the solve runs the CR
+    recurrences from the flexible CR's iterate; without it, GCR(2) on the diagonal. Both meet
+    the residual test from identical states; the handoff's history continues the flexible CR's
+    (one entry per iteration) and it takes no more iterations than GCR(2) plus 10% (the two are
+    the same Krylov iteration in exact arithmetic), and plain CR's plus the iterations before
+    the handoff."""
+    say("fcr handoff: contact snowball 20k, random network, handoff on / off / plain CR")
+    out = {}
+    for name, mode, ho in (("handoff", 1, 1), ("gcr", 1, 0), ("plain", 0, 1)):
+        sc, sim = _small_scene(seed=5)
+        for bad in (-1, 2):
+            with pytest.raises(G.MpmError):
+                sim.ctx.lpc_handoff(bad)
+        sim.ctx.lpc_handoff(ho)
+        for _ in range(2):
+            sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        p = random_params(13)
+        offs, _ = L.layer_offsets()
+        w, b = offs[5]
+        p[w:w + 27 * 16 * 16] *= 50.0
+        sim.ctx.lpc_set_params(p)
+        sim.ctx.lpc_mode(mode)
+        st = sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
+        big, rows = _largest(sim, st)
+        big = sim.body_stats([r["particles"] for r in rows].index(big["particles"]))
+        tol2 = np.float64(sc["tol"]) ** 2
+        assert big["residual_norm2_last"] < tol2 * big["bnorm2"], (name, big["iterations"])
+        assert len(big["residual_norm2"]) == big["iterations"]
+        out[name] = big
+    say("largest body iterations: handoff %d (fallbacks %d), GCR(2) %d, plain CR %d" % (
+        out["handoff"]["iterations"], out["handoff"]["fallbacks"], out["gcr"]["iterations"], out["plain"]["iterations"]))
+    assert out["handoff"]["fallbacks"] == 3
+    assert out["handoff"]["iterations"] <= 1.1 * out["gcr"]["iterations"] + 2
+    assert out["handoff"]["iterations"] <= out["plain"]["iterations"] + 16
+
+
 def test_ablation_trained_network_against_plain_cr():
     """What the trained network gains: a stiffer contact snowball (E = 1.4e6 Pa) is run twice from
     identical states; in one run the network is trained in-solve on two steps and the next steps
