@@ -213,6 +213,12 @@ int mpm_lpc_set_fallback(mpm_context* ctx, double ratio);
 /* directions the flexible CR orthogonalises against, 1-8 (default 2; with the diagonal
  * preconditioner every m >= 1 gives the same Krylov iteration in exact arithmetic) */
 int mpm_lpc_set_directions(mpm_context* ctx, int m);
+/* 1 (default): once no unfinished body of the solve uses the network (each has taken its third
+ * fallback or none had one), the rest of the solve runs the preconditioned CR recurrences of
+ * ConjugateResiduals.cpp:84-151 from the flexible CR's iterate, with r = b - A x recomputed; the
+ * iteration count, residual history and iteration cap continue. 0: the flexible CR runs to the
+ * end on the diagonal. */
+int mpm_lpc_set_handoff(mpm_context* ctx, int on);
 /* CR iterations at which training solves keep their residuals: 1-16 increasing indices
  * (default 0 1 2 3 4 6 8 11 15 20 26 33 41 50 60 72); a solve keeps the first `samples` of them */
 int mpm_lpc_set_capture_iterations(mpm_context* ctx, const int32_t* iters, int n);

@@ -109,6 +109,7 @@ SIGNATURES = {
     "mpm_lpc_set_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_lpc_set_fallback": (C.c_int, [C.c_void_p, C.c_double]),
     "mpm_lpc_set_directions": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_lpc_set_handoff": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_lpc_set_capture_iterations": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "mpm_lpc_train_next": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
     "mpm_lpc_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -342,6 +343,9 @@ class Context:
 
     def lpc_directions(self, m):
         self.check(lib().mpm_lpc_set_directions(self.ptr, int(m)))
+
+    def lpc_handoff(self, on):
+        self.check(lib().mpm_lpc_set_handoff(self.ptr, int(on)))
 
     def lpc_capture_iterations(self, iters):
         a = np.ascontiguousarray(iters, dtype=np.int32)

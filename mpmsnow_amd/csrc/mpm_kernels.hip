@@ -2015,7 +2015,7 @@ __device__ __forceinline__ void cr_setup2_node(const Batch& B, const NodeCtx& c,
 __device__ __forceinline__ bool cr_setup2_finish(CRScalars& S, const double (&tot)[2]) {
   S.rz = tot[0];
   S.ApdPAp = tot[1];
-  S.iter = 0;
+  // S.iter is 0 here from the rhs pass, or the flexible CR's count after a handoff
   if (tot[1] == 0.0) {
     S.early_exit = 1;
     return true;

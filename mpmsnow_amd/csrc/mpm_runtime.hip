@@ -257,11 +257,13 @@ struct mpm_context {
       ccTot, newToOld, ballPos;
   int iterHint = 8;
   int iterHintFcr = 8;
+  int iterHintHandoff = 8;  // CR iterations after a learned solve's handoff (last solve's count)
   // learned preconditioner
   LpcNet net;
   int lpcMode = 0;  // 0: diagonal-preconditioned CR; 1: learned flexible CR
   double lpcFallback = 0.95;  // fallback ratio of a network iteration's |r|^2 to the previous one
   int lpcDirs = 2;            // directions the flexible CR keeps (1..kFcrMax)
+  int lpcHandoff = 1;         // 1: hand the solve to the CR recurrences once the network is off
   LpcCapture cap;
   // measurement
   struct ProfRec {
@@ -685,6 +687,11 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
   };
   const int64_t iterCap = (prm && prm->termination) ? 1000000 : (int64_t)maxIters + 2;
   const bool learned = c.lpcMode == 1 && !hooks;
+  // learned mode: once no unfinished body uses the network any more, the rest of the solve runs
+  // the CR recurrences below from the current iterate (two node passes per iteration instead of
+  // the flexible CR's four)
+  bool handoff = false;
+  int iterAtHandoff = 0;
   if (learned) {
     // flexible conjugate residual with the learned preconditioner (mpm_kernels.hip, k_fcr_*)
     for (DevBuf* d : {&fr, &fp0, &fp1, &fq0, &fq1}) d->ensure(vecBytes());
@@ -693,7 +700,8 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     fP.ensure(vecBytes() * mdirs);
     fQ.ensure(vecBytes() * mdirs);
     const int netOn = c.net.initialised ? 1 : 0;
-    hvp(f(x), true, f(stageA), sc(), -1);
+    pair_chunks(B);
+    hvp(B, f(x), true, f(stageA), sc(), -1, true);
     launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, c.lpcFallback, mdirs, sc(),
                     st(), partials.as<double>(), s);
     fetch_status_errors();
@@ -705,7 +713,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     int it = 0;
     while (hst->bodies_done < nb) {
       if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
-      int batch = it == 0 ? std::max(1, c.iterHintFcr) : 8;
+      int batch = it == 0 ? std::max(1, (c.lpcHandoff && netActive) ? std::min(c.iterHintFcr, 8) : c.iterHintFcr) : 8;
       for (int k = 0; k < batch; ++k, ++it) {
         if (netActive) {
           int pn = c.prof_begin(PK_LPC, this, it);
@@ -713,7 +721,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
           c.prof_end(pn);
         }
         launch_fcr_diag_z(B, f(fr), f(diag), f(z), sc(), s);
-        hvp(f(z), true, f(stageA), sc(), it);
+        hvp(B, f(z), true, f(stageA), sc(), it, true);
         launch_fcr_az(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(z), f(diag), dt, f(Ar), f(fQ), vs,
                       sc(), st(), partials.as<double>(), s);
         launch_fcr_orth(B, f(z), f(Ar), f(fr), f(diag), f(fP), f(fQ), vs, sc(), st(), partials.as<double>(), s);
@@ -725,17 +733,26 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
         // the network runs only while some unfinished body still uses it
         HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
         c.sync();
+        // (a body off the network for one iteration only, lpc_pause, still uses it)
         netActive = false;
         for (int bi = 0; bi < nb; ++bi)
-          if (!hsc[bi].done && !hsc[bi].lpc_off) netActive = true;
+          if (!hsc[bi].done && (!hsc[bi].lpc_off || hsc[bi].lpc_pause)) netActive = true;
+        if (!netActive && c.lpcHandoff && hst->bodies_done < nb) {
+          handoff = true;
+          iterAtHandoff = INT32_MAX;
+          for (int bi = 0; bi < nb; ++bi)
+            if (!hsc[bi].done) iterAtHandoff = std::min(iterAtHandoff, hsc[bi].iter);
+          break;
+        }
       }
     }
-  } else {
+  }
+  if (!learned || handoff) {
     // bodies of a few node blocks run their whole solve in one workgroup each, on the second
     // stream, while the batched loop below iterates the rest
     int nsmall = 0;
     // (the iteration cap bounds the persistent kernel's run time)
-    if (!hooks && c.cap.want == 0 && c.smallMaxParticles > 0 && maxIters <= (1 << 16)) {
+    if (!hooks && !handoff && c.cap.want == 0 && c.smallMaxParticles > 0 && maxIters <= (1 << 16)) {
       int* list = reinterpret_cast<int*>(reinterpret_cast<char*>(c.pinned.p) + scb + 4096);
       for (int bi = 0; bi < nb; ++bi) {
         rec[bi].small = smallLayout[bi] != 0;
@@ -759,8 +776,10 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
       Bb.nblocks = prefixTiles / 4;
       Bb.nchunks = prefixChunks;
     }
-    // CR setup (ConjugateResiduals.cpp:50-76); the HVPs of the solve run over chunk pairs
-    pair_chunks(Bb);
+    // CR setup (ConjugateResiduals.cpp:50-76); the HVPs of the solve run over chunk pairs. After
+    // a handoff it starts from the flexible CR's iterate, r = b - A x recomputed; the bodies'
+    // iteration indices, histories and caps continue
+    if (!handoff) pair_chunks(Bb);
     hvp(Bb, f(x), true, f(stageA), sc(), kProfSetup, true);
     launch_cr_setup1(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
                      s);
@@ -771,7 +790,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     // training pairs: iterate x_k and residual r_k = M z_k at the iterations of the capture schedule
     LpcCapture& cap = c.cap;
     const std::vector<int>& kCaptureIters = cap.schedule;
-    const int capWant = hooks ? 0 : std::min<int>(cap.want, (int)kCaptureIters.size());
+    const int capWant = (hooks || handoff) ? 0 : std::min<int>(cap.want, (int)kCaptureIters.size());
     if (capWant > 0) {
       cap.NN = NN;
       cap.data.ensure(sizeof(float) * 3 * (size_t)NN * capWant);
@@ -819,7 +838,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
       int it = 0;
       while (hst->bodies_done + hst->small_bodies < nb) {
         if (it > iterCap) throw std::runtime_error("conjugate residual did not terminate");
-        int batch = it == 0 ? std::max(1, c.iterHint) : 8;
+        int batch = it == 0 ? std::max(1, handoff ? c.iterHintHandoff : c.iterHint) : 8;
         for (int k = 0; k < batch; ++k, ++it) {
           iterate_x(it);
           iterate_rest(it);
@@ -882,12 +901,17 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     r.residual_norm2 = hsc[bi].rnorm2;
     r.active = hsc[bi].active_nodes;
     r.fallbacks = learned ? hsc[bi].fallbacks : 0;
-    r.restarts = learned ? 0 : hsc[bi].restarts;
+    r.restarts = hsc[bi].restarts;
     r.history.clear();
     if (!r.small) maxIt = std::max(maxIt, r.iterations);
     raise_errors(hsc[bi].errors);
   }
-  if (maxIt > 0) (learned ? c.iterHintFcr : c.iterHint) = maxIt;
+  if (handoff) {
+    c.iterHintFcr = std::max(1, iterAtHandoff);
+    c.iterHintHandoff = std::max(1, maxIt - iterAtHandoff);
+  } else if (maxIt > 0) {
+    (learned ? c.iterHintFcr : c.iterHint) = maxIt;
+  }
 }
 
 // Train the learned preconditioner on the residuals captured in this solve, before the particle
@@ -1748,6 +1772,13 @@ int mpm_lpc_set_directions(mpm_context* ctx, int m) {
   return guarded(ctx, [&] {
     if (m < 1 || m > kFcrMax) throw InvalidArg("the flexible CR keeps 1 to 8 directions");
     ctx->lpcDirs = m;
+  });
+}
+
+int mpm_lpc_set_handoff(mpm_context* ctx, int on) {
+  return guarded(ctx, [&] {
+    if (on != 0 && on != 1) throw InvalidArg("handoff must be 0 or 1");
+    ctx->lpcHandoff = on;
   });
 }
 
