@@ -193,6 +193,27 @@ def lpc_forward_flops(sim):
     return sum(2.0 * vox[lv] * 27 * ci * co for lv, ci, co in layers)
 
 
+def leg_marker(torch):
+    """One small torch kernel just outside each leg's timed region (before and after it): nothing
+    else in a slab run launches a torch kernel, so the rocprofv3 summaries find the learned leg's
+    dispatches between the first two (`learned_window`)."""
+    torch.arange(4, device="cuda")
+
+
+def is_marker(name):
+    return "at::" in name and not any(k in name for k in ("FillFunctor", "reduce_kernel", "copy", "Copy"))
+
+
+def learned_window(rows):
+    """(start, end) timestamps of the learned leg from (kernel name, start timestamp) pairs: the
+    first two leg markers, else the first and last network dispatch (runs without markers)."""
+    marks = sorted({t for n, t in rows if is_marker(n)})  # counter files hold a row per counter
+    if len(marks) >= 2:
+        return marks[0], marks[1]
+    net = [t for n, t in rows if "k_conv" in n]
+    return (min(net), max(net)) if net else None
+
+
 def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
     """Learned flexible CR and plain diagonal CR on the slope slab, --steps each, from one warmed-up
     state; HVP and network launches timed with HIP events."""
@@ -219,6 +240,7 @@ def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
         sim.ctx.lpc_mode(mode)
         sim.profile(["hvp", "lpc_forward"], period=HVP_SAMPLE)
         its, tot, fbs = [], [], []
+        leg_marker(torch)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -230,6 +252,8 @@ def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
             tot.append(st["total_iterations"])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        leg_marker(torch)
+        torch.cuda.synchronize()
         prof = sim.profile_query()
         sim.profile(False)
         hv = prof["hvp"]
@@ -399,12 +423,11 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
         kib = full[len(full) // 2]
         res[counter] = {"copy_kib": copy_kib, "factor": factor, "hvp_kib_per_launch": kib,
                         "hvp_bytes_per_launch": kib * 1024.0 / factor, "launches": len(full)}
-        # every HVP dispatch of the learned leg (between its first and last network dispatch; the
-        # warm-up and the plain leg run no network): the population whose algorithmic bytes the
-        # bench line's roofline averages
-        net = [t for n, _, _, t in rows if "k_conv" in n]
-        if net:
-            lo, hi = min(net), max(net)
+        # every HVP dispatch of the learned leg (between the markers around its timed steps): the
+        # population whose algorithmic bytes the bench line's roofline averages
+        win = learned_window([(n, t) for n, _, _, t in rows])
+        if win:
+            lo, hi = win
             leg = [v for n, _, v, t in rows if "k_hvp" in n and lo <= t <= hi]
             if leg:
                 res[counter]["leg_launches"] = len(leg)
@@ -421,7 +444,7 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
         m = re.search(r'"roofline": (\{[^}]*\})', open(log).read())
         if m:
             algo = json.loads(m.group(1))["bytes_per_launch"]
-    summary = {"kernel": "k_hvp_pair<true>" if not leg else "k_hvp<true> (the flexible CR's HVP)",
+    summary = {"kernel": "k_hvp_pair<true>",
                "workload": WORKLOAD_NAMES[workload], "traffic_bytes_per_launch": traffic,
                "traffic_full_grid_bytes_per_launch": full_traffic, "algorithmic_bytes_per_launch": algo,
                "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"]}
@@ -452,9 +475,8 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
                                                           r["hvp_bytes_per_launch"]))
     lines += ["", "traffic (FETCH + WRITE) per full-grid `k_hvp_pair<true>` launch: %.4g bytes" % full_traffic]
     if leg:
-        lines += ["", "Every HVP dispatch of the learned leg (`k_hvp<true>`, the flexible CR's HVP, one workgroup per "
-                  "chunk; between the leg's first and last network dispatch; the population the bench line's "
-                  "roofline averages): %d dispatches, FETCH %.4g + WRITE %.4g = "
+        lines += ["", "Every HVP dispatch of the learned leg (between the markers around its timed steps; the "
+                  "population the bench line's roofline averages): %d dispatches, FETCH %.4g + WRITE %.4g = "
                   "**%.4g bytes per launch**." % (res["FETCH_SIZE"]["leg_launches"],
                                                   res["FETCH_SIZE"]["leg_bytes_per_launch"],
                                                   res["WRITE_SIZE"]["leg_bytes_per_launch"], traffic)]
@@ -484,26 +506,27 @@ def sq_summary(csv_paths, out_path, kernel="k_hvp_pair<true>", workload="slab"):
     vals = {}
     selection = "the last 40%% of the run's dispatches of that kernel"
     for path in csv_paths:
-        per, start, net = {}, {}, []
+        per, start, names = {}, {}, []
         with open(path) as fh:
             for r in csv.DictReader(fh):
-                if "k_conv" in r["Kernel_Name"] and r.get("Start_Timestamp"):
-                    net.append(int(r["Start_Timestamp"]))
+                if r.get("Start_Timestamp"):
+                    names.append((r["Kernel_Name"], int(r["Start_Timestamp"])))
                 if kernel not in r["Kernel_Name"]:
                     continue
                 d = int(r.get("Dispatch_Id") or r["Correlation_Id"])
                 per.setdefault(d, {})[r["Counter_Name"]] = float(r["Counter_Value"])
                 if r.get("Start_Timestamp"):
                     start[d] = int(r["Start_Timestamp"])
-        if net and start:
-            # the learned leg (between its first and last network dispatch), dispatches that found
+        win = learned_window(names) if start else None
+        if win:
+            # the learned leg (between the markers around its timed steps), dispatches that found
             # work: a launch after every body of the batch has converged lasts a few percent of a
             # full one (its waves exit at the done flag)
-            lo, hi = min(net), max(net)
+            lo, hi = win
             ids = [d for d in sorted(per) if lo <= start.get(d, -1) <= hi]
             top = max((per[d].get("GRBM_GUI_ACTIVE", 0.0) for d in ids), default=0.0)
             ids = [d for d in ids if per[d].get("GRBM_GUI_ACTIVE", 0.0) >= 0.25 * top]
-            selection = ("the learned leg's dispatches (between its first and last network dispatch) that found work "
+            selection = ("the learned leg's dispatches (between the markers around its timed steps) that found work "
                          "(GRBM_GUI_ACTIVE, in every pass, at least 25%% of the leg's longest)")
         else:
             ids = sorted(per)[len(per) * 3 // 5:]  # the steps after the warm-up
@@ -649,18 +672,17 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
     trace_csv = os.path.join(os.path.dirname(stats_csv), os.path.basename(stats_csv).replace("kernel_stats",
                                                                                             "kernel_trace"))
     if os.path.exists(trace_csv):
-        hv, net = [], []
+        hv, names = [], []
         with open(trace_csv) as fh:
             for r in csv.DictReader(fh):
                 nm = _short(r["Kernel_Name"])
                 be = (int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                names.append((r["Kernel_Name"], be[0]))
                 if nm.startswith("k_hvp"):
                     hv.append(be)
-                elif nm.startswith("k_conv"):
-                    net.append(be)
-        if net:
-            lo, hi = min(b for b, _ in net), max(e for _, e in net)
-            hv = [(b, e) for b, e in hv if lo <= b <= hi]
+        win = learned_window(names)
+        if win:
+            hv = [(b, e) for b, e in hv if win[0] <= b <= win[1]]
         else:
             hv.sort()
             hv = hv[-HVP_SAMPLE * bench["roofline"]["launches"]:]
@@ -689,8 +711,8 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
         "",
     ]
     if timed:
-        lines += ["HVP dispatches of the learned leg alone (%d, those between its first and last network "
-                  "dispatch in `run_kernel_trace.csv`): mean %.4f ms (rocprofv3) against the HIP events' %.4f ms "
+        lines += ["HVP dispatches of the learned leg alone (%d, those between the markers bench.py launches around "
+                  "its timed steps, in `run_kernel_trace.csv`): mean %.4f ms (rocprofv3) against the HIP events' %.4f ms "
                   "(ratio %.3f). The run's mean above also covers the warm-up, whose first step solves with the "
                   "particle store still in input order and runs to the iteration cap, and the plain leg." % (timed["launches"], timed["ms"], ev_ms,
                                                        timed["ms"] / ev_ms if ev_ms else float("nan")), ""]
@@ -815,8 +837,7 @@ def main():
         stiff_trace_summary(args.stiff_trace_summary[0], args.stiff_trace_summary[1], STIFF_STATS_FILE)
         return
     if args.sq_summary:
-        sq_summary(args.sq_summary, profile_file("sq", wl), kernel="k_hvp<true>" if wl == "slab" else "k_hvp_pair<true>",
-                   workload=wl)
+        sq_summary(args.sq_summary, profile_file("sq", wl), workload=wl)
         return
     if args.mfma_summary:
         mfma_summary(args.mfma_summary[0], args.mfma_summary[1], MFMA_FILE)

@@ -198,9 +198,10 @@ int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* 
  * a velocity correction inside a flexible conjugate-residual solve; it attaches where
  * ConjugateResiduals takes a preconditioner (ConjugateResiduals.h:20, ProceduralMatrix.h:16).
  * An iteration that used the network and did not lower |r|^2 below `ratio` times its previous
- * value (mpm_lpc_set_fallback; 0.95 by default) is a fallback: the body takes its next iteration
- * with the diagonal preconditioner, and after three fallbacks keeps the diagonal for the rest of
- * its solve; the termination test is unchanged. */
+ * value (mpm_lpc_set_fallback; 0.95 by default), or by less than the body's last iteration on
+ * the diagonal preconditioner did, is a fallback: the body takes its next iteration with the
+ * diagonal preconditioner, and after three fallbacks keeps the diagonal for the rest of its
+ * solve; the termination test is unchanged. */
 int64_t mpm_lpc_num_params(void);
 /* random (He) initialisation; the output layer starts at zero */
 int mpm_lpc_init(mpm_context* ctx, uint64_t seed);

@@ -18,8 +18,9 @@
 // Particle-side kernels run one workgroup per chunk (up to 256 particles whose
 // stencil base node lies in one tile). A particle in tile t only touches nodes
 // in tiles t + {0,1}^3, so the workgroup stages that 8-tile (512-node)
-// footprint in LDS, accumulates its scatter there with ds_add_f32, and writes
-// the footprint to a per-chunk staging slot with plain coalesced stores. Node
+// footprint in LDS, accumulates its scatter there in fixed point with integer
+// LDS atomics (float LDS atomics run ~45x slower per wave-op), and writes the
+// footprint to a per-chunk staging slot with plain coalesced stores. Node
 // kernels gather, for each node, the staged footprints of the chunks of the
 // (up to 8) tiles whose footprint covers it. No device-scope float atomics are
 // used: on MI355X those resolve beyond the per-XCD L2s and were the bottleneck.
@@ -160,6 +161,7 @@ struct CRScalars {
   double rAp;              // <r, A p> of the new direction
   double rr_prev;          // |r|^2 before the last update
   double lpc_ratio;        // a network iteration whose |r|^2 exceeds lpc_ratio * rr_prev is a fallback
+  double diag_rate;        // |r|^2 / rr_prev of the body's last diagonal iteration (lpc_ratio before one)
   double lpc_s2;           // sum over active nodes of (r / diag)^2, and their count (3 per node)
   double lpc_cnt;
   float lpc_scale;         // RMS of r / diag over active nodes: network input scale

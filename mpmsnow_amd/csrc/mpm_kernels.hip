@@ -2545,9 +2545,10 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
 // ConjugateResiduals.cpp (same Krylov space, same minimised norm). The same termination
 // (|r|^2 < tol^2 |b|^2 or the iteration cap, SquareMagnitudeTermination.cpp:19-28) judges
 // the result. Whenever an iteration that used the network fails to lower |r|^2 by the body's
-// fallback ratio (the context's setting: 0.95 by default, 1 = any rise), the body takes its
-// next iteration with the diagonal preconditioner; after kLpcStrikes such fallbacks it keeps
-// the diagonal for the rest of the solve.
+// fallback ratio (the context's setting: 0.95 by default, 1 = any rise) or by as much as the
+// body's last diagonal iteration did, the body takes its next iteration with the diagonal
+// preconditioner; after kLpcStrikes such fallbacks it keeps the diagonal for the rest of the
+// solve.
 
 // r = b - A x0 ; network input scale ; iteration state
 __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
@@ -2580,6 +2581,7 @@ __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, cons
     S.rnorm2 = tot[0];
     S.rr_prev = tot[0];
     S.lpc_ratio = ratio;
+    S.diag_rate = ratio;
     S.lpc_s2 = tot[1];
     S.lpc_cnt = tot[2];
     S.lpc_scale = tot[2] > 0.0 ? (float)sqrt(tot[1] / tot[2]) : 0.0f;
@@ -2738,9 +2740,13 @@ __global__ void k_fcr_update(Batch B, const int8_t* flags, const ColliderSet* co
     S.lpc_s2 = tot[1];
     S.lpc_cnt = tot[2];
     S.lpc_scale = tot[2] > 0.0 ? (float)sqrt(tot[1] / tot[2]) : 0.0f;
+    // a network iteration must lower |r|^2 at least as much as the body's last diagonal
+    // iteration did (and by the fallback ratio)
+    const double bar = fmin(S.lpc_ratio, S.diag_rate);
+    if (S.lpc_off && S.rr_prev > 0.0) S.diag_rate = tot[0] / S.rr_prev;
     if (it >= S.max_iters || tot[0] < S.threshold) {
       set_done(S, bst);
-    } else if (!S.lpc_off && tot[0] > S.lpc_ratio * S.rr_prev) {
+    } else if (!S.lpc_off && tot[0] > bar * S.rr_prev) {
       S.lpc_off = 1;
       S.fallbacks += 1;
       S.lpc_pause = S.fallbacks < kLpcStrikes ? 1 : 0;
