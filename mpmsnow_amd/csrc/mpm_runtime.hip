@@ -752,10 +752,12 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     // stream, while the batched loop below iterates the rest
     int nsmall = 0;
     // (the iteration cap bounds the persistent kernel's run time)
-    if (!hooks && !handoff && c.cap.want == 0 && c.smallMaxParticles > 0 && maxIters <= (1 << 16)) {
+    // (after a handoff: the small bodies still iterating, whose solves restart from their iterate;
+    // hsc holds the scalars read at the handoff)
+    if (!hooks && (handoff || c.cap.want == 0) && c.smallMaxParticles > 0 && maxIters <= (1 << 16)) {
       int* list = reinterpret_cast<int*>(reinterpret_cast<char*>(c.pinned.p) + scb + 4096);
       for (int bi = 0; bi < nb; ++bi) {
-        rec[bi].small = smallLayout[bi] != 0;
+        rec[bi].small = smallLayout[bi] != 0 && !(handoff && hsc[bi].done);
         if (rec[bi].small) list[nsmall++] = bi;
       }
       if (nsmall > 0) {
@@ -779,7 +781,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     // CR setup (ConjugateResiduals.cpp:50-76); the HVPs of the solve run over chunk pairs. After
     // a handoff it starts from the flexible CR's iterate, r = b - A x recomputed; the bodies'
     // iteration indices, histories and caps continue
-    if (!handoff) pair_chunks(Bb);
+    if (!handoff || nsmall > 0) pair_chunks(Bb);
     hvp(Bb, f(x), true, f(stageA), sc(), kProfSetup, true);
     launch_cr_setup1(Bb, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(z), f(p), sc(),
                      s);
