@@ -39,19 +39,32 @@ __device__ __forceinline__ int lv_tile(const LevelGrid& G, int tx, int ty, int t
 __device__ __forceinline__ bool lv_real(const LevelGrid& G, int lt) { return lt < G.T[0] * G.T[1] * G.T[2]; }
 
 // Stage the 6x6x6-voxel input halo of output tile (tx, ty, tz) (zeros outside the body).
+// Every load of a thread is issued before its first LDS store, so the thread waits for global
+// memory once rather than once per 16-byte piece.
 template <int CIN>
 __device__ __forceinline__ void stage_halo(const bf16* in, const int* nb, bf16* halo) {
   constexpr int CG = CIN / 8;
-  for (int q = threadIdx.x; q < 216 * CG; q += kThreads) {
-    const int hv = q / CG, cg = q - (q / CG) * CG;
-    const int hx = hv % 6, hy = (hv / 6) % 6, hz = hv / 36;
-    const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
-                  9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-    const int lv = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
-    const int t = nb[o];
-    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (t >= 0) v = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + cg * 8);
-    *reinterpret_cast<uint4*>(halo + hv * CIN + cg * 8) = v;
+  constexpr int TOT = 216 * CG;
+  constexpr int IT = (TOT + kThreads - 1) / kThreads;
+  uint4 v[IT];
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int q = (int)threadIdx.x + k * kThreads;
+    v[k] = make_uint4(0u, 0u, 0u, 0u);
+    if (q < TOT) {
+      const int hv = q / CG, cg = q - (q / CG) * CG;
+      const int hx = hv % 6, hy = (hv / 6) % 6, hz = hv / 36;
+      const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                    9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+      const int lv = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
+      const int t = nb[o];
+      if (t >= 0) v[k] = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + cg * 8);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int q = (int)threadIdx.x + k * kThreads;
+    if (q < TOT) *reinterpret_cast<uint4*>(halo + q * 8) = v[k];
   }
 }
 // neighbour tiles of an output tile, index (dx + 1) + 3 (dy + 1) + 9 (dz + 1)
@@ -92,6 +105,32 @@ __global__ __launch_bounds__(kThreads) void k_conv(LevelView V, const bf16* __re
   floatx4 acc[NT16];
 #pragma unroll
   for (int t = 0; t < NT16; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  // the epilogue's other inputs (skip activations; residual and diagonal of the output layer),
+  // requested before the K loop so that their latency overlaps it
+  float pre[NT16][4];
+  float pre2[4];
+#pragma unroll
+  for (int t = 0; t < NT16; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pre[t][v] = 0.0f;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) pre2[v] = 1.0f;
+  if (MODE == CONV_FINAL) {
+    if (row < 3 && o.z) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t zi = ((int64_t)tile * 3 + row) * 64 + 16 * w + 4 * q + v;
+        pre[0][v] = o.r[zi];
+        pre2[v] = o.diag[zi];
+      }
+    }
+  } else if (o.out2) {
+#pragma unroll
+    for (int t = 0; t < NT16; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        pre[t][v] = (float)o.skip[((int64_t)tile * 64 + 16 * w + 4 * q + v) * COUT + 16 * t + row];
+  }
   for (int c = 0; c < NCH; ++c) {
     const int f = 32 * c + 8 * q;
     bf16x8 a;
@@ -134,12 +173,12 @@ __global__ __launch_bounds__(kThreads) void k_conv(LevelView V, const bf16* __re
         if (co < 3) {
           const int64_t zi = ((int64_t)tile * 3 + co) * 64 + vox;
           if (o.raw) o.raw[zi] = val;
-          if (o.z) o.z[zi] = o.r[zi] / o.diag[zi] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * o.sc[b].lpc_scale;
+          if (o.z) o.z[zi] = pre[0][v] / pre2[v] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * o.sc[b].lpc_scale;
         }
       } else {
         const bf16 hv = (bf16)val;
         o.out[e] = hv;
-        if (o.out2) o.out2[e] = (bf16)((float)hv + (float)o.skip[e]);
+        if (o.out2) o.out2[e] = (bf16)((float)hv + pre[t][v]);
       }
     }
   }

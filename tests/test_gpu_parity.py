@@ -241,12 +241,18 @@ def implicit_pair(oracle, fv=(0, 0, 0)):
     n = len(pos)
     P = Pair(oracle, np.array(pos, np.float32), np.array(vel, np.float32), np.ones(n, np.float32),
              F=np.array(Fs), E=1.4e5, nu=0.2, xi=0, thc=1e5, ths=1e5, vol=np.ones(n, np.float32), update=False)
-    planes = [((-1, -1, -1, 1), (-0.1, -0.2, -0.3), 0.5, False), ((-1, 0, 0, 1.5), (-0.2, -0.1, -0.4), 0.5, False)]
-    P.ctx.set_colliders(planes)
-    cols = oracle.Colliders.create(F64)
-    for c, v, mu, st in planes:
+    P.ctx.set_colliders(IMPLICIT_PLANES)
+    return P, h, implicit_colliders(oracle, F64)
+
+
+IMPLICIT_PLANES = [((-1, -1, -1, 1), (-0.1, -0.2, -0.3), 0.5, False), ((-1, 0, 0, 1.5), (-0.2, -0.1, -0.4), 0.5, False)]
+
+
+def implicit_colliders(oracle, prec):
+    cols = oracle.Colliders.create(prec)
+    for c, v, mu, st in IMPLICIT_PLANES:
         cols.add_plane(c, v=v, friction=mu, sticky=st)
-    return P, h, cols
+    return cols
 
 
 def test_explicit_momenta_and_collisions(oracle):
@@ -285,11 +291,19 @@ def test_implicit_solve_against_oracle(oracle):
     vo = og.get("velocities")
     assert relerr(v, vo) < 1e-4
     assert abs(st["bnorm2"] - sto["bnorm2"]) < REL * sto["bnorm2"]
-    # residual histories agree while they are above the FP32 floor
-    rh, rho = st["residual_norm2"], sto["residual_norm2"]
+    # residual histories: every entry above the FP32 floor within 1e-5 of FP64, or within 3x the
+    # FP32 oracle's own deviation where the recurrence has amplified rounding beyond that
+    cols32 = implicit_colliders(oracle, F32)
+    og32 = oracle.Grid(P.o[F32], inds, h)
+    og32.compute_particle_volumes()
+    st32 = og32.update_grid_velocities(0.002, P.om[F32], cols32, oracle.Fields.create(F32), 40, 0.0)
+    rh, rho, rh32 = st["residual_norm2"], sto["residual_norm2"], st32["residual_norm2"]
     k = np.nonzero(rho > 1e-8 * rho[0])[0]
-    k = k[k < 10]
-    assert np.all(np.abs(rh[k] - rho[k]) <= 1e-3 * rho[k])
+    k = k[k < min(len(rh), len(rh32))]
+    dev, dev32 = np.abs(rh[k] - rho[k]) / rho[k], np.abs(rh32[k] - rho[k]) / rho[k]
+    say("history vs FP64 over %d entries: gpu max %.2e, fp32 oracle max %.2e" % (len(k), dev.max(), dev32.max()))
+    assert len(k) >= 10
+    assert np.all(dev <= np.maximum(REL, 3.0 * dev32)), (dev, dev32)
 
 
 def test_implicit_solve_tolerance_terminates_like_oracle(oracle):
