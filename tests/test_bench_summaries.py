@@ -94,3 +94,17 @@ def test_pmc_summary_learned_leg_population(tmp_path):
     assert s["fetch"]["leg_launches"] == 4
     assert abs(s["fetch"]["leg_bytes_per_launch"] - 2000.0 * 1024) < 1e-6
     assert abs(s["traffic_bytes_per_launch"] - 4000.0 * 1024) < 1e-6
+
+
+def test_learned_window_from_markers():
+    """The learned leg lies between the first two leg markers (torch kernels bench.py launches
+    around each leg's timed steps); counter files repeat a dispatch once per counter, and the
+    calibration kernels are not markers. Without markers the network's dispatches bound it."""
+    arange = "void (anonymous namespace)::elementwise_kernel_with_index<int, at::native::arange_cuda_out>(x)"
+    rows = [("void at::native::vectorized_elementwise_kernel<4, at::native::FillFunctor<float>>(x)", 1),
+            ("void at::native::reduce_kernel<512, 1>(x)", 2),
+            (arange, 10), (arange, 10), ("void mpm::k_hvp_pair<true>(x)", 11), ("void mpm::k_conv<16, 16, 0>(x)", 12),
+            (arange, 20), (arange, 20), (arange, 30), ("void mpm::k_hvp_pair<true>(x)", 31), (arange, 40)]
+    assert bench.learned_window(rows) == (10, 20)
+    assert bench.learned_window([r for r in rows if "arange" not in r[0]]) == (12, 12)
+    assert bench.learned_window([("void mpm::k_hvp_pair<true>(x)", 5)]) is None

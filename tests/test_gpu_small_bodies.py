@@ -114,3 +114,42 @@ def test_fragmenting_collision_small_solver_bitwise():
     assert ia == ib
     for name in ("p", "v", "F"):
         assert np.array_equal(a.get(name), b.get(name)), name
+
+
+def test_learned_handoff_small_solver_bitwise():
+    """Learned mode with a network that every body abandons within a few iterations: at the
+    handoff to the CR recurrences the small bodies still iterating go to the one-workgroup solver,
+    whose restarted solves must be bitwise those of the batched loop (solver off)."""
+    from oracle import lpc_oracle as L
+    from test_gpu_lpc import random_params
+
+    p = random_params(13)
+    offs, _ = L.layer_offsets()
+    w, b = offs[5]
+    p[w:w + 27 * 16 * 16] *= 50.0
+
+    def run_learned(limit):
+        x, v, m, h = cluster_scene()
+        planes = [((0, 1, 0, 0), (0, 0, 0), 0.2, False)]
+        sim = G.Sim(x, m, h, material=dict(scenes.SNOW), planes=planes, gravity=[(0.0, -9.81, 0.0)], v=v)
+        sim.ctx.small_bodies(limit)
+        sim.ctx.lpc_set_params(p)
+        sim.ctx.lpc_mode(1)
+        sim.profile(["cr_small"])
+        stats = []
+        for _ in range(3):
+            st = sim.advance(1e-3, 1000, 1e-5)
+            stats.append([sim.body_stats(b) for b in range(st["num_solved"])])
+        return sim, stats, sim.profile_query()["cr_small"]["launches"]
+
+    say("clusters, learned mode with a bad network, small-body solver off / on")
+    a, sa, la = run_learned(0)
+    b, sb, lb = run_learned(2048)
+    say("small-solver launches off %d, on %d; bodies per step %s" % (la, lb, [len(s) for s in sb]))
+    assert la == 0 and lb > 0
+    for ra, rb in zip(sa, sb):
+        assert [(r["iterations"], r["fallbacks"]) for r in ra] == [(r["iterations"], r["fallbacks"]) for r in rb]
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x["residual_norm2"], y["residual_norm2"])
+    for name in ("p", "v", "F"):
+        assert np.array_equal(a.get(name), b.get(name)), name
