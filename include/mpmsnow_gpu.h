@@ -238,6 +238,53 @@ int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float 
 int mpm_grid_lpc_gradients(mpm_grid* grid, const float* r, const float* diag, const float* e, float scale,
                            float weight, float* grad, double* loss);
 
+/* ---- full-FP64 step (precision studies) ------------------------------------------
+ * Sim (reference include/MpmSim/Sim.h:32-55; Sim.cpp:17-50 and 57-134) with every scalar in
+ * double on the GPU: particle state, grid quantities, shape weights, stresses, the
+ * conjugate-residual solve and its reductions. It has no counterpart in the reference, whose
+ * types are single precision; it is the full-FP64 run that the FP32-bulk step above is measured
+ * against, on the same GPU at full size. Semantics are those of mpm_sim_*: bodies by the flood
+ * fill |x_i - x_j|^2 < h^2 (Sim.cpp:136-203), one dense grid per body, the diagonal
+ * preconditioned CR of ConjugateResiduals.cpp:19-160 with SquareMagnitudeTermination.
+ * Sums are FP64 atomic adds: results agree run to run to the rounding of double, not bitwise.
+ * Arrays are in the reference's host layouts, in double: Vector3 = 3 doubles, Matrix3 = 9
+ * doubles column-major. */
+typedef struct mpm_sim64 mpm_sim64;
+/* collision object as mpm_collider, in double */
+typedef struct {
+  int type;
+  double coeffs[4];
+  double center[3];
+  double radius;
+  double velocity[3];
+  double friction;
+  int sticky;
+  int dims[3];
+  double origin[3];
+  double voxel;
+  const double* phi;
+  const double* phi_velocity;
+} mpm_collider64;
+/* n particles: v = 0, F = I, volume 0 (MaterialPointData, MaterialPointData.h:14-44) */
+int mpm_sim64_create(mpm_sim64** sim, int device, int64_t n);
+int mpm_sim64_destroy(mpm_sim64* sim);
+const char* mpm_sim64_last_error(const mpm_sim64* sim);
+/* the variables of mpm_particles_set / mpm_particles_get, in double */
+int mpm_sim64_set(mpm_sim64* sim, const char* name, const double* data, int64_t count);
+int mpm_sim64_get(mpm_sim64* sim, const char* name, double* data, int64_t count);
+/* SnowConstitutiveModel(E, nu, xi, theta_c, theta_s) + setParticles (SnowConstitutiveModel.cpp:9-25,
+ * 123-169); per-particle mu0, lambda0 and xi may then be set by name */
+int mpm_sim64_model_snow(mpm_sim64* sim, double youngs_modulus, double poisson_ratio, double hardening,
+                         double theta_c, double theta_s);
+int mpm_sim64_set_colliders(mpm_sim64* sim, const mpm_collider64* colliders, int n);
+int mpm_sim64_set_gravity(mpm_sim64* sim, const double* gravity, int n);
+/* Sim::Sim after the particles and model are set (Sim.cpp:17-50): bodies + volumes */
+int mpm_sim64_init(mpm_sim64* sim, double grid_size);
+/* Sim::advance (Sim.cpp:57-134) with SquareMagnitudeTermination(max_iters, tolerance) */
+int mpm_sim64_advance(mpm_sim64* sim, double dt, int max_iters, double tolerance, mpm_step_stats* stats);
+int mpm_sim64_num_bodies(const mpm_sim64* sim);
+int mpm_sim64_body_solve_stats(const mpm_sim64* sim, int body, mpm_solve_stats* stats);
+
 /* ---- small bodies (no counterpart in the reference) -----------------------------
  * Bodies of at most 8 node blocks (2048 grid nodes) and max_particles particles
  * run their whole conjugate residual solve in one workgroup each, on a second

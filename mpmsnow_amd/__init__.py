@@ -36,6 +36,12 @@ class Collider(C.Structure):
                 ("origin", C.c_float * 3), ("voxel", C.c_float), ("phi", _fp), ("phi_velocity", _fp)]
 
 
+class Collider64(C.Structure):
+    _fields_ = [("type", C.c_int), ("coeffs", C.c_double * 4), ("center", C.c_double * 3), ("radius", C.c_double),
+                ("velocity", C.c_double * 3), ("friction", C.c_double), ("sticky", C.c_int), ("dims", C.c_int * 3),
+                ("origin", C.c_double * 3), ("voxel", C.c_double), ("phi", _dp), ("phi_velocity", _dp)]
+
+
 class SolverParams(C.Structure):
     _fields_ = [("max_iters", C.c_int), ("tolerance", C.c_float), ("termination", TERM_CB), ("debug", DEBUG_CB),
                 ("cancelled", CANCEL_CB), ("user", C.c_void_p)]
@@ -117,6 +123,18 @@ SIGNATURES = {
     "mpm_grid_lpc_apply": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, _fp]),
     "mpm_grid_lpc_gradients": (C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_float, C.c_float, _fp,
                                          C.POINTER(C.c_double)]),
+    "mpm_sim64_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int64]),
+    "mpm_sim64_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_sim64_last_error": (C.c_char_p, [C.c_void_p]),
+    "mpm_sim64_set": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.c_int64]),
+    "mpm_sim64_get": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.c_int64]),
+    "mpm_sim64_model_snow": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
+    "mpm_sim64_set_colliders": (C.c_int, [C.c_void_p, C.POINTER(Collider64), C.c_int]),
+    "mpm_sim64_set_gravity": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "mpm_sim64_init": (C.c_int, [C.c_void_p, C.c_double]),
+    "mpm_sim64_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double, C.POINTER(StepStats)]),
+    "mpm_sim64_num_bodies": (C.c_int, [C.c_void_p]),
+    "mpm_sim64_body_solve_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolveStats)]),
     "mpm_set_small_bodies": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_profile_period": (C.c_int, [C.c_void_p, C.c_int]),
@@ -570,3 +588,104 @@ class Sim:
             self.ctx.check(lib().mpm_profile_query(self.ctx.ptr, k.encode(), C.byref(ms), C.byref(n), C.byref(by)))
             out[k] = {"ms": ms.value, "launches": n.value, "bytes": by.value}
         return out
+
+
+class Sim64:
+    """The full-FP64 step on the GPU: Sim (include/MpmSim/Sim.h) with every scalar in double, the FP64
+    run that precision studies compare the FP32-bulk Sim against. Same constructor arguments as Sim
+    (colliders: planes and spheres); arrays in and out are float64."""
+
+    def __init__(self, x, m, h, material, planes=(), spheres=(), gravity=(), device=0, mu0=None, lambda0=None,
+                 xi=None, v=None):
+        x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+        self.n = len(x)
+        ptr = C.c_void_p()
+        rc = lib().mpm_sim64_create(C.byref(ptr), device, self.n)
+        if rc != MPM_OK:
+            raise MpmError("mpm_sim64_create failed (code %d)" % rc)
+        self.ptr = ptr.value
+        self.set("p", x)
+        self.set("m", m)
+        if v is not None:
+            self.set("v", v)
+        mat = dict(material)
+        self.check(lib().mpm_sim64_model_snow(self.ptr, float(mat["E"]), float(mat["nu"]), float(mat["xi"]),
+                                              float(mat["theta_c"]), float(mat["theta_s"])))
+        if mu0 is not None:
+            self.set("mu0", mu0)
+            self.set("lambda0", lambda0)
+            self.set("xi", xi)
+        items = []
+        for coeffs, vel, mu, sticky in planes:
+            c = Collider64()
+            c.type = 0
+            c.coeffs[:] = list(map(float, coeffs))
+            c.velocity[:] = list(map(float, vel))
+            c.friction = float(mu)
+            c.sticky = int(sticky)
+            items.append(c)
+        for center, r, vel, mu, sticky in spheres:
+            c = Collider64()
+            c.type = 1
+            c.center[:] = list(map(float, center))
+            c.radius = float(r)
+            c.velocity[:] = list(map(float, vel))
+            c.friction = float(mu)
+            c.sticky = int(sticky)
+            items.append(c)
+        arr = (Collider64 * max(1, len(items)))(*items)
+        self.check(lib().mpm_sim64_set_colliders(self.ptr, arr, len(items)))
+        if len(gravity):
+            g = np.ascontiguousarray(np.asarray(gravity, dtype=np.float64).reshape(-1, 3))
+            self.check(lib().mpm_sim64_set_gravity(self.ptr, g.ctypes.data_as(_dp), len(g)))
+        self.check(lib().mpm_sim64_init(self.ptr, float(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.mpm_sim64_destroy(self.ptr)
+            self.ptr = None
+
+    __del__ = close
+
+    def check(self, rc):
+        if rc != MPM_OK:
+            raise MpmError(lib().mpm_sim64_last_error(self.ptr).decode())
+
+    def set(self, name, arr):
+        if name in MAT3:
+            flat = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(-1, 3, 3).transpose(0, 2, 1))
+        else:
+            flat = np.ascontiguousarray(np.asarray(arr, dtype=np.float64))
+        flat = flat.reshape(-1)
+        self.check(lib().mpm_sim64_set(self.ptr, name.encode(), flat.ctypes.data_as(_dp), flat.size))
+
+    def get(self, name):
+        w = 9 if name in MAT3 else 3 if name in VEC3 else 1
+        out = np.zeros(self.n * w, dtype=np.float64)
+        self.check(lib().mpm_sim64_get(self.ptr, name.encode(), out.ctypes.data_as(_dp), out.size))
+        if name in MAT3:
+            return out.reshape(-1, 3, 3).transpose(0, 2, 1).copy()
+        if name in VEC3:
+            return out.reshape(self.n, 3)
+        return out
+
+    def advance(self, dt, max_iters, tol):
+        st = StepStats()
+        self.check(lib().mpm_sim64_advance(self.ptr, float(dt), int(max_iters), float(tol), C.byref(st)))
+        return {"num_bodies": st.num_bodies, "num_ballistic": st.num_ballistic,
+                "total_iterations": st.total_iterations, "max_iterations": st.max_iterations,
+                "num_solved": st.num_solved}
+
+    def num_bodies(self):
+        return lib().mpm_sim64_num_bodies(self.ptr)
+
+    def body_stats(self, b, hist_cap=4096):
+        hist = np.zeros(hist_cap) if hist_cap > 0 else None
+        st = SolveStats()
+        st.history_capacity = hist_cap
+        st.history = hist.ctypes.data_as(_dp) if hist is not None else None
+        rc = lib().mpm_sim64_body_solve_stats(self.ptr, b, C.byref(st))
+        if rc != MPM_OK:
+            raise MpmError("no solve statistics for body %d" % b)
+        return _stats_dict(st, hist)

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libmpmsnow_hip.so")
-SOURCES = ["mpm_kernels.hip", "lpc_kernels.hip", "mpm_runtime.hip"]
+SOURCES = ["mpm_kernels.hip", "lpc_kernels.hip", "mpm_runtime.hip", "mpm_fp64.hip"]
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -19,44 +19,52 @@ namespace mpm {
 
 constexpr int kMaxColliders = 16;
 
+// Scalar helpers for the two precisions the step is built in: float (the library's FP32 bulk)
+// and double (the full-FP64 step of the precision study). Every templated function below
+// evaluates, for T = float, exactly the float expression it did as a float-only function.
 MPM_HD float msqrt(float x) { return sqrtf(x); }
 MPM_HD double msqrt(double x) { return sqrt(x); }
 MPM_HD float mabs(float x) { return fabsf(x); }
 MPM_HD double mabs(double x) { return fabs(x); }
+MPM_HD float mfloor(float x) { return floorf(x); }
+MPM_HD double mfloor(double x) { return floor(x); }
+MPM_HD float mexp(float x) { return expf(x); }
+MPM_HD double mexp(double x) { return exp(x); }
 
 // ------------------------------------------------------------ B-spline
-MPM_HD float bspline_w(float x) {
-  float ax = fabsf(x);
-  if (ax < 1.0f) return (0.5f * ax - 1.0f) * ax * ax + 2.0f / 3.0f;
-  if (ax < 2.0f) return ((-ax / 6.0f + 1.0f) * ax - 2.0f) * ax + 4.0f / 3.0f;
-  return 0.0f;
+template <class T> MPM_HD T bspline_w(T x) {
+  T ax = mabs(x);
+  if (ax < T(1)) return (T(0.5) * ax - T(1)) * ax * ax + T(2) / T(3);
+  if (ax < T(2)) return ((-ax / T(6) + T(1)) * ax - T(2)) * ax + T(4) / T(3);
+  return T(0);
 }
-MPM_HD float bspline_dw(float x) {
-  float s = 1.0f;
-  if (x < 0.0f) { x = -x; s = -1.0f; }
-  if (x < 1.0f) return s * (x * (1.5f * x - 2.0f));
-  if (x < 2.0f) { x -= 2.0f; return s * (-0.5f * x * x); }
-  return 0.0f;
+template <class T> MPM_HD T bspline_dw(T x) {
+  T s = T(1);
+  if (x < T(0)) { x = -x; s = T(-1); }
+  if (x < T(1)) return s * (x * (T(1.5) * x - T(2)));
+  if (x < T(2)) { x -= T(2); return s * (T(-0.5) * x * x); }
+  return T(0);
 }
 
 // Per-particle stencil: base node and the 4 weights (and derivatives) per axis.
-struct Stencil {
+template <class T> struct StencilT {
   int base[3];
-  float w[3][4];
-  float dw[3][4];  // already divided by h
+  T w[3][4];
+  T dw[3][4];  // already divided by h
 };
+using Stencil = StencilT<float>;
 
 // Grid.cpp:1105-1136 for dim = 3: base = floor((p - min)/h) - 1; weights at j - frac, j = -1..2.
-MPM_HD void make_stencil(const float p[3], const float gmin[3], float h, bool derivs, Stencil& s) {
+template <class T> MPM_HD void make_stencil(const T p[3], const T gmin[3], T h, bool derivs, StencilT<T>& s) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    float frac = (p[d] - gmin[d]) / h;
-    int dimPos = (int)floorf(frac);
-    frac -= (float)dimPos;
+    T frac = (p[d] - gmin[d]) / h;
+    int dimPos = (int)mfloor(frac);
+    frac -= (T)dimPos;
     s.base[d] = dimPos - 1;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      float t = (float)(i - 1) - frac;
+      T t = (T)(i - 1) - frac;
       s.w[d][i] = bspline_w(t);
       if (derivs) s.dw[d][i] = bspline_dw(t) / h;
     }
@@ -87,8 +95,8 @@ MPM_HD void make_stencil_fast(const float p[3], const float gmin[3], float h, fl
   }
 }
 // weight of stencil node (a,b,c): w_x * w_y * w_z in the reference's product order
-MPM_HD float st_w(const Stencil& s, int a, int b, int c) { return s.w[0][a] * s.w[1][b] * s.w[2][c]; }
-MPM_HD void st_grad(const Stencil& s, int a, int b, int c, float g[3]) {
+template <class T> MPM_HD T st_w(const StencilT<T>& s, int a, int b, int c) { return s.w[0][a] * s.w[1][b] * s.w[2][c]; }
+template <class T> MPM_HD void st_grad(const StencilT<T>& s, int a, int b, int c, T g[3]) {
   g[0] = -s.dw[0][a] * s.w[1][b] * s.w[2][c];
   g[1] = -s.w[0][a] * s.dw[1][b] * s.w[2][c];
   g[2] = -s.w[0][a] * s.w[1][b] * s.dw[2][c];
@@ -256,35 +264,40 @@ struct ParticleCache {  // the model's per-particle state (SnowConstitutiveModel
 };
 
 // updateParticleData for one particle (SnowConstitutiveModel.cpp:27-121).
-// FP32 Jacobi SVD; when sigma_min < 1e-3 the SVD is redone in FP64.
+// T = float: FP32 Jacobi SVD, redone in FP64 when sigma_min < 1e-3; T = double: FP64 SVD.
 // Returns false when the hardening factor is not finite ("infinite hardness!!!").
-MPM_HD bool snow_update(float F[9], float Fp[9], float Finv[9], float R[9], float Ginv[9], float& J, float& mu,
-                        float& lambda, float mu0, float lambda0, float xi, float thc, float ths) {
-  float U[9], V[9], sv[3];
-  svd3_jacobi<float, 5>(F, U, sv, V);
-  if (!(sv[2] >= 1e-3f)) {
-    double Fd[9], Ud[9], Vd[9], sd[3];
+template <class T>
+MPM_HD bool snow_update(T F[9], T Fp[9], T Finv[9], T R[9], T Ginv[9], T& J, T& mu, T& lambda, T mu0, T lambda0, T xi,
+                        T thc, T ths) {
+  T U[9], V[9], sv[3];
+  if constexpr (sizeof(T) == 4) {
+    svd3_jacobi<float, 5>(F, U, sv, V);
+    if (!(sv[2] >= 1e-3f)) {
+      double Fd[9], Ud[9], Vd[9], sd[3];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fd[i] = (double)F[i];
-    svd3_jacobi<double, 8>(Fd, Ud, sd, Vd);
+      for (int i = 0; i < 9; ++i) Fd[i] = (double)F[i];
+      svd3_jacobi<double, 8>(Fd, Ud, sd, Vd);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) { U[i] = (float)Ud[i]; V[i] = (float)Vd[i]; }
+      for (int i = 0; i < 9; ++i) { U[i] = (float)Ud[i]; V[i] = (float)Vd[i]; }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) sv[i] = (float)sd[i];
+      for (int i = 0; i < 3; ++i) sv[i] = (float)sd[i];
+    }
+  } else {
+    svd3_jacobi<double, 8>(F, U, sv, V);
   }
-  float d[3], di[3];
+  T d[3], di[3];
   bool modified = false;
-  const float hi = 1.0f + ths, lo = 1.0f - thc;
+  const T hi = T(1) + ths, lo = T(1) - thc;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    float x = sv[i];
+    T x = sv[i];
     if (x > hi) { modified = true; x = hi; }
     if (x < lo) { modified = true; x = lo; }
     d[i] = x;
-    di[i] = 1.0f / x;
+    di[i] = T(1) / x;
   }
   // U * diag(d): scale columns
-  float UD[9], UDi[9], VD[9];
+  T UD[9], UDi[9], VD[9];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -294,7 +307,7 @@ MPM_HD bool snow_update(float F[9], float Fp[9], float Finv[9], float R[9], floa
       VD[r + 3 * c] = V[r + 3 * c] * d[c];
     }
   if (modified) {
-    float FN[9], VDi[9], T1[9];
+    T FN[9], VDi[9], T1[9];
     mat_mul(F, Fp, FN);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -306,9 +319,9 @@ MPM_HD bool snow_update(float F[9], float Fp[9], float Finv[9], float R[9], floa
   }
   mat_mul_bt(UDi, V, Finv);  // U D^-1 V^T
   mat_mul_bt(U, V, R);       // U V^T
-  float S[9];
+  T S[9];
   mat_mul_bt(VD, V, S);      // V D V^T
-  float G[9];
+  T G[9];
   at(G, 0, 0) = at(S, 0, 0) + at(S, 1, 1);
   at(G, 1, 1) = at(S, 0, 0) + at(S, 2, 2);
   at(G, 2, 2) = at(S, 1, 1) + at(S, 2, 2);
@@ -317,157 +330,159 @@ MPM_HD bool snow_update(float F[9], float Fp[9], float Finv[9], float R[9], floa
   at(G, 1, 2) = at(G, 2, 1) = at(S, 0, 1);
   inv3(G, Ginv);
   J = d[0] * d[1] * d[2];
-  float hf = xi * (1.0f - det3(Fp));
-  if (hf > 2.0f) hf = 2.0f;
-  hf = expf(hf);
+  T hf = xi * (T(1) - det3(Fp));
+  if (hf > T(2)) hf = T(2);
+  hf = mexp(hf);
   mu = mu0 * hf;
   lambda = lambda0 * hf;
   return isfinite(hf);
 }
 
 // dEnergyDensitydF (SnowConstitutiveModel.cpp:187-216): 2 mu (F - R) + lambda (J-1) J F^-T
-MPM_HD void snow_stress(const float* F, const float* R, const float* Finv, float J, float mu, float lambda, float* P) {
-  float a = 2.0f * mu;
-  float b = lambda * (J - 1.0f) * J;
+template <class T> MPM_HD void snow_stress(const T* F, const T* R, const T* Finv, T J, T mu, T lambda, T* P) {
+  T a = T(2) * mu;
+  T b = lambda * (J - T(1)) * J;
 #pragma unroll
   for (int i = 0; i < 9; ++i) P[i] = a * (F[i] - R[i]) + b * Finv[i];
 }
 // energyDensity (SnowConstitutiveModel.cpp:171-185)
-MPM_HD float snow_energy(const float* F, const float* R, float J, float mu, float lambda) {
-  float rd[9];
+template <class T> MPM_HD T snow_energy(const T* F, const T* R, T J, T mu, T lambda) {
+  T rd[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) rd[i] = F[i] - R[i];
-  float jm1 = J - 1.0f;
-  return mu * ddot(rd, rd) + 0.5f * lambda * jm1 * jm1;
+  T jm1 = J - T(1);
+  return mu * ddot(rd, rd) + T(0.5) * lambda * jm1 * jm1;
 }
 // dEdFDifferential (SnowConstitutiveModel.cpp:218-256) with computeRdifferential (:266-287)
-MPM_HD void snow_stress_differential(const float* dF, const float* R, const float* Finv, const float* Ginv, float J,
-                                     float mu, float lambda, float* dP) {
-  float dJ = J * ddot(Finv, dF);
+template <class T>
+MPM_HD void snow_stress_differential(const T* dF, const T* R, const T* Finv, const T* Ginv, T J, T mu, T lambda, T* dP) {
+  T dJ = J * ddot(Finv, dF);
   // dFinvT = -F^-T dF^T F^-T
-  float T1[9], dFi[9];
+  T T1[9], dFi[9];
   mat_mul_bt(Finv, dF, T1);
   mat_mul(T1, Finv, dFi);
   // dR = R * skew(Ginv * w(R^T dF))
-  float M[9];
+  T M[9];
   mat_mul_at(R, dF, M);
-  float w0 = at(M, 0, 1) - at(M, 1, 0);
-  float w1 = at(M, 0, 2) - at(M, 2, 0);
-  float w2 = at(M, 1, 2) - at(M, 2, 1);
-  float g0 = at(Ginv, 0, 0) * w0 + at(Ginv, 0, 1) * w1 + at(Ginv, 0, 2) * w2;
-  float g1 = at(Ginv, 1, 0) * w0 + at(Ginv, 1, 1) * w1 + at(Ginv, 1, 2) * w2;
-  float g2 = at(Ginv, 2, 0) * w0 + at(Ginv, 2, 1) * w1 + at(Ginv, 2, 2) * w2;
+  T w0 = at(M, 0, 1) - at(M, 1, 0);
+  T w1 = at(M, 0, 2) - at(M, 2, 0);
+  T w2 = at(M, 1, 2) - at(M, 2, 1);
+  T g0 = at(Ginv, 0, 0) * w0 + at(Ginv, 0, 1) * w1 + at(Ginv, 0, 2) * w2;
+  T g1 = at(Ginv, 1, 0) * w0 + at(Ginv, 1, 1) * w1 + at(Ginv, 1, 2) * w2;
+  T g2 = at(Ginv, 2, 0) * w0 + at(Ginv, 2, 1) * w1 + at(Ginv, 2, 2) * w2;
   // R * [[0, g0, g1], [-g0, 0, g2], [-g1, -g2, 0]]
-  float dR[9];
+  T dR[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    float r0 = at(R, r, 0), r1 = at(R, r, 1), r2 = at(R, r, 2);
+    T r0 = at(R, r, 0), r1 = at(R, r, 1), r2 = at(R, r, 2);
     at(dR, r, 0) = -r1 * g0 - r2 * g1;
     at(dR, r, 1) = r0 * g0 - r2 * g2;
     at(dR, r, 2) = r0 * g1 + r1 * g2;
   }
-  float twoMu = 2.0f * mu;
-  float dJJ = dJ * J;
-  float jm1 = J - 1.0f;
+  T twoMu = T(2) * mu;
+  T dJJ = dJ * J;
+  T jm1 = J - T(1);
 #pragma unroll
   for (int i = 0; i < 9; ++i)
     dP[i] = twoMu * (dF[i] - dR[i]) + lambda * (dJJ * Finv[i] + jm1 * (dJ * Finv[i] - J * dFi[i]));
 }
 
 // ------------------------------------------------------------ colliders
-struct Collider {
+template <class T> struct ColliderT {
   int type;  // 0 plane, 1 sphere, 2 level-set volume
-  float c[4];
-  float center[3];
-  float radius;
-  float v[3];
-  float friction;
+  T c[4];
+  T center[3];
+  T radius;
+  T v[3];
+  T friction;
   int sticky;
   // type 2: phi at the points origin + (i, j, k) voxel (point (i, j, k) at i + dims0 (j + dims1 k)),
-  // and optionally a velocity per point (3 floats; null: the uniform v), in device memory
+  // and optionally a velocity per point (3 values; null: the uniform v), in device memory
   int dims[3];
-  float origin[3];
-  float voxel;
-  const float* phi;
-  const float* vel;
+  T origin[3];
+  T voxel;
+  const T* phi;
+  const T* vel;
 };
-struct ColliderSet {
+template <class T> struct ColliderSetT {
   int n;
-  Collider o[kMaxColliders];
+  ColliderT<T> o[kMaxColliders];
 };
+using Collider = ColliderT<float>;
+using ColliderSet = ColliderSetT<float>;
 
 // Collider tests decide the collided flag of a node, which must match the reference bit for bit:
-// they keep its separately rounded float products and sums (hipcc would otherwise fuse a*b + c
+// they keep its separately rounded products and sums (hipcc would otherwise fuse a*b + c
 // into one FMA, which rounds once).
 // Level-set volume sampling (the reference's VDBCollisionObject, houdiniPlugin/VDBCollisionObject.cpp:
 // 11-37, reads phi, its gradient and a velocity field from OpenVDB grids): trilinear
 // interpolation of the point values, clamped to the volume like a 3-D texture with clamp
 // addressing; each lerp is a + t (b - a) with separately rounded operations.
-MPM_HD float vol_lerp(float a, float b, float t) {
+template <class T> MPM_HD T vol_lerp(T a, T b, T t) {
 #pragma clang fp contract(off)
   return a + t * (b - a);
 }
 // nc components per point, component c of point p at f[p * nc + c]
-MPM_HD void vol_sample(const Collider& o, const float* f, int nc, const float x[3], float* out) {
+template <class T> MPM_HD void vol_sample(const ColliderT<T>& o, const T* f, int nc, const T x[3], T* out) {
 #pragma clang fp contract(off)
   int i0[3];
-  float t[3];
+  T t[3];
   for (int a = 0; a < 3; ++a) {
-    float u = (x[a] - o.origin[a]) / o.voxel;
-    const float hi = (float)(o.dims[a] - 1);
-    if (!(u >= 0.0f)) u = 0.0f;  // below the volume, or NaN
+    T u = (x[a] - o.origin[a]) / o.voxel;
+    const T hi = (T)(o.dims[a] - 1);
+    if (!(u >= T(0))) u = T(0);  // below the volume, or NaN
     if (u > hi) u = hi;
-    int i = (int)floorf(u);
+    int i = (int)mfloor(u);
     if (i > o.dims[a] - 2) i = o.dims[a] - 2;
     i0[a] = i;
-    t[a] = u - (float)i;
+    t[a] = u - (T)i;
   }
   const int64_t sx = nc, sy = (int64_t)nc * o.dims[0], sz = sy * o.dims[1];
-  const float* b = f + i0[0] * sx + i0[1] * sy + i0[2] * sz;
+  const T* b = f + i0[0] * sx + i0[1] * sy + i0[2] * sz;
   for (int c = 0; c < nc; ++c) {
-    const float* q = b + c;
-    const float e00 = vol_lerp(q[0], q[sx], t[0]);
-    const float e10 = vol_lerp(q[sy], q[sy + sx], t[0]);
-    const float e01 = vol_lerp(q[sz], q[sz + sx], t[0]);
-    const float e11 = vol_lerp(q[sz + sy], q[sz + sy + sx], t[0]);
+    const T* q = b + c;
+    const T e00 = vol_lerp(q[0], q[sx], t[0]);
+    const T e10 = vol_lerp(q[sy], q[sy + sx], t[0]);
+    const T e01 = vol_lerp(q[sz], q[sz + sx], t[0]);
+    const T e11 = vol_lerp(q[sz + sy], q[sz + sy + sx], t[0]);
     out[c] = vol_lerp(vol_lerp(e00, e10, t[1]), vol_lerp(e01, e11, t[1]), t[2]);
   }
 }
-MPM_HD float col_phi(const Collider& o, const float x[3]) {
+template <class T> MPM_HD T col_phi(const ColliderT<T>& o, const T x[3]) {
 #pragma clang fp contract(off)
   if (o.type == 0) return o.c[0] * x[0] + o.c[1] * x[1] + o.c[2] * x[2] + o.c[3];
   if (o.type == 2) {
-    float p;
+    T p;
     vol_sample(o, o.phi, 1, x, &p);
     return p;
   }
-  float d0 = x[0] - o.center[0], d1 = x[1] - o.center[1], d2 = x[2] - o.center[2];
-  return sqrtf(d0 * d0 + d1 * d1 + d2 * d2) - o.radius;
+  T d0 = x[0] - o.center[0], d1 = x[1] - o.center[1], d2 = x[2] - o.center[2];
+  return msqrt(d0 * d0 + d1 * d1 + d2 * d2) - o.radius;
 }
 // normalised gradient (CollisionObject::grad + Vector3f::normalize); a volume's gradient is the
 // central difference of the sampled phi over one voxel on each axis
-MPM_HD void col_normal(const Collider& o, const float x[3], float n[3]) {
+template <class T> MPM_HD void col_normal(const ColliderT<T>& o, const T x[3], T n[3]) {
 #pragma clang fp contract(off)
   if (o.type == 0) {
     n[0] = o.c[0]; n[1] = o.c[1]; n[2] = o.c[2];
   } else if (o.type == 2) {
     for (int a = 0; a < 3; ++a) {
-      float xp[3] = {x[0], x[1], x[2]}, xm[3] = {x[0], x[1], x[2]};
+      T xp[3] = {x[0], x[1], x[2]}, xm[3] = {x[0], x[1], x[2]};
       xp[a] = x[a] + o.voxel;
       xm[a] = x[a] - o.voxel;
-      float pp, pm;
+      T pp, pm;
       vol_sample(o, o.phi, 1, xp, &pp);
       vol_sample(o, o.phi, 1, xm, &pm);
-      n[a] = (pp - pm) / (2.0f * o.voxel);
+      n[a] = (pp - pm) / (T(2) * o.voxel);
     }
   } else {
     n[0] = x[0] - o.center[0]; n[1] = x[1] - o.center[1]; n[2] = x[2] - o.center[2];
   }
-  float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  T nn = msqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   n[0] /= nn; n[1] /= nn; n[2] /= nn;
 }
 // CollisionObject::velocity: uniform, or sampled from a volume's velocity field
-MPM_HD void col_velocity(const Collider& o, const float x[3], float v[3]) {
+template <class T> MPM_HD void col_velocity(const ColliderT<T>& o, const T x[3], T v[3]) {
   if (o.type == 2 && o.vel) {
     vol_sample(o, o.vel, 3, x, v);
   } else {
@@ -476,37 +491,37 @@ MPM_HD void col_velocity(const Collider& o, const float x[3], float v[3]) {
 }
 
 // CollisionObjectSet::collide (src/CollisionObject.cpp:19-96)
-MPM_HD int collide(const ColliderSet& cs, float v[3], const float x[3], const float fv[3], bool addVel) {
+template <class T> MPM_HD int collide(const ColliderSetT<T>& cs, T v[3], const T x[3], const T fv[3], bool addVel) {
   int obj = -1;
   bool already = false;
   for (int i = 0; i < cs.n; ++i) {
-    const Collider& o = cs.o[i];
-    if (col_phi(o, x) <= 0.0f) {
+    const ColliderT<T>& o = cs.o[i];
+    if (col_phi(o, x) <= T(0)) {
       if (already) {
         obj = -2;
-        v[0] = v[1] = v[2] = 0.0f;
+        v[0] = v[1] = v[2] = T(0);
         break;
       }
       already = true;
-      float vo[3];
+      T vo[3];
       col_velocity(o, x, vo);
       vo[0] -= fv[0]; vo[1] -= fv[1]; vo[2] -= fv[2];
       v[0] -= vo[0]; v[1] -= vo[1]; v[2] -= vo[2];
-      float n[3];
+      T n[3];
       col_normal(o, x, n);
-      float nd = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
-      if (nd < 0.0f) {
+      T nd = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+      if (nd < T(0)) {
         obj = i;
         if (o.sticky) {
-          v[0] = v[1] = v[2] = 0.0f;
+          v[0] = v[1] = v[2] = T(0);
         } else {
-          float t0 = v[0] - nd * n[0], t1 = v[1] - nd * n[1], t2 = v[2] - nd * n[2];
-          float tn = sqrtf(t0 * t0 + t1 * t1 + t2 * t2);
+          T t0 = v[0] - nd * n[0], t1 = v[1] - nd * n[1], t2 = v[2] - nd * n[2];
+          T tn = msqrt(t0 * t0 + t1 * t1 + t2 * t2);
           if (tn >= -nd * o.friction) {
-            float s = 1.0f + o.friction * nd / tn;
+            T s = T(1) + o.friction * nd / tn;
             v[0] = t0 * s; v[1] = t1 * s; v[2] = t2 * s;
           } else {
-            v[0] = v[1] = v[2] = 0.0f;
+            v[0] = v[1] = v[2] = T(0);
           }
         }
       }

@@ -15,8 +15,13 @@ elementwise error); and the CR iteration count of the largest body in all three 
 
     python tests/precision_study.py --n 60000 --steps 300 --csv <per-step csv> --out <summary .md>
 
+With --gpu64 the full-FP64 run is the library's FP64 GPU step (mpm_sim64_*, pinned against the FP64
+oracle by tests/test_gpu_fp64.py) instead, at the configuration's own size (4M particles):
+
+    python tests/precision_study.py --gpu64 --n 4000000 --steps 300 --csv <per-step csv> --out <summary .md>
+
 This script uses the oracle as the checker; it is not part of the library or of bench.py's timed
-path. `tests/test_gpu_precision_study.py` runs it at a small size.
+path. `tests/test_gpu_precision_study.py` runs both modes at a small size.
 """
 import argparse
 import json
@@ -204,6 +209,111 @@ def read_csv(path, n):
     return recs, meta
 
 
+def run_study_gpu64(n, steps, csv_path=None, log=print, full_slab=True):
+    """The same comparison with the full-FP64 run on the GPU (mpm_sim64_*, the reference's algorithm in
+    double), which reaches the configuration's own size: the 4M-particle 0.8 x 0.2 x 0.4 m slab
+    (full_slab), or the study's cut of the slab at n particles. Per step: the FP32-bulk GPU step's
+    momentum, position and F errors against the FP64 GPU step, and both runs' CR iterations."""
+    import mpmsnow_amd as G
+    from mpmsnow_amd import scenes
+
+    if full_slab:
+        sc = scenes.slope_slab(n=n)
+        size = (0.8, 0.2, 0.4)
+    else:
+        sc, size = study_scene(n)
+    n = len(sc["m"])
+    h, dt = sc["h"], sc["dt"]
+    kw = dict(planes=sc["planes"], gravity=[sc["gravity"]], v=sc["v"], mu0=sc["mu0"], lambda0=sc["lambda0"],
+              xi=sc["xi"])
+    gpu = G.Sim(sc["x"], sc["m"], h, material=sc["material"], **kw)
+    g64 = G.Sim64(sc["x"], sc["m"], float(np.float32(h)), sc["material"], **kw)
+    m = sc["m"]
+    recs = []
+    fh = open(csv_path, "w") if csv_path else None
+    cols = ["step", "t"] + ["gpu_" + q for q in QUANTITIES] + ["its_gpu", "its_fp64", "bodies_gpu", "bodies_fp64",
+                                                              "s_fp32", "s_fp64"]
+    if fh:
+        fh.write(",".join(cols) + "\n")
+    t0 = time.time()
+    for step in range(steps):
+        ta = time.time()
+        st = gpu.advance(dt, sc["max_iters"], sc["tol"])
+        tb = time.time()
+        s64 = g64.advance(dt, sc["max_iters"], sc["tol"])
+        tc = time.time()
+        ref = {k: g64.get(k).reshape(n, -1) for k in ("p", "v", "F")}
+        rg = {k: gpu.get(k).reshape(n, -1) for k in ("p", "v", "F")}
+        rec = {"step": step + 1, "t": (step + 1) * dt}
+        for k, v in _errors(m, h, ref, rg).items():
+            rec["gpu_" + k] = v
+        big = max((gpu.body_stats(b, hist_cap=0) for b in range(st["num_solved"])), key=lambda r: r["particles"])
+        big64 = max((g64.body_stats(b, hist_cap=0) for b in range(s64["num_solved"])), key=lambda r: r["particles"])
+        rec.update(its_gpu=big["iterations"], its_fp64=big64["iterations"], bodies_gpu=st["num_solved"],
+                   bodies_fp64=s64["num_solved"], s_fp32=tb - ta, s_fp64=tc - tb)
+        recs.append(rec)
+        if fh:
+            fh.write(",".join("%.6g" % rec[c] if isinstance(rec[c], float) else str(rec[c]) for c in cols) + "\n")
+            fh.flush()
+        log("step %d/%d (%.0f s): FP32 vs FP64 x_max %.2e h F_max %.2e mom %.2e | its FP32 %d FP64 %d | "
+            "bodies %d / %d | %.3f s / %.3f s" % (step + 1, steps, time.time() - t0, rec["gpu_x_max"],
+                                                  rec["gpu_F_max"], rec["gpu_mom_rel"], rec["its_gpu"],
+                                                  rec["its_fp64"], rec["bodies_gpu"], rec["bodies_fp64"],
+                                                  rec["s_fp32"], rec["s_fp64"]))
+    if fh:
+        fh.close()
+    meta = {"particles": n, "size_m": [round(x, 4) for x in size], "h": h, "dt": dt, "steps": steps,
+            "seconds": time.time() - t0, "reference": "gpu64"}
+    return recs, meta
+
+
+def summarize_gpu64(recs, meta, out_path, command):
+    marks = sorted({s for s in (1, 2, 5, 10, 25, 50, 100, 150, 200, 250, 300) if s <= len(recs)} | {len(recs)})
+    lines = [
+        "# Precision study, slope slab: FP32-bulk GPU step against the FP64 GPU step, %d particles, %d steps"
+        % (meta["particles"], meta["steps"]),
+        "",
+        "Command (from the repository root on the GPU host):",
+        "",
+        "    " + command,
+        "",
+        "Scene: slope slab %s m, %d particles, h = 1/256, dt = 5e-4, per-particle E0 / xi (seed 2), 30-degree"
+        % (" x ".join("%.3g" % x for x in meta["size_m"]), meta["particles"]),
+        "slope with friction 0.2, CR tolerance 1e-5. Both runs start from the same particles and step on their",
+        "own. The FP64 run is mpm_sim64_advance: the reference's step (Sim.cpp:57-134) with every scalar in",
+        "double on the GPU (its parity with the FP64 oracle: tests/test_gpu_fp64.py). Errors of the FP32-bulk",
+        "step (mpm_sim_advance: FP32 bulk, FP64 CR reductions) against it: momentum = |sum m v - sum m v64| /",
+        "|sum m v64| (the particles' total momentum, which the transfers carry to and from the grid); x = per",
+        "particle the largest per-axis position error in units of h; F = per particle the largest elementwise",
+        "error; max and 99th percentile over particles. %.0f s in all." % meta["seconds"],
+        "",
+        "| step | momentum | x max (h) | x p99 (h) | F max | F p99 | CR its FP32 / FP64 | bodies FP32 / FP64 |",
+        "|---|---|---|---|---|---|---|---|",
+    ]
+    for st in marks:
+        r = recs[st - 1]
+        lines.append("| %d | %.2e | %.2e | %.2e | %.2e | %.2e | %d / %d | %d / %d |" % (
+            st, r["gpu_mom_rel"], r["gpu_x_max"], r["gpu_x_p99"], r["gpu_F_max"], r["gpu_F_p99"], r["its_gpu"],
+            r["its_fp64"], r["bodies_gpu"], r["bodies_fp64"]))
+    worst = {q: max(r["gpu_" + q] for r in recs) for q in QUANTITIES}
+    lines.append("| max over steps | %.2e | %.2e | %.2e | %.2e | %.2e | | |" % tuple(worst[q] for q in QUANTITIES))
+    its = np.array([[r["its_gpu"], r["its_fp64"]] for r in recs], dtype=float)
+    t32 = np.array([r["s_fp32"] for r in recs])
+    t64 = np.array([r["s_fp64"] for r in recs])
+    lines += ["", "CR iterations of the largest body, mean over steps: FP32-bulk %.1f, FP64 %.1f. Wall time per step"
+              " (median): FP32-bulk %.1f ms, FP64 %.0f ms." % (its[:, 0].mean(), its[:, 1].mean(),
+                                                               1e3 * np.median(t32), 1e3 * np.median(t64))]
+    summary = {"workload": "slope_slab_precision_gpu64", "particles": meta["particles"], "steps": meta["steps"],
+               "final": {k: v for k, v in recs[-1].items() if k not in ("step", "t")},
+               "max_over_steps": {"gpu": worst}, "its_mean": {"fp32": its[:, 0].mean(), "fp64": its[:, 1].mean()},
+               "ms_per_step_median": {"fp32": 1e3 * np.median(t32), "fp64": 1e3 * np.median(t64)},
+               "seconds": meta["seconds"]}
+    lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=60000)
@@ -212,7 +322,16 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--from-csv", default=None, help="summarise a per-step csv of an earlier run instead")
+    ap.add_argument("--gpu64", action="store_true",
+                    help="the full-FP64 run on the GPU (mpm_sim64) instead of the CPU oracle, at the 4M slab "
+                         "(--n sets the particle count of the 0.8 x 0.2 x 0.4 m box)")
     a = ap.parse_args()
+    if a.gpu64:
+        recs, meta = run_study_gpu64(a.n, a.steps, a.csv, log=lambda s: print(s, flush=True))
+        cmd = "python tests/precision_study.py --gpu64 --n %d --steps %d --csv <per-step csv> --out <this file>" % (
+            a.n, a.steps)
+        print(json.dumps(summarize_gpu64(recs, meta, a.out, cmd)))
+        return
     if a.from_csv:
         recs, meta = read_csv(a.from_csv, a.n)
         cmd = "python tests/precision_study.py --n %d --steps %d --csv <per-step csv> --out <this file>" % (
