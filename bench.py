@@ -19,7 +19,9 @@ mean launch time, both measured in the timed steps with HIP events on the librar
 per HVP launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload.
 network = the preconditioner's forward pass (MFMA BF16): mean time per forward (HIP events), an
 upper bound of its FLOPs from the layer shapes, and the MFMA FLOPs that rocprofv3 counted
-(committed summary). precision_study = the committed 300-step FP32-vs-FP64 study of the slab.
+(committed summary). precision_study = the committed 300-step FP32-vs-FP64 study of the slab (at 60k
+particles against the FP64 oracle); precision_study_4M = the same at 4M particles against the
+full-FP64 GPU step (mpm_sim64).
 cpu_baseline = the reference-faithful CPU oracle (float, OpenMP over the 8-colour voxel
 partition like the reference's TBB splats) timed on this host for a bounded sample: a 500k-
 particle slab of the same density, materials, h and dt (0.2 m cube) from the GPU's state after
@@ -734,6 +736,7 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
 STIFF_STATS_FILE = os.path.join(ROOT, "profiles", "stiff_drop_16M_kernel_stats.md")
 MFMA_FILE = os.path.join(ROOT, "profiles", "lpc_conv_mfma_slope_slab_4M.md")
 PRECISION_FILE = os.path.join(ROOT, "profiles", "slope_slab_precision_300_steps.md")
+PRECISION_GPU64_FILE = os.path.join(ROOT, "profiles", "slope_slab_4M_precision_fp32_vs_fp64_gpu.md")
 
 
 def stiff_trace_summary(stats_csv, stiff_json, out_path):
@@ -920,6 +923,14 @@ def main():
                                   "steps": ps["steps"], "max_over_steps": ps["max_over_steps"],
                                   "note": "FP32-bulk GPU step vs the FP64 oracle (and the FP32 oracle) over the "
                                           "same steps at the slab's density, h and dt"}
+    p64 = read_footer(PRECISION_GPU64_FILE)
+    if p64:
+        out["precision_study_4M"] = {"profile": os.path.relpath(PRECISION_GPU64_FILE, ROOT),
+                                     "particles": p64["particles"], "steps": p64["steps"],
+                                     "max_over_steps": p64["max_over_steps"]["gpu"], "its_mean": p64["its_mean"],
+                                     "ms_per_step_median": p64["ms_per_step_median"],
+                                     "note": "FP32-bulk GPU step vs the full-FP64 GPU step (mpm_sim64) on the "
+                                             "workload itself, 300 steps"}
     if not args.no_cpu:
         out["cpu_baseline"], out["error_vs_fp64"] = cpu_leg(args.warmup)
         out["cpu_baseline"]["gpu_over_cpu"] = pl["value"] / out["cpu_baseline"]["value"]
