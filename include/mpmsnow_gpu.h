@@ -230,8 +230,18 @@ int mpm_lpc_set_capture_iterations(mpm_context* ctx, const int32_t* iters, int n
  * 1 - <r, A z>_D^2 / (|r|_D^2 |A z|_D^2) (D = diagonal, z = r / diag + scale * network(r)):
  * the exact correction A^-1 r gives 0. */
 int mpm_lpc_train_next(mpm_context* ctx, int samples, int epochs, float learning_rate);
+/* training loss of mpm_lpc_train_next: 0 (default) the optimal-step residual loss above; 1 the
+ * exact-correction loss: the solve also keeps its iterates x_k, and with its final iterate x*
+ * each sample's exact correction is e = x* - x_k (A e = r_k up to the final residual); per body
+ * the network minimises the fraction of the error energy e^T A e that the best step along z
+ * leaves, 1 - (z^T r)^2 / ((z^T A z)(e^T r)) (samples at least 4 iterations before the solve's
+ * end). Solve the training steps to a tighter tolerance than the production ones so that x* is
+ * close to the exact solution. */
+int mpm_lpc_set_training_loss(mpm_context* ctx, int mode);
 /* mean per-body loss of that training's first and last epoch, and of the diagonal step alone */
 int mpm_lpc_last_loss(mpm_context* ctx, double* first, double* last, double* diagonal);
+/* (body, sample) training pairs that training used (bodies of 256+ particles still iterating) */
+int mpm_lpc_last_pairs(const mpm_context* ctx);
 /* grid API (one body; dense arrays in node order): network output for residual r, diagonal
  * diag and input scale, and the loss w |out - e / scale|^2 with its parameter gradient */
 int mpm_grid_lpc_apply(mpm_grid* grid, const float* r, const float* diag, float scale, float* out);

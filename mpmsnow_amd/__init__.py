@@ -118,6 +118,8 @@ SIGNATURES = {
     "mpm_lpc_set_handoff": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_lpc_set_capture_iterations": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "mpm_lpc_train_next": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
+    "mpm_lpc_set_training_loss": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_lpc_last_pairs": (C.c_int, [C.c_void_p]),
     "mpm_lpc_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
     "mpm_grid_lpc_apply": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, _fp]),
@@ -373,6 +375,13 @@ class Context:
         """Bodies of at most 8 node blocks and this many particles are solved in one workgroup
         each, concurrently with the batched loop (0: off). Same iterates either way."""
         self.check(lib().mpm_set_small_bodies(self.ptr, int(max_particles)))
+
+    def lpc_last_pairs(self):
+        return lib().mpm_lpc_last_pairs(self.ptr)
+
+    def lpc_training_loss(self, mode):
+        """0: optimal-step residual loss; 1: exact-correction energy-norm loss."""
+        self.check(lib().mpm_lpc_set_training_loss(self.ptr, int(mode)))
 
     def lpc_train_next(self, samples, epochs, lr):
         """The next advance trains the network on `samples` residuals of its solve."""

@@ -2853,6 +2853,89 @@ __global__ void k_lpc_res_grad(Batch B, const float* mass, const int8_t* flags, 
   for (int k = 0; k < 16; ++k) dst[k] = (__bf16)(k < 3 ? gv[k] : 0.0f);
 }
 
+// ---- exact-correction (energy-norm) loss. With the solve's final iterate x*, sample k's exact
+// correction is e = x* - x_k (A e = r_k up to the final residual) and its error energy
+// E = e^T A e = e^T r_k. A step alpha z leaves (e - alpha z)^T A (e - alpha z); the best alpha
+// leaves the fraction L = 1 - (z^T r)^2 / ((z^T A z) E) of E, the per-body loss.
+template <int NV>
+__device__ __forceinline__ void block_add_to(double (&s)[NV], double* dst, bool on) {
+  __shared__ double red[NV][kBlock / 64];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double v = s[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV && on) {
+    const double t = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+    if (t != 0.0) atomicAdd(dst + threadIdx.x, t);
+  }
+}
+// energy[b] += (x* - x_k) . r_k over the body's nodes
+__global__ void k_lpc_exact_energy(Batch B, const float* x, const float* xk, const float* rk, double* energy) {
+  NodeCtx c = node_begin(B);
+  double s[1] = {0.0};
+  if (c.valid) {
+    float a[3], b[3], r[3];
+    ld3(x, c.i, a);
+    ld3(xk, c.i, b);
+    ld3(rk, c.i, r);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[0] += ((double)a[k] - (double)b[k]) * (double)r[k];
+  }
+  block_add_to<1>(s, energy + c.b, true);
+}
+// u = A z from the staged force differential; dots[2 b + {0, 1}] += z . r, z . u
+__global__ void k_lpc_energy_dots(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                                  const float* stage, const float* z, const float* r, float dt, const float* weight,
+                                  float* u, double* dots) {
+  NodeCtx c = node_begin(B);
+  float uu[3] = {0.0f, 0.0f, 0.0f};
+  double s[2] = {0.0, 0.0};
+  const bool on = weight[c.b] > 0.0f;
+  if (on && c.valid) {
+    float zz[3], df[3], rr[3];
+    ld3(z, c.i, zz);
+    gather_node3(B, c.G, stage, c.T, c.l, df);
+    apply_A(c, cols, flags[c.i], mass[c.i], zz, df, dt * dt, uu);
+    ld3(r, c.i, rr);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      s[0] += (double)zz[a] * rr[a];
+      s[1] += (double)zz[a] * uu[a];
+    }
+  }
+  st3(u, c.i, uu);
+  block_add_to<2>(s, dots + 2 * c.b, on);
+}
+// the network output's gradient g = scale dL/dz with dL/dz = (2 a / (c E)) ((a / c) u - r),
+// a = z . r, c = z . u (where the output is not clamped; bf16 [node][16 channels]); *loss += the
+// body's L once (first node block of the body)
+__global__ void k_lpc_energy_grad(Batch B, const float* u, const float* r, const float* raw, const float* scale,
+                                  const float* weight, const double* dots, const double* energy, __bf16* g,
+                                  double* loss) {
+  NodeCtx c = node_begin(B);
+  const double a = dots[2 * c.b], cc = dots[2 * c.b + 1], E = energy[c.b];
+  const bool ok = weight[c.b] > 0.0f && cc > 0.0 && E > 0.0;
+  float gv[3] = {0.0f, 0.0f, 0.0f};
+  if (ok && c.valid && raw) {
+    const float alpha = (float)(a / cc), f = (float)(2.0 * a / (cc * E));
+    float uu[3], rr[3], o[3];
+    ld3(u, c.i, uu);
+    ld3(r, c.i, rr);
+    ld3(raw, c.i, o);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gv[k] = fabsf(o[k]) < kLpcClamp ? scale[c.b] * f * (alpha * uu[k] - rr[k]) : 0.0f;
+  }
+  if (g) {
+    __bf16* dst = g + c.i * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) dst[k] = (__bf16)(k < 3 ? gv[k] : 0.0f);
+  }
+  if (ok && threadIdx.x == 0 && blockIdx.x == (unsigned)c.G.blockBase) atomicAdd(loss, 1.0 - a * a / (cc * E));
+}
+
 __global__ void k_residual_vector(Batch B, const float* z, const float* diag, float* r) {
   NodeCtx c = node_begin(B);
   float zz[3], dg[3];
@@ -3375,6 +3458,20 @@ void launch_lpc_res_grad(const Batch& B, const float* mass, const int8_t* flags,
                          const float* stage, const float* y, const float* raw, float dt, const float* scale,
                          const float* weight, __bf16* g, hipStream_t s) {
   NODE_LAUNCH(k_lpc_res_grad, B, mass, flags, cols, stage, y, raw, dt, scale, weight, g);
+}
+void launch_lpc_exact_energy(const Batch& B, const float* x, const float* xk, const float* rk, double* energy,
+                             hipStream_t s) {
+  NODE_LAUNCH(k_lpc_exact_energy, B, x, xk, rk, energy);
+}
+void launch_lpc_energy_dots(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                            const float* stage, const float* z, const float* r, float dt, const float* weight,
+                            float* u, double* dots, hipStream_t s) {
+  NODE_LAUNCH(k_lpc_energy_dots, B, mass, flags, cols, stage, z, r, dt, weight, u, dots);
+}
+void launch_lpc_energy_grad(const Batch& B, const float* u, const float* r, const float* raw, const float* scale,
+                            const float* weight, const double* dots, const double* energy, __bf16* g, double* loss,
+                            hipStream_t s) {
+  NODE_LAUNCH(k_lpc_energy_grad, B, u, r, raw, scale, weight, dots, energy, g, loss);
 }
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s) {
   NODE_LAUNCH(k_residual_vector, B, z, diag, r);

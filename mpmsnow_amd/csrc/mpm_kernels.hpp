@@ -170,6 +170,18 @@ void launch_lpc_res_y(const Batch& B, const float* u, const float* r, const floa
 void launch_lpc_res_grad(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                          const float* stage, const float* y, const float* raw, float dt, const float* scale,
                          const float* weight, __bf16* g, hipStream_t s);
+// ---- training on exact corrections, energy-norm loss 1 - (z.r)^2 / ((z.A z) E) per body
+// energy[b] += (x - xk) . rk (x: the solve's final iterate; E of sample k)
+void launch_lpc_exact_energy(const Batch& B, const float* x, const float* xk, const float* rk, double* energy,
+                             hipStream_t s);
+// with the force differential of z staged: u = A z; dots[2 b + {0, 1}] += z . r, z . u
+void launch_lpc_energy_dots(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
+                            const float* stage, const float* z, const float* r, float dt, const float* weight,
+                            float* u, double* dots, hipStream_t s);
+// network output gradient (g may be null) and *loss += the per-body losses
+void launch_lpc_energy_grad(const Batch& B, const float* u, const float* r, const float* raw, const float* scale,
+                            const float* weight, const double* dots, const double* energy, __bf16* g, double* loss,
+                            hipStream_t s);
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s);
 void launch_finalize(const Batch& B, const float* x, const float* vc, float* vel, const CRScalars* sc,
                      hipStream_t s);

@@ -213,11 +213,15 @@ struct LpcCapture {
   int n = 0;                     // samples kept
   std::vector<int> iters;        // CR iteration of each sample
   std::vector<int> schedule = {0, 1, 2, 3, 4, 6, 8, 11, 15, 20, 26, 33, 41, 50, 60, 72};  // iterations to keep
-  DevBuf data;                   // [sample][3 NN floats]
-  DevBuf scale, weight, sums, loss;  // [sample][body]
+  DevBuf data;                   // [sample][3 NN floats]: residuals r_k
+  DevBuf xdata;                  // [sample][3 NN floats]: iterates x_k (exact-correction loss)
+  DevBuf scale, weight, sums, loss, energy;  // [sample][body]
   int64_t NN = 0;
-  double lossFirst = 0, lossLast = 0, lossDiag = 0;  // mean residual loss: first, last epoch, diagonal only
+  int lossMode = 0;              // 0: optimal-step residual loss; 1: exact-correction energy-norm loss
+  double lossFirst = 0, lossLast = 0, lossDiag = 0;  // mean per-body loss: first, last epoch, diagonal only
+  int pairs = 0;                 // (body, sample) pairs the last training used
   float* r(int k) { return data.as<float>() + (int64_t)k * 3 * NN; }
+  float* xk(int k) { return xdata.as<float>() + (int64_t)k * 3 * NN; }
 };
 
 struct mpm_context {
@@ -796,12 +800,15 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     if (capWant > 0) {
       cap.NN = NN;
       cap.data.ensure(sizeof(float) * 3 * (size_t)NN * capWant);
+      if (cap.lossMode == 1) cap.xdata.ensure(sizeof(float) * 3 * (size_t)NN * capWant);
       cap.iters.clear();
       cap.n = 0;
     }
     auto snapshot = [&](int iterIndex) {
       if (cap.n >= capWant || kCaptureIters[cap.n] != iterIndex) return;
       launch_residual_vector(B, f(z), f(diag), cap.r(cap.n), s);
+      if (cap.lossMode == 1)
+        HIPCHECK(hipMemcpyAsync(cap.xk(cap.n), f(x), vecBytes(), hipMemcpyDeviceToDevice, s));
       cap.iters.push_back(iterIndex);
       ++cap.n;
     };
@@ -937,14 +944,29 @@ void GridBatch::train_lpc(float dt, CRScalars* hsc, size_t scb) {
   HIPCHECK(hipMemcpyAsync(sums.data(), cap.sums.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipMemcpyAsync(hsc, scal.p, scb, hipMemcpyDeviceToHost, s));
   c.sync();
+  const bool exact = cap.lossMode == 1;
+  std::vector<double> energy;
+  if (exact) {
+    // each sample's error energy e^T A e = (x* - x_k) . r_k, x* the solve's final iterate
+    cap.energy.ensure(sizeof(double) * nb * cap.n);
+    HIPCHECK(hipMemsetAsync(cap.energy.p, 0, sizeof(double) * nb * cap.n, s));
+    for (int k = 0; k < cap.n; ++k)
+      launch_lpc_exact_energy(B, f(x), cap.xk(k), cap.r(k), cap.energy.as<double>() + nb * k, s);
+    energy.resize(nb * cap.n);
+    HIPCHECK(hipMemcpyAsync(energy.data(), cap.energy.p, sizeof(double) * energy.size(), hipMemcpyDeviceToHost, s));
+    c.sync();
+  }
   std::vector<float> hs(nb * cap.n), hw(nb * cap.n);
   std::vector<int> useful;
   for (int k = 0; k < cap.n; ++k) {
     bool any = false;
     for (int bi = 0; bi < nb; ++bi) {
       const double* q = &sums[3 * (nb * k + bi)];
-      // bodies of at least 256 particles still iterating at this sample
-      const bool use = hb[bi].np >= 256 && hsc[bi].iterations > cap.iters[k] && q[2] > 0.0 && q[1] > 0.0;
+      // bodies of at least 256 particles still iterating at this sample; for the exact-correction
+      // loss, 4 or more iterations before the end, so that x* - x_k is well above x*'s own error
+      const int ahead = exact ? 4 : 0;
+      const bool use = hb[bi].np >= 256 && hsc[bi].iterations > cap.iters[k] + ahead && q[2] > 0.0 && q[1] > 0.0 &&
+                       (!exact || energy[nb * k + bi] > 0.0);
       hs[nb * k + bi] = (float)std::sqrt(q[1] > 0.0 ? q[0] / q[1] : 0.0);
       hw[nb * k + bi] = use ? 1.0f : 0.0f;
       any = any || use;
@@ -961,9 +983,18 @@ void GridBatch::train_lpc(float dt, CRScalars* hsc, size_t scb) {
   const int8_t* fl = flags.as<int8_t>();
   double* dl = cap.loss.as<double>();
   double* dots = dl + 1;
-  auto step_loss = [&](int k, const float* raw) {
+  const double* ek = exact ? cap.energy.as<double>() : nullptr;
+  // the loss of step z = r / diag + scale clamp(raw) for sample k; with the exact-correction loss
+  // also the network output's gradient into g (when given)
+  auto step_loss = [&](int k, const float* raw, __bf16* g) {
     launch_lpc_compose_z(B, cap.r(k), f(diag), raw, sk + nb * k, wk + nb * k, f(fp0), s);
     hvp(f(fp0), true, f(stageA), nullptr, -1);
+    if (exact) {
+      HIPCHECK(hipMemsetAsync(dots, 0, sizeof(double) * 2 * nb, s));
+      launch_lpc_energy_dots(B, f(mass), fl, c.cols(), f(stageA), f(fp0), cap.r(k), dt, wk + nb * k, f(Ar), dots, s);
+      launch_lpc_energy_grad(B, f(Ar), cap.r(k), raw, sk + nb * k, wk + nb * k, dots, ek + nb * k, g, dl, s);
+      return;
+    }
     HIPCHECK(hipMemsetAsync(dots, 0, sizeof(double) * 3 * nb, s));
     launch_lpc_res_dots(B, f(mass), fl, c.cols(), f(stageA), f(fp0), cap.r(k), f(diag), dt, wk + nb * k, f(Ar),
                         dots, s);
@@ -977,21 +1008,24 @@ void GridBatch::train_lpc(float dt, CRScalars* hsc, size_t scb) {
   };
   // loss of the diagonal step alone
   HIPCHECK(hipMemsetAsync(dl, 0, sizeof(double), s));
-  for (int k : useful) step_loss(k, nullptr);
+  for (int k : useful) step_loss(k, nullptr, nullptr);
   int nbodies_used = 0;
   for (int k : useful)
     for (int bi = 0; bi < nb; ++bi) nbodies_used += hw[nb * k + bi] > 0.0f ? 1 : 0;
   const double norm = nbodies_used > 0 ? 1.0 / nbodies_used : 0.0;
+  cap.pairs = nbodies_used;
   cap.lossDiag = fetch_loss() * norm;
   for (int ep = 0; ep < cap.epochs; ++ep) {
     HIPCHECK(hipMemsetAsync(dl, 0, sizeof(double), s));
     for (int k : useful) {
       float* raw = c.net.raw.as<float>();
       c.net.forward(B, levels, cap.r(k), f(diag), f(mass), fl, sk + nb * k, nullptr, raw, nullptr, s);
-      step_loss(k, raw);
-      hvp(f(fp1), true, f(stageA), nullptr, -1);
-      launch_lpc_res_grad(B, f(mass), fl, c.cols(), f(stageA), f(fp1), raw, dt, sk + nb * k, wk + nb * k,
-                          c.net.g6.as<bf16>(), s);
+      step_loss(k, raw, c.net.g6.as<bf16>());
+      if (!exact) {
+        hvp(f(fp1), true, f(stageA), nullptr, -1);
+        launch_lpc_res_grad(B, f(mass), fl, c.cols(), f(stageA), f(fp1), raw, dt, sk + nb * k, wk + nb * k,
+                            c.net.g6.as<bf16>(), s);
+      }
       c.net.backward(B, levels, s);
       c.net.adam(cap.lr, s);
     }
@@ -1784,6 +1818,13 @@ int mpm_lpc_set_handoff(mpm_context* ctx, int on) {
   });
 }
 
+int mpm_lpc_set_training_loss(mpm_context* ctx, int mode) {
+  return guarded(ctx, [&] {
+    if (mode != 0 && mode != 1) throw InvalidArg("training loss: 0 (optimal-step residual) or 1 (exact correction)");
+    ctx->cap.lossMode = mode;
+  });
+}
+
 int mpm_lpc_set_capture_iterations(mpm_context* ctx, const int32_t* iters, int n) {
   return guarded(ctx, [&] {
     if (!iters || n < 1 || n > 16) throw InvalidArg("1 to 16 capture iterations");
@@ -1812,6 +1853,8 @@ int mpm_lpc_last_loss(mpm_context* ctx, double* first, double* last, double* dia
     if (diagonal) *diagonal = ctx->cap.lossDiag;
   });
 }
+
+int mpm_lpc_last_pairs(const mpm_context* ctx) { return ctx ? ctx->cap.pairs : -1; }
 
 namespace {
 // dense single-body inputs of the grid API into the batch layout

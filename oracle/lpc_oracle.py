@@ -215,3 +215,19 @@ def dense_to_volume(a, n, ncomp):
 def volume_to_dense(v):
     """[i, j, k, c] -> reference node order, (n, c)."""
     return np.ascontiguousarray(v.transpose(2, 1, 0, 3)).reshape(-1, v.shape[3])
+
+
+def energy_loss_and_grad(z, r, Az, E):
+    """Exact-correction (energy-norm) training loss of one body and its gradient in z.
+
+    For a residual r with exact correction e (A e = r) and error energy E = e^T A e = e^T r, the
+    step alpha z with the best alpha leaves (e - alpha z)^T A (e - alpha z) = E - (z^T r)^2 / (z^T A z),
+    i.e. the fraction L = 1 - (z^T r)^2 / ((z^T A z) E) of the error energy. With a = z^T r and
+    c = z^T A z (A symmetric): dL/dz = (2 a / (c E)) ((a / c) A z - r). The library's training
+    kernels (k_lpc_energy_dots, k_lpc_energy_grad) evaluate these from the HVP u = A z."""
+    z, r, Az = (np.asarray(v, dtype=np.float64).ravel() for v in (z, r, Az))
+    a = float(z @ r)
+    c = float(z @ Az)
+    L = 1.0 - a * a / (c * E)
+    g = (2.0 * a / (c * E)) * ((a / c) * Az - r)
+    return L, g

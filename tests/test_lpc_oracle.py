@@ -100,3 +100,29 @@ def test_output_layer_channels_beyond_three_get_no_gradient():
     gW = grad[w:w + 27 * 16 * 16].reshape(16, 27, 16)
     assert np.all(gW[3:] == 0) and np.all(grad[b + 3:b + 16] == 0)
     assert np.abs(gW[:3]).max() > 0
+
+
+def test_energy_loss_pins():
+    """The exact-correction energy loss of oracle/lpc_oracle.py: 0 for any multiple of the exact
+    correction, the error fraction a plain gradient step leaves (by direct evaluation of
+    (e - alpha z)^T A (e - alpha z) at the minimising alpha), in [0, 1], and its gradient against
+    central finite differences on a random SPD system."""
+    rng = np.random.default_rng(4)
+    n = 24
+    M = rng.normal(size=(n, n))
+    A = M @ M.T + n * np.eye(n)
+    r = rng.normal(size=n)
+    e = np.linalg.solve(A, r)
+    E = float(e @ r)
+    loss, _ = L.energy_loss_and_grad(-2.5 * e, r, A @ (-2.5 * e), E)
+    assert abs(loss) < 1e-12
+    z = rng.normal(size=n)
+    loss, g = L.energy_loss_and_grad(z, r, A @ z, E)
+    alphas = np.linspace(-5, 5, 20001) * (z @ r) / (z @ A @ z)
+    left = min((e - al * z) @ A @ (e - al * z) for al in alphas)
+    assert abs(loss - left / E) < 1e-6 and 0.0 <= loss <= 1.0
+    h = 1e-6
+    fd = np.array([(L.energy_loss_and_grad(z + h * np.eye(n)[i], r, A @ (z + h * np.eye(n)[i]), E)[0]
+                    - L.energy_loss_and_grad(z - h * np.eye(n)[i], r, A @ (z - h * np.eye(n)[i]), E)[0]) / (2 * h)
+                   for i in range(n)])
+    assert np.max(np.abs(fd - g)) < 1e-7 * max(1.0, np.abs(g).max())
