@@ -216,7 +216,7 @@ def learned_window(rows):
     return (min(net), max(net)) if net else None
 
 
-def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
+def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None, strikes=None):
     """Learned flexible CR and plain diagonal CR on the slope slab, --steps each, from one warmed-up
     state; HVP and network launches timed with HIP events."""
     import numpy as np
@@ -232,6 +232,8 @@ def slab_legs(steps, warmup, n, weights=None, fallback=None, dirs=None):
         sim.ctx.lpc_fallback(fallback)
     if dirs is not None:
         sim.ctx.lpc_directions(dirs)
+    if strikes is not None:
+        sim.ctx.lpc_strikes(strikes)
     progress("slab: %d particles set up" % n)
     for _ in range(warmup):
         sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
@@ -826,6 +828,8 @@ def main():
     ap.add_argument("--lpc-weights", default=None, help="network parameters (.npy) for the learned legs")
     ap.add_argument("--lpc-fallback", type=float, default=None, help="fallback ratio of the learned legs")
     ap.add_argument("--lpc-dirs", type=int, default=None, help="directions the learned legs' flexible CR keeps")
+    ap.add_argument("--lpc-strikes", type=int, default=None,
+                    help="fallbacks after which a body of the learned legs keeps the diagonal (default 3)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_CSV", "BENCH_LOG"))
@@ -889,7 +893,8 @@ def main():
         return
 
     n = args.particles or 4000000
-    legs, sc = slab_legs(args.steps, args.warmup, n, args.lpc_weights, args.lpc_fallback, args.lpc_dirs)
+    legs, sc = slab_legs(args.steps, args.warmup, n, args.lpc_weights, args.lpc_fallback, args.lpc_dirs,
+                         args.lpc_strikes)
     lr, pl = legs["learned"], legs["plain"]
     h = lr["hvp"]
     out = dict(base, value=lr["value"], ms_per_step=lr["ms_per_step"],

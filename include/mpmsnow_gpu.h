@@ -200,8 +200,8 @@ int mpm_sim_body_solve_stats(const mpm_context* ctx, int body, mpm_solve_stats* 
  * An iteration that used the network and did not lower |r|^2 below `ratio` times its previous
  * value (mpm_lpc_set_fallback; 0.95 by default), or by less than the body's last iteration on
  * the diagonal preconditioner did, is a fallback: the body takes its next iteration with the
- * diagonal preconditioner, and after three fallbacks keeps the diagonal for the rest of its
- * solve; the termination test is unchanged. */
+ * diagonal preconditioner, and after mpm_lpc_set_strikes fallbacks (1 by default) keeps the
+ * diagonal for the rest of its solve; the termination test is unchanged. */
 int64_t mpm_lpc_num_params(void);
 /* random (He) initialisation; the output layer starts at zero */
 int mpm_lpc_init(mpm_context* ctx, uint64_t seed);
@@ -211,6 +211,10 @@ int mpm_lpc_get_params(mpm_context* ctx, float* params, int64_t n);
 int mpm_lpc_set_mode(mpm_context* ctx, int mode);
 /* fallback ratio in (0, 1]: 1 = a fallback only when |r|^2 rises */
 int mpm_lpc_set_fallback(mpm_context* ctx, double ratio);
+/* fallbacks after which a body keeps the diagonal preconditioner for the rest of its solve, 1-16
+ * (default 1): each earlier fallback costs the body one diagonal iteration before the network
+ * is tried again */
+int mpm_lpc_set_strikes(mpm_context* ctx, int strikes);
 /* directions the flexible CR orthogonalises against, 1-8 (default 2; with the diagonal
  * preconditioner every m >= 1 gives the same Krylov iteration in exact arithmetic) */
 int mpm_lpc_set_directions(mpm_context* ctx, int m);

@@ -119,6 +119,7 @@ SIGNATURES = {
     "mpm_lpc_set_capture_iterations": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "mpm_lpc_train_next": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float]),
     "mpm_lpc_set_training_loss": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_lpc_set_strikes": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_lpc_last_pairs": (C.c_int, [C.c_void_p]),
     "mpm_lpc_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
@@ -378,6 +379,9 @@ class Context:
 
     def lpc_last_pairs(self):
         return lib().mpm_lpc_last_pairs(self.ptr)
+
+    def lpc_strikes(self, k):
+        self.check(lib().mpm_lpc_set_strikes(self.ptr, int(k)))
 
     def lpc_training_loss(self, mode):
         """0: optimal-step residual loss; 1: exact-correction energy-norm loss."""

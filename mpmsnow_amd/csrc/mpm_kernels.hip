@@ -2547,13 +2547,15 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
 // the result. Whenever an iteration that used the network fails to lower |r|^2 by the body's
 // fallback ratio (the context's setting: 0.95 by default, 1 = any rise) or by as much as the
 // body's last diagonal iteration did, the body takes its next iteration with the diagonal
-// preconditioner; after kLpcStrikes such fallbacks it keeps the diagonal for the rest of the
+// preconditioner; after the context's strike count of such fallbacks (kLpcStrikes by default) it
+// keeps the diagonal for the rest of the
 // solve.
 
 // r = b - A x0 ; network input scale ; iteration state
 __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                            const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                           int netOn, double ratio, int mdirs, CRScalars* sc, BatchStatus* bst, double* partials) {
+                           int netOn, double ratio, int mdirs, int strikes, CRScalars* sc, BatchStatus* bst,
+                           double* partials) {
   NodeCtx c = node_begin(B);
   CRScalars& S = sc[c.b];
   if (S.done) return;
@@ -2589,6 +2591,7 @@ __global__ void k_fcr_init(Batch B, const float* mass, const int8_t* flags, cons
     S.fallbacks = 0;
     S.retry = 0;
     S.lpc_pause = 0;
+    S.lpc_strikes = strikes;
     S.ndirs = 0;
     S.fcr_m = mdirs;
     S.slot = mdirs - 1;
@@ -2749,7 +2752,7 @@ __global__ void k_fcr_update(Batch B, const int8_t* flags, const ColliderSet* co
     } else if (!S.lpc_off && tot[0] > bar * S.rr_prev) {
       S.lpc_off = 1;
       S.fallbacks += 1;
-      S.lpc_pause = S.fallbacks < kLpcStrikes ? 1 : 0;
+      S.lpc_pause = S.fallbacks < S.lpc_strikes ? 1 : 0;
     } else if (S.lpc_off && S.lpc_pause) {
       S.lpc_off = 0;  // one diagonal iteration done: back to the network
       S.lpc_pause = 0;
@@ -3419,9 +3422,10 @@ void launch_small_cr(PView P, const Batch& B, const int* list, int n, const floa
 }
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                     int netOn, double ratio, int mdirs, CRScalars* sc, BatchStatus* st, double* partials,
-                     hipStream_t s) {
-  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, ratio, mdirs, sc, st, partials);
+                     int netOn, double ratio, int mdirs, int strikes, CRScalars* sc, BatchStatus* st,
+                     double* partials, hipStream_t s) {
+  NODE_LAUNCH(k_fcr_init, B, mass, flags, cols, stage, x, b, diag, dt, r, netOn, ratio, mdirs, strikes, sc, st,
+              partials);
 }
 void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
                        hipStream_t s) {

@@ -141,8 +141,8 @@ void launch_small_cr(PView P, const Batch& B, const int* list, int n, const floa
 // ---- flexible conjugate residual (learned preconditioner path)
 void launch_fcr_init(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,
                      const float* stage, const float* x, const float* b, const float* diag, float dt, float* r,
-                     int netOn, double ratio, int mdirs, CRScalars* sc, BatchStatus* st, double* partials,
-                     hipStream_t s);
+                     int netOn, double ratio, int mdirs, int strikes, CRScalars* sc, BatchStatus* st,
+                     double* partials, hipStream_t s);
 void launch_fcr_diag_z(const Batch& B, const float* r, const float* diag, float* z, const CRScalars* sc,
                        hipStream_t s);
 void launch_fcr_az(const Batch& B, const float* mass, const int8_t* flags, const ColliderSet* cols,

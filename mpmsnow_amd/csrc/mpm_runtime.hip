@@ -268,6 +268,7 @@ struct mpm_context {
   double lpcFallback = 0.95;  // fallback ratio of a network iteration's |r|^2 to the previous one
   int lpcDirs = 2;            // directions the flexible CR keeps (1..kFcrMax)
   int lpcHandoff = 1;         // 1: hand the solve to the CR recurrences once the network is off
+  int lpcStrikes = kLpcStrikes;  // fallbacks after which a body keeps the diagonal preconditioner
   LpcCapture cap;
   // measurement
   struct ProfRec {
@@ -706,7 +707,7 @@ void GridBatch::solve(float dt, const mpm_solver_params* prm) {
     const int netOn = c.net.initialised ? 1 : 0;
     pair_chunks(B);
     hvp(B, f(x), true, f(stageA), sc(), -1, true);
-    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, c.lpcFallback, mdirs, sc(),
+    launch_fcr_init(B, f(mass), flags.as<int8_t>(), c.cols(), f(stageA), f(x), f(b), f(diag), dt, f(fr), netOn, c.lpcFallback, mdirs, c.lpcStrikes, sc(),
                     st(), partials.as<double>(), s);
     fetch_status_errors();
     bool netActive = netOn != 0;
@@ -1815,6 +1816,13 @@ int mpm_lpc_set_handoff(mpm_context* ctx, int on) {
   return guarded(ctx, [&] {
     if (on != 0 && on != 1) throw InvalidArg("handoff must be 0 or 1");
     ctx->lpcHandoff = on;
+  });
+}
+
+int mpm_lpc_set_strikes(mpm_context* ctx, int strikes) {
+  return guarded(ctx, [&] {
+    if (strikes < 1 || strikes > 16) throw InvalidArg("strikes: 1 to 16 fallbacks");
+    ctx->lpcStrikes = strikes;
   });
 }
 

@@ -242,19 +242,24 @@ def test_fallback_ratio_setting():
 
 def test_handoff_to_cr_after_the_third_fallback():
     """A network with a large random output layer sends the body to the diagonal for good
-    within a few iterations. With the handoff (the default) the rest of the solve runs the CR
+    within a few iterations (here after three fallbacks, mpm_lpc_set_strikes(3); the strike
+    count is validated). With the handoff (the default) the rest of the solve runs the CR
     recurrences from the flexible CR's iterate; without it, GCR(2) on the diagonal. Both meet
     the residual test from identical states; the handoff's history continues the flexible CR's
     (one entry per iteration) and it takes no more iterations than GCR(2) plus 10% (the two are
     the same Krylov iteration in exact arithmetic), and plain CR's plus the iterations before
     the handoff."""
-    say("fcr handoff: contact snowball 20k, random network, handoff on / off / plain CR")
+    say("fcr handoff: contact snowball 20k, random network, 3 strikes, handoff on / off / plain CR")
     out = {}
     for name, mode, ho in (("handoff", 1, 1), ("gcr", 1, 0), ("plain", 0, 1)):
         sc, sim = _small_scene(seed=5)
         for bad in (-1, 2):
             with pytest.raises(G.MpmError):
                 sim.ctx.lpc_handoff(bad)
+        for bad in (0, 17):
+            with pytest.raises(G.MpmError):
+                sim.ctx.lpc_strikes(bad)
+        sim.ctx.lpc_strikes(3)
         sim.ctx.lpc_handoff(ho)
         for _ in range(2):
             sim.advance(sc["dt"], sc["max_iters"], sc["tol"])
