@@ -177,6 +177,33 @@ int mpm_grid_update_particle_velocities(mpm_grid* grid);
 /* updateDeformationGradients (Grid.cpp:1018-1047) */
 int mpm_grid_update_deformation_gradients(mpm_grid* grid, float dt);
 
+/* ---- ConjugateResiduals over a caller's matrix --------------------------------
+ * ConjugateResiduals(termination, preconditioner, log)(mat, rhs, x, debug)
+ * (ConjugateResiduals.h:20-60, ConjugateResiduals.cpp:19-160) for a ProceduralMatrix the caller
+ * implements (ProceduralMatrix.h:8-19): the recurrences run on the GPU over n-vectors (FP32, FP64
+ * inner products in a fixed order); the matrix, the preconditioner and the projection are the
+ * caller's callbacks, called with the library's work complete and expected complete on return. */
+typedef struct {
+  /* result = M x (multVector) */
+  void (*mult)(void* user, const float* x, float* result, int64_t n);
+  /* result = M^-1 x (multInverseVector); required of a preconditioner, unused for the matrix */
+  void (*mult_inverse)(void* user, const float* x, float* result, int64_t n);
+  /* x = P x in place (subspaceProject); NULL: the identity */
+  void (*project)(void* user, float* x, int64_t n);
+  void* user;
+  int device_pointers; /* 1: the callbacks get device pointers on the context's device; 0: host */
+} mpm_procedural_matrix;
+/* b (n floats) is the right hand side; x (n floats) the initial guess, overwritten by the result.
+ * params: SquareMagnitudeTermination(max_iters, tolerance) or the termination hook, and the debug
+ * hook (params->cancelled is not used). precond may be NULL (no preconditioner). With
+ * log_capacity > 0 the residual r and search direction p of the setup and of every iteration that
+ * continues are written to residuals and directions (log_capacity x n floats each; the
+ * reference's m_residuals / m_searchDirections), their count to *logged (may be NULL). */
+int mpm_conjugate_residuals(mpm_context* ctx, const mpm_procedural_matrix* A, const mpm_procedural_matrix* precond,
+                            const float* b, float* x, int64_t n, const mpm_solver_params* params,
+                            mpm_solve_stats* stats, float* residuals, float* directions, int log_capacity,
+                            int* logged);
+
 /* ---- simulation driver: Sim (reference include/MpmSim/Sim.h:32-55) -------- */
 /* Sim::Sim after the particles and model are set (Sim.cpp:17-50): bodies + volumes */
 int mpm_sim_init(mpm_context* ctx, float grid_size, int dimension);

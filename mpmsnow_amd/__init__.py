@@ -54,6 +54,15 @@ class SolveStats(C.Structure):
                 ("restarts", C.c_int)]
 
 
+MATVEC_CB = C.CFUNCTYPE(None, C.c_void_p, _fp, _fp, C.c_int64)
+PROJECT_CB = C.CFUNCTYPE(None, C.c_void_p, _fp, C.c_int64)
+
+
+class ProceduralMatrix(C.Structure):
+    _fields_ = [("mult", MATVEC_CB), ("mult_inverse", MATVEC_CB), ("project", PROJECT_CB), ("user", C.c_void_p),
+                ("device_pointers", C.c_int)]
+
+
 class StepStats(C.Structure):
     _fields_ = [("num_bodies", C.c_int), ("num_ballistic", C.c_int), ("total_iterations", C.c_int),
                 ("max_iterations", C.c_int), ("cancelled", C.c_int), ("num_solved", C.c_int)]
@@ -101,6 +110,9 @@ SIGNATURES = {
                                                   C.POINTER(SolveStats)]),
     "mpm_grid_update_particle_velocities": (C.c_int, [C.c_void_p]),
     "mpm_grid_update_deformation_gradients": (C.c_int, [C.c_void_p, C.c_float]),
+    "mpm_conjugate_residuals": (C.c_int, [C.c_void_p, C.POINTER(ProceduralMatrix), C.POINTER(ProceduralMatrix),
+                                          _fp, _fp, C.c_int64, C.POINTER(SolverParams), C.POINTER(SolveStats),
+                                          _fp, _fp, C.c_int, _ip]),
     "mpm_sim_init": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "mpm_sim_advance": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(SolverParams), C.POINTER(StepStats)]),
     "mpm_sim_num_bodies": (C.c_int, [C.c_void_p]),
@@ -215,6 +227,62 @@ def _stats_dict(st, hist):
             "residual_norm2": hist[:st.history_length].copy() if hist is not None else None,
             "particles": st.particles, "active_nodes": st.active_nodes, "fallbacks": st.fallbacks,
             "restarts": st.restarts}
+
+
+def _procedural(mult, mult_inverse=None, project=None):
+    """A host-pointer ProceduralMatrix over numpy callables: mult(x) -> M x, mult_inverse(x) -> M^-1 x,
+    project(x) -> P x (ProceduralMatrix.h:8-19)."""
+    keep = []
+    m = ProceduralMatrix()
+
+    def _wrap(f):
+        def _cb(user, x, out, n):
+            res = np.asarray(f(np.ctypeslib.as_array(x, shape=(n,)).copy()), dtype=np.float32).reshape(n)
+            np.ctypeslib.as_array(out, shape=(n,))[:] = res
+        cb = MATVEC_CB(_cb)
+        keep.append(cb)
+        return cb
+
+    m.mult = _wrap(mult)
+    if mult_inverse is not None:
+        m.mult_inverse = _wrap(mult_inverse)
+    if project is not None:
+        def _p(user, x, n):
+            v = np.ctypeslib.as_array(x, shape=(n,))
+            v[:] = np.asarray(project(v.copy()), dtype=np.float32).reshape(n)
+        cb = PROJECT_CB(_p)
+        keep.append(cb)
+        m.project = cb
+    m.device_pointers = 0
+    m._keep = keep
+    return m
+
+
+def conjugate_residuals(ctx, mult, b, x0, max_iters, tol, preconditioner=None, project=None, log=0,
+                        termination=None, debug=None, hist_cap=4096):
+    """ConjugateResiduals(termination, preconditioner, log)(mat, rhs, x, debug)
+    (ConjugateResiduals.cpp:19-160) on the GPU over a matrix given by the callable mult (and project,
+    its subspaceProject). preconditioner: (mult, mult_inverse) callables or None. log: capacity of the
+    residual / search-direction logs (0: none). Returns x, the stats and the logs."""
+    bb = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    x = np.ascontiguousarray(x0, dtype=np.float32).ravel().copy()
+    n = bb.size
+    A = _procedural(mult, project=project)
+    P = _procedural(*preconditioner) if preconditioner is not None else None
+    prm = make_params(max_iters, tol, termination, debug)
+    hist = np.zeros(hist_cap)
+    st = SolveStats()
+    st.history_capacity = hist_cap
+    st.history = hist.ctypes.data_as(_dp)
+    res = np.zeros((max(log, 1), n), dtype=np.float32)
+    dirs = np.zeros((max(log, 1), n), dtype=np.float32)
+    logged = C.c_int32(0)
+    ctx.check(lib().mpm_conjugate_residuals(ctx.ptr, C.byref(A), C.byref(P) if P is not None else None,
+                                            bb.ctypes.data_as(_fp), x.ctypes.data_as(_fp), n, C.byref(prm),
+                                            C.byref(st), res.ctypes.data_as(_fp), dirs.ctypes.data_as(_fp),
+                                            int(log), C.byref(logged)))
+    k = logged.value
+    return x, _stats_dict(st, hist), res[:k].copy(), dirs[:k].copy()
 
 
 class Context:

@@ -674,6 +674,55 @@ inline mpm_solver_params solver_params(TerminationCriterion& t, LinearSolver::De
   return p;
 }
 
+}  // namespace detail
+
+// ---- ProceduralMatrix (ProceduralMatrix.h:8-19) ------------------------------------------------
+class ProceduralMatrix {
+ public:
+  virtual ~ProceduralMatrix() {}
+  virtual void multVector(const VectorXf& x, VectorXf& result) const = 0;
+  virtual void multInverseVector(const VectorXf& x, VectorXf& result) const = 0;
+  virtual void subspaceProject(VectorXf& x) const = 0;
+};
+
+namespace detail {
+struct ProcHooks {
+  const ProceduralMatrix* mat;
+  VectorXf in, out;
+};
+inline void proc_mult(void* u, const float* x, float* r, int64_t n) {
+  ProcHooks* h = static_cast<ProcHooks*>(u);
+  h->in.assign(x, x + n);
+  h->out.assign(n, 0.0f);
+  h->mat->multVector(h->in, h->out);
+  if ((int64_t)h->out.size() != n) throw std::runtime_error("multVector returned a vector of the wrong size");
+  std::memcpy(r, h->out.data(), sizeof(float) * n);
+}
+inline void proc_mult_inverse(void* u, const float* x, float* r, int64_t n) {
+  ProcHooks* h = static_cast<ProcHooks*>(u);
+  h->in.assign(x, x + n);
+  h->out.assign(n, 0.0f);
+  h->mat->multInverseVector(h->in, h->out);
+  if ((int64_t)h->out.size() != n) throw std::runtime_error("multInverseVector returned a vector of the wrong size");
+  std::memcpy(r, h->out.data(), sizeof(float) * n);
+}
+inline void proc_project(void* u, float* x, int64_t n) {
+  ProcHooks* h = static_cast<ProcHooks*>(u);
+  h->in.assign(x, x + n);
+  h->mat->subspaceProject(h->in);
+  std::memcpy(x, h->in.data(), sizeof(float) * n);
+}
+inline mpm_procedural_matrix procedural(ProcHooks& h) {
+  mpm_procedural_matrix m;
+  std::memset(&m, 0, sizeof(m));
+  m.mult = proc_mult;
+  m.mult_inverse = proc_mult_inverse;
+  m.project = proc_project;
+  m.user = &h;
+  m.device_pointers = 0;
+  return m;
+}
+
 // uploads colliders when their descriptions changed since the last upload
 struct ColliderCache {
   std::vector<mpm_collider> last;
@@ -689,6 +738,67 @@ struct ColliderCache {
   }
 };
 }  // namespace detail
+
+// ---- ConjugateResiduals (ConjugateResiduals.h:20-60, ConjugateResiduals.cpp:19-160) --------------
+// The recurrences run on the GPU (device 0, a context of its own created on the first solve); the
+// matrix and the preconditioner are called on host vectors once per product. With log set, the
+// residuals and search directions of the setup and of every continuing iteration are kept
+// (the reference's m_residuals / m_searchDirections, public here through residuals() and
+// searchDirections()).
+class ConjugateResiduals : public LinearSolver {
+ public:
+  ConjugateResiduals(TerminationCriterion& terminationCriterion, const ProceduralMatrix* preconditioner = 0,
+                     bool log = false)
+      : m_terminationCriterion(terminationCriterion), m_preconditioner(preconditioner), m_log(log) {}
+  ~ConjugateResiduals() {
+    if (m_ctx) mpm_destroy(m_ctx);
+  }
+  ConjugateResiduals(const ConjugateResiduals&) = delete;
+  ConjugateResiduals& operator=(const ConjugateResiduals&) = delete;
+
+  void operator()(const ProceduralMatrix& mat, const VectorXf& rhs, VectorXf& x, Debug* d = 0) const {
+    const int64_t n = (int64_t)rhs.size();
+    if ((int64_t)x.size() != n) throw std::runtime_error("x and rhs differ in size");
+    if (!m_ctx && mpm_create(&m_ctx, 0) != MPM_OK) {
+      m_ctx = nullptr;
+      throw std::runtime_error("mpm_create failed: no usable HIP device");
+    }
+    detail::ProcHooks hm{&mat, {}, {}};
+    detail::ProcHooks hp{m_preconditioner, {}, {}};
+    mpm_procedural_matrix A = detail::procedural(hm);
+    mpm_procedural_matrix P = detail::procedural(hp);
+    detail::Hooks hooks;
+    mpm_solver_params prm = detail::solver_params(m_terminationCriterion, d, hooks);
+    prm.cancelled = nullptr;
+    mpm_solve_stats st;
+    std::memset(&st, 0, sizeof(st));
+    // the setup's entry and one per iteration that continues
+    const int cap = m_log ? (prm.termination ? 4096 : prm.max_iters + 2) : 0;
+    std::vector<float> res((size_t)cap * n), dirs((size_t)cap * n);
+    int logged = 0;
+    detail::check(m_ctx, mpm_conjugate_residuals(m_ctx, &A, m_preconditioner ? &P : nullptr, rhs.data(), x.data(), n,
+                                                 &prm, &st, cap ? res.data() : nullptr,
+                                                 cap ? dirs.data() : nullptr, cap, &logged));
+    m_iterations = st.iterations;
+    for (int k = 0; k < logged; ++k) {
+      m_residuals.emplace_back(res.begin() + (size_t)k * n, res.begin() + (size_t)(k + 1) * n);
+      m_searchDirections.emplace_back(dirs.begin() + (size_t)k * n, dirs.begin() + (size_t)(k + 1) * n);
+    }
+  }
+
+  const std::vector<VectorXf>& residuals() const { return m_residuals; }
+  const std::vector<VectorXf>& searchDirections() const { return m_searchDirections; }
+  int iterations() const { return m_iterations; }
+
+ private:
+  TerminationCriterion& m_terminationCriterion;
+  const ProceduralMatrix* m_preconditioner;
+  bool m_log;
+  mutable mpm_context* m_ctx = nullptr;
+  mutable int m_iterations = 0;
+  mutable std::vector<VectorXf> m_residuals;
+  mutable std::vector<VectorXf> m_searchDirections;
+};
 
 inline void require_cubic_bspline(const ShapeFunction& sf) {
   if (!dynamic_cast<const CubicBsplineShapeFunction*>(&sf))

@@ -2939,6 +2939,34 @@ __global__ void k_lpc_energy_grad(Batch B, const float* u, const float* r, const
   if (ok && threadIdx.x == 0 && blockIdx.x == (unsigned)c.G.blockBase) atomicAdd(loss, 1.0 - a * a / (cc * E));
 }
 
+// ---- vector kernels of the conjugate-residual solve over a caller's matrix (mpm_conjugate_residuals)
+constexpr int kDotBlocks = 512;
+// partials[block] = sum of a[i] b[i] over the block's grid-stride share, in FP64, fixed order
+__global__ __launch_bounds__(kBlock) void k_vec_partial_dots(const float* a, const float* b, int64_t n,
+                                                             double* partials) {
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)kDotBlocks * kBlock)
+    v += (double)a[i] * (double)b[i];
+  __shared__ double sm[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+}
+__global__ __launch_bounds__(kBlock) void k_vec_sum_partials(const double* partials, double* out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kDotBlocks; i += kBlock) v += partials[i];
+  __shared__ double sm[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+}
+// out = a u + c v (v may be null: out = a u)
+__global__ void k_vec_lincomb(float* out, float a, const float* u, float c, const float* v, int64_t n) {
+  GRID_STRIDE(i, n) out[i] = v ? a * u[i] + c * v[i] : a * u[i];
+}
+
 __global__ void k_residual_vector(Batch B, const float* z, const float* diag, float* r) {
   NodeCtx c = node_begin(B);
   float zz[3], dg[3];
@@ -3476,6 +3504,16 @@ void launch_lpc_energy_grad(const Batch& B, const float* u, const float* r, cons
                             const float* weight, const double* dots, const double* energy, __bf16* g, double* loss,
                             hipStream_t s) {
   NODE_LAUNCH(k_lpc_energy_grad, B, u, r, raw, scale, weight, dots, energy, g, loss);
+}
+void launch_vec_dot(const float* a, const float* b, int64_t n, double* partials, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_partial_dots, dim3(kDotBlocks), dim3(kBlock), 0, s, a, b, n, partials);
+  hipLaunchKernelGGL(k_vec_sum_partials, dim3(1), dim3(kBlock), 0, s, partials, out);
+}
+void launch_vec_lincomb(float* out, float a, const float* u, float c, const float* v, int64_t n, hipStream_t s) {
+  int64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_vec_lincomb, dim3((unsigned)blocks), dim3(kBlock), 0, s, out, a, u, c, v, n);
 }
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s) {
   NODE_LAUNCH(k_residual_vector, B, z, diag, r);

@@ -183,6 +183,10 @@ void launch_lpc_energy_grad(const Batch& B, const float* u, const float* r, cons
                             const float* weight, const double* dots, const double* energy, __bf16* g, double* loss,
                             hipStream_t s);
 void launch_residual_vector(const Batch& B, const float* z, const float* diag, float* r, hipStream_t s);
+// ---- plain vectors (mpm_conjugate_residuals): *out = a . b in FP64 (fixed order; partials: 512
+// doubles), out = a u + c v (v null: a u)
+void launch_vec_dot(const float* a, const float* b, int64_t n, double* partials, double* out, hipStream_t s);
+void launch_vec_lincomb(float* out, float a, const float* u, float c, const float* v, int64_t n, hipStream_t s);
 void launch_finalize(const Batch& B, const float* x, const float* vc, float* vel, const CRScalars* sc,
                      hipStream_t s);
 

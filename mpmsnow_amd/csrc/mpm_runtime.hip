@@ -1635,6 +1635,160 @@ int mpm_grid_update_grid_velocities(mpm_grid* grid, float dt, const mpm_solver_p
   });
 }
 
+// ConjugateResiduals::operator() (ConjugateResiduals.cpp:19-160) over a caller's ProceduralMatrix
+// (ProceduralMatrix.h:8-19): the recurrences of the grid solve on n-vectors in device memory (FP32
+// vectors, FP64 inner products in a fixed order), the matrix, its preconditioner and the projection
+// applied by the caller's callbacks on host or device vectors.
+int mpm_conjugate_residuals(mpm_context* ctx, const mpm_procedural_matrix* A, const mpm_procedural_matrix* precond,
+                            const float* b, float* x, int64_t n, const mpm_solver_params* params,
+                            mpm_solve_stats* stats, float* residuals, float* directions, int log_capacity,
+                            int* logged) {
+  return guarded(ctx, [&] {
+    if (!A || !A->mult) throw InvalidArg("a matrix with a multVector callback is required");
+    if (precond && (!precond->mult || !precond->mult_inverse))
+      throw InvalidArg("a preconditioner needs multVector and multInverseVector callbacks");
+    if (!b || !x || n <= 0) throw InvalidArg("b, x and n > 0 are required");
+    if (!params) throw InvalidArg("solver parameters required");
+    if (log_capacity < 0 || (log_capacity > 0 && (!residuals || !directions)))
+      throw InvalidArg("logs need residual and direction arrays of log_capacity x n floats");
+    mpm_context& c = *ctx;
+    hipStream_t s = c.stream;
+    const size_t bytes = sizeof(float) * (size_t)n;
+    DevBuf vb, vx, vr, vrp, vp, vAp, vAr, vpAp, vtmp, red;
+    for (DevBuf* d : {&vb, &vx, &vr, &vrp, &vp, &vAp, &vAr, &vpAp, &vtmp}) d->ensure(bytes);
+    red.ensure(sizeof(double) * 520);
+    std::vector<float> host(n), host2(n);
+    HIPCHECK(hipMemcpyAsync(vb.p, b, bytes, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(vx.p, x, bytes, hipMemcpyHostToDevice, s));
+    float* db = vb.as<float>();
+    float* dx = vx.as<float>();
+    float* r = vr.as<float>();
+    float* rp = vrp.as<float>();
+    float* p = vp.as<float>();
+    float* Ap = vAp.as<float>();
+    float* Ar = vAr.as<float>();
+    float* pAp = vpAp.as<float>();
+    // one callback: in -> out (out may equal in only for the projection)
+    auto call = [&](const mpm_procedural_matrix* M, int which, const float* in, float* out) {
+      c.sync();
+      if (M->device_pointers) {
+        if (which == 0) M->mult(M->user, in, out, n);
+        else if (which == 1) M->mult_inverse(M->user, in, out, n);
+        else if (M->project) M->project(M->user, out, n);
+        return;
+      }
+      if (which == 2 && !M->project) return;
+      // stream-ordered copies, each waited for: the next kernel on the stream reads `out`
+      HIPCHECK(hipMemcpyAsync(host.data(), which == 2 ? out : in, bytes, hipMemcpyDeviceToHost, s));
+      c.sync();
+      float* res = host2.data();
+      if (which == 2) {
+        M->project(M->user, host.data(), n);
+        res = host.data();
+      } else {
+        (which == 0 ? M->mult : M->mult_inverse)(M->user, host.data(), host2.data(), n);
+      }
+      HIPCHECK(hipMemcpyAsync(out, res, bytes, hipMemcpyHostToDevice, s));
+      c.sync();
+    };
+    auto dot = [&](const float* u, const float* v) {
+      launch_vec_dot(u, v, n, red.as<double>(), red.as<double>() + 512, s);
+      double h = 0.0;
+      HIPCHECK(hipMemcpyAsync(&h, red.as<double>() + 512, sizeof(double), hipMemcpyDeviceToHost, s));
+      c.sync();
+      return h;
+    };
+    auto to_host = [&](const float* v, float* dst) {
+      HIPCHECK(hipMemcpyAsync(dst, v, bytes, hipMemcpyDeviceToHost, s));
+      c.sync();
+    };
+    int nlog = 0;
+    auto log = [&]() {
+      if (nlog >= log_capacity) return;
+      to_host(r, residuals + (int64_t)nlog * n);
+      to_host(p, directions + (int64_t)nlog * n);
+      ++nlog;
+    };
+    std::vector<double> hist;
+    int iterations = 0, early = 0;
+    const double bNorm2 = dot(db, db);
+    double rn = 0.0;
+    if (bNorm2 == 0.0) {  // ConjugateResiduals.cpp:34-38
+      launch_vec_lincomb(dx, 0.0f, db, 0.0f, nullptr, n, s);
+    } else {
+      const double threshold = (double)params->tolerance * (double)params->tolerance * bNorm2;
+      const float maxIters = (float)params->max_iters;
+      call(A, 0, dx, Ap);  // Ax, held in Ap until p exists
+      launch_vec_lincomb(r, 1.0f, db, -1.0f, Ap, n, s);
+      if (precond) call(precond, 1, r, rp);
+      else HIPCHECK(hipMemcpyAsync(rp, r, bytes, hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(p, rp, bytes, hipMemcpyDeviceToDevice, s));
+      call(A, 0, p, Ap);
+      call(A, 0, rp, Ar);
+      double rz = dot(rp, Ar);
+      log();
+      for (int i = 0;; ++i) {
+        if (precond) call(precond, 1, Ap, pAp);
+        else HIPCHECK(hipMemcpyAsync(pAp, Ap, bytes, hipMemcpyDeviceToDevice, s));
+        const double den = dot(Ap, pAp);
+        if (den == 0.0) {
+          early = 1;
+          break;
+        }
+        const double alpha = rz / den;
+        launch_vec_lincomb(dx, 1.0f, dx, (float)alpha, p, n, s);
+        call(A, 2, dx, dx);
+        iterations = i + 1;
+        if (params->debug) {
+          to_host(dx, host.data());
+          params->debug(params->user, host.data(), n);
+        }
+        launch_vec_lincomb(rp, 1.0f, rp, (float)-alpha, pAp, n, s);
+        if (precond) call(precond, 0, rp, r);
+        else HIPCHECK(hipMemcpyAsync(r, rp, bytes, hipMemcpyDeviceToDevice, s));
+        bool stop;
+        if (params->termination) {
+          to_host(r, host.data());
+          stop = params->termination(params->user, host.data(), n, i) != 0;
+        } else if ((float)i >= maxIters) {
+          stop = true;
+        } else {
+          rn = dot(r, r);
+          hist.push_back(rn);
+          stop = rn < threshold;
+        }
+        if (stop) break;
+        call(A, 0, rp, Ar);
+        const double rzOld = rz;
+        rz = dot(rp, Ar);
+        if (rzOld == 0.0) {
+          early = 1;
+          break;
+        }
+        const double beta = rz / rzOld;
+        launch_vec_lincomb(p, 1.0f, rp, (float)beta, p, n, s);
+        launch_vec_lincomb(Ap, 1.0f, Ar, (float)beta, Ap, n, s);
+        log();
+      }
+    }
+    to_host(dx, x);
+    if (logged) *logged = nlog;
+    if (stats) {
+      stats->iterations = iterations;
+      stats->early_exit = early;
+      stats->rhs_norm2 = bNorm2;
+      stats->residual_norm2 = rn;
+      stats->particles = 0;
+      stats->active_nodes = 0;
+      stats->fallbacks = 0;
+      stats->restarts = 0;
+      const int L = (int)hist.size();
+      stats->history_length = stats->history ? std::min(L, stats->history_capacity) : 0;
+      for (int k = 0; k < stats->history_length; ++k) stats->history[k] = hist[k];
+    }
+  });
+}
+
 int mpm_grid_update_particle_velocities(mpm_grid* grid) {
   if (!grid) return MPM_ERR_INVALID;
   return guarded(grid->ctx, [&] {

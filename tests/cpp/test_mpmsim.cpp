@@ -433,6 +433,79 @@ static void testSnowballOnGround() {
   CHECK(mean / v2.size() > 4.0);
 }
 
+// ---------------------------------------------------------------- ConjugateResiduals
+
+// a dense symmetric matrix and its inverse (TestConjugateResiduals.cpp:14-38), row-major
+class DenseMatrix : public ProceduralMatrix {
+ public:
+  DenseMatrix(const std::vector<double>& a, int n) : m_a(a), m_n(n) {}
+  void multVector(const VectorXf& x, VectorXf& result) const override {
+    result.assign(m_n, 0.0f);
+    for (int i = 0; i < m_n; ++i) {
+      double s = 0;
+      for (int j = 0; j < m_n; ++j) s += m_a[i * m_n + j] * x[j];
+      result[i] = (float)s;
+    }
+  }
+  // no preconditioner in the test: the inverse is never asked for
+  void multInverseVector(const VectorXf& x, VectorXf& result) const override { result = x; }
+  void subspaceProject(VectorXf&) const override {}
+
+ private:
+  std::vector<double> m_a;
+  int m_n;
+};
+
+static double dot(const VectorXf& a, const VectorXf& b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += (double)a[i] * b[i];
+  return s;
+}
+
+// TestConjugateResiduals::testSolve (TestConjugateResiduals.cpp:43-102): a random symmetric
+// indefinite 6x6 system, solved in 6 iterations with logging; residual norms decrease, residuals
+// are A-orthogonal and search directions A^2-orthogonal, with the reference's bounds
+static void testConjugateResidualsSolve() {
+  const int n = 6;
+  unsigned seed = 12345;
+  auto rnd = [&]() {  // uniform in [-1, 1], as Eigen's Random
+    seed = seed * 1664525u + 1013904223u;
+    return (float)((seed >> 8) * (1.0 / 16777216.0) * 2.0 - 1.0);
+  };
+  std::vector<float> r0(n * n);
+  for (float& v : r0) v = rnd();
+  std::vector<double> a(n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) a[i * n + j] = (double)(r0[i * n + j] + r0[j * n + i]);
+  DenseMatrix A(a, n);
+  SquareMagnitudeTermination t(n, 0.0f);
+  ConjugateResiduals solver(t, 0, true);
+  VectorXf v(n), result(n, 0.0f);
+  for (float& x : v) x = rnd();
+  solver(A, v, result);
+  VectorXf Ax;
+  A.multVector(result, Ax);
+  double err = 0;
+  for (int i = 0; i < n; ++i) err += (double)(Ax[i] - v[i]) * (Ax[i] - v[i]);
+  std::printf("  |A x - v| = %.3e after %d iterations, %zu logged\n", std::sqrt(err), solver.iterations(),
+              solver.residuals().size());
+  CHECK(std::sqrt(err) < 1e-3);
+  const std::vector<VectorXf>& R = solver.residuals();
+  const std::vector<VectorXf>& P = solver.searchDirections();
+  CHECK((int)R.size() >= n && (int)P.size() >= n);
+  for (int i = 1; i < n; ++i) CHECK(dot(R[i], R[i]) < dot(R[i - 1], R[i - 1]));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      if (i == j) continue;
+      VectorXf ARj, APi, APj;
+      A.multVector(R[j], ARj);
+      CHECK(std::fabs(dot(R[i], ARj)) < 1e-4);
+      A.multVector(P[i], APi);
+      A.multVector(P[j], APj);
+      CHECK(std::fabs(dot(APi, APj)) < 1e-4);
+    }
+}
+
 int main(int argc, char** argv) {
   const bool hostOnly = argc > 1 && std::string(argv[1]) == "--host";
   struct T {
@@ -450,6 +523,7 @@ int main(int argc, char** argv) {
       {"testSnowModel", testSnowModel, true},
       {"testSolverHooks", testSolverHooks, true},
       {"testSnowballOnGround", testSnowballOnGround, true},
+      {"testConjugateResidualsSolve", testConjugateResidualsSolve, true},
   };
   for (const T& t : tests) {
     if (t.gpu && hostOnly) continue;

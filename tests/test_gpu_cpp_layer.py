@@ -1,7 +1,8 @@
 """The MpmSim C++ class layer on the GPU: the test program (tests/cpp/test_mpmsim.cpp) runs the
 reference's TestSimClass cases (initialization counts, three-step advance), TestGrid's splat
-checks, the snow model's invariants, the host termination/debug hooks and a snowball landing
-on a plane, all through MpmSim::Sim / Grid / SnowConstitutiveModel over the C-ABI."""
+checks, the snow model's invariants, the host termination/debug hooks, a snowball landing
+on a plane, all through MpmSim::Sim / Grid / SnowConstitutiveModel over the C-ABI, and
+TestConjugateResiduals::testSolve through MpmSim::ConjugateResiduals over a ProceduralMatrix."""
 import os
 import subprocess
 
@@ -20,5 +21,5 @@ def test_cpp_layer_on_gpu():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     for name in ("testInitialization", "testTimestepAdvance", "testSplatting", "testSnowModel",
-                 "testSolverHooks", "testSnowballOnGround"):
+                 "testSolverHooks", "testSnowballOnGround", "testConjugateResidualsSolve"):
         assert "ok " + name in r.stdout
