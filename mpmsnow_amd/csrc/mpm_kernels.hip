@@ -623,8 +623,14 @@ __global__ void k_gather_slots_checked(const int* slot_of, const int* orig, int 
 }
 
 // ------------------------------------------------------------ body partition (Sim.cpp:136-203)
+// Bucket of a cell: blocks of 4x4x4 cells placed by a spatial hash of the block, the 64 cells of
+// a block in consecutive buckets. In bucket order the particles of a block are contiguous, so a
+// workgroup of the sampling pass holds about one block of neighbouring cells.
 __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t mask) {
-  return ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^ (uint32_t)cz * 83492791u) & mask;
+  const uint32_t block = (uint32_t)(cx >> 2) * 73856093u ^ (uint32_t)(cy >> 2) * 19349663u ^
+                         (uint32_t)(cz >> 2) * 83492791u;
+  const uint32_t local = (uint32_t)(cx & 3) | ((uint32_t)(cy & 3) << 2) | ((uint32_t)(cz & 3) << 4);
+  return ((block << 6) | local) & mask;
 }
 // |x - y|^2 as (d0 d0 + d1 d1) + d2 d2 with every product and sum rounded separately, like the
 // host's Vector3f::squaredNorm: the neighbour test d^2 < h^2 must match it bit for bit (the HIP
@@ -640,24 +646,28 @@ __device__ __forceinline__ void cell_of(const float x[3], float cell, int c[3]) 
 }
 // count[t]++ returning the old value, one atomic per group of lanes with the same t (particles in
 // storage order mostly share a tile with their neighbours, whose lanes would otherwise queue on
-// one counter); every active lane calls. Ranks within a tile follow arrival order either way:
-// k_sort_tiles fixes the final order.
+// one counter); every active lane calls. The groups are found first (one ballot per distinct t)
+// and their leaders then add at once, so a wave waits for one atomic round trip, not one per
+// group. Ranks within a tile follow arrival order either way: k_sort_tiles fixes the final order.
 __device__ __forceinline__ int wave_aggregated_inc(int* count, int t) {
   const int lane = threadIdx.x & 63;
-  int rank = 0;
-  for (;;) {
-    const int leader = __ffsll((long long)__ballot(1)) - 1;
-    const int lt = __shfl(t, leader, 64);
+  unsigned long long todo = __ballot(1);
+  int leader = lane, below = 0, size = 1;
+  while (todo) {
+    const int l = __ffsll((long long)todo) - 1;
+    const int lt = __shfl(t, l, 64);
+    const unsigned long long peers = __ballot(t == lt);
     if (t == lt) {
-      const unsigned long long peers = __ballot(1);
-      int base = 0;
-      if (lane == leader) base = atomicAdd(count + t, __popcll(peers));
-      base = __shfl(base, leader, 64);
-      rank = base + __popcll(peers & ((1ull << lane) - 1ull));
-      break;
+      leader = l;
+      below = __popcll(peers & ((1ull << lane) - 1ull));
+      size = __popcll(peers);
     }
+    todo &= ~peers;
   }
-  return rank;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count + t, size);
+  base = __shfl(base, leader, 64);
+  return base + below;
 }
 __global__ void k_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int* rank, int* count) {
   GRID_STRIDE(si, n) {
@@ -678,16 +688,6 @@ __global__ void k_cc_scatter(PView P, const int* bucket, const int* rank, const 
     sorted[offset[bucket[s]] + rank[s]] = make_float4(P.atu(C_X, s), P.atu(C_X + 1, s), P.atu(C_X + 2, s), __int_as_float(s));
   }
 }
-__device__ __forceinline__ int cc_find(int* parent, int x) {
-  int p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  while (p != x) {
-    int gp = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // path halving
-    x = p;
-    p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return x;
-}
 // Link priority: a bijective integer mix, so hooking follows a pseudo-random order of the
 // slots rather than their (spatially sorted) index order, which would build long chains.
 __device__ __forceinline__ uint32_t cc_priority(int x) {
@@ -701,10 +701,10 @@ __device__ __forceinline__ uint32_t cc_priority(int x) {
 }
 // Parent pointers only ever point into their own set (hooking joins two sets, path halving
 // moves a pointer up its own tree), so any value a pointer ever held names a member of the
-// same set. Reads that only need that property go through the cache (workgroup-scope loads):
-// the device-coherent loads of cc_find are kept for the hooking loop, where progress needs the
-// current roots. Without this, every find of a large component ends on the device-coherent
-// line of its root, and the whole grid queues on that one address.
+// same set, and an ancestor of its node. Every read of the forest therefore goes through the
+// cache (workgroup-scope loads); only the hook itself is a device-scope compare-and-swap.
+// Device-coherent loads would end every find of a large component on the line of its root, where
+// the whole grid queues.
 __device__ __forceinline__ int cc_load_cached(const int* parent, int x) {
   return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -714,6 +714,20 @@ __device__ __forceinline__ int cc_find_cached(const int* parent, int x) {
   while (p != x) {
     x = p;
     p = cc_load_cached(parent, x);
+  }
+  return x;
+}
+// Root search for the hooking loop, through the cache, with path halving. Every value a pointer
+// ever held is an ancestor (in the current forest) of its node, of strictly lower priority, so a
+// stale read only moves the search to a lower ancestor; and a stale "root" is caught by the
+// hook's compare-and-swap, whose returned value continues the search one real level up.
+__device__ __forceinline__ int cc_find_halving(int* parent, int x) {
+  int p = cc_load_cached(parent, x);
+  while (p != x) {
+    const int gp = cc_load_cached(parent, p);
+    if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = p;
+    p = gp;
   }
   return x;
 }
@@ -728,34 +742,121 @@ __device__ __forceinline__ void cc_union(int* parent, int a, int b) {
     if (cc_find_cached(parent, pa) == cc_find_cached(parent, pb)) return;
   }
   while (true) {
-    a = cc_find(parent, a);
-    b = cc_find(parent, b);
+    a = cc_find_halving(parent, a);
+    b = cc_find_halving(parent, b);
     if (a == b) return;
     if (cc_priority(a) < cc_priority(b)) {
       int t = a;
       a = b;
       b = t;
     }
-    int old = atomicCAS(parent + a, a, b);
+    const int old = atomicCAS(parent + a, a, b);
     if (old == a) return;
+    a = old;  // a was no longer a root: continue from its current parent
+  }
+}
+// union-find over a workgroup's particles in LDS (indices into the workgroup's range), hooking
+// by the same priorities as the global forest so that a local root is its set's minimum
+__device__ __forceinline__ int cc_local_find(int* lp, int x) {
+  int p = __hip_atomic_load(lp + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (p != x) {
+    x = p;
+    p = __hip_atomic_load(lp + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  return x;
+}
+__device__ __forceinline__ void cc_local_union(int* lp, const uint32_t* prio, int a, int b) {
+  while (true) {
+    a = cc_local_find(lp, a);
+    b = cc_local_find(lp, b);
+    if (a == b) return;
+    if (prio[a] < prio[b]) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(lp + a, a, b) == a) return;
   }
 }
 // Neighbour pairs at |x_i - x_j|^2 < h^2 in the reference's float arithmetic (KdTree.inl:187-190),
 // linked in two passes after the subgraph-sampling scheme of Sutton et al. (Afforest, IPDPS
-// 2018): pass 1 (SAMPLE) hooks every particle to its first kCcSampleLinks
+// 2018): pass 1 (k_cc_sample, k_cc_join) hooks every particle to its first kCcSampleLinks
 // neighbours, which already joins most of each body, and records whether it has any neighbour;
-// after compression and a sampled vote for the largest component (giant), pass 2 hooks every
-// remaining neighbour of the particles outside the giant component. A
-// particle inside it may skip its pairs: for a pair with one end outside, that end links it; a
-// pair with both ends inside is already joined. Threads walk the cell-sorted order so a wave's
-// candidate cells mostly coincide.
+// after compression and a sampled vote for the largest component (giant), pass 2 (k_cc_link)
+// hooks every remaining neighbour of the particles outside the giant component. A particle inside
+// it may skip its pairs: for a pair with one end outside, that end links it; a pair with both
+// ends inside is already joined. Threads walk the bucket order so a wave's candidate cells mostly
+// coincide.
 constexpr int kCcSampleLinks = 2;
-template <bool SAMPLE>
-__global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                          const float4* sorted, int* parent, int* hasNb, const int* giant) {
+// Pass 1 for one workgroup of kBlock consecutive particles in bucket order (about one block of
+// cells): the sample links inside the workgroup's range are joined in LDS, every particle's
+// global pointer is set to its local root (plain stores: no other workgroup writes pointers in
+// this kernel), and the links that leave the range are kept for k_cc_join.
+__global__ __launch_bounds__(kBlock) void k_cc_sample(int n, float h, float cell, uint32_t mask, const int* offset,
+                                                      const int* count, const float4* sorted, int* parent,
+                                                      int* hasNb, int* out0, int* out1) {
+  __shared__ int lp[kBlock];
+  __shared__ uint32_t prio[kBlock];
+  __shared__ int lslot[kBlock];
   const float r2 = h * h;
-  const int links = SAMPLE ? kCcSampleLinks : INT_MAX;
-  const int g = SAMPLE ? -1 : *giant;
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * kBlock;
+  const int ti = base + t;
+  const bool act = ti < n;
+  float4 me = act ? sorted[ti] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const int s = act ? __float_as_int(me.w) : -1;
+  lp[t] = t;
+  prio[t] = act ? cc_priority(s) : 0xffffffffu;
+  lslot[t] = s;
+  __syncthreads();
+  int ext[kCcSampleLinks];
+  int next = 0, found = 0;
+  for (int k = 0; k < kCcSampleLinks; ++k) ext[k] = -1;
+  if (act) {
+    const float x[3] = {me.x, me.y, me.z};
+    int c[3];
+    cell_of(x, cell, c);
+    for (int dz = -1; dz <= 1 && found < kCcSampleLinks; ++dz)
+      for (int dy = -1; dy <= 1 && found < kCcSampleLinks; ++dy)
+        for (int dx = -1; dx <= 1 && found < kCcSampleLinks; ++dx) {
+          uint32_t hb = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, mask);
+          int o = offset[hb], e = o + count[hb];
+          for (int q = o; q < e && found < kCcSampleLinks; ++q) {
+            float4 y = sorted[q];
+            int j = __float_as_int(y.w);
+            if (j == s) continue;
+            if (sqdist_ref(x, y.x, y.y, y.z) < r2) {
+              if (q >= base && q < base + kBlock) cc_local_union(lp, prio, t, q - base);
+              else ext[next++] = j;
+              ++found;
+            }
+          }
+        }
+    hasNb[s] = found > 0 ? 1 : 0;
+    out0[ti] = ext[0];
+    out1[ti] = ext[1];
+  }
+  __syncthreads();
+  if (act) {
+    const int r = cc_local_find(lp, t);
+    if (r != t) parent[s] = lslot[r];
+  }
+}
+// pass 1's links that left their workgroup's range, joined in the global forest
+__global__ void k_cc_join(int n, const float4* sorted, const int* in0, const int* in1, int* parent) {
+  GRID_STRIDE(ti, n) {
+    const int a = in0[ti], b = in1[ti];
+    if (a < 0 && b < 0) continue;
+    const int s = __float_as_int(sorted[ti].w);
+    if (a >= 0) cc_union(parent, s, a);
+    if (b >= 0) cc_union(parent, s, b);
+  }
+}
+// pass 2: every neighbour pair of the particles outside the giant component
+__global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
+                          const float4* sorted, int* parent, const int* giant) {
+  const float r2 = h * h;
+  const int g = *giant;
   GRID_STRIDE(ti, n) {
     float4 me = sorted[ti];
     const int s = __float_as_int(me.w);
@@ -763,23 +864,18 @@ __global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* 
     const float x[3] = {me.x, me.y, me.z};
     int c[3];
     cell_of(x, cell, c);
-    int found = 0;
-    for (int dz = -1; dz <= 1 && found < links; ++dz)
-      for (int dy = -1; dy <= 1 && found < links; ++dy)
-        for (int dx = -1; dx <= 1 && found < links; ++dx) {
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
           uint32_t hb = cell_hash(c[0] + dx, c[1] + dy, c[2] + dz, mask);
           int o = offset[hb], e = o + count[hb];
-          for (int q = o; q < e && found < links; ++q) {
+          for (int q = o; q < e; ++q) {
             float4 y = sorted[q];
             int j = __float_as_int(y.w);
             if (j == s) continue;
-            if (sqdist_ref(x, y.x, y.y, y.z) < r2) {
-              ++found;
-              cc_union(parent, s, j);
-            }
+            if (sqdist_ref(x, y.x, y.y, y.z) < r2) cc_union(parent, s, j);
           }
         }
-    if (SAMPLE) hasNb[s] = found > 0 ? 1 : 0;
   }
 }
 // every pointer straight to its root
@@ -3236,14 +3332,15 @@ void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* o
   if (n > 0) LAUNCH(k_cc_scatter, grid_for(n, 16384), P, bucket, rank, offset, n, sorted);
 }
 void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const float4* sorted, int* parent, int* hasNb, int* giant, hipStream_t s) {
+                    const float4* sorted, int* parent, int* hasNb, int* giant, int* links0, int* links1,
+                    hipStream_t s) {
   if (n <= 0) return;
-  LAUNCH(k_cc_link<true>, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb,
-         (const int*)nullptr);
+  hipLaunchKernelGGL(k_cc_sample, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, n, h, cell, mask,
+                     offset, count, sorted, parent, hasNb, links0, links1);
+  LAUNCH(k_cc_join, grid_for(n, 16384), n, sorted, (const int*)links0, (const int*)links1, parent);
   LAUNCH(k_cc_compress, grid_for(n, 16384), n, parent);
   hipLaunchKernelGGL(k_cc_vote, dim3(1), dim3(kCcVote), 0, s, n, (const int*)parent, giant);
-  LAUNCH(k_cc_link<false>, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, hasNb,
-         (const int*)giant);
+  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, (const int*)giant);
 }
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s) {

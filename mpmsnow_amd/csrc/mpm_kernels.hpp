@@ -31,7 +31,8 @@ void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* o
 // links neighbour pairs (two sampled passes, compression, a vote for the largest component in
 // giant: one device int)
 void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const float4* sorted, int* parent, int* hasNb, int* giant, hipStream_t s);
+                    const float4* sorted, int* parent, int* hasNb, int* giant, int* links0, int* links1,
+                    hipStream_t s);
 // isRoot[s] = 1 for the root of a component of >= 2 particles; minOrig[root] = smallest caller index
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s);

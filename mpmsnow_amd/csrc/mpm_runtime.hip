@@ -1097,8 +1097,10 @@ void calculate_bodies(mpm_context& c, bool keepErrors = false) {
   launch_cc_scatter(c.view(), c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccOffset.as<int>(), n,
                     c.ccSorted.as<float4>(), s);
   launch_iota(c.ccParent.as<int>(), n, s);
+  // pass 1's outgoing links go to ccBucket and ccRootIdx, free between the scatter and the numbering
   launch_cc_link(n, c.h, cell, mask, c.ccOffset.as<int>(), c.ccCount.as<int>(), c.ccSorted.as<float4>(),
-                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccTot.as<int>() + 3, s);
+                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccTot.as<int>() + 3, c.ccBucket.as<int>(),
+                 c.ccRootIdx.as<int>(), s);
   HIPCHECK(hipMemsetAsync(c.ccIsRoot.p, 0, sizeof(int) * n, s));
   launch_fill_int(c.ccMinOrig.as<int>(), INT_MAX, n, s);
   launch_cc_roots(n, c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.orig_of[c.cur].as<int>(), c.ccIsRoot.as<int>(),
