@@ -5,7 +5,9 @@ work is bit-exact).
 The clouds exercise the linking passes: one dense component holding most particles (the
 sampled vote finds it and its particles skip the second pass), two equal halves (only one can
 be skipped), many mid-size clusters with no dominant one, chains whose spacing straddles h
-(pairs decided by the float distance test), and isolated particles only (no component).
+(pairs decided by the float distance test), isolated particles only (no component), clusters at
+negative cell coordinates, a slab spanning hundreds of the first pass's workgroups, and a wide
+sparse domain whose cell blocks alias in the bucket table.
 """
 import numpy as np
 import pytest
@@ -59,6 +61,18 @@ def clouds():
     out["chains at spacing h"] = np.concatenate(chains).astype(np.float32)
     g = np.stack(np.meshgrid(*[np.arange(12)] * 3, indexing="ij"), -1).reshape(-1, 3)
     out["isolated only"] = (0.1 + 1.5 * h * g).astype(np.float32)
+    # negative cell coordinates (the bucket of a cell takes its 4x4x4 block and the position in it)
+    parts = []
+    for k in range(300):
+        c = rng.uniform(-0.3, 0.3, 3)
+        parts.append(scenes.sphere(500 + k, int(rng.integers(2, 300)), c, float(rng.uniform(0.3, 2.0)) * h))
+    out["300 clusters around the origin"] = np.concatenate(parts)
+    # a slab of many 256-particle workgroups, so that pass 1 joins most links across workgroups
+    slab = scenes.box(7, 120000, (0.2, 0.0, 0.3), (0.8, 0.1, 0.7))
+    out["slab, 120k particles"] = np.concatenate([slab, rng.uniform(0, 1, (200, 3)).astype(np.float32)])
+    # a wide sparse domain: the blocks alias in the bucket table
+    wide = rng.uniform(-10.0, 10.0, (3000, 3)).astype(np.float32)
+    out["sparse wide domain + sphere"] = np.concatenate([wide, scenes.sphere(8, 5000, (0.0, 0.0, 0.0), 0.05)])
     return out, h
 
 
