@@ -829,7 +829,7 @@ def main():
     ap.add_argument("--lpc-fallback", type=float, default=None, help="fallback ratio of the learned legs")
     ap.add_argument("--lpc-dirs", type=int, default=None, help="directions the learned legs' flexible CR keeps")
     ap.add_argument("--lpc-strikes", type=int, default=None,
-                    help="fallbacks after which a body of the learned legs keeps the diagonal (default 3)")
+                    help="fallbacks after which a body of the learned legs keeps the diagonal (default 1)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_CSV", "BENCH_LOG"))

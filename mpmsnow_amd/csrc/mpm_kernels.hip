@@ -53,6 +53,13 @@ __device__ __forceinline__ int foot_lds(int j) {
   foot_xyz(j, x, y, z);
   return foot_index(x, y, z);
 }
+// A chunk's particles have their base node in the chunk's tile (local 0-3; the scatters check
+// it), so their stencils reach at most 3 nodes into each next tile: footprint node (f, l) can
+// receive a contribution only where every axis shifted into the next tile (bit of f set) has a
+// local coordinate below 3. 343 of the 512 staged nodes; the others are never written or read.
+__device__ __forceinline__ bool foot_live(int f, int l) {
+  return !(((f & 1) && (l & 3) == 3) || ((f & 2) && ((l >> 2) & 3) == 3) || ((f & 4) && (l >> 4) == 3));
+}
 // grid node position h i + min with the reference's two roundings (no fused multiply-add)
 __device__ __forceinline__ void node_pos(const BodyGrid& G, int i, int j, int k, float x[3]) {
 #pragma clang fp contract(off)
@@ -208,6 +215,7 @@ __device__ __forceinline__ void fx_store(const FxFoot<NC>& a, const FxScale& sc,
   const double inv = (double)sc.invS;
   for (int j = threadIdx.x; j < kFoot; j += kBlock) {
     const int f = j >> 6, l = j & 63, i = foot_lds(j);
+    if (!foot_live(f, l)) continue;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       double v = ((double)a.hi[c][i] * 4194304.0 + (double)a.lo[c][i]) * inv;
@@ -261,6 +269,7 @@ __device__ __forceinline__ void fx_store(const FxFoot1<3>& a, const FxScale3& sc
   float* dst = stage + (int64_t)slot * kStage;
   for (int j = threadIdx.x; j < kFoot; j += kBlock) {
     const int f = j >> 6, l = j & 63, i = foot_lds(j);
+    if (!foot_live(f, l)) continue;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       dst[f * 192 + c * 64 + l] = sc.bad ? __int_as_float(0x7fc00000) : (float)a.v[c][i] * sc.invS[c];
@@ -342,7 +351,7 @@ __device__ __forceinline__ float gather_node(const Batch& B, const BodyGrid& G, 
   float v = 0.0f;
 #pragma unroll
   for (int f = 0; f < 8; ++f) {
-    int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
+    int ts = foot_live(f, l) ? tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2)) : -1;
     if (ts < 0) continue;
     int c0 = B.tileChunk0[ts], cn = B.tileChunkN[ts];
     for (int q = 0; q < cn; ++q) v += stage[(int64_t)(c0 + q) * kStage + f * 192 + c * 64 + l];
@@ -350,7 +359,8 @@ __device__ __forceinline__ float gather_node(const Batch& B, const BodyGrid& G, 
   return v;
 }
 // tc0 / tcn: first chunk and chunk count of each tile from global tile tileOff on; stage holds
-// the chunks from the one numbered tc0 = 0 on. The chunk tables of all 8 source tiles and the
+// the chunks from the one numbered tc0 = 0 on. Only the source tiles whose footprint reaches node
+// l are read (foot_live). The chunk tables of all 8 source tiles and the
 // first chunk of each are loaded before any sum (most tiles have one chunk), so a node waits for
 // two rounds of memory latency instead of sixteen; the sums still run source tile by source
 // tile and chunk by chunk, in the same order as a plain double loop.
@@ -361,7 +371,7 @@ __device__ __forceinline__ void gather3_tab(const BodyGrid& G, const int* tc0, c
   int c0[8], cn[8];
 #pragma unroll
   for (int f = 0; f < 8; ++f) {
-    const int ts = tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2));
+    const int ts = foot_live(f, l) ? tile_id(G, tx - (f & 1), ty - ((f >> 1) & 1), tz - (f >> 2)) : -1;
     c0[f] = 0;
     cn[f] = 0;
     if (ts >= 0) {
