@@ -133,14 +133,15 @@ __device__ __forceinline__ bool chunk_stencil(const ChunkCtx& c, const float x[3
   return rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4;
 }
 // load a batch 3-vector footprint into LDS (optionally projected); d and flags hold tiles from
-// global tile tileOff on (0: the whole batch)
+// global tile tileOff on (0: the whole batch). Nodes no stencil of the chunk reaches (foot_live)
+// are set to zero without a load.
 __device__ __forceinline__ void load_foot3(const ChunkCtx& c, const int* ft, const float* d, float (*sd)[kFootLds],
                                            const int8_t* flags, const ColliderSet* cols, bool proj,
                                            int tileOff = 0) {
   for (int j = threadIdx.x; j < kFoot; j += kBlock) {
     const int gt = ft[j >> 6], l = j & 63, i = foot_lds(j);
     float v[3] = {0.0f, 0.0f, 0.0f};
-    if (gt >= 0) {
+    if (gt >= 0 && foot_live(j >> 6, l)) {
       int64_t b = (int64_t)(gt - tileOff) * 192 + l;
       v[0] = d[b];
       v[1] = d[b + 64];
