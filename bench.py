@@ -485,8 +485,9 @@ def pmc_summary(fetch_csv, write_csv, out_path, workload="slab"):
                                                   res["FETCH_SIZE"]["leg_bytes_per_launch"],
                                                   res["WRITE_SIZE"]["leg_bytes_per_launch"], traffic)]
     if algo:
-        lines += ["", "Algorithmic bytes per HVP launch of the same run (its HIP-event sample of the learned "
-                  "leg): %.4g; traffic / algorithmic = %.2f." % (algo, traffic / algo)]
+        lines += ["", "Algorithmic bytes per HVP launch of the same run (its HIP-event sample of %s): %.4g; "
+                  "traffic / algorithmic = %.2f." % ("the learned leg" if workload == "slab" else "the timed steps",
+                                                     algo, traffic / algo)]
     lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
     with open(out_path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
@@ -715,11 +716,16 @@ def trace_summary(stats_csv, bench_log, out_path, workload="slab"):
         "",
     ]
     if timed:
-        lines += ["HVP dispatches of the learned leg alone (%d, those between the markers bench.py launches around "
-                  "its timed steps, in `run_kernel_trace.csv`): mean %.4f ms (rocprofv3) against the HIP events' %.4f ms "
-                  "(ratio %.3f). The run's mean above also covers the warm-up, whose first step solves with the "
-                  "particle store still in input order and runs to the iteration cap, and the plain leg." % (timed["launches"], timed["ms"], ev_ms,
-                                                       timed["ms"] / ev_ms if ev_ms else float("nan")), ""]
+        which, rest = (("the learned leg alone", "the warm-up, whose first step solves with the particle store "
+                        "still in input order and runs to the iteration cap, and the plain leg")
+                       if workload == "slab" else
+                       ("the timed steps alone", "the warm-up, whose first step solves with the particle store "
+                        "still in input order"))
+        lines += ["HVP dispatches of %s (%d, those between the markers bench.py launches around its timed steps, "
+                  "in `run_kernel_trace.csv`): mean %.4f ms (rocprofv3) against the HIP events' %.4f ms (ratio %.3f). "
+                  "The run's mean above also covers %s."
+                  % (which, timed["launches"], timed["ms"], ev_ms, timed["ms"] / ev_ms if ev_ms else float("nan"),
+                     rest), ""]
     lines += [
         "| kernel | calls | total ms | mean us | share |",
         "|---|---|---|---|---|",
