@@ -37,6 +37,8 @@ Other modes:
                        collider, E0 = 4.8e5 Pa, h = 1/512, dt = 2e-3): plain vs learned CR
                        from the same warmed-up state; one JSON line (--stiff-particles,
                        --stiff-steps, --stiff-warmup)
+  --stiff-summary J    the plain vs learned table of a --stiff run's JSON line J
+                       (profiles/stiff_drop_16M_plain_vs_learned.md)
   --lpc-weights W, --lpc-fallback R, --lpc-dirs M
                        network parameters (.npy), fallback ratio and flexible-CR
                        direction count of the learned legs (defaults: the committed
@@ -787,6 +789,83 @@ def stiff_trace_summary(stats_csv, stiff_json, out_path):
         fh.write("\n".join(lines) + "\n")
 
 
+STIFF_COMPARE_FILE = os.path.join(ROOT, "profiles", "stiff_drop_16M_plain_vs_learned.md")
+
+
+def stiff_summary(stiff_json, out_path):
+    """Plain vs learned table of a `bench.py --stiff` run (its JSON line): per range of steps, before
+    and from the first step whose body count differs from step 0's in either run (the onset of
+    fragmentation), the throughput, CR iterations, fallback triggers and capped solves, and a
+    per-step table (every 4th step, and the last 24)."""
+    d = None
+    for line in open(stiff_json):
+        if line.strip().startswith("{"):
+            d = json.loads(line)
+    if d is None or "plain" not in d or "learned" not in d:
+        raise SystemExit("no plain + learned JSON line in %s" % stiff_json)
+    legs = ("plain", "learned")
+    steps = {k: d[k]["steps"] for k in legs}
+    ns = min(len(steps[k]) for k in legs)
+    n = int(re.search(r"(\d+) particles", d["config"]).group(1))
+    split = ns
+    for i in range(ns):
+        if any(steps[k][i]["bodies"] != steps[k][0]["bodies"] for k in legs):
+            split = i
+            break
+
+    def agg(k, lo, hi):
+        rows = steps[k][lo:hi]
+        ms = sum(r["ms"] for r in rows)
+        m = len(rows)
+        return ("| %s | %d-%d | %.3g | %.1f | %.0f | %.0f | %.1f | %d | %d |"
+                % (k, lo, hi - 1, n * m / (ms * 1e-3), ms / m, sum(r["iters_sum"] for r in rows) / m,
+                   sum(r["iters_max"] for r in rows) / m, sum(r["fallbacks"] for r in rows) / m,
+                   sum(r["capped"] for r in rows), max(r["bodies"] for r in rows)), n * m / (ms * 1e-3))
+
+    ranges = [(0, split), (split, ns), (0, ns)] if 0 < split < ns else [(0, ns)]
+    table, rate = [], {}
+    for lo, hi in ranges:
+        for k in legs:
+            line, v = agg(k, lo, hi)
+            table.append(line)
+            rate[(k, lo, hi)] = v
+    ratios = ["%.2f over steps %d-%d" % (rate[("learned", lo, hi)] / rate[("plain", lo, hi)], lo, hi - 1)
+              for lo, hi in ranges]
+    first_cap = {k: next((i for i, r in enumerate(steps[k]) if r["capped"] > 0), None) for k in legs}
+    lines = ["# Stiff stress case, 16M particles, %d steps: plain diagonal CR vs learned flexible CR (MI355X)" % ns,
+             "", "Commands (repository root):", "",
+             "    python -u bench.py --stiff --stiff-steps %d --stiff-warmup %d > <JSON>" % (ns, d["warmup"]),
+             "    python bench.py --stiff-summary <JSON>", "",
+             "Workload: %s. Each solver runs its own steps from the seeded initial state; ms are wall time per "
+             "`Sim::advance` (synchronised)." % d["config"], "",
+             "Objects (seed 4): " + "; ".join("%s %.3fx%.3fx%.3f m at (%.3f, %.3f, %.3f), %d particles"
+                                              % (o["kind"], *o["size"], *o["center"], o["count"])
+                                              for o in d["objects"]) + ".", "",
+             "| Solver | steps | particle-steps/s | ms/step | CR iterations/step (sum over bodies) | max over bodies "
+             "| fallback triggers/step | capped solves | max bodies |",
+             "|---|---|---|---|---|---|---|---|---|"] + table + [
+             "", "Learned / plain particle-steps/s: " + ", ".join(ratios) + ".", "",
+             "First step with a solve at the iteration cap: plain %s, learned %s. Body count first differs from "
+             "step 0's at step %s." % (first_cap["plain"], first_cap["learned"], split if split < ns else "none"),
+             "", "Per step (every 4th step, and the last 24):", "",
+             "| step | plain ms | bodies | it sum | it max | capped | learned ms | bodies | it sum | it max | "
+             "fallbacks | capped |", "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for i in range(ns):
+        if i % 4 and i < ns - 24:
+            continue
+        a, b = steps["plain"][i], steps["learned"][i]
+        lines.append("| %d | %.0f | %d | %d | %d | %d | %.0f | %d | %d | %d | %d | %d |"
+                     % (i, a["ms"], a["bodies"], a["iters_sum"], a["iters_max"], a["capped"], b["ms"], b["bodies"],
+                        b["iters_sum"], b["iters_max"], b["fallbacks"], b["capped"]))
+    summary = {"plain": d["plain"]["value"], "learned": d["learned"]["value"], "split": split,
+               "ratio_before_split": rate[("learned", 0, split)] / rate[("plain", 0, split)] if 0 < split < ns
+               else rate[("learned", 0, ns)] / rate[("plain", 0, ns)]}
+    lines += ["", "<!-- json: %s -->" % json.dumps(summary)]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(json.dumps(summary))
+
+
 def read_footer(path):
     """The JSON footer of a committed profiles/ summary, or None."""
     if not os.path.exists(path):
@@ -831,6 +910,8 @@ def main():
     ap.add_argument("--stiff-warmup", type=int, default=12)
     ap.add_argument("--stiff-plain-only", action="store_true", help="stiff case: the plain CR leg only")
     ap.add_argument("--stiff-trace-summary", nargs=2, metavar=("STATS_CSV", "STIFF_JSON"))
+    ap.add_argument("--stiff-summary", metavar="STIFF_JSON",
+                    help="write the plain vs learned table of a --stiff run's JSON line to profiles/")
     ap.add_argument("--lpc-weights", default=None, help="network parameters (.npy) for the learned legs")
     ap.add_argument("--lpc-fallback", type=float, default=None, help="fallback ratio of the learned legs")
     ap.add_argument("--lpc-dirs", type=int, default=None, help="directions the learned legs' flexible CR keeps")
@@ -848,6 +929,9 @@ def main():
         return
     if args.stiff_trace_summary:
         stiff_trace_summary(args.stiff_trace_summary[0], args.stiff_trace_summary[1], STIFF_STATS_FILE)
+        return
+    if args.stiff_summary:
+        stiff_summary(args.stiff_summary, STIFF_COMPARE_FILE)
         return
     if args.sq_summary:
         sq_summary(args.sq_summary, profile_file("sq", wl), workload=wl)

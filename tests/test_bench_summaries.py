@@ -108,3 +108,27 @@ def test_learned_window_from_markers():
     assert bench.learned_window(rows) == (10, 20)
     assert bench.learned_window([r for r in rows if "arange" not in r[0]]) == (12, 12)
     assert bench.learned_window([("void mpm::k_hvp_pair<true>(x)", 5)]) is None
+
+
+def test_stiff_summary_ranges_and_rates(tmp_path):
+    """The stiff table splits at the first step whose body count differs from step 0's and computes
+    particle-steps/s from the steps' own wall times."""
+    def leg(ms, fallbacks):
+        return {"value": 0.0, "ms_per_step": ms, "steps": [
+            {"ms": ms, "bodies": 8 if i < 6 else 9, "iters_sum": 100 + i, "iters_max": 50, "fallbacks": fallbacks,
+             "capped": 1 if i == 7 else 0} for i in range(10)]}
+    d = {"plain": leg(500.0, 0), "learned": leg(1000.0, 2),
+         "config": "stiff stress case: 1000 particles in 8 seeded boxes and spheres (seed 4)",
+         "objects": [{"kind": "box", "center": [0.1, 0.2, 0.3], "size": [0.1, 0.1, 0.1], "count": 1000}],
+         "warmup": 0}
+    js = tmp_path / "stiff.json"
+    js.write_text("[bench] progress\n" + json.dumps(d) + "\n")
+    out = str(tmp_path / "stiff.md")
+    bench.stiff_summary(str(js), out)
+    s = footer(out)
+    assert s["split"] == 6
+    assert abs(s["ratio_before_split"] - 0.5) < 1e-12
+    text = open(out).read()
+    # 1000 particles per 0.5 s step = 2000 particle-steps/s over steps 0-5
+    assert "| plain | 0-5 | 2e+03 | 500.0 | 102 | 50 | 0.0 | 0 | 8 |" in text
+    assert "| learned | 6-9 | 1e+03 | 1000.0 | 108 | 50 | 2.0 | 1 | 9 |" in text
