@@ -156,7 +156,7 @@ def test_fp64_slope_slab_matches_fp64_oracle(oracle):
 
 def test_fp32_step_100_contact_steps_against_the_fp64_step():
     """The FP32-bulk step (mpm_sim_advance) against the FP64 step over 100 steps of a snowball
-    (config 0's: 20k particles, r = 0.1 m, h = 1/64, dt = 1e-3) driven into the frictional ground,
+    (the snowball-drop scene: 20k particles, r = 0.1 m, h = 1/64, dt = 1e-3) driven into the frictional ground,
     so that the solves run in contact from step 10 on. Positions: 99th percentile within 1e-4 h
     (measured 3.3e-5 h), median within 2e-5 h, maximum within 5e-4 h (measured 1.5e-4 h, a few
     particles at the contact's edge); F: 99th percentile within 1e-4 (measured 2.4e-5)."""

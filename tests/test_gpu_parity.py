@@ -411,7 +411,7 @@ def oracle_sim(oracle, prec, x, m, v, h, planes, gravity, mat):
 
 
 def test_snowball_drop_100_steps_against_fp64(oracle):
-    """Snowball drop (config 0 geometry at 4000 particles, no ground contact within 100 steps):
+    """Snowball drop (the 20k-particle drop's geometry at 4000 particles, no ground contact within 100 steps):
     positions within 1e-4 h and F within 1e-5 of the FP64 oracle after 100 steps."""
     from mpmsnow_amd import scenes
 
