@@ -337,6 +337,40 @@ __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f
     }
 }
 
+// One z-plane k (0-3, uniform across a wave) of sep_gather and sep_scatter: the same terms, in the
+// same order, so that four waves that take a plane each reproduce the one-thread results bitwise.
+__device__ __forceinline__ float pick4(const float v[4], int k) {
+  return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+}
+__device__ __forceinline__ void sep_gather_plane(const float* v, const Stencil& st, int k, float e[3]) {
+  e[0] = e[1] = e[2] = 0.0f;
+  const float* vk = v + kFootSZ * k;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float d = vk[i + 8 * j];
+      a0 = fmaf(d, st.w[0][i], a0);
+      a1 = fmaf(d, st.dw[0][i], a1);
+    }
+    e[0] = fmaf(a1, st.w[1][j], e[0]);
+    e[1] = fmaf(a0, st.dw[1][j], e[1]);
+    e[2] = fmaf(a0, st.w[1][j], e[2]);
+  }
+}
+__device__ __forceinline__ void sep_scatter_plane(int* acc, const Stencil& st, int k, float f0, float f1, float f2) {
+  const float wz = pick4(st.w[2], k), dwz = pick4(st.dw[2], k);
+  int* ak = acc + kFootSZ * k;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float P = f0 * (st.w[1][j] * wz);
+    const float Q = fmaf(f1, st.dw[1][j] * wz, f2 * (st.w[1][j] * dwz));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) atomicAdd(&ak[i + 8 * j], cvt_round(fmaf(P, st.dw[0][i], Q * st.w[0][i])));
+  }
+}
+
 __device__ __forceinline__ float abs_sum9(const float m[9]) {
   float s = 0.0f;
 #pragma unroll
@@ -1806,6 +1840,93 @@ __device__ __forceinline__ FxScale3 hvp_particles(const PView& P, const Batch& B
   __syncthreads();
   return scl;
 }
+// LDS work area of hvp_particles_split: each plane's gather terms, and the scaled rows of fm
+struct HvpSplitLds {
+  float part[4][9][64];
+  float fms[9][64];
+};
+// hvp_particles for a chunk of at most 64 particles (every thread calls): particle k is taken by
+// thread k of each of the four waves, wave w doing z-plane w of its stencil's gather and scatter,
+// and the first wave the rest, so that a particle's serial work is about a third. The gather
+// planes are combined in sep_gather's order and the scatter values are the same expressions, so
+// the staged footprint is bitwise that of hvp_particles.
+__device__ __forceinline__ FxScale3 hvp_particles_split(const PView& P, const Batch& B, const ChunkCtx& c,
+                                                        float (*sd)[kFootLds], FxFoot1<3>& acc, int* err,
+                                                        HvpSplitLds& W) {
+  const int lane = threadIdx.x & 63, plane = threadIdx.x >> 6;
+  bool have = lane < c.ch.z;
+  int s = -1;
+  float x[3] = {0.0f, 0.0f, 0.0f};
+  if (have) {
+    s = B.perm[c.ch.y + lane];
+    load_x(P, s, x);
+  }
+  Stencil st;
+  int base = 0;
+  if (have) {
+    make_stencil_fast(x, c.G.gmin, c.G.h, 1.0f / c.G.h, st);
+    const int rx = st.base[0] - c.ox, ry = st.base[1] - c.oy, rz = st.base[2] - c.oz;
+    if (!(rx >= 0 && rx < 4 && ry >= 0 && ry < 4 && rz >= 0 && rz < 4)) {
+      if (plane == 0) atomicOr(err, ERR_STENCIL);
+      have = false;
+    } else {
+      base = foot_index(rx, ry, rz);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float e[3];
+        sep_gather_plane(&sd[a][base], st, plane, e);
+        W.part[plane][3 * a][lane] = e[0];
+        W.part[plane][3 * a + 1][lane] = e[1];
+        W.part[plane][3 * a + 2][lane] = e[2];
+      }
+    }
+  }
+  __syncthreads();
+  const bool first = have && plane == 0;
+  float fm[9];
+  if (first) {
+    float G[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        o0 = fmaf(W.part[k][3 * a][lane], st.w[2][k], o0);
+        o1 = fmaf(W.part[k][3 * a + 1][lane], st.w[2][k], o1);
+        o2 = fmaf(W.part[k][3 * a + 2][lane], st.dw[2][k], o2);
+      }
+      G[a] = -o0;
+      G[a + 3] = -o1;
+      G[a + 6] = -o2;
+    }
+    float F[9], R[9], Fi[9], Gi[9], dF[9], dP[9];
+    load_m9(P, C_F, s, F);
+    load_m9(P, C_R, s, R);
+    load_m9(P, C_FI, s, Fi);
+    load_m9(P, C_GI, s, Gi);
+    mat_mul(G, F, dF);
+    snow_stress_differential(dF, R, Fi, Gi, P.atu(C_J, s), P.atu(C_MU, s), P.atu(C_LAM, s), dP);
+    const float vol = P.atu(C_VOL, s);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dP[i] = vol * dP[i];
+    mat_mul_bt(dP, F, fm);
+  }
+  float bound[3];
+  hvp_bound(c, first, fm, bound);
+  const FxScale3 scl = fx_scale3(bound, kFx1Top);
+  if (first) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) W.fms[q][lane] = fm[q] * scl.S[q % 3];
+  }
+  __syncthreads();
+  if (have) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      sep_scatter_plane(&acc.v[a][base], st, plane, W.fms[a][lane], W.fms[a + 3][lane], W.fms[a + 6][lane]);
+  }
+  __syncthreads();
+  return scl;
+}
 // ch = B.chunks[chunk], pre = hvp_prefetch(P, B, ch)
 template <bool PROJ>
 __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, int4 ch, const HvpPre& pre,
@@ -2340,6 +2461,7 @@ struct TinyLds {
   int tc0[kTinyTiles], tcn[kTinyTiles];  // each tile's first chunk (relative to the body's) and count
   int4 ch[kTinyChunks];
   int ft[kTinyChunks][8];               // footprint tiles of each chunk (global ids, -1 outside)
+  HvpSplitLds split;                    // chunks of at most 64 particles: hvp_particles_split
 };
 template <int NV> struct SmallRed {
   double w[kBlock / 64][kSmallBlocks][NV];
@@ -2431,7 +2553,9 @@ __device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const B
     fx_zero(acc);
     load_foot3(c, T.ft[q], T.z, sd, T.flags, cols, true, G.tileBase);
     __syncthreads();
-    const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err, hvp_prefetch(P, B, c.ch));
+    // a chunk of at most 64 particles spreads each particle over the four waves
+    const FxScale3 scl = c.ch.z <= 64 ? hvp_particles_split(P, B, c, sd, acc, err, T.split)
+                                      : hvp_particles(P, B, c, sd, acc, err, hvp_prefetch(P, B, c.ch));
     fx_store(acc, scl, T.stage, q);
   }
   __syncthreads();
