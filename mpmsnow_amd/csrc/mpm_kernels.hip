@@ -1850,16 +1850,37 @@ struct HvpSplitLds {
 // and the first wave the rest, so that a particle's serial work is about a third. The gather
 // planes are combined in sep_gather's order and the scatter values are the same expressions, so
 // the staged footprint is bitwise that of hvp_particles.
+// pre: this lane's particle (slot, position) when the caller holds it (the tiny solver keeps
+// them across iterations), else loaded here. The first wave requests the particle's matrices
+// before the gather, so that their latency overlaps it.
 __device__ __forceinline__ FxScale3 hvp_particles_split(const PView& P, const Batch& B, const ChunkCtx& c,
                                                         float (*sd)[kFootLds], FxFoot1<3>& acc, int* err,
-                                                        HvpSplitLds& W) {
+                                                        HvpSplitLds& W, const HvpPre* pre = nullptr) {
   const int lane = threadIdx.x & 63, plane = threadIdx.x >> 6;
   bool have = lane < c.ch.z;
   int s = -1;
   float x[3] = {0.0f, 0.0f, 0.0f};
   if (have) {
-    s = B.perm[c.ch.y + lane];
-    load_x(P, s, x);
+    if (pre) {
+      s = pre->s;
+      x[0] = pre->x[0];
+      x[1] = pre->x[1];
+      x[2] = pre->x[2];
+    } else {
+      s = B.perm[c.ch.y + lane];
+      load_x(P, s, x);
+    }
+  }
+  float F[9], R[9], Fi[9], Gi[9], Jp = 0.0f, mu = 0.0f, lam = 0.0f, vol = 0.0f;
+  if (have && plane == 0) {
+    load_m9(P, C_F, s, F);
+    load_m9(P, C_R, s, R);
+    load_m9(P, C_FI, s, Fi);
+    load_m9(P, C_GI, s, Gi);
+    Jp = P.atu(C_J, s);
+    mu = P.atu(C_MU, s);
+    lam = P.atu(C_LAM, s);
+    vol = P.atu(C_VOL, s);
   }
   Stencil st;
   int base = 0;
@@ -1899,14 +1920,9 @@ __device__ __forceinline__ FxScale3 hvp_particles_split(const PView& P, const Ba
       G[a + 3] = -o1;
       G[a + 6] = -o2;
     }
-    float F[9], R[9], Fi[9], Gi[9], dF[9], dP[9];
-    load_m9(P, C_F, s, F);
-    load_m9(P, C_R, s, R);
-    load_m9(P, C_FI, s, Fi);
-    load_m9(P, C_GI, s, Gi);
+    float dF[9], dP[9];
     mat_mul(G, F, dF);
-    snow_stress_differential(dF, R, Fi, Gi, P.atu(C_J, s), P.atu(C_MU, s), P.atu(C_LAM, s), dP);
-    const float vol = P.atu(C_VOL, s);
+    snow_stress_differential(dF, R, Fi, Gi, Jp, mu, lam, dP);
 #pragma unroll
     for (int i = 0; i < 9; ++i) dP[i] = vol * dP[i];
     mat_mul_bt(dP, F, fm);
@@ -1928,24 +1944,30 @@ __device__ __forceinline__ FxScale3 hvp_particles_split(const PView& P, const Ba
   return scl;
 }
 // ch = B.chunks[chunk], pre = hvp_prefetch(P, B, ch)
+// split: the small-body solver's work area; a chunk of at most 64 particles then runs
+// hvp_particles_split (same staged bits)
 template <bool PROJ>
 __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, int4 ch, const HvpPre& pre,
                                           const float* d, const int8_t* flags, const ColliderSet* cols, float* stage,
-                                          int* err, float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
+                                          int* err, float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft,
+                                          HvpSplitLds* split = nullptr) {
   ChunkCtx c = chunk_begin_at(B, ft, ch);
   fx_zero(acc);
   __syncthreads();
   load_foot3(c, ft, d, sd, flags, cols, PROJ);
   __syncthreads();
-  const FxScale3 scl = hvp_particles(P, B, c, sd, acc, err, pre);
+  const FxScale3 scl = (split && ch.z <= 64) ? hvp_particles_split(P, B, c, sd, acc, err, *split)
+                                             : hvp_particles(P, B, c, sd, acc, err, pre);
   fx_store(acc, scl, stage, chunk);
 }
 template <bool PROJ>
 __device__ __forceinline__ void hvp_chunk(const PView& P, const Batch& B, int chunk, const float* d,
                                           const int8_t* flags, const ColliderSet* cols, float* stage, int* err,
-                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft) {
+                                          float (*sd)[kFootLds], FxFoot1<3>& acc, int* ft,
+                                          HvpSplitLds* split = nullptr) {
   const int4 ch = B.chunks[chunk];
-  hvp_chunk<PROJ>(P, B, chunk, ch, hvp_prefetch(P, B, ch), d, flags, cols, stage, err, sd, acc, ft);
+  hvp_chunk<PROJ>(P, B, chunk, ch, split && ch.z <= 64 ? HvpPre{} : hvp_prefetch(P, B, ch), d, flags, cols, stage,
+                  err, sd, acc, ft, split);
 }
 // The chunk's particle slots and positions are requested first, beside the done flag of its
 // body: the chain of dependent loads before the stencil work is the chunk, then the slot, body
@@ -2537,7 +2559,7 @@ __device__ __forceinline__ void tiny_sweep(const BodyGrid& G, SmallRed<NV>& R, F
 // HVP of the LDS vector T.z over the body's chunks into T.stage (every thread calls)
 __device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const BodyGrid& G, int nch, TinyLds& T,
                                          const ColliderSet* cols, int* err, float (*sd)[kFootLds],
-                                         FxFoot1<3>& acc) {
+                                         FxFoot1<3>& acc, const HvpPre (&pre)[kTinyChunks]) {
   for (int q = 0; q < nch; ++q) {
     // the previous chunk's fx_store reads acc in another thread mapping than fx_zero writes it
     __syncthreads();
@@ -2553,9 +2575,20 @@ __device__ __forceinline__ void tiny_hvp(const PView& P, const Batch& B, const B
     fx_zero(acc);
     load_foot3(c, T.ft[q], T.z, sd, T.flags, cols, true, G.tileBase);
     __syncthreads();
+    // chunk q's entry by selects (a register array indexed by the loop counter would be moved
+    // to memory)
+    HvpPre pq = pre[0];
+#pragma unroll
+    for (int k = 1; k < kTinyChunks; ++k) {
+      const bool m = q == k;
+      pq.s = m ? pre[k].s : pq.s;
+      pq.x[0] = m ? pre[k].x[0] : pq.x[0];
+      pq.x[1] = m ? pre[k].x[1] : pq.x[1];
+      pq.x[2] = m ? pre[k].x[2] : pq.x[2];
+    }
     // a chunk of at most 64 particles spreads each particle over the four waves
-    const FxScale3 scl = c.ch.z <= 64 ? hvp_particles_split(P, B, c, sd, acc, err, T.split)
-                                      : hvp_particles(P, B, c, sd, acc, err, hvp_prefetch(P, B, c.ch));
+    const FxScale3 scl = c.ch.z <= 64 ? hvp_particles_split(P, B, c, sd, acc, err, T.split, &pq)
+                                      : hvp_particles(P, B, c, sd, acc, err, pq);
     fx_store(acc, scl, T.stage, q);
   }
   __syncthreads();
@@ -2587,6 +2620,22 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
     T.tc0[threadIdx.x] = B.tileChunk0[G.tileBase + threadIdx.x] - c0;
     T.tcn[threadIdx.x] = B.tileChunkN[G.tileBase + threadIdx.x];
   }
+  // each thread's particle of every chunk (slot and position; a chunk's particles are fixed for
+  // the solve): thread t of a chunk of more than 64 particles, lane t % 64 of one of at most 64
+  HvpPre pre[kTinyChunks];
+#pragma unroll
+  for (int q = 0; q < kTinyChunks; ++q) {
+    pre[q].s = -1;
+    pre[q].x[0] = pre[q].x[1] = pre[q].x[2] = 0.0f;
+    if (q < nch) {
+      const int4 ch = B.chunks[c0 + q];
+      const int k = ch.z <= 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+      if (k < ch.z) {
+        pre[q].s = B.perm[ch.y + k];
+        load_x(P, pre[q].s, pre[q].x);
+      }
+    }
+  }
   // this thread's nodes
   NodeCtx cn[kTinyBlocks];
   float xr[kTinyBlocks][3], pr[kTinyBlocks][3], apr[kTinyBlocks][3], arr[kTinyBlocks][3], dgr[kTinyBlocks][3],
@@ -2605,7 +2654,7 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
   }
   __syncthreads();
   // setup (ConjugateResiduals.cpp:50-76)
-  tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
+  tiny_hvp(P, B, G, nch, T, cols, err, sd, acc, pre);
 #pragma unroll
   for (int j = 0; j < kTinyBlocks; ++j) {
     if (j >= G.nblocks) break;
@@ -2623,7 +2672,7 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
     tiny_st3(T, G, cn[j], zr[j]);
   }
   __syncthreads();
-  tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
+  tiny_hvp(P, B, G, nch, T, cols, err, sd, acc, pre);
   double t2[2];
   tiny_sweep<2>(G, R2, [&](int j, double (&v)[2]) {
     float df[3] = {0.0f, 0.0f, 0.0f};
@@ -2652,7 +2701,7 @@ __device__ __forceinline__ void tiny_solve(const PView& P, const Batch& B, int b
     if (it == 0 && threadIdx.x == 0) end = cr_update_x_finish<1>(L, t1, body, hist);
     __syncthreads();
     if (end) break;
-    tiny_hvp(P, B, G, nch, T, cols, err, sd, acc);
+    tiny_hvp(P, B, G, nch, T, cols, err, sd, acc, pre);
     tiny_sweep<3>(G, R3, [&](int j, double (&v)[3]) {
       float df[3] = {0.0f, 0.0f, 0.0f};
       if (cn[j].valid) gather3_tab(G, T.tc0, T.tcn, G.tileBase, T.stage, cn[j].T, cn[j].l, df);
@@ -2718,7 +2767,7 @@ __global__ __launch_bounds__(kBlock) void k_small_cr(PView P, Batch B, const int
   auto hvp = [&](const float* d) {
     for (int ch = c0; ch < c1; ++ch) {
       __syncthreads();  // the previous chunk's fx_store reads acc in another mapping than fx_zero writes it
-      hvp_chunk<true>(P, B, ch, d, flags, cols, stage, err, sd, acc, ft);
+      hvp_chunk<true>(P, B, ch, d, flags, cols, stage, err, sd, acc, ft, &T.split);
     }
     __syncthreads();
   };
