@@ -76,14 +76,51 @@ __device__ __forceinline__ void neighbour_tiles(const LevelGrid& G, bool real, i
 }
 
 // ------------------------------------------------------------ convolution (forward / data gradient)
-template <int CIN, int COUT, int MODE>
-__global__ __launch_bounds__(kThreads) void k_conv(LevelView V, const bf16* __restrict__ in,
-                                                  const bf16* __restrict__ W, const float* __restrict__ bias,
-                                                  ConvOut o) {
+// WAVES waves per output tile (1 or 4): the tile's 64 voxels are four 16-row M blocks, 4 / WAVES
+// of them per wave, and each B fragment (weights) is loaded once per K chunk for a wave's blocks.
+// With one wave per tile a workgroup holds only the tile's halo in LDS, so a CU keeps several
+// times more tiles in flight than with four: the convolutions wait on the tile, halo and weight
+// loads, not on the MFMAs. Levels of few tiles (the coarsest) keep four waves per tile so that
+// the device still has work for every SIMD.
+constexpr int kConvFewTiles = 1024;
+constexpr int kHaloBatch = 8;  // 16-byte halo pieces per thread in flight at once
+template <int CIN, int NTHR>
+__device__ __forceinline__ void stage_halo_n(const bf16* in, const int* nb, bf16* halo) {
+  constexpr int CG = CIN / 8;
+  constexpr int TOT = 216 * CG;
+  for (int q0 = 0; q0 < TOT; q0 += kHaloBatch * NTHR) {
+    uint4 v[kHaloBatch];
+#pragma unroll
+    for (int k = 0; k < kHaloBatch; ++k) {
+      const int q = q0 + (int)threadIdx.x + k * NTHR;
+      v[k] = make_uint4(0u, 0u, 0u, 0u);
+      if (q < TOT) {
+        const int hv = q / CG, cg = q - (q / CG) * CG;
+        const int hx = hv % 6, hy = (hv / 6) % 6, hz = hv / 36;
+        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+        const int lv = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
+        const int t = nb[o];
+        if (t >= 0) v[k] = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + cg * 8);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kHaloBatch; ++k) {
+      const int q = q0 + (int)threadIdx.x + k * NTHR;
+      if (q < TOT) *reinterpret_cast<uint4*>(halo + q * 8) = v[k];
+    }
+  }
+}
+template <int CIN, int COUT, int MODE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __restrict__ in,
+                                                    const bf16* __restrict__ W, const float* __restrict__ bias,
+                                                    ConvOut o) {
   static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
+  static_assert(WAVES == 1 || WAVES == 4, "one or four waves per tile");
   constexpr int KTOT = 27 * CIN;
   constexpr int NCH = (KTOT + 31) / 32;
   constexpr int NT16 = COUT / 16;
+  constexpr int MB = 4 / WAVES;  // M blocks per wave
   __shared__ __align__(16) bf16 halo[216 * CIN];
   __shared__ int nb[27];
   const int tile = blockIdx.x;
@@ -96,92 +133,108 @@ __global__ __launch_bounds__(kThreads) void k_conv(LevelView V, const bf16* __re
   if (real) lv_coords(G, lt, tx, ty, tz);
   neighbour_tiles(G, real, tx, ty, tz, nb);
   __syncthreads();
-  stage_halo<CIN>(in, nb, halo);
+  stage_halo_n<CIN, 64 * WAVES>(in, nb, halo);
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int row = lane & 15, q = lane >> 4;
-  const int va = 16 * w + row;  // this lane's A row (output voxel)
-  const int vx = va & 3, vy = (va >> 2) & 3, vz = va >> 4;
-  floatx4 acc[NT16];
+  floatx4 acc[MB][NT16];
 #pragma unroll
-  for (int t = 0; t < NT16; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int t = 0; t < NT16; ++t) acc[m][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
   // the epilogue's other inputs (skip activations; residual and diagonal of the output layer),
   // requested before the K loop so that their latency overlaps it
-  float pre[NT16][4];
-  float pre2[4];
+  float pre[MB][NT16][4];
+  float pre2[MB][4];
 #pragma unroll
-  for (int t = 0; t < NT16; ++t)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) pre[t][v] = 0.0f;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) pre2[v] = 1.0f;
-  if (MODE == CONV_FINAL) {
-    if (row < 3 && o.z) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t zi = ((int64_t)tile * 3 + row) * 64 + 16 * w + 4 * q + v;
-        pre[0][v] = o.r[zi];
-        pre2[v] = o.diag[zi];
-      }
-    }
-  } else if (o.out2) {
+  for (int m = 0; m < MB; ++m) {
 #pragma unroll
     for (int t = 0; t < NT16; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        pre[t][v] = (float)o.skip[((int64_t)tile * 64 + 16 * w + 4 * q + v) * COUT + 16 * t + row];
+      for (int v = 0; v < 4; ++v) pre[m][t][v] = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) pre2[m][v] = 1.0f;
+  }
+  if (MODE == CONV_FINAL) {
+    if (row < 3 && o.z) {
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t zi = ((int64_t)tile * 3 + row) * 64 + 16 * (w * MB + m) + 4 * q + v;
+          pre[m][0][v] = o.r[zi];
+          pre2[m][v] = o.diag[zi];
+        }
+    }
+  } else if (o.out2) {
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int t = 0; t < NT16; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          pre[m][t][v] = (float)o.skip[((int64_t)tile * 64 + 16 * (w * MB + m) + 4 * q + v) * COUT + 16 * t + row];
   }
   for (int c = 0; c < NCH; ++c) {
     const int f = 32 * c + 8 * q;
-    bf16x8 a;
     const bool inK = f < KTOT;
-    if (inK) {
-      const int off = f / CIN, ci = f - off * CIN;
-      const int hx = vx + off % 3, hy = vy + (off / 3) % 3, hz = vz + off / 9;
-      a = *reinterpret_cast<const bf16x8*>(halo + ((hz * 6 + hy) * 6 + hx) * CIN + ci);
-    } else {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = (bf16)0.0f;
-    }
+    bf16x8 bw[NT16];
 #pragma unroll
     for (int t = 0; t < NT16; ++t) {
-      bf16x8 bw;
       if (inK) {
-        bw = *reinterpret_cast<const bf16x8*>(W + (int64_t)(16 * t + row) * KTOT + f);
+        bw[t] = *reinterpret_cast<const bf16x8*>(W + (int64_t)(16 * t + row) * KTOT + f);
       } else {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) bw[u] = (bf16)0.0f;
+        for (int u = 0; u < 8; ++u) bw[t][u] = (bf16)0.0f;
       }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+    }
+    const int off = inK ? f / CIN : 0, ci = inK ? f - off * CIN : 0;
+    const int ox = off % 3, oy = (off / 3) % 3, oz = off / 9;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const int va = 16 * (w * MB + m) + row;  // the A row (output voxel) of this M block
+      const int vx = va & 3, vy = (va >> 2) & 3, vz = va >> 4;
+      bf16x8 a;
+      if (inK) {
+        a = *reinterpret_cast<const bf16x8*>(halo + (((vz + oz) * 6 + vy + oy) * 6 + vx + ox) * CIN + ci);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = (bf16)0.0f;
+      }
+#pragma unroll
+      for (int t = 0; t < NT16; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[t], acc[m][t], 0, 0, 0);
     }
   }
-  // epilogue: lane holds D[4 q + v][row] of each N tile
+  // epilogue: lane holds D[4 q + v][row] of each (M block, N tile)
 #pragma unroll
-  for (int t = 0; t < NT16; ++t) {
-    const int co = 16 * t + row;
-    const float bco = bias ? bias[co] : 0.0f;
+  for (int m = 0; m < MB; ++m)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int vox = 16 * w + 4 * q + v;
-      const bool valid = real && tx * 4 + (vox & 3) < G.n[0] && ty * 4 + ((vox >> 2) & 3) < G.n[1] &&
-                         tz * 4 + (vox >> 4) < G.n[2];
-      float val = acc[t][v] + bco;
-      if (MODE == CONV_RELU) val = val > 0.0f ? val : 0.0f;
-      if (!valid) val = 0.0f;
-      const int64_t e = ((int64_t)tile * 64 + vox) * COUT + co;
-      if (MODE == CONV_FINAL) {
-        if (co < 3) {
-          const int64_t zi = ((int64_t)tile * 3 + co) * 64 + vox;
-          if (o.raw) o.raw[zi] = val;
-          if (o.z) o.z[zi] = pre[0][v] / pre2[v] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * o.sc[b].lpc_scale;
+    for (int t = 0; t < NT16; ++t) {
+      const int co = 16 * t + row;
+      const float bco = bias ? bias[co] : 0.0f;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int vox = 16 * (w * MB + m) + 4 * q + v;
+        const bool valid = real && tx * 4 + (vox & 3) < G.n[0] && ty * 4 + ((vox >> 2) & 3) < G.n[1] &&
+                           tz * 4 + (vox >> 4) < G.n[2];
+        float val = acc[m][t][v] + bco;
+        if (MODE == CONV_RELU) val = val > 0.0f ? val : 0.0f;
+        if (!valid) val = 0.0f;
+        const int64_t e = ((int64_t)tile * 64 + vox) * COUT + co;
+        if (MODE == CONV_FINAL) {
+          if (co < 3) {
+            const int64_t zi = ((int64_t)tile * 3 + co) * 64 + vox;
+            if (o.raw) o.raw[zi] = val;
+            if (o.z)
+              o.z[zi] = pre[m][0][v] / pre2[m][v] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * o.sc[b].lpc_scale;
+          }
+        } else {
+          const bf16 hv = (bf16)val;
+          o.out[e] = hv;
+          if (o.out2) o.out2[e] = (bf16)((float)hv + pre[m][t][v]);
         }
-      } else {
-        const bf16 hv = (bf16)val;
-        o.out[e] = hv;
-        if (o.out2) o.out2[e] = (bf16)((float)hv + pre[t][v]);
       }
     }
-  }
 }
 
 // ------------------------------------------------------------ weight gradient
@@ -526,7 +579,10 @@ __global__ void k_lpc_sample_stats(Batch B, const float* r, const float* diag, c
 // ============================================================ launchers
 #define CONV_CASE(CI, CO, MD)                                                                                  \
   if (cin == CI && cout == CO && mode == MD) {                                                                 \
-    hipLaunchKernelGGL((k_conv<CI, CO, MD>), dim3(V.NT), dim3(kThreads), 0, s, V, in, W, bias, o);            \
+    if (V.NT < kConvFewTiles)                                                                                  \
+      hipLaunchKernelGGL((k_conv<CI, CO, MD, 4>), dim3(V.NT), dim3(256), 0, s, V, in, W, bias, o);            \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_conv<CI, CO, MD, 1>), dim3(V.NT), dim3(64), 0, s, V, in, W, bias, o);             \
     return;                                                                                                    \
   }
 void launch_conv(int cin, int cout, int mode, const LevelView& V, const bf16* in, const bf16* W, const float* bias,
