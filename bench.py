@@ -583,7 +583,7 @@ MFMA_PASS = ("SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_INSTS_VALU_MFMA_BF16 SQ_VALU_MFMA_
 
 
 def mfma_summary(pmc_csv, trace_csv, out_path):
-    """The network's BF16 MFMA convolutions (`k_conv<CIN, COUT, MODE>`) on the slope slab's learned
+    """The network's BF16 MFMA convolutions (`k_conv<CIN, COUT, MODE, WAVES>`) on the slope slab's learned
     leg: per kernel, MFMA FLOPs per dispatch from SQ_INSTS_VALU_MFMA_MOPS_BF16 (units of 512 FLOPs)
     and, with each kernel's mean duration from a kernel trace of the same workload, the FLOP rate
     against 2.5 PFLOP/s and the MFMA pipe's busy fraction: SQ_VALU_MFMA_BUSY_CYCLES (summed over the
