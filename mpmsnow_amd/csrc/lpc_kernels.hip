@@ -84,10 +84,10 @@ __device__ __forceinline__ void neighbour_tiles(const LevelGrid& G, bool real, i
 // the device still has work for every SIMD.
 constexpr int kConvFewTiles = 1024;
 constexpr int kHaloBatch = 8;  // 16-byte halo pieces per thread in flight at once
+// channels [16 sl, 16 sl + 16) of the halo of an input with CIN channels, 16 per voxel in LDS
 template <int CIN, int NTHR>
-__device__ __forceinline__ void stage_halo_n(const bf16* in, const int* nb, bf16* halo) {
-  constexpr int CG = CIN / 8;
-  constexpr int TOT = 216 * CG;
+__device__ __forceinline__ void stage_halo_slice(const bf16* in, const int* nb, int sl, bf16* halo) {
+  constexpr int TOT = 216 * 2;
   for (int q0 = 0; q0 < TOT; q0 += kHaloBatch * NTHR) {
     uint4 v[kHaloBatch];
 #pragma unroll
@@ -95,13 +95,13 @@ __device__ __forceinline__ void stage_halo_n(const bf16* in, const int* nb, bf16
       const int q = q0 + (int)threadIdx.x + k * NTHR;
       v[k] = make_uint4(0u, 0u, 0u, 0u);
       if (q < TOT) {
-        const int hv = q / CG, cg = q - (q / CG) * CG;
+        const int hv = q >> 1, cg = q & 1;
         const int hx = hv % 6, hy = (hv / 6) % 6, hz = hv / 36;
         const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
                       9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
         const int lv = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
         const int t = nb[o];
-        if (t >= 0) v[k] = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + cg * 8);
+        if (t >= 0) v[k] = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + 16 * sl + cg * 8);
       }
     }
 #pragma unroll
@@ -118,10 +118,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __
   static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
   static_assert(WAVES == 1 || WAVES == 4, "one or four waves per tile");
   constexpr int KTOT = 27 * CIN;
-  constexpr int NCH = (KTOT + 31) / 32;
   constexpr int NT16 = COUT / 16;
   constexpr int MB = 4 / WAVES;  // M blocks per wave
-  __shared__ __align__(16) bf16 halo[216 * CIN];
+  constexpr int NSL = CIN / 16;  // 16-channel slices of the input, staged one at a time
+  constexpr int KS = 27 * 16;    // K of one slice: (offset, channel of the slice)
+  constexpr int NCHS = (KS + 31) / 32;
+  __shared__ __align__(16) bf16 halo[216 * 16];
   __shared__ int nb[27];
   const int tile = blockIdx.x;
   const int b = V.tileBody[tile];
@@ -132,8 +134,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __
   int tx = 0, ty = 0, tz = 0;
   if (real) lv_coords(G, lt, tx, ty, tz);
   neighbour_tiles(G, real, tx, ty, tz, nb);
-  __syncthreads();
-  stage_halo_n<CIN, 64 * WAVES>(in, nb, halo);
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int row = lane & 15, q = lane >> 4;
@@ -155,6 +155,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __
 #pragma unroll
     for (int v = 0; v < 4; ++v) pre2[m][v] = 1.0f;
   }
+  // the output layer's body scale, requested with the other epilogue inputs
+  const float lscale = (MODE == CONV_FINAL && o.z) ? o.sc[b].lpc_scale : 0.0f;
   if (MODE == CONV_FINAL) {
     if (row < 3 && o.z) {
 #pragma unroll
@@ -175,34 +177,53 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __
         for (int v = 0; v < 4; ++v)
           pre[m][t][v] = (float)o.skip[((int64_t)tile * 64 + 16 * (w * MB + m) + 4 * q + v) * COUT + 16 * t + row];
   }
-  for (int c = 0; c < NCH; ++c) {
-    const int f = 32 * c + 8 * q;
-    const bool inK = f < KTOT;
-    bf16x8 bw[NT16];
+  // K runs slice by slice: a slice's halo (16 channels, 6.9 KB) is the workgroup's only LDS, so
+  // that a CU holds as many tiles as at 16 input channels
+  for (int sl = 0; sl < NSL; ++sl) {
+    if (sl > 0) __syncthreads();  // the previous slice's reads of halo are done
+    stage_halo_slice<CIN, 64 * WAVES>(in, nb, sl, halo);
+    __syncthreads();
+    // one K chunk of 32 (within the slice: offset f / 16, channel f % 16)
+    auto kstep = [&](int c) {
+      const int f = 32 * c + 8 * q;
+      const bool inK = f < KS;
+      const int off = inK ? f >> 4 : 0, cl = f & 15;
+      bf16x8 bw[NT16];
 #pragma unroll
-    for (int t = 0; t < NT16; ++t) {
-      if (inK) {
-        bw[t] = *reinterpret_cast<const bf16x8*>(W + (int64_t)(16 * t + row) * KTOT + f);
-      } else {
+      for (int t = 0; t < NT16; ++t) {
+        if (inK) {
+          bw[t] = *reinterpret_cast<const bf16x8*>(W + (int64_t)(16 * t + row) * KTOT + off * CIN + 16 * sl + cl);
+        } else {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) bw[t][u] = (bf16)0.0f;
+          for (int u = 0; u < 8; ++u) bw[t][u] = (bf16)0.0f;
+        }
       }
-    }
-    const int off = inK ? f / CIN : 0, ci = inK ? f - off * CIN : 0;
-    const int ox = off % 3, oy = (off / 3) % 3, oz = off / 9;
+      const int ox = off % 3, oy = (off / 3) % 3, oz = off / 9;
 #pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      const int va = 16 * (w * MB + m) + row;  // the A row (output voxel) of this M block
-      const int vx = va & 3, vy = (va >> 2) & 3, vz = va >> 4;
-      bf16x8 a;
-      if (inK) {
-        a = *reinterpret_cast<const bf16x8*>(halo + (((vz + oz) * 6 + vy + oy) * 6 + vx + ox) * CIN + ci);
-      } else {
+      for (int m = 0; m < MB; ++m) {
+        const int va = 16 * (w * MB + m) + row;  // the A row (output voxel) of this M block
+        const int vx = va & 3, vy = (va >> 2) & 3, vz = va >> 4;
+        bf16x8 a;
+        if (inK) {
+          a = *reinterpret_cast<const bf16x8*>(halo + (((vz + oz) * 6 + vy + oy) * 6 + vx + ox) * 16 + cl);
+        } else {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = (bf16)0.0f;
+          for (int u = 0; u < 8; ++u) a[u] = (bf16)0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < NT16; ++t)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[t], acc[m][t], 0, 0, 0);
       }
+    };
+    // the single slice of 16-channel inputs is unrolled whole; wider inputs by two chunks, which
+    // keeps them at 4 waves per SIMD (measured: full unrolling gives the 16-channel convolutions
+    // 15% and costs the 64-channel one 13%)
+    if constexpr (NSL == 1) {
 #pragma unroll
-      for (int t = 0; t < NT16; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[t], acc[m][t], 0, 0, 0);
+      for (int c = 0; c < NCHS; ++c) kstep(c);
+    } else {
+#pragma unroll 2
+      for (int c = 0; c < NCHS; ++c) kstep(c);
     }
   }
   // epilogue: lane holds D[4 q + v][row] of each (M block, N tile)
@@ -226,7 +247,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __
             const int64_t zi = ((int64_t)tile * 3 + co) * 64 + vox;
             if (o.raw) o.raw[zi] = val;
             if (o.z)
-              o.z[zi] = pre[m][0][v] / pre2[m][v] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * o.sc[b].lpc_scale;
+              o.z[zi] = pre[m][0][v] / pre2[m][v] + fminf(fmaxf(val, -kLpcClamp), kLpcClamp) * lscale;
           }
         } else {
           const bf16 hv = (bf16)val;
