@@ -22,8 +22,11 @@ def say(msg):
     print("[lpc] " + msg, flush=True)
 
 
-def grid_case(n=3000, seed=0):
-    sc = scenes.snowball_drop(n=n, seed=seed)
+def grid_case(n=3000, seed=0, slab=False):
+    # the snowball's grid has a few hundred level-0 tiles (the convolutions' four-wave form); the
+    # slab's, 0.26 x 0.07 x 0.26 m at h = 1/256, about 1600 (the one-wave form of large levels)
+    sc = (scenes.slope_slab(n=n, seed=seed, size=(0.26, 0.07, 0.26)) if slab
+          else scenes.snowball_drop(n=n, seed=seed))
     ctx = G.Context()
     ctx.particles_create(n)
     ctx.set("p", sc["x"])
@@ -55,9 +58,11 @@ def test_param_count_matches_oracle():
     assert G.lib().mpm_lpc_num_params() == L.num_params()
 
 
-def test_network_forward_matches_oracle():
-    say("forward snowball n=3000 seed=0")
-    ctx, g, masses = grid_case()
+@pytest.mark.parametrize("case", ["snowball", "slab"])
+def test_network_forward_matches_oracle(case):
+    say("forward %s" % case)
+    ctx, g, masses = grid_case() if case == "snowball" else grid_case(n=20000, seed=5, slab=True)
+    say("level-0 tiles %d" % int(np.prod((np.asarray(g.n) + 3) // 4)))
     params = random_params(7)
     ctx.lpc_set_params(params)
     r, diag, _ = inputs(g, 8)
@@ -73,9 +78,10 @@ def test_network_forward_matches_oracle():
     assert np.abs(out).max() > 0
 
 
-def test_network_gradients_match_oracle():
-    say("gradients snowball n=2000 seed=1")
-    ctx, g, masses = grid_case(n=2000, seed=1)
+@pytest.mark.parametrize("case", ["snowball", "slab"])
+def test_network_gradients_match_oracle(case):
+    say("gradients %s" % case)
+    ctx, g, masses = grid_case(n=2000, seed=1) if case == "snowball" else grid_case(n=20000, seed=6, slab=True)
     params = random_params(11)
     ctx.lpc_set_params(params)
     r, diag, e = inputs(g, 12)
