@@ -83,32 +83,36 @@ __device__ __forceinline__ void neighbour_tiles(const LevelGrid& G, bool real, i
 // loads, not on the MFMAs. Levels of few tiles (the coarsest) keep four waves per tile so that
 // the device still has work for every SIMD.
 constexpr int kConvFewTiles = 1024;
-constexpr int kHaloBatch = 8;  // 16-byte halo pieces per thread in flight at once
-// channels [16 sl, 16 sl + 16) of the halo of an input with CIN channels, 16 per voxel in LDS
+// A slice's halo is 216 voxels x 16 channels = 432 pieces of 16 bytes, loaded in one round of
+// ceil(432 / NTHR) pieces per thread into registers and then stored to LDS, so that the loads
+// of slice s + 1 are in flight while the MFMAs of slice s run.
+template <int NTHR>
+constexpr int halo_pieces() { return (432 + NTHR - 1) / NTHR; }
 template <int CIN, int NTHR>
-__device__ __forceinline__ void stage_halo_slice(const bf16* in, const int* nb, int sl, bf16* halo) {
-  constexpr int TOT = 216 * 2;
-  for (int q0 = 0; q0 < TOT; q0 += kHaloBatch * NTHR) {
-    uint4 v[kHaloBatch];
+__device__ __forceinline__ void load_halo_slice(const bf16* in, const int* nb, int sl,
+                                                uint4 (&v)[halo_pieces<NTHR>()]) {
 #pragma unroll
-    for (int k = 0; k < kHaloBatch; ++k) {
-      const int q = q0 + (int)threadIdx.x + k * NTHR;
-      v[k] = make_uint4(0u, 0u, 0u, 0u);
-      if (q < TOT) {
-        const int hv = q >> 1, cg = q & 1;
-        const int hx = hv % 6, hy = (hv / 6) % 6, hz = hv / 36;
-        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
-                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-        const int lv = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
-        const int t = nb[o];
-        if (t >= 0) v[k] = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + 16 * sl + cg * 8);
-      }
+  for (int k = 0; k < halo_pieces<NTHR>(); ++k) {
+    const int q = (int)threadIdx.x + k * NTHR;
+    v[k] = make_uint4(0u, 0u, 0u, 0u);
+    if (q < 432) {
+      const int hv = q >> 1, cg = q & 1;
+      const int hx = hv % 6, hy = (hv / 6) % 6, hz = hv / 36;
+      const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                    9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+      const int lv = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
+      const int t = nb[o];
+      if (t >= 0) v[k] = *reinterpret_cast<const uint4*>(in + ((int64_t)t * 64 + lv) * CIN + 16 * sl + cg * 8);
     }
+  }
+}
+// channels [16 sl, 16 sl + 16) of the halo, 16 per voxel in LDS
+template <int NTHR>
+__device__ __forceinline__ void store_halo_slice(const uint4 (&v)[halo_pieces<NTHR>()], bf16* halo) {
 #pragma unroll
-    for (int k = 0; k < kHaloBatch; ++k) {
-      const int q = q0 + (int)threadIdx.x + k * NTHR;
-      if (q < TOT) *reinterpret_cast<uint4*>(halo + q * 8) = v[k];
-    }
+  for (int k = 0; k < halo_pieces<NTHR>(); ++k) {
+    const int q = (int)threadIdx.x + k * NTHR;
+    if (q < 432) *reinterpret_cast<uint4*>(halo + q * 8) = v[k];
   }
 }
 template <int CIN, int COUT, int MODE, int WAVES>
@@ -178,11 +182,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_conv(LevelView V, const bf16* __
           pre[m][t][v] = (float)o.skip[((int64_t)tile * 64 + 16 * (w * MB + m) + 4 * q + v) * COUT + 16 * t + row];
   }
   // K runs slice by slice: a slice's halo (16 channels, 6.9 KB) is the workgroup's only LDS, so
-  // that a CU holds as many tiles as at 16 input channels
+  // that a CU holds as many tiles as at 16 input channels; the next slice's halo loads overlap
+  // this slice's MFMAs
+  uint4 pieces[halo_pieces<64 * WAVES>()];
+  load_halo_slice<CIN, 64 * WAVES>(in, nb, 0, pieces);
   for (int sl = 0; sl < NSL; ++sl) {
     if (sl > 0) __syncthreads();  // the previous slice's reads of halo are done
-    stage_halo_slice<CIN, 64 * WAVES>(in, nb, sl, halo);
+    store_halo_slice<64 * WAVES>(pieces, halo);
     __syncthreads();
+    if (sl + 1 < NSL) load_halo_slice<CIN, 64 * WAVES>(in, nb, sl + 1, pieces);
     // one K chunk of 32 (within the slice: offset f / 16, channel f % 16)
     auto kstep = [&](int c) {
       const int f = 32 * c + 8 * q;
