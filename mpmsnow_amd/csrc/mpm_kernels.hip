@@ -339,6 +339,16 @@ __device__ __forceinline__ void sep_scatter(int* acc, const Stencil& st, float f
 
 // One z-plane k (0-3, uniform across a wave) of sep_gather and sep_scatter: the same terms, in the
 // same order, so that four waves that take a plane each reproduce the one-thread results bitwise.
+// &sd[a][base] as a register the compiler cannot fold into the reads' constant offsets: the
+// stencil's reads then lie within ds_read2's 8-bit word offsets of it (at most
+// 3 + 3 * 8 + 3 * kFootSZ = 225 words) and take no address arithmetic each, where a folded
+// component offset (a * kFootLds words) would put every pair of reads of components 1 and 2 out
+// of that range.
+__device__ __forceinline__ const float* foot_at(float (*sd)[kFootLds], int a, int base) {
+  int off = a * kFootLds + base;
+  asm("" : "+v"(off));
+  return reinterpret_cast<const float*>(sd) + off;
+}
 __device__ __forceinline__ float pick4(const float v[4], int k) {
   return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
 }
@@ -1792,7 +1802,7 @@ __device__ __forceinline__ void hvp_particle(const PView& P, const ChunkCtx& c, 
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         float g[3];
-        sep_gather(&sd[a][base], st, g);
+        sep_gather(foot_at(sd, a, base), st, g);
         G[a] = -g[0];
         G[a + 3] = -g[1];
         G[a + 6] = -g[2];
@@ -1895,7 +1905,7 @@ __device__ __forceinline__ FxScale3 hvp_particles_split(const PView& P, const Ba
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         float e[3];
-        sep_gather_plane(&sd[a][base], st, plane, e);
+        sep_gather_plane(foot_at(sd, a, base), st, plane, e);
         W.part[plane][3 * a][lane] = e[0];
         W.part[plane][3 * a + 1][lane] = e[1];
         W.part[plane][3 * a + 2][lane] = e[2];
