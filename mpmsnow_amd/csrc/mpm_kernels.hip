@@ -837,12 +837,23 @@ __device__ __forceinline__ void cc_local_union(int* lp, const uint32_t* prio, in
 // linked in two passes after the subgraph-sampling scheme of Sutton et al. (Afforest, IPDPS
 // 2018): pass 1 (k_cc_sample, k_cc_join) hooks every particle to its first kCcSampleLinks
 // neighbours, which already joins most of each body, and records whether it has any neighbour;
-// after compression and a sampled vote for the largest component (giant), pass 2 (k_cc_link)
-// hooks every remaining neighbour of the particles outside the giant component. A particle inside
-// it may skip its pairs: for a pair with one end outside, that end links it; a pair with both
-// ends inside is already joined. Threads walk the bucket order so a wave's candidate cells mostly
-// coincide.
+// after compression and a sampled vote for the largest components, pass 2 (k_cc_link) hooks every
+// remaining neighbour of the particles outside the skipped components: the largest, and with it
+// every other voted component whose box (grown by the link distance) meets none of the boxes
+// already skipped. A particle of a skipped component may skip its pairs: for a pair with one end
+// outside them, that end links it; a pair with both ends in one is already joined; no pair has
+// its ends in two of them, whose boxes are farther apart than the link distance. Membership is
+// read from the pointers as compression left them (snap), which pass 2's hooks do not change.
+// Threads walk the bucket order so a wave's candidate cells mostly coincide.
 constexpr int kCcSampleLinks = 2;
+// the components pass 2 skips (the vote's candidates, their boxes, the selection)
+struct CcSkip {
+  int cand[kCcSkipMax];
+  unsigned int box[kCcSkipMax][6];  // min x, y, z, max x, y, z as float_order values
+  int nskip;
+  int skip[kCcSkipMax];
+};
+static_assert(sizeof(CcSkip) <= kCcSkipBytes, "skip state size");
 // Pass 1 for one workgroup of kBlock consecutive particles in bucket order (about one block of
 // cells): the sample links inside the workgroup's range are joined in LDS, every particle's
 // global pointer is set to its local root (plain stores: no other workgroup writes pointers in
@@ -907,15 +918,22 @@ __global__ void k_cc_join(int n, const float4* sorted, const int* in0, const int
     if (b >= 0) cc_union(parent, s, b);
   }
 }
-// pass 2: every neighbour pair of the particles outside the giant component
+// pass 2: every neighbour pair of the particles outside the skipped components
 __global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                          const float4* sorted, int* parent, const int* giant) {
+                          const float4* sorted, int* parent, const int* snap, const CcSkip* S) {
   const float r2 = h * h;
-  const int g = *giant;
+  const int ns = S->nskip;
+  int sk[kCcSkipMax];
+#pragma unroll
+  for (int k = 0; k < kCcSkipMax; ++k) sk[k] = k < ns ? S->skip[k] : -1;
   GRID_STRIDE(ti, n) {
     float4 me = sorted[ti];
     const int s = __float_as_int(me.w);
-    if (g >= 0 && cc_load_cached(parent, s) == g) continue;
+    const int r = snap[s];
+    bool skip = false;
+#pragma unroll
+    for (int k = 0; k < kCcSkipMax; ++k) skip = skip || sk[k] == r;
+    if (skip) continue;
     const float x[3] = {me.x, me.y, me.z};
     int c[3];
     cell_of(x, cell, c);
@@ -933,17 +951,19 @@ __global__ void k_cc_link(int n, float h, float cell, uint32_t mask, const int* 
         }
   }
 }
-// every pointer straight to its root
-__global__ void k_cc_compress(int n, int* parent) {
+// every pointer straight to its root; snap keeps these pass-1 roots
+__global__ void k_cc_compress(int n, int* parent, int* snap) {
   GRID_STRIDE(s, n) {
     const int r = cc_find_cached(parent, (int)s);
     __hip_atomic_store(parent + s, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    snap[s] = r;
   }
 }
-// The most frequent root among kCcVote evenly spaced particles (of compressed pointers), or -1
-// when no root occurs twice. One workgroup of kCcVote threads.
+// The kCcSkipMax most frequent roots among kCcVote evenly spaced particles (of compressed
+// pointers), most frequent first (ties to the smaller root), each occurring at least twice; -1
+// past the last. Also resets the candidates' boxes. One workgroup of kCcVote threads.
 constexpr int kCcVote = 1024;
-__global__ __launch_bounds__(kCcVote) void k_cc_vote(int n, const int* parent, int* giant) {
+__global__ __launch_bounds__(kCcVote) void k_cc_vote(int n, const int* parent, CcSkip* S) {
   __shared__ int v[kCcVote];
   __shared__ int bestCount[kCcVote / 64], bestRoot[kCcVote / 64];
   const int t = threadIdx.x;
@@ -974,29 +994,126 @@ __global__ __launch_bounds__(kCcVote) void k_cc_vote(int n, const int* parent, i
     }
     cnt = t - lo + 1;
   }
-  // longest run; ties to the smaller root
-  int bc = cnt, br = r;
-  for (int off = 32; off > 0; off >>= 1) {
-    const int oc = __shfl_down(bc, off, 64), orr = __shfl_down(br, off, 64);
-    if (oc > bc || (oc == bc && orr < br)) {
-      bc = oc;
-      br = orr;
+  if (t < 6 * kCcSkipMax) (&S->box[0][0])[t] = (t % 6) < 3 ? 0xffffffffu : 0u;
+  // the longest runs one at a time; ties to the smaller root
+  __shared__ int winner;
+  for (int k = 0; k < kCcSkipMax; ++k) {
+    int bc = cnt, br = cnt > 0 ? r : INT_MAX;
+    for (int off = 32; off > 0; off >>= 1) {
+      const int oc = __shfl_down(bc, off, 64), orr = __shfl_down(br, off, 64);
+      if (oc > bc || (oc == bc && orr < br)) {
+        bc = oc;
+        br = orr;
+      }
+    }
+    if ((t & 63) == 0) {
+      bestCount[t >> 6] = bc;
+      bestRoot[t >> 6] = br;
+    }
+    __syncthreads();
+    if (t == 0) {
+      int c = bestCount[0], rr = bestRoot[0];
+      for (int w = 1; w < kCcVote / 64; ++w)
+        if (bestCount[w] > c || (bestCount[w] == c && bestRoot[w] < rr)) {
+          c = bestCount[w];
+          rr = bestRoot[w];
+        }
+      S->cand[k] = c >= 2 ? rr : -1;
+      winner = c >= 2 ? rr : -1;
+    }
+    __syncthreads();
+    if (cnt > 0 && r == winner) cnt = 0;
+  }
+}
+// floats as unsigned integers of the same order (for atomic min / max)
+__device__ __forceinline__ unsigned int float_order(float f) {
+  const unsigned int u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float order_float(unsigned int o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+// bounding boxes of the voted candidates: a workgroup takes kBoxPerThread x kBlock consecutive
+// particles of the bucket order, each wave reduces its particles of each candidate present in it
+// into the workgroup's boxes in LDS, and the workgroup adds those to the global boxes
+constexpr int kBoxPerThread = 16;
+__global__ __launch_bounds__(kBlock) void k_cc_skip_boxes(int n, const float4* sorted, const int* snap, CcSkip* S) {
+  __shared__ unsigned int lbox[kCcSkipMax][6];
+  __shared__ int cand[kCcSkipMax];
+  const int t = threadIdx.x, lane = t & 63;
+  if (t < 6 * kCcSkipMax) (&lbox[0][0])[t] = (t % 6) < 3 ? 0xffffffffu : 0u;
+  if (t < kCcSkipMax) cand[t] = S->cand[t];
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kBlock * kBoxPerThread;
+  for (int it = 0; it < kBoxPerThread; ++it) {
+    const int64_t ti = base + (int64_t)it * kBlock + t;
+    const bool act = ti < n;
+    const float4 me = act ? sorted[ti] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int r = act ? snap[__float_as_int(me.w)] : -1;
+    int j = -1;
+#pragma unroll
+    for (int k = 0; k < kCcSkipMax; ++k)
+      if (cand[k] >= 0 && cand[k] == r) j = k;
+    unsigned long long todo = __ballot(j >= 0);
+    while (todo) {
+      const int l = __ffsll((long long)todo) - 1;
+      const int lj = __shfl(j, l, 64);
+      const bool mine = j == lj;
+      const unsigned long long peers = __ballot(mine);
+      float v[6] = {mine ? me.x : INFINITY, mine ? me.y : INFINITY, mine ? me.z : INFINITY,
+                    mine ? me.x : -INFINITY, mine ? me.y : -INFINITY, mine ? me.z : -INFINITY};
+      for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+          const float o = __shfl_xor(v[a], off, 64);
+          v[a] = a < 3 ? fminf(v[a], o) : fmaxf(v[a], o);
+        }
+      if (lane == l) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          atomicMin(&lbox[lj][a], float_order(v[a]));
+          atomicMax(&lbox[lj][a + 3], float_order(v[a + 3]));
+        }
+      }
+      todo &= ~peers;
     }
   }
-  if ((t & 63) == 0) {
-    bestCount[t >> 6] = bc;
-    bestRoot[t >> 6] = br;
-  }
   __syncthreads();
-  if (t == 0) {
-    int c = bestCount[0], rr = bestRoot[0];
-    for (int w = 1; w < kCcVote / 64; ++w)
-      if (bestCount[w] > c || (bestCount[w] == c && bestRoot[w] < rr)) {
-        c = bestCount[w];
-        rr = bestRoot[w];
-      }
-    *giant = c >= 2 ? rr : -1;
+  if (t < 6 * kCcSkipMax) {
+    const int k = t / 6, a = t % 6;
+    const unsigned int v = lbox[k][a];
+    if (a < 3 && v != 0xffffffffu) atomicMin(&S->box[k][a], v);
+    if (a >= 3 && v != 0u) atomicMax(&S->box[k][a], v);
   }
+}
+// the candidates skipped by pass 2: in vote order, each whose box, grown by the link distance
+// (and 0.1% for the float distance test), meets no box already taken. One thread.
+__global__ void k_cc_skip_select(float h, CcSkip* S) {
+  const float m = 1.001f * h;
+  float lo[kCcSkipMax][3], hi[kCcSkipMax][3];
+  int ns = 0;
+  for (int k = 0; k < kCcSkipMax; ++k) {
+    const int r = S->cand[k];
+    if (r < 0) break;
+    float l[3], u[3];
+    for (int a = 0; a < 3; ++a) {
+      l[a] = order_float(S->box[k][a]);
+      u[a] = order_float(S->box[k][a + 3]);
+    }
+    bool ok = true;
+    for (int q = 0; q < ns && ok; ++q) {
+      bool meet = true;
+      for (int a = 0; a < 3; ++a) meet = meet && l[a] <= hi[q][a] + m && lo[q][a] <= u[a] + m;
+      ok = !meet;
+    }
+    if (!ok) continue;
+    for (int a = 0; a < 3; ++a) {
+      lo[ns][a] = l[a];
+      hi[ns][a] = u[a];
+    }
+    S->skip[ns++] = r;
+  }
+  S->nskip = ns;
 }
 // every slot points at its root; smallest caller index per component. The store is
 // body-sorted, so a wave's slots usually share one root: reduce in registers first.
@@ -3526,15 +3643,22 @@ void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* o
   if (n > 0) LAUNCH(k_cc_scatter, grid_for(n, 16384), P, bucket, rank, offset, n, sorted);
 }
 void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const float4* sorted, int* parent, int* hasNb, int* giant, int* links0, int* links1,
+                    const float4* sorted, int* parent, int* hasNb, int* snap, void* skip, int* links0, int* links1,
                     hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_cc_sample, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, n, h, cell, mask,
-                     offset, count, sorted, parent, hasNb, links0, links1);
+  CcSkip* S = static_cast<CcSkip*>(skip);
+  const dim3 blocks((unsigned)((n + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(k_cc_sample, blocks, dim3(kBlock), 0, s, n, h, cell, mask, offset, count, sorted, parent, hasNb,
+                     links0, links1);
   LAUNCH(k_cc_join, grid_for(n, 16384), n, sorted, (const int*)links0, (const int*)links1, parent);
-  LAUNCH(k_cc_compress, grid_for(n, 16384), n, parent);
-  hipLaunchKernelGGL(k_cc_vote, dim3(1), dim3(kCcVote), 0, s, n, (const int*)parent, giant);
-  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, (const int*)giant);
+  LAUNCH(k_cc_compress, grid_for(n, 16384), n, parent, snap);
+  hipLaunchKernelGGL(k_cc_vote, dim3(1), dim3(kCcVote), 0, s, n, (const int*)parent, S);
+  const int64_t perBlock = (int64_t)kBlock * kBoxPerThread;
+  hipLaunchKernelGGL(k_cc_skip_boxes, dim3((unsigned)((n + perBlock - 1) / perBlock)), dim3(kBlock), 0, s, n, sorted,
+                     (const int*)snap, S);
+  hipLaunchKernelGGL(k_cc_skip_select, dim3(1), dim3(1), 0, s, h, S);
+  LAUNCH(k_cc_link, grid_for(n, 16384), n, h, cell, mask, offset, count, sorted, parent, (const int*)snap,
+         (const CcSkip*)S);
 }
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,
                      hipStream_t s) {

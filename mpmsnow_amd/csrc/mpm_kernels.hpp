@@ -28,10 +28,12 @@ void launch_cc_hash(PView P, int n, float cell, uint32_t mask, int* bucket, int*
 // sorted: cell-ordered float4 (x, y, z, slot bits)
 void launch_cc_scatter(PView P, const int* bucket, const int* rank, const int* offset, int n, float4* sorted,
                        hipStream_t s);
-// links neighbour pairs (two sampled passes, compression, a vote for the largest component in
-// giant: one device int)
+// links neighbour pairs (two sampled passes, compression with the pass-1 roots kept in snap, a
+// vote for the components pass 2 skips, in skip: kCcSkipBytes of device memory)
+constexpr int kCcSkipMax = 8;
+constexpr size_t kCcSkipBytes = 512;
 void launch_cc_link(int n, float h, float cell, uint32_t mask, const int* offset, const int* count,
-                    const float4* sorted, int* parent, int* hasNb, int* giant, int* links0, int* links1,
+                    const float4* sorted, int* parent, int* hasNb, int* snap, void* skip, int* links0, int* links1,
                     hipStream_t s);
 // isRoot[s] = 1 for the root of a component of >= 2 particles; minOrig[root] = smallest caller index
 void launch_cc_roots(int n, int* parent, const int* hasNb, const int* orig_of, int* isRoot, int* minOrig,

@@ -258,7 +258,7 @@ struct mpm_context {
   std::vector<int> ballisticList;
   // body-partition scratch
   DevBuf ccBucket, ccRank, ccCount, ccOffset, ccSorted, ccParent, ccHasNb, ccIsRoot, ccRootIdx, ccMinOrig, ccTmp,
-      ccTot, newToOld, ballPos;
+      ccTot, ccSkip, newToOld, ballPos;
   int iterHint = 8;
   int iterHintFcr = 8;
   int iterHintHandoff = 8;  // CR iterations after a learned solve's handoff (last solve's count)
@@ -1090,6 +1090,7 @@ void calculate_bodies(mpm_context& c, bool keepErrors = false) {
   c.ccMinOrig.ensure(sizeof(int) * n);
   c.ccTmp.ensure(sizeof(int) * scan_tmp_ints(std::max<int64_t>(tsize, n)));
   c.ccTot.ensure(sizeof(int) * 4);
+  c.ccSkip.ensure(kCcSkipBytes);
   c.labels.ensure(sizeof(int) * n);
   HIPCHECK(hipMemsetAsync(c.ccCount.p, 0, sizeof(int) * tsize, s));
   launch_cc_hash(c.view(), n, cell, mask, c.ccBucket.as<int>(), c.ccRank.as<int>(), c.ccCount.as<int>(), s);
@@ -1098,8 +1099,9 @@ void calculate_bodies(mpm_context& c, bool keepErrors = false) {
                     c.ccSorted.as<float4>(), s);
   launch_iota(c.ccParent.as<int>(), n, s);
   // pass 1's outgoing links go to ccBucket and ccRootIdx, free between the scatter and the numbering
+  // pass 1's roots go to ccIsRoot, free until the numbering
   launch_cc_link(n, c.h, cell, mask, c.ccOffset.as<int>(), c.ccCount.as<int>(), c.ccSorted.as<float4>(),
-                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccTot.as<int>() + 3, c.ccBucket.as<int>(),
+                 c.ccParent.as<int>(), c.ccHasNb.as<int>(), c.ccIsRoot.as<int>(), c.ccSkip.p, c.ccBucket.as<int>(),
                  c.ccRootIdx.as<int>(), s);
   HIPCHECK(hipMemsetAsync(c.ccIsRoot.p, 0, sizeof(int) * n, s));
   launch_fill_int(c.ccMinOrig.as<int>(), INT_MAX, n, s);

@@ -3,11 +3,14 @@ KdTree flood fill: every body's particle list and the ballistic list must match 
 work is bit-exact).
 
 The clouds exercise the linking passes: one dense component holding most particles (the
-sampled vote finds it and its particles skip the second pass), two equal halves (only one can
-be skipped), many mid-size clusters with no dominant one, chains whose spacing straddles h
-(pairs decided by the float distance test), isolated particles only (no component), clusters at
-negative cell coordinates, a slab spanning hundreds of the first pass's workgroups, and a wide
-sparse domain whose cell blocks alias in the bucket table.
+sampled vote finds it and its particles skip the second pass), two equal halves and eight
+separate spheres (all skipped: their boxes lie farther apart than h), many mid-size clusters
+with no dominant one, chains whose spacing straddles h (pairs decided by the float distance
+test), isolated particles only (no component), clusters at negative cell coordinates, a slab
+spanning hundreds of the first pass's workgroups, a wide sparse domain whose cell blocks alias
+in the bucket table, and large components whose boxes come within h of each other (0.5 h: one
+body, joined only by the second pass over the component that is not skipped; 1.0005 h: two
+bodies, inside the skip test's margin).
 """
 import numpy as np
 import pytest
@@ -73,6 +76,20 @@ def clouds():
     # a wide sparse domain: the blocks alias in the bucket table
     wide = rng.uniform(-10.0, 10.0, (3000, 3)).astype(np.float32)
     out["sparse wide domain + sphere"] = np.concatenate([wide, scenes.sphere(8, 5000, (0.0, 0.0, 0.0), 0.05)])
+    # eight large components far apart: the second pass skips all of them
+    out["eight separate spheres"] = np.concatenate([
+        scenes.sphere(900 + k, 4000, (0.3 + 0.35 * (k & 1), 0.3 + 0.35 * ((k >> 1) & 1), 0.3 + 0.35 * (k >> 2)), 0.1)
+        for k in range(8)])
+    # large components whose boxes come within the link distance: A and B 0.5 h apart (one body,
+    # linked only through the pass over B), B and C 3 h apart, A and C far apart (both skipped)
+    a = scenes.box(910, 8000, (0.1, 0.4, 0.4), (0.3, 0.6, 0.6))
+    b = scenes.box(911, 7000, (0.3 + 0.5 * h, 0.4, 0.4), (0.5, 0.6, 0.6))
+    c = scenes.box(912, 6000, (0.5 + 3.0 * h, 0.4, 0.4), (0.7, 0.6, 0.6))
+    out["three boxes, the middle one 0.5 h from the first"] = np.concatenate([a, b, c])
+    # two large components a hair over h apart (1.0005 h: two bodies, inside the boxes' margin)
+    a = scenes.box(913, 8000, (0.1, 0.4, 0.4), (0.3, 0.6, 0.6))
+    b = scenes.box(914, 8000, (0.3 + 1.0005 * h, 0.4, 0.4), (0.5, 0.6, 0.6))
+    out["two boxes 1.0005 h apart"] = np.concatenate([a, b])
     return out, h
 
 
