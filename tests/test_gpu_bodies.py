@@ -9,8 +9,8 @@ with no dominant one, chains whose spacing straddles h (pairs decided by the flo
 test), isolated particles only (no component), clusters at negative cell coordinates, a slab
 spanning hundreds of the first pass's workgroups, a wide sparse domain whose cell blocks alias
 in the bucket table, and large components whose boxes come within h of each other (0.5 h: one
-body, joined only by the second pass over the component that is not skipped; 1.0005 h: two
-bodies, inside the skip test's margin).
+body, which the first pass may leave in two parts and the second then joins from the part that
+is not skipped; 1.0005 h: two bodies, inside the skip test's margin).
 """
 import numpy as np
 import pytest
@@ -80,8 +80,9 @@ def clouds():
     out["eight separate spheres"] = np.concatenate([
         scenes.sphere(900 + k, 4000, (0.3 + 0.35 * (k & 1), 0.3 + 0.35 * ((k >> 1) & 1), 0.3 + 0.35 * (k >> 2)), 0.1)
         for k in range(8)])
-    # large components whose boxes come within the link distance: A and B 0.5 h apart (one body,
-    # linked only through the pass over B), B and C 3 h apart, A and C far apart (both skipped)
+    # large components whose boxes come within the link distance: A and B 0.5 h apart (one body;
+    # if pass 1 leaves them apart, only one of them is skipped and pass 2 over the other joins
+    # them), B and C 3 h apart, A and C far apart (both may be skipped)
     a = scenes.box(910, 8000, (0.1, 0.4, 0.4), (0.3, 0.6, 0.6))
     b = scenes.box(911, 7000, (0.3 + 0.5 * h, 0.4, 0.4), (0.5, 0.6, 0.6))
     c = scenes.box(912, 6000, (0.5 + 3.0 * h, 0.4, 0.4), (0.7, 0.6, 0.6))
